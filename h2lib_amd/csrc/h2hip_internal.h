@@ -1,0 +1,157 @@
+// Internal declarations of the h2hip library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "h2hip.h"
+
+namespace h2hip {
+
+// ---- errors -----------------------------------------------------------------
+
+void set_error(const std::string &msg);
+
+#define H2HIP_CHECK(call)                                                    \
+  do {                                                                       \
+    hipError_t err__ = (call);                                               \
+    if (err__ != hipSuccess) {                                               \
+      ::h2hip::set_error(std::string(#call) + ": " +                        \
+                         hipGetErrorString(err__));                          \
+      return H2HIP_EDEVICE;                                                  \
+    }                                                                        \
+  } while (0)
+
+// Device buffer owned by the library.
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)&p, sizeof(T) * count);
+  }
+  hipError_t upload(const T *src, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice);
+  }
+};
+
+// ---- quadrature rules (host) -------------------------------------------------
+
+// One rule on a pair of reference triangles.  Point k of the first triangle
+// is A (1 - x0[k]) + B (x0[k] - x1[k]) + C x1[k], of the second triangle
+// A (1 - y0[k]) + B (y0[k] - y1[k]) + C y1[k]; w[k] is the weight.
+struct PairRule {
+  std::vector<double> x0, x1, y0, y1, w;
+  size_t size() const { return w.size(); }
+};
+
+// Single-triangle rule of order q: points (x0, x1), weights w.
+struct TriRule {
+  std::vector<double> x0, x1, w;
+};
+
+void gauss_legendre01(unsigned m, std::vector<double> &x,
+                      std::vector<double> &w);
+TriRule make_triangle_rule(unsigned q);
+PairRule make_vertex_rule(unsigned q2);
+PairRule make_edge_rule(unsigned q2);
+PairRule make_identical_rule(unsigned q2);
+
+// ---- device-side records -------------------------------------------------------
+
+// Rule classes by number of shared vertices.
+enum { CLS_REGULAR = 0, CLS_VERTEX = 1, CLS_EDGE = 2, CLS_IDENTICAL = 3 };
+
+// Support entry of a tile side: triangle and the tile-local DOF numbers of
+// its three local basis functions (0xff: not in the tile).  For the
+// constant basis loc0 is the tile-local triangle number.
+struct Supp {
+  uint32_t tri;
+  uint32_t loc;  // loc0 | loc1 << 8 | loc2 << 16
+};
+
+constexpr int TILE = 64;
+
+struct Tile {
+  uint64_t out_off;  // element offset of tile entry (0, 0)
+  uint32_t ld;       // leading dimension of the output block
+  uint32_t rsup;     // first row support entry
+  uint32_t csup;     // first column support entry
+  uint32_t row0, col0;  // tile position inside the block (multiples of 64)
+  uint16_t nrsup, ncsup;
+  uint8_t nr, nc;       // tile size
+  uint8_t ntrans;
+  uint8_t flags;        // bit 0: identity term allowed (lc quirk)
+};
+
+struct Fixup {  // identity term of the cc double layer at (t, t), t != s
+  uint64_t pos;
+  uint32_t tri;
+  uint32_t pad;
+};
+
+}  // namespace h2hip
+
+struct h2hip_surface {
+  int device = 0;
+  uint32_t nv = 0, nt = 0;
+  std::vector<double> x, n, g;
+  std::vector<uint32_t> t;
+  std::vector<uint32_t> v2t_ptr, v2t;  // triangles of each vertex
+  h2hip::DevBuf<double> d_x, d_n, d_g;
+  h2hip::DevBuf<uint32_t> d_t;
+};
+
+struct h2hip_op {
+  const h2hip_surface *s = nullptr;
+  int kernel = 0, layer = 0, rb = 0, cb = 0;
+  uint32_t q = 2, q2 = 4;
+  double alpha[2] = {0, 0}, k[2] = {0, 0};
+  double mass[9] = {0};
+  int precision = 0;
+  // singular rules, structure of arrays per class 1..3:
+  // [x0 | x1 | y0 | y1 | w] with n points each, starting at sing_off[cls]
+  h2hip::DevBuf<double> d_sing;
+  uint32_t sing_off[4] = {0, 0, 0, 0};
+  uint32_t sing_n[4] = {0, 0, 0, 0};
+  // regular rule: per-triangle points (3 * q * q doubles per triangle)
+  h2hip::DevBuf<double> d_regpts;
+  h2hip::DevBuf<float> d_regpts_f;
+  // regular weights: [w (q*q) | phi_0 (q*q) | phi_1 | phi_2], basis factors
+  // premultiplied by the weights
+  h2hip::DevBuf<double> d_regw;
+  std::vector<double> regw;
+  mutable std::mutex lock;  // serialises the blocking nearfield callback
+  mutable h2hip::DevBuf<unsigned char> scratch;
+};
+
+struct h2hip_nfplan {
+  const h2hip_surface *s = nullptr;
+  int rb = 0, cb = 0;
+  uint32_t nblocks = 0;
+  uint64_t entries = 0;
+  std::vector<uint64_t> off;
+  std::vector<h2hip::Tile> tiles;
+  std::vector<h2hip::Supp> supp;
+  std::vector<h2hip::Fixup> fix;
+  h2hip::DevBuf<h2hip::Tile> d_tiles;
+  h2hip::DevBuf<h2hip::Supp> d_supp;
+  h2hip::DevBuf<h2hip::Fixup> d_fix;
+};

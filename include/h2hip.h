@@ -1,0 +1,161 @@
+/* h2hip -- MI355X (gfx950) near-field Galerkin quadrature and H2-matrix
+ * matrix-vector products for 3-D boundary element methods.
+ *
+ * Plain C ABI: opaque handles, plain pointers and sizes, integer status
+ * codes.  All functions return H2HIP_OK (0) on success and a negative code
+ * on failure; h2hip_last_error() returns a message for the calling thread.
+ *
+ * Complex values (Helmholtz operators) are stored as interleaved
+ * (re, im) pairs of doubles, i.e. the layout of C99 double complex.
+ * Matrices are column-major with a leading dimension, as in the reference's
+ * amatrix (Library/amatrix.h:43-57).
+ *
+ * "stream" arguments are hipStream_t values passed as void* (NULL: the
+ * default stream).  Device pointers are plain void* as returned by
+ * hipMalloc or by a framework's allocator.
+ *
+ * Every entry point below names the reference interface it replaces
+ * (paths relative to the H2Lib source root).
+ */
+#ifndef H2HIP_H
+#define H2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define H2HIP_OK 0
+#define H2HIP_EINVAL (-1)
+#define H2HIP_EDEVICE (-2)
+#define H2HIP_ENOMEM (-3)
+
+/* kernel families */
+#define H2HIP_LAPLACE 0
+#define H2HIP_HELMHOLTZ 1
+/* layer potentials */
+#define H2HIP_SLP 0
+#define H2HIP_DLP 1
+/* basis functions (basisfunctionbem3d, Library/bem3d.h) */
+#define H2HIP_BASIS_CONSTANT 0
+#define H2HIP_BASIS_LINEAR 1
+
+typedef struct h2hip_surface h2hip_surface;
+typedef struct h2hip_op h2hip_op;
+typedef struct h2hip_nfplan h2hip_nfplan;
+
+/* Message describing the last failure on the calling thread. */
+const char *h2hip_last_error(void);
+
+/* Device number used by the library (hipGetDevice of the caller). */
+int h2hip_device_count(void);
+
+/* ---- surfaces ------------------------------------------------------------
+ * Device copy of a triangulated surface: vertices x[3*vertices], triangles
+ * t[3*triangles] (counter-clockwise), unit normals n[3*triangles] and Gram
+ * determinants g[triangles] (twice the triangle areas).  n and g may be
+ * NULL, then they are computed as prepare_surface3d does.
+ * Replaces the device-side view of surface3d (Library/surface3d.h:45-71,
+ * Library/surface3d.c:51-108) and the vertex-to-triangle map
+ * setup_vertex_to_triangle_map_bem3d (Library/bem3d.c:779-804). */
+int h2hip_surface_create(h2hip_surface **out, uint32_t vertices,
+                         const double *x, uint32_t triangles,
+                         const uint32_t *t, const double *n,
+                         const double *g);
+void h2hip_surface_destroy(h2hip_surface *s);
+/* Copies the normals and Gram determinants held by the surface. */
+int h2hip_surface_geometry(const h2hip_surface *s, double *n, double *g);
+
+/* ---- operators -------------------------------------------------------------
+ * Near-field quadrature state of one boundary integral operator.
+ * kernel: H2HIP_LAPLACE / H2HIP_HELMHOLTZ; layer: H2HIP_SLP / H2HIP_DLP;
+ * bases: H2HIP_BASIS_*; q_regular / q_singular: Gauss orders of the regular
+ * and the singular (Sauter-Schwab) rules; alpha: coefficient of the identity
+ * term added to the double layer (ignored for SLP); k: wave number
+ * (Helmholtz only).  alpha and k are (re, im) pairs and may be NULL (0).
+ * Replaces new_slp_laplace_bem3d / new_dlp_laplace_bem3d
+ * (Library/laplacebem3d.c:843-1177) and new_slp_helmholtz_bem3d /
+ * new_dlp_helmholtz_bem3d (Library/helmholtzbem3d.c:1008-1345), together
+ * with build_singquad2d (Library/singquad2d.c:320-403) and the basis
+ * weighting of the rules (Library/singquad2d.c:456-530). */
+int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
+                    int layer, int row_basis, int col_basis,
+                    uint32_t q_regular, uint32_t q_singular,
+                    const double *alpha, const double *k);
+void h2hip_op_destroy(h2hip_op *op);
+/* Precision of the kernel evaluations: 0 FP64 (default), 1 mixed (FP32
+ * evaluation and accumulation of regular pairs, FP64 geometry differences
+ * and accumulation for singular pairs). */
+int h2hip_op_set_precision(h2hip_op *op, int mode);
+
+/* ---- the near-field callback ---------------------------------------------
+ * Fills the rows x cols window of the column-major HOST array N (leading
+ * dimension ld) with the Galerkin matrix entries for the row DOFs ridx and
+ * the column DOFs cidx (NULL: 0, 1, 2, ...).  With ntrans != 0 the
+ * transposed cols x rows window is written (conjugated for the cc and ll
+ * basis pairs, as in the reference).  far != 0 uses the regular rule for all
+ * pairs (nearfield_far).  Blocks until N is written.
+ * Replaces the callbacks bem->nearfield and bem->nearfield_far
+ * (Library/bem3d.h:380-381, 405-406), i.e. assemble_{cc,cl,lc,ll}_near_bem3d
+ * and assemble_{cc,cl,lc,ll}_far_bem3d (Library/bem3d.c:1150-3436). */
+int h2hip_nearfield(const h2hip_op *op, int far, const uint32_t *ridx,
+                    uint32_t rows, const uint32_t *cidx, uint32_t cols,
+                    int ntrans, void *N, size_t ld);
+
+/* ---- batched, device-resident near field -----------------------------------
+ * A plan covers many blocks of one index set pair at once: block b is the
+ * rows x cols window ridx_all[roff .. roff+rows) x cidx_all[coff ..
+ * coff+cols), given as blocks[4b .. 4b+3] = (roff, rows, coff, cols).  The
+ * packed output stores block b column-major with leading dimension rows at
+ * element offset off[b] (h2hip_nfplan_offsets).  This is the near-field
+ * part of assemble_bem3d_h2matrix (Library/bem3d.c:10313-10346), which calls
+ * bem->nearfield for every inadmissible leaf block. */
+int h2hip_nfplan_create(h2hip_nfplan **out, const h2hip_surface *s,
+                        int row_basis, int col_basis,
+                        const uint32_t *ridx_all, uint32_t nridx,
+                        const uint32_t *cidx_all, uint32_t ncidx,
+                        uint32_t nblocks, const uint32_t *blocks);
+void h2hip_nfplan_destroy(h2hip_nfplan *p);
+/* Total number of entries of all blocks. */
+uint64_t h2hip_nfplan_entries(const h2hip_nfplan *p);
+/* Element offsets of the blocks in the packed output (nblocks values). */
+int h2hip_nfplan_offsets(const h2hip_nfplan *p, uint64_t *off);
+/* Kernel evaluations per rule class (regular, vertex, edge, identical) and
+ * triangle pairs per class, as the reference's loops perform them
+ * (counted on the host; out[0..3] evaluations, out[4..7] pairs). */
+int h2hip_nfplan_work(const h2hip_nfplan *p, const h2hip_op *op, int far,
+                      double *out);
+/* Computes all blocks of the plan into the device array d_out (doubles or
+ * complex pairs, h2hip_nfplan_entries elements), on the given stream,
+ * without synchronising. */
+int h2hip_nearfield_execute(const h2hip_op *op, const h2hip_nfplan *p,
+                            int far, void *d_out, void *stream);
+
+/* Waits for all work queued on the stream. */
+int h2hip_stream_sync(void *stream);
+
+/* ---- device memory, streams and events -----------------------------------
+ * Thin wrappers over the HIP runtime for callers without their own device
+ * allocator (tests, benchmarks, host programs).  No counterpart in the
+ * reference, which keeps all data in host memory (allocmem,
+ * Library/basic.h:512).  kind: 0 host to device, 1 device to host,
+ * 2 device to device. */
+int h2hip_malloc(void **ptr, size_t bytes);
+int h2hip_free(void *ptr);
+int h2hip_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int h2hip_memset(void *ptr, int value, size_t bytes);
+int h2hip_stream_create(void **stream);
+int h2hip_stream_destroy(void *stream);
+int h2hip_event_create(void **event);
+int h2hip_event_destroy(void *event);
+int h2hip_event_record(void *event, void *stream);
+/* Milliseconds between two recorded events (waits for the second). */
+int h2hip_event_elapsed(void *start, void *stop, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

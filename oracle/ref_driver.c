@@ -1,0 +1,510 @@
+/* Flat C entry points over the reference library (H2Lib), for test use only.
+ *
+ * This file is compiled together with the reference's own Library/*.c
+ * sources into oracle/_ref/libh2ref_{real,complex}.so by oracle/Makefile.
+ * It adds nothing to the reference's algorithms: every function below only
+ * converts plain arrays into the reference's objects, calls the reference,
+ * and copies results back into plain arrays, so that the tests can compare
+ * the HIP path with the reference on identical inputs.
+ *
+ * Reference entry points used (paths relative to the reference root):
+ *   new_sphere/cube/cylinder_macrosurface3d   Library/macrosurface3d.c:81,390,432
+ *   build_from_macrosurface3d_surface3d       Library/macrosurface3d.c:707
+ *   build_singquad2d                          Library/singquad2d.c:320
+ *   new_{slp,dlp}_laplace_bem3d               Library/laplacebem3d.c:844,1021
+ *   new_{slp,dlp}_helmholtz_bem3d             Library/helmholtzbem3d.c:1008,1177
+ *   bem->nearfield / nearfield_far            Library/bem3d.h:380,405
+ *   build_bem3d_cluster                       Library/bem3d.c:935
+ *   build_strict_block                        Library/block.c:258
+ *   setup_h2matrix_aprx_inter_bem3d           Library/bem3d.c:9658
+ *   assemble_bem3d_h2matrix(_row/_col_clusterbasis)  Library/bem3d.c:10313,10407
+ *   addeval/addevaltrans_h2matrix_avector     Library/h2matrix.c:1268,1345
+ *   solve_gmres_avector                       Library/krylovsolvers.c:183
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "basic.h"
+#include "block.h"
+#include "cluster.h"
+#include "clusterbasis.h"
+#include "h2matrix.h"
+#include "krylovsolvers.h"
+#include "laplacebem3d.h"
+#include "macrosurface3d.h"
+#include "singquad2d.h"
+#include "surface3d.h"
+#ifdef USE_COMPLEX
+#include "helmholtzbem3d.h"
+#endif
+
+static int ref_initialised = 0;
+
+static void
+ref_init(void)
+{
+  if (!ref_initialised) {
+    int argc = 0;
+    char **argv = NULL;
+    init_h2lib(&argc, &argv);
+    ref_initialised = 1;
+  }
+}
+
+int
+ref_is_complex(void)
+{
+#ifdef USE_COMPLEX
+  return 1;
+#else
+  return 0;
+#endif
+}
+
+/* ---------------------------------------------------------------
+ * Meshes
+ * --------------------------------------------------------------- */
+
+static psurface3d
+ref_build_mesh(int kind, unsigned split)
+{
+  pmacrosurface3d mg;
+  psurface3d gr;
+
+  switch (kind) {
+  case 0:
+    mg = new_sphere_macrosurface3d();
+    break;
+  case 1:
+    mg = new_cube_macrosurface3d();
+    break;
+  case 2:
+    mg = new_cylinder_macrosurface3d();
+    break;
+  default:
+    return NULL;
+  }
+  gr = build_from_macrosurface3d_surface3d(mg, split);
+  del_macrosurface3d(mg);
+  return gr;
+}
+
+/* Sizes of a refined macro surface: vertices and triangles. */
+int
+ref_mesh_size(int kind, unsigned split, unsigned *vertices,
+	      unsigned *triangles)
+{
+  psurface3d gr;
+
+  ref_init();
+  gr = ref_build_mesh(kind, split);
+  if (gr == NULL)
+    return -1;
+  *vertices = gr->vertices;
+  *triangles = gr->triangles;
+  del_surface3d(gr);
+  return 0;
+}
+
+/* Vertex coordinates x[3V], triangles t[3T], unit normals n[3T], g[T]. */
+int
+ref_mesh_fill(int kind, unsigned split, double *x, unsigned *t, double *n,
+	      double *g)
+{
+  psurface3d gr;
+  unsigned  i, j;
+
+  ref_init();
+  gr = ref_build_mesh(kind, split);
+  if (gr == NULL)
+    return -1;
+  for (i = 0; i < gr->vertices; i++)
+    for (j = 0; j < 3; j++)
+      x[3 * i + j] = gr->x[i][j];
+  for (i = 0; i < gr->triangles; i++) {
+    for (j = 0; j < 3; j++) {
+      t[3 * i + j] = gr->t[i][j];
+      n[3 * i + j] = gr->n[i][j];
+    }
+    g[i] = gr->g[i];
+  }
+  del_surface3d(gr);
+  return 0;
+}
+
+/* surface3d from plain arrays; prepare_surface3d computes n and g. */
+static psurface3d
+ref_surface_from_arrays(unsigned nv, const double *x, unsigned nt,
+			const unsigned *t)
+{
+  psurface3d gr;
+  unsigned  i, j;
+
+  gr = new_surface3d(nv, 1, nt);
+  for (i = 0; i < nv; i++)
+    for (j = 0; j < 3; j++)
+      gr->x[i][j] = x[3 * i + j];
+  for (i = 0; i < nt; i++)
+    for (j = 0; j < 3; j++) {
+      gr->t[i][j] = t[3 * i + j];
+      gr->s[i][j] = 0;
+    }
+  gr->e[0][0] = gr->e[0][1] = 0;
+  prepare_surface3d(gr);
+  return gr;
+}
+
+/* ---------------------------------------------------------------
+ * Quadrature tables
+ * --------------------------------------------------------------- */
+
+/* cls: 0 dist, 1 vert, 2 edge, 3 id, 4 single.  Writes n points:
+ * x[2n] (x0 then x1), y[2n], w[n] (unweighted base column), base. */
+int
+ref_singquad(unsigned q, unsigned q2, int cls, double *x, double *y,
+	     double *w, unsigned *n, double *base)
+{
+  psingquad2d sq;
+  real     *xs, *ys, *ws;
+  unsigned  nq, i, wcol;
+
+  ref_init();
+  sq = build_singquad2d(NULL, q, q2);
+  switch (cls) {
+  case 0:
+    xs = sq->x_dist; ys = sq->y_dist; ws = sq->w_dist; nq = sq->n_dist;
+    *base = sq->base_dist; wcol = 9;
+    break;
+  case 1:
+    xs = sq->x_vert; ys = sq->y_vert; ws = sq->w_vert; nq = sq->n_vert;
+    *base = sq->base_vert; wcol = 9;
+    break;
+  case 2:
+    xs = sq->x_edge; ys = sq->y_edge; ws = sq->w_edge; nq = sq->n_edge;
+    *base = sq->base_edge; wcol = 9;
+    break;
+  case 3:
+    xs = sq->x_id; ys = sq->y_id; ws = sq->w_id; nq = sq->n_id;
+    *base = sq->base_id; wcol = 9;
+    break;
+  case 4:
+    xs = sq->x_single; ys = sq->y_single; ws = sq->w_single;
+    nq = sq->n_single; *base = sq->base_single; wcol = 3;
+    break;
+  default:
+    del_singquad2d(sq);
+    return -1;
+  }
+  *n = nq;
+  if (x != NULL) {
+    if (cls == 4) {
+      for (i = 0; i < nq; i++) {
+	x[i] = xs[i];
+	y[i] = ys[i];
+	w[i] = ws[i + wcol * nq];
+      }
+    }
+    else {
+      for (i = 0; i < nq; i++) {
+	x[i] = xs[i];
+	x[i + nq] = xs[i + nq];
+	y[i] = ys[i];
+	y[i + nq] = ys[i + nq];
+	w[i] = ws[i + wcol * nq];
+      }
+    }
+  }
+  del_singquad2d(sq);
+  return 0;
+}
+
+/* ---------------------------------------------------------------
+ * BEM operators and the near-field callback
+ * --------------------------------------------------------------- */
+
+typedef struct {
+  psurface3d gr;
+  pbem3d    bem;
+  int       basis_r, basis_c;
+} refbem;
+
+/* kernel: 0 Laplace, 1 Helmholtz.  op: 0 SLP, 1 DLP.  basis: 0 constant,
+ * 1 linear.  alpha and k are complex numbers as (re, im) pairs. */
+void     *
+ref_bem_new(int kernel, int op, int basis_r, int basis_c, unsigned nv,
+	    const double *x, unsigned nt, const unsigned *t, unsigned q,
+	    unsigned q2, const double *alpha, const double *k)
+{
+  refbem   *rb;
+  basisfunctionbem3d br, bc;
+  field     a;
+
+  ref_init();
+  rb = (refbem *) calloc(1, sizeof(refbem));
+  rb->gr = ref_surface_from_arrays(nv, x, nt, t);
+  rb->basis_r = basis_r;
+  rb->basis_c = basis_c;
+  br = basis_r ? BASIS_LINEAR_BEM3D : BASIS_CONSTANT_BEM3D;
+  bc = basis_c ? BASIS_LINEAR_BEM3D : BASIS_CONSTANT_BEM3D;
+#ifdef USE_COMPLEX
+  a = alpha[0] + alpha[1] * I;
+#else
+  a = alpha[0];
+#endif
+  if (kernel == 0) {
+    if (op == 0)
+      rb->bem = new_slp_laplace_bem3d(rb->gr, q, q2, br, bc);
+    else
+      rb->bem = new_dlp_laplace_bem3d(rb->gr, q, q2, br, bc, a);
+  }
+  else {
+#ifdef USE_COMPLEX
+    field     kk = k[0] + k[1] * I;
+    if (op == 0)
+      rb->bem = new_slp_helmholtz_bem3d(kk, rb->gr, q, q2, br, bc);
+    else
+      rb->bem = new_dlp_helmholtz_bem3d(kk, rb->gr, q, q2, br, bc, a);
+#else
+    (void) k;
+    del_surface3d(rb->gr);
+    free(rb);
+    return NULL;
+#endif
+  }
+  return rb;
+}
+
+void
+ref_bem_del(void *h)
+{
+  refbem   *rb = (refbem *) h;
+
+  if (rb == NULL)
+    return;
+  del_bem3d(rb->bem);
+  del_surface3d(rb->gr);
+  free(rb);
+}
+
+/* Calls bem->nearfield (far != 0: bem->nearfield_far) on the
+ * rows x cols window of the column-major array N (leading dimension ld).
+ * ridx / cidx may be NULL (identity).  With ntrans the callee writes the
+ * cols x rows transposed window, exactly as the reference does. */
+void
+ref_nearfield(void *h, int far, const unsigned *ridx, unsigned rows,
+	      const unsigned *cidx, unsigned cols, int ntrans, void *N,
+	      unsigned ld)
+{
+  refbem   *rb = (refbem *) h;
+  amatrix   tmp;
+  pamatrix  A;
+
+  A = init_pointer_amatrix(&tmp, (field *) N, ntrans ? cols : rows,
+			   ntrans ? rows : cols);
+  /* init_pointer_amatrix (Library/amatrix.c:86-95) stores rows as the
+   * column count; set the window explicitly. */
+  A->rows = ntrans ? cols : rows;
+  A->cols = ntrans ? rows : cols;
+  A->ld = ld;
+  if (far)
+    rb->bem->nearfield_far(ridx, cidx, rb->bem, ntrans ? true : false, A);
+  else
+    rb->bem->nearfield(ridx, cidx, rb->bem, ntrans ? true : false, A);
+}
+
+/* ---------------------------------------------------------------
+ * H2-matrix by interpolation
+ * --------------------------------------------------------------- */
+
+typedef struct {
+  refbem   *rb;
+  pcluster  root;
+  uint     *idx;
+  pblock    broot;
+  pclusterbasis rbasis, cbasis;
+  ph2matrix h2;
+  real      eta;
+} refh2;
+
+/* admis: 0 admissible_max_cluster, 1 admissible_2_cluster. */
+void     *
+ref_h2_new(void *h, unsigned clf, double eta, unsigned m, int admis)
+{
+  refbem   *rb = (refbem *) h;
+  refh2    *r;
+  basisfunctionbem3d basis;
+
+  r = (refh2 *) calloc(1, sizeof(refh2));
+  r->rb = rb;
+  r->eta = eta;
+  /* The reference builds row and column trees from the row basis; the
+   * operators used here have equal row and column bases. */
+  basis = rb->basis_r ? BASIS_LINEAR_BEM3D : BASIS_CONSTANT_BEM3D;
+  r->root = build_bem3d_cluster(rb->bem, clf, basis);
+  r->idx = r->root->idx;
+  r->broot = build_strict_block(r->root, r->root, &r->eta,
+				admis ? admissible_2_cluster :
+				admissible_max_cluster);
+  r->rbasis = build_from_cluster_clusterbasis(r->root);
+  r->cbasis = build_from_cluster_clusterbasis(r->root);
+  setup_h2matrix_aprx_inter_bem3d(rb->bem, r->rbasis, r->cbasis, r->broot, m);
+  assemble_bem3d_h2matrix_row_clusterbasis(rb->bem, r->rbasis);
+  assemble_bem3d_h2matrix_col_clusterbasis(rb->bem, r->cbasis);
+  r->h2 = build_from_block_h2matrix(r->broot, r->rbasis, r->cbasis);
+  assemble_bem3d_h2matrix(rb->bem, r->h2);
+  return r;
+}
+
+void
+ref_h2_del(void *h)
+{
+  refh2    *r = (refh2 *) h;
+
+  if (r == NULL)
+    return;
+  del_h2matrix(r->h2);
+  del_block(r->broot);
+  del_cluster(r->root);
+  freemem(r->idx);
+  free(r);
+}
+
+/* Index permutation of the root cluster (length = number of DOFs). */
+unsigned
+ref_h2_idx(void *h, unsigned *idx)
+{
+  refh2    *r = (refh2 *) h;
+  unsigned  i;
+
+  if (idx)
+    for (i = 0; i < r->root->size; i++)
+      idx[i] = r->root->idx[i];
+  return r->root->size;
+}
+
+static void
+ref_collect_near(pch2matrix G, const uint *idx0, unsigned *cnt,
+		 unsigned *blk, unsigned cap)
+{
+  uint      i, n;
+
+  if (G->f) {
+    if (blk && *cnt < cap) {
+      blk[4 * *cnt + 0] = (unsigned) (G->rb->t->idx - idx0);
+      blk[4 * *cnt + 1] = G->rb->t->size;
+      blk[4 * *cnt + 2] = (unsigned) (G->cb->t->idx - idx0);
+      blk[4 * *cnt + 3] = G->cb->t->size;
+    }
+    (*cnt)++;
+  }
+  else if (G->son) {
+    n = G->rsons * G->csons;
+    for (i = 0; i < n; i++)
+      ref_collect_near(G->son[i], idx0, cnt, blk, cap);
+  }
+}
+
+/* Near-field (inadmissible leaf) blocks in depth-first order.  Each block
+ * is (row offset into idx, rows, column offset into idx, cols).  Returns the
+ * number of blocks; writes at most cap blocks when blk != NULL. */
+unsigned
+ref_h2_near_blocks(void *h, unsigned *blk, unsigned cap)
+{
+  refh2    *r = (refh2 *) h;
+  unsigned  cnt = 0;
+
+  ref_collect_near(r->h2, r->root->idx, &cnt, blk, cap);
+  return cnt;
+}
+
+/* Copies the entries of near block number b (depth-first order) into a
+ * column-major rows x cols array. */
+static int
+ref_copy_near(pch2matrix G, unsigned *cnt, unsigned b, void *out)
+{
+  uint      i, j, n;
+
+  if (G->f) {
+    if (*cnt == b) {
+      field    *o = (field *) out;
+      for (j = 0; j < G->f->cols; j++)
+	for (i = 0; i < G->f->rows; i++)
+	  o[i + j * G->f->rows] = G->f->a[i + j * G->f->ld];
+      return 1;
+    }
+    (*cnt)++;
+  }
+  else if (G->son) {
+    n = G->rsons * G->csons;
+    for (i = 0; i < n; i++)
+      if (ref_copy_near(G->son[i], cnt, b, out))
+	return 1;
+  }
+  return 0;
+}
+
+int
+ref_h2_near_entries(void *h, unsigned b, void *out)
+{
+  refh2    *r = (refh2 *) h;
+  unsigned  cnt = 0;
+
+  return ref_copy_near(r->h2, &cnt, b, out) ? 0 : -1;
+}
+
+/* Sizes: number of admissible blocks, near entries, far (coupling)
+ * entries, and bytes of the whole H2-matrix as the reference counts them. */
+void
+ref_h2_sizes(void *h, double *out)
+{
+  refh2    *r = (refh2 *) h;
+
+  out[0] = (double) r->h2->desc;
+  out[1] = getnearsize_h2matrix(r->h2);
+  out[2] = getfarsize_h2matrix(r->h2);
+  out[3] = getsize_h2matrix(r->h2);
+  out[4] = getsize_clusterbasis(r->rbasis);
+  out[5] = getsize_clusterbasis(r->cbasis);
+}
+
+/* y += alpha H x (trans: y += alpha H^* x); alpha as (re, im). */
+void
+ref_h2_addeval(void *h, int trans, const double *alpha, const void *x,
+	       void *y)
+{
+  refh2    *r = (refh2 *) h;
+  avector   xt, yt;
+  pavector  xv, yv;
+  uint      n = r->root->size;
+  field     a;
+
+#ifdef USE_COMPLEX
+  a = alpha[0] + alpha[1] * I;
+#else
+  a = alpha[0];
+#endif
+  xv = init_pointer_avector(&xt, (field *) x, n);
+  yv = init_pointer_avector(&yt, (field *) y, n);
+  if (trans)
+    addevaltrans_h2matrix_avector(a, r->h2, xv, yv);
+  else
+    addeval_h2matrix_avector(a, r->h2, xv, yv);
+}
+
+/* GMRES(kmax) on the H2-matrix: solves H x = b to |r| <= eps |b|.
+ * x holds the start vector on entry.  Returns the iteration count. */
+unsigned
+ref_h2_gmres(void *h, const void *b, void *x, double eps, unsigned maxiter,
+	     unsigned kmax)
+{
+  refh2    *r = (refh2 *) h;
+  avector   bt, xt;
+  pavector  bv, xv;
+  uint      n = r->root->size;
+
+  bv = init_pointer_avector(&bt, (field *) b, n);
+  xv = init_pointer_avector(&xt, (field *) x, n);
+  return solve_gmres_avector((void *) r->h2,
+			     (addeval_t) addeval_h2matrix_avector, bv, xv,
+			     eps, maxiter, kmax);
+}

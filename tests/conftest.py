@@ -1,0 +1,53 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TESTS = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(TESTS, "golden")
+for p in (ROOT, TESTS):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers",
+                            "gpu: needs an MI355X (runs the HIP library)")
+
+
+def golden(name):
+    """Load a committed fixture (.npz, no pickles)."""
+    with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+@pytest.fixture(scope="session")
+def sphere4():
+    return golden("mesh_sphere_s4.npz")
+
+
+@pytest.fixture(scope="session")
+def sphere8():
+    return golden("mesh_sphere_s8.npz")
+
+
+def gpu_available():
+    try:
+        import h2lib_amd
+        return h2lib_amd.hip().h2hip_device_count() > 0
+    except Exception:
+        return False
+
+
+def rel_block_err(a, b):
+    """max |a - b| / max |b| over the finite entries; inf if the non-finite
+    patterns differ."""
+    fa, fb = np.isfinite(a), np.isfinite(b)
+    if not np.array_equal(fa, fb):
+        return np.inf
+    if not fb.any():
+        return 0.0
+    den = np.abs(b[fb]).max()
+    return float(np.abs(a[fb] - b[fb]).max() / (den if den > 0 else 1.0))

@@ -1,0 +1,309 @@
+"""ctypes access to the test oracles.
+
+* ``Oracle``   -- the CPU restatement (oracle/liboracle.so).
+* ``Reference`` -- the reference library compiled from its sources together
+  with oracle/ref_driver.c (oracle/_ref/libh2ref_{real,complex}.so).
+
+Both are checkers only: the product path never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = {False: os.path.join(ROOT, "oracle", "_ref", "libh2ref_real.so"),
+          True: os.path.join(ROOT, "oracle", "_ref", "libh2ref_complex.so")}
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _idx(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def have_reference(complex_=False):
+    return os.path.exists(REF_SO[complex_])
+
+
+class Oracle:
+    """The CPU restatement of the reference loops."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            lib = C.CDLL(ORACLE_SO)
+            lib.orc_bem_new.restype = C.c_void_p
+            lib.orc_bem_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_uint, C.c_void_p, C.c_uint,
+                                        C.c_void_p, C.c_uint, C.c_uint,
+                                        C.c_void_p, C.c_void_p]
+            lib.orc_bem_del.argtypes = [C.c_void_p]
+            lib.orc_nearfield.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
+                                          C.c_uint, C.c_void_p, C.c_uint,
+                                          C.c_int, C.c_void_p, C.c_size_t]
+            lib.orc_nearfield_evals.restype = C.c_double
+            lib.orc_nearfield_evals.argtypes = [C.c_void_p, C.c_int,
+                                                C.c_void_p, C.c_uint,
+                                                C.c_void_p, C.c_uint]
+            lib.orc_sq_new.restype = C.c_void_p
+            lib.orc_sq_new.argtypes = [C.c_uint, C.c_uint]
+            lib.orc_sq_del.argtypes = [C.c_void_p]
+            lib.orc_sq_export.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_void_p,
+                                          C.POINTER(C.c_uint),
+                                          C.POINTER(C.c_double)]
+            lib.orc_gauss.argtypes = [C.c_uint, C.c_void_p, C.c_void_p]
+            lib.orc_bem_gram.restype = C.c_void_p
+            lib.orc_bem_gram.argtypes = [C.c_void_p]
+            lib.orc_bem_normals.restype = C.c_void_p
+            lib.orc_bem_normals.argtypes = [C.c_void_p]
+            cls._lib = lib
+        return cls._lib
+
+    @classmethod
+    def gauss(cls, m):
+        x = np.zeros(m)
+        w = np.zeros(m)
+        cls.lib().orc_gauss(m, _ptr(x), _ptr(w))
+        return x, w
+
+    @classmethod
+    def rule(cls, q, q2, cls_id):
+        lib = cls.lib()
+        sq = lib.orc_sq_new(q, q2)
+        n = C.c_uint()
+        base = C.c_double()
+        lib.orc_sq_export(sq, cls_id, None, None, None, C.byref(n),
+                          C.byref(base))
+        xc = 1 if cls_id == 4 else 2
+        x = np.zeros(xc * n.value)
+        y = np.zeros(xc * n.value)
+        w = np.zeros(n.value)
+        lib.orc_sq_export(sq, cls_id, _ptr(x), _ptr(y), _ptr(w), C.byref(n),
+                          C.byref(base))
+        lib.orc_sq_del(sq)
+        return x, y, w, base.value
+
+
+class _BemBase:
+    """Common interface of the restatement and the reference operators."""
+
+    def __init__(self, kernel, op, basis_r, basis_c, x, t, q=2, q2=4,
+                 alpha=0.0, k=0.0):
+        self.kernel, self.op = kernel, op
+        self.basis_r, self.basis_c = basis_r, basis_c
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        self.t = np.ascontiguousarray(t, dtype=np.uint32).reshape(-1, 3)
+        self.complex = kernel == 1
+        self.dtype = np.complex128 if self.complex else np.float64
+        a = np.array([np.real(alpha), np.imag(alpha)], dtype=np.float64)
+        kk = np.array([np.real(k), np.imag(k)], dtype=np.float64)
+        self._keep = (a, kk)
+        self.h = self._new(kernel, op, basis_r, basis_c, len(self.x),
+                           _ptr(self.x), len(self.t), _ptr(self.t), q, q2,
+                           _ptr(a), _ptr(kk))
+        if not self.h:
+            raise RuntimeError("operator construction failed")
+
+    def ndof(self, basis):
+        return len(self.x) if basis else len(self.t)
+
+    def nearfield(self, ridx=None, cidx=None, far=False, ntrans=False,
+                  rows=None, cols=None):
+        """Dense block from the nearfield callback (column-major -> ndarray
+        of shape (rows, cols), or (cols, rows) with ntrans)."""
+        ridx = _idx(ridx)
+        cidx = _idx(cidx)
+        rows = len(ridx) if ridx is not None else (
+            rows or self.ndof(self.basis_r))
+        cols = len(cidx) if cidx is not None else (
+            cols or self.ndof(self.basis_c))
+        shp = (cols, rows) if ntrans else (rows, cols)
+        # The reference adds the cc identity term at (t, t), which can lie
+        # outside the window when rows > cols: allocate max(rows, cols)
+        # columns so that such writes stay inside the buffer, and return the
+        # window only.
+        ncol = max(rows, cols)
+        N = np.full(shp[0] * ncol, np.nan, dtype=self.dtype)
+        self._nearfield(self.h, int(far), _ptr(ridx), rows, _ptr(cidx), cols,
+                        int(ntrans), _ptr(N), shp[0])
+        return N.reshape(ncol, shp[0]).T[:, :shp[1]].copy()
+
+
+class OracleBem(_BemBase):
+    def __init__(self, *a, **kw):
+        lib = Oracle.lib()
+        self._lib = lib
+        self._new = lib.orc_bem_new
+        self._nearfield = lib.orc_nearfield
+        super().__init__(*a, **kw)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.orc_bem_del(self.h)
+            self.h = None
+
+    def evals(self, ridx=None, cidx=None, far=False):
+        ridx = _idx(ridx)
+        cidx = _idx(cidx)
+        rows = len(ridx) if ridx is not None else self.ndof(self.basis_r)
+        cols = len(cidx) if cidx is not None else self.ndof(self.basis_c)
+        return self._lib.orc_nearfield_evals(self.h, int(far), _ptr(ridx),
+                                             rows, _ptr(cidx), cols)
+
+    def gram(self):
+        p = self._lib.orc_bem_gram(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)),
+                                     shape=(len(self.t),)).copy()
+
+
+class Reference:
+    """The reference library built from its sources (real or complex)."""
+
+    _libs = {}
+
+    @classmethod
+    def lib(cls, complex_=False):
+        if complex_ not in cls._libs:
+            lib = C.CDLL(REF_SO[complex_], mode=C.RTLD_LOCAL)
+            lib.ref_mesh_size.argtypes = [C.c_int, C.c_uint,
+                                          C.POINTER(C.c_uint),
+                                          C.POINTER(C.c_uint)]
+            lib.ref_mesh_fill.argtypes = [C.c_int, C.c_uint, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]
+            lib.ref_singquad.argtypes = [C.c_uint, C.c_uint, C.c_int,
+                                         C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.POINTER(C.c_uint),
+                                         C.POINTER(C.c_double)]
+            lib.ref_bem_new.restype = C.c_void_p
+            lib.ref_bem_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_uint, C.c_void_p, C.c_uint,
+                                        C.c_void_p, C.c_uint, C.c_uint,
+                                        C.c_void_p, C.c_void_p]
+            lib.ref_bem_del.argtypes = [C.c_void_p]
+            lib.ref_nearfield.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
+                                          C.c_uint, C.c_void_p, C.c_uint,
+                                          C.c_int, C.c_void_p, C.c_uint]
+            lib.ref_h2_new.restype = C.c_void_p
+            lib.ref_h2_new.argtypes = [C.c_void_p, C.c_uint, C.c_double,
+                                       C.c_uint, C.c_int]
+            lib.ref_h2_del.argtypes = [C.c_void_p]
+            lib.ref_h2_idx.restype = C.c_uint
+            lib.ref_h2_idx.argtypes = [C.c_void_p, C.c_void_p]
+            lib.ref_h2_near_blocks.restype = C.c_uint
+            lib.ref_h2_near_blocks.argtypes = [C.c_void_p, C.c_void_p,
+                                               C.c_uint]
+            lib.ref_h2_near_entries.argtypes = [C.c_void_p, C.c_uint,
+                                                C.c_void_p]
+            lib.ref_h2_sizes.argtypes = [C.c_void_p, C.c_void_p]
+            lib.ref_h2_addeval.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
+            lib.ref_h2_gmres.restype = C.c_uint
+            lib.ref_h2_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_double, C.c_uint, C.c_uint]
+            cls._libs[complex_] = lib
+        return cls._libs[complex_]
+
+    @classmethod
+    def mesh(cls, kind, split):
+        """kind: 0 sphere, 1 cube, 2 cylinder.  Returns x, t, n, g."""
+        lib = cls.lib(False)
+        nv, nt = C.c_uint(), C.c_uint()
+        lib.ref_mesh_size(kind, split, C.byref(nv), C.byref(nt))
+        x = np.zeros((nv.value, 3))
+        t = np.zeros((nt.value, 3), dtype=np.uint32)
+        n = np.zeros((nt.value, 3))
+        g = np.zeros(nt.value)
+        lib.ref_mesh_fill(kind, split, _ptr(x), _ptr(t), _ptr(n), _ptr(g))
+        return x, t, n, g
+
+    @classmethod
+    def rule(cls, q, q2, cls_id):
+        lib = cls.lib(False)
+        n = C.c_uint()
+        base = C.c_double()
+        lib.ref_singquad(q, q2, cls_id, None, None, None, C.byref(n),
+                         C.byref(base))
+        xc = 1 if cls_id == 4 else 2
+        x = np.zeros(xc * n.value)
+        y = np.zeros(xc * n.value)
+        w = np.zeros(n.value)
+        lib.ref_singquad(q, q2, cls_id, _ptr(x), _ptr(y), _ptr(w),
+                         C.byref(n), C.byref(base))
+        return x, y, w, base.value
+
+
+class ReferenceBem(_BemBase):
+    def __init__(self, kernel, op, basis_r, basis_c, x, t, **kw):
+        lib = Reference.lib(kernel == 1)
+        self._lib = lib
+        self._new = lib.ref_bem_new
+        self._nearfield = lib.ref_nearfield
+        super().__init__(kernel, op, basis_r, basis_c, x, t, **kw)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.ref_bem_del(self.h)
+            self.h = None
+
+
+class ReferenceH2:
+    """H2-matrix of a reference operator, built by interpolation."""
+
+    def __init__(self, bem, clf, eta, m, admis=0):
+        self.bem = bem
+        self._lib = bem._lib
+        self.h = self._lib.ref_h2_new(bem.h, clf, eta, m, admis)
+        self.n = self._lib.ref_h2_idx(self.h, None)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.ref_h2_del(self.h)
+            self.h = None
+
+    def idx(self):
+        out = np.zeros(self.n, dtype=np.uint32)
+        self._lib.ref_h2_idx(self.h, _ptr(out))
+        return out
+
+    def near_blocks(self):
+        nb = self._lib.ref_h2_near_blocks(self.h, None, 0)
+        out = np.zeros((nb, 4), dtype=np.uint32)
+        self._lib.ref_h2_near_blocks(self.h, _ptr(out), nb)
+        return out
+
+    def near_entries(self, b, rows, cols):
+        out = np.zeros(rows * cols, dtype=self.bem.dtype)
+        self._lib.ref_h2_near_entries(self.h, b, _ptr(out))
+        return out.reshape(cols, rows).T.copy()
+
+    def sizes(self):
+        out = np.zeros(6)
+        self._lib.ref_h2_sizes(self.h, _ptr(out))
+        return out
+
+    def addeval(self, x, y, alpha=1.0, trans=False):
+        a = np.array([np.real(alpha), np.imag(alpha)])
+        xx = np.ascontiguousarray(x, dtype=self.bem.dtype)
+        yy = np.ascontiguousarray(y, dtype=self.bem.dtype).copy()
+        self._lib.ref_h2_addeval(self.h, int(trans), _ptr(a), _ptr(xx),
+                                 _ptr(yy))
+        return yy
+
+    def gmres(self, b, x0, eps, maxiter, kmax):
+        bb = np.ascontiguousarray(b, dtype=self.bem.dtype)
+        xx = np.ascontiguousarray(x0, dtype=self.bem.dtype).copy()
+        it = self._lib.ref_h2_gmres(self.h, _ptr(bb), _ptr(xx), eps, maxiter,
+                                    kmax)
+        return xx, it
