@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the near-field Galerkin quadrature on one MI355X.
+
+Workload (the reference's H2 Laplace configuration): unit sphere refined
+from the octahedron to 32768 flat triangles (split 64, 16386 vertices),
+piecewise linear basis, single and double layer potential (alpha = 1/2),
+Gauss orders q = 2 (regular) and q2 = 4 (Sauter-Schwab), FP64.  The blocks
+are the near-field leaves of the strict block tree over the adaptive
+cluster tree with leaf size 64 and admissibility max-norm, eta = 2.
+
+One step assembles the near field of both operators: every near-field
+block, into a packed device buffer, with all inputs already resident in
+HBM.  value = near-field entries per second over the timed steps.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("nearfield entries/s; Dirichlet solves/s + p99 latency at "
+          "rel.err 1e-6, 1 GPU")
+PEAK_FP64_TFLOPS = 78.6
+# algorithmic flops per kernel evaluation, following the reference's loop
+# body (Library/bem3d.c:1207-1225): Laplace SLP 46, DLP 53; the linear basis
+# adds 2 * 9 flops per evaluation for the nine weighted sums
+# (Library/bem3d.c:3169-3203) and every entry 2 for the final scaling.
+FLOPS_PER_EVAL = {0: 46.0, 1: 53.0}
+FLOPS_LL_PER_EVAL = 18.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(split, clf, eta):
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(split)
+    tree = G.ClusterTree(mesh, 1, clf)
+    blocks = G.BlockTree(tree, tree, eta, 0)
+    return mesh, tree, blocks, blocks.near_block_list()
+
+
+def cpu_baseline(mesh, tree, nb, entries):
+    """The reference library (built from its sources) on the host cores:
+    the SLP near field of all blocks, OpenMP over blocks."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ffi import ReferenceBem, have_reference
+    if not have_reference(False):
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref = ReferenceBem(0, 0, 1, 1, mesh.x, mesh.t, q=2, q2=4)
+    t0 = time.perf_counter()
+    ref.nearfield_blocks(tree.idx, tree.idx, nb, threads)
+    dt = time.perf_counter() - t0
+    return {"value": entries / dt, "unit": "entries/s", "cores": threads,
+            "kind": "reference",
+            "sample": ("Laplace SLP near field of all %d near-field blocks "
+                       "(%d entries) of the same workload; reference library "
+                       "compiled from its sources with gcc -O2, OpenMP over "
+                       "blocks; %.2f s" % (len(nb), entries, dt))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--split", type=int, default=64)
+    ap.add_argument("--clf", type=int, default=64)
+    ap.add_argument("--eta", type=float, default=2.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run device-to-device copies of this size "
+                    "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import h2lib_amd as h
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        a, b = h.DeviceArray(n), h.DeviceArray(n)
+        a.zero()
+        for _ in range(3):
+            h.hip().h2hip_memcpy(b._p, a._p, a.nbytes, 2)
+        log("bench: copied %d MiB three times" % args.calibrate_copy_mib)
+        return
+
+    log("bench: building sphere split %d, cluster tree clf %d, eta %g" %
+        (args.split, args.clf, args.eta))
+    t0 = time.perf_counter()
+    mesh, tree, btree, nb = build_workload(args.split, args.clf, args.eta)
+    log("bench: %d triangles, %d vertices, %d near blocks (%.2f s host)" %
+        (mesh.triangles, mesh.vertices, len(nb), time.perf_counter() - t0))
+
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    ops = [h.new_slp_laplace_bem3d(gr, 2, 4, 1, 1),
+           h.new_dlp_laplace_bem3d(gr, 2, 4, 1, 1, 0.5)]
+    plan = h.NearfieldPlan(gr, 1, 1, tree.idx, tree.idx, nb)
+    outs = [h.DeviceArray(plan.entries, np.float64) for _ in ops]
+    stream = h.Stream()
+    entries_per_step = plan.entries * len(ops)
+
+    # algorithmic work per launch
+    flops = []
+    for k, op in enumerate(ops):
+        evals, pairs = plan.work(op)
+        flops.append(evals.sum() * (FLOPS_PER_EVAL[k] + FLOPS_LL_PER_EVAL) +
+                     2.0 * plan.entries)
+    work = plan.work(ops[0])
+
+    for _ in range(args.warmup):
+        for op, out in zip(ops, outs):
+            plan.execute(op, out.ptr, stream.handle)
+    stream.sync()
+
+    # timed region: K steps; events around every launch give the kernel
+    # durations on the stream the kernels run on
+    ev = [[h.Event(), h.Event()] for _ in range(args.steps * len(ops))]
+    t_start, t_stop = h.Event(), h.Event()
+    stream.sync()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    i = 0
+    for _ in range(args.steps):
+        for op, out in zip(ops, outs):
+            ev[i][0].record(stream)
+            plan.execute(op, out.ptr, stream.handle)
+            ev[i][1].record(stream)
+            i += 1
+    t_stop.record(stream)
+    stream.sync()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_ms(t_stop)
+    kms = np.array([a.elapsed_ms(b) for a, b in ev]).reshape(args.steps,
+                                                            len(ops))
+    ms_per_step = total_ms / args.steps
+    value = entries_per_step / (ms_per_step * 1e-3)
+    k_avg = kms.mean(axis=0)
+    achieved = sum(flops) / (k_avg.sum() * 1e-3) / 1e12
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_nearfield_ll_sphere64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: timing the reference on the host cores")
+        cpu = cpu_baseline(mesh, tree, nb, plan.entries)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "entries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (octahedral sphere mesh generated in-process)",
+        "config": {
+            "workload": ("Laplace SLP + DLP near field, piecewise linear "
+                         "basis, unit sphere %d triangles / %d vertices, "
+                         "strict block tree clf=%d eta=%g (max-norm "
+                         "admissibility), q=2 q2=4, FP64" %
+                         (mesh.triangles, mesh.vertices, args.clf, args.eta)),
+            "near_blocks": int(len(nb)),
+            "entries_per_step": int(entries_per_step),
+            "kernel_evaluations_per_launch": float(work[0].sum()),
+            "pairs_per_launch_by_class": [int(v) for v in work[1]],
+            "kernel_ms": {"slp": float(k_avg[0]), "dlp": float(k_avg[1])},
+            "wall_s_timed": wall,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "nf_tile_kernel (ll, Laplace SLP and DLP)",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

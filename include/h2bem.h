@@ -1,0 +1,100 @@
+/* h2bem -- C host library around h2hip: surfaces, cluster trees, block
+ * trees, operators with the reference's near-field callback signature,
+ * H2-matrices built by interpolation, and Krylov solvers.
+ *
+ * It mirrors the reference's (H2Lib's) interfaces on this path with an h2b_
+ * prefix; each declaration names the reference function it corresponds to
+ * (paths relative to the H2Lib source root).  All compute-heavy work goes
+ * through include/h2hip.h.
+ */
+#ifndef H2BEM_H
+#define H2BEM_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "h2hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- surfaces -----------------------------------------------------------
+ * Mirror of surface3d (Library/surface3d.h:45-71). */
+typedef struct {
+  uint32_t vertices, triangles;
+  double *x;   /* 3 * vertices */
+  uint32_t *t; /* 3 * triangles, counter-clockwise */
+  double *n;   /* 3 * triangles, unit normals */
+  double *g;   /* triangles, twice the areas */
+  double hmin, hmax;
+} h2b_surface;
+
+/* Refinement of the octahedral sphere: 8 split^2 triangles, points projected
+ * to the unit sphere.  new_sphere_macrosurface3d (Library/macrosurface3d.c:81)
+ * + build_from_macrosurface3d_surface3d (Library/macrosurface3d.c:707). */
+h2b_surface *h2b_sphere_surface(uint32_t split);
+/* Refinement of the cube [-1,1]^3: 12 split^2 triangles
+ * (new_cube_macrosurface3d, Library/macrosurface3d.c:390). */
+h2b_surface *h2b_cube_surface(uint32_t split);
+/* Surface from arrays; computes n, g, hmin, hmax (prepare_surface3d,
+ * Library/surface3d.c:51-108). */
+h2b_surface *h2b_surface_from_arrays(uint32_t vertices, const double *x,
+                                     uint32_t triangles, const uint32_t *t);
+/* Applies x -> s R x + c to all vertices (R row-major 3x3) and updates n, g. */
+void h2b_surface_transform(h2b_surface *s, const double *R, double scale,
+                           const double *shift);
+void h2b_surface_free(h2b_surface *s);
+
+/* ---- cluster trees ---------------------------------------------------------
+ * Mirror of cluster (Library/cluster.h:40-70), flattened: nodes in
+ * depth-first order, each owning the index range [off, off + size) of the
+ * tree's permutation idx.  basis: 0 constant (DOFs are triangles,
+ * characteristic points are centroids), 1 linear (DOFs are vertices). */
+typedef struct {
+  uint32_t off, size;
+  uint32_t sons;     /* 0 or 2 */
+  uint32_t son[2];   /* node numbers */
+  uint32_t desc;     /* nodes in the subtree */
+  uint32_t level;
+  double bmin[3], bmax[3];
+} h2b_cnode;
+
+typedef struct {
+  uint32_t n;        /* number of DOFs */
+  uint32_t *idx;     /* permutation of the DOFs */
+  uint32_t nnodes;
+  h2b_cnode *nodes;  /* nodes[0] is the root */
+} h2b_ctree;
+
+/* build_bem3d_cluster (Library/bem3d.c:932-954) with
+ * build_adaptive_cluster (Library/cluster.c:80-168). */
+h2b_ctree *h2b_build_cluster(const h2b_surface *s, int basis, uint32_t clf);
+void h2b_ctree_free(h2b_ctree *t);
+
+/* ---- block trees -------------------------------------------------------
+ * build_strict_block (Library/block.c:258-322) with admissibility
+ * admis = 0: admissible_max_cluster (Library/block.c:52-63),
+ * admis = 1: admissible_2_cluster (Library/block.c:29-50).
+ * Leaves are listed in depth-first order of the block tree (sons in the
+ * reference's storage order).  Each leaf is (row node, column node). */
+typedef struct {
+  uint32_t nnear, nfar;
+  uint32_t *near; /* 2 per leaf: row node, column node */
+  uint32_t *far;
+} h2b_blocks;
+
+h2b_blocks *h2b_build_strict_block(const h2b_ctree *rt, const h2b_ctree *ct,
+                                   double eta, int admis);
+void h2b_blocks_free(h2b_blocks *b);
+/* Near-field leaves as (row offset, rows, column offset, cols) into the
+ * trees' idx arrays: the block list of h2hip_nfplan_create. */
+void h2b_near_block_list(const h2b_blocks *b, const h2b_ctree *rt,
+                         const h2b_ctree *ct, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -312,6 +312,35 @@ ref_nearfield(void *h, int far, const unsigned *ridx, unsigned rows,
     rb->bem->nearfield(ridx, cidx, rb->bem, ntrans ? true : false, A);
 }
 
+/* Near-field blocks (roff, rows, coff, cols) over the index arrays ridx /
+ * cidx, block b written column-major to out + off[b] (in field elements).
+ * Blocks run in parallel on `threads` OpenMP threads, as the reference's
+ * H2 assembly runs its leaf blocks on OpenMP tasks (Library/bem3d.c:
+ * 10313-10346); inside a block the callback itself stays sequential for
+ * blocks narrower than 256 columns (Library/bem3d.c:1164-1165). */
+void
+ref_nearfield_blocks(void *h, const unsigned *ridx, const unsigned *cidx,
+		     unsigned nblocks, const unsigned *blocks,
+		     const unsigned long *off, void *out, int threads)
+{
+  refbem   *rb = (refbem *) h;
+  int       b;
+
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+  for (b = 0; b < (int) nblocks; b++) {
+    amatrix   tmp;
+    pamatrix  A;
+    unsigned  rows = blocks[4 * b + 1], cols = blocks[4 * b + 3];
+
+    A = init_pointer_amatrix(&tmp, (field *) out + off[b], rows, cols);
+    A->rows = rows;
+    A->cols = cols;
+    A->ld = rows;
+    rb->bem->nearfield(ridx + blocks[4 * b], cidx + blocks[4 * b + 2],
+		       rb->bem, false, A);
+  }
+}
+
 /* ---------------------------------------------------------------
  * H2-matrix by interpolation
  * --------------------------------------------------------------- */

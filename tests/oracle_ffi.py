@@ -195,6 +195,9 @@ class Reference:
             lib.ref_nearfield.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
                                           C.c_uint, C.c_void_p, C.c_uint,
                                           C.c_int, C.c_void_p, C.c_uint]
+            lib.ref_nearfield_blocks.argtypes = [
+                C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p,
+                C.c_void_p, C.c_void_p, C.c_int]
             lib.ref_h2_new.restype = C.c_void_p
             lib.ref_h2_new.argtypes = [C.c_void_p, C.c_uint, C.c_double,
                                        C.c_uint, C.c_int]
@@ -256,6 +259,21 @@ class ReferenceBem(_BemBase):
         if getattr(self, "h", None):
             self._lib.ref_bem_del(self.h)
             self.h = None
+
+    def nearfield_blocks(self, ridx, cidx, blocks, threads=1):
+        """All blocks (roff, rows, coff, cols) packed as the device plan
+        packs them; returns the flat array."""
+        ridx = _idx(ridx)
+        cidx = _idx(cidx)
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint32).reshape(-1, 4)
+        sizes = blocks[:, 1].astype(np.uint64) * blocks[:, 3]
+        off = np.zeros(len(blocks), dtype=np.uint64)
+        off[1:] = np.cumsum(sizes)[:-1]
+        out = np.zeros(int(sizes.sum()), dtype=self.dtype)
+        self._lib.ref_nearfield_blocks(self.h, _ptr(ridx), _ptr(cidx),
+                                       len(blocks), _ptr(blocks), _ptr(off),
+                                       _ptr(out), int(threads))
+        return out
 
 
 class ReferenceH2:
