@@ -189,7 +189,8 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "nf_tile_kernel (ll, Laplace SLP and DLP)",
+            "kernel": ("nf_regular_kernel + nf_singular_kernel (ll, Laplace SLP "
+                       "and DLP), one launch pair per operator"),
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",

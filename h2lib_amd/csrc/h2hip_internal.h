@@ -79,12 +79,25 @@ PairRule make_identical_rule(unsigned q2);
 // Rule classes by number of shared vertices.
 enum { CLS_REGULAR = 0, CLS_VERTEX = 1, CLS_EDGE = 2, CLS_IDENTICAL = 3 };
 
-// Support entry of a tile side: triangle and the tile-local DOF numbers of
-// its three local basis functions (0xff: not in the tile).  For the
-// constant basis loc0 is the tile-local triangle number.
+// Support entry of a tile side: triangle, its vertices, and the tile-local
+// DOF numbers of its three local basis functions (0xff: not in the tile).
+// For the constant basis loc0 is the tile-local triangle number.
 struct Supp {
   uint32_t tri;
   uint32_t loc;  // loc0 | loc1 << 8 | loc2 << 16
+  uint32_t v[3];
+  uint32_t pad;
+};
+
+// A pair of support triangles with common vertices, evaluated by the
+// singular kernel: tile, row / column support positions, rule class
+// (1 vertex, 2 edge, 3 identical) and the vertex permutations that put the
+// shared vertices first (tp | sp << 2 | ... two bits per entry).
+struct SPair {
+  uint32_t tile;
+  uint16_t r, c;
+  uint16_t cls;
+  uint16_t perm;  // tp0 | tp1 << 2 | tp2 << 4 | sp0 << 6 | sp1 << 8 | sp2 << 10
 };
 
 constexpr int TILE = 64;
@@ -107,6 +120,39 @@ struct Fixup {  // identity term of the cc double layer at (t, t), t != s
   uint32_t pad;
 };
 
+namespace nf {
+
+// Arguments of the near-field kernels.
+struct KArgs {
+  const Tile *tiles;
+  const Supp *supp;
+  const double *x;       // vertices
+  const double *n;       // unit normals
+  const double *g;       // Gram determinants
+  const double *regpts;  // regular-rule points, 3 q^2 per triangle
+  const double *regw;    // regular weights: w | phi0 w | phi1 w | phi2 w
+  const double *sing;    // singular rules (structure of arrays per class)
+  uint32_t sing_off[4];
+  uint32_t sing_n[4];
+  double kc;             // 1 / (4 pi)
+  double kr, ki;         // e^{i k r} with k = kr - i ki
+  double are, aim;       // identity coefficient alpha
+  double mass[9];
+  int far;
+  double *out;
+};
+
+// Launches the regular tile kernel over `ntiles` tiles and then (unless
+// far) the singular kernel over `nsp` pairs, on `stream`.  Returns
+// H2HIP_EINVAL if the order q has no kernel.
+int launch_laplace(int layer, int rb, int cb, uint32_t q, const KArgs &a,
+                   uint32_t ntiles, const SPair *sp, uint32_t nsp,
+                   hipStream_t stream);
+int launch_helmholtz(int layer, int rb, int cb, uint32_t q, const KArgs &a,
+                     uint32_t ntiles, const SPair *sp, uint32_t nsp,
+                     hipStream_t stream);
+
+}  // namespace nf
 }  // namespace h2hip
 
 struct h2hip_surface {
@@ -151,7 +197,10 @@ struct h2hip_nfplan {
   std::vector<h2hip::Tile> tiles;
   std::vector<h2hip::Supp> supp;
   std::vector<h2hip::Fixup> fix;
+  std::vector<h2hip::SPair> spairs;  // sorted by class, identical first
+  uint64_t pairs[4] = {0, 0, 0, 0};  // support pairs per class
   h2hip::DevBuf<h2hip::Tile> d_tiles;
   h2hip::DevBuf<h2hip::Supp> d_supp;
   h2hip::DevBuf<h2hip::Fixup> d_fix;
+  h2hip::DevBuf<h2hip::SPair> d_spairs;
 };

@@ -1,32 +1,21 @@
-// Near-field Galerkin quadrature on gfx950.
+// Near-field Galerkin quadrature on gfx950: planning, execution and the
+// C ABI.  The kernels live in nf_device.h (instantiated per kernel family
+// in nf_laplace.hip and nf_helmholtz.hip).
 //
-// Work decomposition.  Every requested block of the Galerkin matrix is cut
-// into tiles of at most 64 x 64 degrees of freedom.  One workgroup (four
-// wave64s) computes one tile in LDS and writes it out once, coalesced.
+// Planning cuts every requested block into tiles of at most 64 x 64 DOFs.
 // A tile lists the support triangles of its rows and of its columns (for
 // the piecewise constant basis the triangles themselves, for the piecewise
 // linear basis all triangles touching one of the tile's vertices, each with
-// the tile-local numbers of its three vertices).  The tile's work is the set
-// of all pairs (row support triangle, column support triangle), exactly the
-// pairs the reference's loops visit for the same block
-// (Library/bem3d.c:1150-1247, 1798-1971, 2394-2571, 3038-3250).
-//
-//  * Regular pairs (no common vertex) are evaluated one pair per lane with
-//    the tensor-product Gauss rule, using quadrature points precomputed per
-//    triangle: with q = 2 a pair is 16 kernel evaluations from registers.
-//    The local 1x1, 1x3, 3x1 or 3x3 element matrix is added into the LDS tile.
-//  * Singular pairs (common vertex, edge or identical) are queued in LDS
-//    and evaluated afterwards by a whole wavefront each: the lanes stride
-//    over the 2 q2^4 or 6 q2^4 Sauter-Schwab points, the partial sums are
-//    reduced across the wavefront and lane 0 adds the element matrix.  The
-//    geometry differences x - y of these pairs are formed from differences of
-//    the (permuted) vertices, so the shared vertex cancels exactly.
-//
-// The kernel is templated on the kernel function, the two bases and the
-// regular order, so all loops over points and basis functions unroll.
+// the tile-local numbers of its three vertices); its work is the set of all
+// pairs (row support triangle, column support triangle), exactly the pairs
+// the reference's loops visit for the same block (Library/bem3d.c:1150-
+// 1247, 1798-1971, 2394-2571, 3038-3250).  Pairs with common vertices are
+// found from the vertex-to-triangle map (no all-pairs scan) and collected
+// into the singular list with their rule class and vertex permutation.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <unordered_map>
 
 #include "h2hip_internal.h"
@@ -38,409 +27,8 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 namespace {
 
-constexpr int NTHREADS = 256;
-constexpr int NWAVES = NTHREADS / 64;
-constexpr int CHUNK = 1024;  // pairs classified per round (singular queue)
-
-enum { K_LAP_SLP = 0, K_LAP_DLP = 1, K_HELM_SLP = 2, K_HELM_DLP = 3 };
-
-struct KArgs {
-  const Tile *tiles;
-  const Supp *supp;
-  const uint32_t *tri;
-  const double *x;
-  const double *n;
-  const double *g;
-  const double *regpts;
-  const double *regw;
-  const double *sing;
-  uint32_t sing_off[4];
-  uint32_t sing_n[4];
-  double kc;
-  double kr, ki;  // wave number: e^{i k r} with k = kr + i ki' (ki = -Im k)
-  double are, aim;  // alpha
-  double mass[9];
-  int far;
-  double *out;
-};
-
-// Kernel function values without the 1/(4 pi) factor.
-// Laplace SLP 1/r, DLP <d, n_y>/r^3 (Library/laplacebem3d.c:35-108);
-// Helmholtz SLP e^{ikr}/r, DLP e^{ikr} (1 - ikr) <d, n_y>/r^3
-// (Library/helmholtzbem3d.c:26-170), d = x - y, r = |d|.
-template <int KER>
-__device__ __forceinline__ void kval(double dx, double dy, double dz,
-                                     double nx, double ny, double nz,
-                                     double kr, double ki, double &re,
-                                     double &im) {
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  const double rs = rsqrt(r2);
-  if constexpr (KER == K_LAP_SLP) {
-    re = rs;
-    im = 0.0;
-  } else if constexpr (KER == K_LAP_DLP) {
-    re = (dx * nx + dy * ny + dz * nz) * (rs * rs * rs);
-    im = 0.0;
-  } else if constexpr (KER == K_HELM_SLP) {
-    const double r = r2 * rs;
-    double amp = rs;
-    if (ki != 0.0) amp *= exp(ki * r);
-    double s, c;
-    sincos(kr * r, &s, &c);
-    re = amp * c;
-    im = amp * s;
-  } else {
-    const double r = r2 * rs;
-    double amp = (dx * nx + dy * ny + dz * nz) * (rs * rs * rs);
-    if (ki != 0.0) amp *= exp(ki * r);
-    const double kr_r = kr * r;
-    double s, c;
-    sincos(kr_r, &s, &c);
-    re = (c + kr_r * s) * amp;
-    im = (s - c * kr_r) * amp;
-  }
-}
-
-__device__ __forceinline__ uint32_t loc_of(uint32_t packed, int l) {
-  return (packed >> (8 * l)) & 0xffu;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Vertex permutation of a singular pair: shared vertices first, in the
-// order of the first triangle, then the others in their original order
-// (Library/singquad2d.c:549-633).  Returns the number of shared vertices.
-__device__ __forceinline__ int pair_permutation(const uint32_t *tv,
-                                                const uint32_t *sv, int *tp,
-                                                int *sp) {
-  int p = 0;
-  bool tused[3] = {false, false, false}, sused[3] = {false, false, false};
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    bool found = false;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      if (!found && tv[i] == sv[j]) {
-        tp[p] = i;
-        sp[p] = j;
-        tused[i] = true;
-        sused[j] = true;
-        found = true;
-      }
-    }
-    if (found) p++;
-  }
-  int ct = p, cs = p;
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    if (!tused[i]) tp[ct++] = i;
-    if (!sused[i]) sp[cs++] = i;
-  }
-  return p;
-}
-
-template <int KER, int BR, int BC, int Q>
-__global__ __launch_bounds__(NTHREADS) void nf_tile_kernel(KArgs a) {
-  constexpr bool CPLX = KER >= K_HELM_SLP;
-  constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
-  constexpr bool CC = (BR == 0) && (BC == 0);
-  constexpr int NR = BR ? 3 : 1;  // local basis functions per row triangle
-  constexpr int NC = BC ? 3 : 1;
-  constexpr int QQ = Q * Q;
-
-  __shared__ double acc_re[TILE * TILE];
-  __shared__ double acc_im[CPLX ? TILE * TILE : 1];
-  __shared__ uint32_t squeue[CHUNK];
-  __shared__ uint32_t scount;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const Tile T = a.tiles[blockIdx.x];
-  const uint32_t nR = T.nrsup, nC = T.ncsup;
-  const Supp *RS = a.supp + T.rsup;
-  const Supp *CS = a.supp + T.csup;
-
-  if (!CC) {
-    for (int i = tid; i < TILE * TILE; i += NTHREADS) {
-      acc_re[i] = 0.0;
-      if constexpr (CPLX) acc_im[i] = 0.0;
-    }
-  }
-
-  // regular weights: w[QQ] | phi0[QQ] | phi1[QQ] | phi2[QQ]
-  const double *__restrict__ W = a.regw;
-
-  const uint32_t npairs = nR * nC;
-  for (uint32_t base = 0; base < npairs; base += CHUNK) {
-    if (tid == 0) scount = 0;
-    __syncthreads();
-    const uint32_t end = min(npairs, base + CHUNK);
-    for (uint32_t p = base + tid; p < end; p += NTHREADS) {
-      const uint32_t r = p % nR, c = p / nR;
-      const Supp sr = RS[r], sc = CS[c];
-      const uint32_t *tv = a.tri + 3 * sr.tri;
-      const uint32_t *sv = a.tri + 3 * sc.tri;
-      int shared = 0;
-      if (!a.far) {
-        const uint32_t t0 = tv[0], t1 = tv[1], t2 = tv[2];
-        const uint32_t s0 = sv[0], s1 = sv[1], s2 = sv[2];
-        shared = (t0 == s0) + (t0 == s1) + (t0 == s2) + (t1 == s0) +
-                 (t1 == s1) + (t1 == s2) + (t2 == s0) + (t2 == s1) +
-                 (t2 == s2);
-      }
-      if (shared) {
-        const uint32_t slot = atomicAdd(&scount, 1u);
-        squeue[slot] = p - base;
-        continue;
-      }
-      // ---- regular pair: tensor Gauss rule from precomputed points.
-      // The reference's lc far integrator maps the first rule coordinates
-      // onto the column triangle (Library/bem3d.c:2637-2649); kept as is.
-      const bool swapgeo = (BR && !BC) && a.far;
-      const double *__restrict__ Pt =
-          a.regpts + (size_t)(swapgeo ? sc.tri : sr.tri) * (3 * QQ);
-      const double *__restrict__ Ps =
-          a.regpts + (size_t)(swapgeo ? sr.tri : sc.tri) * (3 * QQ);
-      double ptx[QQ], pty[QQ], ptz[QQ];
-#pragma unroll
-      for (int i = 0; i < QQ; i++) {
-        ptx[i] = Pt[3 * i];
-        pty[i] = Pt[3 * i + 1];
-        ptz[i] = Pt[3 * i + 2];
-      }
-      double nx = 0.0, ny = 0.0, nz = 0.0;
-      if constexpr (DLP) {
-        nx = a.n[3 * sc.tri];
-        ny = a.n[3 * sc.tri + 1];
-        nz = a.n[3 * sc.tri + 2];
-      }
-      double Sre[NR][NC], Sim[NR][NC];
-#pragma unroll
-      for (int l = 0; l < NR; l++)
-#pragma unroll
-        for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
-#pragma unroll
-      for (int b = 0; b < QQ; b++) {
-        const double qx = Ps[3 * b], qy = Ps[3 * b + 1], qz = Ps[3 * b + 2];
-        double Ure[NR], Uim[NR];
-#pragma unroll
-        for (int l = 0; l < NR; l++) Ure[l] = Uim[l] = 0.0;
-#pragma unroll
-        for (int i = 0; i < QQ; i++) {
-          double vre, vim;
-          kval<KER>(ptx[i] - qx, pty[i] - qy, ptz[i] - qz, nx, ny, nz, a.kr,
-                    a.ki, vre, vim);
-#pragma unroll
-          for (int l = 0; l < NR; l++) {
-            const double wr = BR ? W[QQ * (1 + l) + i] : W[i];
-            Ure[l] = fma(wr, vre, Ure[l]);
-            if constexpr (CPLX) Uim[l] = fma(wr, vim, Uim[l]);
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-          const double wc = BC ? W[QQ * (1 + k) + b] : W[b];
-#pragma unroll
-          for (int l = 0; l < NR; l++) {
-            Sre[l][k] = fma(wc, Ure[l], Sre[l][k]);
-            if constexpr (CPLX) Sim[l][k] = fma(wc, Uim[l], Sim[l][k]);
-          }
-        }
-      }
-      const double f2 = a.kc * a.g[sr.tri] * a.g[sc.tri];
-#pragma unroll
-      for (int l = 0; l < NR; l++) {
-        const uint32_t lr = loc_of(sr.loc, l);
-        if (lr == 0xffu) continue;
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-          const uint32_t lc = loc_of(sc.loc, k);
-          if (lc == 0xffu) continue;
-          const int e = lr + TILE * lc;
-          if constexpr (CC) {
-            acc_re[e] = f2 * Sre[0][0];
-            if constexpr (CPLX) acc_im[e] = f2 * Sim[0][0];
-          } else {
-            atomicAdd(&acc_re[e], f2 * Sre[l][k]);
-            if constexpr (CPLX) atomicAdd(&acc_im[e], f2 * Sim[l][k]);
-          }
-        }
-      }
-    }
-    __syncthreads();
-
-    // ---- singular pairs: one wavefront per pair
-    const uint32_t ns = scount;
-    for (uint32_t it = wave; it < ns; it += NWAVES) {
-      const uint32_t p = base + squeue[it];
-      const uint32_t r = p % nR, c = p / nR;
-      const Supp sr = RS[r], sc = CS[c];
-      uint32_t tv[3], sv[3];
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        tv[i] = a.tri[3 * sr.tri + i];
-        sv[i] = a.tri[3 * sc.tri + i];
-      }
-      int tp[3], sp[3];
-      const int cls = pair_permutation(tv, sv, tp, sp);
-      const uint32_t nq = a.sing_n[cls];
-      const double *__restrict__ tab = a.sing + a.sing_off[cls];
-      double At[3], Bt[3], Ct[3], As[3], Bs[3], Cs[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        At[d] = a.x[3 * tv[tp[0]] + d];
-        Bt[d] = a.x[3 * tv[tp[1]] + d];
-        Ct[d] = a.x[3 * tv[tp[2]] + d];
-        As[d] = a.x[3 * sv[sp[0]] + d];
-        Bs[d] = a.x[3 * sv[sp[1]] + d];
-        Cs[d] = a.x[3 * sv[sp[2]] + d];
-      }
-      // x - y = D0 + E1 tx + E2 sx - F1 ty - F2 sy
-      double D0[3], E1[3], E2[3], F1[3], F2[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        D0[d] = At[d] - As[d];
-        E1[d] = Bt[d] - At[d];
-        E2[d] = Ct[d] - Bt[d];
-        F1[d] = Bs[d] - As[d];
-        F2[d] = Cs[d] - Bs[d];
-      }
-      double nx = 0.0, ny = 0.0, nz = 0.0;
-      if constexpr (DLP) {
-        nx = a.n[3 * sc.tri];
-        ny = a.n[3 * sc.tri + 1];
-        nz = a.n[3 * sc.tri + 2];
-      }
-      double Sre[NR][NC], Sim[NR][NC];
-#pragma unroll
-      for (int l = 0; l < NR; l++)
-#pragma unroll
-        for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
-      for (uint32_t q = lane; q < nq; q += 64) {
-        const double tx = tab[q], sx = tab[nq + q];
-        const double ty = tab[2 * nq + q], sy = tab[3 * nq + q];
-        const double w = tab[4 * nq + q];
-        double d[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-          d[k] = fma(-F2[k], sy,
-                     fma(-F1[k], ty, fma(E2[k], sx, fma(E1[k], tx, D0[k]))));
-        double vre, vim;
-        kval<KER>(d[0], d[1], d[2], nx, ny, nz, a.kr, a.ki, vre, vim);
-        const double wre = w * vre, wim = w * vim;
-        double phi[NR], psi[NC];
-        if constexpr (BR) {
-          phi[0] = 1.0 - tx;
-          phi[1] = tx - sx;
-          phi[2] = sx;
-        } else {
-          phi[0] = 1.0;
-        }
-        if constexpr (BC) {
-          psi[0] = 1.0 - ty;
-          psi[1] = ty - sy;
-          psi[2] = sy;
-        } else {
-          psi[0] = 1.0;
-        }
-#pragma unroll
-        for (int l = 0; l < NR; l++) {
-          const double ar = wre * phi[l], ai = wim * phi[l];
-#pragma unroll
-          for (int k = 0; k < NC; k++) {
-            Sre[l][k] = fma(ar, psi[k], Sre[l][k]);
-            if constexpr (CPLX) Sim[l][k] = fma(ai, psi[k], Sim[l][k]);
-          }
-        }
-      }
-#pragma unroll
-      for (int l = 0; l < NR; l++)
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-          Sre[l][k] = wave_sum(Sre[l][k]);
-          if constexpr (CPLX) Sim[l][k] = wave_sum(Sim[l][k]);
-        }
-      if (lane == 0) {
-        const double gt = a.g[sr.tri], gs = a.g[sc.tri];
-        const double f2 = a.kc * gt * gs;
-#pragma unroll
-        for (int l = 0; l < NR; l++) {
-          const uint32_t lr = loc_of(sr.loc, BR ? tp[l] : 0);
-          if (lr == 0xffu) continue;
-#pragma unroll
-          for (int k = 0; k < NC; k++) {
-            const uint32_t lc = loc_of(sc.loc, BC ? sp[k] : 0);
-            if (lc == 0xffu) continue;
-            const int e = lr + TILE * lc;
-            if constexpr (CC) {
-              double vre = f2 * Sre[0][0], vim = f2 * Sim[0][0];
-              // identity term of the double layer, added at (t, t)
-              if (cls == 3 && (T.row0 + lr == T.col0 + lc)) {
-                vre += 0.5 * a.are * gt;
-                vim += 0.5 * a.aim * gt;
-              }
-              acc_re[e] = vre;
-              if constexpr (CPLX) acc_im[e] = vim;
-            } else {
-              atomicAdd(&acc_re[e], f2 * Sre[l][k]);
-              if constexpr (CPLX) atomicAdd(&acc_im[e], f2 * Sim[l][k]);
-            }
-          }
-        }
-        // identity term of the double layer on identical triangles:
-        // alpha * g * mass, unpermuted local numbers
-        // (Library/bem3d.c:3205-3242, 1934-1966, 2530-2562)
-        if constexpr (!CC) {
-          const bool allowed = (BR && !BC) ? (T.flags & 1) : true;
-          if (cls == 3 && allowed && (a.are != 0.0 || a.aim != 0.0)) {
-#pragma unroll
-            for (int l = 0; l < NR; l++) {
-              const uint32_t lr = loc_of(sr.loc, l);
-              if (lr == 0xffu) continue;
-#pragma unroll
-              for (int k = 0; k < NC; k++) {
-                const uint32_t lc = loc_of(sc.loc, k);
-                if (lc == 0xffu) continue;
-                const double m = (BR && BC) ? a.mass[l + 3 * k]
-                                            : a.mass[BR ? l : k];
-                const int e = lr + TILE * lc;
-                atomicAdd(&acc_re[e], m * a.are * gt);
-                if constexpr (CPLX) atomicAdd(&acc_im[e], m * a.aim * gt);
-              }
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-
-  // ---- write the tile: column-major, or transposed with ntrans
-  // (conjugated for the cc and ll basis pairs, as in the reference)
-  const int nr = T.nr, nc = T.nc;
-  const bool conj = T.ntrans && (BR == BC);
-  for (int e = tid; e < nr * nc; e += NTHREADS) {
-    const int i = e % nr, j = e / nr;
-    const size_t pos =
-        T.out_off + (T.ntrans ? (size_t)j + (size_t)i * T.ld
-                              : (size_t)i + (size_t)j * T.ld);
-    const int s = i + TILE * j;
-    if constexpr (CPLX) {
-      a.out[2 * pos] = acc_re[s];
-      a.out[2 * pos + 1] = conj ? -acc_im[s] : acc_im[s];
-    } else {
-      a.out[pos] = acc_re[s];
-    }
-  }
-}
-
 // Identity term of the constant-basis double layer for pairs with equal
-// triangles but different block positions: the reference adds it at (t, t)
+// triangles at different block positions: the reference adds it at (t, t)
 // (Library/bem3d.c:1234-1241).
 __global__ void nf_fixup_kernel(const Fixup *fix, uint32_t nfix,
                                 const double *g, double are, double aim,
@@ -474,37 +62,6 @@ __global__ void regpts_kernel(const double *x, const uint32_t *t, uint32_t nt,
     pts[3 * (size_t)i + d] = A[d] * ca + B[d] * cb + C[d] * cc;
 }
 
-typedef void (*tile_kernel_t)(KArgs);
-
-template <int KER, int BR, int BC>
-tile_kernel_t pick_q(uint32_t q) {
-  switch (q) {
-    case 1: return nf_tile_kernel<KER, BR, BC, 1>;
-    case 2: return nf_tile_kernel<KER, BR, BC, 2>;
-    case 3: return nf_tile_kernel<KER, BR, BC, 3>;
-    case 4: return nf_tile_kernel<KER, BR, BC, 4>;
-    default: return nullptr;
-  }
-}
-
-template <int KER>
-tile_kernel_t pick_basis(int rb, int cb, uint32_t q) {
-  if (!rb && !cb) return pick_q<KER, 0, 0>(q);
-  if (rb && cb) return pick_q<KER, 1, 1>(q);
-  if (cb) return pick_q<KER, 0, 1>(q);
-  return pick_q<KER, 1, 0>(q);
-}
-
-tile_kernel_t pick_kernel(const h2hip_op *op) {
-  const int ker = (op->kernel == H2HIP_HELMHOLTZ ? 2 : 0) + op->layer;
-  switch (ker) {
-    case K_LAP_SLP: return pick_basis<K_LAP_SLP>(op->rb, op->cb, op->q);
-    case K_LAP_DLP: return pick_basis<K_LAP_DLP>(op->rb, op->cb, op->q);
-    case K_HELM_SLP: return pick_basis<K_HELM_SLP>(op->rb, op->cb, op->q);
-    default: return pick_basis<K_HELM_DLP>(op->rb, op->cb, op->q);
-  }
-}
-
 // 1/(4 pi) (Library/laplacebem3d.c:33)
 constexpr double KERNEL_CONST = 0.0795774715459476679;
 
@@ -513,13 +70,23 @@ constexpr double KERNEL_CONST = 0.0795774715459476679;
 // Support list of one tile side.  Constant basis: the DOFs are triangles.
 // Linear basis: all triangles touching a DOF vertex, each with the
 // tile-local numbers of its vertices.  Uses and restores pos[] (vertex ->
-// tile-local number, or -1).
+// tile-local number, or -1) and tpos[] (triangle -> support position).
 void tile_side(const h2hip_surface *s, int basis, const uint32_t *idx,
                uint32_t cnt, std::vector<int32_t> &pos,
                std::vector<int32_t> &tpos, std::vector<Supp> &out) {
+  auto make = [&](uint32_t tri, uint32_t loc) {
+    Supp e;
+    e.tri = tri;
+    e.loc = loc;
+    e.v[0] = s->t[3 * tri];
+    e.v[1] = s->t[3 * tri + 1];
+    e.v[2] = s->t[3 * tri + 2];
+    e.pad = 0;
+    return e;
+  };
   if (!basis) {
     for (uint32_t i = 0; i < cnt; i++)
-      out.push_back(Supp{idx[i], i | 0xffff00u});
+      out.push_back(make(idx[i], i | 0xffff00u));
     return;
   }
   for (uint32_t i = 0; i < cnt; i++) pos[idx[i]] = (int32_t)i;
@@ -535,19 +102,47 @@ void tile_side(const h2hip_surface *s, int basis, const uint32_t *idx,
         const int32_t q = pos[s->t[3 * tri + l]];
         packed |= (uint32_t)(q >= 0 ? q : 0xff) << (8 * l);
       }
-      out.push_back(Supp{tri, packed});
+      out.push_back(make(tri, packed));
     }
   }
   for (size_t k = first; k < out.size(); k++) tpos[out[k].tri] = -1;
   for (uint32_t i = 0; i < cnt; i++) pos[idx[i]] = -1;
 }
 
+// Shared vertices of two triangles and the reference's vertex permutation:
+// shared vertices first in the order of the first triangle, then the
+// remaining ones in their original order (Library/singquad2d.c:549-633).
+int classify(const uint32_t *tv, const uint32_t *sv, uint16_t &perm) {
+  int tp[3], sp[3], p = 0;
+  bool tu[3] = {false, false, false}, su[3] = {false, false, false};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (tv[i] == sv[j]) {
+        tp[p] = i;
+        sp[p] = j;
+        tu[i] = su[j] = true;
+        p++;
+        break;
+      }
+  if (p == 3) {  // identical triangles keep their order
+    for (int i = 0; i < 3; i++) tp[i] = sp[i] = i;
+  } else {
+    int ct = p, cs = p;
+    for (int i = 0; i < 3; i++) {
+      if (!tu[i]) tp[ct++] = i;
+      if (!su[i]) sp[cs++] = i;
+    }
+  }
+  perm = 0;
+  for (int i = 0; i < 3; i++)
+    perm |= (uint16_t)((tp[i] << (2 * i)) | (sp[i] << (6 + 2 * i)));
+  return p;
+}
+
 int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
                const uint32_t *ridx_all, uint32_t nridx,
                const uint32_t *cidx_all, uint32_t ncidx, uint32_t nblocks,
-               const uint32_t *blocks, int ntrans,
-               const std::vector<uint64_t> *offsets,
-               const std::vector<uint32_t> *lds) {
+               const uint32_t *blocks, int ntrans) {
   P->s = s;
   P->rb = rb;
   P->cb = cb;
@@ -556,7 +151,7 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
   const uint32_t nrdof = rb ? s->nv : s->nt;
   const uint32_t ncdof = cb ? s->nv : s->nt;
   std::vector<int32_t> pos(s->nv, -1), tpos(s->nt, -1);
-  std::vector<uint32_t> ridx_t, cidx_t;
+  std::vector<Tile> tiles;
   uint64_t total = 0;
   for (uint32_t b = 0; b < nblocks; b++) {
     const uint32_t roff = blocks[4 * b], rows = blocks[4 * b + 1];
@@ -576,8 +171,8 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
         set_error("column index out of range");
         return H2HIP_EINVAL;
       }
-    const uint64_t boff = offsets ? (*offsets)[b] : total;
-    const uint32_t ld = lds ? (*lds)[b] : (ntrans ? cols : rows);
+    const uint64_t boff = total;
+    const uint32_t ld = ntrans ? cols : rows;
     P->off[b] = boff;
     total += (uint64_t)rows * cols;
     for (uint32_t r0 = 0; r0 < rows; r0 += TILE) {
@@ -612,10 +207,10 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
         // most three columns (Library/bem3d.c:2533 tests a loop variable
         // that equals 3 there).
         T.flags = (cols > 3) ? 1 : 0;
-        P->tiles.push_back(T);
+        tiles.push_back(T);
       }
     }
-    // cc identity terms whose position (t, t) differs from (t, s)
+    // cc identity terms the reference leaves at (t, t) with t != s
     if (!rb && !cb) {
       std::unordered_map<uint32_t, std::vector<uint32_t>> where;
       for (uint32_t j = 0; j < cols; j++) where[ci[j]].push_back(j);
@@ -623,11 +218,10 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
         auto it = where.find(ri[i]);
         if (it == where.end()) continue;
         for (uint32_t j : it->second) {
-          // i == j: added by the tile kernel itself.  The reference visits
-          // the columns in order and overwrites (t, t) when it computes
-          // column t, so a term added from column s < t does not survive;
-          // only s > t leaves it in place (which also keeps (t, t) inside
-          // the window).
+          // i == j: added with the entry itself.  The reference visits the
+          // columns in order and overwrites (t, t) when it computes column
+          // t, so a term added from column s < t does not survive; only
+          // s > t leaves it in place (and keeps (t, t) inside the window).
           if (j <= i) continue;
           Fixup f;
           f.pos = boff + (uint64_t)i + (uint64_t)i * ld;
@@ -639,9 +233,59 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
     }
   }
   P->entries = total;
+
+  // largest tiles first: the tail of the launch runs the small ones
+  std::vector<uint32_t> order(tiles.size());
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    return (uint64_t)tiles[x].nrsup * tiles[x].ncsup >
+           (uint64_t)tiles[y].nrsup * tiles[y].ncsup;
+  });
+  P->tiles.resize(tiles.size());
+  for (size_t i = 0; i < order.size(); i++) P->tiles[i] = tiles[order[i]];
+
+  // pairs with common vertices, found through the vertex-to-triangle map
+  std::vector<SPair> by_cls[4];
+  std::vector<int32_t> cpos(s->nt, -1);
+  for (uint32_t ti = 0; ti < (uint32_t)P->tiles.size(); ti++) {
+    const Tile &T = P->tiles[ti];
+    P->pairs[0] += (uint64_t)T.nrsup * T.ncsup;
+    for (uint32_t c = 0; c < T.ncsup; c++)
+      cpos[P->supp[T.csup + c].tri] = (int32_t)c;
+    for (uint32_t r = 0; r < T.nrsup; r++) {
+      const Supp &sr = P->supp[T.rsup + r];
+      uint32_t seen[64];
+      int nseen = 0;
+      for (int l = 0; l < 3; l++) {
+        const uint32_t v = sr.v[l];
+        for (uint32_t p = s->v2t_ptr[v]; p < s->v2t_ptr[v + 1]; p++) {
+          const uint32_t tri = s->v2t[p];
+          const int32_t c = cpos[tri];
+          if (c < 0) continue;
+          bool dup = false;
+          for (int k = 0; k < nseen; k++) dup |= (seen[k] == tri);
+          if (dup) continue;
+          if (nseen < 64) seen[nseen++] = tri;
+          SPair q;
+          q.tile = ti;
+          q.r = (uint16_t)r;
+          q.c = (uint16_t)c;
+          q.cls = (uint16_t)classify(sr.v, P->supp[T.csup + c].v, q.perm);
+          by_cls[q.cls].push_back(q);
+        }
+      }
+    }
+    for (uint32_t c = 0; c < T.ncsup; c++) cpos[P->supp[T.csup + c].tri] = -1;
+  }
+  for (int c = 3; c >= 1; c--) {
+    P->pairs[c] = by_cls[c].size();
+    P->pairs[0] -= by_cls[c].size();
+    P->spairs.insert(P->spairs.end(), by_cls[c].begin(), by_cls[c].end());
+  }
   H2HIP_CHECK(P->d_tiles.upload(P->tiles.data(), P->tiles.size()));
   H2HIP_CHECK(P->d_supp.upload(P->supp.data(), P->supp.size()));
   H2HIP_CHECK(P->d_fix.upload(P->fix.data(), P->fix.size()));
+  H2HIP_CHECK(P->d_spairs.upload(P->spairs.data(), P->spairs.size()));
   return H2HIP_OK;
 }
 
@@ -652,16 +296,10 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
     return H2HIP_EINVAL;
   }
   if (P->tiles.empty()) return H2HIP_OK;
-  tile_kernel_t kern = pick_kernel(op);
-  if (!kern) {
-    set_error("unsupported regular quadrature order (1..4 supported)");
-    return H2HIP_EINVAL;
-  }
   const h2hip_surface *s = op->s;
-  KArgs a;
+  nf::KArgs a;
   a.tiles = P->d_tiles.p;
   a.supp = P->d_supp.p;
-  a.tri = s->d_t.p;
   a.x = s->d_x.p;
   a.n = s->d_n.p;
   a.g = s->d_g.p;
@@ -681,11 +319,21 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.far = far;
   a.out = (double *)d_out;
   const uint32_t ntiles = (uint32_t)P->tiles.size();
-  hipLaunchKernelGGL(kern, dim3(ntiles), dim3(NTHREADS), 0, stream, a);
+  const uint32_t nsp = (uint32_t)P->spairs.size();
+  const int rc =
+      op->kernel == H2HIP_HELMHOLTZ
+          ? nf::launch_helmholtz(op->layer, op->rb, op->cb, op->q, a, ntiles,
+                                 P->d_spairs.p, nsp, stream)
+          : nf::launch_laplace(op->layer, op->rb, op->cb, op->q, a, ntiles,
+                               P->d_spairs.p, nsp, stream);
+  if (rc != H2HIP_OK) {
+    set_error("unsupported regular quadrature order (1..4 supported)");
+    return rc;
+  }
   H2HIP_CHECK(hipGetLastError());
   if (!far && !P->fix.empty() && (a.are != 0.0 || a.aim != 0.0)) {
     const uint32_t nfix = (uint32_t)P->fix.size();
-    const int ntr = P->tiles.empty() ? 0 : P->tiles[0].ntrans;
+    const int ntr = P->tiles[0].ntrans;
     hipLaunchKernelGGL(nf_fixup_kernel, dim3((nfix + 255) / 256), dim3(256),
                        0, stream, P->d_fix.p, nfix, s->d_g.p, a.are, a.aim,
                        op->kernel == H2HIP_HELMHOLTZ ? 1 : 0, ntr,
@@ -921,7 +569,7 @@ int h2hip_nearfield(const h2hip_op *op, int far, const uint32_t *ridx,
   h2hip_nfplan P;
   const uint32_t blk[4] = {0, rows, 0, cols};
   int rc = build_plan(&P, op->s, op->rb, op->cb, ridx, rows, cidx, cols, 1,
-                      blk, ntrans, nullptr, nullptr);
+                      blk, ntrans);
   if (rc != H2HIP_OK) return rc;
   const size_t es = op->kernel == H2HIP_HELMHOLTZ ? 16 : 8;
   const size_t bytes = es * wr * wc;
@@ -950,8 +598,7 @@ int h2hip_nfplan_create(h2hip_nfplan **out, const h2hip_surface *s,
   }
   h2hip_nfplan *P = new h2hip_nfplan;
   int rc = build_plan(P, s, row_basis ? 1 : 0, col_basis ? 1 : 0, ridx_all,
-                      nridx, cidx_all, ncidx, nblocks, blocks, 0, nullptr,
-                      nullptr);
+                      nridx, cidx_all, ncidx, nblocks, blocks, 0);
   if (rc != H2HIP_OK) {
     delete P;
     return rc;
@@ -975,20 +622,11 @@ int h2hip_nfplan_offsets(const h2hip_nfplan *p, uint64_t *off) {
 int h2hip_nfplan_work(const h2hip_nfplan *p, const h2hip_op *op, int far,
                       double *out) {
   if (!p || !op || !out) return H2HIP_EINVAL;
-  const h2hip_surface *s = p->s;
-  double pairs[4] = {0, 0, 0, 0};
-  for (const Tile &T : p->tiles) {
-    for (uint32_t c = 0; c < T.ncsup; c++) {
-      const uint32_t *sv = &s->t[3 * p->supp[T.csup + c].tri];
-      for (uint32_t r = 0; r < T.nrsup; r++) {
-        const uint32_t *tv = &s->t[3 * p->supp[T.rsup + r].tri];
-        int sh = 0;
-        if (!far)
-          for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) sh += (tv[i] == sv[j]);
-        pairs[sh] += 1.0;
-      }
-    }
+  double pairs[4];
+  for (int c = 0; c < 4; c++) pairs[c] = (double)p->pairs[c];
+  if (far) {
+    pairs[0] += pairs[1] + pairs[2] + pairs[3];
+    pairs[1] = pairs[2] = pairs[3] = 0.0;
   }
   const double nq[4] = {(double)op->q * op->q * op->q * op->q,
                         (double)op->sing_n[1], (double)op->sing_n[2],

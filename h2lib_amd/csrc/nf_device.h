@@ -1,0 +1,409 @@
+// Device code of the near-field quadrature (included by the per-family
+// translation units nf_laplace.hip and nf_helmholtz.hip).
+//
+// Two kernels per operator, run in this order on one stream:
+//
+//  nf_regular_kernel  one workgroup (four wave64s) per tile of at most
+//      64 x 64 DOFs.  Every pair (row support triangle, column support
+//      triangle) without common vertices is evaluated by one lane with the
+//      tensor-product Gauss rule from quadrature points precomputed per
+//      triangle (16 kernel evaluations at q = 2, all from registers).  The
+//      element matrix (1x1, 1x3, 3x1 or 3x3) is added into an LDS tile that
+//      is stored once, coalesced; the constant-basis case stores entries
+//      directly.  Pairs with common vertices are skipped (cc: stored as 0).
+//
+//  nf_singular_kernel one wave64 per pair with common vertices, taken from
+//      a list built at planning time (identical and edge pairs first, then
+//      vertex pairs).  The lanes stride over the 2 q2^4 or 6 q2^4
+//      Sauter-Schwab points, the sums are reduced across the wavefront and
+//      up to nine lanes add the element matrix into the output with FP64
+//      atomics.  x - y is formed from differences of the permuted vertices,
+//      so the shared vertices cancel exactly.
+#pragma once
+
+#include "h2hip_internal.h"
+
+namespace h2hip {
+namespace nf {
+
+constexpr int NTHREADS = 256;
+constexpr int NWAVES = NTHREADS / 64;
+
+enum { K_LAP_SLP = 0, K_LAP_DLP = 1, K_HELM_SLP = 2, K_HELM_DLP = 3 };
+
+// 1/sqrt(x): hardware estimate and one third-order correction
+// y (1 + e/2 + 3 e^2/8), e = 1 - x y^2.  Positive finite x only (x = 0
+// gives NaN where 1/sqrt gives inf; both are non-finite).
+__device__ __forceinline__ double rsqrt_fast(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// Kernel function values without the 1/(4 pi) factor.
+// Laplace SLP 1/r, DLP <d, n_y>/r^3 (Library/laplacebem3d.c:35-108);
+// Helmholtz SLP e^{ikr}/r, DLP e^{ikr} (1 - ikr) <d, n_y>/r^3
+// (Library/helmholtzbem3d.c:26-170), d = x - y, r = |d|,
+// k = kr + i ki' with ki = -ki' (damping e^{ki r}).
+template <int KER>
+__device__ __forceinline__ void kval(double dx, double dy, double dz,
+                                     double nx, double ny, double nz,
+                                     double kr, double ki, double &re,
+                                     double &im) {
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rs = rsqrt_fast(r2);
+  if constexpr (KER == K_LAP_SLP) {
+    re = rs;
+    im = 0.0;
+  } else if constexpr (KER == K_LAP_DLP) {
+    re = fma(dx, nx, fma(dy, ny, dz * nz)) * (rs * rs * rs);
+    im = 0.0;
+  } else if constexpr (KER == K_HELM_SLP) {
+    const double r = r2 * rs;
+    double amp = rs;
+    if (ki != 0.0) amp *= exp(ki * r);
+    double s, c;
+    sincos(kr * r, &s, &c);
+    re = amp * c;
+    im = amp * s;
+  } else {
+    const double r = r2 * rs;
+    double amp = fma(dx, nx, fma(dy, ny, dz * nz)) * (rs * rs * rs);
+    if (ki != 0.0) amp *= exp(ki * r);
+    const double kr_r = kr * r;
+    double s, c;
+    sincos(kr_r, &s, &c);
+    re = fma(kr_r, s, c) * amp;
+    im = fma(-c, kr_r, s) * amp;
+  }
+}
+
+__device__ __forceinline__ uint32_t loc_of(uint32_t packed, int l) {
+  return (packed >> (8 * l)) & 0xffu;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int KER, int BR, int BC, int Q>
+__global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
+  constexpr bool CPLX = KER >= K_HELM_SLP;
+  constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
+  constexpr bool CC = (BR == 0) && (BC == 0);
+  constexpr int NR = BR ? 3 : 1;
+  constexpr int NC = BC ? 3 : 1;
+  constexpr int QQ = Q * Q;
+
+  __shared__ double acc_re[CC ? 1 : TILE * TILE];
+  __shared__ double acc_im[(CC || !CPLX) ? 1 : TILE * TILE];
+
+  const int tid = threadIdx.x;
+  const Tile T = a.tiles[blockIdx.x];
+  const uint32_t nR = T.nrsup, nC = T.ncsup;
+  const Supp *RS = a.supp + T.rsup;
+  const Supp *CS = a.supp + T.csup;
+  const double *__restrict__ W = a.regw;  // w | phi0 w | phi1 w | phi2 w
+  const bool conj = T.ntrans && (BR == BC);
+
+  if constexpr (!CC) {
+    for (int i = tid; i < TILE * TILE; i += NTHREADS) {
+      acc_re[i] = 0.0;
+      if constexpr (CPLX) acc_im[i] = 0.0;
+    }
+    __syncthreads();
+  }
+
+  const uint32_t npairs = nR * nC;
+  for (uint32_t p = tid; p < npairs; p += NTHREADS) {
+    const uint32_t r = p % nR, c = p / nR;
+    const Supp sr = RS[r], sc = CS[c];
+    int shared = 0;
+    if (!a.far) {
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) shared += (sr.v[i] == sc.v[j]);
+    }
+    double Sre[NR][NC], Sim[NR][NC];
+#pragma unroll
+    for (int l = 0; l < NR; l++)
+#pragma unroll
+      for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
+    double f2 = 0.0;
+    if (!shared) {
+      // The reference's lc far integrator maps the first rule coordinates
+      // onto the column triangle (Library/bem3d.c:2637-2649); kept as is.
+      const bool swapgeo = (BR && !BC) && a.far;
+      const double *__restrict__ Pt =
+          a.regpts + (size_t)(swapgeo ? sc.tri : sr.tri) * (3 * QQ);
+      const double *__restrict__ Ps =
+          a.regpts + (size_t)(swapgeo ? sr.tri : sc.tri) * (3 * QQ);
+      double ptx[QQ], pty[QQ], ptz[QQ];
+#pragma unroll
+      for (int i = 0; i < QQ; i++) {
+        ptx[i] = Pt[3 * i];
+        pty[i] = Pt[3 * i + 1];
+        ptz[i] = Pt[3 * i + 2];
+      }
+      double nx = 0.0, ny = 0.0, nz = 0.0;
+      if constexpr (DLP) {
+        nx = a.n[3 * sc.tri];
+        ny = a.n[3 * sc.tri + 1];
+        nz = a.n[3 * sc.tri + 2];
+      }
+      f2 = a.kc * a.g[sr.tri] * a.g[sc.tri];
+#pragma unroll
+      for (int b = 0; b < QQ; b++) {
+        const double qx = Ps[3 * b], qy = Ps[3 * b + 1], qz = Ps[3 * b + 2];
+        double Ure[NR], Uim[NR];
+#pragma unroll
+        for (int l = 0; l < NR; l++) Ure[l] = Uim[l] = 0.0;
+#pragma unroll
+        for (int i = 0; i < QQ; i++) {
+          double vre, vim;
+          kval<KER>(ptx[i] - qx, pty[i] - qy, ptz[i] - qz, nx, ny, nz, a.kr,
+                    a.ki, vre, vim);
+#pragma unroll
+          for (int l = 0; l < NR; l++) {
+            const double wr = BR ? W[QQ * (1 + l) + i] : W[i];
+            Ure[l] = fma(wr, vre, Ure[l]);
+            if constexpr (CPLX) Uim[l] = fma(wr, vim, Uim[l]);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          const double wc = BC ? W[QQ * (1 + k) + b] : W[b];
+#pragma unroll
+          for (int l = 0; l < NR; l++) {
+            Sre[l][k] = fma(wc, Ure[l], Sre[l][k]);
+            if constexpr (CPLX) Sim[l][k] = fma(wc, Uim[l], Sim[l][k]);
+          }
+        }
+      }
+    }
+    if constexpr (CC) {
+      // entry (r, c) of the tile; pairs with common vertices are stored as
+      // zero and completed by the singular kernel
+      const size_t pos = T.out_off + (T.ntrans ? (size_t)c + (size_t)r * T.ld
+                                               : (size_t)r + (size_t)c * T.ld);
+      if constexpr (CPLX) {
+        a.out[2 * pos] = f2 * Sre[0][0];
+        a.out[2 * pos + 1] = conj ? -(f2 * Sim[0][0]) : f2 * Sim[0][0];
+      } else {
+        a.out[pos] = f2 * Sre[0][0];
+      }
+    } else if (!shared) {
+#pragma unroll
+      for (int l = 0; l < NR; l++) {
+        const uint32_t lr = loc_of(sr.loc, l);
+        if (lr == 0xffu) continue;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          const uint32_t lc = loc_of(sc.loc, k);
+          if (lc == 0xffu) continue;
+          const int e = lr + TILE * lc;
+          atomicAdd(&acc_re[e], f2 * Sre[l][k]);
+          if constexpr (CPLX) atomicAdd(&acc_im[e], f2 * Sim[l][k]);
+        }
+      }
+    }
+  }
+
+  if constexpr (!CC) {
+    __syncthreads();
+    const int nr = T.nr, nc = T.nc;
+    for (int e = tid; e < nr * nc; e += NTHREADS) {
+      const int i = e % nr, j = e / nr;
+      const size_t pos =
+          T.out_off + (T.ntrans ? (size_t)j + (size_t)i * T.ld
+                                : (size_t)i + (size_t)j * T.ld);
+      const int s = i + TILE * j;
+      if constexpr (CPLX) {
+        a.out[2 * pos] = acc_re[s];
+        a.out[2 * pos + 1] = conj ? -acc_im[s] : acc_im[s];
+      } else {
+        a.out[pos] = acc_re[s];
+      }
+    }
+  }
+}
+
+template <int KER, int BR, int BC>
+__global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
+    KArgs a, const SPair *__restrict__ list, uint32_t npairs) {
+  constexpr bool CPLX = KER >= K_HELM_SLP;
+  constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
+  constexpr bool CC = (BR == 0) && (BC == 0);
+  constexpr int NR = BR ? 3 : 1;
+  constexpr int NC = BC ? 3 : 1;
+
+  const uint32_t w = blockIdx.x * NWAVES + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= npairs) return;  // uniform per wavefront
+
+  const SPair P = list[w];
+  const Tile T = a.tiles[P.tile];
+  const Supp sr = a.supp[T.rsup + P.r], sc = a.supp[T.csup + P.c];
+  const int cls = P.cls;
+  int tp[3], sp[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    tp[i] = (P.perm >> (2 * i)) & 3;
+    sp[i] = (P.perm >> (6 + 2 * i)) & 3;
+  }
+  const uint32_t nq = a.sing_n[cls];
+  const double *__restrict__ tab = a.sing + a.sing_off[cls];
+  double D0[3], E1[3], E2[3], F1[3], F2[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double At = a.x[3 * sr.v[tp[0]] + d], Bt = a.x[3 * sr.v[tp[1]] + d],
+                 Ct = a.x[3 * sr.v[tp[2]] + d];
+    const double As = a.x[3 * sc.v[sp[0]] + d], Bs = a.x[3 * sc.v[sp[1]] + d],
+                 Cs = a.x[3 * sc.v[sp[2]] + d];
+    D0[d] = At - As;
+    E1[d] = Bt - At;
+    E2[d] = Ct - Bt;
+    F1[d] = Bs - As;
+    F2[d] = Cs - Bs;
+  }
+  double nx = 0.0, ny = 0.0, nz = 0.0;
+  if constexpr (DLP) {
+    nx = a.n[3 * sc.tri];
+    ny = a.n[3 * sc.tri + 1];
+    nz = a.n[3 * sc.tri + 2];
+  }
+  double Sre[NR][NC], Sim[NR][NC];
+#pragma unroll
+  for (int l = 0; l < NR; l++)
+#pragma unroll
+    for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
+  for (uint32_t q = lane; q < nq; q += 64) {
+    const double tx = tab[q], sx = tab[nq + q];
+    const double ty = tab[2 * nq + q], sy = tab[3 * nq + q];
+    const double wq = tab[4 * nq + q];
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      d[k] = fma(-F2[k], sy,
+                 fma(-F1[k], ty, fma(E2[k], sx, fma(E1[k], tx, D0[k]))));
+    double vre, vim;
+    kval<KER>(d[0], d[1], d[2], nx, ny, nz, a.kr, a.ki, vre, vim);
+    const double wre = wq * vre, wim = wq * vim;
+    double phi[NR], psi[NC];
+    if constexpr (BR) {
+      phi[0] = 1.0 - tx;
+      phi[1] = tx - sx;
+      phi[2] = sx;
+    } else {
+      phi[0] = 1.0;
+    }
+    if constexpr (BC) {
+      psi[0] = 1.0 - ty;
+      psi[1] = ty - sy;
+      psi[2] = sy;
+    } else {
+      psi[0] = 1.0;
+    }
+#pragma unroll
+    for (int l = 0; l < NR; l++) {
+      const double ar = wre * phi[l], ai = wim * phi[l];
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        Sre[l][k] = fma(ar, psi[k], Sre[l][k]);
+        if constexpr (CPLX) Sim[l][k] = fma(ai, psi[k], Sim[l][k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NR; l++)
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      Sre[l][k] = wave_sum(Sre[l][k]);
+      if constexpr (CPLX) Sim[l][k] = wave_sum(Sim[l][k]);
+    }
+  if (lane >= NR * NC) return;
+  // lane e adds the entry of permuted local functions (l, k) = (e % NR,
+  // e / NR)
+  const int l = lane % NR, k = lane / NR;
+  double vre = 0.0, vim = 0.0;
+#pragma unroll
+  for (int ll = 0; ll < NR; ll++)
+#pragma unroll
+    for (int kk = 0; kk < NC; kk++)
+      if (ll == l && kk == k) {
+        vre = Sre[ll][kk];
+        vim = Sim[ll][kk];
+      }
+  const uint32_t lr = loc_of(sr.loc, BR ? tp[l] : 0);
+  const uint32_t lc = loc_of(sc.loc, BC ? sp[k] : 0);
+  if (lr == 0xffu || lc == 0xffu) return;
+  const double gt = a.g[sr.tri];
+  const double f2 = a.kc * gt * a.g[sc.tri];
+  vre *= f2;
+  vim *= f2;
+  if (cls == 3) {
+    // identity term of the double layer on identical triangles (tp, sp are
+    // the identity here): cc at (t, t) of the block
+    // (Library/bem3d.c:1234-1241), otherwise alpha g mass[l, k]
+    // (Library/bem3d.c:3205-3242, 1934-1966, 2530-2562)
+    if constexpr (CC) {
+      if (T.row0 + lr == T.col0 + lc) {
+        vre += 0.5 * a.are * gt;
+        vim += 0.5 * a.aim * gt;
+      }
+    } else {
+      const bool allowed = (BR && !BC) ? (T.flags & 1) : true;
+      if (allowed) {
+        const double m = (BR && BC) ? a.mass[l + 3 * k] : a.mass[BR ? l : k];
+        vre += m * a.are * gt;
+        vim += m * a.aim * gt;
+      }
+    }
+  }
+  const bool conj = T.ntrans && (BR == BC);
+  const size_t pos =
+      T.out_off + (T.ntrans ? (size_t)lc + (size_t)lr * T.ld
+                            : (size_t)lr + (size_t)lc * T.ld);
+  if constexpr (CPLX) {
+    atomicAdd(&a.out[2 * pos], vre);
+    atomicAdd(&a.out[2 * pos + 1], conj ? -vim : vim);
+  } else {
+    atomicAdd(&a.out[pos], vre);
+  }
+}
+
+template <int KER, int BR, int BC>
+int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
+                 const SPair *sp, uint32_t nsp, hipStream_t stream) {
+  void (*reg)(KArgs) = nullptr;
+  switch (q) {
+    case 1: reg = nf_regular_kernel<KER, BR, BC, 1>; break;
+    case 2: reg = nf_regular_kernel<KER, BR, BC, 2>; break;
+    case 3: reg = nf_regular_kernel<KER, BR, BC, 3>; break;
+    case 4: reg = nf_regular_kernel<KER, BR, BC, 4>; break;
+    default: return H2HIP_EINVAL;
+  }
+  if (ntiles)
+    hipLaunchKernelGGL(reg, dim3(ntiles), dim3(NTHREADS), 0, stream, a);
+  if (!a.far && nsp)
+    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC>),
+                       dim3((nsp + NWAVES - 1) / NWAVES), dim3(NTHREADS), 0,
+                       stream, a, sp, nsp);
+  return H2HIP_OK;
+}
+
+template <int KER>
+int launch_family(int rb, int cb, uint32_t q, const KArgs &a,
+                  uint32_t ntiles, const SPair *sp, uint32_t nsp,
+                  hipStream_t stream) {
+  if (!rb && !cb) return launch_bases<KER, 0, 0>(q, a, ntiles, sp, nsp, stream);
+  if (rb && cb) return launch_bases<KER, 1, 1>(q, a, ntiles, sp, nsp, stream);
+  if (cb) return launch_bases<KER, 0, 1>(q, a, ntiles, sp, nsp, stream);
+  return launch_bases<KER, 1, 0>(q, a, ntiles, sp, nsp, stream);
+}
+
+}  // namespace nf
+}  // namespace h2hip
