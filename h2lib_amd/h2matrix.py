@@ -1,0 +1,160 @@
+"""H2-matrices on the device: construction by interpolation (C host code in
+libh2bem.so, restating setup_h2matrix_aprx_inter_bem3d and
+assemble_bem3d_h2matrix, Library/bem3d.c:9658-9686, 10313-10446) with the
+near field from the HIP quadrature, and the products addeval / addevaltrans
+(Library/h2matrix.c:1268-1286, 1345-1364) in libh2hip.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, hip
+from .bem3d import DLP, LAPLACE, NearfieldPlan
+from .geometry import bem
+
+
+class _H2Interp(C.Structure):
+    _fields_ = [("m", C.c_uint32), ("k", C.c_uint32),
+                ("rpts", C.POINTER(C.c_double)),
+                ("cpts", C.POINTER(C.c_double)),
+                ("rvoff", C.POINTER(C.c_uint64)),
+                ("cvoff", C.POINTER(C.c_uint64)),
+                ("rV", C.POINTER(C.c_double)), ("cV", C.POINTER(C.c_double)),
+                ("rvlen", C.c_uint64), ("cvlen", C.c_uint64),
+                ("rE", C.POINTER(C.c_double)), ("cE", C.POINTER(C.c_double)),
+                ("nfar", C.c_uint32), ("S", C.POINTER(C.c_double))]
+
+
+_sigs_done = False
+
+
+def _sigs():
+    global _sigs_done
+    if _sigs_done:
+        return
+    b = bem()
+    ip = C.POINTER(_H2Interp)
+    b.h2b_build_h2interp.restype = ip
+    b.h2b_build_h2interp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                     C.c_uint, C.c_uint]
+    b.h2b_h2interp_coupling_laplace.argtypes = [ip, C.c_void_p]
+    b.h2b_h2interp_free.argtypes = [ip]
+    h = hip()
+    vp, u32, dp = C.c_void_p, C.c_uint32, C.c_void_p
+    h.h2hip_h2matrix_create.restype = C.c_int
+    h.h2hip_h2matrix_create.argtypes = [C.POINTER(vp), C.c_int, u32, dp, dp,
+                                        u32, dp, u32, dp, dp, u32, dp, u32,
+                                        dp, u32, dp]
+    h.h2hip_h2matrix_destroy.argtypes = [vp]
+    h.h2hip_h2matrix_set_basis.argtypes = [vp, C.c_int, dp, dp, dp,
+                                           C.c_uint64]
+    h.h2hip_h2matrix_set_couplings.argtypes = [vp, dp, C.c_uint64]
+    h.h2hip_h2matrix_set_nearfield.argtypes = [vp, dp]
+    h.h2hip_h2matrix_nearfield_device.restype = C.c_void_p
+    h.h2hip_h2matrix_nearfield_device.argtypes = [vp]
+    h.h2hip_h2matrix_bytes.argtypes = [vp, dp]
+    h.h2hip_addeval_h2matrix.argtypes = [dp, vp, vp, vp, vp]
+    h.h2hip_addevaltrans_h2matrix.argtypes = [dp, vp, vp, vp, vp]
+    _sigs_done = True
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def tree_arrays(t):
+    """(off, size, father) per node and the ranks of a ClusterTree."""
+    father = np.full(t.nnodes, 0xffffffff, dtype=np.uint32)
+    for i in range(t.nnodes):
+        for s in range(int(t.sons[i])):
+            father[t.son[i][s]] = i
+    tree = np.stack([t.off, t.size, father], axis=1).astype(np.uint32)
+    return np.ascontiguousarray(tree)
+
+
+class H2Matrix:
+    """Device H2-matrix of one operator (mirror of h2matrix,
+    Library/h2matrix.h:48-71)."""
+
+    def __init__(self, op, mesh, rt, ct, btree, m, build_nearfield=True,
+                 stream=None):
+        _sigs()
+        if op.kernel != LAPLACE:
+            raise NotImplementedError("interpolation couplings: Laplace only")
+        self.op, self.rt, self.ct, self.btree = op, rt, ct, btree
+        self.dtype = op.dtype
+        self.m = m
+        ip = bem().h2b_build_h2interp(mesh._sp, rt._tp, ct._tp, btree._bp,
+                                      op.row_basis, op.col_basis,
+                                      int(op.layer == DLP), m, op.q_regular)
+        if not ip:
+            raise RuntimeError("interpolation build failed")
+        bem().h2b_h2interp_coupling_laplace(ip, btree._bp)
+        self._ip = ip
+        I = ip.contents
+        self.k = I.k
+        self._rtree = tree_arrays(rt)
+        self._ctree = tree_arrays(ct)
+        rrank = np.full(rt.nnodes, I.k, dtype=np.uint32)
+        crank = np.full(ct.nnodes, I.k, dtype=np.uint32)
+        self._keep = (rrank, crank)
+        near = np.ascontiguousarray(btree.near, dtype=np.uint32)
+        far = np.ascontiguousarray(btree.far, dtype=np.uint32)
+        self._blocks = (near, far)
+        h = C.c_void_p()
+        check(hip().h2hip_h2matrix_create(
+            C.byref(h), int(op.is_complex), rt.nnodes, _ptr(self._rtree),
+            _ptr(rrank), rt.n, _ptr(rt.idx), ct.nnodes, _ptr(self._ctree),
+            _ptr(crank), ct.n, _ptr(ct.idx), len(near), _ptr(near), len(far),
+            _ptr(far)), "h2hip_h2matrix_create")
+        self._h = h
+        kk = I.k * I.k
+        check(hip().h2hip_h2matrix_set_basis(
+            h, 0, C.cast(I.rV, C.c_void_p), C.cast(I.rvoff, C.c_void_p),
+            C.cast(I.rE, C.c_void_p), kk), "h2hip_h2matrix_set_basis")
+        check(hip().h2hip_h2matrix_set_basis(
+            h, 1, C.cast(I.cV, C.c_void_p), C.cast(I.cvoff, C.c_void_p),
+            C.cast(I.cE, C.c_void_p), kk), "h2hip_h2matrix_set_basis")
+        check(hip().h2hip_h2matrix_set_couplings(
+            h, C.cast(I.S, C.c_void_p), kk), "h2hip_h2matrix_set_couplings")
+        self.nearplan = None
+        if build_nearfield:
+            self.nearplan = NearfieldPlan(op.gr, op.row_basis, op.col_basis,
+                                          rt.idx, ct.idx,
+                                          btree.near_block_list())
+            self.assemble_nearfield(stream)
+
+    def assemble_nearfield(self, stream=None):
+        """Near-field blocks by the HIP quadrature, straight into the
+        matrix's device storage."""
+        d = hip().h2hip_h2matrix_nearfield_device(self._h)
+        self.nearplan.execute(self.op, d, stream.handle if stream else None)
+        if stream is None:
+            check(hip().h2hip_stream_sync(None), "h2hip_stream_sync")
+
+    def bytes(self):
+        out = np.zeros(3, dtype=np.uint64)
+        check(hip().h2hip_h2matrix_bytes(self._h, _ptr(out)),
+              "h2hip_h2matrix_bytes")
+        return {"near": int(out[0]), "coupling": int(out[1]),
+                "bases": int(out[2])}
+
+    def addeval(self, alpha, d_x, d_y, stream=None, trans=False):
+        """y += alpha H x (trans: y += alpha H^* x) on device arrays."""
+        a = np.array([np.real(alpha), np.imag(alpha)], dtype=np.float64)
+        f = (hip().h2hip_addevaltrans_h2matrix if trans
+             else hip().h2hip_addeval_h2matrix)
+        check(f(_ptr(a), self._h, C.c_void_p(d_x.ptr), C.c_void_p(d_y.ptr),
+                C.c_void_p(stream.handle if stream else 0)),
+              "h2hip_addeval_h2matrix")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            hip().h2hip_h2matrix_destroy(h)
+            self._h = None
+        ip = getattr(self, "_ip", None)
+        if ip:
+            bem().h2b_h2interp_free(ip)
+            self._ip = None

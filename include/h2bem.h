@@ -93,6 +93,34 @@ void h2b_blocks_free(h2b_blocks *b);
 void h2b_near_block_list(const h2b_blocks *b, const h2b_ctree *rt,
                          const h2b_ctree *ct, uint32_t *out);
 
+/* ---- H2-matrix approximation by interpolation -----------------------------
+ * setup_h2matrix_aprx_inter_bem3d (Library/bem3d.c:9658-9686) with
+ * assemble_bem3d_h2matrix_row/col_clusterbasis (Library/bem3d.c:10377-
+ * 10446) and the couplings of assemble_bem3d_h2matrix
+ * (Library/bem3d.c:10313-10346).  Rank k = m^3 on every cluster. */
+typedef struct {
+  uint32_t m, k;
+  double *rpts, *cpts;     /* 3 m per node: x, y, z interpolation points */
+  uint64_t *rvoff, *cvoff; /* per node: leaf matrix offset, UINT64_MAX inner */
+  double *rV, *cV;         /* leaf matrices, |t| x k column-major */
+  uint64_t rvlen, cvlen;
+  double *rE, *cE;         /* per node: k x k transfer to the father */
+  uint32_t nfar;
+  double *S;               /* per far block: k x k coupling, column-major */
+} h2b_h2interp;
+
+/* Cluster bases of both trees; col_dn selects the normal derivative of the
+ * Lagrange polynomials for the column basis (double layer potential,
+ * assemble_bem3d_dn_lagrange_{c,l}_amatrix, Library/bem3d.c:8024-8336).
+ * q: order of the single-triangle Gauss rule. */
+h2b_h2interp *h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
+                                 const h2b_ctree *ct, const h2b_blocks *bl,
+                                 int row_basis, int col_basis, int col_dn,
+                                 unsigned m, unsigned q);
+/* Laplace couplings S = g(xi_r, xi_c) / (4 pi) for all far blocks. */
+void h2b_h2interp_coupling_laplace(h2b_h2interp *H, const h2b_blocks *bl);
+void h2b_h2interp_free(h2b_h2interp *H);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,7 @@ extern "C" {
 typedef struct h2hip_surface h2hip_surface;
 typedef struct h2hip_op h2hip_op;
 typedef struct h2hip_nfplan h2hip_nfplan;
+typedef struct h2hip_h2matrix h2hip_h2matrix;
 
 /* Message describing the last failure on the calling thread. */
 const char *h2hip_last_error(void);
@@ -132,6 +133,56 @@ int h2hip_nfplan_work(const h2hip_nfplan *p, const h2hip_op *op, int far,
  * without synchronising. */
 int h2hip_nearfield_execute(const h2hip_op *op, const h2hip_nfplan *p,
                             int far, void *d_out, void *stream);
+
+/* ---- H2-matrices -----------------------------------------------------------
+ * Device H2-matrix: a row and a column cluster tree, nested cluster bases
+ * (leaf matrices V and transfer matrices E, real), couplings S of the
+ * admissible leaves and dense near-field blocks N of the inadmissible
+ * leaves (real for Laplace, complex for Helmholtz).  Replaces h2matrix,
+ * clusterbasis and uniform (Library/h2matrix.h:48-71,
+ * Library/clusterbasis.h:40-71, Library/uniform.h:69-84) and
+ * build_from_block_h2matrix (Library/h2matrix.c:438-485).
+ *
+ * Trees: node i has tree[3i .. 3i+2] = (offset into idx, size, father),
+ * father = 0xffffffff for the root; rank[i] is its rank k_i; idx is the
+ * DOF permutation.  Leaves are the nodes without sons.
+ * near / far: 2 values per block, (row node, column node). */
+int h2hip_h2matrix_create(h2hip_h2matrix **out, int is_complex,
+                          uint32_t rnodes, const uint32_t *rtree,
+                          const uint32_t *rrank, uint32_t nrdof,
+                          const uint32_t *ridx, uint32_t cnodes,
+                          const uint32_t *ctree, const uint32_t *crank,
+                          uint32_t ncdof, const uint32_t *cidx,
+                          uint32_t nnear, const uint32_t *near, uint32_t nfar,
+                          const uint32_t *far);
+void h2hip_h2matrix_destroy(h2hip_h2matrix *H);
+/* Cluster basis of one side (0 row, 1 column): leaf matrices V_t
+ * (size x rank, column-major) packed in node order at voff[t] (entries for
+ * inner nodes ignored), transfer matrices E_t (rank_t x rank_father,
+ * column-major) packed at node * eld where eld >= max rank^2. */
+int h2hip_h2matrix_set_basis(h2hip_h2matrix *H, int side, const double *V,
+                             const uint64_t *voff, const double *E,
+                             uint64_t eld);
+/* Couplings, block b (rank_r x rank_c, column-major) at S + b * sld
+ * (doubles or complex pairs). */
+int h2hip_h2matrix_set_couplings(h2hip_h2matrix *H, const void *S,
+                                 uint64_t sld);
+/* Near-field blocks packed in block order, rows x cols column-major
+ * (host array), or the device storage for h2hip_nearfield_execute with a
+ * plan over the same blocks. */
+int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N);
+void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H);
+/* Storage of the matrix in bytes: near field, couplings, bases. */
+int h2hip_h2matrix_bytes(const h2hip_h2matrix *H, uint64_t *out3);
+/* y += alpha H x (addeval_h2matrix_avector, Library/h2matrix.c:1268-1286)
+ * and y += alpha H^* x (addevaltrans_h2matrix_avector, Library/h2matrix.c:
+ * 1345-1364) on device vectors in the original DOF order; alpha = (re, im).
+ * Calls on the same matrix must not overlap (shared coefficient
+ * workspace). */
+int h2hip_addeval_h2matrix(const double *alpha, const h2hip_h2matrix *H,
+                           const void *d_x, void *d_y, void *stream);
+int h2hip_addevaltrans_h2matrix(const double *alpha, const h2hip_h2matrix *H,
+                                const void *d_x, void *d_y, void *stream);
 
 /* Waits for all work queued on the stream. */
 int h2hip_stream_sync(void *stream);

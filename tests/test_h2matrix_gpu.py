@@ -1,0 +1,103 @@
+"""H2-matrix construction and products on the GPU against the reference
+library (needs a GPU).
+
+The reference builds its H2-matrix from the same mesh with the same cluster
+tree, block tree and interpolation order; the products y += alpha H x and
+y += alpha H^* x must agree to 1e-12 relative (max-norm).  Independent of
+the reference: <H x, y> = <x, H^* y>, linearity, and the product of a
+matrix without admissible blocks equals the dense near field.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle_ffi import ReferenceBem, ReferenceH2, have_reference
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def build(mesh, layer, basis, clf, eta, m, alpha=0.5):
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    if layer == 0:
+        op = h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis)
+    else:
+        op = h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, alpha)
+    rt = G.ClusterTree(mesh, basis, clf)
+    bt = G.BlockTree(rt, rt, eta, 0)
+    return op, rt, bt, H2Matrix(op, mesh, rt, rt, bt, m)
+
+
+def product(H, x, trans=False, alpha=1.0, y0=None):
+    import h2lib_amd as h
+    dx = h.DeviceArray.from_host(x)
+    dy = h.DeviceArray.from_host(np.zeros_like(x) if y0 is None else y0)
+    H.addeval(alpha, dx, dy, trans=trans)
+    return dy.download()
+
+
+def test_matches_reference_fixture():
+    from h2lib_amd import geometry as G
+    g = golden("h2_sphere_s8.npz")
+    mesh = G.sphere(8)
+    op, rt, bt, H = build(mesh, 0, 1, 16, 2.0, 3)
+    assert np.array_equal(rt.idx, g["idx"])
+    y = product(H, g["x"])
+    assert np.abs(y - g["y"]).max() < TOL * np.abs(g["y"]).max()
+    yt = product(H, g["x"], trans=True)
+    assert np.abs(yt - g["yt"]).max() < TOL * np.abs(g["yt"]).max()
+
+
+@pytest.mark.skipif(not have_reference(False), reason="reference not built")
+@pytest.mark.parametrize("layer,basis", [(0, 1), (1, 1), (0, 0), (1, 0)])
+def test_matches_live_reference(layer, basis):
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(16)
+    op, rt, bt, H = build(mesh, layer, basis, 32, 2.0, 3)
+    ref = ReferenceBem(0, layer, basis, basis, mesh.x, mesh.t, q=2, q2=4,
+                       alpha=0.5 if layer else 0.0)
+    rh = ReferenceH2(ref, 32, 2.0, 3, 0)
+    assert np.array_equal(rh.idx(), rt.idx)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, len(rt.idx))
+    y0 = rng.uniform(-1, 1, len(rt.idx))
+    print("h2 layer", layer, "basis", basis, flush=True)
+    for trans in (False, True):
+        want = rh.addeval(x, y0, alpha=-0.75, trans=trans)
+        got = product(H, x, trans=trans, alpha=-0.75, y0=y0)
+        assert np.abs(got - want).max() < TOL * np.abs(want).max(), trans
+
+
+def test_adjoint_and_linearity():
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(12)
+    op, rt, bt, H = build(mesh, 1, 1, 24, 1.5, 3)
+    rng = np.random.default_rng(8)
+    n = len(rt.idx)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    Hx = product(H, x)
+    Hty = product(H, y, trans=True)
+    assert abs(np.dot(Hx, y) - np.dot(x, Hty)) < 1e-12 * np.abs(Hx).max() * \
+        np.abs(y).sum()
+    z = rng.standard_normal(n)
+    Hz = product(H, z)
+    Hxz = product(H, 2.0 * x - 3.0 * z)
+    assert np.abs(Hxz - (2.0 * Hx - 3.0 * Hz)).max() < 1e-12 * \
+        np.abs(Hxz).max()
+
+
+def test_all_near_equals_dense():
+    # eta = 0: no admissible blocks, H is the near field of all leaf pairs,
+    # i.e. the dense Galerkin matrix
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(6)
+    op, rt, bt, H = build(mesh, 0, 1, 20, 0.0, 2)
+    assert len(bt.far) == 0
+    dense = op.nearfield(None, None)
+    x = np.random.default_rng(1).standard_normal(dense.shape[1])
+    y = product(H, x)
+    assert np.abs(y - dense @ x).max() < 1e-13 * np.abs(dense @ x).max()
