@@ -1,0 +1,204 @@
+/* Boundary data, right-hand sides and error functionals of the Laplace
+ * Dirichlet problem with the piecewise linear basis.
+ *
+ * Restates integrate_bem3d_l_avector (Library/bem3d.c:8631-8702),
+ * projectL2_bem3d_l_avector (Library/bem3d.c:9009-9088, CG on the mass
+ * matrix), addeval_mass_linear_bem3d (Library/bem3d.c:8878-8908),
+ * normL2_bem3d (Library/bem3d.c:8704-8751) and normL2diff_l_bem3d
+ * (Library/bem3d.c:8954-9007), with the boundary data of
+ * eval_dirichlet_fundamental_laplacebem3d and
+ * eval_neumann_fundamental_laplacebem3d (Library/laplacebem3d.c:1390-1430)
+ * for a point source at x0 (the reference fixes x0 = (1.2, 1.2, 1.2)).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "h2bem.h"
+
+/* f(x, n): 0 Dirichlet data 1/|x - x0|, 1 Neumann data
+ * -<x - x0, n>/|x - x0|^3 */
+static double
+field_value(int kind, const double *x0, const double *x, const double *n)
+{
+  double d[3] = {x[0] - x0[0], x[1] - x0[1], x[2] - x0[2]};
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (kind == 0)
+    return 1.0 / sqrt(r2);
+  return -(d[0] * n[0] + d[1] * n[1] + d[2] * n[2]) / (r2 * sqrt(r2));
+}
+
+static unsigned
+tri_rule(unsigned q, double *x0, double *x1, double *w)
+{
+  /* Gauss-Legendre on [0, 1] (Newton on the Legendre recurrence), then the
+   * Duffy-collapsed tensor rule on the reference triangle */
+  double g[16], gw[16];
+  for (unsigned i = 0; i < q; i++) {
+    double z = -cos(M_PI * (4.0 * i + 3.0) / (4.0 * q + 2.0)), dp = 1.0;
+    for (int it = 0; it < 64; it++) {
+      double p0 = 0.0, p1 = 1.0;
+      for (unsigned n = 0; n < q; n++) {
+        double p2 = ((2.0 * n + 1.0) * z * p1 - n * p0) / (n + 1.0);
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = q * (p0 - z * p1) / (1.0 - z * z);
+      double step = p1 / dp;
+      z -= step;
+      if (fabs(step) <= 1e-16 * fabs(z) + 1e-300)
+        break;
+    }
+    double p0 = 0.0, p1 = 1.0;
+    for (unsigned n = 0; n < q; n++) {
+      double p2 = ((2.0 * n + 1.0) * z * p1 - n * p0) / (n + 1.0);
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = q * (p0 - z * p1) / (1.0 - z * z);
+    g[i] = 0.5 + 0.5 * z;
+    gw[i] = 0.5 * (2.0 / ((1.0 - z * z) * dp * dp));
+  }
+  unsigned k = 0;
+  for (unsigned i = 0; i < q; i++)
+    for (unsigned j = 0; j < q; j++, k++) {
+      x0[k] = g[i];
+      x1[k] = g[i] * g[j];
+      w[k] = gw[i] * gw[j] * g[i];
+    }
+  return k;
+}
+
+void
+h2b_integrate_l(const h2b_surface *s, unsigned q, int kind, const double *x0,
+                double *out)
+{
+  double rx0[256], rx1[256], rw[256];
+  const unsigned nq = tri_rule(q, rx0, rx1, rw);
+  memset(out, 0, sizeof(double) * s->vertices);
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double *A = s->x + 3 * tv[0], *B = s->x + 3 * tv[1],
+      *C = s->x + 3 * tv[2];
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (unsigned k = 0; k < nq; k++) {
+      double u = rx0[k], v = rx1[k], x[3];
+      for (int d = 0; d < 3; d++)
+        x[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+      double f = rw[k] * field_value(kind, x0, x, s->n + 3 * t);
+      sum[0] += f * (1.0 - u);
+      sum[1] += f * (u - v);
+      sum[2] += f * v;
+    }
+    for (int l = 0; l < 3; l++)
+      out[tv[l]] += sum[l] * s->g[t];
+  }
+}
+
+void
+h2b_mass_l_addeval(const h2b_surface *s, double alpha, const double *x,
+                   double *y)
+{
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double gt = s->g[t];
+    for (int l = 0; l < 3; l++) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; k++)
+        acc += x[tv[k]] * gt * (k == l ? 1.0 / 12.0 : 1.0 / 24.0);
+      y[tv[l]] += alpha * acc;
+    }
+  }
+}
+
+int
+h2b_projectL2_l(const h2b_surface *s, unsigned q, int kind, const double *x0,
+                double *out)
+{
+  const uint32_t n = s->vertices;
+  double *b = (double *) malloc(sizeof(double) * n);
+  double *r = (double *) malloc(sizeof(double) * n);
+  double *p = (double *) malloc(sizeof(double) * n);
+  double *ap = (double *) malloc(sizeof(double) * n);
+  h2b_integrate_l(s, q, kind, x0, b);
+  /* CG from x = 0 on the (symmetric positive definite) mass matrix to
+   * |r| <= 1e-15 |b|, at most 500 steps as the reference */
+  memset(out, 0, sizeof(double) * n);
+  memcpy(r, b, sizeof(double) * n);
+  memcpy(p, b, sizeof(double) * n);
+  double rr = 0.0, bb;
+  for (uint32_t i = 0; i < n; i++)
+    rr += r[i] * r[i];
+  bb = rr;
+  int it;
+  for (it = 0; it < 500 && rr > 1e-30 * bb; it++) {
+    memset(ap, 0, sizeof(double) * n);
+    h2b_mass_l_addeval(s, 1.0, p, ap);
+    double pap = 0.0;
+    for (uint32_t i = 0; i < n; i++)
+      pap += p[i] * ap[i];
+    double a = rr / pap, rr2 = 0.0;
+    for (uint32_t i = 0; i < n; i++) {
+      out[i] += a * p[i];
+      r[i] -= a * ap[i];
+      rr2 += r[i] * r[i];
+    }
+    double beta = rr2 / rr;
+    rr = rr2;
+    for (uint32_t i = 0; i < n; i++)
+      p[i] = r[i] + beta * p[i];
+  }
+  free(b);
+  free(r);
+  free(p);
+  free(ap);
+  return it;
+}
+
+double
+h2b_normL2diff_l(const h2b_surface *s, unsigned q, const double *xv,
+                 int kind, const double *x0)
+{
+  double rx0[256], rx1[256], rw[256];
+  const unsigned nq = tri_rule(q, rx0, rx1, rw);
+  double norm = 0.0;
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double *A = s->x + 3 * tv[0], *B = s->x + 3 * tv[1],
+      *C = s->x + 3 * tv[2];
+    double sum = 0.0;
+    for (unsigned k = 0; k < nq; k++) {
+      double u = rx0[k], v = rx1[k], x[3];
+      for (int d = 0; d < 3; d++)
+        x[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+      double bf = xv[tv[0]] * (1.0 - u) + xv[tv[1]] * (u - v) + xv[tv[2]] * v;
+      double e = field_value(kind, x0, x, s->n + 3 * t) - bf;
+      sum += rw[k] * e * e;
+    }
+    norm += s->g[t] * sum;
+  }
+  return sqrt(norm);
+}
+
+double
+h2b_normL2(const h2b_surface *s, unsigned q, int kind, const double *x0)
+{
+  double rx0[256], rx1[256], rw[256];
+  const unsigned nq = tri_rule(q, rx0, rx1, rw);
+  double norm = 0.0;
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double *A = s->x + 3 * tv[0], *B = s->x + 3 * tv[1],
+      *C = s->x + 3 * tv[2];
+    double sum = 0.0;
+    for (unsigned k = 0; k < nq; k++) {
+      double u = rx0[k], v = rx1[k], x[3];
+      for (int d = 0; d < 3; d++)
+        x[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+      double f = field_value(kind, x0, x, s->n + 3 * t);
+      sum += rw[k] * f * f;
+    }
+    norm += s->g[t] * sum;
+  }
+  return sqrt(norm);
+}

@@ -1,0 +1,116 @@
+"""Krylov solves with device H2-matrices, and the interior Dirichlet problem
+of the Laplace equation with the piecewise linear basis.
+
+solve_gmres mirrors solve_gmres_avector (Library/krylovsolvers.c:183-227).
+LaplaceDirichlet follows the reference's Dirichlet test
+(Tests/test_laplacebem3d.c:100-160): f = L2 projection of the Dirichlet data
+u(x) = 1/|x - x0|, b = (K + M/2) f with the double layer H2-matrix, V x = b
+with the single layer H2-matrix, error against the Neumann data of u.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, hip
+from .bem3d import (BASIS_LINEAR_BEM3D, Surface3d, new_dlp_laplace_bem3d,
+                    new_slp_laplace_bem3d)
+from .device import DeviceArray
+from .geometry import BlockTree, ClusterTree, bem
+from .h2matrix import H2Matrix
+
+_done = False
+
+
+def _sigs():
+    global _done
+    if _done:
+        return
+    h = hip()
+    h.h2hip_solve_gmres.restype = C.c_int
+    h.h2hip_solve_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_uint64, C.c_void_p, C.c_void_p,
+                                    C.c_double, C.c_uint32, C.c_uint32,
+                                    C.c_void_p, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_double)]
+    b = bem()
+    dp = C.c_void_p
+    b.h2b_integrate_l.argtypes = [dp, C.c_uint, C.c_int, dp, dp]
+    b.h2b_projectL2_l.restype = C.c_int
+    b.h2b_projectL2_l.argtypes = [dp, C.c_uint, C.c_int, dp, dp]
+    b.h2b_normL2diff_l.restype = C.c_double
+    b.h2b_normL2diff_l.argtypes = [dp, C.c_uint, dp, C.c_int, dp]
+    b.h2b_normL2.restype = C.c_double
+    b.h2b_normL2.argtypes = [dp, C.c_uint, C.c_int, dp]
+    _done = True
+
+
+def solve_gmres(H, d_b, d_x, eps, maxiter, kmax, stream=None):
+    """GMRES(kmax) for H x = b on device arrays; x holds the start vector.
+    Returns (steps, residual estimate)."""
+    _sigs()
+    fn = C.cast(hip().h2hip_addeval_h2matrix, C.c_void_p)
+    it = C.c_uint32()
+    res = C.c_double()
+    check(hip().h2hip_solve_gmres(fn, H._h, int(H.op.is_complex), d_b.n,
+                                  C.c_void_p(d_b.ptr), C.c_void_p(d_x.ptr),
+                                  eps, maxiter, kmax,
+                                  C.c_void_p(stream.handle if stream else 0),
+                                  C.byref(it), C.byref(res)),
+          "h2hip_solve_gmres")
+    return it.value, res.value
+
+
+class LaplaceDirichlet:
+    """Interior Dirichlet problem on a mesh (piecewise linear basis)."""
+
+    def __init__(self, mesh, clf=64, eta=2.0, m=4, q=2, q2=4):
+        _sigs()
+        self.mesh = mesh
+        self.q = q
+        self.gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+        lin = BASIS_LINEAR_BEM3D
+        self.slp = new_slp_laplace_bem3d(self.gr, q, q2, lin, lin)
+        self.dlp = new_dlp_laplace_bem3d(self.gr, q, q2, lin, lin, 0.5)
+        self.tree = ClusterTree(mesh, lin, clf)
+        self.blocks = BlockTree(self.tree, self.tree, eta, 0)
+        self.V = H2Matrix(self.slp, mesh, self.tree, self.tree, self.blocks,
+                          m)
+        self.KM = H2Matrix(self.dlp, mesh, self.tree, self.tree, self.blocks,
+                           m)
+        self.n = mesh.vertices
+
+    def project(self, x0, kind=0):
+        """L2 projection of the Dirichlet (kind 0) or Neumann (kind 1) data
+        of a point source at x0 (host array)."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        out = np.zeros(self.n)
+        bem().h2b_projectL2_l(self.mesh._sp, self.q, kind,
+                              x0.ctypes.data_as(C.c_void_p),
+                              out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def rhs(self, x0, stream=None):
+        """Device vectors f and b = (K + M/2) f."""
+        f = self.project(x0)
+        d_f = DeviceArray.from_host(f)
+        d_b = DeviceArray(self.n)
+        d_b.zero()
+        self.KM.addeval(1.0, d_f, d_b, stream)
+        return d_f, d_b
+
+    def solve(self, d_b, eps=1e-8, maxiter=1000, kmax=30, stream=None,
+              d_x=None):
+        if d_x is None:
+            d_x = DeviceArray(self.n)
+            d_x.zero()
+        it, res = solve_gmres(self.V, d_b, d_x, eps, maxiter, kmax, stream)
+        return d_x, it, res
+
+    def neumann_error(self, x, x0):
+        """Relative L2 error of the Neumann data of a point source at x0."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        p0 = x0.ctypes.data_as(C.c_void_p)
+        e = bem().h2b_normL2diff_l(self.mesh._sp, self.q,
+                                   x.ctypes.data_as(C.c_void_p), 1, p0)
+        return e / bem().h2b_normL2(self.mesh._sp, self.q, 1, p0)

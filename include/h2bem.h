@@ -121,6 +121,28 @@ h2b_h2interp *h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
 void h2b_h2interp_coupling_laplace(h2b_h2interp *H, const h2b_blocks *bl);
 void h2b_h2interp_free(h2b_h2interp *H);
 
+/* ---- boundary data and functionals (piecewise linear basis) ---------------
+ * kind 0: Dirichlet data 1/|x - x0|, kind 1: Neumann data
+ * -<x - x0, n>/|x - x0|^3 (eval_{dirichlet,neumann}_fundamental_
+ * laplacebem3d, Library/laplacebem3d.c:1390-1430, with x0 as parameter).
+ * q: order of the single-triangle Gauss rule. */
+/* out[i] = int f phi_i (integrate_bem3d_l_avector, Library/bem3d.c:8631) */
+void h2b_integrate_l(const h2b_surface *s, unsigned q, int kind,
+                     const double *x0, double *out);
+/* y += alpha M x, M the mass matrix (Library/bem3d.c:8878-8908) */
+void h2b_mass_l_addeval(const h2b_surface *s, double alpha, const double *x,
+                        double *y);
+/* L2 projection M^{-1} int f phi (projectL2_bem3d_l_avector,
+ * Library/bem3d.c:9009-9088); returns the CG steps */
+int h2b_projectL2_l(const h2b_surface *s, unsigned q, int kind,
+                    const double *x0, double *out);
+/* |sum_i x_i phi_i - f|_L2 (normL2diff_l_bem3d, Library/bem3d.c:8954) */
+double h2b_normL2diff_l(const h2b_surface *s, unsigned q, const double *x,
+                        int kind, const double *x0);
+/* |f|_L2 (normL2_bem3d, Library/bem3d.c:8704) */
+double h2b_normL2(const h2b_surface *s, unsigned q, int kind,
+                  const double *x0);
+
 #ifdef __cplusplus
 }
 #endif

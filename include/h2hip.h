@@ -184,6 +184,23 @@ int h2hip_addeval_h2matrix(const double *alpha, const h2hip_h2matrix *H,
 int h2hip_addevaltrans_h2matrix(const double *alpha, const h2hip_h2matrix *H,
                                 const void *d_x, void *d_y, void *stream);
 
+/* ---- Krylov solvers ----------------------------------------------------------
+ * y += alpha A x on device vectors, the counterpart of the reference's
+ * addeval_t (Library/krylov.h:41) with a stream; h2hip_addeval_h2matrix has
+ * this form. */
+typedef int (*h2hip_addeval_t)(const double *alpha, void *A, const void *x,
+                               void *y, void *stream);
+/* Restarted GMRES(kmax) for A x = b on device vectors (x holds the start
+ * vector): stops when the least-squares residual is at most eps |b| or
+ * after maxiter - 1 steps, restarts when the Krylov dimension reaches
+ * kmax - 1.  solve_gmres_avector (Library/krylovsolvers.c:183-227) with
+ * init/step/finish_gmres (Library/krylov.c:612-770).  FP64 inner products.
+ * Returns the number of steps and the final residual estimate. */
+int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A, int is_complex,
+                      uint64_t n, const void *d_b, void *d_x, double eps,
+                      uint32_t maxiter, uint32_t kmax, void *stream,
+                      uint32_t *iterations, double *residual);
+
 /* Waits for all work queued on the stream. */
 int h2hip_stream_sync(void *stream);
 

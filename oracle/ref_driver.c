@@ -520,6 +520,46 @@ ref_h2_addeval(void *h, int trans, const double *alpha, const void *x,
     addeval_h2matrix_avector(a, r->h2, xv, yv);
 }
 
+/* Interior Dirichlet problem of the Laplace equation with the piecewise
+ * linear basis, as Tests/test_laplacebem3d.c:100-160 sets it up but solved
+ * with GMRES(kmax): f = L2 projection of u(x) = 1/|x - (1.2, 1.2, 1.2)|,
+ * b = (K + M/2) f with the H2 double layer, V x = b with the H2 single
+ * layer.  slp / dlp are H2 handles of ref_h2_new on the SLP and on the DLP
+ * (alpha = 1/2) operator.  Writes f, b, x; returns the GMRES steps and
+ * stores the relative L2 error of x against the Neumann data in *relerr. */
+unsigned
+ref_dirichlet_laplace(void *slp, void *dlp, double eps, unsigned maxiter,
+		      unsigned kmax, void *fo, void *bo, void *xo,
+		      double *relerr)
+{
+  refh2    *V = (refh2 *) slp, *KM = (refh2 *) dlp;
+  uint      n = V->root->size;
+  pavector  f, b, x;
+  unsigned  it;
+
+  f = new_avector(n);
+  b = new_avector(n);
+  x = new_avector(n);
+  projectL2_bem3d_l_avector(KM->rb->bem,
+			    eval_dirichlet_fundamental_laplacebem3d, f, NULL);
+  clear_avector(b);
+  addeval_h2matrix_avector(1.0, KM->h2, f, b);
+  clear_avector(x);
+  it = solve_gmres_avector((void *) V->h2,
+			   (addeval_t) addeval_h2matrix_avector, b, x, eps,
+			   maxiter, kmax);
+  *relerr = normL2diff_l_bem3d(V->rb->bem, x,
+			       eval_neumann_fundamental_laplacebem3d, NULL)
+    / normL2_bem3d(V->rb->bem, eval_neumann_fundamental_laplacebem3d, NULL);
+  memcpy(fo, f->v, sizeof(field) * n);
+  memcpy(bo, b->v, sizeof(field) * n);
+  memcpy(xo, x->v, sizeof(field) * n);
+  del_avector(x);
+  del_avector(b);
+  del_avector(f);
+  return it;
+}
+
 /* GMRES(kmax) on the H2-matrix: solves H x = b to |r| <= eps |b|.
  * x holds the start vector on entry.  Returns the iteration count. */
 unsigned

@@ -325,3 +325,24 @@ class ReferenceH2:
         it = self._lib.ref_h2_gmres(self.h, _ptr(bb), _ptr(xx), eps, maxiter,
                                     kmax)
         return xx, it
+
+
+def reference_dirichlet(mesh_x, mesh_t, clf, eta, m, eps, maxiter, kmax):
+    """The reference's interior Dirichlet problem with GMRES(kmax): returns
+    f, b, x, steps, relative Neumann error."""
+    lib = Reference.lib(False)
+    lib.ref_dirichlet_laplace.restype = C.c_uint
+    lib.ref_dirichlet_laplace.argtypes = [C.c_void_p, C.c_void_p, C.c_double,
+                                          C.c_uint, C.c_uint, C.c_void_p,
+                                          C.c_void_p, C.c_void_p,
+                                          C.POINTER(C.c_double)]
+    slp = ReferenceBem(0, 0, 1, 1, mesh_x, mesh_t, q=2, q2=4)
+    dlp = ReferenceBem(0, 1, 1, 1, mesh_x, mesh_t, q=2, q2=4, alpha=0.5)
+    hs = ReferenceH2(slp, clf, eta, m, 0)
+    hd = ReferenceH2(dlp, clf, eta, m, 0)
+    n = hs.n
+    f, b, x = np.zeros(n), np.zeros(n), np.zeros(n)
+    err = C.c_double()
+    it = lib.ref_dirichlet_laplace(hs.h, hd.h, eps, maxiter, kmax, _ptr(f),
+                                   _ptr(b), _ptr(x), C.byref(err))
+    return f, b, x, it, err.value
