@@ -1,26 +1,28 @@
 // H2-matrix products y += alpha H x and y += alpha H^* x on gfx950.
 //
 // The reference evaluates the product by three recursions over its pointer
-// trees (addeval_h2matrix_avector, Library/h2matrix.c:1268-1286):
-// forward transform of x into the column basis coefficients
+// trees (addeval_h2matrix_avector, Library/h2matrix.c:1268-1286): forward
+// transform of x into the column basis coefficients
 // (Library/clusterbasis.c:523-573), coupling and near-field blocks
 // (fastaddeval_h2matrix_avector, Library/h2matrix.c:1210-1266), backward
-// transform into y (Library/clusterbasis.c:748-797).  Here the trees are
-// flattened into level lists and every small matrix-vector product is one
-// wavefront, lane = output entry, looping over the input entries: each
-// matrix is stored in the orientation that makes that loop read 64
-// consecutive entries per step (leaf and transfer matrices in both
-// orientations, couplings and near blocks column-major for H x).
+// transform into y (Library/clusterbasis.c:748-797).  Here one product is
+// three launches over flattened trees:
 //
-//   1. forward, leaves:   xt_t = V_t^T x|_t               (one wave per leaf)
-//   2. forward, upwards:  xt_t = sum_sons E_s^T xt_s       (one launch/level)
-//   3. couplings:         yt_r += S_rc xt_c                (wave per block,
-//                                                           FP64 atomics)
-//   4. backward, down:    yt_s += E_s yt_father            (one launch/level)
-//   5. backward, leaves:  y|_t += alpha V_t yt_t           (one wave per leaf)
-//   6. near field:        y|_r += alpha N_rc x|_c          (wave per block,
-//                                                           FP64 atomics)
-// All phases are bound by HBM bandwidth: two flops per stored entry.
+//  1. fwd_up: one wavefront per leaf computes xt_t = V_t^T x|_t, then climbs
+//     the tree: the last son to finish (an atomic counter per cluster, reset
+//     by that son) computes xt_f = sum_s E_s^T xt_s for its father and goes
+//     on upwards.  No wavefront ever waits for another one.  The same launch
+//     clears the coupling accumulator yt.
+//  2. couple_near: one wavefront per admissible block, yt_r += S_rc xt_c
+//     (FP64 atomics), and one per near-field block, y|_r += alpha N_rc x|_c.
+//  3. down_back: one wavefront per leaf of the output tree walks its path
+//     from the root, v = yt_s + E_s v, and adds y|_t += alpha V_t v.
+//
+// Every small product uses lane = output entry and loops over the input
+// entries, with each matrix stored in the orientation that makes that loop
+// read 64 consecutive entries per step (leaf and transfer matrices are kept
+// in both orientations, couplings and near blocks column-major).  The
+// products are bound by HBM bandwidth: two flops per stored entry.
 #include <algorithm>
 #include <cstring>
 
@@ -30,11 +32,15 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int WPB = NT / 64;
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr int MAXRANK = 128;
+constexpr int MAXPATH = 48;
 
 struct NodeD {
   uint32_t off, size, rank, father;
   uint64_t coef;  // offset of the node's coefficients
   uint64_t voff;  // offset of the leaf matrix
+  uint32_t sonbeg, nsons;
 };
 
 struct FarD {
@@ -59,6 +65,22 @@ __device__ __forceinline__ void ld(const double *p, uint64_t i, double &re,
   }
 }
 
+// load written by another wavefront of the same launch
+__device__ __forceinline__ double ld_agent(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int C>
+__device__ __forceinline__ void st(double *p, uint64_t i, double re,
+                                   double im) {
+  if constexpr (C) {
+    p[2 * i] = re;
+    p[2 * i + 1] = im;
+  } else {
+    p[i] = re;
+  }
+}
+
 template <int C>
 __device__ __forceinline__ void atomic_add(double *p, uint64_t i, double re,
                                            double im) {
@@ -70,202 +92,263 @@ __device__ __forceinline__ void atomic_add(double *p, uint64_t i, double re,
   }
 }
 
-__global__ void zero_kernel(double *p, uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
-}
+struct FwdArgs {
+  const NodeD *nodes;
+  const uint32_t *leaves;
+  uint32_t nleaves;
+  const uint32_t *idx;
+  const uint32_t *sons;
+  const double *VT;  // leaf: rank x size column-major
+  const double *ET;  // per son s: rank_father x rank_s column-major
+  uint64_t eld;
+  const double *x;
+  double *xt;
+  uint32_t *cnt;
+  double *yt;  // cleared here (doubles)
+  uint64_t nyt;
+};
 
-// xt_t = V_t^T x|_t; VT of leaf t is rank x size column-major
-template <int C>
-__global__ __launch_bounds__(NT) void fwd_leaf(
-    const NodeD *__restrict__ nodes, const uint32_t *__restrict__ leaves,
-    uint32_t nleaves, const uint32_t *__restrict__ idx,
-    const double *__restrict__ VT, const double *__restrict__ x,
-    double *__restrict__ xt) {
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= nleaves) return;
-  const NodeD t = nodes[leaves[w]];
-  const double *Vt = VT + t.voff;
-  for (uint32_t j = lane; j < t.rank; j += 64) {
-    double ar = 0.0, ai = 0.0;
-    for (uint32_t i = 0; i < t.size; i++) {
-      const double v = Vt[j + (uint64_t)i * t.rank];
+template <int C, int K>
+__global__ __launch_bounds__(64) void fwd_up(FwdArgs a) {
+  __shared__ double sv[2 * MAXRANK];
+  const uint32_t w = blockIdx.x;
+  const int lane = threadIdx.x;
+  {  // clear a slice of the coupling accumulator
+    const uint64_t chunk = (a.nyt + a.nleaves - 1) / a.nleaves;
+    const uint64_t e0 = (uint64_t)w * chunk, e1 = min(a.nyt, e0 + chunk);
+    for (uint64_t e = e0 + lane; e < e1; e += 64) a.yt[e] = 0.0;
+  }
+  uint32_t node = a.leaves[w];
+  {
+    const NodeD t = a.nodes[node];
+    const uint32_t rank = K ? K : t.rank;
+    // x|_t into LDS (leaf size <= MAXRANK is checked at creation)
+    for (uint32_t i = lane; i < t.size; i += 64) {
       double xr, xi;
-      ld<C>(x, idx[t.off + i], xr, xi);
-      ar = fma(v, xr, ar);
-      ai = fma(v, xi, ai);
+      ld<C>(a.x, a.idx[t.off + i], xr, xi);
+      sv[2 * i] = xr;
+      sv[2 * i + 1] = xi;
     }
-    if constexpr (C) {
-      xt[2 * (t.coef + j)] = ar;
-      xt[2 * (t.coef + j) + 1] = ai;
-    } else {
-      xt[t.coef + j] = ar;
+    __syncthreads();
+    const double *Vt = a.VT + t.voff;
+    for (uint32_t j = lane; j < rank; j += 64) {
+      double ar = 0.0, ai = 0.0;
+#pragma unroll 8
+      for (uint32_t i = 0; i < t.size; i++) {
+        const double v = Vt[j + (uint64_t)i * rank];
+        ar = fma(v, sv[2 * i], ar);
+        if constexpr (C) ai = fma(v, sv[2 * i + 1], ai);
+      }
+      st<C>(a.xt, t.coef + j, ar, ai);
     }
   }
-}
-
-// xt_t = sum over sons s of E_s^T xt_s; ET of son s is rank_t x rank_s
-// column-major
-template <int C>
-__global__ __launch_bounds__(NT) void up_level(
-    const NodeD *__restrict__ nodes, const uint32_t *__restrict__ list,
-    uint32_t n, const uint32_t *__restrict__ sonptr,
-    const uint32_t *__restrict__ sons, const double *__restrict__ ET,
-    uint64_t eld, double *__restrict__ xt) {
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= n) return;
-  const uint32_t tn = list[w];
-  const NodeD t = nodes[tn];
-  for (uint32_t j = lane; j < t.rank; j += 64) {
-    double ar = 0.0, ai = 0.0;
-    for (uint32_t so = sonptr[tn]; so < sonptr[tn + 1]; so++) {
-      const uint32_t sn = sons[so];
-      const NodeD s = nodes[sn];
-      const double *E = ET + (uint64_t)sn * eld;
-      for (uint32_t i = 0; i < s.rank; i++) {
-        const double e = E[j + (uint64_t)i * t.rank];
-        double xr, xi;
-        ld<C>(xt, s.coef + i, xr, xi);
-        ar = fma(e, xr, ar);
-        ai = fma(e, xi, ai);
+  // climb: the last son of a cluster computes the cluster
+  while (true) {
+    const uint32_t f = a.nodes[node].father;
+    if (f == NONE) break;
+    __threadfence();  // release this wavefront's coefficients
+    uint32_t old = 0;
+    if (lane == 0) old = atomicAdd(&a.cnt[f], 1u);
+    old = __shfl(old, 0, 64);
+    const NodeD F = a.nodes[f];
+    if (old + 1 < F.nsons) break;
+    __threadfence();  // acquire the other sons' coefficients
+    if (lane == 0) a.cnt[f] = 0;
+    const uint32_t rank = K ? K : F.rank;
+    double accr[MAXRANK / 64], acci[MAXRANK / 64];
+#pragma unroll
+    for (int c = 0; c < MAXRANK / 64; c++) accr[c] = acci[c] = 0.0;
+    for (uint32_t so = 0; so < F.nsons; so++) {
+      const uint32_t sn = a.sons[F.sonbeg + so];
+      const NodeD s = a.nodes[sn];
+      const uint32_t srank = K ? K : s.rank;
+      __syncthreads();
+      for (uint32_t i = lane; i < srank; i += 64) {
+        if constexpr (C) {
+          sv[2 * i] = ld_agent(a.xt + 2 * (s.coef + i));
+          sv[2 * i + 1] = ld_agent(a.xt + 2 * (s.coef + i) + 1);
+        } else {
+          sv[2 * i] = ld_agent(a.xt + s.coef + i);
+        }
+      }
+      __syncthreads();
+      const double *E = a.ET + (uint64_t)sn * a.eld;
+#pragma unroll
+      for (int c = 0; c < MAXRANK / 64; c++) {
+        const uint32_t j = lane + 64 * c;
+        if (j < rank) {
+#pragma unroll 8
+          for (uint32_t i = 0; i < srank; i++) {
+            const double e = E[j + (uint64_t)i * rank];
+            accr[c] = fma(e, sv[2 * i], accr[c]);
+            if constexpr (C) acci[c] = fma(e, sv[2 * i + 1], acci[c]);
+          }
+        }
       }
     }
-    if constexpr (C) {
-      xt[2 * (t.coef + j)] = ar;
-      xt[2 * (t.coef + j) + 1] = ai;
-    } else {
-      xt[t.coef + j] = ar;
+#pragma unroll
+    for (int c = 0; c < MAXRANK / 64; c++) {
+      const uint32_t j = lane + 64 * c;
+      if (j < rank) st<C>(a.xt, F.coef + j, accr[c], acci[c]);
     }
+    node = f;
   }
 }
 
-// yt_r += S_rc xt_c (trans: yt_c += S_rc^* xt_r)
+struct CNArgs {
+  const FarD *far;
+  uint32_t nfar;
+  const NearD *near;
+  uint32_t nnear;
+  const NodeD *rn, *cn;
+  const uint32_t *ridx, *cidx;
+  const double *S, *N;
+  const double *x, *xt;
+  double *yt, *y;
+  double are, aim;
+};
+
+// yt_r += S_rc xt_c and y|_r += alpha N_rc x|_c; TRANS: yt_c += S^* xt_r and
+// y|_c += alpha N^* x|_r
 template <int C, bool TRANS>
-__global__ __launch_bounds__(NT) void coupling(
-    const FarD *__restrict__ far, uint32_t nfar,
-    const NodeD *__restrict__ rn, const NodeD *__restrict__ cn,
-    const double *__restrict__ S, const double *__restrict__ xt,
-    double *__restrict__ yt) {
+__global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (w >= nfar) return;
-  const FarD b = far[w];
-  const NodeD r = rn[b.r], c = cn[b.c];
-  const NodeD out = TRANS ? c : r, in = TRANS ? r : c;
-  for (uint32_t i = lane; i < out.rank; i += 64) {
-    double ar = 0.0, ai = 0.0;
-    for (uint32_t j = 0; j < in.rank; j++) {
-      // entry S[rr, cc], rr = row of S, cc = column of S
-      const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
-                               : (uint64_t)i + (uint64_t)j * r.rank;
-      double sr, si, xr, xi;
-      ld<C>(S, b.soff + e, sr, si);
-      if (TRANS) si = -si;
-      ld<C>(xt, in.coef + j, xr, xi);
-      ar = fma(sr, xr, ar);
-      if constexpr (C) {
-        ar = fma(-si, xi, ar);
-        ai = fma(sr, xi, fma(si, xr, ai));
+  if (w < a.nfar) {
+    const FarD b = a.far[w];
+    const NodeD r = a.rn[b.r], c = a.cn[b.c];
+    const NodeD out = TRANS ? c : r, in = TRANS ? r : c;
+    for (uint32_t i = lane; i < out.rank; i += 64) {
+      double ar = 0.0, ai = 0.0;
+#pragma unroll 8
+      for (uint32_t j = 0; j < in.rank; j++) {
+        const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
+                                 : (uint64_t)i + (uint64_t)j * r.rank;
+        double sr, si, xr, xi;
+        ld<C>(a.S, b.soff + e, sr, si);
+        if (TRANS) si = -si;
+        ld<C>(a.xt, in.coef + j, xr, xi);
+        ar = fma(sr, xr, ar);
+        if constexpr (C) {
+          ar = fma(-si, xi, ar);
+          ai = fma(sr, xi, fma(si, xr, ai));
+        }
       }
+      atomic_add<C>(a.yt, out.coef + i, ar, ai);
     }
-    atomic_add<C>(yt, out.coef + i, ar, ai);
+    return;
   }
-}
-
-// yt_s += E_s yt_father; E of node s is rank_s x rank_father column-major
-template <int C>
-__global__ __launch_bounds__(NT) void down_level(
-    const NodeD *__restrict__ nodes, const uint32_t *__restrict__ list,
-    uint32_t n, const double *__restrict__ E, uint64_t eld,
-    double *__restrict__ yt) {
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= n) return;
-  const uint32_t sn = list[w];
-  const NodeD s = nodes[sn];
-  const NodeD f = nodes[s.father];
-  const double *Es = E + (uint64_t)sn * eld;
-  for (uint32_t i = lane; i < s.rank; i += 64) {
-    double ar = 0.0, ai = 0.0;
-    for (uint32_t j = 0; j < f.rank; j++) {
-      const double e = Es[i + (uint64_t)j * s.rank];
-      double yr, yi;
-      ld<C>(yt, f.coef + j, yr, yi);
-      ar = fma(e, yr, ar);
-      ai = fma(e, yi, ai);
-    }
-    if constexpr (C) {
-      yt[2 * (s.coef + i)] += ar;
-      yt[2 * (s.coef + i) + 1] += ai;
-    } else {
-      yt[s.coef + i] += ar;
-    }
-  }
-}
-
-// y|_t += alpha V_t yt_t; V of leaf t is size x rank column-major
-template <int C>
-__global__ __launch_bounds__(NT) void back_leaf(
-    const NodeD *__restrict__ nodes, const uint32_t *__restrict__ leaves,
-    uint32_t nleaves, const uint32_t *__restrict__ idx,
-    const double *__restrict__ V, const double *__restrict__ yt, double are,
-    double aim, double *__restrict__ y) {
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= nleaves) return;
-  const NodeD t = nodes[leaves[w]];
-  const double *Vt = V + t.voff;
-  for (uint32_t i = lane; i < t.size; i += 64) {
-    double ar = 0.0, ai = 0.0;
-    for (uint32_t j = 0; j < t.rank; j++) {
-      const double v = Vt[i + (uint64_t)j * t.size];
-      double yr, yi;
-      ld<C>(yt, t.coef + j, yr, yi);
-      ar = fma(v, yr, ar);
-      ai = fma(v, yi, ai);
-    }
-    const uint32_t d = idx[t.off + i];
-    if constexpr (C) {
-      y[2 * d] += are * ar - aim * ai;
-      y[2 * d + 1] += are * ai + aim * ar;
-    } else {
-      y[d] += are * ar;
-    }
-  }
-}
-
-// y|_r += alpha N x|_c (trans: y|_c += alpha N^* x|_r)
-template <int C, bool TRANS>
-__global__ __launch_bounds__(NT) void near_blocks(
-    const NearD *__restrict__ nb, uint32_t nnear,
-    const uint32_t *__restrict__ ridx, const uint32_t *__restrict__ cidx,
-    const double *__restrict__ N, const double *__restrict__ x, double are,
-    double aim, double *__restrict__ y) {
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= nnear) return;
-  const NearD b = nb[w];
+  const uint32_t nb = w - a.nfar;
+  if (nb >= a.nnear) return;
+  const NearD b = a.near[nb];
   const uint32_t nout = TRANS ? b.cols : b.rows, nin = TRANS ? b.rows : b.cols;
-  const uint32_t *oidx = TRANS ? cidx + b.coff : ridx + b.roff;
-  const uint32_t *iidx = TRANS ? ridx + b.roff : cidx + b.coff;
+  const uint32_t *oidx = TRANS ? a.cidx + b.coff : a.ridx + b.roff;
+  const uint32_t *iidx = TRANS ? a.ridx + b.roff : a.cidx + b.coff;
   for (uint32_t i = lane; i < nout; i += 64) {
     double ar = 0.0, ai = 0.0;
+#pragma unroll 8
     for (uint32_t j = 0; j < nin; j++) {
       const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * b.rows
                                : (uint64_t)i + (uint64_t)j * b.rows;
       double nr, ni, xr, xi;
-      ld<C>(N, b.noff + e, nr, ni);
+      ld<C>(a.N, b.noff + e, nr, ni);
       if (TRANS) ni = -ni;
-      ld<C>(x, iidx[j], xr, xi);
+      ld<C>(a.x, iidx[j], xr, xi);
       ar = fma(nr, xr, ar);
       if constexpr (C) {
         ar = fma(-ni, xi, ar);
         ai = fma(nr, xi, fma(ni, xr, ai));
       }
     }
-    atomic_add<C>(y, oidx[i], are * ar - aim * ai, are * ai + aim * ar);
+    atomic_add<C>(a.y, oidx[i], a.are * ar - a.aim * ai,
+                  a.are * ai + a.aim * ar);
+  }
+}
+
+struct DBArgs {
+  const NodeD *nodes;
+  const uint32_t *leaves;
+  uint32_t nleaves;
+  const uint32_t *paths;  // per leaf: MAXPATH node numbers, root first
+  const uint32_t *plen;
+  const uint32_t *idx;
+  const double *V;  // leaf: size x rank column-major
+  const double *E;  // per node s: rank_s x rank_father column-major
+  uint64_t eld;
+  const double *yt;
+  double *y;
+  double are, aim;
+};
+
+// v = yt_root; for each node s on the path below the root v = yt_s + E_s v;
+// at the leaf y|_t += alpha V_t v
+template <int C, int K>
+__global__ __launch_bounds__(64) void down_back(DBArgs a) {
+  __shared__ double sv[2 * MAXRANK];
+  const uint32_t w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint32_t *path = a.paths + (uint64_t)w * MAXPATH;
+  const uint32_t len = a.plen[w];
+  {
+    const NodeD r = a.nodes[path[0]];
+    for (uint32_t i = lane; i < r.rank; i += 64) {
+      double vr, vi;
+      ld<C>(a.yt, r.coef + i, vr, vi);
+      sv[2 * i] = vr;
+      sv[2 * i + 1] = vi;
+    }
+  }
+  for (uint32_t p = 1; p < len; p++) {
+    const uint32_t sn = path[p];
+    const NodeD s = a.nodes[sn];
+    const uint32_t kf = a.nodes[s.father].rank;
+    const uint32_t rank = K ? K : s.rank;
+    const double *E = a.E + (uint64_t)sn * a.eld;
+    double accr[MAXRANK / 64], acci[MAXRANK / 64];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXRANK / 64; c++) {
+      accr[c] = acci[c] = 0.0;
+      const uint32_t i = lane + 64 * c;
+      if (i < rank) {
+        ld<C>(a.yt, s.coef + i, accr[c], acci[c]);
+#pragma unroll 8
+        for (uint32_t j = 0; j < kf; j++) {
+          const double e = E[i + (uint64_t)j * rank];
+          accr[c] = fma(e, sv[2 * j], accr[c]);
+          if constexpr (C) acci[c] = fma(e, sv[2 * j + 1], acci[c]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXRANK / 64; c++) {
+      const uint32_t i = lane + 64 * c;
+      if (i < rank) {
+        sv[2 * i] = accr[c];
+        sv[2 * i + 1] = acci[c];
+      }
+    }
+  }
+  __syncthreads();
+  const NodeD t = a.nodes[a.leaves[w]];
+  const uint32_t rank = K ? K : t.rank;
+  const double *Vt = a.V + t.voff;
+  for (uint32_t i = lane; i < t.size; i += 64) {
+    double ar = 0.0, ai = 0.0;
+#pragma unroll 8
+    for (uint32_t j = 0; j < rank; j++) {
+      const double v = Vt[i + (uint64_t)j * t.size];
+      ar = fma(v, sv[2 * j], ar);
+      if constexpr (C) ai = fma(v, sv[2 * j + 1], ai);
+    }
+    const uint32_t d = a.idx[t.off + i];
+    if constexpr (C) {
+      a.y[2 * d] += a.are * ar - a.aim * ai;
+      a.y[2 * d + 1] += a.are * ai + a.aim * ar;
+    } else {
+      a.y[d] += a.are * ar;
+    }
   }
 }
 
@@ -273,17 +356,14 @@ __global__ __launch_bounds__(NT) void near_blocks(
 
 struct h2hip_h2matrix {
   int cplx = 0;
+  uint32_t urank = 0;  // common rank of all clusters, 0 if they differ
   struct Side {
     uint32_t nnodes = 0, ndof = 0, depth = 0;
     std::vector<NodeD> nodes;
-    std::vector<uint32_t> level;
-    std::vector<uint32_t> leaves;
-    std::vector<std::vector<uint32_t>> inner_at, all_at;  // per level
-    std::vector<uint32_t> sonptr, sons;
+    std::vector<uint32_t> leaves, sons, paths, plen;
     uint64_t ncoef = 0, vlen = 0, eld = 0;
     h2hip::DevBuf<NodeD> d_nodes;
-    h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sonptr, d_sons, d_lists;
-    std::vector<uint64_t> inner_off, all_off;  // into d_lists
+    h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sons, d_paths, d_plen, d_cnt;
     h2hip::DevBuf<double> d_V, d_VT, d_E, d_ET;
   } side[2];
   uint32_t nnear = 0, nfar = 0;
@@ -304,9 +384,8 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
   S.nnodes = nnodes;
   S.ndof = ndof;
   S.nodes.resize(nnodes);
-  S.level.assign(nnodes, 0);
-  std::vector<uint32_t> nsons(nnodes, 0);
-  uint32_t root = 0xffffffffu;
+  std::vector<uint32_t> nsons(nnodes, 0), level(nnodes, 0);
+  uint32_t root = NONE;
   for (uint32_t i = 0; i < nnodes; i++) {
     NodeD &d = S.nodes[i];
     d.off = tree[3 * i];
@@ -317,8 +396,12 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       set_error("h2matrix: cluster exceeds the index array");
       return H2HIP_EINVAL;
     }
-    if (d.father == 0xffffffffu) {
-      if (root != 0xffffffffu) {
+    if (d.rank > (uint32_t)MAXRANK) {
+      set_error("h2matrix: rank above 128");
+      return H2HIP_EINVAL;
+    }
+    if (d.father == NONE) {
+      if (root != NONE) {
         set_error("h2matrix: more than one root");
         return H2HIP_EINVAL;
       }
@@ -330,143 +413,154 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       nsons[d.father]++;
     }
   }
-  if (root == 0xffffffffu) {
+  if (root == NONE) {
     set_error("h2matrix: no root");
     return H2HIP_EINVAL;
   }
-  // levels (father chains; nodes may come in any order)
   for (uint32_t i = 0; i < nnodes; i++) {
     uint32_t l = 0, f = S.nodes[i].father;
-    while (f != 0xffffffffu) {
+    while (f != NONE) {
       l++;
       f = S.nodes[f].father;
-      if (l > nnodes) {
-        set_error("h2matrix: cycle in the tree");
+      if (l >= (uint32_t)MAXPATH) {
+        set_error("h2matrix: tree deeper than 47 levels or cyclic");
         return H2HIP_EINVAL;
       }
     }
-    S.level[i] = l;
+    level[i] = l;
     S.depth = std::max(S.depth, l);
   }
-  S.sonptr.assign(nnodes + 1, 0);
-  for (uint32_t i = 0; i < nnodes; i++) S.sonptr[i + 1] = S.sonptr[i] + nsons[i];
-  S.sons.assign(S.sonptr[nnodes], 0);
+  std::vector<uint32_t> beg(nnodes + 1, 0);
+  for (uint32_t i = 0; i < nnodes; i++) beg[i + 1] = beg[i] + nsons[i];
+  S.sons.assign(beg[nnodes], 0);
   std::vector<uint32_t> fill(nnodes, 0);
   for (uint32_t i = 0; i < nnodes; i++) {
     const uint32_t f = S.nodes[i].father;
-    if (f != 0xffffffffu) S.sons[S.sonptr[f] + fill[f]++] = i;
+    if (f != NONE) S.sons[beg[f] + fill[f]++] = i;
   }
-  S.inner_at.assign(S.depth + 1, {});
-  S.all_at.assign(S.depth + 1, {});
   S.leaves.clear();
   uint64_t coef = 0, v = 0, maxr = 0;
   for (uint32_t i = 0; i < nnodes; i++) {
     NodeD &d = S.nodes[i];
     d.coef = coef;
+    d.sonbeg = beg[i];
+    d.nsons = nsons[i];
     coef += d.rank;
     maxr = std::max<uint64_t>(maxr, d.rank);
     if (nsons[i] == 0) {
+      if (d.size > (uint32_t)MAXRANK) {
+        set_error("h2matrix: leaf with more than 128 DOFs");
+        return H2HIP_EINVAL;
+      }
       d.voff = v;
       v += (uint64_t)d.size * d.rank;
       S.leaves.push_back(i);
     } else {
       d.voff = 0;
-      S.inner_at[S.level[i]].push_back(i);
     }
-    S.all_at[S.level[i]].push_back(i);
+  }
+  // root-to-leaf paths
+  S.paths.assign((uint64_t)S.leaves.size() * MAXPATH, 0);
+  S.plen.assign(S.leaves.size(), 0);
+  for (size_t l = 0; l < S.leaves.size(); l++) {
+    uint32_t chain[MAXPATH], n = 0;
+    for (uint32_t t = S.leaves[l]; t != NONE; t = S.nodes[t].father)
+      chain[n++] = t;
+    for (uint32_t k = 0; k < n; k++)
+      S.paths[l * MAXPATH + k] = chain[n - 1 - k];
+    S.plen[l] = n;
   }
   S.ncoef = coef;
   S.vlen = v;
   S.eld = maxr * maxr;
-  std::vector<uint32_t> lists;
-  S.inner_off.assign(S.depth + 2, 0);
-  S.all_off.assign(S.depth + 2, 0);
-  for (uint32_t l = 0; l <= S.depth; l++) {
-    S.inner_off[l] = lists.size();
-    lists.insert(lists.end(), S.inner_at[l].begin(), S.inner_at[l].end());
-  }
-  S.inner_off[S.depth + 1] = lists.size();
-  for (uint32_t l = 0; l <= S.depth; l++) {
-    S.all_off[l] = lists.size();
-    lists.insert(lists.end(), S.all_at[l].begin(), S.all_at[l].end());
-  }
-  S.all_off[S.depth + 1] = lists.size();
+  std::vector<uint32_t> zero(nnodes, 0);
   if (S.d_nodes.upload(S.nodes.data(), nnodes) != hipSuccess ||
       S.d_idx.upload(idx, ndof) != hipSuccess ||
       S.d_leaves.upload(S.leaves.data(), S.leaves.size()) != hipSuccess ||
-      S.d_sonptr.upload(S.sonptr.data(), S.sonptr.size()) != hipSuccess ||
       S.d_sons.upload(S.sons.data(), S.sons.size()) != hipSuccess ||
-      S.d_lists.upload(lists.data(), lists.size()) != hipSuccess) {
+      S.d_paths.upload(S.paths.data(), S.paths.size()) != hipSuccess ||
+      S.d_plen.upload(S.plen.data(), S.plen.size()) != hipSuccess ||
+      S.d_cnt.upload(zero.data(), nnodes) != hipSuccess) {
     set_error("h2matrix: device allocation failed");
     return H2HIP_ENOMEM;
   }
   return H2HIP_OK;
 }
 
-inline dim3 waves(uint32_t n) { return dim3((n + WPB - 1) / WPB); }
+template <int C, int K>
+void launch(const double *alpha, const h2hip_h2matrix *H, const double *x,
+            double *y, hipStream_t st, bool trans) {
+  const auto &in = H->side[trans ? 0 : 1];
+  const auto &out = H->side[trans ? 1 : 0];
+  FwdArgs f;
+  f.nodes = in.d_nodes.p;
+  f.leaves = in.d_leaves.p;
+  f.nleaves = (uint32_t)in.leaves.size();
+  f.idx = in.d_idx.p;
+  f.sons = in.d_sons.p;
+  f.VT = in.d_VT.p;
+  f.ET = in.d_ET.p;
+  f.eld = in.eld;
+  f.x = x;
+  f.xt = H->d_xt.p;
+  f.cnt = in.d_cnt.p;
+  f.yt = H->d_yt.p;
+  f.nyt = (C ? 2 : 1) * out.ncoef;
+  hipLaunchKernelGGL((fwd_up<C, K>), dim3(f.nleaves), dim3(64), 0, st, f);
+
+  CNArgs c;
+  c.far = H->d_far.p;
+  c.nfar = H->nfar;
+  c.near = H->d_near.p;
+  c.nnear = H->nnear;
+  c.rn = H->side[0].d_nodes.p;
+  c.cn = H->side[1].d_nodes.p;
+  c.ridx = H->side[0].d_idx.p;
+  c.cidx = H->side[1].d_idx.p;
+  c.S = H->d_S.p;
+  c.N = H->d_N.p;
+  c.x = x;
+  c.xt = H->d_xt.p;
+  c.yt = H->d_yt.p;
+  c.y = y;
+  c.are = alpha[0];
+  c.aim = alpha[1];
+  const uint32_t nw = H->nfar + H->nnear;
+  if (nw) {
+    if (trans)
+      hipLaunchKernelGGL((couple_near<C, true>), dim3((nw + WPB - 1) / WPB),
+                         dim3(NT), 0, st, c);
+    else
+      hipLaunchKernelGGL((couple_near<C, false>), dim3((nw + WPB - 1) / WPB),
+                         dim3(NT), 0, st, c);
+  }
+
+  DBArgs d;
+  d.nodes = out.d_nodes.p;
+  d.leaves = out.d_leaves.p;
+  d.nleaves = (uint32_t)out.leaves.size();
+  d.paths = out.d_paths.p;
+  d.plen = out.d_plen.p;
+  d.idx = out.d_idx.p;
+  d.V = out.d_V.p;
+  d.E = out.d_E.p;
+  d.eld = out.eld;
+  d.yt = H->d_yt.p;
+  d.y = y;
+  d.are = alpha[0];
+  d.aim = alpha[1];
+  hipLaunchKernelGGL((down_back<C, K>), dim3(d.nleaves), dim3(64), 0, st, d);
+}
 
 template <int C>
 int apply(const double *alpha, const h2hip_h2matrix *H, const double *x,
           double *y, hipStream_t st, bool trans) {
-  // forward on the side x lives in, backward on the other one
-  const auto &in = H->side[trans ? 0 : 1];
-  const auto &out = H->side[trans ? 1 : 0];
-  double *xt = H->d_xt.p, *yt = H->d_yt.p;
-  const uint64_t nyt = (C ? 2 : 1) * out.ncoef;
-  if (nyt)
-    hipLaunchKernelGGL(zero_kernel, dim3((nyt + 255) / 256), dim3(256), 0, st,
-                       yt, nyt);
-  // 1. forward, leaves
-  if (!in.leaves.empty())
-    hipLaunchKernelGGL(fwd_leaf<C>, waves(in.leaves.size()), dim3(NT), 0, st,
-                       in.d_nodes.p, in.d_leaves.p,
-                       (uint32_t)in.leaves.size(), in.d_idx.p, in.d_VT.p, x,
-                       xt);
-  // 2. forward, upwards
-  for (int l = (int)in.depth; l >= 0; l--) {
-    const uint32_t n = (uint32_t)in.inner_at[l].size();
-    if (!n) continue;
-    hipLaunchKernelGGL(up_level<C>, waves(n), dim3(NT), 0, st, in.d_nodes.p,
-                       in.d_lists.p + in.inner_off[l], n, in.d_sonptr.p,
-                       in.d_sons.p, in.d_ET.p, in.eld, xt);
-  }
-  // 3. couplings
-  if (H->nfar) {
-    if (trans)
-      hipLaunchKernelGGL((coupling<C, true>), waves(H->nfar), dim3(NT), 0, st,
-                         H->d_far.p, H->nfar, H->side[0].d_nodes.p,
-                         H->side[1].d_nodes.p, H->d_S.p, xt, yt);
-    else
-      hipLaunchKernelGGL((coupling<C, false>), waves(H->nfar), dim3(NT), 0,
-                         st, H->d_far.p, H->nfar, H->side[0].d_nodes.p,
-                         H->side[1].d_nodes.p, H->d_S.p, xt, yt);
-  }
-  // 4. backward, downwards
-  for (uint32_t l = 1; l <= out.depth; l++) {
-    const uint32_t n = (uint32_t)out.all_at[l].size();
-    if (!n) continue;
-    hipLaunchKernelGGL(down_level<C>, waves(n), dim3(NT), 0, st,
-                       out.d_nodes.p, out.d_lists.p + out.all_off[l], n,
-                       out.d_E.p, out.eld, yt);
-  }
-  // 5. backward, leaves
-  if (!out.leaves.empty())
-    hipLaunchKernelGGL(back_leaf<C>, waves(out.leaves.size()), dim3(NT), 0,
-                       st, out.d_nodes.p, out.d_leaves.p,
-                       (uint32_t)out.leaves.size(), out.d_idx.p, out.d_V.p,
-                       yt, alpha[0], alpha[1], y);
-  // 6. near field
-  if (H->nnear) {
-    if (trans)
-      hipLaunchKernelGGL((near_blocks<C, true>), waves(H->nnear), dim3(NT), 0,
-                         st, H->d_near.p, H->nnear, H->side[0].d_idx.p,
-                         H->side[1].d_idx.p, H->d_N.p, x, alpha[0], alpha[1],
-                         y);
-    else
-      hipLaunchKernelGGL((near_blocks<C, false>), waves(H->nnear), dim3(NT),
-                         0, st, H->d_near.p, H->nnear, H->side[0].d_idx.p,
-                         H->side[1].d_idx.p, H->d_N.p, x, alpha[0], alpha[1],
-                         y);
+  switch (H->urank) {
+    case 8: launch<C, 8>(alpha, H, x, y, st, trans); break;
+    case 27: launch<C, 27>(alpha, H, x, y, st, trans); break;
+    case 64: launch<C, 64>(alpha, H, x, y, st, trans); break;
+    case 125: launch<C, 125>(alpha, H, x, y, st, trans); break;
+    default: launch<C, 0>(alpha, H, x, y, st, trans); break;
   }
   H2HIP_CHECK(hipGetLastError());
   return H2HIP_OK;
@@ -498,6 +592,11 @@ int h2hip_h2matrix_create(h2hip_h2matrix **out, int is_complex,
     delete H;
     return rc;
   }
+  H->urank = rrank[0];
+  for (uint32_t i = 0; i < rnodes; i++)
+    if (rrank[i] != H->urank) H->urank = 0;
+  for (uint32_t i = 0; i < cnodes; i++)
+    if (crank[i] != H->urank) H->urank = 0;
   H->nnear = nnear;
   H->nfar = nfar;
   uint64_t no = 0, so = 0;
@@ -553,7 +652,7 @@ int h2hip_h2matrix_set_basis(h2hip_h2matrix *H, int side, const double *V,
   std::vector<double> e(S.eld * S.nnodes, 0.0), et(S.eld * S.nnodes, 0.0);
   for (uint32_t i = 0; i < S.nnodes; i++) {
     const NodeD &d = S.nodes[i];
-    if (S.sonptr[i + 1] == S.sonptr[i]) {
+    if (d.nsons == 0) {
       const double *src = V + voff[i];
       for (uint32_t c = 0; c < d.rank; c++)
         for (uint32_t r = 0; r < d.size; r++) {
@@ -562,7 +661,7 @@ int h2hip_h2matrix_set_basis(h2hip_h2matrix *H, int side, const double *V,
               src[r + (uint64_t)c * d.size];
         }
     }
-    if (d.father != 0xffffffffu) {
+    if (d.father != NONE) {
       const uint32_t kf = S.nodes[d.father].rank;
       const double *src = E + (uint64_t)i * eld;
       for (uint32_t c = 0; c < kf; c++)
@@ -620,7 +719,7 @@ int h2hip_h2matrix_bytes(const h2hip_h2matrix *H, uint64_t *out3) {
     b += 8ull * H->side[s].vlen;
     for (uint32_t i = 0; i < H->side[s].nnodes; i++) {
       const NodeD &d = H->side[s].nodes[i];
-      if (d.father != 0xffffffffu)
+      if (d.father != NONE)
         b += 8ull * d.rank * H->side[s].nodes[d.father].rank;
     }
   }
