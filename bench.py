@@ -12,7 +12,15 @@ One step assembles the near field of both operators: every near-field
 block, into a packed device buffer, with all inputs already resident in
 HBM.  value = near-field entries per second over the timed steps.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+The second half of the metric is measured on the same mesh after the
+timed steps: the H2 matrices of both operators (interpolation order 4) are
+built once and kept on the device, then Dirichlet solves run one after the
+other (boundary data -> right-hand side -> GMRES(30) -> solution on the
+host); the line reports solves/s, p50/p99 latency and the relative error
+of the solution against the reference library's tightly solved solution
+(tests/golden/dirichlet_sphere_s64.npz).
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--solves R]
 """
 import argparse
 import json
@@ -68,6 +76,121 @@ def cpu_baseline(mesh, tree, nb, entries):
                        "blocks; %.2f s" % (len(nb), entries, dt))}
 
 
+def bench_dense(h, stream, reps=20):
+    """The dense workload: Laplace SLP, piecewise constant basis, unit sphere
+    split 16 (2048 triangles), the full 2048 x 2048 matrix, q=2, q2=4.
+    Timed device-resident (one block into a device buffer) and through the
+    callback h2hip_nearfield into a host matrix (PCIe copy included)."""
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(16)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = h.new_slp_laplace_bem3d(gr, 2, 4, 0, 0)
+    T = mesh.triangles
+    idx = np.arange(T, dtype=np.uint32)
+    plan = h.NearfieldPlan(gr, 0, 0, idx, idx,
+                           np.array([[0, T, 0, T]], dtype=np.uint32))
+    out = h.DeviceArray(plan.entries)
+    for _ in range(3):
+        plan.execute(op, out.ptr, stream.handle)
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(reps):
+        plan.execute(op, out.ptr, stream.handle)
+    e1.record(stream)
+    stream.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    N = np.zeros((T, T), order="F")
+    op.nearfield(None, None, N=N)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.nearfield(None, None, N=N)
+    host_s = (time.perf_counter() - t0) / reps
+    return {"workload": "Laplace SLP cc, sphere 2048 triangles, dense "
+                        "2048 x 2048, q=2 q2=4, FP64",
+            "entries_per_s": T * T / (ms * 1e-3), "ms": ms,
+            "entries_per_s_host_matrix": T * T / host_s,
+            "host_matrix_ms": host_s * 1e3}
+
+
+def bench_dirichlet(h, mesh, args, stream):
+    """Dirichlet solves of the interior Laplace problem on the same mesh with
+    the operators resident on the device: each solve projects the boundary
+    data u = 1/|x - x0|, x0 = (1.2, 1.2, 1.2), forms b = (K + M/2) f and
+    runs GMRES(30) on V x = b; the latency of a solve runs from the boundary
+    data on the host to the solution on the host."""
+    from h2lib_amd.solve import LaplaceDirichlet
+    x0 = np.array([1.2, 1.2, 1.2])
+    t0 = time.perf_counter()
+    prob = LaplaceDirichlet(mesh, clf=args.clf, eta=args.eta, m=4)
+    h.hip().h2hip_stream_sync(None)
+    build_s = time.perf_counter() - t0
+
+    # matvec of the single layer operator, events on its stream
+    n = prob.n
+    d_x = h.DeviceArray.from_host(
+        np.random.default_rng(1).uniform(-1.0, 1.0, n))
+    d_y = h.DeviceArray(n)
+    d_y.zero()
+    for _ in range(3):
+        prob.V.addeval(1.0, d_x, d_y, stream)
+    reps = 50
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(reps):
+        prob.V.addeval(1.0, d_x, d_y, stream)
+    e1.record(stream)
+    stream.sync()
+    mv_ms = e0.elapsed_ms(e1) / reps
+    nbytes = prob.V.bytes()
+    mv_bytes = sum(nbytes.values()) + 3 * 8 * n
+    mv_gbs = mv_bytes / (mv_ms * 1e-3) / 1e9
+
+    ref = None
+    gpath = os.path.join(ROOT, "tests", "golden", "dirichlet_sphere_s64.npz")
+    if (args.split, args.clf, args.eta) == (64, 64, 2.0) and \
+            os.path.exists(gpath):
+        ref = np.load(gpath)
+
+    lat, steps = [], []
+    x = None
+    for r in range(args.solve_warmup + args.solves):
+        t = time.perf_counter()
+        d_f, d_b = prob.rhs(x0, stream)
+        d_sol, it, res = prob.solve(d_b, eps=args.eps, maxiter=1000, kmax=30,
+                                    stream=stream)
+        x = d_sol.download()
+        dt = time.perf_counter() - t
+        if r >= args.solve_warmup:
+            lat.append(dt)
+            steps.append(it)
+    lat = np.array(lat)
+    out = {
+        "solves_per_s": len(lat) / lat.sum(),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "solves": len(lat),
+        "gmres_eps": args.eps,
+        "gmres_steps": int(np.median(steps)),
+        "neumann_rel_err": prob.neumann_error(x, x0),
+        "operator_build_s": build_s,
+        "matvec": {
+            "kernels": "fwd_up + couple_near + down_back (Laplace SLP)",
+            "ms": mv_ms, "bytes": int(mv_bytes), "GB/s": mv_gbs,
+            "hbm_frac": mv_gbs / 8000.0},
+    }
+    if ref is not None:
+        xr = ref["x"]
+        out["rel_err"] = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+        out["rel_err_reference_eps8"] = float(
+            np.linalg.norm(ref["x_eps8"] - xr) / np.linalg.norm(xr))
+        out["rel_err_against"] = ("the reference library's solution of the "
+                                  "same discrete problem at residual "
+                                  "1e-12 |b|")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +200,11 @@ def main():
     ap.add_argument("--clf", type=int, default=64)
     ap.add_argument("--eta", type=float, default=2.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--solves", type=int, default=20,
+                    help="timed Dirichlet solves (0: skip them)")
+    ap.add_argument("--solve-warmup", type=int, default=2)
+    ap.add_argument("--eps", type=float, default=1e-10,
+                    help="GMRES stopping tolerance relative to |b|")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -156,6 +284,14 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    log("bench: dense 2048 x 2048 matrix")
+    dense = bench_dense(h, stream)
+
+    dirichlet = None
+    if args.solves > 0:
+        log("bench: Dirichlet solves")
+        dirichlet = bench_dirichlet(h, mesh, args, stream)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the reference on the host cores")
@@ -198,6 +334,8 @@ def main():
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "dirichlet": dirichlet,
+        "dense": dense,
     }
     print(json.dumps(line), flush=True)
 

@@ -73,6 +73,48 @@ def tree_arrays(t):
     return np.ascontiguousarray(tree)
 
 
+class Interpolation:
+    """Host arrays of an H2-matrix by interpolation (cluster bases of both
+    trees, transfer and coupling matrices), built by libh2bem.so without
+    touching the device: setup_h2matrix_aprx_inter_bem3d and the basis and
+    coupling fills of assemble_bem3d_h2matrix (Library/bem3d.c:9658-9686,
+    10313-10446).  V of node i: size x k column-major at voff[i]; E of node
+    i: k x k column-major at i k^2; S of far block b at b k^2."""
+
+    def __init__(self, mesh, rt, ct, btree, row_basis, col_basis, dlp, m, q):
+        _sigs()
+        ip = bem().h2b_build_h2interp(mesh._sp, rt._tp, ct._tp, btree._bp,
+                                      row_basis, col_basis, int(dlp), m, q)
+        if not ip:
+            raise RuntimeError("interpolation build failed")
+        bem().h2b_h2interp_coupling_laplace(ip, btree._bp)
+        self._ip = ip
+        I = ip.contents
+        self.m, self.k = I.m, I.k
+        kk = I.k * I.k
+        self.eld = kk
+        self.rtree, self.ctree = tree_arrays(rt), tree_arrays(ct)
+        self.rrank = np.full(rt.nnodes, I.k, dtype=np.uint32)
+        self.crank = np.full(ct.nnodes, I.k, dtype=np.uint32)
+        self.ridx, self.cidx = rt.idx, ct.idx
+        self.near = np.ascontiguousarray(btree.near, dtype=np.uint32)
+        self.far = np.ascontiguousarray(btree.far, dtype=np.uint32)
+        as_np = np.ctypeslib.as_array
+        self.rvoff = as_np(I.rvoff, (rt.nnodes,))
+        self.cvoff = as_np(I.cvoff, (ct.nnodes,))
+        self.rV = as_np(I.rV, (max(1, I.rvlen),))
+        self.cV = as_np(I.cV, (max(1, I.cvlen),))
+        self.rE = as_np(I.rE, (rt.nnodes * kk,))
+        self.cE = as_np(I.cE, (ct.nnodes * kk,))
+        self.S = as_np(I.S, (max(1, I.nfar * kk),))
+
+    def __del__(self):
+        ip = getattr(self, "_ip", None)
+        if ip:
+            bem().h2b_h2interp_free(ip)
+            self._ip = None
+
+
 class H2Matrix:
     """Device H2-matrix of one operator (mirror of h2matrix,
     Library/h2matrix.h:48-71)."""
@@ -85,39 +127,25 @@ class H2Matrix:
         self.op, self.rt, self.ct, self.btree = op, rt, ct, btree
         self.dtype = op.dtype
         self.m = m
-        ip = bem().h2b_build_h2interp(mesh._sp, rt._tp, ct._tp, btree._bp,
-                                      op.row_basis, op.col_basis,
-                                      int(op.layer == DLP), m, op.q_regular)
-        if not ip:
-            raise RuntimeError("interpolation build failed")
-        bem().h2b_h2interp_coupling_laplace(ip, btree._bp)
-        self._ip = ip
-        I = ip.contents
+        I = Interpolation(mesh, rt, ct, btree, op.row_basis, op.col_basis,
+                          op.layer == DLP, m, op.q_regular)
+        self.interp = I
         self.k = I.k
-        self._rtree = tree_arrays(rt)
-        self._ctree = tree_arrays(ct)
-        rrank = np.full(rt.nnodes, I.k, dtype=np.uint32)
-        crank = np.full(ct.nnodes, I.k, dtype=np.uint32)
-        self._keep = (rrank, crank)
-        near = np.ascontiguousarray(btree.near, dtype=np.uint32)
-        far = np.ascontiguousarray(btree.far, dtype=np.uint32)
-        self._blocks = (near, far)
         h = C.c_void_p()
         check(hip().h2hip_h2matrix_create(
-            C.byref(h), int(op.is_complex), rt.nnodes, _ptr(self._rtree),
-            _ptr(rrank), rt.n, _ptr(rt.idx), ct.nnodes, _ptr(self._ctree),
-            _ptr(crank), ct.n, _ptr(ct.idx), len(near), _ptr(near), len(far),
-            _ptr(far)), "h2hip_h2matrix_create")
+            C.byref(h), int(op.is_complex), rt.nnodes, _ptr(I.rtree),
+            _ptr(I.rrank), rt.n, _ptr(rt.idx), ct.nnodes, _ptr(I.ctree),
+            _ptr(I.crank), ct.n, _ptr(ct.idx), len(I.near), _ptr(I.near),
+            len(I.far), _ptr(I.far)), "h2hip_h2matrix_create")
         self._h = h
-        kk = I.k * I.k
         check(hip().h2hip_h2matrix_set_basis(
-            h, 0, C.cast(I.rV, C.c_void_p), C.cast(I.rvoff, C.c_void_p),
-            C.cast(I.rE, C.c_void_p), kk), "h2hip_h2matrix_set_basis")
+            h, 0, _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld),
+            "h2hip_h2matrix_set_basis")
         check(hip().h2hip_h2matrix_set_basis(
-            h, 1, C.cast(I.cV, C.c_void_p), C.cast(I.cvoff, C.c_void_p),
-            C.cast(I.cE, C.c_void_p), kk), "h2hip_h2matrix_set_basis")
-        check(hip().h2hip_h2matrix_set_couplings(
-            h, C.cast(I.S, C.c_void_p), kk), "h2hip_h2matrix_set_couplings")
+            h, 1, _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld),
+            "h2hip_h2matrix_set_basis")
+        check(hip().h2hip_h2matrix_set_couplings(h, _ptr(I.S), I.eld),
+              "h2hip_h2matrix_set_couplings")
         self.nearplan = None
         if build_nearfield:
             self.nearplan = NearfieldPlan(op.gr, op.row_basis, op.col_basis,
@@ -154,7 +182,3 @@ class H2Matrix:
         if h:
             hip().h2hip_h2matrix_destroy(h)
             self._h = None
-        ip = getattr(self, "_ip", None)
-        if ip:
-            bem().h2b_h2interp_free(ip)
-            self._ip = None

@@ -24,6 +24,7 @@
 
 #include <complex.h>
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -104,6 +105,34 @@ void orc_nearfield(const orc_bem *b, int far, const unsigned *ridx,
 double orc_nearfield_evals(const orc_bem *b, int far, const unsigned *ridx,
                            unsigned rows, const unsigned *cidx,
                            unsigned cols);
+
+/* ---- H2-matrix products -------------------------------------------------- */
+
+/* One cluster basis over a flattened cluster tree: tree[3 i + 0..2] =
+ * (offset into idx, size, father or 0xffffffff), rank[i]; leaf matrices V
+ * (size x rank, column-major) at voff[i]; transfer matrices E (rank_i x
+ * rank_father, column-major) at i * eld. */
+typedef struct {
+  uint32_t nnodes;
+  const uint32_t *tree, *rank, *idx;
+  const double *V;
+  const uint64_t *voff;
+  const double *E;
+  uint64_t eld;
+} orc_cbasis;
+
+/* y += alpha H x (trans = 0) or y += alpha H^* x (trans = 1).  far / near:
+ * (row node, column node) pairs; S: coupling block b (rank_r x rank_c) at
+ * b * sld; N: near-field blocks packed in order (rows x cols each).  Complex
+ * values (S, N, x, y, alpha) are interleaved (re, im).  Restates
+ * addeval_h2matrix_avector / addevaltrans_h2matrix_avector,
+ * Library/h2matrix.c:1210-1364, with forward / backward_clusterbasis_avector,
+ * Library/clusterbasis.c:523-573, 748-797.  Returns 0, or -1 on bad input. */
+int orc_h2_addeval(int is_complex, int trans, const double *alpha,
+                   const orc_cbasis *rb, const orc_cbasis *cb, uint32_t nfar,
+                   const uint32_t *far, const double *S, uint64_t sld,
+                   uint32_t nnear, const uint32_t *near, const double *N,
+                   const double *x, double *y);
 
 #ifdef __cplusplus
 }

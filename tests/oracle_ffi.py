@@ -95,6 +95,61 @@ class Oracle:
         lib.orc_sq_del(sq)
         return x, y, w, base.value
 
+    @classmethod
+    def h2_addeval(cls, interp, N, x, y, alpha=1.0, trans=False):
+        """y += alpha H x (H^* x) for the host arrays of an
+        h2lib_amd.h2matrix.Interpolation and packed near-field blocks N
+        (in the order of interp.near)."""
+        lib = cls.lib()
+        if not hasattr(lib, "_h2sig"):
+            lib.orc_h2_addeval.restype = C.c_int
+            lib.orc_h2_addeval.argtypes = [C.c_int, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_uint32,
+                                           C.c_void_p, C.c_void_p,
+                                           C.c_uint64, C.c_uint32, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
+            lib._h2sig = True
+        I = interp
+        cplx = np.iscomplexobj(N) or np.iscomplexobj(x) or \
+            np.iscomplexobj(alpha)
+        dt = np.complex128 if cplx else np.float64
+        N = np.ascontiguousarray(N, dtype=dt)
+        xv = np.ascontiguousarray(x, dtype=dt)
+        if y.dtype != dt or not y.flags.c_contiguous:
+            raise TypeError("y must be a contiguous %s array" % dt)
+        S = np.ascontiguousarray(I.S, dtype=np.float64)
+        if cplx:
+            S = S.astype(np.complex128)
+        a = np.array([np.real(alpha), np.imag(alpha)], dtype=np.float64)
+
+        class CB(C.Structure):
+            _fields_ = [("nnodes", C.c_uint32), ("tree", C.c_void_p),
+                        ("rank", C.c_void_p), ("idx", C.c_void_p),
+                        ("V", C.c_void_p), ("voff", C.c_void_p),
+                        ("E", C.c_void_p), ("eld", C.c_uint64)]
+
+        rb = CB(len(I.rtree), _ptr(I.rtree), _ptr(I.rrank), _ptr(I.ridx),
+                _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld)
+        cb = CB(len(I.ctree), _ptr(I.ctree), _ptr(I.crank), _ptr(I.cidx),
+                _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld)
+        rc = lib.orc_h2_addeval(int(cplx), int(trans), _ptr(a), C.byref(rb),
+                                C.byref(cb), len(I.far), _ptr(I.far),
+                                _ptr(S), I.eld, len(I.near), _ptr(I.near),
+                                _ptr(N), _ptr(xv), _ptr(y))
+        if rc != 0:
+            raise ValueError("orc_h2_addeval rejected its input")
+        return y
+
+
+def oracle_nearfield_packed(ob, blocks, ridx_all, cidx_all):
+    """Near-field blocks (roff, rows, coff, cols) of an OracleBem computed
+    one by one and packed column-major one after the other."""
+    out = []
+    for ro, nr, co, nc in np.asarray(blocks, dtype=np.int64):
+        N = ob.nearfield(ridx_all[ro:ro + nr], cidx_all[co:co + nc])
+        out.append(np.asarray(N).reshape(nr, nc).ravel(order="F"))
+    return np.concatenate(out) if out else np.zeros(0)
+
 
 class _BemBase:
     """Common interface of the restatement and the reference operators."""
