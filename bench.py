@@ -278,11 +278,17 @@ def main():
     k_avg = kms.mean(axis=0)
     achieved = sum(flops) / (k_avg.sum() * 1e-3) / 1e12
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_nearfield_ll_sphere64.json")
-    if os.path.exists(tpath):
+    # HBM bytes per launch pair from the committed counter summary
+    # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a plain copy); valid for
+    # the default workload only
+    traffic, counters = None, {}
+    tpath = os.path.join(ROOT, "profiles",
+                         "traffic_nearfield_ll_sphere32768.json")
+    if os.path.exists(tpath) and (args.split, args.clf, args.eta) == \
+            (64, 64, 2.0):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            counters = json.load(f)
+        traffic = counters.get("bytes_per_launch")
 
     log("bench: dense 2048 x 2048 matrix")
     dense = bench_dense(h, stream)
@@ -291,6 +297,8 @@ def main():
     if args.solves > 0:
         log("bench: Dirichlet solves")
         dirichlet = bench_dirichlet(h, mesh, args, stream)
+        if counters.get("h2_product_bytes"):
+            dirichlet["matvec"]["traffic"] = counters["h2_product_bytes"]
 
     cpu = None
     if not args.no_cpu_baseline:
