@@ -15,6 +15,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = {False: os.path.join(ROOT, "oracle", "_ref", "libh2ref_real.so"),
           True: os.path.join(ROOT, "oracle", "_ref", "libh2ref_complex.so")}
+# the reference plus the H2Lib-side binding of the GPU library
+REF_HIP_SO = {
+    False: os.path.join(ROOT, "oracle", "_ref", "libh2ref_hip_real.so"),
+    True: os.path.join(ROOT, "oracle", "_ref", "libh2ref_hip_complex.so")}
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -30,8 +34,8 @@ def _idx(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
-def have_reference(complex_=False):
-    return os.path.exists(REF_SO[complex_])
+def have_reference(complex_=False, hip=False):
+    return os.path.exists((REF_HIP_SO if hip else REF_SO)[complex_])
 
 
 class Oracle:
@@ -229,9 +233,11 @@ class Reference:
     _libs = {}
 
     @classmethod
-    def lib(cls, complex_=False):
-        if complex_ not in cls._libs:
-            lib = C.CDLL(REF_SO[complex_], mode=C.RTLD_LOCAL)
+    def lib(cls, complex_=False, hip=False):
+        key = (bool(complex_), bool(hip))
+        if key not in cls._libs:
+            lib = C.CDLL((REF_HIP_SO if hip else REF_SO)[complex_],
+                         mode=C.RTLD_LOCAL)
             lib.ref_mesh_size.argtypes = [C.c_int, C.c_uint,
                                           C.POINTER(C.c_uint),
                                           C.POINTER(C.c_uint)]
@@ -270,8 +276,19 @@ class Reference:
             lib.ref_h2_gmres.restype = C.c_uint
             lib.ref_h2_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_double, C.c_uint, C.c_uint]
-            cls._libs[complex_] = lib
-        return cls._libs[complex_]
+            if hip:
+                lib.ref_bem_use_hip.argtypes = [C.c_void_p]
+                lib.ref_h2_hip_pack.argtypes = [C.c_void_p]
+                lib.ref_h2_hip_addeval.argtypes = [C.c_void_p, C.c_int,
+                                                   C.c_void_p, C.c_void_p,
+                                                   C.c_void_p]
+                lib.ref_h2_hip_gmres.restype = C.c_uint
+                lib.ref_h2_hip_gmres.argtypes = [C.c_void_p, C.c_int,
+                                                 C.c_void_p, C.c_void_p,
+                                                 C.c_double, C.c_uint,
+                                                 C.c_uint]
+            cls._libs[key] = lib
+        return cls._libs[key]
 
     @classmethod
     def mesh(cls, kind, split):
@@ -303,9 +320,14 @@ class Reference:
 
 
 class ReferenceBem(_BemBase):
-    def __init__(self, kernel, op, basis_r, basis_c, x, t, **kw):
-        lib = Reference.lib(kernel == 1)
+    """A reference operator.  hip=True loads the reference build that also
+    holds the H2Lib-side binding of the GPU library; use_hip() then installs
+    the GPU callbacks into the reference's own bem3d object."""
+
+    def __init__(self, kernel, op, basis_r, basis_c, x, t, hip=False, **kw):
+        lib = Reference.lib(kernel == 1, hip)
         self._lib = lib
+        self.hip = hip
         self._new = lib.ref_bem_new
         self._nearfield = lib.ref_nearfield
         super().__init__(kernel, op, basis_r, basis_c, x, t, **kw)
@@ -314,6 +336,13 @@ class ReferenceBem(_BemBase):
         if getattr(self, "h", None):
             self._lib.ref_bem_del(self.h)
             self.h = None
+
+    def use_hip(self):
+        """bem->nearfield / nearfield_far := the GPU callbacks
+        (setup_hip_bem3d)."""
+        if not self.hip:
+            raise RuntimeError("load the reference with hip=True")
+        self._lib.ref_bem_use_hip(self.h)
 
     def nearfield_blocks(self, ridx, cidx, blocks, threads=1):
         """All blocks (roff, rows, coff, cols) packed as the device plan
@@ -373,6 +402,26 @@ class ReferenceH2:
         self._lib.ref_h2_addeval(self.h, int(trans), _ptr(a), _ptr(xx),
                                  _ptr(yy))
         return yy
+
+    def hip_addeval(self, x, y, alpha=1.0, trans=False):
+        """The same product through new_hip_h2matrix +
+        addeval_hip_h2matrix_avector (GPU)."""
+        a = np.array([np.real(alpha), np.imag(alpha)])
+        xx = np.ascontiguousarray(x, dtype=self.bem.dtype)
+        yy = np.ascontiguousarray(y, dtype=self.bem.dtype).copy()
+        self._lib.ref_h2_hip_addeval(self.h, int(trans), _ptr(a), _ptr(xx),
+                                     _ptr(yy))
+        return yy
+
+    def hip_gmres(self, b, x0, eps, maxiter, kmax, device):
+        """GMRES through the binding: device=False runs the reference's
+        solve_gmres_avector with the GPU product as callback, device=True the
+        device-resident solver."""
+        bb = np.ascontiguousarray(b, dtype=self.bem.dtype)
+        xx = np.ascontiguousarray(x0, dtype=self.bem.dtype).copy()
+        it = self._lib.ref_h2_hip_gmres(self.h, int(bool(device)), _ptr(bb),
+                                        _ptr(xx), eps, maxiter, kmax)
+        return xx, it
 
     def gmres(self, b, x0, eps, maxiter, kmax):
         bb = np.ascontiguousarray(b, dtype=self.bem.dtype)

@@ -11,7 +11,7 @@ in summation order, fused multiply-adds and the Gauss nodes' last bit).
 import numpy as np
 import pytest
 
-from conftest import golden, rel_block_err
+from conftest import golden, rel_block_err, subsets
 from oracle_ffi import OracleBem
 
 pytestmark = pytest.mark.gpu
@@ -47,19 +47,6 @@ def oracle_op(mesh, name, q=2, q2=4):
     kern, op, br, bc, alpha, k = CASES[name]
     return OracleBem(kern, op, br, bc, mesh["x"], mesh["t"], q=q, q2=q2,
                      alpha=alpha, k=k)
-
-
-def subsets(rng, nr, nc, rows, cols, same_dofs):
-    cidx = rng.permutation(nc)[:cols]
-    if not same_dofs:
-        return rng.permutation(nr)[:rows], cidx
-    cset = set(int(v) for v in cidx)
-    ridx = [int(v) for v in rng.permutation(nr) if int(v) not in cset]
-    ridx = ridx[:rows]
-    positions = list(range(0, min(rows, cols), 3))
-    for pos, j in zip(positions, rng.permutation(cols)[:len(positions)]):
-        ridx[pos] = int(cidx[j])
-    return np.array(ridx), cidx
 
 
 @pytest.mark.parametrize("name", sorted(CASES))

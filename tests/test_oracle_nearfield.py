@@ -13,7 +13,7 @@
 import numpy as np
 import pytest
 
-from conftest import golden, rel_block_err
+from conftest import golden, rel_block_err, subsets
 from oracle_ffi import OracleBem, ReferenceBem, have_reference
 
 CASES = {
@@ -29,25 +29,6 @@ CASES = {
     "helm_slp_ll": (1, 0, 1, 1, 0.0, 2.0),
     "helm_dlp_ll": (1, 1, 1, 1, 0.5, 2.0),
 }
-
-
-def subsets(rng, nr, nc, rows, cols, same_dofs):
-    """Random row and column index sets.  When rows and columns index the
-    same DOFs, a few shared DOFs are placed at row positions below cols: the
-    reference adds the constant-basis identity term at (t, t), which for
-    t >= cols lies outside the window (undefined behaviour there)."""
-    cidx = rng.permutation(nc)[:cols]
-    if not same_dofs:
-        return rng.permutation(nr)[:rows], cidx
-    cset = set(int(v) for v in cidx)
-    ridx = [int(v) for v in rng.permutation(nr) if int(v) not in cset]
-    ridx = ridx[:rows]
-    positions = list(range(0, min(rows, cols), 3))
-    picks = rng.permutation(cols)[:len(positions)]
-    for pos, j in zip(positions, picks):
-        ridx[pos] = int(cidx[j])
-    assert len(set(ridx)) == len(ridx)
-    return np.array(ridx), cidx
 
 
 def make(cls, mesh, name):
