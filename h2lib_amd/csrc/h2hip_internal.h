@@ -54,14 +54,6 @@ template <class T> struct DevBuf {
 
 // ---- quadrature rules (host) -------------------------------------------------
 
-// One rule on a pair of reference triangles.  Point k of the first triangle
-// is A (1 - x0[k]) + B (x0[k] - x1[k]) + C x1[k], of the second triangle
-// A (1 - y0[k]) + B (y0[k] - y1[k]) + C y1[k]; w[k] is the weight.
-struct PairRule {
-  std::vector<double> x0, x1, y0, y1, w;
-  size_t size() const { return w.size(); }
-};
-
 // Single-triangle rule of order q: points (x0, x1), weights w.
 struct TriRule {
   std::vector<double> x0, x1, w;
@@ -70,9 +62,6 @@ struct TriRule {
 void gauss_legendre01(unsigned m, std::vector<double> &x,
                       std::vector<double> &w);
 TriRule make_triangle_rule(unsigned q);
-PairRule make_vertex_rule(unsigned q2);
-PairRule make_edge_rule(unsigned q2);
-PairRule make_identical_rule(unsigned q2);
 
 // ---- device-side records -------------------------------------------------------
 
@@ -131,9 +120,8 @@ struct KArgs {
   const double *g;       // Gram determinants
   const double *regpts;  // regular-rule points, 3 q^2 per triangle
   const double *regw;    // regular weights: w | phi0 w | phi1 w | phi2 w
-  const double *sing;    // singular rules (structure of arrays per class)
-  uint32_t sing_off[4];
-  uint32_t sing_n[4];
+  const double *gauss;   // Gauss-Legendre nodes and weights on [0, 1], q2 each
+  uint32_t q2;           // order of the singular rules
   double kc;             // 1 / (4 pi)
   double kr, ki;         // e^{i k r} with k = kr - i ki
   double are, aim;       // identity coefficient alpha
@@ -143,13 +131,14 @@ struct KArgs {
 };
 
 // Launches the regular tile kernel over `ntiles` tiles and then (unless
-// far) the singular kernel over `nsp` pairs, on `stream`.  Returns
-// H2HIP_EINVAL if the order q has no kernel.
+// far) the singular kernel once per rule class over the pairs sp (sorted
+// identical, edge, vertex; ncls[c] pairs of class c), on `stream`.
+// Returns H2HIP_EINVAL if the order q has no kernel.
 int launch_laplace(int layer, int rb, int cb, uint32_t q, const KArgs &a,
-                   uint32_t ntiles, const SPair *sp, uint32_t nsp,
+                   uint32_t ntiles, const SPair *sp, const uint32_t *ncls,
                    hipStream_t stream);
 int launch_helmholtz(int layer, int rb, int cb, uint32_t q, const KArgs &a,
-                     uint32_t ntiles, const SPair *sp, uint32_t nsp,
+                     uint32_t ntiles, const SPair *sp, const uint32_t *ncls,
                      hipStream_t stream);
 
 }  // namespace nf
@@ -172,11 +161,9 @@ struct h2hip_op {
   double alpha[2] = {0, 0}, k[2] = {0, 0};
   double mass[9] = {0};
   int precision = 0;
-  // singular rules, structure of arrays per class 1..3:
-  // [x0 | x1 | y0 | y1 | w] with n points each, starting at sing_off[cls]
-  h2hip::DevBuf<double> d_sing;
-  uint32_t sing_off[4] = {0, 0, 0, 0};
-  uint32_t sing_n[4] = {0, 0, 0, 0};
+  // Gauss-Legendre nodes and weights of order q2 on [0, 1]: [g | w]; the
+  // singular kernels form the Sauter-Schwab points from them
+  h2hip::DevBuf<double> d_gauss;
   // regular rule: per-triangle points (3 * q * q doubles per triangle)
   h2hip::DevBuf<double> d_regpts;
   h2hip::DevBuf<float> d_regpts_f;

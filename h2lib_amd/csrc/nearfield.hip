@@ -305,11 +305,8 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.g = s->d_g.p;
   a.regpts = op->d_regpts.p;
   a.regw = op->d_regw.p;
-  a.sing = op->d_sing.p;
-  for (int i = 0; i < 4; i++) {
-    a.sing_off[i] = op->sing_off[i];
-    a.sing_n[i] = op->sing_n[i];
-  }
+  a.gauss = op->d_gauss.p;
+  a.q2 = op->q2;
   a.kc = KERNEL_CONST;
   a.kr = op->k[0];
   a.ki = -op->k[1];
@@ -319,13 +316,14 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.far = far;
   a.out = (double *)d_out;
   const uint32_t ntiles = (uint32_t)P->tiles.size();
-  const uint32_t nsp = (uint32_t)P->spairs.size();
+  uint32_t ncls[4];
+  for (int c = 0; c < 4; c++) ncls[c] = (uint32_t)P->pairs[c];
   const int rc =
       op->kernel == H2HIP_HELMHOLTZ
           ? nf::launch_helmholtz(op->layer, op->rb, op->cb, op->q, a, ntiles,
-                                 P->d_spairs.p, nsp, stream)
+                                 P->d_spairs.p, ncls, stream)
           : nf::launch_laplace(op->layer, op->rb, op->cb, op->q, a, ntiles,
-                               P->d_spairs.p, nsp, stream);
+                               P->d_spairs.p, ncls, stream);
   if (rc != H2HIP_OK) {
     set_error("unsupported regular quadrature order (1..4 supported)");
     return rc;
@@ -475,22 +473,11 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
     op->mass[0] = op->mass[1] = op->mass[2] = 1.0 / 6.0;
   }
 
-  // singular rules
-  PairRule rules[4];
-  rules[CLS_VERTEX] = make_vertex_rule(q_singular);
-  rules[CLS_EDGE] = make_edge_rule(q_singular);
-  rules[CLS_IDENTICAL] = make_identical_rule(q_singular);
-  std::vector<double> sing;
-  for (int c = 1; c <= 3; c++) {
-    const PairRule &r = rules[c];
-    op->sing_off[c] = (uint32_t)sing.size();
-    op->sing_n[c] = (uint32_t)r.size();
-    sing.insert(sing.end(), r.x0.begin(), r.x0.end());
-    sing.insert(sing.end(), r.x1.begin(), r.x1.end());
-    sing.insert(sing.end(), r.y0.begin(), r.y0.end());
-    sing.insert(sing.end(), r.y1.begin(), r.y1.end());
-    sing.insert(sing.end(), r.w.begin(), r.w.end());
-  }
+  // Gauss nodes of the singular rules (the kernels form the Sauter-Schwab
+  // points from them)
+  std::vector<double> gq, gqw;
+  gauss_legendre01(q_singular, gq, gqw);
+  gq.insert(gq.end(), gqw.begin(), gqw.end());
   // regular rule: single-triangle points and (basis-weighted) weights
   TriRule tr = make_triangle_rule(q_regular);
   const int qq = (int)tr.w.size();
@@ -504,7 +491,7 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
   }
   DevBuf<double> d_x0, d_x1;
   int rc = H2HIP_OK;
-  if (op->d_sing.upload(sing.data(), sing.size()) != hipSuccess ||
+  if (op->d_gauss.upload(gq.data(), gq.size()) != hipSuccess ||
       op->d_regw.upload(op->regw.data(), op->regw.size()) != hipSuccess ||
       d_x0.upload(tr.x0.data(), qq) != hipSuccess ||
       d_x1.upload(tr.x1.data(), qq) != hipSuccess ||
@@ -628,9 +615,9 @@ int h2hip_nfplan_work(const h2hip_nfplan *p, const h2hip_op *op, int far,
     pairs[0] += pairs[1] + pairs[2] + pairs[3];
     pairs[1] = pairs[2] = pairs[3] = 0.0;
   }
-  const double nq[4] = {(double)op->q * op->q * op->q * op->q,
-                        (double)op->sing_n[1], (double)op->sing_n[2],
-                        (double)op->sing_n[3]};
+  const double q4 = (double)op->q * op->q * op->q * op->q;
+  const double s4 = (double)op->q2 * op->q2 * op->q2 * op->q2;
+  const double nq[4] = {q4, 2.0 * s4, 6.0 * s4, 6.0 * s4};
   for (int c = 0; c < 4; c++) {
     out[c] = pairs[c] * nq[c];
     out[4 + c] = pairs[c];

@@ -12,13 +12,14 @@
 //      is stored once, coalesced; the constant-basis case stores entries
 //      directly.  Pairs with common vertices are skipped (cc: stored as 0).
 //
-//  nf_singular_kernel one wave64 per pair with common vertices, taken from
-//      a list built at planning time (identical and edge pairs first, then
-//      vertex pairs).  The lanes stride over the 2 q2^4 or 6 q2^4
-//      Sauter-Schwab points, the sums are reduced across the wavefront and
-//      up to nine lanes add the element matrix into the output with FP64
-//      atomics.  x - y is formed from differences of the permuted vertices,
-//      so the shared vertices cancel exactly.
+//  nf_singular_kernel one row of 16 lanes per pair with common vertices,
+//      one launch per rule class (identical, edge, vertex) over the pair
+//      list built at planning time.  The 2 q2^4 or 6 q2^4 Sauter-Schwab
+//      points are formed on the fly from the q2 Gauss nodes (no rule tables
+//      are read), the sums are reduced across the row and up to nine lanes
+//      add the element matrix into the output with FP64 atomics.  x - y is
+//      formed from differences of the permuted vertices, so the shared
+//      vertices cancel exactly.
 #pragma once
 
 #include "h2hip_internal.h"
@@ -231,7 +232,84 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
   }
 }
 
-template <int KER, int BR, int BC>
+// Sauter-Schwab points of one tensor tuple (g_i, g_j, g_k, g_l) with
+// weights (w_i, w_j, w_k, w_l): the 2 (vertex) or 6 (edge, identical)
+// subregion points (tx, sx; ty, sy) and their common weight, exactly as the
+// tables of the reference are built (Library/singquad2d.c:80-281).
+template <int CLS>
+struct SSTuple;
+
+template <>
+struct SSTuple<CLS_VERTEX> {
+  static constexpr int N = 2;
+  template <class F>
+  __device__ __forceinline__ static void run(double gi, double gj, double gk,
+                                             double gl, double wi, double wj,
+                                             double wk, double wl, F &&f) {
+    const double e1 = gi, e2 = gj, e3 = gk, xi = gl;
+    const double w = wi * wj * wk * wl * xi * xi * xi * e1;
+    const double a = e2 * xi, b = e1 * xi, c = e1 * e3 * xi;
+    f(w, xi, a, b, c);
+    f(w, b, c, xi, a);
+  }
+};
+
+template <>
+struct SSTuple<CLS_EDGE> {
+  static constexpr int N = 6;
+  template <class F>
+  __device__ __forceinline__ static void run(double gi, double gj, double gk,
+                                             double gl, double wi, double wj,
+                                             double wk, double wl, F &&f) {
+    const double e1 = gi, e2 = gj, x1 = gk;
+    const double x2 = gl * (1.0 - x1);
+    const double w = wi * wj * wk * (wl * (1.0 - x1)) * x1 * x1 * e1;
+    const double s = x1 + x2;
+    const double p = x1 * e1, pq = x1 * e1 * e2;
+    f(w, s, pq, x1 * (1.0 - e1) + x2, x1 * (1.0 - e1));
+    f(w, s, p, x1 * (1.0 - e1 * e2) + x2, x1 * (1.0 - e1 * e2));
+    f(w, s, x1, p + x2, pq);
+    f(w, x1 * (1.0 - e1 + e1 * e2) + x2, pq, s, x1);
+    f(w, pq + x2, pq, s, p);
+    f(w, p + x2, p, s, pq);
+  }
+};
+
+template <>
+struct SSTuple<CLS_IDENTICAL> {
+  static constexpr int N = 6;
+  template <class F>
+  __device__ __forceinline__ static void run(double gi, double gj, double gk,
+                                             double gl, double wi, double wj,
+                                             double wk, double wl, F &&f) {
+    const double eta = gi, x1 = gj;
+    const double x2 = gk * (1.0 - x1);
+    const double x3 = gl * (1.0 - x1 - x2);
+    const double w = wi * wj * (wk * (1.0 - x1)) * (wl * (1.0 - x1 - x2)) * x1;
+    const double a = x1 + x2 + x3;
+    const double b = x1 * (1.0 - eta);
+    f(w, a, x1 + x2, b + x2 + x3, x2);
+    f(w, b + x2 + x3, b + x2, a, x2);
+    f(w, a, x1 * eta + x2, x2 + x3, x2);
+    f(w, b + x2 + x3, x2, a, x1 + x2);
+    f(w, a, x2, b + x2 + x3, b + x2);
+    f(w, x2 + x3, x2, a, x1 * eta + x2);
+  }
+};
+
+// sum over the 16 lanes of a row
+__device__ __forceinline__ double row_sum(double v) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+  return v;
+}
+
+// One pair with common vertices per 16 lanes (16 pairs per workgroup), all
+// pairs of one rule class per launch.  Lane s of a row takes the tensor
+// index pairs (k, l) = (s / q2, s % q2), (s + 16) / q2, ... and loops over
+// (i, j); the quadrature points are formed on the fly from the Gauss nodes
+// (a.gauss = [g | w], q2 each), so the kernel reads no rule tables.
+template <int KER, int BR, int BC, int CLS>
 __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
     KArgs a, const SPair *__restrict__ list, uint32_t npairs) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
@@ -240,22 +318,19 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   constexpr int NR = BR ? 3 : 1;
   constexpr int NC = BC ? 3 : 1;
 
-  const uint32_t w = blockIdx.x * NWAVES + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= npairs) return;  // uniform per wavefront
+  const uint32_t w = blockIdx.x * (NTHREADS / 16) + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & 15;
+  if (w >= npairs) return;  // uniform per row of 16 lanes
 
   const SPair P = list[w];
   const Tile T = a.tiles[P.tile];
   const Supp sr = a.supp[T.rsup + P.r], sc = a.supp[T.csup + P.c];
-  const int cls = P.cls;
   int tp[3], sp[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     tp[i] = (P.perm >> (2 * i)) & 3;
     sp[i] = (P.perm >> (6 + 2 * i)) & 3;
   }
-  const uint32_t nq = a.sing_n[cls];
-  const double *__restrict__ tab = a.sing + a.sing_off[cls];
   double D0[3], E1[3], E2[3], F1[3], F2[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -280,10 +355,8 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   for (int l = 0; l < NR; l++)
 #pragma unroll
     for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
-  for (uint32_t q = lane; q < nq; q += 64) {
-    const double tx = tab[q], sx = tab[nq + q];
-    const double ty = tab[2 * nq + q], sy = tab[3 * nq + q];
-    const double wq = tab[4 * nq + q];
+
+  auto point = [&](double wq, double tx, double sx, double ty, double sy) {
     double d[3];
 #pragma unroll
     for (int k = 0; k < 3; k++)
@@ -316,18 +389,31 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
         if constexpr (CPLX) Sim[l][k] = fma(ai, psi[k], Sim[l][k]);
       }
     }
+  };
+
+  const uint32_t q2 = a.q2;
+  const double *__restrict__ gn = a.gauss;
+  const double *__restrict__ gw = a.gauss + q2;
+  for (uint32_t kl = sub; kl < q2 * q2; kl += 16) {
+    const uint32_t k = kl / q2, l = kl - k * q2;
+    const double gk = gn[k], gl = gn[l], wk = gw[k], wl = gw[l];
+    for (uint32_t i = 0; i < q2; i++) {
+      const double gi = gn[i], wi = gw[i];
+      for (uint32_t j = 0; j < q2; j++)
+        SSTuple<CLS>::run(gi, gn[j], gk, gl, wi, gw[j], wk, wl, point);
+    }
   }
 #pragma unroll
   for (int l = 0; l < NR; l++)
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-      Sre[l][k] = wave_sum(Sre[l][k]);
-      if constexpr (CPLX) Sim[l][k] = wave_sum(Sim[l][k]);
+      Sre[l][k] = row_sum(Sre[l][k]);
+      if constexpr (CPLX) Sim[l][k] = row_sum(Sim[l][k]);
     }
-  if (lane >= NR * NC) return;
+  if (sub >= NR * NC) return;
   // lane e adds the entry of permuted local functions (l, k) = (e % NR,
   // e / NR)
-  const int l = lane % NR, k = lane / NR;
+  const int l = sub % NR, k = sub / NR;
   double vre = 0.0, vim = 0.0;
 #pragma unroll
   for (int ll = 0; ll < NR; ll++)
@@ -344,7 +430,7 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   const double f2 = a.kc * gt * a.g[sc.tri];
   vre *= f2;
   vim *= f2;
-  if (cls == 3) {
+  if constexpr (CLS == CLS_IDENTICAL) {
     // identity term of the double layer on identical triangles (tp, sp are
     // the identity here): cc at (t, t) of the block
     // (Library/bem3d.c:1234-1241), otherwise alpha g mass[l, k]
@@ -375,9 +461,12 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   }
 }
 
+// Launches the regular tile kernel over `ntiles` tiles and then (unless
+// far) one singular launch per rule class; sp holds the pairs sorted
+// identical, edge, vertex with ncls[c] pairs of class c.
 template <int KER, int BR, int BC>
 int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
-                 const SPair *sp, uint32_t nsp, hipStream_t stream) {
+                 const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
   void (*reg)(KArgs) = nullptr;
   switch (q) {
     case 1: reg = nf_regular_kernel<KER, BR, BC, 1>; break;
@@ -388,21 +477,33 @@ int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
   }
   if (ntiles)
     hipLaunchKernelGGL(reg, dim3(ntiles), dim3(NTHREADS), 0, stream, a);
-  if (!a.far && nsp)
-    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC>),
-                       dim3((nsp + NWAVES - 1) / NWAVES), dim3(NTHREADS), 0,
-                       stream, a, sp, nsp);
+  if (a.far) return H2HIP_OK;
+  constexpr uint32_t PPB = NTHREADS / 16;  // pairs per workgroup
+  const uint32_t n3 = ncls[CLS_IDENTICAL], n2 = ncls[CLS_EDGE],
+                 n1 = ncls[CLS_VERTEX];
+  if (n3)
+    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL>),
+                       dim3((n3 + PPB - 1) / PPB), dim3(NTHREADS), 0, stream,
+                       a, sp, n3);
+  if (n2)
+    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC, CLS_EDGE>),
+                       dim3((n2 + PPB - 1) / PPB), dim3(NTHREADS), 0, stream,
+                       a, sp + n3, n2);
+  if (n1)
+    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC, CLS_VERTEX>),
+                       dim3((n1 + PPB - 1) / PPB), dim3(NTHREADS), 0, stream,
+                       a, sp + n3 + n2, n1);
   return H2HIP_OK;
 }
 
 template <int KER>
 int launch_family(int rb, int cb, uint32_t q, const KArgs &a,
-                  uint32_t ntiles, const SPair *sp, uint32_t nsp,
+                  uint32_t ntiles, const SPair *sp, const uint32_t *ncls,
                   hipStream_t stream) {
-  if (!rb && !cb) return launch_bases<KER, 0, 0>(q, a, ntiles, sp, nsp, stream);
-  if (rb && cb) return launch_bases<KER, 1, 1>(q, a, ntiles, sp, nsp, stream);
-  if (cb) return launch_bases<KER, 0, 1>(q, a, ntiles, sp, nsp, stream);
-  return launch_bases<KER, 1, 0>(q, a, ntiles, sp, nsp, stream);
+  if (!rb && !cb) return launch_bases<KER, 0, 0>(q, a, ntiles, sp, ncls, stream);
+  if (rb && cb) return launch_bases<KER, 1, 1>(q, a, ntiles, sp, ncls, stream);
+  if (cb) return launch_bases<KER, 0, 1>(q, a, ntiles, sp, ncls, stream);
+  return launch_bases<KER, 1, 0>(q, a, ntiles, sp, ncls, stream);
 }
 
 }  // namespace nf
