@@ -118,8 +118,17 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
   }
 
   const uint32_t npairs = nR * nC;
+  // pair p = r + nR c, advanced by NTHREADS without a division per pair
+  const uint32_t dc = nR ? NTHREADS / nR : 0, dr = NTHREADS - dc * nR;
+  uint32_t rr = nR ? tid % nR : 0, cc = nR ? tid / nR : 0;
   for (uint32_t p = tid; p < npairs; p += NTHREADS) {
-    const uint32_t r = p % nR, c = p / nR;
+    const uint32_t r = rr, c = cc;
+    rr += dr;
+    cc += dc;
+    if (rr >= nR) {
+      rr -= nR;
+      cc++;
+    }
     const Supp sr = RS[r], sc = CS[c];
     int shared = 0;
     if (!a.far) {
@@ -309,7 +318,7 @@ __device__ __forceinline__ double row_sum(double v) {
 // index pairs (k, l) = (s / q2, s % q2), (s + 16) / q2, ... and loops over
 // (i, j); the quadrature points are formed on the fly from the Gauss nodes
 // (a.gauss = [g | w], q2 each), so the kernel reads no rule tables.
-template <int KER, int BR, int BC, int CLS>
+template <int KER, int BR, int BC, int CLS, int Q2>
 __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
     KArgs a, const SPair *__restrict__ list, uint32_t npairs) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
@@ -356,6 +365,9 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
 #pragma unroll
     for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
 
+  // Moments M[a][b] = sum w v m_a(x) m_b(y) with m = (1, tx, sx) resp.
+  // (1, ty, sy); the linear basis functions are phi = (1 - tx, tx - sx, sx)
+  // = A m, so the element matrix A M A^T is formed once after the sum.
   auto point = [&](double wq, double tx, double sx, double ty, double sy) {
     double d[3];
 #pragma unroll
@@ -364,43 +376,83 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
                  fma(-F1[k], ty, fma(E2[k], sx, fma(E1[k], tx, D0[k]))));
     double vre, vim;
     kval<KER>(d[0], d[1], d[2], nx, ny, nz, a.kr, a.ki, vre, vim);
-    const double wre = wq * vre, wim = wq * vim;
-    double phi[NR], psi[NC];
+    double ure[NR], uim[NR];
+    ure[0] = wq * vre;
+    uim[0] = wq * vim;
     if constexpr (BR) {
-      phi[0] = 1.0 - tx;
-      phi[1] = tx - sx;
-      phi[2] = sx;
-    } else {
-      phi[0] = 1.0;
-    }
-    if constexpr (BC) {
-      psi[0] = 1.0 - ty;
-      psi[1] = ty - sy;
-      psi[2] = sy;
-    } else {
-      psi[0] = 1.0;
+      ure[1] = ure[0] * tx;
+      ure[2] = ure[0] * sx;
+      uim[1] = uim[0] * tx;
+      uim[2] = uim[0] * sx;
     }
 #pragma unroll
     for (int l = 0; l < NR; l++) {
-      const double ar = wre * phi[l], ai = wim * phi[l];
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        Sre[l][k] = fma(ar, psi[k], Sre[l][k]);
-        if constexpr (CPLX) Sim[l][k] = fma(ai, psi[k], Sim[l][k]);
+      Sre[l][0] += ure[l];
+      if constexpr (CPLX) Sim[l][0] += uim[l];
+      if constexpr (BC) {
+        Sre[l][1] = fma(ure[l], ty, Sre[l][1]);
+        Sre[l][2] = fma(ure[l], sy, Sre[l][2]);
+        if constexpr (CPLX) {
+          Sim[l][1] = fma(uim[l], ty, Sim[l][1]);
+          Sim[l][2] = fma(uim[l], sy, Sim[l][2]);
+        }
       }
     }
   };
 
-  const uint32_t q2 = a.q2;
-  const double *__restrict__ gn = a.gauss;
-  const double *__restrict__ gw = a.gauss + q2;
-  for (uint32_t kl = sub; kl < q2 * q2; kl += 16) {
-    const uint32_t k = kl / q2, l = kl - k * q2;
-    const double gk = gn[k], gl = gn[l], wk = gw[k], wl = gw[l];
-    for (uint32_t i = 0; i < q2; i++) {
-      const double gi = gn[i], wi = gw[i];
-      for (uint32_t j = 0; j < q2; j++)
-        SSTuple<CLS>::run(gi, gn[j], gk, gl, wi, gw[j], wk, wl, point);
+  if constexpr (Q2 > 0) {
+    // order known at compile time (Q2^2 <= 16): one (k, l) per lane, the
+    // nodes in registers, the (i, j) loops unrolled
+    static_assert(Q2 * Q2 <= 16, "one tensor index pair per lane");
+    double gn[Q2], gw[Q2];
+#pragma unroll
+    for (int i = 0; i < Q2; i++) {
+      gn[i] = a.gauss[i];
+      gw[i] = a.gauss[Q2 + i];
+    }
+    if (sub < Q2 * Q2) {
+      const int k = sub / Q2, l = sub % Q2;
+      const double gk = a.gauss[k], gl = a.gauss[l];
+      const double wk = a.gauss[Q2 + k], wl = a.gauss[Q2 + l];
+      if constexpr (CLS == CLS_VERTEX) {
+#pragma unroll
+        for (int i = 0; i < Q2; i++)
+#pragma unroll
+          for (int j = 0; j < Q2; j++)
+            SSTuple<CLS>::run(gn[i], gn[j], gk, gl, gw[i], gw[j], wk, wl,
+                              point);
+      } else if constexpr (CLS == CLS_EDGE) {
+        // six points per tuple: unrolling i as well costs too many registers
+#pragma unroll 1
+        for (int i = 0; i < Q2; i++) {
+          const double gi = a.gauss[i], wi = a.gauss[Q2 + i];
+#pragma unroll
+          for (int j = 0; j < Q2; j++)
+            SSTuple<CLS>::run(gi, gn[j], gk, gl, wi, gw[j], wk, wl, point);
+        }
+      } else {
+#pragma unroll 1
+        for (int i = 0; i < Q2; i++) {
+          const double gi = a.gauss[i], wi = a.gauss[Q2 + i];
+#pragma unroll 1
+          for (int j = 0; j < Q2; j++)
+            SSTuple<CLS>::run(gi, a.gauss[j], gk, gl, wi, a.gauss[Q2 + j], wk,
+                              wl, point);
+        }
+      }
+    }
+  } else {
+    const uint32_t q2 = a.q2;
+    const double *__restrict__ gn = a.gauss;
+    const double *__restrict__ gw = a.gauss + q2;
+    for (uint32_t kl = sub; kl < q2 * q2; kl += 16) {
+      const uint32_t k = kl / q2, l = kl - k * q2;
+      const double gk = gn[k], gl = gn[l], wk = gw[k], wl = gw[l];
+      for (uint32_t i = 0; i < q2; i++) {
+        const double gi = gn[i], wi = gw[i];
+        for (uint32_t j = 0; j < q2; j++)
+          SSTuple<CLS>::run(gi, gn[j], gk, gl, wi, gw[j], wk, wl, point);
+      }
     }
   }
 #pragma unroll
@@ -410,6 +462,29 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
       Sre[l][k] = row_sum(Sre[l][k]);
       if constexpr (CPLX) Sim[l][k] = row_sum(Sim[l][k]);
     }
+  // moments -> element matrix: (1, t, s) -> (1 - t, t - s, s) on each side
+  if constexpr (BR) {
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      Sre[0][k] -= Sre[1][k];
+      Sre[1][k] -= Sre[2][k];
+      if constexpr (CPLX) {
+        Sim[0][k] -= Sim[1][k];
+        Sim[1][k] -= Sim[2][k];
+      }
+    }
+  }
+  if constexpr (BC) {
+#pragma unroll
+    for (int l = 0; l < NR; l++) {
+      Sre[l][0] -= Sre[l][1];
+      Sre[l][1] -= Sre[l][2];
+      if constexpr (CPLX) {
+        Sim[l][0] -= Sim[l][1];
+        Sim[l][1] -= Sim[l][2];
+      }
+    }
+  }
   if (sub >= NR * NC) return;
   // lane e adds the entry of permuted local functions (l, k) = (e % NR,
   // e / NR)
@@ -481,18 +556,23 @@ int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
   constexpr uint32_t PPB = NTHREADS / 16;  // pairs per workgroup
   const uint32_t n3 = ncls[CLS_IDENTICAL], n2 = ncls[CLS_EDGE],
                  n1 = ncls[CLS_VERTEX];
+  using SK = void (*)(KArgs, const SPair *, uint32_t);
+  const bool q4 = a.q2 == 4;
+  const SK k3 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 4>
+                   : nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 0>;
+  const SK k2 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_EDGE, 4>
+                   : nf_singular_kernel<KER, BR, BC, CLS_EDGE, 0>;
+  const SK k1 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 4>
+                   : nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 0>;
   if (n3)
-    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL>),
-                       dim3((n3 + PPB - 1) / PPB), dim3(NTHREADS), 0, stream,
-                       a, sp, n3);
+    hipLaunchKernelGGL(k3, dim3((n3 + PPB - 1) / PPB), dim3(NTHREADS), 0,
+                       stream, a, sp, n3);
   if (n2)
-    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC, CLS_EDGE>),
-                       dim3((n2 + PPB - 1) / PPB), dim3(NTHREADS), 0, stream,
-                       a, sp + n3, n2);
+    hipLaunchKernelGGL(k2, dim3((n2 + PPB - 1) / PPB), dim3(NTHREADS), 0,
+                       stream, a, sp + n3, n2);
   if (n1)
-    hipLaunchKernelGGL((nf_singular_kernel<KER, BR, BC, CLS_VERTEX>),
-                       dim3((n1 + PPB - 1) / PPB), dim3(NTHREADS), 0, stream,
-                       a, sp + n3 + n2, n1);
+    hipLaunchKernelGGL(k1, dim3((n1 + PPB - 1) / PPB), dim3(NTHREADS), 0,
+                       stream, a, sp + n3 + n2, n1);
   return H2HIP_OK;
 }
 
