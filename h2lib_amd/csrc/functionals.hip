@@ -1,0 +1,228 @@
+// Boundary data on the device: L2 projection of the Dirichlet or Neumann
+// data of a point source onto the piecewise linear basis.
+//
+// Restates projectL2_bem3d_l_avector (Library/bem3d.c:9009-9088): the load
+// vector b_i = int phi_i f (integrate_bem3d_l_avector,
+// Library/bem3d.c:8631-8702, with the single-triangle Gauss rule of order q)
+// and conjugate gradients on the mass matrix (addeval_mass_linear_bem3d,
+// Library/bem3d.c:8878-8908) from x = 0 to |r| <= 1e-15 |b|, at most 500
+// steps.  f is eval_dirichlet_fundamental_laplacebem3d or
+// eval_neumann_fundamental_laplacebem3d (Library/laplacebem3d.c:1390-1430)
+// for the source point x0.
+//
+// The whole projection is one workgroup of 1024 threads: the vectors are
+// short (one entry per vertex) and the conjugate-gradient loop runs on the
+// device without any host round trip.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "h2hip_internal.h"
+
+namespace {
+
+constexpr int PT = 1024;
+constexpr int PW = PT / 64;
+constexpr int MAXQ2 = 64;  // points of the triangle rule (q <= 8)
+
+struct ProjArgs {
+  const double *x, *n, *g;
+  const uint32_t *t;
+  uint32_t nv, nt;
+  const uint32_t *vt_ptr, *vt;      // vertex -> 4 * triangle + local vertex
+  const uint32_t *mrow, *mcol;      // mass matrix, CSR
+  const double *mval;
+  double rx0[MAXQ2], rx1[MAXQ2], rw[MAXQ2];
+  uint32_t nq;
+  double src[3];
+  int kind;
+  double *tri;                      // 3 nt
+  double *b, *r, *p, *ap;           // nv each
+  double *out;
+};
+
+__device__ double block_sum(double v, double *red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < PW; w++) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(PT) void project_kernel(ProjArgs a) {
+  __shared__ double red[PW];
+  const uint32_t tid = threadIdx.x;
+  // per triangle: g_t sum_k w_k f(x_k) phi_l(x_k), l = 0, 1, 2
+  for (uint32_t t = tid; t < a.nt; t += PT) {
+    const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
+                 *C = a.x + 3 * a.t[3 * t + 2];
+    const double *nn = a.n + 3 * t;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t k = 0; k < a.nq; k++) {
+      const double u = a.rx0[k], v = a.rx1[k];
+      double d[3];
+      for (int c = 0; c < 3; c++)
+        d[c] = A[c] * (1.0 - u) + B[c] * (u - v) + C[c] * v - a.src[c];
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      const double rr = sqrt(r2);
+      const double f =
+          a.kind == 0 ? 1.0 / rr
+                      : -(d[0] * nn[0] + d[1] * nn[1] + d[2] * nn[2]) /
+                            (r2 * rr);
+      const double wf = a.rw[k] * f;
+      s0 += wf * (1.0 - u);
+      s1 += wf * (u - v);
+      s2 += wf * v;
+    }
+    a.tri[3 * t] = s0 * a.g[t];
+    a.tri[3 * t + 1] = s1 * a.g[t];
+    a.tri[3 * t + 2] = s2 * a.g[t];
+  }
+  __syncthreads();
+  // b_i = sum over the triangles of vertex i; CG start x = 0, r = p = b
+  double part = 0.0;
+  for (uint32_t i = tid; i < a.nv; i += PT) {
+    double s = 0.0;
+    for (uint32_t e = a.vt_ptr[i]; e < a.vt_ptr[i + 1]; e++) {
+      const uint32_t code = a.vt[e];
+      s += a.tri[3 * (code >> 2) + (code & 3)];
+    }
+    a.b[i] = s;
+    a.r[i] = s;
+    a.p[i] = s;
+    a.out[i] = 0.0;
+    part += s * s;
+  }
+  double rr = block_sum(part, red);
+  const double bb = rr;
+  for (int it = 0; it < 500 && rr > 1e-30 * bb; it++) {
+    // ap = M p, pap = <p, ap>
+    part = 0.0;
+    for (uint32_t i = tid; i < a.nv; i += PT) {
+      double s = 0.0;
+      for (uint32_t e = a.mrow[i]; e < a.mrow[i + 1]; e++)
+        s += a.mval[e] * a.p[a.mcol[e]];
+      a.ap[i] = s;
+      part += a.p[i] * s;
+    }
+    const double pap = block_sum(part, red);
+    const double alpha = rr / pap;
+    part = 0.0;
+    for (uint32_t i = tid; i < a.nv; i += PT) {
+      a.out[i] += alpha * a.p[i];
+      const double ri = a.r[i] - alpha * a.ap[i];
+      a.r[i] = ri;
+      part += ri * ri;
+    }
+    const double rr2 = block_sum(part, red);
+    const double beta = rr2 / rr;
+    rr = rr2;
+    // p is read by other threads in the next product: update after the
+    // barrier of block_sum, and barrier again before the product
+    for (uint32_t i = tid; i < a.nv; i += PT) a.p[i] = a.r[i] + beta * a.p[i];
+    __syncthreads();
+  }
+}
+
+// vertex -> (triangle, local vertex) lists and the CSR mass matrix
+// M_ij = sum over triangles containing i and j of g_t (1/12 if i = j,
+// 1/24 otherwise) (Library/laplacebem3d.c:985-994)
+int build_projection_data(const h2hip_surface *s) {
+  const uint32_t nv = s->nv, nt = s->nt;
+  std::vector<uint32_t> vt_ptr(nv + 1, 0), vt(3 * (size_t)nt);
+  for (size_t i = 0; i < 3 * (size_t)nt; i++) vt_ptr[s->t[i] + 1]++;
+  for (uint32_t v = 0; v < nv; v++) vt_ptr[v + 1] += vt_ptr[v];
+  std::vector<uint32_t> fill(vt_ptr.begin(), vt_ptr.end() - 1);
+  for (uint32_t t = 0; t < nt; t++)
+    for (uint32_t l = 0; l < 3; l++) vt[fill[s->t[3 * t + l]]++] = 4 * t + l;
+  std::vector<uint32_t> mrow(nv + 1, 0), mcol;
+  std::vector<double> mval;
+  std::vector<int64_t> pos(nv, -1);
+  std::vector<uint32_t> touched;
+  for (uint32_t i = 0; i < nv; i++) {
+    touched.clear();
+    for (uint32_t e = vt_ptr[i]; e < vt_ptr[i + 1]; e++) {
+      const uint32_t t = vt[e] >> 2;
+      for (uint32_t k = 0; k < 3; k++) {
+        const uint32_t j = s->t[3 * t + k];
+        const double m = s->g[t] * (j == i ? 1.0 / 12.0 : 1.0 / 24.0);
+        if (pos[j] < 0) {
+          pos[j] = (int64_t)mcol.size();
+          mcol.push_back(j);
+          mval.push_back(m);
+          touched.push_back(j);
+        } else {
+          mval[pos[j]] += m;
+        }
+      }
+    }
+    for (uint32_t j : touched) pos[j] = -1;
+    mrow[i + 1] = (uint32_t)mcol.size();
+  }
+  if (s->d_vt_ptr.upload(vt_ptr.data(), vt_ptr.size()) != hipSuccess ||
+      s->d_vt.upload(vt.data(), vt.size()) != hipSuccess ||
+      s->d_mrow.upload(mrow.data(), mrow.size()) != hipSuccess ||
+      s->d_mcol.upload(mcol.data(), mcol.size()) != hipSuccess ||
+      s->d_mval.upload(mval.data(), mval.size()) != hipSuccess ||
+      s->d_fwork.alloc(3 * (size_t)nt + 4 * (size_t)nv) != hipSuccess) {
+    h2hip::set_error("h2hip_projectL2: device allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  return H2HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
+                                            uint32_t q, int kind,
+                                            const double *x0, double *d_out,
+                                            void *stream) {
+  if (!s || !x0 || !d_out || q < 1 || q > 8 || (kind != 0 && kind != 1)) {
+    h2hip::set_error("h2hip_projectL2_point_source: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  {
+    std::lock_guard<std::mutex> guard(s->fl_lock);
+    if (!s->fl_ready) {
+      const int rc = build_projection_data(s);
+      if (rc != H2HIP_OK) return rc;
+      s->fl_ready = true;
+    }
+  }
+  h2hip::TriRule tr = h2hip::make_triangle_rule(q);
+  ProjArgs a;
+  a.x = s->d_x.p;
+  a.n = s->d_n.p;
+  a.g = s->d_g.p;
+  a.t = s->d_t.p;
+  a.nv = s->nv;
+  a.nt = s->nt;
+  a.vt_ptr = s->d_vt_ptr.p;
+  a.vt = s->d_vt.p;
+  a.mrow = s->d_mrow.p;
+  a.mcol = s->d_mcol.p;
+  a.mval = s->d_mval.p;
+  a.nq = (uint32_t)tr.w.size();
+  for (uint32_t k = 0; k < a.nq; k++) {
+    a.rx0[k] = tr.x0[k];
+    a.rx1[k] = tr.x1[k];
+    a.rw[k] = tr.w[k];
+  }
+  for (int c = 0; c < 3; c++) a.src[c] = x0[c];
+  a.kind = kind;
+  double *w = s->d_fwork.p;
+  a.tri = w;
+  a.b = w + 3 * (size_t)s->nt;
+  a.r = a.b + s->nv;
+  a.p = a.r + s->nv;
+  a.ap = a.p + s->nv;
+  a.out = d_out;
+  hipLaunchKernelGGL(project_kernel, dim3(1), dim3(PT), 0,
+                     (hipStream_t)stream, a);
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
+}

@@ -152,6 +152,13 @@ struct h2hip_surface {
   std::vector<uint32_t> v2t_ptr, v2t;  // triangles of each vertex
   h2hip::DevBuf<double> d_x, d_n, d_g;
   h2hip::DevBuf<uint32_t> d_t;
+  // data of the L2 projection onto the linear basis, built on first use:
+  // vertex -> (triangle, local vertex) lists, the mass matrix in CSR form
+  // and the workspace of the projection kernel
+  mutable std::mutex fl_lock;
+  mutable bool fl_ready = false;
+  mutable h2hip::DevBuf<uint32_t> d_vt_ptr, d_vt, d_mrow, d_mcol;
+  mutable h2hip::DevBuf<double> d_mval, d_fwork;
 };
 
 struct h2hip_op {

@@ -8,23 +8,27 @@
 // transform into y (Library/clusterbasis.c:748-797).  Here one product is
 // three launches over flattened trees:
 //
-//  1. fwd_up: one wavefront per leaf computes xt_t = V_t^T x|_t, then climbs
+//  1. fwd_up: one workgroup per leaf computes xt_t = V_t^T x|_t, then climbs
 //     the tree: the last son to finish (an atomic counter per cluster, reset
-//     by that son) computes xt_f = sum_s E_s^T xt_s for its father and goes
-//     on upwards.  No wavefront ever waits for another one.  The same launch
-//     clears the coupling accumulator yt.
+//     by that son, agent-scope fences around it) computes
+//     xt_f = sum_s E_s^T xt_s for its father and goes on upwards.  No
+//     workgroup ever waits for another one.  The same launch clears the
+//     coupling accumulator yt.
 //  2. couple_near: one wavefront per admissible block, yt_r += S_rc xt_c
 //     (FP64 atomics), and one per near-field block, y|_r += alpha N_rc x|_c.
-//  3. down_back: one wavefront per leaf of the output tree walks its path
+//  3. down_back: one workgroup per leaf of the output tree walks its path
 //     from the root, v = yt_s + E_s v, and adds y|_t += alpha V_t v.
 //
 // Every small product uses lane = output entry and loops over the input
-// entries, with each matrix stored in the orientation that makes that loop
+// entries (in 1. and 3. the four waves of the workgroup take a quarter of
+// the inputs each, so that the dependent chain up or down the tree is
+// short), with each matrix stored in the orientation that makes that loop
 // read 64 consecutive entries per step (leaf and transfer matrices are kept
 // in both orientations, couplings and near blocks column-major).  The
 // products are bound by HBM bandwidth: two flops per stored entry.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "h2hip_internal.h"
 
@@ -65,11 +69,6 @@ __device__ __forceinline__ void ld(const double *p, uint64_t i, double &re,
   }
 }
 
-// load written by another wavefront of the same launch
-__device__ __forceinline__ double ld_agent(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <int C>
 __device__ __forceinline__ void st(double *p, uint64_t i, double re,
                                    double im) {
@@ -92,105 +91,241 @@ __device__ __forceinline__ void atomic_add(double *p, uint64_t i, double re,
   }
 }
 
+// Vectors of one product call: NV right-hand sides at most, nv in use.
+// Coefficient vectors xt / yt hold the vectors one after the other,
+// cstride doubles apart.
+constexpr int MAXNV = 8;
+
+struct VecSet {
+  const double *x[MAXNV];
+  double *y[MAXNV];
+  double are[MAXNV], aim[MAXNV];
+  uint32_t nv;
+};
+
+// rank (and leaf size) bound of the LDS vectors: 128 for one vector, 64
+// when several vectors share a workgroup
+template <int NV>
+constexpr int rank_cap() {
+  return NV == 1 ? MAXRANK : 64;
+}
+
 struct FwdArgs {
   const NodeD *nodes;
-  const uint32_t *leaves;
-  uint32_t nleaves;
+  // one entry per workgroup: a leaf, or (bit 31 set) a cluster whose sons
+  // are all leaves
+  const uint32_t *work;
+  uint32_t nwork;
   const uint32_t *idx;
   const uint32_t *sons;
   const double *VT;  // leaf: rank x size column-major
   const double *ET;  // per son s: rank_father x rank_s column-major
   uint64_t eld;
-  const double *x;
   double *xt;
+  uint64_t cstride;  // doubles between the coefficient vectors
   uint32_t *cnt;
-  double *yt;  // cleared here (doubles)
+  double *yt;  // cleared here (nyt doubles per vector)
   uint64_t nyt;
+  VecSet v;
 };
 
-template <int C, int K>
-__global__ __launch_bounds__(64) void fwd_up(FwdArgs a) {
-  __shared__ double sv[2 * MAXRANK];
-  const uint32_t w = blockIdx.x;
-  const int lane = threadIdx.x;
-  {  // clear a slice of the coupling accumulator
-    const uint64_t chunk = (a.nyt + a.nleaves - 1) / a.nleaves;
-    const uint64_t e0 = (uint64_t)w * chunk, e1 = min(a.nyt, e0 + chunk);
-    for (uint64_t e = e0 + lane; e < e1; e += 64) a.yt[e] = 0.0;
-  }
-  uint32_t node = a.leaves[w];
-  {
-    const NodeD t = a.nodes[node];
-    const uint32_t rank = K ? K : t.rank;
-    // x|_t into LDS (leaf size <= MAXRANK is checked at creation)
-    for (uint32_t i = lane; i < t.size; i += 64) {
-      double xr, xi;
-      ld<C>(a.x, a.idx[t.off + i], xr, xi);
-      sv[2 * i] = xr;
-      sv[2 * i + 1] = xi;
-    }
-    __syncthreads();
-    const double *Vt = a.VT + t.voff;
-    for (uint32_t j = lane; j < rank; j += 64) {
-      double ar = 0.0, ai = 0.0;
-#pragma unroll 8
-      for (uint32_t i = 0; i < t.size; i++) {
-        const double v = Vt[j + (uint64_t)i * rank];
-        ar = fma(v, sv[2 * i], ar);
-        if constexpr (C) ai = fma(v, sv[2 * i + 1], ai);
+// Product of one small column-major matrix M (nout x nin, leading
+// dimension ldm, nout <= RK) with the NV LDS vectors sv[v * 2RK + ...],
+// split over the four waves of a workgroup: output o = lane + 64 c reads
+// M[o + i * ldm] for the wave's quarter of the inputs i.  Wave w leaves its
+// partial sums in part[(w NV + v) 2RK + 2o (+1)]; the caller adds the four
+// after the closing barrier.
+template <int C, int NV, int RK>
+__device__ __forceinline__ void wg_matvec(const double *__restrict__ M,
+                                          uint32_t ldm, uint32_t nout,
+                                          uint32_t nin, const double *sv,
+                                          double *part) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t q = (nin + WPB - 1) / WPB;
+  const uint32_t i0 = wave * q, i1 = min(nin, i0 + q);
+  constexpr int UNR = NV == 1 ? 8 : 2;
+#pragma unroll
+  for (int c = 0; c < RK / 64; c++) {
+    const uint32_t o = lane + 64 * c;
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+    if (o < nout) {
+#pragma unroll UNR
+      for (uint32_t i = i0; i < i1; i++) {
+        const double m = M[o + (uint64_t)i * ldm];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          ar[v] = fma(m, sv[v * 2 * RK + 2 * i], ar[v]);
+          if constexpr (C) ai[v] = fma(m, sv[v * 2 * RK + 2 * i + 1], ai[v]);
+        }
       }
-      st<C>(a.xt, t.coef + j, ar, ai);
     }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      part[(wave * NV + v) * 2 * RK + 2 * o] = ar[v];
+      part[(wave * NV + v) * 2 * RK + 2 * o + 1] = ai[v];
+    }
+  }
+  __syncthreads();
+}
+
+template <int NV, int RK>
+__device__ __forceinline__ double part_sum(const double *part, int v,
+                                           uint32_t e) {
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < WPB; w++) s += part[(w * NV + v) * 2 * RK + e];
+  return s;
+}
+
+// xt_t = V_t^T x|_t of leaf t into global memory; with keep != nullptr the
+// result also goes to the LDS vectors keep (for a product that follows)
+template <int C, int K, int NV>
+__device__ __forceinline__ void leaf_forward(const FwdArgs &a,
+                                             const NodeD &t, double *sv,
+                                             double *part, double *keep) {
+  constexpr int RK = rank_cap<NV>();
+  const int tid = threadIdx.x;
+  const uint32_t rank = K ? K : t.rank;
+  // x|_t into LDS (leaf size <= RK is checked before the launch)
+  for (uint32_t e = tid; e < NV * t.size; e += NT) {
+    const uint32_t v = e / t.size, i = e - v * t.size;
+    double xr = 0.0, xi = 0.0;
+    if (v < a.v.nv) ld<C>(a.v.x[v], a.idx[t.off + i], xr, xi);
+    sv[v * 2 * RK + 2 * i] = xr;
+    sv[v * 2 * RK + 2 * i + 1] = xi;
+  }
+  __syncthreads();
+  wg_matvec<C, NV, RK>(a.VT + t.voff, rank, rank, t.size, sv, part);
+  for (uint32_t e = tid; e < NV * rank; e += NT) {
+    const uint32_t v = e / rank, j = e - v * rank;
+    const double sr = part_sum<NV, RK>(part, v, 2 * j);
+    const double si = part_sum<NV, RK>(part, v, 2 * j + 1);
+    if (v < a.v.nv) st<C>(a.xt + v * a.cstride, t.coef + j, sr, si);
+    if (keep) {
+      keep[v * 2 * RK + 2 * j] = sr;
+      keep[v * 2 * RK + 2 * j + 1] = si;
+    }
+  }
+  __syncthreads();
+}
+
+template <int C, int K, int NV>
+__global__ __launch_bounds__(NT) void fwd_up(FwdArgs a) {
+  constexpr int RK = rank_cap<NV>();
+  __shared__ double sv[NV * 2 * RK], sv2[NV * 2 * RK];
+  __shared__ double part[WPB * NV * 2 * RK];
+  __shared__ uint32_t s_old;
+  const uint32_t w = blockIdx.x;
+  const int tid = threadIdx.x;
+  {  // clear a slice of the coupling accumulators
+    const uint64_t chunk = (a.nyt + a.nwork - 1) / a.nwork;
+    const uint64_t e0 = (uint64_t)w * chunk, e1 = min(a.nyt, e0 + chunk);
+    for (uint32_t v = 0; v < a.v.nv; v++)
+      for (uint64_t e = e0 + tid; e < e1; e += NT)
+        a.yt[v * a.cstride + e] = 0.0;
+  }
+  const uint32_t item = a.work[w];
+  uint32_t node = item & 0x7fffffffu;
+  // sums of the current cluster's coefficients, kept by wave 0: output
+  // j = lane + 64 c of vector v
+  double tot_re[RK / 64][NV], tot_im[RK / 64][NV];
+  auto clear_tot = [&]() {
+#pragma unroll
+    for (int c = 0; c < RK / 64; c++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) tot_re[c][v] = tot_im[c][v] = 0.0;
+  };
+  auto add_tot = [&](uint32_t rank) {
+    if (tid < 64) {
+#pragma unroll
+      for (int c = 0; c < RK / 64; c++) {
+        const uint32_t j = tid + 64 * c;
+        if (j < rank) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            tot_re[c][v] += part_sum<NV, RK>(part, v, 2 * j);
+            tot_im[c][v] += part_sum<NV, RK>(part, v, 2 * j + 1);
+          }
+        }
+      }
+    }
+  };
+  auto store_tot = [&](const NodeD &F, uint32_t rank) {
+    if (tid < 64) {
+#pragma unroll
+      for (int c = 0; c < RK / 64; c++) {
+        const uint32_t j = tid + 64 * c;
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+          if (j < rank && v < (int)a.v.nv)
+            st<C>(a.xt + v * a.cstride, F.coef + j, tot_re[c][v],
+                  tot_im[c][v]);
+      }
+    }
+  };
+  if (item >> 31) {
+    // a cluster whose sons are all leaves: the sons and the cluster in this
+    // workgroup, without counters
+    const NodeD F = a.nodes[node];
+    const uint32_t rank = K ? K : F.rank;
+    clear_tot();
+    for (uint32_t so = 0; so < F.nsons; so++) {
+      const uint32_t sn = a.sons[F.sonbeg + so];
+      const NodeD t = a.nodes[sn];
+      leaf_forward<C, K, NV>(a, t, sv, part, sv2);
+      wg_matvec<C, NV, RK>(a.ET + (uint64_t)sn * a.eld, rank, rank,
+                           K ? K : t.rank, sv2, part);
+      add_tot(rank);
+      __syncthreads();
+    }
+    store_tot(F, rank);
+  } else {
+    leaf_forward<C, K, NV>(a, a.nodes[node], sv, part, nullptr);
   }
   // climb: the last son of a cluster computes the cluster
   while (true) {
     const uint32_t f = a.nodes[node].father;
     if (f == NONE) break;
-    __threadfence();  // release this wavefront's coefficients
-    uint32_t old = 0;
-    if (lane == 0) old = atomicAdd(&a.cnt[f], 1u);
-    old = __shfl(old, 0, 64);
+    // release this workgroup's coefficients: the barrier completes every
+    // wave's stores, one agent-scope fence then publishes them before the
+    // counter increment
+    __syncthreads();
+    if (tid == 0) {
+      __threadfence();
+      s_old = atomicAdd(&a.cnt[f], 1u);
+    }
+    __syncthreads();
     const NodeD F = a.nodes[f];
-    if (old + 1 < F.nsons) break;
-    __threadfence();  // acquire the other sons' coefficients
-    if (lane == 0) a.cnt[f] = 0;
+    if (s_old + 1 < F.nsons) break;  // uniform
+    // acquire the other sons' coefficients (one fence for the workgroup,
+    // the barrier keeps every wave's loads behind it)
+    if (tid == 0) {
+      __threadfence();
+      a.cnt[f] = 0;
+    }
+    __syncthreads();
     const uint32_t rank = K ? K : F.rank;
-    double accr[MAXRANK / 64], acci[MAXRANK / 64];
-#pragma unroll
-    for (int c = 0; c < MAXRANK / 64; c++) accr[c] = acci[c] = 0.0;
+    clear_tot();
     for (uint32_t so = 0; so < F.nsons; so++) {
       const uint32_t sn = a.sons[F.sonbeg + so];
       const NodeD s = a.nodes[sn];
       const uint32_t srank = K ? K : s.rank;
-      __syncthreads();
-      for (uint32_t i = lane; i < srank; i += 64) {
-        if constexpr (C) {
-          sv[2 * i] = ld_agent(a.xt + 2 * (s.coef + i));
-          sv[2 * i + 1] = ld_agent(a.xt + 2 * (s.coef + i) + 1);
-        } else {
-          sv[2 * i] = ld_agent(a.xt + s.coef + i);
-        }
+      for (uint32_t e = tid; e < NV * srank; e += NT) {
+        const uint32_t v = e / srank, i = e - v * srank;
+        double xr = 0.0, xi = 0.0;
+        if (v < a.v.nv) ld<C>(a.xt + v * a.cstride, s.coef + i, xr, xi);
+        sv[v * 2 * RK + 2 * i] = xr;
+        sv[v * 2 * RK + 2 * i + 1] = xi;
       }
       __syncthreads();
-      const double *E = a.ET + (uint64_t)sn * a.eld;
-#pragma unroll
-      for (int c = 0; c < MAXRANK / 64; c++) {
-        const uint32_t j = lane + 64 * c;
-        if (j < rank) {
-#pragma unroll 8
-          for (uint32_t i = 0; i < srank; i++) {
-            const double e = E[j + (uint64_t)i * rank];
-            accr[c] = fma(e, sv[2 * i], accr[c]);
-            if constexpr (C) acci[c] = fma(e, sv[2 * i + 1], acci[c]);
-          }
-        }
-      }
+      wg_matvec<C, NV, RK>(a.ET + (uint64_t)sn * a.eld, rank, rank, srank,
+                           sv, part);
+      add_tot(rank);
+      __syncthreads();
     }
-#pragma unroll
-    for (int c = 0; c < MAXRANK / 64; c++) {
-      const uint32_t j = lane + 64 * c;
-      if (j < rank) st<C>(a.xt, F.coef + j, accr[c], acci[c]);
-    }
+    store_tot(F, rank);
     node = f;
   }
 }
@@ -203,38 +338,71 @@ struct CNArgs {
   const NodeD *rn, *cn;
   const uint32_t *ridx, *cidx;
   const double *S, *N;
-  const double *x, *xt;
-  double *yt, *y;
-  double are, aim;
+  const double *xt;
+  double *yt;
+  uint64_t cstride;
+  VecSet v;
 };
 
 // yt_r += S_rc xt_c and y|_r += alpha N_rc x|_c; TRANS: yt_c += S^* xt_r and
-// y|_c += alpha N^* x|_r
-template <int C, bool TRANS>
+// y|_c += alpha N^* x|_r.  One wave per block; with several vectors the
+// wave first gathers its block's input entries into its own LDS slice.
+template <int C, bool TRANS, int NV>
 __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
+  constexpr int XS = NV == 1 ? 1 : NV * 2 * 64;  // LDS doubles per wave
+  constexpr int UNR = NV == 1 ? 8 : 4;
+  __shared__ double xs_all[WPB * XS];
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  double *xs = xs_all + (threadIdx.x >> 6) * XS;
   if (w < a.nfar) {
     const FarD b = a.far[w];
     const NodeD r = a.rn[b.r], c = a.cn[b.c];
     const NodeD out = TRANS ? c : r, in = TRANS ? r : c;
+    if constexpr (NV > 1) {
+      // ranks <= 64 in this mode: one pass
+      for (uint32_t e = lane; e < NV * in.rank; e += 64) {
+        const uint32_t v = e / in.rank, j = e - v * in.rank;
+        double xr = 0.0, xi = 0.0;
+        if (v < a.v.nv) ld<C>(a.xt + v * a.cstride, in.coef + j, xr, xi);
+        xs[v * 128 + 2 * j] = xr;
+        xs[v * 128 + 2 * j + 1] = xi;
+      }
+      // the wave reads what its own lanes wrote: LDS keeps the order of
+      // one wave's accesses, the compiler must not move them
+      __builtin_amdgcn_wave_barrier();
+    }
     for (uint32_t i = lane; i < out.rank; i += 64) {
-      double ar = 0.0, ai = 0.0;
-#pragma unroll 8
+      double ar[NV], ai[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+#pragma unroll UNR
       for (uint32_t j = 0; j < in.rank; j++) {
         const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
                                  : (uint64_t)i + (uint64_t)j * r.rank;
-        double sr, si, xr, xi;
+        double sr, si;
         ld<C>(a.S, b.soff + e, sr, si);
         if (TRANS) si = -si;
-        ld<C>(a.xt, in.coef + j, xr, xi);
-        ar = fma(sr, xr, ar);
-        if constexpr (C) {
-          ar = fma(-si, xi, ar);
-          ai = fma(sr, xi, fma(si, xr, ai));
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          double xr, xi;
+          if constexpr (NV == 1) {
+            ld<C>(a.xt, in.coef + j, xr, xi);
+          } else {
+            xr = xs[v * 128 + 2 * j];
+            xi = xs[v * 128 + 2 * j + 1];
+          }
+          ar[v] = fma(sr, xr, ar[v]);
+          if constexpr (C) {
+            ar[v] = fma(-si, xi, ar[v]);
+            ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
+          }
         }
       }
-      atomic_add<C>(a.yt, out.coef + i, ar, ai);
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        if (v < (int)a.v.nv)
+          atomic_add<C>(a.yt + v * a.cstride, out.coef + i, ar[v], ai[v]);
     }
     return;
   }
@@ -244,24 +412,50 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
   const uint32_t nout = TRANS ? b.cols : b.rows, nin = TRANS ? b.rows : b.cols;
   const uint32_t *oidx = TRANS ? a.cidx + b.coff : a.ridx + b.roff;
   const uint32_t *iidx = TRANS ? a.ridx + b.roff : a.cidx + b.coff;
+  if constexpr (NV > 1) {
+    // leaf sizes <= 64 in this mode: one pass
+    for (uint32_t e = lane; e < NV * nin; e += 64) {
+      const uint32_t v = e / nin, j = e - v * nin;
+      double xr = 0.0, xi = 0.0;
+      if (v < a.v.nv) ld<C>(a.v.x[v], iidx[j], xr, xi);
+      xs[v * 128 + 2 * j] = xr;
+      xs[v * 128 + 2 * j + 1] = xi;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
   for (uint32_t i = lane; i < nout; i += 64) {
-    double ar = 0.0, ai = 0.0;
-#pragma unroll 8
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+#pragma unroll UNR
     for (uint32_t j = 0; j < nin; j++) {
       const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * b.rows
                                : (uint64_t)i + (uint64_t)j * b.rows;
-      double nr, ni, xr, xi;
+      double nr, ni;
       ld<C>(a.N, b.noff + e, nr, ni);
       if (TRANS) ni = -ni;
-      ld<C>(a.x, iidx[j], xr, xi);
-      ar = fma(nr, xr, ar);
-      if constexpr (C) {
-        ar = fma(-ni, xi, ar);
-        ai = fma(nr, xi, fma(ni, xr, ai));
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        double xr, xi;
+        if constexpr (NV == 1) {
+          ld<C>(a.v.x[0], iidx[j], xr, xi);
+        } else {
+          xr = xs[v * 128 + 2 * j];
+          xi = xs[v * 128 + 2 * j + 1];
+        }
+        ar[v] = fma(nr, xr, ar[v]);
+        if constexpr (C) {
+          ar[v] = fma(-ni, xi, ar[v]);
+          ai[v] = fma(nr, xi, fma(ni, xr, ai[v]));
+        }
       }
     }
-    atomic_add<C>(a.y, oidx[i], a.are * ar - a.aim * ai,
-                  a.are * ai + a.aim * ar);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      if (v < (int)a.v.nv)
+        atomic_add<C>(a.v.y[v], oidx[i],
+                      a.v.are[v] * ar[v] - a.v.aim[v] * ai[v],
+                      a.v.are[v] * ai[v] + a.v.aim[v] * ar[v]);
   }
 }
 
@@ -276,78 +470,65 @@ struct DBArgs {
   const double *E;  // per node s: rank_s x rank_father column-major
   uint64_t eld;
   const double *yt;
-  double *y;
-  double are, aim;
+  uint64_t cstride;
+  VecSet v;
 };
 
 // v = yt_root; for each node s on the path below the root v = yt_s + E_s v;
-// at the leaf y|_t += alpha V_t v
-template <int C, int K>
-__global__ __launch_bounds__(64) void down_back(DBArgs a) {
-  __shared__ double sv[2 * MAXRANK];
+// at the leaf y|_t += alpha V_t v.  Four waves split the inputs of every
+// product.
+template <int C, int K, int NV>
+__global__ __launch_bounds__(NT) void down_back(DBArgs a) {
+  constexpr int RK = rank_cap<NV>();
+  __shared__ double sv[NV * 2 * RK];
+  __shared__ double part[WPB * NV * 2 * RK];
   const uint32_t w = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const uint32_t *path = a.paths + (uint64_t)w * MAXPATH;
   const uint32_t len = a.plen[w];
   {
     const NodeD r = a.nodes[path[0]];
-    for (uint32_t i = lane; i < r.rank; i += 64) {
-      double vr, vi;
-      ld<C>(a.yt, r.coef + i, vr, vi);
-      sv[2 * i] = vr;
-      sv[2 * i + 1] = vi;
+    for (uint32_t e = tid; e < NV * r.rank; e += NT) {
+      const uint32_t v = e / r.rank, i = e - v * r.rank;
+      double vr = 0.0, vi = 0.0;
+      if (v < a.v.nv) ld<C>(a.yt + v * a.cstride, r.coef + i, vr, vi);
+      sv[v * 2 * RK + 2 * i] = vr;
+      sv[v * 2 * RK + 2 * i + 1] = vi;
     }
   }
+  __syncthreads();
   for (uint32_t p = 1; p < len; p++) {
     const uint32_t sn = path[p];
     const NodeD s = a.nodes[sn];
     const uint32_t kf = a.nodes[s.father].rank;
     const uint32_t rank = K ? K : s.rank;
-    const double *E = a.E + (uint64_t)sn * a.eld;
-    double accr[MAXRANK / 64], acci[MAXRANK / 64];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < MAXRANK / 64; c++) {
-      accr[c] = acci[c] = 0.0;
-      const uint32_t i = lane + 64 * c;
-      if (i < rank) {
-        ld<C>(a.yt, s.coef + i, accr[c], acci[c]);
-#pragma unroll 8
-        for (uint32_t j = 0; j < kf; j++) {
-          const double e = E[i + (uint64_t)j * rank];
-          accr[c] = fma(e, sv[2 * j], accr[c]);
-          if constexpr (C) acci[c] = fma(e, sv[2 * j + 1], acci[c]);
-        }
-      }
+    wg_matvec<C, NV, RK>(a.E + (uint64_t)sn * a.eld, rank, rank, kf, sv,
+                         part);
+    // sv is free again (all waves passed the barrier in wg_matvec)
+    for (uint32_t e = tid; e < NV * rank; e += NT) {
+      const uint32_t v = e / rank, i = e - v * rank;
+      double vr = 0.0, vi = 0.0;
+      if (v < a.v.nv) ld<C>(a.yt + v * a.cstride, s.coef + i, vr, vi);
+      sv[v * 2 * RK + 2 * i] = vr + part_sum<NV, RK>(part, v, 2 * i);
+      sv[v * 2 * RK + 2 * i + 1] = vi + part_sum<NV, RK>(part, v, 2 * i + 1);
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < MAXRANK / 64; c++) {
-      const uint32_t i = lane + 64 * c;
-      if (i < rank) {
-        sv[2 * i] = accr[c];
-        sv[2 * i + 1] = acci[c];
-      }
-    }
   }
-  __syncthreads();
   const NodeD t = a.nodes[a.leaves[w]];
   const uint32_t rank = K ? K : t.rank;
-  const double *Vt = a.V + t.voff;
-  for (uint32_t i = lane; i < t.size; i += 64) {
-    double ar = 0.0, ai = 0.0;
-#pragma unroll 8
-    for (uint32_t j = 0; j < rank; j++) {
-      const double v = Vt[i + (uint64_t)j * t.size];
-      ar = fma(v, sv[2 * j], ar);
-      if constexpr (C) ai = fma(v, sv[2 * j + 1], ai);
-    }
+  wg_matvec<C, NV, RK>(a.V + t.voff, t.size, t.size, rank, sv, part);
+  for (uint32_t e = tid; e < NV * t.size; e += NT) {
+    const uint32_t v = e / t.size, i = e - v * t.size;
+    if (v >= a.v.nv) continue;
+    const double ar = part_sum<NV, RK>(part, v, 2 * i);
+    const double ai = part_sum<NV, RK>(part, v, 2 * i + 1);
     const uint32_t d = a.idx[t.off + i];
+    double *y = a.v.y[v];
     if constexpr (C) {
-      a.y[2 * d] += a.are * ar - a.aim * ai;
-      a.y[2 * d + 1] += a.are * ai + a.aim * ar;
+      y[2 * d] += a.v.are[v] * ar - a.v.aim[v] * ai;
+      y[2 * d + 1] += a.v.are[v] * ai + a.v.aim[v] * ar;
     } else {
-      a.y[d] += a.are * ar;
+      y[d] += a.v.are[v] * ar;
     }
   }
 }
@@ -360,10 +541,11 @@ struct h2hip_h2matrix {
   struct Side {
     uint32_t nnodes = 0, ndof = 0, depth = 0;
     std::vector<NodeD> nodes;
-    std::vector<uint32_t> leaves, sons, paths, plen;
+    std::vector<uint32_t> leaves, sons, paths, plen, fwork;
     uint64_t ncoef = 0, vlen = 0, eld = 0;
     h2hip::DevBuf<NodeD> d_nodes;
-    h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sons, d_paths, d_plen, d_cnt;
+    h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sons, d_paths, d_plen, d_cnt,
+        d_fwork;
     h2hip::DevBuf<double> d_V, d_VT, d_E, d_ET;
   } side[2];
   uint32_t nnear = 0, nfar = 0;
@@ -373,6 +555,12 @@ struct h2hip_h2matrix {
   h2hip::DevBuf<FarD> d_far;
   h2hip::DevBuf<NearD> d_near;
   h2hip::DevBuf<double> d_S, d_N, d_xt, d_yt;
+  uint32_t maxleaf = 0, maxrank = 0;  // over both sides
+  uint64_t cstride = 0;               // doubles per coefficient vector
+  // coefficient vectors of the batched products (MAXNV vectors), allocated
+  // on first use
+  mutable std::mutex batch_lock;
+  mutable h2hip::DevBuf<double> d_xtb, d_ytb;
 };
 
 using h2hip::set_error;
@@ -470,6 +658,25 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       S.paths[l * MAXPATH + k] = chain[n - 1 - k];
     S.plen[l] = n;
   }
+  // upward-sweep work items: clusters whose sons are all leaves, then the
+  // leaves not covered by such a cluster
+  S.fwork.clear();
+  std::vector<char> covered(nnodes, 0);
+  for (uint32_t i = 0; i < nnodes; i++) {
+    if (nsons[i] == 0) continue;
+    bool twig = true;
+    for (uint32_t k = beg[i]; k < beg[i + 1]; k++)
+      twig = twig && nsons[S.sons[k]] == 0;
+    if (!twig) continue;
+    S.fwork.push_back(i | 0x80000000u);
+    for (uint32_t k = beg[i]; k < beg[i + 1]; k++) covered[S.sons[k]] = 1;
+  }
+  for (uint32_t l : S.leaves)
+    if (!covered[l]) S.fwork.push_back(l);
+  if (nnodes > 0x7fffffffu) {
+    set_error("h2matrix: too many clusters");
+    return H2HIP_EINVAL;
+  }
   S.ncoef = coef;
   S.vlen = v;
   S.eld = maxr * maxr;
@@ -480,33 +687,35 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       S.d_sons.upload(S.sons.data(), S.sons.size()) != hipSuccess ||
       S.d_paths.upload(S.paths.data(), S.paths.size()) != hipSuccess ||
       S.d_plen.upload(S.plen.data(), S.plen.size()) != hipSuccess ||
-      S.d_cnt.upload(zero.data(), nnodes) != hipSuccess) {
+      S.d_cnt.upload(zero.data(), nnodes) != hipSuccess ||
+      S.d_fwork.upload(S.fwork.data(), S.fwork.size()) != hipSuccess) {
     set_error("h2matrix: device allocation failed");
     return H2HIP_ENOMEM;
   }
   return H2HIP_OK;
 }
 
-template <int C, int K>
-void launch(const double *alpha, const h2hip_h2matrix *H, const double *x,
-            double *y, hipStream_t st, bool trans) {
+template <int C, int K, int NV>
+void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
+            hipStream_t st, bool trans) {
   const auto &in = H->side[trans ? 0 : 1];
   const auto &out = H->side[trans ? 1 : 0];
   FwdArgs f;
   f.nodes = in.d_nodes.p;
-  f.leaves = in.d_leaves.p;
-  f.nleaves = (uint32_t)in.leaves.size();
+  f.work = in.d_fwork.p;
+  f.nwork = (uint32_t)in.fwork.size();
   f.idx = in.d_idx.p;
   f.sons = in.d_sons.p;
   f.VT = in.d_VT.p;
   f.ET = in.d_ET.p;
   f.eld = in.eld;
-  f.x = x;
-  f.xt = H->d_xt.p;
+  f.xt = xt;
+  f.cstride = H->cstride;
   f.cnt = in.d_cnt.p;
-  f.yt = H->d_yt.p;
+  f.yt = yt;
   f.nyt = (C ? 2 : 1) * out.ncoef;
-  hipLaunchKernelGGL((fwd_up<C, K>), dim3(f.nleaves), dim3(64), 0, st, f);
+  f.v = vs;
+  hipLaunchKernelGGL((fwd_up<C, K, NV>), dim3(f.nwork), dim3(NT), 0, st, f);
 
   CNArgs c;
   c.far = H->d_far.p;
@@ -519,20 +728,18 @@ void launch(const double *alpha, const h2hip_h2matrix *H, const double *x,
   c.cidx = H->side[1].d_idx.p;
   c.S = H->d_S.p;
   c.N = H->d_N.p;
-  c.x = x;
-  c.xt = H->d_xt.p;
-  c.yt = H->d_yt.p;
-  c.y = y;
-  c.are = alpha[0];
-  c.aim = alpha[1];
+  c.xt = xt;
+  c.yt = yt;
+  c.cstride = H->cstride;
+  c.v = vs;
   const uint32_t nw = H->nfar + H->nnear;
   if (nw) {
     if (trans)
-      hipLaunchKernelGGL((couple_near<C, true>), dim3((nw + WPB - 1) / WPB),
-                         dim3(NT), 0, st, c);
+      hipLaunchKernelGGL((couple_near<C, true, NV>),
+                         dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
     else
-      hipLaunchKernelGGL((couple_near<C, false>), dim3((nw + WPB - 1) / WPB),
-                         dim3(NT), 0, st, c);
+      hipLaunchKernelGGL((couple_near<C, false, NV>),
+                         dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
   }
 
   DBArgs d;
@@ -545,23 +752,78 @@ void launch(const double *alpha, const h2hip_h2matrix *H, const double *x,
   d.V = out.d_V.p;
   d.E = out.d_E.p;
   d.eld = out.eld;
-  d.yt = H->d_yt.p;
-  d.y = y;
-  d.are = alpha[0];
-  d.aim = alpha[1];
-  hipLaunchKernelGGL((down_back<C, K>), dim3(d.nleaves), dim3(64), 0, st, d);
+  d.yt = yt;
+  d.cstride = H->cstride;
+  d.v = vs;
+  hipLaunchKernelGGL((down_back<C, K, NV>), dim3(d.nleaves), dim3(NT), 0, st,
+                     d);
 }
 
 template <int C>
 int apply(const double *alpha, const h2hip_h2matrix *H, const double *x,
           double *y, hipStream_t st, bool trans) {
+  VecSet vs = {};
+  vs.x[0] = x;
+  vs.y[0] = y;
+  vs.are[0] = alpha[0];
+  vs.aim[0] = alpha[1];
+  vs.nv = 1;
+  double *xt = H->d_xt.p, *yt = H->d_yt.p;
   switch (H->urank) {
-    case 8: launch<C, 8>(alpha, H, x, y, st, trans); break;
-    case 27: launch<C, 27>(alpha, H, x, y, st, trans); break;
-    case 64: launch<C, 64>(alpha, H, x, y, st, trans); break;
-    case 125: launch<C, 125>(alpha, H, x, y, st, trans); break;
-    default: launch<C, 0>(alpha, H, x, y, st, trans); break;
+    case 8: launch<C, 8, 1>(vs, H, xt, yt, st, trans); break;
+    case 27: launch<C, 27, 1>(vs, H, xt, yt, st, trans); break;
+    case 64: launch<C, 64, 1>(vs, H, xt, yt, st, trans); break;
+    case 125: launch<C, 125, 1>(vs, H, xt, yt, st, trans); break;
+    default: launch<C, 0, 1>(vs, H, xt, yt, st, trans); break;
   }
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
+}
+
+template <int C, int NV>
+void launch_batch(const VecSet &vs, const h2hip_h2matrix *H, double *xt,
+                  double *yt, hipStream_t st, bool trans) {
+  if (H->urank == 64)
+    launch<C, 64, NV>(vs, H, xt, yt, st, trans);
+  else
+    launch<C, 0, NV>(vs, H, xt, yt, st, trans);
+}
+
+template <int C>
+int apply_batch(uint32_t nvec, const double *alpha, const h2hip_h2matrix *H,
+                const void *const *x, void *const *y, hipStream_t st,
+                bool trans) {
+  if (nvec == 1) return apply<C>(alpha, H, (const double *)x[0],
+                                 (double *)y[0], st, trans);
+  if (H->maxleaf > 64 || H->maxrank > 64) {
+    set_error("batched H2 product: leaves and ranks of at most 64 needed");
+    return H2HIP_EINVAL;
+  }
+  {
+    std::lock_guard<std::mutex> guard(H->batch_lock);
+    if (!H->d_xtb.p) {
+      if (H->d_xtb.alloc(MAXNV * H->cstride) != hipSuccess ||
+          H->d_ytb.alloc(MAXNV * H->cstride) != hipSuccess) {
+        set_error("batched H2 product: device allocation failed");
+        return H2HIP_ENOMEM;
+      }
+    }
+  }
+  VecSet vs = {};
+  for (uint32_t v = 0; v < nvec; v++) {
+    vs.x[v] = (const double *)x[v];
+    vs.y[v] = (double *)y[v];
+    vs.are[v] = alpha[2 * v];
+    vs.aim[v] = alpha[2 * v + 1];
+  }
+  vs.nv = nvec;
+  double *xt = H->d_xtb.p, *yt = H->d_ytb.p;
+  if (nvec <= 2)
+    launch_batch<C, 2>(vs, H, xt, yt, st, trans);
+  else if (nvec <= 4)
+    launch_batch<C, 4>(vs, H, xt, yt, st, trans);
+  else
+    launch_batch<C, 8>(vs, H, xt, yt, st, trans);
   H2HIP_CHECK(hipGetLastError());
   return H2HIP_OK;
 }
@@ -627,6 +889,12 @@ int h2hip_h2matrix_create(h2hip_h2matrix **out, int is_complex,
   const int cm = H->cplx ? 2 : 1;
   const uint64_t ncoef =
       std::max(H->side[0].ncoef, H->side[1].ncoef) * (uint64_t)cm;
+  H->cstride = ncoef;
+  for (int sd = 0; sd < 2; sd++)
+    for (const NodeD &d : H->side[sd].nodes) {
+      H->maxrank = std::max(H->maxrank, d.rank);
+      if (d.nsons == 0) H->maxleaf = std::max(H->maxleaf, d.size);
+    }
   if (H->d_near.upload(H->near.data(), H->near.size()) != hipSuccess ||
       H->d_far.upload(H->far.data(), H->far.size()) != hipSuccess ||
       H->d_N.alloc(cm * no) != hipSuccess ||
@@ -734,6 +1002,25 @@ int h2hip_addeval_h2matrix(const double *alpha, const h2hip_h2matrix *H,
                             (hipStream_t)stream, false)
                  : apply<0>(alpha, H, (const double *)d_x, (double *)d_y,
                             (hipStream_t)stream, false);
+}
+
+int h2hip_addeval_h2matrix_batch(const h2hip_h2matrix *H, uint32_t nvec,
+                                 const double *alpha, const void *const *d_x,
+                                 void *const *d_y, int trans, void *stream) {
+  if (!H || !alpha || !d_x || !d_y || nvec == 0 || nvec > MAXNV) {
+    set_error("h2hip_addeval_h2matrix_batch: invalid arguments (1..8 "
+              "vectors)");
+    return H2HIP_EINVAL;
+  }
+  for (uint32_t v = 0; v < nvec; v++)
+    if (!d_x[v] || !d_y[v]) {
+      set_error("h2hip_addeval_h2matrix_batch: null vector");
+      return H2HIP_EINVAL;
+    }
+  return H->cplx ? apply_batch<1>(nvec, alpha, H, d_x, d_y,
+                                  (hipStream_t)stream, trans != 0)
+                 : apply_batch<0>(nvec, alpha, H, d_x, d_y,
+                                  (hipStream_t)stream, trans != 0);
 }
 
 int h2hip_addevaltrans_h2matrix(const double *alpha, const h2hip_h2matrix *H,

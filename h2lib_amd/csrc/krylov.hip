@@ -1,4 +1,5 @@
-// Restarted GMRES on device vectors.
+// Restarted GMRES on device vectors, with the Arnoldi process, the Givens
+// rotations and the restarts on the device.
 //
 // Semantics of the reference's solve_gmres_avector
 // (Library/krylovsolvers.c:183-227) with init/step/finish_gmres
@@ -7,13 +8,25 @@
 // residual |rhat_k| <= eps |b| or after maxiter - 1 steps, and the solution
 // is updated at every restart and at the end.  The reference orthogonalises
 // with Householder reflections; here the Arnoldi vectors are
-// orthogonalised by classical Gram-Schmidt with one reorthogonalisation
-// (two passes, each one multi-dot kernel and one update kernel), which
-// spans the same Krylov spaces and agrees to rounding.  All inner products
-// are FP64; the small Hessenberg least-squares problem (Givens rotations)
-// is solved on the host.
+// orthogonalised by classical Gram-Schmidt with one reorthogonalisation,
+// which spans the same Krylov spaces and agrees to rounding.  All inner
+// products are FP64.
+//
+// No step waits for the host.  The host enqueues up to CHUNK steps (never
+// past a restart or maxiter), then reads the residual estimates those steps
+// left in a small device array, and stops at the first step that meets the
+// reference's criterion.  Steps enqueued after that one are discarded: they
+// only touch Krylov columns beyond k and the rotation of the next column,
+// never the first k columns of R or g, so the update x += Q_k y is exactly
+// the one of the stopped iteration.
+//
+// Per step: the product w = A q_k (caller's addeval), two Gram-Schmidt
+// passes (multi-dot with the last block reducing the partial sums, then
+// w -= Q h; the second update also forms |w|^2 the same way), and one
+// kernel that scales q_{k+1} = w / |w| while its first thread applies the
+// Givens rotations.
+#include <algorithm>
 #include <cmath>
-#include <complex>
 #include <vector>
 
 #include "h2hip_internal.h"
@@ -21,20 +34,56 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int NB = 240;  // blocks of the reduction kernels
+constexpr int NB = 240;    // blocks of the multi-dot kernel
+constexpr int MAXK = 64;   // kmax limit
+constexpr int CHUNK = 8;   // steps between two looks at the residual
 
-template <int C>
-struct Val {
-  double re, im;
-};
+// Sums, over the blocks of a grid, 2 m doubles (m <= 64) per block stored
+// at part[(b * 64 + j) * 2 + {0, 1}]; run by the last block of the grid to
+// finish (cnt counts the finished blocks and is reset to 0 here).  The
+// agent-scope fences around the counter make the other blocks' partial sums
+// visible to the last block.  Its threads split the blocks between
+// NT / (2 m) groups per value and combine the groups in LDS.
+__device__ void last_block_sum(const double *part, int m, uint32_t *cnt,
+                               double *out) {
+  __shared__ bool last;
+  __shared__ double red[NT];
+  // the barrier completes every wave's stores; one agent-scope fence then
+  // publishes them before the counter increment (and, in the last block,
+  // makes the other blocks' sums visible before the barrier releases the
+  // loads below)
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+    if (last) __threadfence();
+  }
+  __syncthreads();
+  if (!last) return;
+  const int nv = 2 * m, groups = NT / nv, t = threadIdx.x;
+  double s = 0.0;
+  if (t < groups * nv) {
+    const int v = t % nv;
+#pragma unroll 4
+    for (uint32_t b = t / nv; b < gridDim.x; b += groups)
+      s += part[((uint64_t)b * 64 + (v >> 1)) * 2 + (v & 1)];
+  }
+  red[t] = s;
+  __syncthreads();
+  if (t < nv) {
+    double r = 0.0;
+    for (int g = 0; g < groups; g++) r += red[g * nv + t];
+    out[t] = r;
+  }
+  if (t == 0) *cnt = 0;
+}
 
-// partial[b * m + j] = sum over the block's share of conj(Q_j) . w
+// out[0 .. 2m) = Q[:, 0..m)^* w (complex pairs; imaginary parts 0 when real)
 template <int C>
-__global__ __launch_bounds__(NT) void multidot(const double *__restrict__ Q,
-                                               uint64_t ldq, int m,
-                                               const double *__restrict__ w,
-                                               uint64_t n,
-                                               double *__restrict__ part) {
+__global__ __launch_bounds__(NT) void multidot(
+    const double *__restrict__ Q, uint64_t ldq, int m,
+    const double *__restrict__ w, uint64_t n, double *__restrict__ part,
+    uint32_t *cnt, double *out) {
   __shared__ double red[NT / 64][2 * 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int j0 = 0; j0 < m; j0 += 32) {
@@ -88,42 +137,55 @@ __global__ __launch_bounds__(NT) void multidot(const double *__restrict__ Q,
     }
     __syncthreads();
   }
+  last_block_sum(part, m, cnt, out);
 }
 
-// w -= Q h (h: m coefficients, complex pairs when C)
+// w -= sign Q h (h: m complex pairs); with nrm != nullptr also
+// nrm[0] = |w|^2 of the result (last-block reduction)
 template <int C>
-__global__ __launch_bounds__(NT) void multiaxpy(const double *__restrict__ Q,
-                                                uint64_t ldq, int m,
-                                                const double *__restrict__ h,
-                                                double *__restrict__ w,
-                                                uint64_t n) {
+__global__ __launch_bounds__(NT) void multiaxpy(
+    const double *__restrict__ Q, uint64_t ldq, int m,
+    const double *__restrict__ h, double sign, double *__restrict__ w,
+    uint64_t n, double *__restrict__ part, uint32_t *cnt, double *nrm) {
   const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
-  if (i >= n) return;
-  double ar = 0.0, ai = 0.0;
-  for (int j = 0; j < m; j++) {
-    const uint64_t q = (uint64_t)j * ldq + i;
+  double sq = 0.0;
+  if (i < n) {
+    double ar = 0.0, ai = 0.0;
+    for (int j = 0; j < m; j++) {
+      const uint64_t q = (uint64_t)j * ldq + i;
+      if constexpr (C) {
+        const double qr = Q[2 * q], qi = Q[2 * q + 1];
+        const double hr = h[2 * j], hi = h[2 * j + 1];
+        ar = fma(qr, hr, fma(-qi, hi, ar));
+        ai = fma(qr, hi, fma(qi, hr, ai));
+      } else {
+        ar = fma(Q[q], h[2 * j], ar);
+      }
+    }
     if constexpr (C) {
-      const double qr = Q[2 * q], qi = Q[2 * q + 1];
-      const double hr = h[2 * j], hi = h[2 * j + 1];
-      ar = fma(qr, hr, fma(-qi, hi, ar));
-      ai = fma(qr, hi, fma(qi, hr, ai));
+      const double vr = fma(-sign, ar, w[2 * i]);
+      const double vi = fma(-sign, ai, w[2 * i + 1]);
+      w[2 * i] = vr;
+      w[2 * i + 1] = vi;
+      sq = fma(vr, vr, vi * vi);
     } else {
-      ar = fma(Q[q], h[j], ar);
+      const double v = fma(-sign, ar, w[i]);
+      w[i] = v;
+      sq = v * v;
     }
   }
-  if constexpr (C) {
-    w[2 * i] -= ar;
-    w[2 * i + 1] -= ai;
-  } else {
-    w[i] -= ar;
+  if (nrm == nullptr) return;  // uniform over the grid
+  __shared__ double red[NT / 64];
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int v = 0; v < NT / 64; v++) s += red[v];
+    part[(uint64_t)blockIdx.x * 128] = s;
+    part[(uint64_t)blockIdx.x * 128 + 1] = 0.0;
   }
-}
-
-// dst = s * src (real scale)
-__global__ void scale_copy(const double *src, double *dst, double s,
-                           uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
-  if (i < n) dst[i] = s * src[i];
+  last_block_sum(part, 1, cnt, nrm);
 }
 
 __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
@@ -131,200 +193,295 @@ __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
   if (i < n) dst[i] = src[i];
 }
 
-using cd = std::complex<double>;
+__global__ void zero_kernel(double *dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n) dst[i] = 0.0;
+}
 
-struct Workspace {
-  int C;
-  uint64_t n, ldq;  // ldq in elements (complex: pairs)
+// ---- small complex helpers for the single-thread parts ----------------------
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) {
+  return {a.re + b.re, a.im + b.im};
+}
+__device__ __forceinline__ cplx csub(cplx a, cplx b) {
+  return {a.re - b.re, a.im - b.im};
+}
+__device__ __forceinline__ double cabs_(cplx a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+// Least-squares state of the current restart cycle (device memory):
+// R (kmax x kmax, column-major), g (kmax + 1), rotations cs / sn (kmax),
+// all complex pairs.
+struct LSQ {
+  double *R, *g, *cs, *sn;
   int kmax;
-  h2hip::DevBuf<double> Q, w, part, h;
-  std::vector<double> hpart;
-  hipStream_t st;
 };
 
-// h[0..m) = Q[:, 0..m)^* w, sum of the block partials on the host
-template <int C>
-int mdot(Workspace &W, int m, const double *w, std::vector<cd> &out) {
-  hipLaunchKernelGGL(multidot<C>, dim3(NB), dim3(NT), 0, W.st, W.Q.p, W.ldq,
-                     m, w, W.n, W.part.p);
-  H2HIP_CHECK(hipGetLastError());
-  H2HIP_CHECK(hipMemcpyAsync(W.hpart.data(), W.part.p,
-                             sizeof(double) * NB * 64 * 2,
-                             hipMemcpyDeviceToHost, W.st));
-  H2HIP_CHECK(hipStreamSynchronize(W.st));
-  out.assign(m, cd(0.0, 0.0));
-  for (int b = 0; b < NB; b++)
-    for (int j = 0; j < m; j++)
-      out[j] += cd(W.hpart[((uint64_t)b * 64 + j) * 2],
-                   W.hpart[((uint64_t)b * 64 + j) * 2 + 1]);
-  return H2HIP_OK;
-}
-
-template <int C>
-int maxpy(Workspace &W, int m, const std::vector<cd> &h, double *w) {
-  std::vector<double> hh(2 * m);
-  for (int j = 0; j < m; j++) {
-    hh[C ? 2 * j : j] = h[j].real();
-    if (C) hh[2 * j + 1] = h[j].imag();
+// q_{k+1} = w / |w| by all threads; thread 0 of block 0 adds column k of the
+// Hessenberg matrix (h = hA + hB, subdiagonal |w|) to the QR factorisation
+// with Givens rotations and stores the new residual estimate |g[k+1]|.
+__global__ __launch_bounds__(NT) void scale_givens(
+    const double *__restrict__ w, double *__restrict__ qn, uint64_t n2,
+    const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
+    double *err) {
+  const double hn = sqrt(nrm2[0]);
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n2) qn[i] = hn > 0.0 ? w[i] / hn : 0.0;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  // previous rotations, streamed: rotation j touches entries j and j + 1,
+  // and entry j is final (column k of R) after it
+  cplx cur = {hA[0] + hB[0], hA[1] + hB[1]};
+  for (int j = 0; j < k; j++) {
+    const cplx c = {s.cs[2 * j], s.cs[2 * j + 1]};
+    const cplx sn = {s.sn[2 * j], s.sn[2 * j + 1]};
+    const cplx nx = {hA[2 * (j + 1)] + hB[2 * (j + 1)],
+                     hA[2 * (j + 1) + 1] + hB[2 * (j + 1) + 1]};
+    const cplx t = cadd(cmul(c, cur), cmul(sn, nx));
+    cur = csub(cmul(c, nx), cmul(cconj(sn), cur));
+    s.R[2 * (j + (uint64_t)k * s.kmax)] = t.re;
+    s.R[2 * (j + (uint64_t)k * s.kmax) + 1] = t.im;
   }
-  H2HIP_CHECK(hipMemcpyAsync(W.h.p, hh.data(), sizeof(double) * (C ? 2 : 1) * m,
-                             hipMemcpyHostToDevice, W.st));
-  hipLaunchKernelGGL(multiaxpy<C>, dim3((W.n + NT - 1) / NT), dim3(NT), 0,
-                     W.st, W.Q.p, W.ldq, m, W.h.p, w, W.n);
-  H2HIP_CHECK(hipGetLastError());
-  return H2HIP_OK;
-}
-
-// norm of a device vector: dot with itself via the multi-dot kernel
-template <int C>
-int vnorm(Workspace &W, const double *v, double &nrm) {
-  hipLaunchKernelGGL(multidot<C>, dim3(NB), dim3(NT), 0, W.st, v, (uint64_t)0,
-                     1, v, W.n, W.part.p);
-  H2HIP_CHECK(hipGetLastError());
-  H2HIP_CHECK(hipMemcpyAsync(W.hpart.data(), W.part.p,
-                             sizeof(double) * NB * 64 * 2,
-                             hipMemcpyDeviceToHost, W.st));
-  H2HIP_CHECK(hipStreamSynchronize(W.st));
-  double s = 0.0;
-  for (int b = 0; b < NB; b++) s += W.hpart[(uint64_t)b * 128];
-  nrm = std::sqrt(s);
-  return H2HIP_OK;
-}
-
-// Givens rotation that maps (a, b) to (r, 0)
-void givens(cd a, cd b, cd &c, cd &s) {
-  const double na = std::abs(a), nb = std::abs(b);
+  // rotation mapping (h_k, hn) to (r, 0)
+  const double na = cabs_(cur), nb = hn;
+  cplx c, sn;
   if (nb == 0.0) {
-    c = 1.0;
-    s = 0.0;
-    return;
+    c = {1.0, 0.0};
+    sn = {0.0, 0.0};
+  } else if (na == 0.0) {
+    c = {0.0, 0.0};
+    sn = {1.0, 0.0};
+  } else {
+    const double nr = hypot(na, nb);
+    c = {na / nr, 0.0};
+    sn = {cur.re / na * nb / nr, cur.im / na * nb / nr};
   }
-  if (na == 0.0) {
-    c = 0.0;
-    s = std::conj(b) / nb;
-    return;
-  }
-  const double nr = std::hypot(na, nb);
-  c = na / nr;
-  s = (a / na) * std::conj(b) / nr;
+  s.cs[2 * k] = c.re;
+  s.cs[2 * k + 1] = c.im;
+  s.sn[2 * k] = sn.re;
+  s.sn[2 * k + 1] = sn.im;
+  const cplx rkk = cadd(cmul(c, cur), {sn.re * hn, sn.im * hn});
+  s.R[2 * (k + (uint64_t)k * s.kmax)] = rkk.re;
+  s.R[2 * (k + (uint64_t)k * s.kmax) + 1] = rkk.im;
+  const cplx gk = {s.g[2 * k], s.g[2 * k + 1]};
+  const cplx g1 = cmul({-sn.re, sn.im}, gk);  // -conj(s) g_k
+  const cplx g0 = cmul(c, gk);
+  s.g[2 * k] = g0.re;
+  s.g[2 * k + 1] = g0.im;
+  s.g[2 * (k + 1)] = g1.re;
+  s.g[2 * (k + 1) + 1] = g1.im;
+  err[0] = cabs_(g1);
 }
+
+// q_0 = r / |r| by all threads; thread 0 of block 0 starts the
+// least-squares problem: g = (|r|, 0, ...), err = |r|
+__global__ __launch_bounds__(NT) void start_cycle(double *q0, uint64_t n2,
+                                                  const double *nrm2, LSQ s,
+                                                  double *err) {
+  const double beta = sqrt(nrm2[0]);
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n2 && beta > 0.0) q0[i] = q0[i] / beta;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int j = 0; j <= s.kmax; j++) {
+    s.g[2 * j] = 0.0;
+    s.g[2 * j + 1] = 0.0;
+  }
+  s.g[0] = beta;
+  err[0] = beta;
+}
+
+// y = -R^{-1} g for the first k columns (single thread), as coefficients
+// for multiaxpy (which subtracts)
+__global__ void back_substitute(LSQ s, int k, double *y) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int i = k - 1; i >= 0; i--) {
+    cplx t = {s.g[2 * i], s.g[2 * i + 1]};
+    for (int j = i + 1; j < k; j++) {
+      const cplx r = {s.R[2 * (i + (uint64_t)j * s.kmax)],
+                      s.R[2 * (i + (uint64_t)j * s.kmax) + 1]};
+      t = cadd(t, cmul(r, {y[2 * j], y[2 * j + 1]}));  // y holds -y_j
+    }
+    const cplx rii = {s.R[2 * (i + (uint64_t)i * s.kmax)],
+                      s.R[2 * (i + (uint64_t)i * s.kmax) + 1]};
+    const cplx yi = cdiv(t, rii);
+    y[2 * i] = -yi.re;
+    y[2 * i + 1] = -yi.im;
+  }
+}
+
+struct Workspace {
+  uint64_t n = 0, n2 = 0;  // entries, doubles per vector
+  int kmax = 0;
+  h2hip::DevBuf<double> Q, w, part, small;
+  h2hip::DevBuf<uint32_t> cnt;
+  // views into `small`
+  double *hA = nullptr, *hB = nullptr, *nrm = nullptr, *y = nullptr,
+         *err = nullptr, *R = nullptr, *g = nullptr, *cs = nullptr,
+         *sn = nullptr;
+  hipStream_t st = nullptr;
+  dim3 gv;
+
+  int alloc(int C, uint64_t n_, int kmax_, hipStream_t s) {
+    n = n_;
+    n2 = (C ? 2 : 1) * n;
+    kmax = kmax_;
+    st = s;
+    gv = dim3((uint32_t)((n2 + NT - 1) / NT));
+    const uint64_t nsmall = 128 * 3 + 8 + CHUNK + 2 * (uint64_t)kmax * kmax +
+                            2 * ((uint64_t)kmax + 1) + 4 * (uint64_t)kmax;
+    if (Q.alloc(n2 * kmax) != hipSuccess || w.alloc(n2) != hipSuccess ||
+        part.alloc((uint64_t)NB * 64 * 2) != hipSuccess ||
+        small.alloc(nsmall) != hipSuccess || cnt.alloc(4) != hipSuccess)
+      return H2HIP_ENOMEM;
+    double *p = small.p;
+    hA = p;
+    p += 128;
+    hB = p;
+    p += 128;
+    y = p;
+    p += 128;
+    nrm = p;
+    p += 8;
+    err = p;
+    p += CHUNK;
+    R = p;
+    p += 2 * (uint64_t)kmax * kmax;
+    g = p;
+    p += 2 * ((uint64_t)kmax + 1);
+    cs = p;
+    p += 2 * (uint64_t)kmax;
+    sn = p;
+    // the block counters start at zero; every reduction resets its own
+    if (hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * 4, st) != hipSuccess)
+      return H2HIP_EDEVICE;
+    return H2HIP_OK;
+  }
+  LSQ lsq() const { return LSQ{R, g, cs, sn, kmax}; }
+  double *col(int j) const { return Q.p + n2 * j; }
+};
 
 template <int C>
 int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
           double *x, double eps, uint32_t maxiter, uint32_t kmax,
           hipStream_t st, uint32_t *iters, double *resid) {
-  const int cm = C ? 2 : 1;
   Workspace W;
-  W.C = C;
-  W.n = n;
-  W.ldq = n;
-  W.kmax = (int)kmax;
-  W.st = st;
-  if (W.Q.alloc((uint64_t)cm * n * kmax) != hipSuccess ||
-      W.w.alloc((uint64_t)cm * n) != hipSuccess ||
-      W.part.alloc((uint64_t)NB * 64 * 2) != hipSuccess ||
-      W.h.alloc(2 * 64) != hipSuccess) {
+  if (W.alloc(C, n, (int)kmax, st) != H2HIP_OK) {
     h2hip::set_error("h2hip_solve_gmres: device allocation failed");
     return H2HIP_ENOMEM;
   }
-  W.hpart.resize((uint64_t)NB * 64 * 2);
   const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
-  const dim3 gv((uint32_t)((cm * n + NT - 1) / NT));
-  double normb;
-  int rc = vnorm<C>(W, b, normb);
-  if (rc) return rc;
+  const dim3 gax((uint32_t)((n + NT - 1) / NT));
+  // multi-dot grid: one row per thread up to NB blocks
+  const dim3 gdot((uint32_t)std::min<uint64_t>(NB, (n + NT - 1) / NT));
 
-  std::vector<cd> g(kmax + 1), cs(kmax), sn(kmax), h, h2;
-  std::vector<std::vector<cd>> R(kmax, std::vector<cd>(kmax));
-  int k = 0;
-  double error = 0.0;
+  // |b|
+  hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st, b, (uint64_t)0,
+                     1, b, n, W.part.p, W.cnt.p, W.nrm);
+  double nb2[2];
+  H2HIP_CHECK(hipMemcpyAsync(nb2, W.nrm, sizeof(nb2), hipMemcpyDeviceToHost,
+                             st));
+  H2HIP_CHECK(hipStreamSynchronize(st));
+  const double normb = std::sqrt(nb2[0]);
 
-  // r = b - A x in column 0 of Q, normalised
+  // r = b - A x in column 0, normalised; starts the least-squares problem
   auto init = [&]() -> int {
-    double *q0 = W.Q.p;
-    hipLaunchKernelGGL(copy_kernel, gv, dim3(NT), 0, st, b, q0,
-                       (uint64_t)cm * n);
+    double *q0 = W.col(0);
+    hipLaunchKernelGGL(copy_kernel, W.gv, dim3(NT), 0, st, b, q0, W.n2);
     int r = addeval(mone, A, x, q0, st);
     if (r) return r;
-    double beta;
-    r = vnorm<C>(W, q0, beta);
-    if (r) return r;
-    if (beta > 0.0)
-      hipLaunchKernelGGL(scale_copy, gv, dim3(NT), 0, st, q0, q0, 1.0 / beta,
-                         (uint64_t)cm * n);
-    std::fill(g.begin(), g.end(), cd(0.0, 0.0));
-    g[0] = beta;
-    k = 0;
-    error = beta;
+    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+                       (const double *)q0, (uint64_t)0, 1,
+                       (const double *)q0, n, W.part.p, W.cnt.p, W.nrm);
+    hipLaunchKernelGGL(start_cycle, W.gv, dim3(NT), 0, st, q0, W.n2,
+                       (const double *)W.nrm, W.lsq(), W.err);
     return H2HIP_OK;
   };
-  // x += Q y with R y = g (upper triangular, k x k)
-  auto finish = [&]() -> int {
+  // x += Q_k y with R y = g
+  auto finish = [&](int k) -> int {
     if (k == 0) return H2HIP_OK;
-    std::vector<cd> y(k);
-    for (int i = k - 1; i >= 0; i--) {
-      cd s = g[i];
-      for (int j = i + 1; j < k; j++) s -= R[i][j] * y[j];
-      y[i] = s / R[i][i];
-    }
-    for (auto &v : y) v = -v;  // maxpy subtracts
-    int r = maxpy<C>(W, k, y, x);
+    hipLaunchKernelGGL(back_substitute, dim3(1), dim3(64), 0, st, W.lsq(), k,
+                       W.y);
+    hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
+                       (const double *)W.Q.p, n, k, (const double *)W.y, 1.0,
+                       x, n, W.part.p, W.cnt.p + 1, (double *)nullptr);
+    return H2HIP_OK;
+  };
+  // one Arnoldi step on column k; its residual estimate goes to err[slot]
+  auto step = [&](int k, int slot) -> int {
+    double *qk = W.col(k), *w = W.w.p;
+    hipLaunchKernelGGL(zero_kernel, W.gv, dim3(NT), 0, st, w, W.n2);
+    int r = addeval(one, A, qk, w, st);
     if (r) return r;
-    return init();
+    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+                       (const double *)W.Q.p, n, k + 1, (const double *)w, n,
+                       W.part.p, W.cnt.p + 2, W.hA);
+    hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
+                       (const double *)W.Q.p, n, k + 1, (const double *)W.hA,
+                       1.0, w, n, W.part.p, W.cnt.p + 1, (double *)nullptr);
+    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+                       (const double *)W.Q.p, n, k + 1, (const double *)w, n,
+                       W.part.p, W.cnt.p + 2, W.hB);
+    hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
+                       (const double *)W.Q.p, n, k + 1, (const double *)W.hB,
+                       1.0, w, n, W.part.p, W.cnt.p + 1, W.nrm);
+    hipLaunchKernelGGL(scale_givens, W.gv, dim3(NT), 0, st,
+                       (const double *)w, W.col(k + 1), W.n2,
+                       (const double *)W.nrm, (const double *)W.hA,
+                       (const double *)W.hB, W.lsq(), k, W.err + slot);
+    return H2HIP_OK;
   };
 
-  rc = init();
+  int rc = init();
   if (rc) return rc;
+  double error;
+  H2HIP_CHECK(hipMemcpyAsync(&error, W.err, sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+  H2HIP_CHECK(hipStreamSynchronize(st));
+  H2HIP_CHECK(hipGetLastError());
+
+  int k = 0;
   uint32_t iter = 0;
+  double errs[CHUNK];
   while (error > eps * normb && iter + 1 != maxiter) {
     if (k + 1 >= (int)kmax) {
-      rc = finish();
+      rc = finish(k);
+      if (rc) return rc;
+      rc = init();
+      if (rc) return rc;
+      k = 0;
+    }
+    // steps until the next restart, maxiter, or the chunk size
+    uint32_t len = (uint32_t)((int)kmax - 1 - k);
+    if (len > (uint32_t)CHUNK) len = CHUNK;
+    if (maxiter != 0 && len > maxiter - 1 - iter) len = maxiter - 1 - iter;
+    for (uint32_t j = 0; j < len; j++) {
+      rc = step(k + (int)j, (int)j);
       if (rc) return rc;
     }
-    // step: w = A q_k, orthogonalise against q_0 .. q_k (two passes)
-    double *qk = W.Q.p + (uint64_t)cm * k * n;
-    double *w = W.w.p;
-    hipLaunchKernelGGL(scale_copy, gv, dim3(NT), 0, st, qk, w, 0.0,
-                       (uint64_t)cm * n);
-    rc = addeval(one, A, qk, w, st);
-    if (rc) return rc;
-    rc = mdot<C>(W, k + 1, w, h);
-    if (rc) return rc;
-    rc = maxpy<C>(W, k + 1, h, w);
-    if (rc) return rc;
-    rc = mdot<C>(W, k + 1, w, h2);
-    if (rc) return rc;
-    rc = maxpy<C>(W, k + 1, h2, w);
-    if (rc) return rc;
-    for (int j = 0; j <= k; j++) h[j] += h2[j];
-    double hn;
-    rc = vnorm<C>(W, w, hn);
-    if (rc) return rc;
-    double *qn = W.Q.p + (uint64_t)cm * (k + 1) * n;
-    if (hn > 0.0)
-      hipLaunchKernelGGL(scale_copy, gv, dim3(NT), 0, st, w, qn, 1.0 / hn,
-                         (uint64_t)cm * n);
-    // previous rotations, then the new one
-    for (int i = 0; i < k; i++) {
-      const cd t = cs[i] * h[i] + sn[i] * h[i + 1];
-      h[i + 1] = -std::conj(sn[i]) * h[i] + cs[i] * h[i + 1];
-      h[i] = t;
+    H2HIP_CHECK(hipGetLastError());
+    H2HIP_CHECK(hipMemcpyAsync(errs, W.err, sizeof(double) * len,
+                               hipMemcpyDeviceToHost, st));
+    H2HIP_CHECK(hipStreamSynchronize(st));
+    // replay the reference's loop over the steps just done
+    for (uint32_t j = 0; j < len; j++) {
+      k++;
+      iter++;
+      error = errs[j];
+      if (!(error > eps * normb && iter + 1 != maxiter)) break;
     }
-    cd c, s;
-    givens(h[k], cd(hn, 0.0), c, s);
-    cs[k] = c;
-    sn[k] = s;
-    const cd rkk = c * h[k] + s * hn;
-    for (int i = 0; i < k; i++) R[i][k] = h[i];
-    R[k][k] = rkk;
-    g[k + 1] = -std::conj(s) * g[k];
-    g[k] = c * g[k];
-    k++;
-    error = std::abs(g[k]);
-    iter++;
   }
-  rc = finish();
+  rc = finish(k);
   if (rc) return rc;
+  H2HIP_CHECK(hipGetLastError());
   H2HIP_CHECK(hipStreamSynchronize(st));
   if (iters) *iters = iter;
   if (resid) *resid = error;
@@ -338,7 +495,7 @@ extern "C" int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A,
                                  void *d_x, double eps, uint32_t maxiter,
                                  uint32_t kmax, void *stream,
                                  uint32_t *iterations, double *residual) {
-  if (!addeval || !d_b || !d_x || n == 0 || kmax < 2 || kmax > 64) {
+  if (!addeval || !d_b || !d_x || n == 0 || kmax < 2 || kmax > MAXK) {
     h2hip::set_error("h2hip_solve_gmres: invalid arguments (kmax 2..64)");
     return H2HIP_EINVAL;
   }

@@ -26,6 +26,10 @@ def _sigs():
     if _done:
         return
     h = hip()
+    h.h2hip_projectL2_point_source.restype = C.c_int
+    h.h2hip_projectL2_point_source.argtypes = [C.c_void_p, C.c_uint32,
+                                               C.c_int, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
     h.h2hip_solve_gmres.restype = C.c_int
     h.h2hip_solve_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_uint64, C.c_void_p, C.c_void_p,
@@ -89,10 +93,20 @@ class LaplaceDirichlet:
                               out.ctypes.data_as(C.c_void_p))
         return out
 
+    def project_device(self, x0, d_out, kind=0, stream=None):
+        """The same projection on the device (h2hip_projectL2_point_source),
+        into the device array d_out, ordered on the stream."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        check(hip().h2hip_projectL2_point_source(
+            self.gr.handle, self.q, kind, x0.ctypes.data_as(C.c_void_p),
+            C.c_void_p(d_out.ptr), C.c_void_p(stream.handle if stream else 0)),
+            "h2hip_projectL2_point_source")
+
     def rhs(self, x0, stream=None):
-        """Device vectors f and b = (K + M/2) f."""
-        f = self.project(x0)
-        d_f = DeviceArray.from_host(f)
+        """Device vectors f (projected Dirichlet data) and b = (K + M/2) f,
+        computed on the device."""
+        d_f = DeviceArray(self.n)
+        self.project_device(x0, d_f, 0, stream)
         d_b = DeviceArray(self.n)
         d_b.zero()
         self.KM.addeval(1.0, d_f, d_b, stream)

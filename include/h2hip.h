@@ -184,6 +184,17 @@ int h2hip_addeval_h2matrix(const double *alpha, const h2hip_h2matrix *H,
 int h2hip_addevaltrans_h2matrix(const double *alpha, const h2hip_h2matrix *H,
                                 const void *d_x, void *d_y, void *stream);
 
+/* The same products for nvec (1..8) vectors at once, which read the
+ * matrix once: y_v += alpha_v H x_v (trans: H^*), alpha = 2 nvec doubles
+ * (re, im), d_x / d_y host arrays of nvec device pointers.  For more than
+ * one vector every leaf and every rank must be at most 64.  Shares the
+ * calls' coefficient workspace like the single products.  No counterpart
+ * in the reference (it multiplies one vector at a time); serves batched
+ * solvers. */
+int h2hip_addeval_h2matrix_batch(const h2hip_h2matrix *H, uint32_t nvec,
+                                 const double *alpha, const void *const *d_x,
+                                 void *const *d_y, int trans, void *stream);
+
 /* ---- Krylov solvers ----------------------------------------------------------
  * y += alpha A x on device vectors, the counterpart of the reference's
  * addeval_t (Library/krylov.h:41) with a stream; h2hip_addeval_h2matrix has
@@ -200,6 +211,21 @@ int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A, int is_complex,
                       uint64_t n, const void *d_b, void *d_x, double eps,
                       uint32_t maxiter, uint32_t kmax, void *stream,
                       uint32_t *iterations, double *residual);
+
+/* ---- boundary data ----------------------------------------------------------
+ * L2 projection of the Dirichlet (kind 0: 1/|x - x0|) or Neumann (kind 1:
+ * -<x - x0, n>/|x - x0|^3) data of a point source at x0 onto the piecewise
+ * linear basis of the surface, written to the device vector d_out (one
+ * entry per vertex): the load vector with the single-triangle Gauss rule of
+ * order q (1..8) and conjugate gradients on the mass matrix to
+ * |r| <= 1e-15 |b| (at most 500 steps), all on the device, ordered on the
+ * stream, no host synchronisation.  projectL2_bem3d_l_avector
+ * (Library/bem3d.c:9009-9088) with eval_dirichlet_fundamental_laplacebem3d /
+ * eval_neumann_fundamental_laplacebem3d (Library/laplacebem3d.c:1390-1430).
+ * Calls on one surface share a workspace and must not overlap. */
+int h2hip_projectL2_point_source(const h2hip_surface *s, uint32_t q, int kind,
+                                 const double *x0, double *d_out,
+                                 void *stream);
 
 /* Waits for all work queued on the stream. */
 int h2hip_stream_sync(void *stream);

@@ -52,3 +52,24 @@ def test_gmres_residual_on_random_rhs():
         r = h.DeviceArray.from_host(b)
         prob.V.addeval(-1.0, d_x, r)
         assert np.linalg.norm(r.download()) <= 2e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kind,q", [(0, 2), (1, 2), (0, 3)])
+def test_device_projection_matches_host(kind, q):
+    # the device projection against the host restatement of
+    # projectL2_bem3d_l_avector (same rule, same stopping rule)
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import LaplaceDirichlet
+    mesh = G.sphere(12)
+    prob = LaplaceDirichlet(mesh, clf=24, eta=2.0, m=2, q=q)
+    x0 = np.array([1.2, 1.2, 1.2]) if kind == 0 else np.array([0.3, 0.2, 0.1])
+    want = prob.project(x0, kind)
+    d = h.DeviceArray(prob.n)
+    prob.project_device(x0, d, kind)
+    got = d.download()
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    # a second call on the same surface reuses the cached data
+    prob.project_device(2.0 * x0, d, kind)
+    want2 = prob.project(2.0 * x0, kind)
+    assert np.abs(d.download() - want2).max() < 1e-12 * np.abs(want2).max()
