@@ -188,6 +188,75 @@ def bench_dirichlet(h, mesh, args, stream):
         out["rel_err_against"] = ("the reference library's solution of the "
                                   "same discrete problem at residual "
                                   "1e-12 |b|")
+    out["_prob"] = prob
+    out["_ref"] = ref
+    return out
+
+
+def bench_serving(h, prob, ref, args):
+    """Continuous batching of Dirichlet solves with the operators resident
+    on the device: requests arrive as a Poisson stream (seeded), each with
+    the data of a point source x0 outside the unit sphere (the first one at
+    x0 = (1.2, 1.2, 1.2), whose solution is checked against the reference);
+    its right-hand side is formed on a second stream as soon as it arrives
+    and it enters the batched GMRES(30) as soon as one of the slots is free.
+    Latency: arrival to solution complete on the device."""
+    from h2lib_amd.solve import GmresBatch
+    rng = np.random.default_rng(2024)
+    R, rate = args.serve_requests, args.serve_rate
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    arrivals -= arrivals[0]
+    x0s = [np.array([1.2, 1.2, 1.2])]
+    for _ in range(R - 1):
+        d = rng.standard_normal(3)
+        x0s.append(d / np.linalg.norm(d) * rng.uniform(1.5, 3.0))
+    run_st, rhs_st = h.Stream(), h.Stream()
+    G = GmresBatch(prob.V, slots=args.serve_slots, kmax=30, stream=run_st)
+    n = prob.n
+    d_x = [h.DeviceArray(n) for _ in range(R)]
+    for d in d_x:
+        d.zero()
+    rhs = [None] * R
+    done_t = np.zeros(R)
+    steps = np.zeros(R, dtype=int)
+    formed, submitted, ndone = 0, 0, 0
+    t0 = time.perf_counter()
+    while ndone < R:
+        now = time.perf_counter() - t0
+        # right-hand sides of the requests that have arrived
+        while formed < R and arrivals[formed] <= now:
+            rhs[formed] = prob.rhs(x0s[formed], rhs_st)
+            formed += 1
+        # admissions into free slots (in arrival order)
+        if submitted < formed and G.free_slots() > 0:
+            rhs_st.sync()
+            while submitted < formed and G.free_slots() > 0:
+                G.submit(rhs[submitted][1], d_x[submitted], args.eps, 1000,
+                         submitted)
+                submitted += 1
+        if submitted > ndone:
+            for tag, it, res in G.run():
+                done_t[tag] = time.perf_counter() - t0
+                steps[tag] = it
+                rhs[tag] = None
+                ndone += 1
+        elif formed < R:
+            time.sleep(max(0.0, arrivals[formed] - (time.perf_counter() - t0)))
+    makespan = time.perf_counter() - t0
+    lat = done_t - arrivals
+    out = {
+        "requests": R, "arrival_rate_per_s": rate,
+        "slots": args.serve_slots,
+        "solves_per_s": R / makespan,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "gmres_steps_median": int(np.median(steps)),
+        "gmres_eps": args.eps,
+    }
+    if ref is not None:
+        x = d_x[0].download()
+        xr = ref["x"]
+        out["rel_err"] = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
     return out
 
 
@@ -205,6 +274,11 @@ def main():
     ap.add_argument("--solve-warmup", type=int, default=2)
     ap.add_argument("--eps", type=float, default=1e-10,
                     help="GMRES stopping tolerance relative to |b|")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the serving run (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=60.0,
+                    help="Poisson arrival rate of the serving run (1/s)")
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -299,6 +373,10 @@ def main():
         dirichlet = bench_dirichlet(h, mesh, args, stream)
         if counters.get("h2_product_bytes"):
             dirichlet["matvec"]["traffic"] = counters["h2_product_bytes"]
+        prob, ref = dirichlet.pop("_prob"), dirichlet.pop("_ref")
+        if args.serve_requests > 0:
+            log("bench: serving")
+            dirichlet["serving"] = bench_serving(h, prob, ref, args)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -80,7 +80,7 @@ __device__ void last_block_sum(const double *part, int m, uint32_t *cnt,
 
 // out[0 .. 2m) = Q[:, 0..m)^* w (complex pairs; imaginary parts 0 when real)
 template <int C>
-__global__ __launch_bounds__(NT) void multidot(
+__device__ __forceinline__ void multidot_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
     uint32_t *cnt, double *out) {
@@ -140,10 +140,18 @@ __global__ __launch_bounds__(NT) void multidot(
   last_block_sum(part, m, cnt, out);
 }
 
+template <int C>
+__global__ __launch_bounds__(NT) void multidot(
+    const double *__restrict__ Q, uint64_t ldq, int m,
+    const double *__restrict__ w, uint64_t n, double *__restrict__ part,
+    uint32_t *cnt, double *out) {
+  multidot_body<C>(Q, ldq, m, w, n, part, cnt, out);
+}
+
 // w -= sign Q h (h: m complex pairs); with nrm != nullptr also
 // nrm[0] = |w|^2 of the result (last-block reduction)
 template <int C>
-__global__ __launch_bounds__(NT) void multiaxpy(
+__device__ __forceinline__ void multiaxpy_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ h, double sign, double *__restrict__ w,
     uint64_t n, double *__restrict__ part, uint32_t *cnt, double *nrm) {
@@ -188,6 +196,14 @@ __global__ __launch_bounds__(NT) void multiaxpy(
   last_block_sum(part, 1, cnt, nrm);
 }
 
+template <int C>
+__global__ __launch_bounds__(NT) void multiaxpy(
+    const double *__restrict__ Q, uint64_t ldq, int m,
+    const double *__restrict__ h, double sign, double *__restrict__ w,
+    uint64_t n, double *__restrict__ part, uint32_t *cnt, double *nrm) {
+  multiaxpy_body<C>(Q, ldq, m, h, sign, w, n, part, cnt, nrm);
+}
+
 __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -230,7 +246,7 @@ struct LSQ {
 // q_{k+1} = w / |w| by all threads; thread 0 of block 0 adds column k of the
 // Hessenberg matrix (h = hA + hB, subdiagonal |w|) to the QR factorisation
 // with Givens rotations and stores the new residual estimate |g[k+1]|.
-__global__ __launch_bounds__(NT) void scale_givens(
+__device__ __forceinline__ void scale_givens_body(
     const double *__restrict__ w, double *__restrict__ qn, uint64_t n2,
     const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
     double *err) {
@@ -282,11 +298,18 @@ __global__ __launch_bounds__(NT) void scale_givens(
   err[0] = cabs_(g1);
 }
 
+__global__ __launch_bounds__(NT) void scale_givens(
+    const double *__restrict__ w, double *__restrict__ qn, uint64_t n2,
+    const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
+    double *err) {
+  scale_givens_body(w, qn, n2, nrm2, hA, hB, s, k, err);
+}
+
 // q_0 = r / |r| by all threads; thread 0 of block 0 starts the
 // least-squares problem: g = (|r|, 0, ...), err = |r|
-__global__ __launch_bounds__(NT) void start_cycle(double *q0, uint64_t n2,
-                                                  const double *nrm2, LSQ s,
-                                                  double *err) {
+__device__ __forceinline__ void start_cycle_body(double *q0, uint64_t n2,
+                                                 const double *nrm2, LSQ s,
+                                                 double *err) {
   const double beta = sqrt(nrm2[0]);
   const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
   if (i < n2 && beta > 0.0) q0[i] = q0[i] / beta;
@@ -297,6 +320,12 @@ __global__ __launch_bounds__(NT) void start_cycle(double *q0, uint64_t n2,
   }
   s.g[0] = beta;
   err[0] = beta;
+}
+
+__global__ __launch_bounds__(NT) void start_cycle(double *q0, uint64_t n2,
+                                                  const double *nrm2, LSQ s,
+                                                  double *err) {
+  start_cycle_body(q0, n2, nrm2, s, err);
 }
 
 // y = -R^{-1} g for the first k columns (single thread), as coefficients
@@ -488,6 +517,345 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   return H2HIP_OK;
 }
 
+// ---- batched GMRES: several independent solves sharing every product --------
+//
+// Up to MAXSLOT solves with the same H2-matrix advance in lock step: one
+// "tick" is one batched product (h2hip_addeval_h2matrix_batch, which reads
+// the matrix once for all of them) followed by the Gram-Schmidt passes and
+// the rotations of every slot, each kernel launched once for all slots
+// (grid dimension y = slot).  A slot's tick is either an Arnoldi step on
+// column k or the start of a cycle (r = b - A x into column 0).  Each solve
+// keeps the semantics of solve_gmres_avector exactly as the single solver
+// above does; slots come and go independently (continuous batching).
+
+constexpr int MAXSLOT = 8;
+
+struct TickArgs {
+  double *Q[MAXSLOT];   // Krylov basis, kmax columns of n2 doubles
+  double *w[MAXSLOT];   // Arnoldi: work vector; cycle start: column 0
+  const double *b[MAXSLOT];
+  double *hA[MAXSLOT], *hB[MAXSLOT], *nrm[MAXSLOT], *err[MAXSLOT];
+  double *part[MAXSLOT];
+  uint32_t *cnt[MAXSLOT];
+  LSQ lsq[MAXSLOT];
+  int k[MAXSLOT];       // Arnoldi column, -1: start of a cycle
+  uint64_t n, n2;
+};
+
+// Arnoldi slots: w = 0; cycle starts: column 0 = b (the product subtracts
+// A x from it)
+__global__ __launch_bounds__(NT) void tick_prep(TickArgs a) {
+  const int s = blockIdx.y;
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i >= a.n2) return;
+  if (a.k[s] < 0)
+    a.Q[s][i] = a.b[s][i];
+  else
+    a.w[s][i] = 0.0;
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void tick_dot(TickArgs a, int pass) {
+  const int s = blockIdx.y;
+  if (a.k[s] < 0) return;  // uniform per block
+  multidot_body<C>(a.Q[s], a.n, a.k[s] + 1, a.w[s], a.n, a.part[s],
+                   a.cnt[s] + 2, pass == 0 ? a.hA[s] : a.hB[s]);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void tick_axpy(TickArgs a, int pass) {
+  const int s = blockIdx.y;
+  const int m = a.k[s] < 0 ? 0 : a.k[s] + 1;
+  if (pass == 0 && m == 0) return;  // uniform per block
+  multiaxpy_body<C>(a.Q[s], a.n, m, pass == 0 ? a.hA[s] : a.hB[s], 1.0,
+                    a.w[s], a.n, a.part[s], a.cnt[s] + 1,
+                    pass == 0 ? nullptr : a.nrm[s]);
+}
+
+__global__ __launch_bounds__(NT) void tick_final(TickArgs a) {
+  const int s = blockIdx.y;
+  if (a.k[s] < 0)
+    start_cycle_body(a.Q[s], a.n2, a.nrm[s], a.lsq[s], a.err[s]);
+  else
+    scale_givens_body(a.w[s], a.Q[s] + a.n2 * (a.k[s] + 1), a.n2, a.nrm[s],
+                      a.hA[s], a.hB[s], a.lsq[s], a.k[s], a.err[s]);
+}
+
+}  // namespace
+
+struct h2hip_gmres_batch {
+  const h2hip_h2matrix *H = nullptr;
+  int C = 0;
+  uint64_t n = 0, n2 = 0;
+  int kmax = 0, nslots = 0;
+  hipStream_t st = nullptr;
+  struct Slot {
+    Workspace W;
+    bool active = false, started = false, need_start = false;
+    int k = 0;
+    uint32_t iter = 0, maxiter = 0;
+    double eps = 0.0, normb = 0.0, error = 0.0;
+    const double *b = nullptr;
+    double *x = nullptr;
+    uint64_t tag = 0;
+  } slot[MAXSLOT];
+  h2hip::DevBuf<double> d_errs;  // MAXSLOT x CHUNK
+};
+
+namespace {
+
+int batch_finish(h2hip_gmres_batch *G, h2hip_gmres_batch::Slot &S) {
+  if (S.k == 0) return H2HIP_OK;
+  const dim3 gax((uint32_t)((G->n + NT - 1) / NT));
+  hipLaunchKernelGGL(back_substitute, dim3(1), dim3(64), 0, G->st,
+                     S.W.lsq(), S.k, S.W.y);
+  if (G->C)
+    hipLaunchKernelGGL(multiaxpy<1>, gax, dim3(NT), 0, G->st,
+                       (const double *)S.W.Q.p, G->n, S.k,
+                       (const double *)S.W.y, 1.0, S.x, G->n, S.W.part.p,
+                       S.W.cnt.p + 1, (double *)nullptr);
+  else
+    hipLaunchKernelGGL(multiaxpy<0>, gax, dim3(NT), 0, G->st,
+                       (const double *)S.W.Q.p, G->n, S.k,
+                       (const double *)S.W.y, 1.0, S.x, G->n, S.W.part.p,
+                       S.W.cnt.p + 1, (double *)nullptr);
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
+}
+
+// one tick for the slots in `act` (k[s] = -1: start of a cycle)
+template <int C>
+int batch_tick(h2hip_gmres_batch *G, const int *act, const int *ks, int na,
+               int pos) {
+  TickArgs a = {};
+  a.n = G->n;
+  a.n2 = G->n2;
+  const void *xs[MAXSLOT];
+  void *ys[MAXSLOT];
+  double alpha[2 * MAXSLOT];
+  for (int j = 0; j < na; j++) {
+    auto &S = G->slot[act[j]];
+    a.Q[j] = S.W.Q.p;
+    a.w[j] = ks[j] < 0 ? S.W.Q.p : S.W.w.p;
+    a.b[j] = S.b;
+    a.hA[j] = S.W.hA;
+    a.hB[j] = S.W.hB;
+    a.nrm[j] = S.W.nrm;
+    a.err[j] = G->d_errs.p + (uint64_t)act[j] * CHUNK + pos;
+    a.part[j] = S.W.part.p;
+    a.cnt[j] = S.W.cnt.p;
+    a.lsq[j] = S.W.lsq();
+    a.k[j] = ks[j];
+    xs[j] = ks[j] < 0 ? (const double *)S.x : S.W.col(ks[j]);
+    ys[j] = ks[j] < 0 ? S.W.Q.p : S.W.w.p;
+    alpha[2 * j] = ks[j] < 0 ? -1.0 : 1.0;
+    alpha[2 * j + 1] = 0.0;
+  }
+  const uint32_t gv = (uint32_t)((G->n2 + NT - 1) / NT);
+  const uint32_t gax = (uint32_t)((G->n + NT - 1) / NT);
+  const uint32_t gdot = (uint32_t)std::min<uint64_t>(NB, gax);
+  hipLaunchKernelGGL(tick_prep, dim3(gv, na), dim3(NT), 0, G->st, a);
+  int rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, alpha, xs, ys, 0,
+                                        G->st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(tick_dot<C>, dim3(gdot, na), dim3(NT), 0, G->st, a, 0);
+  hipLaunchKernelGGL(tick_axpy<C>, dim3(gax, na), dim3(NT), 0, G->st, a, 0);
+  hipLaunchKernelGGL(tick_dot<C>, dim3(gdot, na), dim3(NT), 0, G->st, a, 1);
+  hipLaunchKernelGGL(tick_axpy<C>, dim3(gax, na), dim3(NT), 0, G->st, a, 1);
+  hipLaunchKernelGGL(tick_final, dim3(gv, na), dim3(NT), 0, G->st, a);
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
+}
+
+// One chunk: restarts due now, up to CHUNK ticks (none past a restart or
+// maxiter of any slot), one look at the residuals, completions.
+template <int C>
+int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
+                uint32_t *iters, double *resid, uint32_t cap) {
+  // restarts due at this point
+  for (int s = 0; s < G->nslots; s++) {
+    auto &S = G->slot[s];
+    if (S.active && S.started && !S.need_start && S.k + 1 >= G->kmax) {
+      int rc = batch_finish(G, S);
+      if (rc) return rc;
+      S.need_start = true;
+    }
+  }
+  // plan the chunk length
+  int len = CHUNK, nact = 0;
+  for (int s = 0; s < G->nslots; s++) {
+    auto &S = G->slot[s];
+    if (!S.active) continue;
+    nact++;
+    const int steps_cycle = S.need_start ? G->kmax - 1 : G->kmax - 1 - S.k;
+    int lim = S.need_start ? 1 + steps_cycle : steps_cycle;
+    if (S.maxiter != 0) {
+      const int left = (int)(S.maxiter - 1 - S.iter);
+      lim = std::min(lim, (S.need_start ? 1 : 0) + left);
+    }
+    len = std::min(len, std::max(lim, 1));
+  }
+  if (nact == 0) return H2HIP_OK;
+  // enqueue; the host follows each slot's (speculative) column index
+  int kk[MAXSLOT], first_tick_start[MAXSLOT];
+  for (int s = 0; s < G->nslots; s++) {
+    kk[s] = G->slot[s].need_start ? -1 : G->slot[s].k;
+    first_tick_start[s] = G->slot[s].need_start ? 1 : 0;
+  }
+  for (int t = 0; t < len; t++) {
+    int act[MAXSLOT], ks[MAXSLOT], na = 0;
+    for (int s = 0; s < G->nslots; s++) {
+      if (!G->slot[s].active) continue;
+      act[na] = s;
+      ks[na] = kk[s];
+      na++;
+      kk[s] = kk[s] < 0 ? 0 : kk[s] + 1;
+    }
+    int rc = batch_tick<C>(G, act, ks, na, t);
+    if (rc) return rc;
+  }
+  double errs[MAXSLOT * CHUNK];
+  H2HIP_CHECK(hipMemcpyAsync(errs, G->d_errs.p,
+                             sizeof(double) * MAXSLOT * CHUNK,
+                             hipMemcpyDeviceToHost, G->st));
+  H2HIP_CHECK(hipStreamSynchronize(G->st));
+  // replay the reference's loop per slot
+  uint32_t nd = 0;
+  for (int s = 0; s < G->nslots; s++) {
+    auto &S = G->slot[s];
+    if (!S.active) continue;
+    bool done = false;
+    int t = 0;
+    if (first_tick_start[s]) {
+      S.need_start = false;
+      S.k = 0;
+      if (!S.started) {
+        // the check before the first step (error = |r_0|)
+        S.started = true;
+        S.error = errs[s * CHUNK];
+        done = !(S.error > S.eps * S.normb && S.iter + 1 != S.maxiter);
+      }
+      t = 1;
+    }
+    for (; !done && t < len; t++) {
+      S.k++;
+      S.iter++;
+      S.error = errs[s * CHUNK + t];
+      done = !(S.error > S.eps * S.normb && S.iter + 1 != S.maxiter);
+    }
+    if (done) {
+      int rc = batch_finish(G, S);
+      if (rc) return rc;
+      S.active = false;
+      if (nd < cap) {
+        if (tags) tags[nd] = S.tag;
+        if (iters) iters[nd] = S.iter;
+        if (resid) resid[nd] = S.error;
+      }
+      nd++;
+    }
+  }
+  if (nd) H2HIP_CHECK(hipStreamSynchronize(G->st));
+  *ndone = nd;
+  return H2HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int h2hip_gmres_batch_create(h2hip_gmres_batch **out, const h2hip_h2matrix *H,
+                             int is_complex, uint64_t n, uint32_t slots,
+                             uint32_t kmax, void *stream) {
+  if (!out || !H || n == 0 || slots == 0 || slots > MAXSLOT || kmax < 2 ||
+      kmax > MAXK) {
+    h2hip::set_error("h2hip_gmres_batch_create: invalid arguments "
+                     "(1..8 slots, kmax 2..64)");
+    return H2HIP_EINVAL;
+  }
+  h2hip_gmres_batch *G = new h2hip_gmres_batch;
+  G->H = H;
+  G->C = is_complex ? 1 : 0;
+  G->n = n;
+  G->n2 = (G->C ? 2 : 1) * n;
+  G->kmax = (int)kmax;
+  G->nslots = (int)slots;
+  G->st = (hipStream_t)stream;
+  for (uint32_t s = 0; s < slots; s++)
+    if (G->slot[s].W.alloc(G->C, n, (int)kmax, G->st) != H2HIP_OK) {
+      delete G;
+      h2hip::set_error("h2hip_gmres_batch_create: device allocation failed");
+      return H2HIP_ENOMEM;
+    }
+  if (G->d_errs.alloc(MAXSLOT * CHUNK) != hipSuccess) {
+    delete G;
+    h2hip::set_error("h2hip_gmres_batch_create: device allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  *out = G;
+  return H2HIP_OK;
+}
+
+void h2hip_gmres_batch_destroy(h2hip_gmres_batch *G) { delete G; }
+
+int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G) {
+  if (!G) return H2HIP_EINVAL;
+  int f = 0;
+  for (int s = 0; s < G->nslots; s++) f += !G->slot[s].active;
+  return f;
+}
+
+int h2hip_gmres_batch_submit(h2hip_gmres_batch *G, const void *d_b,
+                             void *d_x, double eps, uint32_t maxiter,
+                             uint64_t tag) {
+  if (!G || !d_b || !d_x) return H2HIP_EINVAL;
+  int s = 0;
+  while (s < G->nslots && G->slot[s].active) s++;
+  if (s == G->nslots) {
+    h2hip::set_error("h2hip_gmres_batch_submit: no free slot");
+    return H2HIP_EINVAL;
+  }
+  auto &S = G->slot[s];
+  S.b = (const double *)d_b;
+  S.x = (double *)d_x;
+  S.eps = eps;
+  S.maxiter = maxiter;
+  S.tag = tag;
+  S.k = 0;
+  S.iter = 0;
+  S.started = false;
+  S.need_start = true;
+  // |b| (one small reduction and a wait, once per request)
+  const uint32_t gdot = (uint32_t)std::min<uint64_t>(NB, (G->n + NT - 1) / NT);
+  if (G->C)
+    hipLaunchKernelGGL(multidot<1>, dim3(gdot), dim3(NT), 0, G->st, S.b,
+                       (uint64_t)0, 1, S.b, G->n, S.W.part.p, S.W.cnt.p,
+                       S.W.nrm);
+  else
+    hipLaunchKernelGGL(multidot<0>, dim3(gdot), dim3(NT), 0, G->st, S.b,
+                       (uint64_t)0, 1, S.b, G->n, S.W.part.p, S.W.cnt.p,
+                       S.W.nrm);
+  double nb2[2];
+  H2HIP_CHECK(hipMemcpyAsync(nb2, S.W.nrm, sizeof(nb2), hipMemcpyDeviceToHost,
+                             G->st));
+  H2HIP_CHECK(hipStreamSynchronize(G->st));
+  S.normb = std::sqrt(nb2[0]);
+  S.active = true;
+  return s;
+}
+
+int h2hip_gmres_batch_run(h2hip_gmres_batch *G, uint32_t *ndone,
+                          uint64_t *tags, uint32_t *iterations,
+                          double *residuals, uint32_t cap) {
+  if (!G || !ndone) return H2HIP_EINVAL;
+  *ndone = 0;
+  return G->C ? batch_chunk<1>(G, ndone, tags, iterations, residuals, cap)
+              : batch_chunk<0>(G, ndone, tags, iterations, residuals, cap);
+}
+
+}  // extern "C"
+
+namespace {
 }  // namespace
 
 extern "C" int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A,

@@ -56,6 +56,9 @@ def _sigs():
     h.h2hip_h2matrix_bytes.argtypes = [vp, dp]
     h.h2hip_addeval_h2matrix.argtypes = [dp, vp, vp, vp, vp]
     h.h2hip_addevaltrans_h2matrix.argtypes = [dp, vp, vp, vp, vp]
+    h.h2hip_addeval_h2matrix_batch.restype = C.c_int
+    h.h2hip_addeval_h2matrix_batch.argtypes = [vp, C.c_uint32, dp, dp, dp,
+                                               C.c_int, vp]
     _sigs_done = True
 
 
@@ -153,6 +156,43 @@ class H2Matrix:
                                           btree.near_block_list())
             self.assemble_nearfield(stream)
 
+    @classmethod
+    def from_arrays(cls, interp, near_values, couplings=None, rt=None,
+                    is_complex=False):
+        """A device H2-matrix from host arrays: the cluster bases of an
+        Interpolation, near-field blocks packed in the order of interp.near
+        and couplings (default: interp.S), real or complex."""
+        _sigs()
+        self = cls.__new__(cls)
+        self.op = None
+        self.rt = rt
+        self.nearplan = None
+        self.interp = I = interp
+        self.k = I.k
+        self.dtype = np.complex128 if is_complex else np.float64
+        h = C.c_void_p()
+        check(hip().h2hip_h2matrix_create(
+            C.byref(h), int(is_complex), len(I.rtree), _ptr(I.rtree),
+            _ptr(I.rrank), len(I.ridx), _ptr(I.ridx), len(I.ctree),
+            _ptr(I.ctree), _ptr(I.crank), len(I.cidx), _ptr(I.cidx),
+            len(I.near), _ptr(I.near), len(I.far), _ptr(I.far)),
+            "h2hip_h2matrix_create")
+        self._h = h
+        check(hip().h2hip_h2matrix_set_basis(
+            h, 0, _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld),
+            "h2hip_h2matrix_set_basis")
+        check(hip().h2hip_h2matrix_set_basis(
+            h, 1, _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld),
+            "h2hip_h2matrix_set_basis")
+        S = I.S if couplings is None else couplings
+        S = np.ascontiguousarray(S, dtype=self.dtype)
+        check(hip().h2hip_h2matrix_set_couplings(h, _ptr(S), I.eld),
+              "h2hip_h2matrix_set_couplings")
+        N = np.ascontiguousarray(near_values, dtype=self.dtype)
+        check(hip().h2hip_h2matrix_set_nearfield(h, _ptr(N)),
+              "h2hip_h2matrix_set_nearfield")
+        return self
+
     def assemble_nearfield(self, stream=None):
         """Near-field blocks by the HIP quadrature, straight into the
         matrix's device storage."""
@@ -176,6 +216,21 @@ class H2Matrix:
         check(f(_ptr(a), self._h, C.c_void_p(d_x.ptr), C.c_void_p(d_y.ptr),
                 C.c_void_p(stream.handle if stream else 0)),
               "h2hip_addeval_h2matrix")
+
+    def addeval_batch(self, alphas, d_xs, d_ys, stream=None, trans=False):
+        """y_v += alpha_v H x_v (trans: H^*) for up to 8 device arrays at
+        once; the matrix is read once for all of them."""
+        nv = len(d_xs)
+        a = np.zeros(2 * nv)
+        a[0::2] = np.real(alphas)
+        a[1::2] = np.imag(alphas)
+        xs = (C.c_void_p * nv)(*[d.ptr for d in d_xs])
+        ys = (C.c_void_p * nv)(*[d.ptr for d in d_ys])
+        check(hip().h2hip_addeval_h2matrix_batch(
+            self._h, nv, _ptr(a), C.cast(xs, C.c_void_p),
+            C.cast(ys, C.c_void_p), int(bool(trans)),
+            C.c_void_p(stream.handle if stream else 0)),
+            "h2hip_addeval_h2matrix_batch")
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -38,6 +38,21 @@ def _sigs():
                                     C.POINTER(C.c_double)]
     b = bem()
     dp = C.c_void_p
+    h.h2hip_gmres_batch_create.restype = C.c_int
+    h.h2hip_gmres_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_void_p,
+                                           C.c_int, C.c_uint64, C.c_uint32,
+                                           C.c_uint32, C.c_void_p]
+    h.h2hip_gmres_batch_destroy.argtypes = [C.c_void_p]
+    h.h2hip_gmres_batch_free_slots.restype = C.c_int
+    h.h2hip_gmres_batch_free_slots.argtypes = [C.c_void_p]
+    h.h2hip_gmres_batch_submit.restype = C.c_int
+    h.h2hip_gmres_batch_submit.argtypes = [C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_double,
+                                           C.c_uint32, C.c_uint64]
+    h.h2hip_gmres_batch_run.restype = C.c_int
+    h.h2hip_gmres_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint32),
+                                        C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_uint32]
     b.h2b_integrate_l.argtypes = [dp, C.c_uint, C.c_int, dp, dp]
     b.h2b_projectL2_l.restype = C.c_int
     b.h2b_projectL2_l.argtypes = [dp, C.c_uint, C.c_int, dp, dp]
@@ -55,13 +70,71 @@ def solve_gmres(H, d_b, d_x, eps, maxiter, kmax, stream=None):
     fn = C.cast(hip().h2hip_addeval_h2matrix, C.c_void_p)
     it = C.c_uint32()
     res = C.c_double()
-    check(hip().h2hip_solve_gmres(fn, H._h, int(H.op.is_complex), d_b.n,
+    check(hip().h2hip_solve_gmres(fn, H._h, int(H.dtype == np.complex128),
+                                  d_b.n,
                                   C.c_void_p(d_b.ptr), C.c_void_p(d_x.ptr),
                                   eps, maxiter, kmax,
                                   C.c_void_p(stream.handle if stream else 0),
                                   C.byref(it), C.byref(res)),
           "h2hip_solve_gmres")
     return it.value, res.value
+
+
+class GmresBatch:
+    """Up to 8 independent GMRES(kmax) solves with one H2-matrix, advanced
+    together so that every step reads the matrix once (continuous batching:
+    solves enter free slots and leave when done, each with the semantics of
+    solve_gmres)."""
+
+    def __init__(self, H, slots=8, kmax=30, stream=None):
+        _sigs()
+        self.H = H
+        self.stream = stream
+        h = C.c_void_p()
+        check(hip().h2hip_gmres_batch_create(
+            C.byref(h), H._h, int(H.dtype == np.complex128), len(H.interp.ridx),
+            slots, kmax, C.c_void_p(stream.handle if stream else 0)),
+            "h2hip_gmres_batch_create")
+        self._h = h
+        self._keep = {}
+
+    def free_slots(self):
+        return hip().h2hip_gmres_batch_free_slots(self._h)
+
+    def submit(self, d_b, d_x, eps, maxiter, tag):
+        """Starts a solve of H x = b (x holds the start vector); returns the
+        slot."""
+        s = hip().h2hip_gmres_batch_submit(self._h, C.c_void_p(d_b.ptr),
+                                            C.c_void_p(d_x.ptr), eps,
+                                            maxiter, tag)
+        if s < 0:
+            check(s, "h2hip_gmres_batch_submit")
+        self._keep[tag] = (d_b, d_x)
+        return s
+
+    def run(self):
+        """Advances all solves by one chunk of steps; returns the solves
+        that finished as (tag, steps, residual)."""
+        cap = 8
+        nd = C.c_uint32()
+        tags = np.zeros(cap, dtype=np.uint64)
+        its = np.zeros(cap, dtype=np.uint32)
+        res = np.zeros(cap)
+        check(hip().h2hip_gmres_batch_run(
+            self._h, C.byref(nd), tags.ctypes.data_as(C.c_void_p),
+            its.ctypes.data_as(C.c_void_p), res.ctypes.data_as(C.c_void_p),
+            cap), "h2hip_gmres_batch_run")
+        out = []
+        for i in range(nd.value):
+            out.append((int(tags[i]), int(its[i]), float(res[i])))
+            self._keep.pop(int(tags[i]), None)
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            hip().h2hip_gmres_batch_destroy(h)
+            self._h = None
 
 
 class LaplaceDirichlet:

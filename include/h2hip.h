@@ -227,6 +227,36 @@ int h2hip_projectL2_point_source(const h2hip_surface *s, uint32_t q, int kind,
                                  const double *x0, double *d_out,
                                  void *stream);
 
+/* ---- batched GMRES (serving) --------------------------------------------------
+ * Up to 8 independent GMRES(kmax) solves with one H2-matrix advance
+ * together: every step is one h2hip_addeval_h2matrix_batch product for all
+ * of them followed by the per-solve Gram-Schmidt and rotation kernels, each
+ * launched once for all solves.  Each solve has the semantics of
+ * h2hip_solve_gmres; solves enter free slots (submit) and leave when they
+ * finish (run reports them), so requests arriving at different times share
+ * the products (continuous batching).  No counterpart in the reference; for
+ * callers that serve many right-hand sides.  The matrix must satisfy the
+ * batched product's limits (leaves and ranks up to 64). */
+typedef struct h2hip_gmres_batch h2hip_gmres_batch;
+int h2hip_gmres_batch_create(h2hip_gmres_batch **out, const h2hip_h2matrix *H,
+                             int is_complex, uint64_t n, uint32_t slots,
+                             uint32_t kmax, void *stream);
+void h2hip_gmres_batch_destroy(h2hip_gmres_batch *G);
+/* Number of free slots. */
+int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G);
+/* Starts a solve of A x = b (device vectors; x holds the start vector and
+ * receives the solution); returns the slot (>= 0) or an error code. */
+int h2hip_gmres_batch_submit(h2hip_gmres_batch *G, const void *d_b,
+                             void *d_x, double eps, uint32_t maxiter,
+                             uint64_t tag);
+/* Advances every active solve by one chunk of steps (at most 8, never past
+ * a restart) and reports the solves that finished: their tags, steps and
+ * residual estimates (up to cap of them; *ndone counts all).  Solutions of
+ * finished solves are complete on return. */
+int h2hip_gmres_batch_run(h2hip_gmres_batch *G, uint32_t *ndone,
+                          uint64_t *tags, uint32_t *iterations,
+                          double *residuals, uint32_t cap);
+
 /* Waits for all work queued on the stream. */
 int h2hip_stream_sync(void *stream);
 

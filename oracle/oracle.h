@@ -18,6 +18,10 @@
  *   orc_h2_*             the reference's addeval_h2matrix_avector (libh2ref),
  *                        the dense product when every block is near-field,
  *                        adjointness <Hx, y> = <x, H^* y>
+ *   orc_solve_gmres      the reference's solve_gmres_avector on the same H2
+ *                        system (steps and solution), the residual bound
+ *                        |b - A x| <= eps |b| and a direct solve on random
+ *                        dense systems
  */
 #ifndef ORACLE_H
 #define ORACLE_H
@@ -133,6 +137,21 @@ int orc_h2_addeval(int is_complex, int trans, const double *alpha,
                    const uint32_t *far, const double *S, uint64_t sld,
                    uint32_t nnear, const uint32_t *near, const double *N,
                    const double *x, double *y);
+
+/* ---- GMRES ----------------------------------------------------------------- */
+
+/* y += alpha A x (the reference's addeval_t, Library/krylov.h:41, real). */
+typedef void (*orc_addeval_t)(double alpha, void *A, const double *x,
+                              double *y);
+
+/* Restarted GMRES(kmax) with Householder orthogonalisation for A x = b, x
+ * holding the start vector; returns the steps, *residual the final
+ * estimate |rhat_k|.  Restates solve_gmres_avector
+ * (Library/krylovsolvers.c:183-227) with init/step/finish_gmres
+ * (Library/krylov.c:535-770). */
+unsigned orc_solve_gmres(orc_addeval_t addeval, void *A, size_t n,
+                         const double *b, double *x, double eps,
+                         unsigned maxiter, unsigned kmax, double *residual);
 
 #ifdef __cplusplus
 }

@@ -100,7 +100,7 @@ class Oracle:
         return x, y, w, base.value
 
     @classmethod
-    def h2_addeval(cls, interp, N, x, y, alpha=1.0, trans=False):
+    def h2_addeval(cls, interp, N, x, y, alpha=1.0, trans=False, S=None):
         """y += alpha H x (H^* x) for the host arrays of an
         h2lib_amd.h2matrix.Interpolation and packed near-field blocks N
         (in the order of interp.near)."""
@@ -115,15 +115,14 @@ class Oracle:
             lib._h2sig = True
         I = interp
         cplx = np.iscomplexobj(N) or np.iscomplexobj(x) or \
-            np.iscomplexobj(alpha)
+            np.iscomplexobj(alpha) or np.iscomplexobj(S)
         dt = np.complex128 if cplx else np.float64
         N = np.ascontiguousarray(N, dtype=dt)
         xv = np.ascontiguousarray(x, dtype=dt)
         if y.dtype != dt or not y.flags.c_contiguous:
             raise TypeError("y must be a contiguous %s array" % dt)
-        S = np.ascontiguousarray(I.S, dtype=np.float64)
-        if cplx:
-            S = S.astype(np.complex128)
+        S = np.ascontiguousarray(I.S if S is None else S)
+        S = S.astype(np.complex128 if cplx else np.float64)
         a = np.array([np.real(alpha), np.imag(alpha)], dtype=np.float64)
 
         class CB(C.Structure):
@@ -143,6 +142,36 @@ class Oracle:
         if rc != 0:
             raise ValueError("orc_h2_addeval rejected its input")
         return y
+
+
+ADDEVAL_T = C.CFUNCTYPE(None, C.c_double, C.c_void_p,
+                        C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def oracle_gmres(matvec, n, b, x0, eps, maxiter, kmax):
+    """The restated GMRES with y += alpha A x computed by the Python
+    callable matvec(x) -> A x (float64 arrays); returns x, steps, residual."""
+    lib = Oracle.lib()
+    if not hasattr(lib, "_gmsig"):
+        lib.orc_solve_gmres.restype = C.c_uint
+        lib.orc_solve_gmres.argtypes = [ADDEVAL_T, C.c_void_p, C.c_size_t,
+                                        C.c_void_p, C.c_void_p, C.c_double,
+                                        C.c_uint, C.c_uint,
+                                        C.POINTER(C.c_double)]
+        lib._gmsig = True
+
+    def cb(alpha, _A, xp, yp):
+        xv = np.ctypeslib.as_array(xp, shape=(n,))
+        yv = np.ctypeslib.as_array(yp, shape=(n,))
+        yv += alpha * matvec(xv.copy())
+
+    f = ADDEVAL_T(cb)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    xx = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    res = C.c_double()
+    it = lib.orc_solve_gmres(f, None, n, _ptr(bb), _ptr(xx), eps, maxiter,
+                             kmax, C.byref(res))
+    return xx, it, res.value
 
 
 def oracle_nearfield_packed(ob, blocks, ridx_all, cidx_all):

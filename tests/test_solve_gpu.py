@@ -73,3 +73,34 @@ def test_device_projection_matches_host(kind, q):
     prob.project_device(2.0 * x0, d, kind)
     want2 = prob.project(2.0 * x0, kind)
     assert np.abs(d.download() - want2).max() < 1e-12 * np.abs(want2).max()
+
+
+@pytest.mark.parametrize("kmax", [5, 30])
+def test_device_gmres_matches_restatement(kmax):
+    # device GMRES on an H2-matrix built from host arrays against the CPU
+    # restatement of the reference's GMRES with the restated product, on
+    # the same matrix: same steps (up to one), solutions to 1e-8
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix, Interpolation
+    from h2lib_amd.solve import solve_gmres
+    from oracle_ffi import (Oracle, OracleBem, oracle_gmres,
+                            oracle_nearfield_packed)
+    mesh = G.sphere(8)
+    rt = G.ClusterTree(mesh, 1, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    I = Interpolation(mesh, rt, rt, bt, 1, 1, False, 3, 2)
+    ob = OracleBem(0, 0, 1, 1, mesh.x, mesh.t, q=2, q2=4)
+    N = oracle_nearfield_packed(ob, bt.near_block_list(), rt.idx, rt.idx)
+    H = H2Matrix.from_arrays(I, N, rt=rt)
+    n = rt.n
+    b = np.random.default_rng(6).standard_normal(n)
+    want, it_o, _ = oracle_gmres(
+        lambda v: Oracle.h2_addeval(I, N, v, np.zeros(n)), n, b, np.zeros(n),
+        1e-10, 2000, kmax)
+    d_b = h.DeviceArray.from_host(b)
+    d_x = h.DeviceArray.from_host(np.zeros(n))
+    it, res = solve_gmres(H, d_b, d_x, 1e-10, 2000, kmax)
+    print("gmres steps device", it, "restatement", it_o, flush=True)
+    assert abs(it - it_o) <= 1
+    assert np.abs(d_x.download() - want).max() < 1e-8 * np.abs(want).max()
