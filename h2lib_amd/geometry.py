@@ -64,6 +64,8 @@ def bem():
         lib.h2b_sphere_surface.argtypes = [C.c_uint32]
         lib.h2b_cube_surface.restype = sp
         lib.h2b_cube_surface.argtypes = [C.c_uint32]
+        lib.h2b_cylinder_surface.restype = sp
+        lib.h2b_cylinder_surface.argtypes = [C.c_uint32]
         lib.h2b_surface_from_arrays.restype = sp
         lib.h2b_surface_from_arrays.argtypes = [C.c_uint32, C.c_void_p,
                                                 C.c_uint32, C.c_void_p]
@@ -133,6 +135,12 @@ def sphere(split):
 def cube(split):
     """Cube [-1, 1]^3 refined to 12 split^2 triangles."""
     return Mesh(bem().h2b_cube_surface(split))
+
+
+def cylinder(split):
+    """Closed cylinder of radius 1, |x| <= 4, refined to 40 split^2
+    triangles."""
+    return Mesh(bem().h2b_cylinder_surface(split))
 
 
 def mesh_from_arrays(x, t):

@@ -21,7 +21,7 @@ typedef struct {
   const double (*x)[3];
   const uint32_t (*e)[2];
   const uint32_t (*t)[3];
-  int project; /* 1: project onto the unit sphere */
+  int project; /* 1: project onto the unit sphere, 2: onto the cylinder */
 } macro;
 
 static const double sphere_x[6][3] = {
@@ -45,6 +45,40 @@ static const uint32_t cube_t[12][3] = {
     {1, 0, 3}, {3, 0, 2}, {4, 5, 7}, {4, 7, 6}, {3, 2, 6}, {3, 6, 7},
     {4, 0, 1}, {4, 1, 5}, {0, 6, 2}, {4, 6, 0}, {1, 3, 7}, {1, 7, 5}};
 
+/* The cylinder of radius 1 along the x axis from -4 to 4, closed by two
+ * discs (Library/macrosurface3d.c:431-700): 22 vertices on five circles,
+ * 60 edges, 40 triangles. */
+#define HALF_SQRT M_SQRT1_2 /* sqrt(0.5), correctly rounded */
+static const double cyl_x[22][3] = {
+    {-4.0, 0.0, 1.0}, {-4.0, 1.0, 0.0}, {-4.0, 0.0, -1.0}, {-4.0, -1.0, 0.0},
+    {-4.0, 0.0, 0.0},
+    {-2.0, HALF_SQRT, HALF_SQRT}, {-2.0, HALF_SQRT, -HALF_SQRT},
+    {-2.0, -HALF_SQRT, -HALF_SQRT}, {-2.0, -HALF_SQRT, HALF_SQRT},
+    {0.0, 0.0, 1.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, -1.0}, {0.0, -1.0, 0.0},
+    {2.0, HALF_SQRT, HALF_SQRT}, {2.0, HALF_SQRT, -HALF_SQRT},
+    {2.0, -HALF_SQRT, -HALF_SQRT}, {2.0, -HALF_SQRT, HALF_SQRT},
+    {4.0, 0.0, 1.0}, {4.0, 1.0, 0.0}, {4.0, 0.0, -1.0}, {4.0, -1.0, 0.0},
+    {4.0, 0.0, 0.0}};
+static const uint32_t cyl_e[60][2] = {
+    {0, 1},   {1, 4},   {4, 0},   {4, 2},   {2, 1},   {2, 3},   {3, 4},
+    {0, 3},   {0, 5},   {0, 8},   {1, 5},   {1, 6},   {2, 6},   {2, 7},
+    {3, 7},   {3, 8},   {5, 9},   {5, 10},  {6, 10},  {6, 11},  {7, 11},
+    {7, 12},  {8, 9},   {8, 12},  {0, 9},   {1, 10},  {2, 11},  {3, 12},
+    {17, 9},  {18, 10}, {19, 11}, {20, 12}, {13, 9},  {13, 10}, {14, 10},
+    {14, 11}, {15, 11}, {15, 12}, {16, 9},  {16, 12}, {17, 13}, {17, 16},
+    {18, 13}, {18, 14}, {19, 14}, {19, 15}, {20, 15}, {20, 16}, {17, 18},
+    {18, 21}, {21, 17}, {21, 19}, {19, 18}, {19, 20}, {20, 21}, {17, 20},
+    {5, 13},  {6, 14},  {7, 15},  {8, 16}};
+static const uint32_t cyl_t[40][3] = {
+    {0, 1, 4},    {1, 2, 4},    {2, 3, 4},    {0, 4, 3},    {0, 5, 1},
+    {1, 6, 2},    {2, 7, 3},    {3, 8, 0},    {0, 9, 5},    {0, 8, 9},
+    {1, 5, 10},   {1, 10, 6},   {2, 6, 11},   {2, 11, 7},   {3, 7, 12},
+    {3, 12, 8},   {5, 9, 13},   {5, 13, 10},  {6, 10, 14},  {6, 14, 11},
+    {7, 11, 15},  {7, 15, 12},  {8, 12, 16},  {8, 16, 9},   {9, 17, 13},
+    {9, 16, 17},  {10, 13, 18}, {10, 18, 14}, {11, 14, 19}, {11, 19, 15},
+    {12, 15, 20}, {12, 20, 16}, {13, 17, 18}, {14, 18, 19}, {15, 19, 20},
+    {16, 20, 17}, {17, 21, 18}, {18, 21, 19}, {19, 21, 20}, {17, 20, 21}};
+
 /* Point of macro triangle i at local coordinates (a, b). */
 static void
 macro_point(const macro *m, uint32_t i, double a, double b, double *out)
@@ -54,11 +88,26 @@ macro_point(const macro *m, uint32_t i, double a, double b, double *out)
   const double *p2 = m->x[m->t[i][2]];
   for (int d = 0; d < 3; d++)
     out[d] = p0[d] * (1.0 - a - b) + p1[d] * a + p2[d] * b;
-  if (m->project) {
+  if (m->project == 1) {
     double r = sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
     out[0] /= r;
     out[1] /= r;
     out[2] /= r;
+  } else if (m->project == 2) {
+    /* cylinder_parametrization, Library/macrosurface3d.c:394-428: the
+     * mantle is projected onto the unit circle in (y, z); on the end discs
+     * the L1 ball is mapped onto the L2 ball */
+    const double n2 = sqrt(out[1] * out[1] + out[2] * out[2]);
+    if (fabs(out[0]) != 4.0) {
+      out[1] /= n2;
+      out[2] /= n2;
+    } else {
+      const double n1 = fabs(out[1]) + fabs(out[2]);
+      if (n2 > 0.0) {
+        out[1] = out[1] * n1 / n2;
+        out[2] = out[2] * n1 / n2;
+      }
+    }
   }
 }
 
@@ -256,6 +305,13 @@ h2b_surface *
 h2b_cube_surface(uint32_t split)
 {
   macro m = {8, 18, 12, cube_x, cube_e, cube_t, 0};
+  return refine(&m, split);
+}
+
+h2b_surface *
+h2b_cylinder_surface(uint32_t split)
+{
+  macro m = {22, 60, 40, cyl_x, cyl_e, cyl_t, 2};
   return refine(&m, split);
 }
 

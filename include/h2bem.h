@@ -38,6 +38,9 @@ h2b_surface *h2b_sphere_surface(uint32_t split);
 /* Refinement of the cube [-1,1]^3: 12 split^2 triangles
  * (new_cube_macrosurface3d, Library/macrosurface3d.c:390). */
 h2b_surface *h2b_cube_surface(uint32_t split);
+/* Cylinder of radius 1 along the x axis, |x| <= 4, closed by discs
+ * (new_cylinder_macrosurface3d, Library/macrosurface3d.c:431-700). */
+h2b_surface *h2b_cylinder_surface(uint32_t split);
 /* Surface from arrays; computes n, g, hmin, hmax (prepare_surface3d,
  * Library/surface3d.c:51-108). */
 h2b_surface *h2b_surface_from_arrays(uint32_t vertices, const double *x,
