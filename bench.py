@@ -113,6 +113,198 @@ def bench_dense(h, stream, reps=20):
             "host_matrix_ms": host_s * 1e3}
 
 
+def time_plan(h, ops, plan, outs, stream, steps):
+    """Mean milliseconds of one pass of plan.execute over all operators."""
+    for op, out in zip(ops, outs):
+        plan.execute(op, out.ptr, stream.handle)
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(steps):
+        for op, out in zip(ops, outs):
+            plan.execute(op, out.ptr, stream.handle)
+    e1.record(stream)
+    stream.sync()
+    return e0.elapsed_ms(e1) / steps
+
+
+def entry_errors(mp, f64):
+    """Error of mixed-precision entries against the FP64 ones: the largest
+    |difference| relative to the largest |entry|, and the Frobenius-norm
+    ratio."""
+    d = np.abs(mp - f64)
+    return {"max_rel_to_max": float(d.max() / np.abs(f64).max()),
+            "frobenius_rel": float(np.linalg.norm(mp - f64) /
+                                   np.linalg.norm(f64))}
+
+
+def bench_mixed(h, ops, plan, outs, stream, steps):
+    """The same near-field blocks with mixed-precision kernel evaluations
+    (FP32 kernel values, FP64 geometry differences and outer sums), timed
+    like the FP64 steps, with the entry error against the FP64 result."""
+    mp_outs = [h.DeviceArray(plan.entries, o.dtype) for o in outs]
+    for op in ops:
+        op.set_precision(1)
+    try:
+        ms = time_plan(h, ops, plan, mp_outs, stream, steps)
+        errs = [entry_errors(m.download(), o.download())
+                for m, o in zip(mp_outs, outs)]
+    finally:
+        for op in ops:
+            op.set_precision(0)
+    return {"entries_per_s": plan.entries * len(ops) / (ms * 1e-3),
+            "ms_per_step": ms,
+            "entry_err_vs_fp64": {"slp": errs[0], "dlp": errs[1]}}
+
+
+def bench_helmholtz_mixed(h, args, stream):
+    """Helmholtz single layer, piecewise constant basis, on the unit sphere
+    split 128 (131072 triangles): the near field of the strict block tree
+    (leaf size 64, eta 2) in FP64 and in mixed precision at k = 2, 4, 8,
+    with entries/s of both and the entry error of mixed against FP64."""
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(args.helm_split)
+    tree = G.ClusterTree(mesh, 0, 64)
+    nb = G.BlockTree(tree, tree, 2.0, 0).near_block_list()
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    plan = h.NearfieldPlan(gr, 0, 0, tree.idx, tree.idx, nb)
+    out64 = h.DeviceArray(plan.entries, np.complex128)
+    out32 = h.DeviceArray(plan.entries, np.complex128)
+    res = {"workload": ("Helmholtz SLP cc near field, unit sphere %d "
+                        "triangles, clf=64 eta=2, q=2 q2=4" %
+                        mesh.triangles),
+           "entries": int(plan.entries), "k": {}}
+    for k in (2.0, 4.0, 8.0):
+        op64 = h.new_slp_helmholtz_bem3d(k, gr, 2, 4, 0, 0)
+        op32 = h.new_slp_helmholtz_bem3d(k, gr, 2, 4, 0, 0)
+        op32.set_precision(1)
+        ms64 = time_plan(h, [op64], plan, [out64], stream, args.helm_steps)
+        ms32 = time_plan(h, [op32], plan, [out32], stream, args.helm_steps)
+        res["k"]["%g" % k] = {
+            "fp64_entries_per_s": plan.entries / (ms64 * 1e-3),
+            "mixed_entries_per_s": plan.entries / (ms32 * 1e-3),
+            "fp64_ms": ms64, "mixed_ms": ms32,
+            "entry_err_vs_fp64": entry_errors(out32.download(),
+                                              out64.download())}
+        log("bench: helmholtz k=%g fp64 %.2f ms mixed %.2f ms" %
+            (k, ms64, ms32))
+    return res
+
+
+def bench_batched(h, args, stream):
+    """Many meshes at once: args.batch_meshes meshes drawn by seed from the
+    sphere / cube / cylinder macrosurfaces (triangles log-uniform in
+    [512, 32768], rotation uniform on SO(3), scale uniform in [0.5, 2]);
+    Laplace SLP, piecewise constant basis, q=2 q2=4, FP64.  The autotuner
+    picks each size class's leaf size (candidates 32..128, by measured host
+    build + near field + 50 H2 products) and the launch schedule; all
+    near-field blocks of all meshes are assembled by one launch set (one
+    regular launch, one per singular class), timed over args.steps; then
+    every mesh's H2-matrix (interpolation order 3) solves V s = 1 (the
+    charge of a unit potential) by GMRES(30) to 1e-8 with one captured HIP
+    graph per Arnoldi step of all meshes, and again with direct launches."""
+    from h2lib_amd.autotune import Autotuner
+    from h2lib_amd.batch import MultiMesh, random_meshes
+    from h2lib_amd.solve import GmresMulti
+    t0 = time.perf_counter()
+    specs = random_meshes(args.batch_meshes, args.batch_seed)
+    t_mesh = time.perf_counter() - t0
+    tuner = Autotuner(stream, m=3, candidates=(32, 48, 64, 96, 128),
+                      matvecs=50)
+    t0 = time.perf_counter()
+    clfs = [tuner.leaf_size(sp.mesh) for sp in specs]
+    t_tune_leaf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mm = MultiMesh(specs, clfs, basis=0)
+    t_plan = time.perf_counter() - t0
+    op = h.new_slp_laplace_bem3d(mm.gr, 2, 4, 0, 0)
+    out = h.DeviceArray(mm.entries)
+    best = tuner.tune_schedule("batched near field", mm.plan, op, out.ptr)
+    ms = time_plan(h, [op], mm.plan, [out], stream, args.steps)
+    evals, pairs = mm.plan.work(op)
+    # the same blocks as one plan per mesh, launched one after another
+    per_mesh = []
+    for i, sp in enumerate(specs):
+        m = sp.mesh
+        gr = h.Surface3d(m.x, m.t, m.n, m.g)
+        o1 = h.new_slp_laplace_bem3d(gr, 2, 4, 0, 0)
+        p1 = h.NearfieldPlan(gr, 0, 0, mm.trees[i].idx, mm.trees[i].idx,
+                             mm.btrees[i].near_block_list())
+        per_mesh.append((gr, o1, p1, h.DeviceArray(p1.entries)))
+    e0, e1 = h.Event(), h.Event()
+    for _, o1, p1, d1 in per_mesh:
+        p1.execute(o1, d1.ptr, stream.handle)
+    stream.sync()
+    e0.record(stream)
+    for _ in range(args.steps):
+        for _, o1, p1, d1 in per_mesh:
+            p1.execute(o1, d1.ptr, stream.handle)
+    e1.record(stream)
+    stream.sync()
+    ms_sep = e0.elapsed_ms(e1) / args.steps
+    del per_mesh
+
+    t0 = time.perf_counter()
+    mats = mm.h2matrices(out, 3)
+    t_h2 = time.perf_counter() - t0
+    G = GmresMulti(mats, kmax=30, stream=stream)
+    # b_i = integral of 1 over triangle i = g_i / 2 (original DOF order)
+    bs = [h.DeviceArray.from_host(0.5 * sp.mesh.g) for sp in specs]
+    gm = {}
+    for use_graph in (True, False):
+        xs = [h.DeviceArray(H.n) for H in mats]
+        for x in xs:
+            x.zero()
+        stream.sync()
+        t0 = time.perf_counter()
+        it, res = G.solve(bs, xs, 1e-8, args.batch_maxiter,
+                          use_graph=use_graph)
+        dt = time.perf_counter() - t0
+        gm["graph" if use_graph else "direct"] = {
+            "s": dt, "steps_max": int(it.max()),
+            "steps_median": int(np.median(it)),
+            "ms_per_step_all_meshes": 1e3 * dt / max(1, int(it.max()))}
+        if use_graph:
+            # true relative residuals |b - V s| / |b| per mesh
+            rel = []
+            for H, b, x in zip(mats, bs, xs):
+                r = h.DeviceArray(H.n)
+                h.hip().h2hip_memcpy(r._p, b._p, b.nbytes, 2)
+                H.addeval(-1.0, x, r, stream)
+                stream.sync()
+                rel.append(float(np.linalg.norm(r.download()) /
+                                 np.linalg.norm(b.download())))
+            gm["per_mesh"] = [[sp.kind, sp.triangles, round(sp.scale, 3),
+                               c, int(k), float("%.3g" % e)]
+                              for sp, c, k, e in zip(specs, clfs, it, rel)]
+            gm["per_mesh_fields"] = ["kind", "triangles", "scale", "clf",
+                                     "steps", "true_rel_residual"]
+    gm["graph_captures"] = G.graph_builds
+    gm["speedup_graph_vs_direct"] = gm["direct"]["s"] / gm["graph"]["s"]
+    tri = [sp.triangles for sp in specs]
+    return {
+        "workload": ("%d meshes (seed %d): sphere/cube/cylinder, %d..%d "
+                     "triangles (total %d), random rotation and scale; "
+                     "Laplace SLP cc, q=2 q2=4, FP64" %
+                     (len(specs), args.batch_seed, min(tri), max(tri),
+                      sum(tri))),
+        "kinds": {k: sum(sp.kind == k for sp in specs)
+                  for k in ("sphere", "cube", "cylinder")},
+        "leaf_sizes": {str(c): clfs.count(c) for c in sorted(set(clfs))},
+        "schedule": {"sorted_tiles": int(best[0]), "persistent": int(best[1]),
+                     "ms": tuner.schedules["batched near field"]["ms"]},
+        "entries": int(mm.entries),
+        "kernel_evaluations": float(evals.sum()),
+        "entries_per_s": mm.entries / (ms * 1e-3), "ms": ms,
+        "entries_per_s_one_plan_per_mesh": mm.entries / (ms_sep * 1e-3),
+        "ms_one_plan_per_mesh": ms_sep,
+        "host_s": {"meshes": t_mesh, "autotune_leaf": t_tune_leaf,
+                   "trees_and_plan": t_plan, "h2_bases": t_h2},
+        "gmres": gm,
+        "autotuner": tuner.to_json(),
+    }
+
+
 def bench_dirichlet(h, mesh, args, stream):
     """Dirichlet solves of the interior Laplace problem on the same mesh with
     the operators resident on the device: each solve projects the boundary
@@ -279,6 +471,14 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=60.0,
                     help="Poisson arrival rate of the serving run (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--helm-split", type=int, default=128,
+                    help="sphere split of the Helmholtz FP64 / mixed "
+                    "near-field comparison (0: skip it)")
+    ap.add_argument("--helm-steps", type=int, default=5)
+    ap.add_argument("--batch-meshes", type=int, default=64,
+                    help="meshes of the batched run (0: skip it)")
+    ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-maxiter", type=int, default=1000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -364,6 +564,18 @@ def main():
             counters = json.load(f)
         traffic = counters.get("bytes_per_launch")
 
+    log("bench: mixed precision, same blocks")
+    mixed = bench_mixed(h, ops, plan, outs, stream, args.steps)
+    helm = None
+    if args.helm_split > 0:
+        log("bench: Helmholtz near field, FP64 and mixed")
+        helm = bench_helmholtz_mixed(h, args, stream)
+
+    batched = None
+    if args.batch_meshes > 0:
+        log("bench: %d meshes in one launch set" % args.batch_meshes)
+        batched = bench_batched(h, args, stream)
+
     log("bench: dense 2048 x 2048 matrix")
     dense = bench_dense(h, stream)
 
@@ -420,6 +632,9 @@ def main():
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
+        "helmholtz_mixed": helm,
+        "batched": batched,
         "dirichlet": dirichlet,
         "dense": dense,
     }

@@ -52,6 +52,7 @@ def hip():
         _sig(lib, "h2hip_nfplan_entries", C.c_uint64, [vp])
         _sig(lib, "h2hip_nfplan_offsets", i32, [vp, dp])
         _sig(lib, "h2hip_nfplan_work", i32, [vp, vp, i32, dp])
+        _sig(lib, "h2hip_nfplan_set_schedule", i32, [vp, i32, i32])
         _sig(lib, "h2hip_nearfield_execute", i32, [vp, vp, i32, dp, vp])
         _sig(lib, "h2hip_stream_sync", i32, [vp])
         _sig(lib, "h2hip_malloc", i32, [C.POINTER(vp), sz])

@@ -105,6 +105,13 @@ class Bem3d:
             hip().h2hip_op_destroy(h)
             self._h = None
 
+    def set_precision(self, mode):
+        """0: FP64 kernel evaluations (default); 1: mixed precision (FP32
+        kernel values, FP64 geometry differences and outer sums)."""
+        check(hip().h2hip_op_set_precision(self._h, int(mode)),
+              "h2hip_op_set_precision")
+        self.precision = int(mode)
+
     def ndof(self, basis):
         return self.gr.vertices if basis else self.gr.triangles
 
@@ -198,6 +205,15 @@ class NearfieldPlan:
         check(hip().h2hip_nfplan_work(self._h, op.handle, int(far),
                                       _ptr(out)), "h2hip_nfplan_work")
         return out[:4], out[4:]
+
+    def set_schedule(self, sorted_tiles, persistent):
+        """Launch schedule (entries are the same up to the order of atomic
+        sums of the linear basis; constant basis bit for bit): tiles in
+        decreasing order of work, and/or the persistent regular kernel."""
+        check(hip().h2hip_nfplan_set_schedule(self._h, int(bool(sorted_tiles)),
+                                              int(bool(persistent))),
+              "h2hip_nfplan_set_schedule")
+        self.schedule = (bool(sorted_tiles), bool(persistent))
 
     def execute(self, op, d_out, stream=None, far=False):
         """Computes all blocks into the device buffer at address d_out."""

@@ -121,13 +121,22 @@ struct KArgs {
   const double *regpts;  // regular-rule points, 3 q^2 per triangle
   const double *regw;    // regular weights: w | phi0 w | phi1 w | phi2 w
   const double *gauss;   // Gauss-Legendre nodes and weights on [0, 1], q2 each
+  // mixed precision only: offsets of the regular points from point 0 of
+  // their triangle and the regular weights, in FP32
+  const float *regoff;
+  const float *regwf;
   uint32_t q2;           // order of the singular rules
+  int mp;                // 1: kernel values in FP32 (mixed precision)
   double kc;             // 1 / (4 pi)
   double kr, ki;         // e^{i k r} with k = kr - i ki
+  float krf, krev, kil2;  // FP32: kr, kr / (2 pi), ki log2(e)
   double are, aim;       // identity coefficient alpha
   double mass[9];
   int far;
   double *out;
+  // persistent schedule of the regular kernel: tile counter (nullptr: one
+  // workgroup per tile)
+  uint32_t *counter;
 };
 
 // Launches the regular tile kernel over `ntiles` tiles and then (unless
@@ -173,7 +182,9 @@ struct h2hip_op {
   h2hip::DevBuf<double> d_gauss;
   // regular rule: per-triangle points (3 * q * q doubles per triangle)
   h2hip::DevBuf<double> d_regpts;
-  h2hip::DevBuf<float> d_regpts_f;
+  // mixed precision: regular points relative to point 0 of their triangle
+  // and the regular weights, in FP32
+  h2hip::DevBuf<float> d_regoff, d_regwf;
   // regular weights: [w (q*q) | phi_0 (q*q) | phi_1 | phi_2], basis factors
   // premultiplied by the weights
   h2hip::DevBuf<double> d_regw;
@@ -197,4 +208,10 @@ struct h2hip_nfplan {
   h2hip::DevBuf<h2hip::Supp> d_supp;
   h2hip::DevBuf<h2hip::Fixup> d_fix;
   h2hip::DevBuf<h2hip::SPair> d_spairs;
+  // schedule: tiles in decreasing order of their pair count, and the
+  // persistent regular kernel with its tile counter
+  int sorted = 0, persistent = 0;
+  std::vector<h2hip::Tile> tiles0;    // plan order, kept once sorted
+  std::vector<h2hip::SPair> spairs0;
+  h2hip::DevBuf<uint32_t> d_counter;
 };

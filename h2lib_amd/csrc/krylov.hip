@@ -27,6 +27,7 @@
 // Givens rotations.
 #include <algorithm>
 #include <cmath>
+#include <deque>
 #include <vector>
 
 #include "h2hip_internal.h"
@@ -855,8 +856,400 @@ int h2hip_gmres_batch_run(h2hip_gmres_batch *G, uint32_t *ndone,
 
 }  // extern "C"
 
+// ---- many systems, one Arnoldi step of all of them per graph launch ---------
+//
+// Independent systems (typically one per mesh, each with its own H2-matrix)
+// step in lock step: they start together and step together, and a system
+// leaves when it meets its stopping rule.  The step kernels read the Krylov
+// column index k from device memory and the last one increments it, so one
+// step of every active system is a fixed sequence of launches that is
+// captured once into a HIP graph (one branch per H2-matrix, the branches
+// independent) and replayed for every step until the set of active systems
+// changes.  The product reads a copy of column k (made by the first kernel
+// of the step), so its launch arguments do not depend on k either.
+
 namespace {
+
+__global__ __launch_bounds__(NT) void mstep_prep(const int *dk,
+                                                 const double *Q, uint64_t n2,
+                                                 double *xb, double *w) {
+  const uint64_t k = (uint64_t)*dk;
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n2) {
+    xb[i] = Q[n2 * k + i];
+    w[i] = 0.0;
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void multidot_dk(
+    const double *__restrict__ Q, uint64_t ldq, const int *dk,
+    const double *__restrict__ w, uint64_t n, double *__restrict__ part,
+    uint32_t *cnt, double *out) {
+  multidot_body<C>(Q, ldq, *dk + 1, w, n, part, cnt, out);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void multiaxpy_dk(
+    const double *__restrict__ Q, uint64_t ldq, const int *dk,
+    const double *__restrict__ h, double *__restrict__ w, uint64_t n,
+    double *__restrict__ part, uint32_t *cnt, double *nrm) {
+  multiaxpy_body<C>(Q, ldq, *dk + 1, h, 1.0, w, n, part, cnt, nrm);
+}
+
+// scale_givens for column k = *dk; the residual estimate goes to err[k]
+__global__ __launch_bounds__(NT) void scale_givens_dk(
+    const double *__restrict__ w, double *__restrict__ Q, uint64_t n2,
+    const double *nrm2, const double *hA, const double *hB, LSQ s,
+    const int *dk, double *err) {
+  const int k = *dk;
+  scale_givens_body(w, Q + n2 * (uint64_t)(k + 1), n2, nrm2, hA, hB, s, k,
+                    err + k);
+}
+
+__global__ void bump_k(int *dk) { *dk += 1; }
+__global__ void set_k(int *dk, int v) { *dk = v; }
+
 }  // namespace
+
+struct h2hip_gmres_multi {
+  int C = 0, kmax = 0;
+  hipStream_t st = nullptr;
+  struct Sys {
+    const h2hip_h2matrix *H = nullptr;
+    uint64_t n = 0, n2 = 0;
+    Workspace W;
+    h2hip::DevBuf<double> xb;
+    int *dk = nullptr;       // device: Krylov column of the next step
+    double *derr = nullptr;  // device: kmax step estimates, then |r_0|
+    bool active = false;
+    int k = 0;
+    uint32_t iter = 0;
+    double normb = 0.0, error = 0.0;
+    const double *b = nullptr;
+    double *x = nullptr;
+  };
+  std::deque<Sys> sys;  // Sys holds device buffers and is not movable
+  h2hip::DevBuf<int> d_k;
+  h2hip::DevBuf<double> d_err;  // count x (kmax + 1)
+  std::vector<hipStream_t> branch;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint32_t> exec_set;  // active systems the graph was made for
+  uint64_t graph_builds = 0;
+  ~h2hip_gmres_multi() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (auto e : join) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto b : branch) (void)hipStreamDestroy(b);
+  }
+};
+
+namespace {
+
+using Multi = h2hip_gmres_multi;
+
+template <int C>
+int multi_init(Multi *G, Multi::Sys &S, hipStream_t st) {
+  const double mone[2] = {-1.0, 0.0};
+  const dim3 gdot((uint32_t)std::min<uint64_t>(NB, (S.n + NT - 1) / NT));
+  double *q0 = S.W.col(0);
+  hipLaunchKernelGGL(copy_kernel, S.W.gv, dim3(NT), 0, st, S.b, q0, S.n2);
+  int r = h2hip_addeval_h2matrix(mone, S.H, S.x, q0, st);
+  if (r) return r;
+  hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st, (const double *)q0,
+                     (uint64_t)0, 1, (const double *)q0, S.n, S.W.part.p,
+                     S.W.cnt.p, S.W.nrm);
+  hipLaunchKernelGGL(start_cycle, S.W.gv, dim3(NT), 0, st, q0, S.n2,
+                     (const double *)S.W.nrm, S.W.lsq(),
+                     S.derr + G->kmax);
+  hipLaunchKernelGGL(set_k, dim3(1), dim3(1), 0, st, S.dk, 0);
+  return H2HIP_OK;
+}
+
+template <int C>
+int multi_finish(Multi::Sys &S, int k, hipStream_t st) {
+  if (k == 0) return H2HIP_OK;
+  const dim3 gax((uint32_t)((S.n + NT - 1) / NT));
+  hipLaunchKernelGGL(back_substitute, dim3(1), dim3(64), 0, st, S.W.lsq(), k,
+                     S.W.y);
+  hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
+                     (const double *)S.W.Q.p, S.n, k, (const double *)S.W.y,
+                     1.0, S.x, S.n, S.W.part.p, S.W.cnt.p + 1,
+                     (double *)nullptr);
+  return H2HIP_OK;
+}
+
+// one Arnoldi step of system S on stream st (column index on the device)
+template <int C>
+int multi_step(Multi::Sys &S, hipStream_t st) {
+  const double one[2] = {1.0, 0.0};
+  const dim3 gax((uint32_t)((S.n + NT - 1) / NT));
+  const dim3 gdot((uint32_t)std::min<uint64_t>(NB, (S.n + NT - 1) / NT));
+  hipLaunchKernelGGL(mstep_prep, S.W.gv, dim3(NT), 0, st,
+                     (const int *)S.dk, (const double *)S.W.Q.p, S.n2,
+                     S.xb.p, S.W.w.p);
+  int r = h2hip_addeval_h2matrix(one, S.H, S.xb.p, S.W.w.p, st);
+  if (r) return r;
+  hipLaunchKernelGGL(multidot_dk<C>, gdot, dim3(NT), 0, st,
+                     (const double *)S.W.Q.p, S.n, (const int *)S.dk,
+                     (const double *)S.W.w.p, S.n, S.W.part.p, S.W.cnt.p + 2,
+                     S.W.hA);
+  hipLaunchKernelGGL(multiaxpy_dk<C>, gax, dim3(NT), 0, st,
+                     (const double *)S.W.Q.p, S.n, (const int *)S.dk,
+                     (const double *)S.W.hA, S.W.w.p, S.n, S.W.part.p,
+                     S.W.cnt.p + 1, (double *)nullptr);
+  hipLaunchKernelGGL(multidot_dk<C>, gdot, dim3(NT), 0, st,
+                     (const double *)S.W.Q.p, S.n, (const int *)S.dk,
+                     (const double *)S.W.w.p, S.n, S.W.part.p, S.W.cnt.p + 2,
+                     S.W.hB);
+  hipLaunchKernelGGL(multiaxpy_dk<C>, gax, dim3(NT), 0, st,
+                     (const double *)S.W.Q.p, S.n, (const int *)S.dk,
+                     (const double *)S.W.hB, S.W.w.p, S.n, S.W.part.p,
+                     S.W.cnt.p + 1, S.W.nrm);
+  hipLaunchKernelGGL(scale_givens_dk, S.W.gv, dim3(NT), 0, st,
+                     (const double *)S.W.w.p, S.W.Q.p, S.n2,
+                     (const double *)S.W.nrm, (const double *)S.W.hA,
+                     (const double *)S.W.hB, S.W.lsq(), (const int *)S.dk,
+                     S.derr);
+  hipLaunchKernelGGL(bump_k, dim3(1), dim3(1), 0, st, S.dk);
+  return H2HIP_OK;
+}
+
+// Captures one step of the active systems into G->exec: one branch per
+// distinct H2-matrix (systems sharing a matrix share its coefficient
+// buffers, so they step one after the other on their branch).
+template <int C>
+int multi_capture(Multi *G, const std::vector<uint32_t> &act) {
+  if (G->exec) {
+    H2HIP_CHECK(hipGraphExecDestroy(G->exec));
+    G->exec = nullptr;
+  }
+  std::vector<const h2hip_h2matrix *> mats;
+  std::vector<int> br(act.size());
+  for (size_t j = 0; j < act.size(); j++) {
+    const h2hip_h2matrix *H = G->sys[act[j]].H;
+    size_t b = 0;
+    while (b < mats.size() && mats[b] != H) b++;
+    if (b == mats.size()) mats.push_back(H);
+    br[j] = (int)b;
+  }
+  while (G->branch.size() < mats.size()) {
+    hipStream_t s;
+    hipEvent_t e;
+    H2HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    G->branch.push_back(s);
+    H2HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    G->join.push_back(e);
+  }
+  H2HIP_CHECK(hipStreamBeginCapture(G->st, hipStreamCaptureModeThreadLocal));
+  int rc = H2HIP_OK;
+  hipError_t e = hipEventRecord(G->fork, G->st);
+  for (size_t b = 0; b < mats.size() && e == hipSuccess && !rc; b++) {
+    e = hipStreamWaitEvent(G->branch[b], G->fork, 0);
+    for (size_t j = 0; j < act.size() && e == hipSuccess && !rc; j++)
+      if (br[j] == (int)b) rc = multi_step<C>(G->sys[act[j]], G->branch[b]);
+    if (e == hipSuccess && !rc) e = hipEventRecord(G->join[b], G->branch[b]);
+    if (e == hipSuccess && !rc) e = hipStreamWaitEvent(G->st, G->join[b], 0);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(G->st, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    h2hip::set_error(std::string("GMRES step capture failed: ") +
+                     hipGetErrorString(e != hipSuccess ? e : e2));
+    return H2HIP_EDEVICE;
+  }
+  const hipError_t e3 = hipGraphInstantiate(&G->exec, graph, nullptr,
+                                            nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e3 != hipSuccess) {
+    G->exec = nullptr;
+    h2hip::set_error(std::string("GMRES step graph: ") +
+                     hipGetErrorString(e3));
+    return H2HIP_EDEVICE;
+  }
+  G->exec_set = act;
+  G->graph_builds++;
+  return H2HIP_OK;
+}
+
+template <int C>
+int multi_solve(Multi *G, double eps, uint32_t maxiter, int use_graph,
+                uint32_t *iters, double *resid) {
+  const uint32_t count = (uint32_t)G->sys.size();
+  const int kmax = G->kmax;
+  const uint64_t stride = (uint64_t)kmax + 1;
+  // |b| of every system
+  std::vector<double> nb2(2 * count);
+  for (uint32_t s = 0; s < count; s++) {
+    auto &S = G->sys[s];
+    const dim3 gdot((uint32_t)std::min<uint64_t>(NB, (S.n + NT - 1) / NT));
+    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, G->st, S.b,
+                       (uint64_t)0, 1, S.b, S.n, S.W.part.p, S.W.cnt.p,
+                       S.W.nrm);
+    H2HIP_CHECK(hipMemcpyAsync(&nb2[2 * s], S.W.nrm, 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, G->st));
+  }
+  for (uint32_t s = 0; s < count; s++) {
+    int rc = multi_init<C>(G, G->sys[s], G->st);
+    if (rc) return rc;
+  }
+  std::vector<double> errs(count * stride);
+  H2HIP_CHECK(hipMemcpyAsync(errs.data(), G->d_err.p,
+                             sizeof(double) * errs.size(),
+                             hipMemcpyDeviceToHost, G->st));
+  H2HIP_CHECK(hipStreamSynchronize(G->st));
+  H2HIP_CHECK(hipGetLastError());
+  std::vector<uint32_t> act;
+  for (uint32_t s = 0; s < count; s++) {
+    auto &S = G->sys[s];
+    S.normb = std::sqrt(nb2[2 * s]);
+    S.k = 0;
+    S.iter = 0;
+    S.error = errs[s * stride + kmax];
+    // the check before the first step (Library/krylovsolvers.c:183-227)
+    S.active = S.error > eps * S.normb && S.iter + 1 != maxiter;
+    if (S.active) act.push_back(s);
+  }
+  int k = 0;
+  uint32_t iter = 0;
+  while (!act.empty()) {
+    if (k + 1 >= kmax) {
+      for (uint32_t s : act) {
+        int rc = multi_finish<C>(G->sys[s], k, G->st);
+        if (!rc) rc = multi_init<C>(G, G->sys[s], G->st);
+        if (rc) return rc;
+      }
+      k = 0;
+    }
+    uint32_t len = (uint32_t)(kmax - 1 - k);
+    if (len > (uint32_t)CHUNK) len = CHUNK;
+    if (maxiter != 0 && len > maxiter - 1 - iter) len = maxiter - 1 - iter;
+    if (use_graph && G->exec_set != act) {
+      int rc = multi_capture<C>(G, act);
+      if (rc) return rc;
+    }
+    for (uint32_t j = 0; j < len; j++) {
+      if (use_graph) {
+        H2HIP_CHECK(hipGraphLaunch(G->exec, G->st));
+      } else {
+        for (uint32_t s : act) {
+          int rc = multi_step<C>(G->sys[s], G->st);
+          if (rc) return rc;
+        }
+      }
+    }
+    H2HIP_CHECK(hipGetLastError());
+    H2HIP_CHECK(hipMemcpyAsync(errs.data(), G->d_err.p,
+                               sizeof(double) * errs.size(),
+                               hipMemcpyDeviceToHost, G->st));
+    H2HIP_CHECK(hipStreamSynchronize(G->st));
+    // replay the reference's loop per system over the steps just done
+    std::vector<uint32_t> still;
+    for (uint32_t s : act) {
+      auto &S = G->sys[s];
+      bool done = false;
+      for (uint32_t j = 0; j < len && !done; j++) {
+        S.k = k + (int)j + 1;
+        S.iter = iter + j + 1;
+        S.error = errs[s * stride + k + j];
+        done = !(S.error > eps * S.normb && S.iter + 1 != maxiter);
+      }
+      if (done) {
+        int rc = multi_finish<C>(S, S.k, G->st);
+        if (rc) return rc;
+      } else {
+        still.push_back(s);
+      }
+    }
+    act.swap(still);
+    k += (int)len;
+    iter += len;
+  }
+  H2HIP_CHECK(hipGetLastError());
+  H2HIP_CHECK(hipStreamSynchronize(G->st));
+  for (uint32_t s = 0; s < count; s++) {
+    if (iters) iters[s] = G->sys[s].iter;
+    if (resid) resid[s] = G->sys[s].error;
+  }
+  return H2HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int h2hip_gmres_multi_create(h2hip_gmres_multi **out, uint32_t count,
+                             const h2hip_h2matrix *const *H,
+                             const uint64_t *n, int is_complex, uint32_t kmax,
+                             void *stream) {
+  if (!out || count == 0 || !H || !n || !stream || kmax < 2 || kmax > MAXK) {
+    h2hip::set_error("h2hip_gmres_multi_create: invalid arguments (count > "
+                     "0, a non-default stream, kmax 2..64)");
+    return H2HIP_EINVAL;
+  }
+  for (uint32_t s = 0; s < count; s++)
+    if (!H[s] || n[s] == 0) {
+      h2hip::set_error("h2hip_gmres_multi_create: missing matrix or size");
+      return H2HIP_EINVAL;
+    }
+  Multi *G = new Multi;
+  G->C = is_complex ? 1 : 0;
+  G->kmax = (int)kmax;
+  G->st = (hipStream_t)stream;
+  for (uint32_t s = 0; s < count; s++) G->sys.emplace_back();
+  bool ok = G->d_k.alloc(count) == hipSuccess &&
+            G->d_err.alloc((uint64_t)count * (kmax + 1)) == hipSuccess &&
+            hipEventCreateWithFlags(&G->fork, hipEventDisableTiming) ==
+                hipSuccess;
+  for (uint32_t s = 0; s < count && ok; s++) {
+    auto &S = G->sys[s];
+    S.H = H[s];
+    S.n = n[s];
+    S.n2 = (G->C ? 2 : 1) * n[s];
+    S.dk = G->d_k.p + s;
+    S.derr = G->d_err.p + (uint64_t)s * (kmax + 1);
+    ok = S.W.alloc(G->C, S.n, (int)kmax, G->st) == H2HIP_OK &&
+         S.xb.alloc(S.n2) == hipSuccess;
+  }
+  if (!ok) {
+    delete G;
+    h2hip::set_error("h2hip_gmres_multi_create: device allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  *out = G;
+  return H2HIP_OK;
+}
+
+void h2hip_gmres_multi_destroy(h2hip_gmres_multi *G) { delete G; }
+
+int h2hip_gmres_multi_solve(h2hip_gmres_multi *G, const void *const *d_b,
+                            void *const *d_x, double eps, uint32_t maxiter,
+                            int use_graph, uint32_t *iterations,
+                            double *residuals) {
+  if (!G || !d_b || !d_x) return H2HIP_EINVAL;
+  for (size_t s = 0; s < G->sys.size(); s++) {
+    if (!d_b[s] || !d_x[s]) return H2HIP_EINVAL;
+    G->sys[s].b = (const double *)d_b[s];
+    G->sys[s].x = (double *)d_x[s];
+  }
+  return G->C ? multi_solve<1>(G, eps, maxiter, use_graph, iterations,
+                               residuals)
+              : multi_solve<0>(G, eps, maxiter, use_graph, iterations,
+                               residuals);
+}
+
+uint64_t h2hip_gmres_multi_graph_builds(const h2hip_gmres_multi *G) {
+  return G ? G->graph_builds : 0;
+}
+
+}  // extern "C"
 
 extern "C" int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A,
                                  int is_complex, uint64_t n, const void *d_b,

@@ -46,6 +46,19 @@ __global__ void nf_fixup_kernel(const Fixup *fix, uint32_t nfix,
 }
 
 // Quadrature points of the regular rule on every triangle.
+// FP32 offsets of the regular points from point 0 of their triangle (mixed
+// precision); point 0 itself stays in FP64 in the regular point array.
+__global__ void regoff_kernel(const double *pts, uint32_t nt, int qq,
+                              float *off) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt * (uint32_t)qq) return;
+  const uint32_t tri = i / qq;
+  const double *p0 = pts + 3 * (size_t)tri * qq;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    off[3 * (size_t)i + d] = (float)(pts[3 * (size_t)i + d] - p0[d]);
+}
+
 __global__ void regpts_kernel(const double *x, const uint32_t *t, uint32_t nt,
                               const double *x0, const double *x1, int qq,
                               double *pts) {
@@ -306,15 +319,22 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.regpts = op->d_regpts.p;
   a.regw = op->d_regw.p;
   a.gauss = op->d_gauss.p;
+  a.regoff = op->d_regoff.p;
+  a.regwf = op->d_regwf.p;
   a.q2 = op->q2;
+  a.mp = op->precision == 1 ? 1 : 0;
   a.kc = KERNEL_CONST;
   a.kr = op->k[0];
   a.ki = -op->k[1];
+  a.krf = (float)a.kr;
+  a.krev = (float)(a.kr / (2.0 * M_PI));
+  a.kil2 = (float)(a.ki * M_LOG2E);
   a.are = op->layer == H2HIP_DLP ? op->alpha[0] : 0.0;
   a.aim = op->layer == H2HIP_DLP ? op->alpha[1] : 0.0;
   for (int i = 0; i < 9; i++) a.mass[i] = op->mass[i];
   a.far = far;
   a.out = (double *)d_out;
+  a.counter = P->persistent ? P->d_counter.p : nullptr;
   const uint32_t ntiles = (uint32_t)P->tiles.size();
   uint32_t ncls[4];
   for (int c = 0; c < 4; c++) ncls[c] = (uint32_t)P->pairs[c];
@@ -520,10 +540,30 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
 void h2hip_op_destroy(h2hip_op *op) { delete op; }
 
 int h2hip_op_set_precision(h2hip_op *op, int mode) {
-  if (!op || mode < 0 || mode > 1) return H2HIP_EINVAL;
-  if (mode != 0) {
-    set_error("mixed precision is not available in this build");
+  if (!op || mode < 0 || mode > 1) {
+    set_error("h2hip_op_set_precision: invalid arguments");
     return H2HIP_EINVAL;
+  }
+  std::lock_guard<std::mutex> guard(op->lock);
+  if (mode == 1 && !op->d_regoff.p) {
+    const uint32_t qq = op->q * op->q;
+    std::vector<float> wf(op->regw.begin(), op->regw.end());
+    if (op->d_regwf.upload(wf.data(), wf.size()) != hipSuccess ||
+        op->d_regoff.alloc(3 * (size_t)qq * op->s->nt) != hipSuccess) {
+      op->d_regwf.release();
+      op->d_regoff.release();
+      set_error("h2hip_op_set_precision: device allocation failed");
+      return H2HIP_ENOMEM;
+    }
+    const uint32_t n = op->s->nt * qq;
+    hipLaunchKernelGGL(regoff_kernel, dim3((n + 255) / 256), dim3(256), 0, 0,
+                       op->d_regpts.p, op->s->nt, qq, op->d_regoff.p);
+    if (hipGetLastError() != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+      op->d_regoff.release();
+      set_error("h2hip_op_set_precision: offset kernel failed");
+      return H2HIP_EDEVICE;
+    }
   }
   op->precision = mode;
   return H2HIP_OK;
@@ -603,6 +643,43 @@ uint64_t h2hip_nfplan_entries(const h2hip_nfplan *p) {
 int h2hip_nfplan_offsets(const h2hip_nfplan *p, uint64_t *off) {
   if (!p || !off) return H2HIP_EINVAL;
   std::memcpy(off, p->off.data(), sizeof(uint64_t) * p->off.size());
+  return H2HIP_OK;
+}
+
+int h2hip_nfplan_set_schedule(h2hip_nfplan *p, int sorted, int persistent) {
+  if (!p || sorted < 0 || sorted > 1 || persistent < 0 || persistent > 1) {
+    set_error("h2hip_nfplan_set_schedule: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  if (persistent && !p->d_counter.p)
+    H2HIP_CHECK(p->d_counter.alloc(1));
+  p->persistent = persistent;
+  if (sorted == p->sorted) return H2HIP_OK;
+  if (sorted) {
+    // tiles by decreasing number of support pairs (the regular work), the
+    // singular pairs renumbered to follow their tiles
+    p->tiles0 = p->tiles;
+    p->spairs0 = p->spairs;
+    const size_t nt = p->tiles.size();
+    std::vector<uint32_t> perm(nt), pos(nt);
+    for (size_t i = 0; i < nt; i++) perm[i] = (uint32_t)i;
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) {
+      const uint64_t cx = (uint64_t)p->tiles0[x].nrsup * p->tiles0[x].ncsup;
+      const uint64_t cy = (uint64_t)p->tiles0[y].nrsup * p->tiles0[y].ncsup;
+      return cx > cy;
+    });
+    for (size_t i = 0; i < nt; i++) {
+      p->tiles[i] = p->tiles0[perm[i]];
+      pos[perm[i]] = (uint32_t)i;
+    }
+    for (auto &sp : p->spairs) sp.tile = pos[sp.tile];
+  } else {
+    p->tiles = p->tiles0;
+    p->spairs = p->spairs0;
+  }
+  H2HIP_CHECK(p->d_tiles.upload(p->tiles.data(), p->tiles.size()));
+  H2HIP_CHECK(p->d_spairs.upload(p->spairs.data(), p->spairs.size()));
+  p->sorted = sorted;
   return H2HIP_OK;
 }
 

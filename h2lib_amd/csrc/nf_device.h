@@ -22,6 +22,8 @@
 //      vertices cancel exactly.
 #pragma once
 
+#include <algorithm>
+
 #include "h2hip_internal.h"
 
 namespace h2hip {
@@ -79,6 +81,51 @@ __device__ __forceinline__ void kval(double dx, double dy, double dz,
   }
 }
 
+// The same kernel values in FP32 (mixed precision), from a difference
+// vector d that the caller formed in FP64 (singular pairs) or from an FP64
+// difference of anchor points plus FP32 offsets (regular pairs).  1/r is
+// the hardware reciprocal square root (1 ulp); the phase k r is reduced to
+// [-1/2, 1/2] revolutions before the hardware sine and cosine, so their
+// argument is always small; the damping e^{ki r} is the hardware 2^x of
+// ki log2(e) r.
+template <int KER>
+__device__ __forceinline__ void kval32(float dx, float dy, float dz, float nx,
+                                       float ny, float nz, float krf,
+                                       float krev, float kil2, float &re,
+                                       float &im) {
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float rs = __builtin_amdgcn_rsqf(r2);
+  if constexpr (KER == K_LAP_SLP) {
+    re = rs;
+    im = 0.0f;
+  } else if constexpr (KER == K_LAP_DLP) {
+    re = fmaf(dx, nx, fmaf(dy, ny, dz * nz)) * (rs * rs * rs);
+    im = 0.0f;
+  } else {
+    const float r = r2 * rs;
+    float rev = r * krev;
+    rev -= __builtin_rintf(rev);
+    const float s = __builtin_amdgcn_sinf(rev);
+    const float c = __builtin_amdgcn_cosf(rev);
+    float amp;
+    if constexpr (KER == K_HELM_SLP)
+      amp = rs;
+    else
+      amp = fmaf(dx, nx, fmaf(dy, ny, dz * nz)) * (rs * rs * rs);
+    if (kil2 != 0.0f) amp *= __builtin_amdgcn_exp2f(kil2 * r);
+    if constexpr (KER == K_HELM_SLP) {
+      re = amp * c;
+      im = amp * s;
+    } else {
+      const float kr_r = krf * r;
+      re = fmaf(kr_r, s, c) * amp;
+      im = fmaf(-c, kr_r, s) * amp;
+    }
+  }
+}
+
+static __global__ void zero_counter(uint32_t *c) { *c = 0u; }
+
 __device__ __forceinline__ uint32_t loc_of(uint32_t packed, int l) {
   return (packed >> (8 * l)) & 0xffu;
 }
@@ -89,8 +136,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int KER, int BR, int BC, int Q>
-__global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
+// One tile of the regular kernel, run by all NTHREADS threads of a
+// workgroup.
+template <int KER, int BR, int BC, int Q, bool MP>
+__device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
   constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
   constexpr bool CC = (BR == 0) && (BC == 0);
@@ -102,7 +151,7 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
   __shared__ double acc_im[(CC || !CPLX) ? 1 : TILE * TILE];
 
   const int tid = threadIdx.x;
-  const Tile T = a.tiles[blockIdx.x];
+  const Tile T = a.tiles[tile];
   const uint32_t nR = T.nrsup, nC = T.ncsup;
   const Supp *RS = a.supp + T.rsup;
   const Supp *CS = a.supp + T.csup;
@@ -143,7 +192,62 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
 #pragma unroll
       for (int k = 0; k < NC; k++) Sre[l][k] = Sim[l][k] = 0.0;
     double f2 = 0.0;
-    if (!shared) {
+    if (MP && !shared) {
+      // Mixed precision: x_i - y_b = (P_t,i - P_s,0) - (P_s,b - P_s,0), the
+      // first difference in FP64 (then rounded), the offsets of the column
+      // points in FP32; kernel values and the sums over i in FP32, the sums
+      // over b in FP64.
+      const bool swapgeo = (BR && !BC) && a.far;
+      const uint32_t tt = swapgeo ? sc.tri : sr.tri;
+      const uint32_t ts = swapgeo ? sr.tri : sc.tri;
+      const double *__restrict__ Pt = a.regpts + (size_t)tt * (3 * QQ);
+      const double *__restrict__ Ps = a.regpts + (size_t)ts * (3 * QQ);
+      const float *__restrict__ Os = a.regoff + (size_t)ts * (3 * QQ);
+      const float *__restrict__ WF = a.regwf;
+      const double sx0 = Ps[0], sy0 = Ps[1], sz0 = Ps[2];
+      float ptx[QQ], pty[QQ], ptz[QQ];
+#pragma unroll
+      for (int i = 0; i < QQ; i++) {
+        ptx[i] = (float)(Pt[3 * i] - sx0);
+        pty[i] = (float)(Pt[3 * i + 1] - sy0);
+        ptz[i] = (float)(Pt[3 * i + 2] - sz0);
+      }
+      float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+      if constexpr (DLP) {
+        nx = (float)a.n[3 * sc.tri];
+        ny = (float)a.n[3 * sc.tri + 1];
+        nz = (float)a.n[3 * sc.tri + 2];
+      }
+      f2 = a.kc * a.g[sr.tri] * a.g[sc.tri];
+#pragma unroll
+      for (int b = 0; b < QQ; b++) {
+        const float qx = Os[3 * b], qy = Os[3 * b + 1], qz = Os[3 * b + 2];
+        float Ure[NR], Uim[NR];
+#pragma unroll
+        for (int l = 0; l < NR; l++) Ure[l] = Uim[l] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < QQ; i++) {
+          float vre, vim;
+          kval32<KER>(ptx[i] - qx, pty[i] - qy, ptz[i] - qz, nx, ny, nz,
+                      a.krf, a.krev, a.kil2, vre, vim);
+#pragma unroll
+          for (int l = 0; l < NR; l++) {
+            const float wr = BR ? WF[QQ * (1 + l) + i] : WF[i];
+            Ure[l] = fmaf(wr, vre, Ure[l]);
+            if constexpr (CPLX) Uim[l] = fmaf(wr, vim, Uim[l]);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          const double wc = BC ? W[QQ * (1 + k) + b] : W[b];
+#pragma unroll
+          for (int l = 0; l < NR; l++) {
+            Sre[l][k] = fma(wc, (double)Ure[l], Sre[l][k]);
+            if constexpr (CPLX) Sim[l][k] = fma(wc, (double)Uim[l], Sim[l][k]);
+          }
+        }
+      }
+    } else if (!MP && !shared) {
       // The reference's lc far integrator maps the first rule coordinates
       // onto the column triangle (Library/bem3d.c:2637-2649); kept as is.
       const bool swapgeo = (BR && !BC) && a.far;
@@ -241,6 +345,30 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
   }
 }
 
+template <int KER, int BR, int BC, int Q, bool MP>
+__global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
+  regular_tile<KER, BR, BC, Q, MP>(a, blockIdx.x);
+}
+
+// Persistent form: a grid sized to the resident workgroups takes tiles from
+// a counter (zeroed before the launch) until all ntiles are taken; every
+// workgroup leaves the loop once the counter has passed ntiles.
+template <int KER, int BR, int BC, int Q, bool MP>
+__global__ __launch_bounds__(NTHREADS) void nf_regular_persistent(
+    KArgs a, uint32_t ntiles, uint32_t *counter) {
+  __shared__ uint32_t cur;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = cur;
+    // every thread has read cur (and finished the previous tile's LDS
+    // stores) before thread 0 takes the next one
+    __syncthreads();
+    if (t >= ntiles) break;
+    regular_tile<KER, BR, BC, Q, MP>(a, t);
+  }
+}
+
 // Sauter-Schwab points of one tensor tuple (g_i, g_j, g_k, g_l) with
 // weights (w_i, w_j, w_k, w_l): the 2 (vertex) or 6 (edge, identical)
 // subregion points (tx, sx; ty, sy) and their common weight, exactly as the
@@ -318,7 +446,7 @@ __device__ __forceinline__ double row_sum(double v) {
 // index pairs (k, l) = (s / q2, s % q2), (s + 16) / q2, ... and loops over
 // (i, j); the quadrature points are formed on the fly from the Gauss nodes
 // (a.gauss = [g | w], q2 each), so the kernel reads no rule tables.
-template <int KER, int BR, int BC, int CLS, int Q2>
+template <int KER, int BR, int BC, int CLS, int Q2, bool MP>
 __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
     KArgs a, const SPair *__restrict__ list, uint32_t npairs) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
@@ -375,7 +503,17 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
       d[k] = fma(-F2[k], sy,
                  fma(-F1[k], ty, fma(E2[k], sx, fma(E1[k], tx, D0[k]))));
     double vre, vim;
-    kval<KER>(d[0], d[1], d[2], nx, ny, nz, a.kr, a.ki, vre, vim);
+    if constexpr (MP) {
+      // mixed precision: the Sauter-Schwab points and d in FP64, the kernel
+      // value in FP32, the moment sums in FP64
+      float fre, fim;
+      kval32<KER>((float)d[0], (float)d[1], (float)d[2], (float)nx,
+                  (float)ny, (float)nz, a.krf, a.krev, a.kil2, fre, fim);
+      vre = fre;
+      vim = fim;
+    } else {
+      kval<KER>(d[0], d[1], d[2], nx, ny, nz, a.kr, a.ki, vre, vim);
+    }
     double ure[NR], uim[NR];
     ure[0] = wq * vre;
     uim[0] = wq * vim;
@@ -536,34 +674,54 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   }
 }
 
-// Launches the regular tile kernel over `ntiles` tiles and then (unless
-// far) one singular launch per rule class; sp holds the pairs sorted
-// identical, edge, vertex with ncls[c] pairs of class c.
-template <int KER, int BR, int BC>
-int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
-                 const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
-  void (*reg)(KArgs) = nullptr;
-  switch (q) {
-    case 1: reg = nf_regular_kernel<KER, BR, BC, 1>; break;
-    case 2: reg = nf_regular_kernel<KER, BR, BC, 2>; break;
-    case 3: reg = nf_regular_kernel<KER, BR, BC, 3>; break;
-    case 4: reg = nf_regular_kernel<KER, BR, BC, 4>; break;
-    default: return H2HIP_EINVAL;
-  }
-  if (ntiles)
+// The launches of one precision (MP: mixed).
+template <int KER, int BR, int BC, bool MP>
+int launch_bases_p(uint32_t q, const KArgs &a, uint32_t ntiles,
+                   const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
+  if (ntiles && a.counter) {
+    void (*reg)(KArgs, uint32_t, uint32_t *) = nullptr;
+    switch (q) {
+      case 1: reg = nf_regular_persistent<KER, BR, BC, 1, MP>; break;
+      case 2: reg = nf_regular_persistent<KER, BR, BC, 2, MP>; break;
+      case 3: reg = nf_regular_persistent<KER, BR, BC, 3, MP>; break;
+      case 4: reg = nf_regular_persistent<KER, BR, BC, 4, MP>; break;
+      default: return H2HIP_EINVAL;
+    }
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reg, NTHREADS,
+                                                     0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess)
+      return H2HIP_EDEVICE;
+    const uint32_t resident = (uint32_t)std::max(1, per_cu) * (uint32_t)ncu;
+    const uint32_t grid = std::min(ntiles, std::max(1u, resident));
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, stream, a.counter);
+    hipLaunchKernelGGL(reg, dim3(grid), dim3(NTHREADS), 0, stream, a, ntiles,
+                       a.counter);
+  } else if (ntiles) {
+    void (*reg)(KArgs) = nullptr;
+    switch (q) {
+      case 1: reg = nf_regular_kernel<KER, BR, BC, 1, MP>; break;
+      case 2: reg = nf_regular_kernel<KER, BR, BC, 2, MP>; break;
+      case 3: reg = nf_regular_kernel<KER, BR, BC, 3, MP>; break;
+      case 4: reg = nf_regular_kernel<KER, BR, BC, 4, MP>; break;
+      default: return H2HIP_EINVAL;
+    }
     hipLaunchKernelGGL(reg, dim3(ntiles), dim3(NTHREADS), 0, stream, a);
+  }
   if (a.far) return H2HIP_OK;
   constexpr uint32_t PPB = NTHREADS / 16;  // pairs per workgroup
   const uint32_t n3 = ncls[CLS_IDENTICAL], n2 = ncls[CLS_EDGE],
                  n1 = ncls[CLS_VERTEX];
   using SK = void (*)(KArgs, const SPair *, uint32_t);
   const bool q4 = a.q2 == 4;
-  const SK k3 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 4>
-                   : nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 0>;
-  const SK k2 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_EDGE, 4>
-                   : nf_singular_kernel<KER, BR, BC, CLS_EDGE, 0>;
-  const SK k1 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 4>
-                   : nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 0>;
+  const SK k3 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 4, MP>
+                   : nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 0, MP>;
+  const SK k2 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_EDGE, 4, MP>
+                   : nf_singular_kernel<KER, BR, BC, CLS_EDGE, 0, MP>;
+  const SK k1 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 4, MP>
+                   : nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 0, MP>;
   if (n3)
     hipLaunchKernelGGL(k3, dim3((n3 + PPB - 1) / PPB), dim3(NTHREADS), 0,
                        stream, a, sp, n3);
@@ -574,6 +732,19 @@ int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
     hipLaunchKernelGGL(k1, dim3((n1 + PPB - 1) / PPB), dim3(NTHREADS), 0,
                        stream, a, sp + n3 + n2, n1);
   return H2HIP_OK;
+}
+
+// Launches the regular tile kernel over `ntiles` tiles and then (unless
+// far) one singular launch per rule class; sp holds the pairs sorted
+// identical, edge, vertex with ncls[c] pairs of class c.  a.mp selects the
+// mixed-precision kernels.
+template <int KER, int BR, int BC>
+int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
+                 const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
+  return a.mp ? launch_bases_p<KER, BR, BC, true>(q, a, ntiles, sp, ncls,
+                                                  stream)
+              : launch_bases_p<KER, BR, BC, false>(q, a, ntiles, sp, ncls,
+                                                   stream);
 }
 
 template <int KER>

@@ -134,6 +134,7 @@ class H2Matrix:
                           op.layer == DLP, m, op.q_regular)
         self.interp = I
         self.k = I.k
+        self.n = rt.n
         h = C.c_void_p()
         check(hip().h2hip_h2matrix_create(
             C.byref(h), int(op.is_complex), rt.nnodes, _ptr(I.rtree),
@@ -161,7 +162,8 @@ class H2Matrix:
                     is_complex=False):
         """A device H2-matrix from host arrays: the cluster bases of an
         Interpolation, near-field blocks packed in the order of interp.near
-        and couplings (default: interp.S), real or complex."""
+        (None: left unset, e.g. for copy_nearfield_from_device) and
+        couplings (default: interp.S), real or complex."""
         _sigs()
         self = cls.__new__(cls)
         self.op = None
@@ -169,6 +171,7 @@ class H2Matrix:
         self.nearplan = None
         self.interp = I = interp
         self.k = I.k
+        self.n = len(I.ridx)
         self.dtype = np.complex128 if is_complex else np.float64
         h = C.c_void_p()
         check(hip().h2hip_h2matrix_create(
@@ -188,10 +191,19 @@ class H2Matrix:
         S = np.ascontiguousarray(S, dtype=self.dtype)
         check(hip().h2hip_h2matrix_set_couplings(h, _ptr(S), I.eld),
               "h2hip_h2matrix_set_couplings")
-        N = np.ascontiguousarray(near_values, dtype=self.dtype)
-        check(hip().h2hip_h2matrix_set_nearfield(h, _ptr(N)),
-              "h2hip_h2matrix_set_nearfield")
+        if near_values is not None:
+            N = np.ascontiguousarray(near_values, dtype=self.dtype)
+            check(hip().h2hip_h2matrix_set_nearfield(h, _ptr(N)),
+                  "h2hip_h2matrix_set_nearfield")
         return self
+
+    def copy_nearfield_from_device(self, d_src, entries):
+        """Near-field blocks (packed in the order of the matrix's near
+        blocks) from device memory at address d_src, entries elements."""
+        es = 16 if self.dtype == np.complex128 else 8
+        dst = hip().h2hip_h2matrix_nearfield_device(self._h)
+        check(hip().h2hip_memcpy(C.c_void_p(dst), C.c_void_p(d_src),
+                                 es * entries, 2), "h2hip_memcpy")
 
     def assemble_nearfield(self, stream=None):
         """Near-field blocks by the HIP quadrature, straight into the

@@ -53,6 +53,15 @@ def _sigs():
     h.h2hip_gmres_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint32),
                                         C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_uint32]
+    h.h2hip_gmres_multi_create.restype = C.c_int
+    h.h2hip_gmres_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_uint32,
+                                           dp, dp, C.c_int, C.c_uint32, dp]
+    h.h2hip_gmres_multi_destroy.argtypes = [dp]
+    h.h2hip_gmres_multi_solve.restype = C.c_int
+    h.h2hip_gmres_multi_solve.argtypes = [dp, dp, dp, C.c_double, C.c_uint32,
+                                          C.c_int, dp, dp]
+    h.h2hip_gmres_multi_graph_builds.restype = C.c_uint64
+    h.h2hip_gmres_multi_graph_builds.argtypes = [dp]
     b.h2b_integrate_l.argtypes = [dp, C.c_uint, C.c_int, dp, dp]
     b.h2b_projectL2_l.restype = C.c_int
     b.h2b_projectL2_l.argtypes = [dp, C.c_uint, C.c_int, dp, dp]
@@ -134,6 +143,54 @@ class GmresBatch:
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_gmres_batch_destroy(h)
+            self._h = None
+
+
+class GmresMulti:
+    """GMRES(kmax) on many independent systems A_s x_s = b_s (one device
+    H2-matrix each), stepping in lock step; with use_graph one Arnoldi step
+    of all active systems is one replay of a captured HIP graph.  Each
+    system follows solve_gmres_avector (Library/krylovsolvers.c:183-227)."""
+
+    def __init__(self, mats, kmax=30, stream=None):
+        _sigs()
+        if stream is None:
+            raise ValueError("GmresMulti needs a stream (it is captured)")
+        self.mats = list(mats)
+        self.dtype = self.mats[0].dtype
+        self.stream = stream
+        hs = (C.c_void_p * len(self.mats))(*[H._h.value for H in self.mats])
+        ns = np.array([H.n for H in self.mats], dtype=np.uint64)
+        h = C.c_void_p()
+        check(hip().h2hip_gmres_multi_create(
+            C.byref(h), len(self.mats), C.cast(hs, C.c_void_p),
+            ns.ctypes.data_as(C.c_void_p),
+            int(self.dtype == np.complex128), kmax, stream.handle),
+            "h2hip_gmres_multi_create")
+        self._h = h
+
+    def solve(self, d_bs, d_xs, eps, maxiter, use_graph=True):
+        """Solves all systems (x_s: start vectors, overwritten); returns
+        (iterations, residual estimates) per system."""
+        c = len(self.mats)
+        bs = (C.c_void_p * c)(*[d.ptr for d in d_bs])
+        xs = (C.c_void_p * c)(*[d.ptr for d in d_xs])
+        it = np.zeros(c, dtype=np.uint32)
+        res = np.zeros(c)
+        check(hip().h2hip_gmres_multi_solve(
+            self._h, C.cast(bs, C.c_void_p), C.cast(xs, C.c_void_p), eps,
+            maxiter, int(bool(use_graph)), it.ctypes.data_as(C.c_void_p),
+            res.ctypes.data_as(C.c_void_p)), "h2hip_gmres_multi_solve")
+        return it, res
+
+    @property
+    def graph_builds(self):
+        return int(hip().h2hip_gmres_multi_graph_builds(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            hip().h2hip_gmres_multi_destroy(h)
             self._h = None
 
 

@@ -86,9 +86,13 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
                     uint32_t q_regular, uint32_t q_singular,
                     const double *alpha, const double *k);
 void h2hip_op_destroy(h2hip_op *op);
-/* Precision of the kernel evaluations: 0 FP64 (default), 1 mixed (FP32
- * evaluation and accumulation of regular pairs, FP64 geometry differences
- * and accumulation for singular pairs). */
+/* Precision of the kernel evaluations: 0 FP64 (default), 1 mixed.  Mixed:
+ * kernel values (1/r, the phase, e^{ikr}) in FP32 for every pair; regular
+ * pairs form x - y from an FP64 difference of anchor points plus FP32
+ * offsets and sum the inner rule dimension in FP32, the outer in FP64;
+ * singular pairs keep the Sauter-Schwab points, x - y and the sums in FP64.
+ * Entries agree with FP64 to about 1e-6 relative.  No counterpart in the
+ * reference (it computes in its build's field type only). */
 int h2hip_op_set_precision(h2hip_op *op, int mode);
 
 /* ---- the near-field callback ---------------------------------------------
@@ -123,6 +127,15 @@ void h2hip_nfplan_destroy(h2hip_nfplan *p);
 uint64_t h2hip_nfplan_entries(const h2hip_nfplan *p);
 /* Element offsets of the blocks in the packed output (nblocks values). */
 int h2hip_nfplan_offsets(const h2hip_nfplan *p, uint64_t *off);
+/* Schedule of the plan's launches (no counterpart in the reference; the
+ * entries do not depend on it beyond the order of atomic sums, which
+ * varies from launch to launch anyway for the linear basis; constant-basis
+ * entries are the same bit for bit).  sorted = 1 orders the tiles
+ * by decreasing regular work (longest first); persistent = 1 runs the
+ * regular kernel as a persistent grid of resident workgroups that take
+ * tiles from a device counter.  Not thread-safe against concurrent
+ * executions of the same plan. */
+int h2hip_nfplan_set_schedule(h2hip_nfplan *p, int sorted, int persistent);
 /* Kernel evaluations per rule class (regular, vertex, edge, identical) and
  * triangle pairs per class, as the reference's loops perform them
  * (counted on the host; out[0..3] evaluations, out[4..7] pairs). */
@@ -256,6 +269,31 @@ int h2hip_gmres_batch_submit(h2hip_gmres_batch *G, const void *d_b,
 int h2hip_gmres_batch_run(h2hip_gmres_batch *G, uint32_t *ndone,
                           uint64_t *tags, uint32_t *iterations,
                           double *residuals, uint32_t cap);
+
+/* ---- many independent systems, one graph launch per Arnoldi step ----------
+ * Solves A_s x_s = b_s for count systems (typically one H2-matrix per mesh)
+ * with the semantics of solve_gmres_avector (Library/krylovsolvers.c:
+ * 183-227) for each of them, all with the same eps, maxiter and kmax.  The
+ * systems step in lock step; one step of every active system is captured
+ * once into a HIP graph (one branch per distinct matrix) and replayed for
+ * each step (use_graph = 1), or launched directly (use_graph = 0; the
+ * same kernels: results agree to the rounding of the product's atomic
+ * sums).  The graph is captured again only
+ * when a system leaves.  stream must not be the default stream (it is
+ * captured).  x_s holds the start vector on entry.  No counterpart in the
+ * reference (it solves one system at a time). */
+typedef struct h2hip_gmres_multi h2hip_gmres_multi;
+int h2hip_gmres_multi_create(h2hip_gmres_multi **out, uint32_t count,
+                             const h2hip_h2matrix *const *H,
+                             const uint64_t *n, int is_complex, uint32_t kmax,
+                             void *stream);
+void h2hip_gmres_multi_destroy(h2hip_gmres_multi *G);
+int h2hip_gmres_multi_solve(h2hip_gmres_multi *G, const void *const *d_b,
+                            void *const *d_x, double eps, uint32_t maxiter,
+                            int use_graph, uint32_t *iterations,
+                            double *residuals);
+/* Number of graphs captured so far (one per change of the active set). */
+uint64_t h2hip_gmres_multi_graph_builds(const h2hip_gmres_multi *G);
 
 /* Waits for all work queued on the stream. */
 int h2hip_stream_sync(void *stream);

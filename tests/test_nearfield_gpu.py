@@ -154,3 +154,48 @@ def test_batched_plan_matches_single_calls(sphere8):
                                cidx[coff:coff + cols])
             got = plan.block(flat, i)
             assert rel_block_err(got, want) < 1e-14, (name, i)
+
+
+@pytest.mark.parametrize("sorted_tiles,persistent", [(1, 0), (0, 1), (1, 1)])
+def test_schedules_give_identical_entries(sorted_tiles, persistent, sphere8):
+    # the launch schedule changes the order of the tiles only: the entries
+    # are the same (bit for bit where no atomic sums are involved), in FP64
+    # and in mixed precision
+    import h2lib_amd as h
+    rng = np.random.default_rng(21)
+    for name in ("lap_slp_cc", "helm_dlp_ll"):
+        kern, op, br, bc, alpha, k = CASES[name]
+        gr = h.Surface3d(sphere8["x"], sphere8["t"])
+        b = h.Bem3d(gr, kern, op, 2, 4, br, bc, alpha=alpha, k=k)
+        nr = gr.vertices if br else gr.triangles
+        nc = gr.vertices if bc else gr.triangles
+        ridx = rng.permutation(nr).astype(np.uint32)
+        cidx = rng.permutation(nc).astype(np.uint32)
+        blocks = []
+        for _ in range(60):
+            rows = int(rng.integers(1, 140))
+            cols = int(rng.integers(1, 140))
+            blocks.append((int(rng.integers(0, nr - rows)), rows,
+                           int(rng.integers(0, nc - cols)), cols))
+        plan = h.NearfieldPlan(gr, br, bc, ridx, cidx, blocks)
+        stream = h.Stream()
+        for mode in (0, 1):
+            b.set_precision(mode)
+            plan.set_schedule(0, 0)
+            ref = h.DeviceArray(plan.entries, b.dtype)
+            plan.execute(b, ref.ptr, stream.handle)
+            plan.set_schedule(sorted_tiles, persistent)
+            out = h.DeviceArray(plan.entries, b.dtype)
+            out.fill_nan()
+            for _ in range(2):  # the tile counter is reset per launch
+                plan.execute(b, out.ptr, stream.handle)
+            stream.sync()
+            got, want = out.download(), ref.download()
+            if br == 0 and bc == 0:
+                # one store or one atomic add per entry: deterministic
+                assert np.array_equal(got, want), (name, mode)
+            else:
+                # the linear basis sums element matrices with atomic adds,
+                # whose order varies from run to run
+                err = np.abs(got - want).max() / np.abs(want).max()
+                assert err < 1e-14, (name, mode, err)
