@@ -279,6 +279,14 @@ def bench_batched(h, args, stream):
                               for sp, c, k, e in zip(specs, clfs, it, rel)]
             gm["per_mesh_fields"] = ["kind", "triangles", "scale", "clf",
                                      "steps", "true_rel_residual"]
+            gm["converged"] = int(sum(e <= 1.5e-8 for e in rel))
+            gm["note"] = ("meshes that reach the step limit are cylinders: "
+                          "their single layer matrices at q=2 q2=4 are "
+                          "indefinite (symmetric part has a negative "
+                          "eigenvalue; -1.4e-4 against 6.6e-2 at 1960 "
+                          "triangles, the same with the reference's "
+                          "quadrature), so restarted GMRES stagnates; the "
+                          "timing per step is unaffected")
     gm["graph_captures"] = G.graph_builds
     gm["speedup_graph_vs_direct"] = gm["direct"]["s"] / gm["graph"]["s"]
     tri = [sp.triangles for sp in specs]
@@ -303,6 +311,55 @@ def bench_batched(h, args, stream):
         "gmres": gm,
         "autotuner": tuner.to_json(),
     }
+
+
+def bench_helmholtz_solve(h, args, stream):
+    """Exterior Helmholtz problem on the unit sphere split args.helm_split
+    (131072 triangles by default), piecewise constant basis, point source
+    (0.3, 0.2, 0.1) inside: f = projected Dirichlet data, b = (K - M/2) f,
+    V x = b by GMRES(30) to 1e-8, with both near fields in FP64 and in
+    mixed precision, H2-matrices by interpolation of order 4 (leaf size
+    64, eta 2) with complex FP64 couplings.  Per k: solve time, steps,
+    Neumann error of each solution and the mixed solution's error against
+    the FP64 one."""
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import HelmholtzExterior
+    mesh = G.sphere(args.helm_split)
+    x0 = np.array([0.3, 0.2, 0.1])
+    out = {"workload": ("exterior Helmholtz, unit sphere %d triangles, cc, "
+                        "point source (0.3, 0.2, 0.1), GMRES(30) to 1e-8, "
+                        "interpolation order 4, clf 64" % mesh.triangles),
+           "k": {}}
+    for k in args.helm_k:
+        res = {}
+        sol = {}
+        for prec, name in ((0, "fp64"), (1, "mixed")):
+            t0 = time.perf_counter()
+            prob = HelmholtzExterior(mesh, k, clf=64, m=4, precision=prec,
+                                     stream=stream)
+            stream.sync()
+            build = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            _, d_b = prob.rhs(x0, stream)
+            d_x, it, r = prob.solve(d_b, eps=1e-8, maxiter=args.helm_maxiter,
+                                    kmax=30, stream=stream)
+            x = d_x.download()
+            dt = time.perf_counter() - t0
+            sol[name] = x
+            res[name] = {"solve_s": dt, "steps": int(it),
+                         "operator_build_s": build,
+                         "neumann_rel_err": prob.neumann_error(x, x0)}
+            del prob
+        res["mixed_solution_err_vs_fp64"] = float(
+            np.linalg.norm(sol["mixed"] - sol["fp64"]) /
+            np.linalg.norm(sol["fp64"]))
+        out["k"]["%g" % k] = res
+        log("bench: helmholtz solve k=%g fp64 %d steps %.2f s, mixed %d "
+            "steps %.2f s, mixed vs fp64 %.2e" %
+            (k, res["fp64"]["steps"], res["fp64"]["solve_s"],
+             res["mixed"]["steps"], res["mixed"]["solve_s"],
+             res["mixed_solution_err_vs_fp64"]))
+    return out
 
 
 def bench_dirichlet(h, mesh, args, stream):
@@ -475,10 +532,16 @@ def main():
                     help="sphere split of the Helmholtz FP64 / mixed "
                     "near-field comparison (0: skip it)")
     ap.add_argument("--helm-steps", type=int, default=5)
+    ap.add_argument("--helm-k", type=float, nargs="*", default=[2.0, 4.0, 8.0],
+                    help="wave numbers of the Helmholtz solves (none: skip)")
+    ap.add_argument("--helm-maxiter", type=int, default=2000)
     ap.add_argument("--batch-meshes", type=int, default=64,
                     help="meshes of the batched run (0: skip it)")
     ap.add_argument("--batch-seed", type=int, default=3)
-    ap.add_argument("--batch-maxiter", type=int, default=1000)
+    ap.add_argument("--batch-maxiter", type=int, default=300,
+                    help="GMRES step limit of the batched solves (the "
+                    "cylinder meshes give indefinite matrices at q=2 q2=4 "
+                    "and stagnate under restarted GMRES)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -576,6 +639,11 @@ def main():
         log("bench: %d meshes in one launch set" % args.batch_meshes)
         batched = bench_batched(h, args, stream)
 
+    helm_solve = None
+    if args.helm_split > 0 and args.helm_k:
+        log("bench: exterior Helmholtz solves, FP64 and mixed")
+        helm_solve = bench_helmholtz_solve(h, args, stream)
+
     log("bench: dense 2048 x 2048 matrix")
     dense = bench_dense(h, stream)
 
@@ -634,6 +702,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
         "helmholtz_mixed": helm,
+        "helmholtz_solve": helm_solve,
         "batched": batched,
         "dirichlet": dirichlet,
         "dense": dense,

@@ -646,7 +646,7 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   if constexpr (CLS == CLS_IDENTICAL) {
     // identity term of the double layer on identical triangles (tp, sp are
     // the identity here): cc at (t, t) of the block
-    // (Library/bem3d.c:1234-1241), otherwise alpha g mass[l, k]
+    // (Library/bem3d.c:1234-1241), otherwise Re(alpha) g mass[l, k]
     // (Library/bem3d.c:3205-3242, 1934-1966, 2530-2562)
     if constexpr (CC) {
       if (T.row0 + lr == T.col0 + lc) {
@@ -654,11 +654,13 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
         vim += 0.5 * a.aim * gt;
       }
     } else {
+      // the linear-basis integrators hold alpha g in a real variable
+      // (Library/bem3d.c:1822-1823, 2418-2419, 3061-3062): only Re(alpha)
+      // reaches the mass term
       const bool allowed = (BR && !BC) ? (T.flags & 1) : true;
       if (allowed) {
         const double m = (BR && BC) ? a.mass[l + 3 * k] : a.mass[BR ? l : k];
         vre += m * a.are * gt;
-        vim += m * a.aim * gt;
       }
     }
   }

@@ -39,6 +39,9 @@ def _sigs():
                                      C.c_void_p, C.c_int, C.c_int, C.c_int,
                                      C.c_uint, C.c_uint]
     b.h2b_h2interp_coupling_laplace.argtypes = [ip, C.c_void_p]
+    b.h2b_h2interp_coupling_helmholtz.argtypes = [ip, C.c_void_p,
+                                                  C.c_double, C.c_double,
+                                                  C.c_void_p]
     b.h2b_h2interp_free.argtypes = [ip]
     h = hip()
     vp, u32, dp = C.c_void_p, C.c_uint32, C.c_void_p
@@ -110,6 +113,17 @@ class Interpolation:
         self.rE = as_np(I.rE, (rt.nnodes * kk,))
         self.cE = as_np(I.cE, (ct.nnodes * kk,))
         self.S = as_np(I.S, (max(1, I.nfar * kk),))
+        self._bp = btree._bp
+        self.nfar = I.nfar
+
+    def helmholtz_couplings(self, k):
+        """Complex couplings of the Helmholtz kernel with wave number k
+        (same layout as S)."""
+        out = np.zeros(2 * max(1, self.nfar) * self.eld)
+        bem().h2b_h2interp_coupling_helmholtz(
+            self._ip, self._bp, float(np.real(k)), float(np.imag(k)),
+            out.ctypes.data_as(C.c_void_p))
+        return out.view(np.complex128)
 
     def __del__(self):
         ip = getattr(self, "_ip", None)
@@ -125,8 +139,6 @@ class H2Matrix:
     def __init__(self, op, mesh, rt, ct, btree, m, build_nearfield=True,
                  stream=None):
         _sigs()
-        if op.kernel != LAPLACE:
-            raise NotImplementedError("interpolation couplings: Laplace only")
         self.op, self.rt, self.ct, self.btree = op, rt, ct, btree
         self.dtype = op.dtype
         self.m = m
@@ -148,7 +160,8 @@ class H2Matrix:
         check(hip().h2hip_h2matrix_set_basis(
             h, 1, _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld),
             "h2hip_h2matrix_set_basis")
-        check(hip().h2hip_h2matrix_set_couplings(h, _ptr(I.S), I.eld),
+        S = I.S if op.kernel == LAPLACE else I.helmholtz_couplings(op.k)
+        check(hip().h2hip_h2matrix_set_couplings(h, _ptr(S), I.eld),
               "h2hip_h2matrix_set_couplings")
         self.nearplan = None
         if build_nearfield:

@@ -202,3 +202,84 @@ h2b_normL2(const h2b_surface *s, unsigned q, int kind, const double *x0)
   }
   return sqrt(norm);
 }
+
+/* ---- Helmholtz point source, piecewise constant basis (complex) ----------
+ * g(x, n) = e^{ik r} / r (kind 0, rhs_dirichlet_point_helmholtzbem3d) or
+ * e^{ik r} / r^3 (ik r - 1) <n, x - x0> (kind 1,
+ * rhs_neumann_point_helmholtzbem3d), r = |x - x0|
+ * (Library/helmholtzbem3d.c:1521-1578), real k. */
+static void
+helm_value(int kind, double k, const double *x0, const double *x,
+           const double *n, double *re, double *im)
+{
+  const double d[3] = {x[0] - x0[0], x[1] - x0[1], x[2] - x0[2]};
+  const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double rs = 1.0 / sqrt(r2), r = r2 * rs;
+  double a = cos(k * r) * rs, b = sin(k * r) * rs;
+  if (kind == 1) {
+    /* (a + ib) (ikr - 1) <n, d> / r^2 */
+    const double f = (n[0] * d[0] + n[1] * d[1] + n[2] * d[2]) * rs * rs;
+    const double kr = k * r;
+    const double na = -a - b * kr, nb = a * kr - b;
+    a = na * f;
+    b = nb * f;
+  }
+  *re = a;
+  *im = b;
+}
+
+void
+h2b_projectL2_c_helmholtz(const h2b_surface *s, unsigned q, int kind,
+                          double k, const double *x0, double *out)
+{
+  double rx0[256], rx1[256], rw[256];
+  const unsigned nq = tri_rule(q, rx0, rx1, rw);
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double *A = s->x + 3 * tv[0], *B = s->x + 3 * tv[1],
+      *C = s->x + 3 * tv[2];
+    double sre = 0.0, sim = 0.0;
+    for (unsigned j = 0; j < nq; j++) {
+      const double u = rx0[j], v = rx1[j];
+      double x[3], re, im;
+      for (int d = 0; d < 3; d++)
+        x[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+      helm_value(kind, k, x0, x, s->n + 3 * t, &re, &im);
+      sre += rw[j] * re;
+      sim += rw[j] * im;
+    }
+    /* the rule's weights sum to 1/2: twice the sum is the mean value */
+    out[2 * t] = 2.0 * sre;
+    out[2 * t + 1] = 2.0 * sim;
+  }
+}
+
+void
+h2b_normL2_c_helmholtz(const h2b_surface *s, unsigned q, const double *xv,
+                       int kind, double k, const double *x0, double *out)
+{
+  double rx0[256], rx1[256], rw[256];
+  const unsigned nq = tri_rule(q, rx0, rx1, rw);
+  double diff = 0.0, norm = 0.0;
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double *A = s->x + 3 * tv[0], *B = s->x + 3 * tv[1],
+      *C = s->x + 3 * tv[2];
+    double sd = 0.0, sn = 0.0;
+    for (unsigned j = 0; j < nq; j++) {
+      const double u = rx0[j], v = rx1[j];
+      double x[3], re, im;
+      for (int d = 0; d < 3; d++)
+        x[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+      helm_value(kind, k, x0, x, s->n + 3 * t, &re, &im);
+      const double er = re - (xv ? xv[2 * t] : 0.0);
+      const double ei = im - (xv ? xv[2 * t + 1] : 0.0);
+      sd += rw[j] * (er * er + ei * ei);
+      sn += rw[j] * (re * re + im * im);
+    }
+    diff += s->g[t] * sd;
+    norm += s->g[t] * sn;
+  }
+  out[0] = sqrt(diff);
+  out[1] = sqrt(norm);
+}

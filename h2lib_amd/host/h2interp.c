@@ -353,6 +353,52 @@ h2b_h2interp_coupling_laplace(h2b_h2interp *H, const h2b_blocks *bl)
   }
 }
 
+/* Couplings of the Helmholtz kernel, complex (re, im interleaved) into
+ * S_out (2 k^2 per admissible block): kernel_const e^{i kr r} e^{-ki r} / r
+ * at the interpolation points of both boxes, k = kr + i ki
+ * (fill_kernel_helmholtzbem3d with slp_kernel_helmholtzbem3d,
+ * Library/helmholtzbem3d.c:26-55, 828-832, through
+ * assemble_bem3d_inter_uniform, Library/bem3d.c:6079-6108). */
+void
+h2b_h2interp_coupling_helmholtz(const h2b_h2interp *H, const h2b_blocks *bl,
+                                double kr, double ki, double *S_out)
+{
+  const uint32_t m = H->m, k = H->k;
+  const double kdamp = -ki;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (uint32_t b = 0; b < bl->nfar; b++) {
+    const double *pr = H->rpts + 3 * m * bl->far[2 * b];
+    const double *pc = H->cpts + 3 * m * bl->far[2 * b + 1];
+    double *S = S_out + (size_t) 2 * b * k * k;
+    double X[512][3], Y[512][3];
+    uint32_t i = 0;
+    for (unsigned a = 0; a < m; a++)
+      for (unsigned bb = 0; bb < m; bb++)
+        for (unsigned c = 0; c < m; c++, i++) {
+          X[i][0] = pr[a];
+          X[i][1] = pr[m + bb];
+          X[i][2] = pr[2 * m + c];
+          Y[i][0] = pc[a];
+          Y[i][1] = pc[m + bb];
+          Y[i][2] = pc[2 * m + c];
+        }
+    for (uint32_t j = 0; j < k; j++)
+      for (uint32_t ii = 0; ii < k; ii++) {
+        const double d0 = X[ii][0] - Y[j][0], d1 = X[ii][1] - Y[j][1],
+          d2 = X[ii][2] - Y[j][2];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double rs = 1.0 / sqrt(r2);
+        const double r = r2 * rs;
+        if (kdamp != 0.0)
+          rs *= exp(kdamp * r);
+        const double ph = kr * r;
+        const size_t o = 2 * (ii + (size_t) j * k);
+        S[o] = KCONST * (rs * cos(ph));
+        S[o + 1] = KCONST * (sin(ph) * rs);
+      }
+  }
+}
+
 void
 h2b_h2interp_free(h2b_h2interp *H)
 {

@@ -69,7 +69,38 @@ def _sigs():
     b.h2b_normL2diff_l.argtypes = [dp, C.c_uint, dp, C.c_int, dp]
     b.h2b_normL2.restype = C.c_double
     b.h2b_normL2.argtypes = [dp, C.c_uint, C.c_int, dp]
+    b.h2b_projectL2_c_helmholtz.argtypes = [dp, C.c_uint, C.c_int,
+                                            C.c_double, dp, dp]
+    b.h2b_normL2_c_helmholtz.argtypes = [dp, C.c_uint, dp, C.c_int,
+                                         C.c_double, dp, dp]
     _done = True
+
+
+def helmholtz_point_data(mesh, kind, k, x0, q=2):
+    """Point-source data at x0 (kind 0: Dirichlet e^{ikr}/r, 1: Neumann)
+    projected onto the piecewise constants: the mean over each triangle
+    (projectL2_bem3d_c_avector, Library/bem3d.c:8827-8876)."""
+    _sigs()
+    out = np.zeros(mesh.triangles, dtype=np.complex128)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    bem().h2b_projectL2_c_helmholtz(mesh._sp, q, kind, float(k),
+                                    x0.ctypes.data_as(C.c_void_p),
+                                    out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def helmholtz_point_errors(mesh, x, kind, k, x0, q=2):
+    """(|x - data|_L2, |data|_L2) for piecewise constant x
+    (normL2diff_c_bem3d, normL2_bem3d)."""
+    _sigs()
+    out = np.zeros(2)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    xx = None if x is None else np.ascontiguousarray(x, dtype=np.complex128)
+    bem().h2b_normL2_c_helmholtz(
+        mesh._sp, q, None if xx is None else xx.ctypes.data_as(C.c_void_p),
+        kind, float(k), x0.ctypes.data_as(C.c_void_p),
+        out.ctypes.data_as(C.c_void_p))
+    return out
 
 
 def solve_gmres(H, d_b, d_x, eps, maxiter, kmax, stream=None):
@@ -258,3 +289,51 @@ class LaplaceDirichlet:
         e = bem().h2b_normL2diff_l(self.mesh._sp, self.q,
                                    x.ctypes.data_as(C.c_void_p), 1, p0)
         return e / bem().h2b_normL2(self.mesh._sp, self.q, 1, p0)
+
+
+class HelmholtzExterior:
+    """Exterior Dirichlet problem of the Helmholtz equation with the data of
+    a point source x0 inside the surface, piecewise constant basis: f = the
+    projected Dirichlet data e^{ik|x - x0|}/|x - x0|, b = (K - M/2) f with
+    the double layer H2-matrix (alpha = -1/2), V x = b with the single layer
+    H2-matrix; x approximates the Neumann data of the radiating field.
+    precision 1 assembles both near fields in mixed precision."""
+
+    def __init__(self, mesh, k, clf=64, eta=2.0, m=4, q=2, q2=4,
+                 precision=0, stream=None):
+        from .bem3d import new_dlp_helmholtz_bem3d, new_slp_helmholtz_bem3d
+        _sigs()
+        self.mesh, self.k, self.q = mesh, k, q
+        self.gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+        self.slp = new_slp_helmholtz_bem3d(k, self.gr, q, q2, 0, 0)
+        self.dlp = new_dlp_helmholtz_bem3d(k, self.gr, q, q2, 0, 0, -0.5)
+        if precision:
+            self.slp.set_precision(precision)
+            self.dlp.set_precision(precision)
+        self.rt = ClusterTree(mesh, 0, clf)
+        self.bt = BlockTree(self.rt, self.rt, eta, 0)
+        self.V = H2Matrix(self.slp, mesh, self.rt, self.rt, self.bt, m,
+                          stream=stream)
+        self.K = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
+                          stream=stream)
+        self.n = mesh.triangles
+
+    def rhs(self, x0, stream=None):
+        """(f, b) on the device for the point source at x0."""
+        f = helmholtz_point_data(self.mesh, 0, self.k, x0, self.q)
+        d_f = DeviceArray.from_host(f)
+        d_b = DeviceArray(self.n, np.complex128)
+        d_b.zero()
+        self.K.addeval(1.0, d_f, d_b, stream)
+        return d_f, d_b
+
+    def solve(self, d_b, eps=1e-8, maxiter=1000, kmax=30, stream=None):
+        d_x = DeviceArray(self.n, np.complex128)
+        d_x.zero()
+        it, res = solve_gmres(self.V, d_b, d_x, eps, maxiter, kmax, stream)
+        return d_x, it, res
+
+    def neumann_error(self, x, x0):
+        """|x - Neumann data|_L2 / |Neumann data|_L2."""
+        d, nrm = helmholtz_point_errors(self.mesh, x, 1, self.k, x0, self.q)
+        return d / nrm

@@ -122,6 +122,13 @@ h2b_h2interp *h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
                                  unsigned m, unsigned q);
 /* Laplace couplings S = g(xi_r, xi_c) / (4 pi) for all far blocks. */
 void h2b_h2interp_coupling_laplace(h2b_h2interp *H, const h2b_blocks *bl);
+/* Helmholtz couplings (complex, re / im interleaved, 2 k^2 doubles per
+ * admissible block in the order of the block list) for k = kr + i ki into
+ * S_out (assemble_bem3d_inter_uniform with fill_kernel_helmholtzbem3d,
+ * Library/bem3d.c:6079-6108, Library/helmholtzbem3d.c:828-832). */
+void h2b_h2interp_coupling_helmholtz(const h2b_h2interp *H,
+                                     const h2b_blocks *bl, double kr,
+                                     double ki, double *S_out);
 void h2b_h2interp_free(h2b_h2interp *H);
 
 /* ---- boundary data and functionals (piecewise linear basis) ---------------
@@ -145,6 +152,22 @@ double h2b_normL2diff_l(const h2b_surface *s, unsigned q, const double *x,
 /* |f|_L2 (normL2_bem3d, Library/bem3d.c:8704) */
 double h2b_normL2(const h2b_surface *s, unsigned q, int kind,
                   const double *x0);
+
+/* Helmholtz point source at x0 with real wave number k, piecewise constant
+ * basis, complex values as (re, im) pairs.  kind 0: Dirichlet data
+ * e^{ikr}/r (rhs_dirichlet_point_helmholtzbem3d), 1: Neumann data
+ * (rhs_neumann_point_helmholtzbem3d), Library/helmholtzbem3d.c:1521-1578.
+ * Projection onto the constants: out[t] = mean of the data over triangle t
+ * by the rule of order q (projectL2_bem3d_c_avector, Library/bem3d.c:
+ * 8827-8876). */
+void h2b_projectL2_c_helmholtz(const h2b_surface *s, unsigned q, int kind,
+                               double k, const double *x0, double *out);
+/* out[0] = |x - data|_L2 (normL2diff_c_bem3d, Library/bem3d.c:8776-8825;
+ * x may be NULL for 0), out[1] = |data|_L2 (normL2_bem3d,
+ * Library/bem3d.c:8704-8751). */
+void h2b_normL2_c_helmholtz(const h2b_surface *s, unsigned q,
+                            const double *x, int kind, double k,
+                            const double *x0, double *out);
 
 #ifdef __cplusplus
 }

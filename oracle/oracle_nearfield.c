@@ -400,7 +400,10 @@ ll(const orc_bem *b, int far, const unsigned *ridx, unsigned rows,
       }
       if (!far && b->alpha != 0.0 && tt == ss) {
         const unsigned *tri = b->t + 3 * tt;
-        double complex f2 = b->alpha * b->g[tt];
+        /* factor2 is a real variable in the reference's ll integrator
+         * (Library/bem3d.c:3061-3062): only Re(alpha) reaches the mass
+         * term */
+        double complex f2 = creal(b->alpha) * b->g[tt];
         for (unsigned a = 0; a < C.nloc[s]; a++) {
           unsigned j = C.loc[3 * s + a];
           unsigned jj = cidx ? cidx[j] : j;
@@ -463,7 +466,9 @@ cl(const orc_bem *b, int far, const unsigned *ridx, unsigned rows,
       }
       if (!far && b->alpha != 0.0 && tt == ss) {
         const unsigned *tri = b->t + 3 * ss;
-        double complex f2 = b->alpha * b->g[tt];
+        /* real factor2 in the reference's cl integrator
+         * (Library/bem3d.c:1822-1823): Re(alpha) only */
+        double complex f2 = creal(b->alpha) * b->g[tt];
         for (unsigned a = 0; a < C.nloc[s]; a++) {
           unsigned j = C.loc[3 * s + a];
           unsigned jj = cidx ? cidx[j] : j;
@@ -522,7 +527,9 @@ lc(const orc_bem *b, int far, const unsigned *ridx, unsigned rows,
       }
       if (!far && b->alpha != 0.0 && tt == ss && cols > 3) {
         const unsigned *tri = b->t + 3 * tt;
-        double complex f2 = b->alpha * b->g[tt];
+        /* real factor2 in the reference's lc integrator
+         * (Library/bem3d.c:2418-2419): Re(alpha) only */
+        double complex f2 = creal(b->alpha) * b->g[tt];
         for (unsigned a = 0; a < R.nloc[t]; a++) {
           unsigned i = R.loc[3 * t + a];
           unsigned ii = ridx ? ridx[i] : i;

@@ -575,6 +575,57 @@ ref_dirichlet_laplace(void *slp, void *dlp, double eps, unsigned maxiter,
   return it;
 }
 
+#ifdef USE_COMPLEX
+/* Boundary data of a Helmholtz point source at x0 with wave number k
+ * (rhs_dirichlet_point_helmholtzbem3d for kind 0,
+ * rhs_neumann_point_helmholtzbem3d for kind 1,
+ * Library/helmholtzbem3d.c:1521-1578) projected onto the constant basis of
+ * the operator h (projectL2_bem3d_c_avector, Library/bem3d.c:8827-8876);
+ * out: one complex value per triangle. */
+void
+ref_helmholtz_point_project(void *h, int kind, double k, const double *x0,
+			    void *out)
+{
+  refbem   *rb = (refbem *) h;
+  real      kvec[3] = { k, 0.0, 0.0 }, src[3] = { x0[0], x0[1], x0[2] };
+  helmholtz_data hd;
+  pavector  f;
+
+  hd.kvec = kvec;
+  hd.source = src;
+  hd.eta = 0.0;
+  f = new_avector(rb->gr->triangles);
+  projectL2_bem3d_c_avector(rb->bem, kind ? rhs_neumann_point_helmholtzbem3d
+			    : rhs_dirichlet_point_helmholtzbem3d, f,
+			    (void *) &hd);
+  memcpy(out, f->v, sizeof(field) * rb->gr->triangles);
+  del_avector(f);
+}
+
+/* out[0] = |x - g|_L2 (normL2diff_c_bem3d, Library/bem3d.c:8776-8825) and
+ * out[1] = |g|_L2 (normL2_bem3d, Library/bem3d.c:8704-8751) for the point
+ * source data g of kind 0 (Dirichlet) or 1 (Neumann). */
+void
+ref_helmholtz_point_errors(void *h, int kind, double k, const double *x0,
+			   const void *x, double *out)
+{
+  refbem   *rb = (refbem *) h;
+  real      kvec[3] = { k, 0.0, 0.0 }, src[3] = { x0[0], x0[1], x0[2] };
+  helmholtz_data hd;
+  avector   xt;
+  pavector  xv;
+  boundary_func3d g = kind ? rhs_neumann_point_helmholtzbem3d
+    : rhs_dirichlet_point_helmholtzbem3d;
+
+  hd.kvec = kvec;
+  hd.source = src;
+  hd.eta = 0.0;
+  xv = init_pointer_avector(&xt, (field *) x, rb->gr->triangles);
+  out[0] = normL2diff_c_bem3d(rb->bem, xv, g, (void *) &hd);
+  out[1] = normL2_bem3d(rb->bem, g, (void *) &hd);
+}
+#endif
+
 /* GMRES(kmax) on the H2-matrix: solves H x = b to |r| <= eps |b|.
  * x holds the start vector on entry.  Returns the iteration count. */
 unsigned

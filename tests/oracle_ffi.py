@@ -282,6 +282,12 @@ class Reference:
                                         C.c_void_p, C.c_uint, C.c_uint,
                                         C.c_void_p, C.c_void_p]
             lib.ref_bem_del.argtypes = [C.c_void_p]
+            if complex_:
+                lib.ref_helmholtz_point_project.argtypes = [
+                    C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+                lib.ref_helmholtz_point_errors.argtypes = [
+                    C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                    C.c_void_p]
             lib.ref_nearfield.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
                                           C.c_uint, C.c_void_p, C.c_uint,
                                           C.c_int, C.c_void_p, C.c_uint]
@@ -365,6 +371,24 @@ class ReferenceBem(_BemBase):
         if getattr(self, "h", None):
             self._lib.ref_bem_del(self.h)
             self.h = None
+
+    def helmholtz_point_project(self, kind, k, x0):
+        """Point-source data (0 Dirichlet, 1 Neumann) projected onto the
+        constant basis (complex build only)."""
+        out = np.zeros(len(self.t), dtype=np.complex128)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self._lib.ref_helmholtz_point_project(self.h, kind, k, _ptr(x0),
+                                              _ptr(out))
+        return out
+
+    def helmholtz_point_errors(self, kind, k, x0, x):
+        """(|x - data|_L2, |data|_L2) for the point-source data."""
+        out = np.zeros(2)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        xx = np.ascontiguousarray(x, dtype=np.complex128)
+        self._lib.ref_helmholtz_point_errors(self.h, kind, k, _ptr(x0),
+                                             _ptr(xx), _ptr(out))
+        return out
 
     def use_hip(self):
         """bem->nearfield / nearfield_far := the GPU callbacks

@@ -101,3 +101,37 @@ def test_all_near_equals_dense():
     x = np.random.default_rng(1).standard_normal(dense.shape[1])
     y = product(H, x)
     assert np.abs(y - dense @ x).max() < 1e-13 * np.abs(dense @ x).max()
+
+
+@pytest.mark.skipif(not have_reference(True), reason="reference not built")
+@pytest.mark.parametrize("layer,basis,k", [(0, 0, 2.0), (1, 1, 4.0),
+                                           (0, 1, 2.0 + 0.5j)])
+def test_helmholtz_matches_live_reference(layer, basis, k):
+    # complex H2-matrix of a Helmholtz operator: real interpolation bases,
+    # complex couplings (host) and near field (GPU quadrature)
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    alpha = 0.5 + 0.25j if layer else 0.0
+    if layer == 0:
+        op = h.new_slp_helmholtz_bem3d(k, gr, 2, 4, basis, basis)
+    else:
+        op = h.new_dlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, alpha)
+    rt = G.ClusterTree(mesh, basis, 24)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, 3)
+    ref = ReferenceBem(1, layer, basis, basis, mesh.x, mesh.t, q=2, q2=4,
+                       alpha=alpha, k=k)
+    rh = ReferenceH2(ref, 24, 2.0, 3, 0)
+    assert np.array_equal(rh.idx(), rt.idx)
+    rng = np.random.default_rng(13)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    print("helmholtz h2 layer", layer, "basis", basis, "k", k, flush=True)
+    for trans in (False, True):
+        want = rh.addeval(x, y0, alpha=-0.75 + 0.5j, trans=trans)
+        got = product(H, x, trans=trans, alpha=-0.75 + 0.5j, y0=y0)
+        assert np.abs(got - want).max() < TOL * np.abs(want).max(), trans

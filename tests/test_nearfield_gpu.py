@@ -33,6 +33,9 @@ CASES = {
     "helm_dlp_ll": (1, 1, 1, 1, 0.5, 2.0),
     "helm_slp_cl": (1, 0, 0, 1, 0.0, 3.0),
     "helm_dlp_lc": (1, 1, 1, 0, -0.5, 3.0),
+    # complex alpha with the linear basis (Re(alpha) only, as the reference)
+    "helm_dlp_ll_calpha": (1, 1, 1, 1, 0.5 + 0.25j, 3.0),
+    "helm_dlp_cl_calpha": (1, 1, 0, 1, -0.5 + 0.5j, 2.0),
 }
 
 

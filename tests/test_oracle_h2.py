@@ -103,3 +103,36 @@ def test_adjoint(cplx):
         Hty_wrong = Oracle.h2_addeval(interp, np.conj(N), y,
                                       np.zeros(n, dtype=dt), trans=True)
         assert abs(lhs - np.vdot(Hty_wrong, x)) > 1e-6 * abs(lhs)
+
+
+@pytest.mark.skipif(not have_reference(True), reason="reference not built")
+@pytest.mark.parametrize("layer,basis,k", [(0, 0, 2.0), (1, 1, 3.0),
+                                           (0, 1, 2.0 + 0.5j), (1, 0, 2.0)])
+def test_helmholtz_couplings_match_live_reference(layer, basis, k):
+    # the host Helmholtz couplings (h2b_h2interp_coupling_helmholtz) with
+    # real bases and the complex near field: the restated product against
+    # the reference's complex H2-matrix of the same operator
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import Interpolation
+    mesh = G.sphere(8)
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    interp = Interpolation(mesh, rt, rt, bt, basis, basis, layer == 1, 3, 2)
+    alpha = 0.5 if layer else 0.0
+    ob = OracleBem(1, layer, basis, basis, mesh.x, mesh.t, q=2, q2=4,
+                   alpha=alpha, k=k)
+    N = oracle_nearfield_packed(ob, bt.near_block_list(), rt.idx, rt.idx)
+    S = interp.helmholtz_couplings(k)
+    ref = ReferenceBem(1, layer, basis, basis, mesh.x, mesh.t, q=2, q2=4,
+                       alpha=alpha, k=k)
+    rh = ReferenceH2(ref, 16, 2.0, 3, 0)
+    assert np.array_equal(rh.idx(), rt.idx)
+    rng = np.random.default_rng(8)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    for trans in (False, True):
+        want = rh.addeval(x, y0, alpha=0.5 - 0.25j, trans=trans)
+        got = Oracle.h2_addeval(interp, N, x, y0.copy(), alpha=0.5 - 0.25j,
+                                trans=trans, S=S)
+        assert np.abs(got - want).max() < TOL * np.abs(want).max(), trans

@@ -28,6 +28,13 @@ CASES = {
     "helm_dlp_cc": (1, 1, 0, 0, 0.5, 2.0),
     "helm_slp_ll": (1, 0, 1, 1, 0.0, 2.0),
     "helm_dlp_ll": (1, 1, 1, 1, 0.5, 2.0),
+    # complex alpha: the reference keeps it for cc but holds alpha g in a
+    # real variable for the linear bases (only Re(alpha) reaches the mass
+    # term, Library/bem3d.c:1822-1823, 2418-2419, 3061-3062)
+    "helm_dlp_cc_calpha": (1, 1, 0, 0, 0.5 + 0.25j, 3.0),
+    "helm_dlp_ll_calpha": (1, 1, 1, 1, 0.5 + 0.25j, 3.0),
+    "helm_dlp_cl_calpha": (1, 1, 0, 1, -0.5 + 0.5j, 2.0),
+    "helm_dlp_lc_calpha": (1, 1, 1, 0, 0.25j, 2.0),
 }
 
 
@@ -37,7 +44,8 @@ def make(cls, mesh, name):
                alpha=alpha, k=k)
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("name", [n for n in sorted(CASES)
+                                  if not n.endswith("_calpha")])
 def test_dense_against_reference_fixture(name, sphere4):
     g = golden("dense_sphere_s4.npz")
     o = make(OracleBem, sphere4, name)
