@@ -368,8 +368,12 @@ struct Workspace {
     gv = dim3((uint32_t)((n2 + NT - 1) / NT));
     const uint64_t nsmall = 128 * 3 + 8 + CHUNK + 2 * (uint64_t)kmax * kmax +
                             2 * ((uint64_t)kmax + 1) + 4 * (uint64_t)kmax;
+    // partial sums, 128 doubles per block: the multi-dot grid has at most
+    // NB blocks, the multi-axpy grid (whose second pass also reduces |w|^2)
+    // one block per NT entries
+    const uint64_t nblk = std::max<uint64_t>(NB, (n + NT - 1) / NT);
     if (Q.alloc(n2 * kmax) != hipSuccess || w.alloc(n2) != hipSuccess ||
-        part.alloc((uint64_t)NB * 64 * 2) != hipSuccess ||
+        part.alloc(nblk * 64 * 2) != hipSuccess ||
         small.alloc(nsmall) != hipSuccess || cnt.alloc(4) != hipSuccess)
       return H2HIP_ENOMEM;
     double *p = small.p;

@@ -104,3 +104,25 @@ def test_device_gmres_matches_restatement(kmax):
     print("gmres steps device", it, "restatement", it_o, flush=True)
     assert abs(it - it_o) <= 1
     assert np.abs(d_x.download() - want).max() < 1e-8 * np.abs(want).max()
+
+
+def test_gmres_with_more_rows_than_reduction_blocks():
+    # 73728 complex unknowns: the update kernels run 288 workgroups of 256
+    # rows, more than the 240 of the multi-dot grid, and every one of them
+    # leaves a partial |w|^2 for the last one to sum (regression: the
+    # partial-sum buffer was sized for 240 workgroups)
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import HelmholtzExterior
+    mesh = G.sphere(96)
+    prob = HelmholtzExterior(mesh, 2.0, clf=64, m=3)
+    _, d_b = prob.rhs(np.array([0.3, 0.2, 0.1]))
+    d_x, it, res = prob.solve(d_b, eps=1e-6, maxiter=40, kmax=30)
+    b = d_b.download()
+    r = h.DeviceArray.from_host(b)
+    prob.V.addeval(-1.0, d_x, r)
+    true = np.linalg.norm(r.download())
+    print("steps", it, "estimate %.3e true %.3e" % (res, true), flush=True)
+    assert it > 0 and np.isfinite(true)
+    # GMRES's least-squares residual is the true residual up to rounding
+    assert abs(true - res) <= 1e-3 * res + 1e-9 * np.linalg.norm(b)
