@@ -160,6 +160,13 @@ release_hip_bem3d(pcbem3d bem)
   }
 }
 
+void
+set_precision_hip_bem3d(pcbem3d bem, int mode)
+{
+  if (h2hip_op_set_precision(lookup_op(bem), mode) != H2HIP_OK)
+    hip_fail("h2hip_op_set_precision");
+}
+
 pbem3d
 new_slp_laplace_hip_bem3d(pcsurface3d gr, uint q_regular, uint q_singular,
                           basisfunctionbem3d row_basis,

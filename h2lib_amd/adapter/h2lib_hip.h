@@ -36,6 +36,11 @@ void setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp);
 /* Releases the device data of bem (call before del_bem3d). */
 void release_hip_bem3d(pcbem3d bem);
 
+/* Precision of the GPU near field of bem (set up by setup_hip_bem3d):
+ * 0 FP64 (default), 1 mixed (h2hip_op_set_precision).  No counterpart in
+ * the reference. */
+void set_precision_hip_bem3d(pcbem3d bem, int mode);
+
 pbem3d new_slp_laplace_hip_bem3d(pcsurface3d gr, uint q_regular,
                                  uint q_singular,
                                  basisfunctionbem3d row_basis,

@@ -662,6 +662,15 @@ ref_bem_use_hip(void *h)
   rb->hip = 1;
 }
 
+/* Precision of the GPU near field installed by ref_bem_use_hip. */
+void
+ref_bem_hip_precision(void *h, int mode)
+{
+  refbem   *rb = (refbem *) h;
+
+  set_precision_hip_bem3d(rb->bem, mode);
+}
+
 /* Packs the assembled H2-matrix into device memory (new_hip_h2matrix). */
 void
 ref_h2_hip_pack(void *h)

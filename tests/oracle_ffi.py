@@ -313,6 +313,7 @@ class Reference:
                                          C.c_double, C.c_uint, C.c_uint]
             if hip:
                 lib.ref_bem_use_hip.argtypes = [C.c_void_p]
+                lib.ref_bem_hip_precision.argtypes = [C.c_void_p, C.c_int]
                 lib.ref_h2_hip_pack.argtypes = [C.c_void_p]
                 lib.ref_h2_hip_addeval.argtypes = [C.c_void_p, C.c_int,
                                                    C.c_void_p, C.c_void_p,
@@ -396,6 +397,11 @@ class ReferenceBem(_BemBase):
         if not self.hip:
             raise RuntimeError("load the reference with hip=True")
         self._lib.ref_bem_use_hip(self.h)
+
+    def hip_precision(self, mode):
+        """Precision of the installed GPU callbacks
+        (set_precision_hip_bem3d)."""
+        self._lib.ref_bem_hip_precision(self.h, int(mode))
 
     def nearfield_blocks(self, ridx, cidx, blocks, threads=1):
         """All blocks (roff, rows, coff, cols) packed as the device plan

@@ -119,3 +119,26 @@ def test_gmres_through_binding():
         print("binding gmres device", device, "steps", it, it0, flush=True)
         assert np.abs(got - want).max() < 1e-6 * np.abs(want).max()
         assert abs(int(it) - int(it0)) <= max(2, it0 // 10)
+
+
+@pytest.mark.parametrize("case", [(0, 1, 1, 1, -0.5, 0.0),
+                                  (1, 0, 0, 0, 0.0, 4.0)])
+def test_mixed_precision_through_binding(case):
+    # set_precision_hip_bem3d on the reference's own bem3d object: its
+    # dense fill now gets mixed-precision entries (within 2e-6 of the
+    # unmodified reference), and switching back restores 1e-12
+    kern, layer, br, bc, alpha, k = case
+    g = golden("mesh_sphere_s4.npz")
+    x, t = g["x"], g["t"]
+    plain = ReferenceBem(kern, layer, br, bc, x, t, q=2, q2=4, alpha=alpha,
+                         k=k)
+    hip = ReferenceBem(kern, layer, br, bc, x, t, hip=True, q=2, q2=4,
+                       alpha=alpha, k=k)
+    hip.use_hip()
+    want = plain.nearfield()
+    hip.hip_precision(1)
+    err = rel_block_err(hip.nearfield(), want)
+    print("binding mixed precision", case, "err %.2e" % err, flush=True)
+    assert 0.0 < err < 2e-6
+    hip.hip_precision(0)
+    assert rel_block_err(hip.nearfield(), want) < TOL
