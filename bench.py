@@ -362,6 +362,78 @@ def bench_helmholtz_solve(h, args, stream):
     return out
 
 
+def bench_serving_multi(h, args):
+    """Serving across meshes and problems: requests arrive as a Poisson
+    stream (args.multi_rate per second, seeded); each picks a mesh from a
+    catalog of sphere / cube / cylinder meshes with 2048 to 131072
+    triangles, Laplace (interior Dirichlet, linear basis, sources outside)
+    or Helmholtz (exterior Dirichlet, constant basis, sources inside, k in
+    {1, 1.5, 2}: below the first interior Dirichlet eigenvalue of all three
+    shapes, where the single layer operator is invertible), with one to four
+    unit point sources.  Operators (H2, interpolation order 4, clf 64) are
+    cached on the device within a budget of args.multi_budget_gb GB and
+    built on demand by a worker thread; GMRES(30) to 1e-10 in batches of up
+    to 8 per operator, chunks of different operators overlapping.  The same
+    request stream is served twice: cold (empty cache) and warm.  Accuracy:
+    the first request of each problem is solved again alone to 1e-13 and
+    compared."""
+    from h2lib_amd import geometry as G
+    from h2lib_amd.serve import Server, draw_requests, mesh_bounds
+    from h2lib_amd.solve import solve_gmres
+    catalog = {"sphere2048": ("sphere", 16), "cube6912": ("cube", 24),
+               "cylinder10240": ("cylinder", 16),
+               "sphere18432": ("sphere", 48), "cube43200": ("cube", 60),
+               "sphere131072": ("sphere", 128)}
+    meshes = {mid: getattr(G, kind)(s) for mid, (kind, s) in catalog.items()}
+    bounds = {mid: mesh_bounds(m) for mid, m in meshes.items()}
+    reqs = draw_requests(args.multi_requests, args.multi_rate,
+                         sorted(meshes), bounds, seed=77)
+    srv = Server(meshes, args.multi_budget_gb * 1e9, eps=1e-10,
+                 maxiter=3000)
+    out = {"catalog": {mid: m.triangles for mid, m in meshes.items()},
+           "requests": len(reqs), "arrival_rate_per_s": args.multi_rate,
+           "budget_gb": args.multi_budget_gb, "gmres_eps": 1e-10}
+    for name in ("cold", "warm"):
+        for r in reqs:
+            r.admitted = r.done = None
+        t0 = time.perf_counter()
+        srv.run(reqs)
+        span = time.perf_counter() - t0
+        lat = np.array([r.done - r.arrival for r in reqs])
+        out[name] = {"solves_per_s": len(reqs) / span,
+                     "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "builds": srv.builds, "evictions": srv.evictions,
+                     "steps_median": int(np.median([r.steps for r in reqs]))}
+        log("bench: serving across meshes (%s): %.1f solves/s, p50 %.0f ms, "
+            "p99 %.0f ms" % (name, out[name]["solves_per_s"],
+                             out[name]["p50_ms"], out[name]["p99_ms"]))
+    # accuracy: first request of each problem, solved alone to 1e-13
+    errs = {}
+    seen = set()
+    for r in reqs:
+        if r.key in seen or len(errs) >= args.multi_check:
+            continue
+        seen.add(r.key)
+        e = srv.cache.get(r.key)
+        if e is None:
+            continue
+        f, b = e.prob.rhs_sources(r.sources, e.stream)
+        x = h.DeviceArray(e.prob.n, b.dtype)
+        x.zero()
+        solve_gmres(e.prob.V, b, x, 1e-13, 6000, 30, e.stream)
+        xr = x.download()
+        errs["%s/%s/k=%g" % r.key] = float(np.linalg.norm(r.x - xr) /
+                                           np.linalg.norm(xr))
+    out["rel_err_vs_tight_solve"] = errs
+    out["per_request"] = [[r.mesh_id, r.kind, r.k, len(r.sources),
+                           round(r.arrival, 4), round(r.done - r.arrival, 4),
+                           r.steps] for r in reqs]
+    out["per_request_fields"] = ["mesh", "kind", "k", "sources", "arrival_s",
+                                 "latency_s_warm", "steps"]
+    return out
+
+
 def bench_dirichlet(h, mesh, args, stream):
     """Dirichlet solves of the interior Laplace problem on the same mesh with
     the operators resident on the device: each solve projects the boundary
@@ -542,6 +614,15 @@ def main():
                     help="GMRES step limit of the batched solves (the "
                     "cylinder meshes give indefinite matrices at q=2 q2=4 "
                     "and stagnate under restarted GMRES)")
+    ap.add_argument("--multi-requests", type=int, default=48,
+                    help="requests of the serving run across meshes (0: "
+                    "skip it)")
+    ap.add_argument("--multi-rate", type=float, default=40.0)
+    ap.add_argument("--multi-budget-gb", type=float, default=24.0,
+                    help="device-memory budget of the operator cache")
+    ap.add_argument("--multi-check", type=int, default=6,
+                    help="requests whose solutions are checked against a "
+                    "tight solve")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -644,6 +725,11 @@ def main():
         log("bench: exterior Helmholtz solves, FP64 and mixed")
         helm_solve = bench_helmholtz_solve(h, args, stream)
 
+    serving_multi = None
+    if args.multi_requests > 0:
+        log("bench: serving across meshes")
+        serving_multi = bench_serving_multi(h, args)
+
     log("bench: dense 2048 x 2048 matrix")
     dense = bench_dense(h, stream)
 
@@ -704,6 +790,7 @@ def main():
         "helmholtz_mixed": helm,
         "helmholtz_solve": helm_solve,
         "batched": batched,
+        "serving_multi": serving_multi,
         "dirichlet": dirichlet,
         "dense": dense,
     }

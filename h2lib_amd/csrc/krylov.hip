@@ -605,6 +605,18 @@ struct h2hip_gmres_batch {
     uint64_t tag = 0;
   } slot[MAXSLOT];
   h2hip::DevBuf<double> d_errs;  // MAXSLOT x CHUNK
+  // the chunk in flight (enqueued, not yet polled): its length, the slots
+  // that take part and whether their first tick starts a cycle; the
+  // residual estimates land in pinned host memory, `done` marks the copy
+  bool inflight = false;
+  int len = 0;
+  int part[MAXSLOT] = {0}, first_start[MAXSLOT] = {0};
+  double *h_errs = nullptr;
+  hipEvent_t done = nullptr;
+  ~h2hip_gmres_batch() {
+    if (h_errs) (void)hipHostFree(h_errs);
+    if (done) (void)hipEventDestroy(done);
+  }
 };
 
 namespace {
@@ -672,11 +684,11 @@ int batch_tick(h2hip_gmres_batch *G, const int *act, const int *ks, int na,
   return H2HIP_OK;
 }
 
-// One chunk: restarts due now, up to CHUNK ticks (none past a restart or
-// maxiter of any slot), one look at the residuals, completions.
+// Enqueues one chunk: restarts due now, up to CHUNK ticks (none past a
+// restart or maxiter of any slot), the copy of the residual estimates to
+// pinned host memory and an event behind it.  Nothing waits.
 template <int C>
-int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
-                uint32_t *iters, double *resid, uint32_t cap) {
+int batch_enqueue(h2hip_gmres_batch *G) {
   // restarts due at this point
   for (int s = 0; s < G->nslots; s++) {
     auto &S = G->slot[s];
@@ -690,6 +702,7 @@ int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
   int len = CHUNK, nact = 0;
   for (int s = 0; s < G->nslots; s++) {
     auto &S = G->slot[s];
+    G->part[s] = S.active ? 1 : 0;
     if (!S.active) continue;
     nact++;
     const int steps_cycle = S.need_start ? G->kmax - 1 : G->kmax - 1 - S.k;
@@ -702,15 +715,15 @@ int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
   }
   if (nact == 0) return H2HIP_OK;
   // enqueue; the host follows each slot's (speculative) column index
-  int kk[MAXSLOT], first_tick_start[MAXSLOT];
+  int kk[MAXSLOT];
   for (int s = 0; s < G->nslots; s++) {
     kk[s] = G->slot[s].need_start ? -1 : G->slot[s].k;
-    first_tick_start[s] = G->slot[s].need_start ? 1 : 0;
+    G->first_start[s] = G->slot[s].need_start ? 1 : 0;
   }
   for (int t = 0; t < len; t++) {
     int act[MAXSLOT], ks[MAXSLOT], na = 0;
     for (int s = 0; s < G->nslots; s++) {
-      if (!G->slot[s].active) continue;
+      if (!G->part[s]) continue;
       act[na] = s;
       ks[na] = kk[s];
       na++;
@@ -719,19 +732,40 @@ int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
     int rc = batch_tick<C>(G, act, ks, na, t);
     if (rc) return rc;
   }
-  double errs[MAXSLOT * CHUNK];
-  H2HIP_CHECK(hipMemcpyAsync(errs, G->d_errs.p,
+  H2HIP_CHECK(hipMemcpyAsync(G->h_errs, G->d_errs.p,
                              sizeof(double) * MAXSLOT * CHUNK,
                              hipMemcpyDeviceToHost, G->st));
-  H2HIP_CHECK(hipStreamSynchronize(G->st));
-  // replay the reference's loop per slot
+  H2HIP_CHECK(hipEventRecord(G->done, G->st));
+  G->len = len;
+  G->inflight = true;
+  return H2HIP_OK;
+}
+
+// Completes the chunk in flight once its estimates have arrived: replays
+// the reference's loop per slot, finishes the slots that stop (waiting for
+// their solution update).  *pending = 1 if the chunk is still running.
+int batch_complete(h2hip_gmres_batch *G, int *pending, uint32_t *ndone,
+                   uint64_t *tags, uint32_t *iters, double *resid,
+                   uint32_t cap) {
+  *ndone = 0;
+  *pending = 0;
+  if (!G->inflight) return H2HIP_OK;
+  const hipError_t q = hipEventQuery(G->done);
+  if (q == hipErrorNotReady) {
+    *pending = 1;
+    return H2HIP_OK;
+  }
+  H2HIP_CHECK(q);
+  G->inflight = false;
+  const double *errs = G->h_errs;
+  const int len = G->len;
   uint32_t nd = 0;
   for (int s = 0; s < G->nslots; s++) {
     auto &S = G->slot[s];
-    if (!S.active) continue;
+    if (!G->part[s] || !S.active) continue;
     bool done = false;
     int t = 0;
-    if (first_tick_start[s]) {
+    if (G->first_start[s]) {
       S.need_start = false;
       S.k = 0;
       if (!S.started) {
@@ -765,6 +799,21 @@ int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
   return H2HIP_OK;
 }
 
+// One chunk, enqueued and completed before returning.
+template <int C>
+int batch_chunk(h2hip_gmres_batch *G, uint32_t *ndone, uint64_t *tags,
+                uint32_t *iters, double *resid, uint32_t cap) {
+  *ndone = 0;
+  if (!G->inflight) {
+    int rc = batch_enqueue<C>(G);
+    if (rc) return rc;
+    if (!G->inflight) return H2HIP_OK;
+  }
+  H2HIP_CHECK(hipEventSynchronize(G->done));
+  int pending = 0;
+  return batch_complete(G, &pending, ndone, tags, iters, resid, cap);
+}
+
 }  // namespace
 
 extern "C" {
@@ -792,9 +841,13 @@ int h2hip_gmres_batch_create(h2hip_gmres_batch **out, const h2hip_h2matrix *H,
       h2hip::set_error("h2hip_gmres_batch_create: device allocation failed");
       return H2HIP_ENOMEM;
     }
-  if (G->d_errs.alloc(MAXSLOT * CHUNK) != hipSuccess) {
+  if (G->d_errs.alloc(MAXSLOT * CHUNK) != hipSuccess ||
+      hipHostMalloc((void **)&G->h_errs, sizeof(double) * MAXSLOT * CHUNK,
+                    hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&G->done, hipEventDisableTiming) !=
+          hipSuccess) {
     delete G;
-    h2hip::set_error("h2hip_gmres_batch_create: device allocation failed");
+    h2hip::set_error("h2hip_gmres_batch_create: allocation failed");
     return H2HIP_ENOMEM;
   }
   *out = G;
@@ -814,6 +867,11 @@ int h2hip_gmres_batch_submit(h2hip_gmres_batch *G, const void *d_b,
                              void *d_x, double eps, uint32_t maxiter,
                              uint64_t tag) {
   if (!G || !d_b || !d_x) return H2HIP_EINVAL;
+  if (G->inflight) {
+    h2hip::set_error("h2hip_gmres_batch_submit: a chunk is in flight "
+                     "(poll it first)");
+    return H2HIP_EINVAL;
+  }
   int s = 0;
   while (s < G->nslots && G->slot[s].active) s++;
   if (s == G->nslots) {
@@ -856,6 +914,30 @@ int h2hip_gmres_batch_run(h2hip_gmres_batch *G, uint32_t *ndone,
   *ndone = 0;
   return G->C ? batch_chunk<1>(G, ndone, tags, iterations, residuals, cap)
               : batch_chunk<0>(G, ndone, tags, iterations, residuals, cap);
+}
+
+int h2hip_gmres_batch_enqueue(h2hip_gmres_batch *G) {
+  if (!G) return H2HIP_EINVAL;
+  if (G->inflight) return H2HIP_OK;
+  return G->C ? batch_enqueue<1>(G) : batch_enqueue<0>(G);
+}
+
+int h2hip_gmres_batch_poll(h2hip_gmres_batch *G, uint32_t *ndone,
+                           uint64_t *tags, uint32_t *iterations,
+                           double *residuals, uint32_t cap) {
+  if (!G || !ndone) return H2HIP_EINVAL;
+  int pending = 0;
+  int rc = batch_complete(G, &pending, ndone, tags, iterations, residuals,
+                          cap);
+  if (rc) return rc;
+  return pending ? 1 : 0;
+}
+
+int h2hip_gmres_batch_busy(const h2hip_gmres_batch *G) {
+  if (!G) return H2HIP_EINVAL;
+  int a = 0;
+  for (int s = 0; s < G->nslots; s++) a += G->slot[s].active;
+  return (G->inflight ? 2 : 0) | (a ? 1 : 0);
 }
 
 }  // extern "C"

@@ -53,6 +53,14 @@ def _sigs():
     h.h2hip_gmres_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint32),
                                         C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_uint32]
+    h.h2hip_gmres_batch_enqueue.restype = C.c_int
+    h.h2hip_gmres_batch_enqueue.argtypes = [C.c_void_p]
+    h.h2hip_gmres_batch_poll.restype = C.c_int
+    h.h2hip_gmres_batch_poll.argtypes = [C.c_void_p, C.POINTER(C.c_uint32),
+                                         C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_uint32]
+    h.h2hip_gmres_batch_busy.restype = C.c_int
+    h.h2hip_gmres_batch_busy.argtypes = [C.c_void_p]
     h.h2hip_gmres_multi_create.restype = C.c_int
     h.h2hip_gmres_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_uint32,
                                            dp, dp, C.c_int, C.c_uint32, dp]
@@ -152,23 +160,46 @@ class GmresBatch:
         self._keep[tag] = (d_b, d_x)
         return s
 
-    def run(self):
-        """Advances all solves by one chunk of steps; returns the solves
-        that finished as (tag, steps, residual)."""
+    def _finished(self, fn):
         cap = 8
         nd = C.c_uint32()
         tags = np.zeros(cap, dtype=np.uint64)
         its = np.zeros(cap, dtype=np.uint32)
         res = np.zeros(cap)
-        check(hip().h2hip_gmres_batch_run(
-            self._h, C.byref(nd), tags.ctypes.data_as(C.c_void_p),
-            its.ctypes.data_as(C.c_void_p), res.ctypes.data_as(C.c_void_p),
-            cap), "h2hip_gmres_batch_run")
+        rc = fn(self._h, C.byref(nd), tags.ctypes.data_as(C.c_void_p),
+                its.ctypes.data_as(C.c_void_p),
+                res.ctypes.data_as(C.c_void_p), cap)
+        if rc < 0:
+            check(rc, "h2hip_gmres_batch")
         out = []
         for i in range(nd.value):
             out.append((int(tags[i]), int(its[i]), float(res[i])))
             self._keep.pop(int(tags[i]), None)
-        return out
+        return rc, out
+
+    def run(self):
+        """Advances all solves by one chunk of steps; returns the solves
+        that finished as (tag, steps, residual)."""
+        return self._finished(hip().h2hip_gmres_batch_run)[1]
+
+    def enqueue(self):
+        """Launches the next chunk without waiting for it."""
+        check(hip().h2hip_gmres_batch_enqueue(self._h),
+              "h2hip_gmres_batch_enqueue")
+
+    def poll(self):
+        """None while the chunk in flight runs; otherwise the solves it
+        finished as (tag, steps, residual)."""
+        rc, out = self._finished(hip().h2hip_gmres_batch_poll)
+        return None if rc == 1 else out
+
+    @property
+    def active(self):
+        return bool(hip().h2hip_gmres_batch_busy(self._h) & 1)
+
+    @property
+    def inflight(self):
+        return bool(hip().h2hip_gmres_batch_busy(self._h) & 2)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -273,6 +304,19 @@ class LaplaceDirichlet:
         self.KM.addeval(1.0, d_f, d_b, stream)
         return d_f, d_b
 
+    def rhs_sources(self, x0s, stream=None):
+        """(f, b) with b = (K + M/2) f for the data of several unit point
+        sources, f = the projected Dirichlet data of the last one and b the
+        sum over all (the problem is linear in the data).  Keep f alive
+        until the stream has passed the product."""
+        d_f = DeviceArray(self.n)
+        d_b = DeviceArray(self.n)
+        d_b.zero()
+        for x0 in x0s:
+            self.project_device(x0, d_f, 0, stream)
+            self.KM.addeval(1.0, d_f, d_b, stream)
+        return d_f, d_b
+
     def solve(self, d_b, eps=1e-8, maxiter=1000, kmax=30, stream=None,
               d_x=None):
         if d_x is None:
@@ -322,6 +366,18 @@ class HelmholtzExterior:
         """(f, b) on the device for the point source at x0."""
         f = helmholtz_point_data(self.mesh, 0, self.k, x0, self.q)
         d_f = DeviceArray.from_host(f)
+        d_b = DeviceArray(self.n, np.complex128)
+        d_b.zero()
+        self.K.addeval(1.0, d_f, d_b, stream)
+        return d_f, d_b
+
+    def rhs_sources(self, x0s, stream=None):
+        """(f, b) with b = (K - M/2) f for several unit point sources (f =
+        sum of their projected Dirichlet data).  Keep f alive until the
+        stream has passed the product."""
+        f = sum(helmholtz_point_data(self.mesh, 0, self.k, x0, self.q)
+                for x0 in x0s)
+        d_f = DeviceArray.from_host(np.asarray(f, dtype=np.complex128))
         d_b = DeviceArray(self.n, np.complex128)
         d_b.zero()
         self.K.addeval(1.0, d_f, d_b, stream)

@@ -269,6 +269,18 @@ int h2hip_gmres_batch_submit(h2hip_gmres_batch *G, const void *d_b,
 int h2hip_gmres_batch_run(h2hip_gmres_batch *G, uint32_t *ndone,
                           uint64_t *tags, uint32_t *iterations,
                           double *residuals, uint32_t cap);
+/* The same chunk in two halves, so that solves with different matrices
+ * (each batch on its own stream) overlap on the device: enqueue launches
+ * the chunk and returns at once (nothing happens if a chunk is in flight or
+ * no solve is active); poll returns 1 while the chunk runs and otherwise
+ * completes it like run (0, with *ndone and the finished solves).  submit
+ * is refused while a chunk is in flight.  busy: bit 0 some solve active,
+ * bit 1 a chunk in flight.  No counterpart in the reference. */
+int h2hip_gmres_batch_enqueue(h2hip_gmres_batch *G);
+int h2hip_gmres_batch_poll(h2hip_gmres_batch *G, uint32_t *ndone,
+                           uint64_t *tags, uint32_t *iterations,
+                           double *residuals, uint32_t cap);
+int h2hip_gmres_batch_busy(const h2hip_gmres_batch *G);
 
 /* ---- many independent systems, one graph launch per Arnoldi step ----------
  * Solves A_s x_s = b_s for count systems (typically one H2-matrix per mesh)

@@ -1,0 +1,75 @@
+"""Serving across meshes and problems (needs a GPU).
+
+* Each request's solution is the same in the mix (admitted while solves on
+  other meshes and problems run, batched with other requests of its
+  operators) as solved alone with the one-system GMRES on the same
+  operators: to 1e-8 relative (both stop at residual 1e-10 |b|; the batched
+  product sums in another order).
+* With a device-memory budget that holds about one operator pair, idle
+  operators are evicted and rebuilt on demand, and every request still
+  completes with its residual bound met.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def catalog():
+    from h2lib_amd import geometry as G
+    from h2lib_amd.serve import mesh_bounds
+    meshes = {"sphere": G.sphere(6), "cube": G.cube(4),
+              "cylinder": G.cylinder(2)}
+    return meshes, {k: mesh_bounds(m) for k, m in meshes.items()}
+
+
+def check_against_alone(srv, reqs, tol):
+    import h2lib_amd as h
+    from h2lib_amd.solve import solve_gmres
+    for r in reqs:
+        e = srv.cache.get(r.key)
+        if e is None:  # evicted: rebuild through the server's builder
+            srv._start_build(r.key)
+            srv._finish_build(wait=True)
+            e = srv.cache[r.key]
+        f, b = e.prob.rhs_sources(r.sources, e.stream)
+        x = h.DeviceArray(e.prob.n, b.dtype)
+        x.zero()
+        it, _ = solve_gmres(e.prob.V, b, x, 1e-10, 3000, 30, e.stream)
+        xa = x.download()
+        err = np.abs(r.x - xa).max() / np.abs(xa).max()
+        print("request", r.rid, r.key, "steps", r.steps, it,
+              "err %.2e" % err, flush=True)
+        assert err < tol, r.rid
+        assert abs(int(r.steps) - int(it)) <= 2
+
+
+def test_each_result_same_alone_and_in_mix():
+    from h2lib_amd.serve import Server, draw_requests
+    meshes, bounds = catalog()
+    reqs = draw_requests(14, 2000.0, sorted(meshes), bounds, seed=4)
+    srv = Server(meshes, 64e9, eps=1e-10, maxiter=3000, clf=16,
+                 m_laplace=3, m_helmholtz=3)
+    srv.run(reqs, timeout=300)
+    kinds = {r.kind for r in reqs}
+    assert kinds == {"laplace", "helmholtz"}
+    assert all(r.done is not None and r.x is not None for r in reqs)
+    # requests really overlapped: some were admitted before others finished
+    assert any(a.admitted < b.done and a.rid != b.rid and a.key != b.key
+               for a in reqs for b in reqs)
+    check_against_alone(srv, reqs, 1e-8)
+
+
+def test_admission_by_footprint_evicts_idle_operators():
+    from h2lib_amd.serve import Server, draw_requests
+    meshes, bounds = catalog()
+    reqs = draw_requests(10, 50.0, sorted(meshes), bounds, seed=9)
+    probe = Server(meshes, 1e12, clf=16, m_laplace=3, m_helmholtz=3)
+    fps = sorted(probe._footprint(r.key) for r in reqs)
+    srv = Server(meshes, 1.3 * fps[-1], eps=1e-10, maxiter=3000, clf=16,
+                 m_laplace=3, m_helmholtz=3)
+    srv.run(reqs, timeout=300)
+    print("builds", srv.builds, "evictions", srv.evictions, flush=True)
+    assert len({r.key for r in reqs}) > 1 and srv.evictions > 0
+    assert srv.used <= srv.budget
+    check_against_alone(srv, reqs, 1e-8)
