@@ -606,7 +606,7 @@ def main():
     ap.add_argument("--helm-steps", type=int, default=5)
     ap.add_argument("--helm-k", type=float, nargs="*", default=[2.0, 4.0, 8.0],
                     help="wave numbers of the Helmholtz solves (none: skip)")
-    ap.add_argument("--helm-maxiter", type=int, default=2000)
+    ap.add_argument("--helm-maxiter", type=int, default=1000)
     ap.add_argument("--batch-meshes", type=int, default=64,
                     help="meshes of the batched run (0: skip it)")
     ap.add_argument("--batch-seed", type=int, default=3)
@@ -618,7 +618,7 @@ def main():
                     help="requests of the serving run across meshes (0: "
                     "skip it)")
     ap.add_argument("--multi-rate", type=float, default=40.0)
-    ap.add_argument("--multi-budget-gb", type=float, default=24.0,
+    ap.add_argument("--multi-budget-gb", type=float, default=96.0,
                     help="device-memory budget of the operator cache")
     ap.add_argument("--multi-check", type=int, default=6,
                     help="requests whose solutions are checked against a "

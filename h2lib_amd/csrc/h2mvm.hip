@@ -533,6 +533,53 @@ __global__ __launch_bounds__(NT) void down_back(DBArgs a) {
   }
 }
 
+// Couplings of interpolation H2-matrices on the device: S_b(i, j) =
+// kernel_const G(xi_i, eta_j) at the tensor Chebyshev points of the row
+// and column boxes of admissible block b (i = (a m + b) m + c over the
+// three coordinates), column-major at the block's coupling offset.
+// G = 1/r (Laplace) or e^{i kr r} e^{-ki r}/r (Helmholtz; complex pairs),
+// as assemble_bem3d_inter_uniform fills them (Library/bem3d.c:6079-6108).
+struct CplArgs {
+  const FarD *far;
+  uint32_t nfar;
+  const double *rpts, *cpts;  // 3 m per node (node records keep the order
+                              // of the tree arrays)
+  uint32_t m;
+  int helmholtz;
+  double kr, kdamp;  // phase factor kr, damping exponent -ki
+  double *S;
+};
+
+__global__ __launch_bounds__(NT) void interp_couplings(CplArgs a) {
+  const uint32_t b = blockIdx.x;
+  if (b >= a.nfar) return;
+  const FarD f = a.far[b];
+  const uint32_t m = a.m, k = m * m * m, kk = k * k;
+  const double *pr = a.rpts + 3 * (uint64_t)m * f.r;
+  const double *pc = a.cpts + 3 * (uint64_t)m * f.c;
+  constexpr double KCONST = 0.0795774715459476679;
+  for (uint32_t e = threadIdx.x; e < kk; e += NT) {
+    const uint32_t i = e % k, j = e / k;
+    const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
+    const uint32_t ja = j / (m * m), jb = (j / m) % m, jc = j % m;
+    const double d0 = pr[ia] - pc[ja], d1 = pr[m + ib] - pc[m + jb],
+                 d2 = pr[2 * m + ic] - pc[2 * m + jc];
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    double rs = 1.0 / sqrt(r2);
+    const uint64_t o = f.soff + e;
+    if (!a.helmholtz) {
+      a.S[o] = KCONST * rs;
+    } else {
+      const double r = r2 * rs;
+      if (a.kdamp != 0.0) rs *= exp(a.kdamp * r);
+      double sn, cs;
+      sincos(a.kr * r, &sn, &cs);
+      a.S[2 * o] = KCONST * (rs * cs);
+      a.S[2 * o + 1] = KCONST * (sn * rs);
+    }
+  }
+}
+
 }  // namespace
 
 struct h2hip_h2matrix {
@@ -970,6 +1017,47 @@ int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N) {
   const int cm = H->cplx ? 2 : 1;
   H2HIP_CHECK(hipMemcpy(H->d_N.p, N, sizeof(double) * cm * H->nlen,
                         hipMemcpyHostToDevice));
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
+                                    const double *k, uint32_t m,
+                                    const double *rpts, const double *cpts,
+                                    void *stream) {
+  if (!H || m < 1 || m > 8 || (!rpts && H->nfar) || (!cpts && H->nfar) ||
+      (kernel != H2HIP_LAPLACE && kernel != H2HIP_HELMHOLTZ) ||
+      (kernel == H2HIP_HELMHOLTZ && (!k || !H->cplx)) ||
+      (kernel == H2HIP_LAPLACE && H->cplx)) {
+    set_error("h2hip_h2matrix_interp_couplings: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  const uint32_t kr = m * m * m;
+  for (int sd = 0; sd < 2; sd++)
+    for (const NodeD &d : H->side[sd].nodes)
+      if (d.rank != kr) {
+        set_error("h2hip_h2matrix_interp_couplings: ranks differ from m^3");
+        return H2HIP_EINVAL;
+      }
+  if (H->nfar == 0) return H2HIP_OK;
+  const uint32_t rn = H->side[0].nnodes, cn = H->side[1].nnodes;
+  h2hip::DevBuf<double> d_rp, d_cp;
+  H2HIP_CHECK(d_rp.upload(rpts, 3 * (uint64_t)m * rn));
+  H2HIP_CHECK(d_cp.upload(cpts, 3 * (uint64_t)m * cn));
+  CplArgs a;
+  a.far = H->d_far.p;
+  a.nfar = H->nfar;
+  a.rpts = d_rp.p;
+  a.cpts = d_cp.p;
+  a.m = m;
+  a.helmholtz = kernel == H2HIP_HELMHOLTZ;
+  a.kr = a.helmholtz ? k[0] : 0.0;
+  a.kdamp = a.helmholtz ? -k[1] : 0.0;
+  a.S = H->d_S.p;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(interp_couplings, dim3(H->nfar), dim3(NT), 0, st, a);
+  H2HIP_CHECK(hipGetLastError());
+  // the point buffers are freed on return: wait for the kernel
+  H2HIP_CHECK(hipStreamSynchronize(st));
   return H2HIP_OK;
 }
 

@@ -54,6 +54,9 @@ def _sigs():
                                            C.c_uint64]
     h.h2hip_h2matrix_set_couplings.argtypes = [vp, dp, C.c_uint64]
     h.h2hip_h2matrix_set_nearfield.argtypes = [vp, dp]
+    h.h2hip_h2matrix_interp_couplings.restype = C.c_int
+    h.h2hip_h2matrix_interp_couplings.argtypes = [vp, C.c_int, dp, u32, dp,
+                                                  dp, vp]
     h.h2hip_h2matrix_nearfield_device.restype = C.c_void_p
     h.h2hip_h2matrix_nearfield_device.argtypes = [vp]
     h.h2hip_h2matrix_bytes.argtypes = [vp, dp]
@@ -87,13 +90,15 @@ class Interpolation:
     10313-10446).  V of node i: size x k column-major at voff[i]; E of node
     i: k x k column-major at i k^2; S of far block b at b k^2."""
 
-    def __init__(self, mesh, rt, ct, btree, row_basis, col_basis, dlp, m, q):
+    def __init__(self, mesh, rt, ct, btree, row_basis, col_basis, dlp, m, q,
+                 couplings=True):
         _sigs()
         ip = bem().h2b_build_h2interp(mesh._sp, rt._tp, ct._tp, btree._bp,
                                       row_basis, col_basis, int(dlp), m, q)
         if not ip:
             raise RuntimeError("interpolation build failed")
-        bem().h2b_h2interp_coupling_laplace(ip, btree._bp)
+        if couplings:
+            bem().h2b_h2interp_coupling_laplace(ip, btree._bp)
         self._ip = ip
         I = ip.contents
         self.m, self.k = I.m, I.k
@@ -112,7 +117,9 @@ class Interpolation:
         self.cV = as_np(I.cV, (max(1, I.cvlen),))
         self.rE = as_np(I.rE, (rt.nnodes * kk,))
         self.cE = as_np(I.cE, (ct.nnodes * kk,))
-        self.S = as_np(I.S, (max(1, I.nfar * kk),))
+        self.S = as_np(I.S, (max(1, I.nfar * kk),)) if couplings else None
+        self.rpts = as_np(I.rpts, (3 * self.m * rt.nnodes,))
+        self.cpts = as_np(I.cpts, (3 * self.m * ct.nnodes,))
         self._bp = btree._bp
         self.nfar = I.nfar
 
@@ -137,13 +144,15 @@ class H2Matrix:
     Library/h2matrix.h:48-71)."""
 
     def __init__(self, op, mesh, rt, ct, btree, m, build_nearfield=True,
-                 stream=None):
+                 stream=None, device_couplings=True):
         _sigs()
         self.op, self.rt, self.ct, self.btree = op, rt, ct, btree
         self.dtype = op.dtype
         self.m = m
         I = Interpolation(mesh, rt, ct, btree, op.row_basis, op.col_basis,
-                          op.layer == DLP, m, op.q_regular)
+                          op.layer == DLP, m, op.q_regular,
+                          couplings=not device_couplings
+                          and op.kernel == LAPLACE)
         self.interp = I
         self.k = I.k
         self.n = rt.n
@@ -160,9 +169,16 @@ class H2Matrix:
         check(hip().h2hip_h2matrix_set_basis(
             h, 1, _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld),
             "h2hip_h2matrix_set_basis")
-        S = I.S if op.kernel == LAPLACE else I.helmholtz_couplings(op.k)
-        check(hip().h2hip_h2matrix_set_couplings(h, _ptr(S), I.eld),
-              "h2hip_h2matrix_set_couplings")
+        if device_couplings:
+            kk = np.array([np.real(op.k), np.imag(op.k)], dtype=np.float64)
+            check(hip().h2hip_h2matrix_interp_couplings(
+                h, op.kernel, _ptr(kk), m, _ptr(I.rpts), _ptr(I.cpts),
+                C.c_void_p(stream.handle if stream else 0)),
+                "h2hip_h2matrix_interp_couplings")
+        else:
+            S = I.S if op.kernel == LAPLACE else I.helmholtz_couplings(op.k)
+            check(hip().h2hip_h2matrix_set_couplings(h, _ptr(S), I.eld),
+                  "h2hip_h2matrix_set_couplings")
         self.nearplan = None
         if build_nearfield:
             self.nearplan = NearfieldPlan(op.gr, op.row_basis, op.col_basis,

@@ -10,7 +10,7 @@ free.  Each cached operator pair owns a stream and a batched GMRES
 different operators are launched without waiting, so solves on different
 meshes overlap on the device, and a request enters between two chunks of
 the solves already running.  New operators are built by a worker thread
-while the loop keeps serving.
+while the loop keeps serving, smallest first.
 
 Problems (the reference's Dirichlet tests, Tests/test_laplacebem3d.c and
 Tests/test_helmholtzbem3d.c):
@@ -19,7 +19,6 @@ Tests/test_helmholtzbem3d.c):
 * Helmholtz: exterior Dirichlet problem, piecewise constant basis, data of
   unit point sources inside the surface, V x = (K - M/2) f.
 """
-import math
 import threading
 import time
 from collections import OrderedDict, deque
@@ -193,12 +192,18 @@ class Server:
                     req.done, req.steps, req.residual = t, steps, res
                     req.x = x.download()
                     ndone += 1
+            # the next operator to build: the smallest one that a queued
+            # request waits for (shortest job first keeps small meshes
+            # from waiting behind large ones)
+            if self._build is None:
+                missing = {r.key for r in queue if r.key not in self.cache}
+                for key in sorted(missing, key=self._footprint):
+                    if self._start_build(key):
+                        break
             # admissions, in arrival order
             for req in list(queue):
                 e = self.cache.get(req.key)
                 if e is None:
-                    if self._build is None:
-                        self._start_build(req.key)
                     continue
                 if e.batch.inflight or e.batch.free_slots() == 0:
                     continue

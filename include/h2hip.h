@@ -185,6 +185,18 @@ int h2hip_h2matrix_set_couplings(h2hip_h2matrix *H, const void *S,
  * plan over the same blocks. */
 int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N);
 void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H);
+/* Couplings of an interpolation H2-matrix computed on the device (instead
+ * of h2hip_h2matrix_set_couplings): kernel_const G(xi, eta) at the tensor
+ * Chebyshev points of both boxes of every admissible block, G = 1/r
+ * (H2HIP_LAPLACE, real matrix) or e^{ik r}/r (H2HIP_HELMHOLTZ, complex
+ * matrix, k = (re, im)), as assemble_bem3d_inter_uniform
+ * (Library/bem3d.c:6079-6108) with fill_kernel_* fills them.  rpts / cpts:
+ * 3 m points per node of each tree (x, y, z), ranks m^3.  Returns when
+ * the couplings are written. */
+int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
+                                    const double *k, uint32_t m,
+                                    const double *rpts, const double *cpts,
+                                    void *stream);
 /* Storage of the matrix in bytes: near field, couplings, bases. */
 int h2hip_h2matrix_bytes(const h2hip_h2matrix *H, uint64_t *out3);
 /* y += alpha H x (addeval_h2matrix_avector, Library/h2matrix.c:1268-1286)

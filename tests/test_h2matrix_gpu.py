@@ -135,3 +135,38 @@ def test_helmholtz_matches_live_reference(layer, basis, k):
         want = rh.addeval(x, y0, alpha=-0.75 + 0.5j, trans=trans)
         got = product(H, x, trans=trans, alpha=-0.75 + 0.5j, y0=y0)
         assert np.abs(got - want).max() < TOL * np.abs(want).max(), trans
+
+
+@pytest.mark.parametrize("kern,layer,basis,k", [(0, 0, 0, 0.0),
+                                                 (0, 1, 1, 0.0),
+                                                 (1, 0, 0, 3.0 + 0.5j),
+                                                 (1, 1, 1, 2.0)])
+def test_device_couplings_equal_host_couplings(kern, layer, basis, k):
+    # couplings computed on the device (h2hip_h2matrix_interp_couplings)
+    # against the host restatement (h2b_h2interp_coupling_*), which is
+    # pinned to the reference: products agree to 1e-13
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.cube(6)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    if kern == 0:
+        op = (h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5) if layer
+              else h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis))
+    else:
+        op = (h.new_dlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, 0.5)
+              if layer else h.new_slp_helmholtz_bem3d(k, gr, 2, 4, basis,
+                                                      basis))
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    Hd = H2Matrix(op, mesh, rt, rt, bt, 3, device_couplings=True)
+    Hh = H2Matrix(op, mesh, rt, rt, bt, 3, device_couplings=False)
+    rng = np.random.default_rng(4)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n)
+    if kern == 1:
+        x = x + 1j * rng.uniform(-1, 1, n)
+    for trans in (False, True):
+        a = product(Hd, x, trans=trans)
+        b = product(Hh, x, trans=trans)
+        assert np.abs(a - b).max() < 1e-13 * np.abs(b).max(), trans
