@@ -59,6 +59,7 @@ def hip():
         _sig(lib, "h2hip_free", i32, [vp])
         _sig(lib, "h2hip_memcpy", i32, [vp, vp, sz, i32])
         _sig(lib, "h2hip_memset", i32, [vp, i32, sz])
+        _sig(lib, "h2hip_memset_async", i32, [vp, i32, sz, vp])
         _sig(lib, "h2hip_stream_create", i32, [C.POINTER(vp)])
         _sig(lib, "h2hip_stream_destroy", i32, [vp])
         _sig(lib, "h2hip_event_create", i32, [C.POINTER(vp)])

@@ -732,18 +732,29 @@ int h2hip_free(void *ptr) {
   return H2HIP_OK;
 }
 
+// The blocking copies and fills complete before they return: work that the
+// caller launches afterwards on a non-blocking stream (which is not ordered
+// after the default stream) sees their result.
 int h2hip_memcpy(void *dst, const void *src, size_t bytes, int kind) {
   if (bytes == 0) return H2HIP_OK;
   const hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
                           : kind == 1 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;
-  H2HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+  H2HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, nullptr));
+  H2HIP_CHECK(hipStreamSynchronize(nullptr));
   return H2HIP_OK;
 }
 
 int h2hip_memset(void *ptr, int value, size_t bytes) {
   if (bytes == 0) return H2HIP_OK;
-  H2HIP_CHECK(hipMemset(ptr, value, bytes));
+  H2HIP_CHECK(hipMemsetAsync(ptr, value, bytes, nullptr));
+  H2HIP_CHECK(hipStreamSynchronize(nullptr));
+  return H2HIP_OK;
+}
+
+int h2hip_memset_async(void *ptr, int value, size_t bytes, void *stream) {
+  if (bytes == 0) return H2HIP_OK;
+  H2HIP_CHECK(hipMemsetAsync(ptr, value, bytes, (hipStream_t)stream));
   return H2HIP_OK;
 }
 

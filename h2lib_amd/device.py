@@ -44,8 +44,16 @@ class DeviceArray:
         # 0xff bytes are NaN for float64 / complex128
         check(hip().h2hip_memset(self._p, 0xff, self.nbytes), "h2hip_memset")
 
-    def zero(self):
-        check(hip().h2hip_memset(self._p, 0, self.nbytes), "h2hip_memset")
+    def zero(self, stream=None):
+        """Zeroes the array: ordered on `stream` if given, otherwise
+        complete on return (so later work on any stream sees it)."""
+        if stream is not None:
+            check(hip().h2hip_memset_async(self._p, 0, self.nbytes,
+                                           C.c_void_p(stream.handle)),
+                  "h2hip_memset_async")
+        else:
+            check(hip().h2hip_memset(self._p, 0, self.nbytes),
+                  "h2hip_memset")
 
     def free(self):
         if getattr(self, "_p", None) and self._p.value:

@@ -210,7 +210,7 @@ class Server:
                 self.cache.move_to_end(req.key)
                 f, b = e.prob.rhs_sources(req.sources, e.stream)
                 x = DeviceArray(e.prob.n, b.dtype)
-                x.zero()
+                x.zero(e.stream)
                 e.batch.submit(b, x, self.eps, self.maxiter, req.rid)
                 e.running[req.rid] = (req, f, b, x)
                 req.admitted = time.perf_counter() - t0

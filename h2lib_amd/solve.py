@@ -300,7 +300,7 @@ class LaplaceDirichlet:
         d_f = DeviceArray(self.n)
         self.project_device(x0, d_f, 0, stream)
         d_b = DeviceArray(self.n)
-        d_b.zero()
+        d_b.zero(stream)
         self.KM.addeval(1.0, d_f, d_b, stream)
         return d_f, d_b
 
@@ -311,7 +311,7 @@ class LaplaceDirichlet:
         until the stream has passed the product."""
         d_f = DeviceArray(self.n)
         d_b = DeviceArray(self.n)
-        d_b.zero()
+        d_b.zero(stream)
         for x0 in x0s:
             self.project_device(x0, d_f, 0, stream)
             self.KM.addeval(1.0, d_f, d_b, stream)
@@ -321,7 +321,7 @@ class LaplaceDirichlet:
               d_x=None):
         if d_x is None:
             d_x = DeviceArray(self.n)
-            d_x.zero()
+            d_x.zero(stream)
         it, res = solve_gmres(self.V, d_b, d_x, eps, maxiter, kmax, stream)
         return d_x, it, res
 
@@ -367,7 +367,7 @@ class HelmholtzExterior:
         f = helmholtz_point_data(self.mesh, 0, self.k, x0, self.q)
         d_f = DeviceArray.from_host(f)
         d_b = DeviceArray(self.n, np.complex128)
-        d_b.zero()
+        d_b.zero(stream)
         self.K.addeval(1.0, d_f, d_b, stream)
         return d_f, d_b
 
@@ -379,13 +379,13 @@ class HelmholtzExterior:
                 for x0 in x0s)
         d_f = DeviceArray.from_host(np.asarray(f, dtype=np.complex128))
         d_b = DeviceArray(self.n, np.complex128)
-        d_b.zero()
+        d_b.zero(stream)
         self.K.addeval(1.0, d_f, d_b, stream)
         return d_f, d_b
 
     def solve(self, d_b, eps=1e-8, maxiter=1000, kmax=30, stream=None):
         d_x = DeviceArray(self.n, np.complex128)
-        d_x.zero()
+        d_x.zero(stream)
         it, res = solve_gmres(self.V, d_b, d_x, eps, maxiter, kmax, stream)
         return d_x, it, res
 

@@ -332,6 +332,9 @@ int h2hip_malloc(void **ptr, size_t bytes);
 int h2hip_free(void *ptr);
 int h2hip_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int h2hip_memset(void *ptr, int value, size_t bytes);
+/* Fill ordered on a stream (not for code under graph capture: a captured
+ * fill node is not replayed reliably; use a kernel there). */
+int h2hip_memset_async(void *ptr, int value, size_t bytes, void *stream);
 int h2hip_stream_create(void **stream);
 int h2hip_stream_destroy(void *stream);
 int h2hip_event_create(void **event);

@@ -73,3 +73,40 @@ def test_admission_by_footprint_evicts_idle_operators():
     assert len({r.key for r in reqs}) > 1 and srv.evictions > 0
     assert srv.used <= srv.budget
     check_against_alone(srv, reqs, 1e-8)
+
+
+def test_fill_completes_before_work_on_other_streams():
+    # Regression: a zero fill issued on the default stream while that stream
+    # was busy (an operator being built) ran after a product launched on a
+    # request's non-blocking stream and wiped the result.  The blocking fill
+    # now completes before it returns; the stream-ordered one follows the
+    # stream.
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(24)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = h.new_slp_laplace_bem3d(gr, 2, 4, 1, 1)
+    rt = G.ClusterTree(mesh, 1, 32)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, 3)
+    stream = h.Stream()
+    n = rt.n
+    x = h.DeviceArray.from_host(np.random.default_rng(1).uniform(-1, 1, n))
+    want = h.DeviceArray(n)
+    want.zero()
+    H.addeval(1.0, x, want, stream)
+    stream.sync()
+    want = want.download()
+    busy = h.DeviceArray(H.nearplan.entries)
+    for stream_ordered in (False, True):
+        for _ in range(30):  # keep the default stream busy
+            H.nearplan.execute(op, busy.ptr, None)
+        y = h.DeviceArray(n)
+        y.fill_nan()
+        y.zero(stream if stream_ordered else None)
+        H.addeval(1.0, x, y, stream)
+        stream.sync()
+        h.hip().h2hip_stream_sync(None)
+        got = y.download()
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
