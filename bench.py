@@ -453,17 +453,26 @@ def bench_dirichlet(h, mesh, args, stream):
         np.random.default_rng(1).uniform(-1.0, 1.0, n))
     d_y = h.DeviceArray(n)
     d_y.zero()
-    for _ in range(3):
-        prob.V.addeval(1.0, d_x, d_y, stream)
-    reps = 50
-    e0, e1 = h.Event(), h.Event()
-    stream.sync()
-    e0.record(stream)
-    for _ in range(reps):
-        prob.V.addeval(1.0, d_x, d_y, stream)
-    e1.record(stream)
-    stream.sync()
-    mv_ms = e0.elapsed_ms(e1) / reps
+    # both schedules of the upward sweep; the faster one serves the solves
+    mv_by = {}
+    for sched in (0, 1):
+        for H in (prob.V, prob.KM):
+            H.set_schedule(sched)
+        for _ in range(3):
+            prob.V.addeval(1.0, d_x, d_y, stream)
+        reps = 50
+        e0, e1 = h.Event(), h.Event()
+        stream.sync()
+        e0.record(stream)
+        for _ in range(reps):
+            prob.V.addeval(1.0, d_x, d_y, stream)
+        e1.record(stream)
+        stream.sync()
+        mv_by[sched] = e0.elapsed_ms(e1) / reps
+    best = min(mv_by, key=mv_by.get)
+    for H in (prob.V, prob.KM):
+        H.set_schedule(best)
+    mv_ms = mv_by[best]
     nbytes = prob.V.bytes()
     mv_bytes = sum(nbytes.values()) + 3 * 8 * n
     mv_gbs = mv_bytes / (mv_ms * 1e-3) / 1e9
@@ -499,7 +508,10 @@ def bench_dirichlet(h, mesh, args, stream):
         "matvec": {
             "kernels": "fwd_up + couple_near + down_back (Laplace SLP)",
             "ms": mv_ms, "bytes": int(mv_bytes), "GB/s": mv_gbs,
-            "hbm_frac": mv_gbs / 8000.0},
+            "hbm_frac": mv_gbs / 8000.0,
+            "ms_by_upward_schedule": {"counters": mv_by[0],
+                                      "level_sync": mv_by[1]},
+            "schedule": "level_sync" if best else "counters"},
     }
     if ref is not None:
         xr = ref["x"]

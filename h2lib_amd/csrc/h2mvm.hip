@@ -127,6 +127,7 @@ struct FwdArgs {
   double *yt;  // cleared here (nyt doubles per vector)
   uint64_t nyt;
   VecSet v;
+  int climb;  // 0: the clusters above the work items are left to fwd_level
 };
 
 // Product of one small column-major matrix M (nout x nin, leading
@@ -284,6 +285,7 @@ __global__ __launch_bounds__(NT) void fwd_up(FwdArgs a) {
   } else {
     leaf_forward<C, K, NV>(a, a.nodes[node], sv, part, nullptr);
   }
+  if (!a.climb) return;
   // climb: the last son of a cluster computes the cluster
   while (true) {
     const uint32_t f = a.nodes[node].father;
@@ -327,6 +329,67 @@ __global__ __launch_bounds__(NT) void fwd_up(FwdArgs a) {
     }
     store_tot(F, rank);
     node = f;
+  }
+}
+
+// Level-synchronous form of the climb: one workgroup per cluster of one
+// height (all its sons lower, computed by earlier launches), xt_f =
+// sum_s E_s^T xt_s.  Launched once per height above the work items of
+// fwd_up (run with climb = 0); kernel boundaries order the levels, so no
+// counters or fences are needed.
+template <int C, int K, int NV>
+__global__ __launch_bounds__(NT) void fwd_level(FwdArgs a,
+                                                const uint32_t *list) {
+  constexpr int RK = rank_cap<NV>();
+  __shared__ double sv[NV * 2 * RK];
+  __shared__ double part[WPB * NV * 2 * RK];
+  const int tid = threadIdx.x;
+  const NodeD F = a.nodes[list[blockIdx.x]];
+  const uint32_t rank = K ? K : F.rank;
+  double tot_re[RK / 64][NV], tot_im[RK / 64][NV];
+#pragma unroll
+  for (int c = 0; c < RK / 64; c++)
+#pragma unroll
+    for (int v = 0; v < NV; v++) tot_re[c][v] = tot_im[c][v] = 0.0;
+  for (uint32_t so = 0; so < F.nsons; so++) {
+    const uint32_t sn = a.sons[F.sonbeg + so];
+    const NodeD s = a.nodes[sn];
+    const uint32_t srank = K ? K : s.rank;
+    for (uint32_t e = tid; e < NV * srank; e += NT) {
+      const uint32_t v = e / srank, i = e - v * srank;
+      double xr = 0.0, xi = 0.0;
+      if (v < a.v.nv) ld<C>(a.xt + v * a.cstride, s.coef + i, xr, xi);
+      sv[v * 2 * RK + 2 * i] = xr;
+      sv[v * 2 * RK + 2 * i + 1] = xi;
+    }
+    __syncthreads();
+    wg_matvec<C, NV, RK>(a.ET + (uint64_t)sn * a.eld, rank, rank, srank, sv,
+                         part);
+    if (tid < 64) {
+#pragma unroll
+      for (int c = 0; c < RK / 64; c++) {
+        const uint32_t j = tid + 64 * c;
+        if (j < rank) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            tot_re[c][v] += part_sum<NV, RK>(part, v, 2 * j);
+            tot_im[c][v] += part_sum<NV, RK>(part, v, 2 * j + 1);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < 64) {
+#pragma unroll
+    for (int c = 0; c < RK / 64; c++) {
+      const uint32_t j = tid + 64 * c;
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        if (j < rank && v < (int)a.v.nv)
+          st<C>(a.xt + v * a.cstride, F.coef + j, tot_re[c][v],
+                tot_im[c][v]);
+    }
   }
 }
 
@@ -589,10 +652,13 @@ struct h2hip_h2matrix {
     uint32_t nnodes = 0, ndof = 0, depth = 0;
     std::vector<NodeD> nodes;
     std::vector<uint32_t> leaves, sons, paths, plen, fwork;
+    // clusters above the fwd_up work items by height (2, 3, ...): nodes of
+    // height h at hnodes[hoff[h - 2] .. hoff[h - 1])
+    std::vector<uint32_t> hnodes, hoff;
     uint64_t ncoef = 0, vlen = 0, eld = 0;
     h2hip::DevBuf<NodeD> d_nodes;
     h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sons, d_paths, d_plen, d_cnt,
-        d_fwork;
+        d_fwork, d_hnodes;
     h2hip::DevBuf<double> d_V, d_VT, d_E, d_ET;
   } side[2];
   uint32_t nnear = 0, nfar = 0;
@@ -603,6 +669,9 @@ struct h2hip_h2matrix {
   h2hip::DevBuf<NearD> d_near;
   h2hip::DevBuf<double> d_S, d_N, d_xt, d_yt;
   uint32_t maxleaf = 0, maxrank = 0;  // over both sides
+  // upward sweep: 0 one launch with counters (the last son computes its
+  // father), 1 one launch per height above the leaves and twigs
+  int level_sync = 0;
   uint64_t cstride = 0;               // doubles per coefficient vector
   // coefficient vectors of the batched products (MAXNV vectors), allocated
   // on first use
@@ -724,6 +793,20 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
     set_error("h2matrix: too many clusters");
     return H2HIP_EINVAL;
   }
+  // heights (leaves 0); sons follow their father in the depth-first arrays
+  std::vector<uint32_t> height(nnodes, 0);
+  for (uint32_t i = nnodes; i-- > 0;) {
+    const uint32_t f = S.nodes[i].father;
+    if (f != NONE) height[f] = std::max(height[f], height[i] + 1);
+  }
+  const uint32_t hmax = height[root];
+  S.hnodes.clear();
+  S.hoff.assign(1, 0);
+  for (uint32_t h = 2; h <= hmax; h++) {
+    for (uint32_t i = 0; i < nnodes; i++)
+      if (height[i] == h) S.hnodes.push_back(i);
+    S.hoff.push_back((uint32_t)S.hnodes.size());
+  }
   S.ncoef = coef;
   S.vlen = v;
   S.eld = maxr * maxr;
@@ -735,7 +818,8 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       S.d_paths.upload(S.paths.data(), S.paths.size()) != hipSuccess ||
       S.d_plen.upload(S.plen.data(), S.plen.size()) != hipSuccess ||
       S.d_cnt.upload(zero.data(), nnodes) != hipSuccess ||
-      S.d_fwork.upload(S.fwork.data(), S.fwork.size()) != hipSuccess) {
+      S.d_fwork.upload(S.fwork.data(), S.fwork.size()) != hipSuccess ||
+      S.d_hnodes.upload(S.hnodes.data(), S.hnodes.size()) != hipSuccess) {
     set_error("h2matrix: device allocation failed");
     return H2HIP_ENOMEM;
   }
@@ -762,7 +846,16 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   f.yt = yt;
   f.nyt = (C ? 2 : 1) * out.ncoef;
   f.v = vs;
+  f.climb = H->level_sync ? 0 : 1;
   hipLaunchKernelGGL((fwd_up<C, K, NV>), dim3(f.nwork), dim3(NT), 0, st, f);
+  if (H->level_sync) {
+    for (size_t h = 0; h + 1 < in.hoff.size(); h++) {
+      const uint32_t cnt = in.hoff[h + 1] - in.hoff[h];
+      if (cnt)
+        hipLaunchKernelGGL((fwd_level<C, K, NV>), dim3(cnt), dim3(NT), 0, st,
+                           f, (const uint32_t *)(in.d_hnodes.p + in.hoff[h]));
+    }
+  }
 
   CNArgs c;
   c.far = H->d_far.p;
@@ -1017,6 +1110,15 @@ int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N) {
   const int cm = H->cplx ? 2 : 1;
   H2HIP_CHECK(hipMemcpy(H->d_N.p, N, sizeof(double) * cm * H->nlen,
                         hipMemcpyHostToDevice));
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync) {
+  if (!H || level_sync < 0 || level_sync > 1) {
+    set_error("h2hip_h2matrix_set_schedule: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  H->level_sync = level_sync;
   return H2HIP_OK;
 }
 

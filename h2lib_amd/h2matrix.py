@@ -54,6 +54,8 @@ def _sigs():
                                            C.c_uint64]
     h.h2hip_h2matrix_set_couplings.argtypes = [vp, dp, C.c_uint64]
     h.h2hip_h2matrix_set_nearfield.argtypes = [vp, dp]
+    h.h2hip_h2matrix_set_schedule.restype = C.c_int
+    h.h2hip_h2matrix_set_schedule.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_interp_couplings.restype = C.c_int
     h.h2hip_h2matrix_interp_couplings.argtypes = [vp, C.c_int, dp, u32, dp,
                                                   dp, vp]
@@ -241,6 +243,13 @@ class H2Matrix:
         self.nearplan.execute(self.op, d, stream.handle if stream else None)
         if stream is None:
             check(hip().h2hip_stream_sync(None), "h2hip_stream_sync")
+
+    def set_schedule(self, level_sync):
+        """Upward sweep in one launch with counters (0) or one launch per
+        tree height (1)."""
+        check(hip().h2hip_h2matrix_set_schedule(self._h, int(level_sync)),
+              "h2hip_h2matrix_set_schedule")
+        self.level_sync = int(level_sync)
 
     def bytes(self):
         out = np.zeros(3, dtype=np.uint64)

@@ -193,6 +193,13 @@ void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H);
  * (Library/bem3d.c:6079-6108) with fill_kernel_* fills them.  rpts / cpts:
  * 3 m points per node of each tree (x, y, z), ranks m^3.  Returns when
  * the couplings are written. */
+/* Schedule of the upward sweep of the products (results agree to the
+ * rounding of the atomic sums): 0 one launch in which the last son to
+ * finish computes its father (counters and fences), 1 one launch for the
+ * leaves and the clusters whose sons are all leaves, then one launch per
+ * height above them.  Not thread-safe against products in flight.  No
+ * counterpart in the reference. */
+int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync);
 int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
                                     const double *k, uint32_t m,
                                     const double *rpts, const double *cpts,

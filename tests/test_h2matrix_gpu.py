@@ -170,3 +170,38 @@ def test_device_couplings_equal_host_couplings(kern, layer, basis, k):
         a = product(Hd, x, trans=trans)
         b = product(Hh, x, trans=trans)
         assert np.abs(a - b).max() < 1e-13 * np.abs(b).max(), trans
+
+
+@pytest.mark.parametrize("kern,basis", [(0, 1), (0, 0), (1, 0)])
+def test_level_synchronous_sweep_equals_counters(kern, basis):
+    # the two schedules of the upward sweep compute the same coefficients;
+    # products (single and batched) agree to the rounding of the atomic sums
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = (h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis) if kern == 0 else
+          h.new_slp_helmholtz_bem3d(2.0, gr, 2, 4, basis, basis))
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, 3)
+    rng = np.random.default_rng(7)
+    n = len(rt.idx)
+    xs = [rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if kern else 0)
+          for _ in range(3)]
+    res = {}
+    for sched in (0, 1):
+        H.set_schedule(sched)
+        for trans in (False, True):
+            res[sched, trans, "one"] = product(H, xs[0], trans=trans)
+            d_x = [h.DeviceArray.from_host(x) for x in xs]
+            d_y = [h.DeviceArray.from_host(np.zeros_like(x)) for x in xs]
+            H.addeval_batch(np.ones(3), d_x, d_y, trans=trans)
+            res[sched, trans, "batch"] = np.concatenate(
+                [d.download() for d in d_y])
+    for trans in (False, True):
+        for kind in ("one", "batch"):
+            a, b = res[1, trans, kind], res[0, trans, kind]
+            assert np.abs(a - b).max() < 1e-13 * np.abs(b).max(), (trans,
+                                                                    kind)
