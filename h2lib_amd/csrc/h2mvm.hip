@@ -522,6 +522,121 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
   }
 }
 
+// Self-adjoint product from the lower block triangle (row and column
+// bases equal): one wave per block.  A far block (r, c) with r > c adds
+// S xt_c to yt_r and S^* xt_r to yt_c, reading S once: lane i keeps row
+// i's sum, and the column sums are reduced across the wave, one per
+// column, and kept by lane j mod 64.  A near block below the diagonal adds
+// alpha N x|_c to y|_r and alpha N^* x|_r to y|_c; a diagonal near block
+// its lower triangle and the conjugate of the part below the diagonal.
+// Blocks above the diagonal are skipped.  (addevalsymm_diag /
+// addevalsymm_offdiag, Library/h2matrix.c:1366-1567.)
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void couple_symm(CNArgs a) {
+  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  double a1r[2] = {0.0, 0.0}, a1i[2] = {0.0, 0.0};  // rows lane, lane + 64
+  double a2r[2] = {0.0, 0.0}, a2i[2] = {0.0, 0.0};  // cols lane, lane + 64
+  if (w < a.nfar) {
+    const FarD b = a.far[w];
+    if (b.r <= b.c) return;  // uniform per wave
+    const NodeD r = a.rn[b.r], c = a.cn[b.c];
+    double xr[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int h = 0; h < 2; h++)
+      if (lane + 64 * h < r.rank)
+        ld<C>(a.xt, r.coef + lane + 64 * h, xr[h][0], xr[h][1]);
+    for (uint32_t j = 0; j < c.rank; j++) {
+      double xcr, xci;
+      ld<C>(a.xt, c.coef + j, xcr, xci);
+      double pr = 0.0, pi = 0.0;
+      for (int h = 0; h < 2; h++) {
+        const uint32_t i = lane + 64 * h;
+        if (i < r.rank) {
+          double sr, si;
+          ld<C>(a.S, b.soff + i + (uint64_t)j * r.rank, sr, si);
+          a1r[h] = fma(sr, xcr, a1r[h]);
+          pr = fma(sr, xr[h][0], pr);
+          if constexpr (C) {
+            a1r[h] = fma(-si, xci, a1r[h]);
+            a1i[h] = fma(sr, xci, fma(si, xcr, a1i[h]));
+            // conj(s) x_r
+            pr = fma(si, xr[h][1], pr);
+            pi = fma(sr, xr[h][1], fma(-si, xr[h][0], pi));
+          }
+        }
+      }
+      pr = wave_sum64(pr);
+      if constexpr (C) pi = wave_sum64(pi);
+      if (lane == (j & 63)) {
+        a2r[j >> 6] = pr;
+        a2i[j >> 6] = pi;
+      }
+    }
+    for (int h = 0; h < 2; h++) {
+      if (lane + 64 * h < r.rank)
+        atomic_add<C>(a.yt, r.coef + lane + 64 * h, a1r[h], a1i[h]);
+      if (lane + 64 * h < c.rank)
+        atomic_add<C>(a.yt, c.coef + lane + 64 * h, a2r[h], a2i[h]);
+    }
+    return;
+  }
+  const uint32_t nb = w - a.nfar;
+  if (nb >= a.nnear) return;
+  const NearD b = a.near[nb];
+  if (b.roff < b.coff) return;  // above the diagonal: skipped
+  const bool diag = b.roff == b.coff;
+  const uint32_t *ridx = a.ridx + b.roff, *cidx = a.cidx + b.coff;
+  double xr[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int h = 0; h < 2; h++)
+    if (lane + 64 * h < b.rows)
+      ld<C>(a.v.x[0], ridx[lane + 64 * h], xr[h][0], xr[h][1]);
+  for (uint32_t j = 0; j < b.cols; j++) {
+    double xcr, xci;
+    ld<C>(a.v.x[0], cidx[j], xcr, xci);
+    double pr = 0.0, pi = 0.0;
+    for (int h = 0; h < 2; h++) {
+      const uint32_t i = lane + 64 * h;
+      if (i < b.rows && (!diag || i >= j)) {
+        double nr, ni;
+        ld<C>(a.N, b.noff + i + (uint64_t)j * b.rows, nr, ni);
+        a1r[h] = fma(nr, xcr, a1r[h]);
+        if constexpr (C) {
+          a1r[h] = fma(-ni, xci, a1r[h]);
+          a1i[h] = fma(nr, xci, fma(ni, xcr, a1i[h]));
+        }
+        if (!diag || i > j) {
+          pr = fma(nr, xr[h][0], pr);
+          if constexpr (C) {
+            pr = fma(ni, xr[h][1], pr);
+            pi = fma(nr, xr[h][1], fma(-ni, xr[h][0], pi));
+          }
+        }
+      }
+    }
+    pr = wave_sum64(pr);
+    if constexpr (C) pi = wave_sum64(pi);
+    if (lane == (j & 63)) {
+      a2r[j >> 6] = pr;
+      a2i[j >> 6] = pi;
+    }
+  }
+  const double ar = a.v.are[0], ai = a.v.aim[0];
+  for (int h = 0; h < 2; h++) {
+    if (lane + 64 * h < b.rows)
+      atomic_add<C>(a.v.y[0], ridx[lane + 64 * h], ar * a1r[h] - ai * a1i[h],
+                    ar * a1i[h] + ai * a1r[h]);
+    if (lane + 64 * h < b.cols)
+      atomic_add<C>(a.v.y[0], cidx[lane + 64 * h], ar * a2r[h] - ai * a2i[h],
+                    ar * a2i[h] + ai * a2r[h]);
+  }
+}
+
 struct DBArgs {
   const NodeD *nodes;
   const uint32_t *leaves;
@@ -672,6 +787,10 @@ struct h2hip_h2matrix {
   // upward sweep: 0 one launch with counters (the last son computes its
   // father), 1 one launch per height above the leaves and twigs
   int level_sync = 0;
+  // checksums of the bases as set, per side (the self-adjoint product
+  // requires equal row and column bases)
+  double basis_sum[2] = {0.0, 0.0};
+  int basis_set[2] = {0, 0};
   uint64_t cstride = 0;               // doubles per coefficient vector
   // coefficient vectors of the batched products (MAXNV vectors), allocated
   // on first use
@@ -828,7 +947,7 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
 
 template <int C, int K, int NV>
 void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
-            hipStream_t st, bool trans) {
+            hipStream_t st, bool trans, bool symm = false) {
   const auto &in = H->side[trans ? 0 : 1];
   const auto &out = H->side[trans ? 1 : 0];
   FwdArgs f;
@@ -873,7 +992,11 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   c.cstride = H->cstride;
   c.v = vs;
   const uint32_t nw = H->nfar + H->nnear;
-  if (nw) {
+  if (nw && symm) {
+    if constexpr (NV == 1)
+      hipLaunchKernelGGL((couple_symm<C>), dim3((nw + WPB - 1) / WPB),
+                         dim3(NT), 0, st, c);
+  } else if (nw) {
     if (trans)
       hipLaunchKernelGGL((couple_near<C, true, NV>),
                          dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
@@ -897,6 +1020,27 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   d.v = vs;
   hipLaunchKernelGGL((down_back<C, K, NV>), dim3(d.nleaves), dim3(NT), 0, st,
                      d);
+}
+
+template <int C>
+int apply_symm(const double *alpha, const h2hip_h2matrix *H, const double *x,
+               double *y, hipStream_t st) {
+  VecSet vs = {};
+  vs.x[0] = x;
+  vs.y[0] = y;
+  vs.are[0] = alpha[0];
+  vs.aim[0] = alpha[1];
+  vs.nv = 1;
+  double *xt = H->d_xt.p, *yt = H->d_yt.p;
+  switch (H->urank) {
+    case 8: launch<C, 8, 1>(vs, H, xt, yt, st, false, true); break;
+    case 27: launch<C, 27, 1>(vs, H, xt, yt, st, false, true); break;
+    case 64: launch<C, 64, 1>(vs, H, xt, yt, st, false, true); break;
+    case 125: launch<C, 125, 1>(vs, H, xt, yt, st, false, true); break;
+    default: launch<C, 0, 1>(vs, H, xt, yt, st, false, true); break;
+  }
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
 }
 
 template <int C>
@@ -1081,6 +1225,15 @@ int h2hip_h2matrix_set_basis(h2hip_h2matrix *H, int side, const double *V,
         }
     }
   }
+  // checksum of the side's bases (for the self-adjoint product, which
+  // needs both sides equal)
+  {
+    double cs = 0.0, w = 1.0;
+    for (size_t i = 0; i < v.size(); i++, w += 1e-7) cs += w * v[i];
+    for (size_t i = 0; i < e.size(); i++, w += 1e-7) cs += w * e[i];
+    H->basis_sum[side] = cs;
+    H->basis_set[side] = 1;
+  }
   H2HIP_CHECK(S.d_V.upload(v.data(), v.size()));
   H2HIP_CHECK(S.d_VT.upload(vt.data(), vt.size()));
   H2HIP_CHECK(S.d_E.upload(e.data(), e.size()));
@@ -1111,6 +1264,35 @@ int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N) {
   H2HIP_CHECK(hipMemcpy(H->d_N.p, N, sizeof(double) * cm * H->nlen,
                         hipMemcpyHostToDevice));
   return H2HIP_OK;
+}
+
+int h2hip_addevalsymm_h2matrix(const double *alpha, const h2hip_h2matrix *H,
+                               const void *x, void *y, void *stream) {
+  if (!alpha || !H || !x || !y) {
+    set_error("h2hip_addevalsymm_h2matrix: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  const auto &r = H->side[0], &c = H->side[1];
+  bool same = r.nnodes == c.nnodes && r.ncoef == c.ncoef &&
+              H->basis_set[0] && H->basis_set[1] &&
+              H->basis_sum[0] == H->basis_sum[1];
+  for (uint32_t i = 0; same && i < r.nnodes; i++)
+    same = r.nodes[i].off == c.nodes[i].off &&
+           r.nodes[i].size == c.nodes[i].size &&
+           r.nodes[i].rank == c.nodes[i].rank;
+  if (!same) {
+    set_error("h2hip_addevalsymm_h2matrix: row and column trees and bases "
+              "must be equal");
+    return H2HIP_EINVAL;
+  }
+  if (H->maxleaf > 128 || H->maxrank > 128) {
+    set_error("h2hip_addevalsymm_h2matrix: leaves and ranks up to 128");
+    return H2HIP_EINVAL;
+  }
+  return H->cplx ? apply_symm<1>(alpha, H, (const double *)x, (double *)y,
+                                 (hipStream_t)stream)
+                 : apply_symm<0>(alpha, H, (const double *)x, (double *)y,
+                                 (hipStream_t)stream);
 }
 
 int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync) {

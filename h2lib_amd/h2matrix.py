@@ -54,6 +54,8 @@ def _sigs():
                                            C.c_uint64]
     h.h2hip_h2matrix_set_couplings.argtypes = [vp, dp, C.c_uint64]
     h.h2hip_h2matrix_set_nearfield.argtypes = [vp, dp]
+    h.h2hip_addevalsymm_h2matrix.restype = C.c_int
+    h.h2hip_addevalsymm_h2matrix.argtypes = [dp, vp, vp, vp, vp]
     h.h2hip_h2matrix_set_schedule.restype = C.c_int
     h.h2hip_h2matrix_set_schedule.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_interp_couplings.restype = C.c_int
@@ -146,8 +148,17 @@ class H2Matrix:
     Library/h2matrix.h:48-71)."""
 
     def __init__(self, op, mesh, rt, ct, btree, m, build_nearfield=True,
-                 stream=None, device_couplings=True):
+                 stream=None, device_couplings=True, symmetric=False):
+        """symmetric: keep only the blocks on and below the block diagonal
+        (half the storage) for addevalsymm; the operator must be
+        self-adjoint with equal row and column bases (a Laplace single
+        layer operator)."""
         _sigs()
+        if symmetric and (rt is not ct or op.row_basis != op.col_basis or
+                          op.layer == DLP or op.kernel != LAPLACE):
+            raise ValueError("the symmetric storage needs a self-adjoint "
+                             "operator on one tree with equal bases")
+        self.symmetric = symmetric
         self.op, self.rt, self.ct, self.btree = op, rt, ct, btree
         self.dtype = op.dtype
         self.m = m
@@ -158,12 +169,22 @@ class H2Matrix:
         self.interp = I
         self.k = I.k
         self.n = rt.n
+        near, far = I.near, I.far
+        nbl = btree.near_block_list()
+        if symmetric:
+            # blocks on or below the block diagonal: row node >= column
+            # node (sons are numbered in order, as the reference walks them)
+            keep = near[:, 0] >= near[:, 1]
+            near = np.ascontiguousarray(near[keep])
+            nbl = np.ascontiguousarray(np.asarray(nbl)[keep])
+            far = np.ascontiguousarray(far[far[:, 0] > far[:, 1]])
+        self.near_nodes, self.far_nodes = near, far
         h = C.c_void_p()
         check(hip().h2hip_h2matrix_create(
             C.byref(h), int(op.is_complex), rt.nnodes, _ptr(I.rtree),
             _ptr(I.rrank), rt.n, _ptr(rt.idx), ct.nnodes, _ptr(I.ctree),
-            _ptr(I.crank), ct.n, _ptr(ct.idx), len(I.near), _ptr(I.near),
-            len(I.far), _ptr(I.far)), "h2hip_h2matrix_create")
+            _ptr(I.crank), ct.n, _ptr(ct.idx), len(near), _ptr(near),
+            len(far), _ptr(far)), "h2hip_h2matrix_create")
         self._h = h
         check(hip().h2hip_h2matrix_set_basis(
             h, 0, _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld),
@@ -178,14 +199,16 @@ class H2Matrix:
                 C.c_void_p(stream.handle if stream else 0)),
                 "h2hip_h2matrix_interp_couplings")
         else:
+            if symmetric:
+                raise ValueError("symmetric storage computes its couplings "
+                                 "on the device")
             S = I.S if op.kernel == LAPLACE else I.helmholtz_couplings(op.k)
             check(hip().h2hip_h2matrix_set_couplings(h, _ptr(S), I.eld),
                   "h2hip_h2matrix_set_couplings")
         self.nearplan = None
         if build_nearfield:
             self.nearplan = NearfieldPlan(op.gr, op.row_basis, op.col_basis,
-                                          rt.idx, ct.idx,
-                                          btree.near_block_list())
+                                          rt.idx, ct.idx, nbl)
             self.assemble_nearfield(stream)
 
     @classmethod
@@ -266,6 +289,15 @@ class H2Matrix:
         check(f(_ptr(a), self._h, C.c_void_p(d_x.ptr), C.c_void_p(d_y.ptr),
                 C.c_void_p(stream.handle if stream else 0)),
               "h2hip_addeval_h2matrix")
+
+    def addevalsymm(self, alpha, d_x, d_y, stream=None):
+        """y += alpha H x from the blocks on and below the block diagonal
+        (addevalsymm_h2matrix_avector)."""
+        a = np.array([np.real(alpha), np.imag(alpha)], dtype=np.float64)
+        check(hip().h2hip_addevalsymm_h2matrix(
+            _ptr(a), self._h, C.c_void_p(d_x.ptr), C.c_void_p(d_y.ptr),
+            C.c_void_p(stream.handle if stream else 0)),
+            "h2hip_addevalsymm_h2matrix")
 
     def addeval_batch(self, alphas, d_xs, d_ys, stream=None, trans=False):
         """y_v += alpha_v H x_v (trans: H^*) for up to 8 device arrays at

@@ -193,6 +193,15 @@ void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H);
  * (Library/bem3d.c:6079-6108) with fill_kernel_* fills them.  rpts / cpts:
  * 3 m points per node of each tree (x, y, z), ranks m^3.  Returns when
  * the couplings are written. */
+/* y += alpha H x for a self-adjoint H (addevalsymm_h2matrix_avector,
+ * Library/h2matrix.c:1570-1604): only the blocks on and below the block
+ * diagonal are used (row cluster after column cluster; the matrix may be
+ * created with just those, which halves its storage), each below the
+ * diagonal also as its adjoint, diagonal near blocks with their lower
+ * triangle.  Row and column trees and cluster bases must be equal (as the
+ * interpolation bases of a single layer operator are). */
+int h2hip_addevalsymm_h2matrix(const double *alpha, const h2hip_h2matrix *H,
+                               const void *x, void *y, void *stream);
 /* Schedule of the upward sweep of the products (results agree to the
  * rounding of the atomic sums): 0 one launch in which the last son to
  * finish computes its father (counters and fences), 1 one launch for the

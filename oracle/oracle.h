@@ -138,6 +138,17 @@ int orc_h2_addeval(int is_complex, int trans, const double *alpha,
                    uint32_t nnear, const uint32_t *near, const double *N,
                    const double *x, double *y);
 
+/* y += alpha H x for a self-adjoint H from its lower block triangle
+ * (addevalsymm_h2matrix_avector, Library/h2matrix.c:1366-1604): blocks with
+ * row node > column node act with S and S^*, diagonal leaf blocks with their
+ * lower triangle and its conjugate, blocks with row node < column node are
+ * ignored.  Same arguments as orc_h2_addeval.  Returns 0, or -1. */
+int orc_h2_addevalsymm(int is_complex, const double *alpha,
+                       const orc_cbasis *rb, const orc_cbasis *cb,
+                       uint32_t nfar, const uint32_t *far, const double *S,
+                       uint64_t sld, uint32_t nnear, const uint32_t *near,
+                       const double *N, const double *x, double *y);
+
 /* ---- GMRES ----------------------------------------------------------------- */
 
 /* y += alpha A x (the reference's addeval_t, Library/krylov.h:41, real). */

@@ -258,3 +258,111 @@ out:
   side_free(&c);
   return rc;
 }
+
+/* y += alpha H x for a self-adjoint H given by its lower block triangle
+ * (addevalsymm_h2matrix_avector, Library/h2matrix.c:1570-1604, with
+ * addevalsymm_diag / addevalsymm_offdiag, Library/h2matrix.c:1366-1567):
+ * the blocks (r, c) with r > c (node numbers; sons are numbered in order,
+ * so this is the reference's "row son after column son") act as
+ * S xt_c -> yt_r and S^* xta_r -> yta_c, diagonal leaf blocks (r == c)
+ * with their lower triangle and its conjugate; blocks with r < c are
+ * skipped.  xt / yta use the column basis, xta / yt the row basis. */
+int orc_h2_addevalsymm(int is_complex, const double *alpha,
+                       const orc_cbasis *rb, const orc_cbasis *cb,
+                       uint32_t nfar, const uint32_t *far, const double *S,
+                       uint64_t sld, uint32_t nnear, const uint32_t *near,
+                       const double *N, const double *x, double *y) {
+  side_t r, c;
+  memset(&r, 0, sizeof r);
+  memset(&c, 0, sizeof c);
+  int rc = -1;
+  if (side_init(&r, rb) || side_init(&c, cb) || r.n != c.n) goto out;
+  const int cm = is_complex ? 1 : 0;
+  const cplx a = alpha[0] + (is_complex ? I * alpha[1] : 0.0);
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < c.n; i++)
+    if (c.tree[3 * i + 2] == NONE) n += c.tree[3 * i + 1];
+  cplx *xv = calloc(n + 1, sizeof(cplx));
+  cplx *yv = calloc(n + 1, sizeof(cplx));
+  cplx *xt = calloc(c.ncoef + 1, sizeof(cplx));
+  cplx *xta = calloc(r.ncoef + 1, sizeof(cplx));
+  cplx *yt = calloc(r.ncoef + 1, sizeof(cplx));
+  cplx *yta = calloc(c.ncoef + 1, sizeof(cplx));
+  cplx *yp = calloc(r.nleaf + 1, sizeof(cplx));
+  cplx *ypa = calloc(c.nleaf + 1, sizeof(cplx));
+  if (!xv || !yv || !xt || !xta || !yt || !yta || !yp || !ypa) goto out2;
+  for (uint64_t i = 0; i < n; i++) xv[i] = val(x, i, cm);
+  for (uint32_t i = 0; i < c.n; i++)
+    if (c.tree[3 * i + 2] == NONE) {
+      forward(&c, i, xv, xt);
+      forward(&r, i, xv, xta);
+    }
+  for (uint32_t b = 0; b < nfar; b++) {
+    const uint32_t tr = far[2 * b], tc = far[2 * b + 1];
+    if (tr <= tc) continue;
+    const uint32_t kr = r.rank[tr], kc = c.rank[tc];
+    const uint64_t so = (uint64_t)b * sld;
+    cplx *o = yt + r.coef[tr], *oa = yta + c.coef[tc];
+    const cplx *i0 = xt + c.coef[tc], *ia = xta + r.coef[tr];
+    for (uint32_t j = 0; j < kc; j++) {
+      cplx sum = 0.0;
+      for (uint32_t i = 0; i < kr; i++) {
+        const cplx sv = val(S, so + i + (uint64_t)j * kr, cm);
+        o[i] += a * sv * i0[j];
+        sum += conj(sv) * ia[i];
+      }
+      oa[j] += a * sum;
+    }
+  }
+  uint64_t no = 0;
+  for (uint32_t b = 0; b < nnear; b++) {
+    const uint32_t tr = near[2 * b], tc = near[2 * b + 1];
+    const uint32_t roff = r.tree[3 * tr], rows = r.tree[3 * tr + 1];
+    const uint32_t coff = c.tree[3 * tc], cols = c.tree[3 * tc + 1];
+    if (r.nsons[tr] || c.nsons[tc]) goto out2;
+    if (tr >= tc) {
+      cplx *o = yp + r.leafoff[tr], *oa = ypa + c.leafoff[tc];
+      for (uint32_t j = 0; j < cols; j++) {
+        const cplx xj = xv[c.idx[coff + j]];
+        /* Library/h2matrix.c:1486-1492 on the diagonal: entry (j, j) once,
+         * i > j as a_ij and conj(a_ij) */
+        const uint32_t i0 = tr == tc ? j : 0;
+        cplx sum = 0.0;
+        for (uint32_t i = i0; i < rows; i++) {
+          const cplx av = val(N, no + i + (uint64_t)j * rows, cm);
+          o[i] += a * av * xj;
+          if (tr != tc || i > j) sum += conj(av) * xv[r.idx[roff + i]];
+        }
+        oa[j] += a * sum;
+      }
+    }
+    no += (uint64_t)rows * cols;
+  }
+  for (uint32_t i = 0; i < r.n; i++)
+    if (r.tree[3 * i + 2] == NONE) {
+      backward(&r, i, yt, yp, yv);
+      backward(&c, i, yta, ypa, yv);
+    }
+  for (uint64_t i = 0; i < n; i++) {
+    if (cm) {
+      y[2 * i] += creal(yv[i]);
+      y[2 * i + 1] += cimag(yv[i]);
+    } else {
+      y[i] += creal(yv[i]);
+    }
+  }
+  rc = 0;
+out2:
+  free(xv);
+  free(yv);
+  free(xt);
+  free(xta);
+  free(yt);
+  free(yta);
+  free(yp);
+  free(ypa);
+out:
+  side_free(&r);
+  side_free(&c);
+  return rc;
+}

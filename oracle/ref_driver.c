@@ -535,6 +535,27 @@ ref_h2_addeval(void *h, int trans, const double *alpha, const void *x,
     addeval_h2matrix_avector(a, r->h2, xv, yv);
 }
 
+/* y += alpha H x with H self-adjoint, from its lower block triangle
+ * (addevalsymm_h2matrix_avector, Library/h2matrix.c:1570-1604). */
+void
+ref_h2_addevalsymm(void *h, const double *alpha, const void *x, void *y)
+{
+  refh2    *r = (refh2 *) h;
+  avector   xt, yt;
+  pavector  xv, yv;
+  uint      n = r->root->size;
+  field     a;
+
+#ifdef USE_COMPLEX
+  a = alpha[0] + alpha[1] * I;
+#else
+  a = alpha[0];
+#endif
+  xv = init_pointer_avector(&xt, (field *) x, n);
+  yv = init_pointer_avector(&yt, (field *) y, n);
+  addevalsymm_h2matrix_avector(a, r->h2, xv, yv);
+}
+
 /* Interior Dirichlet problem of the Laplace equation with the piecewise
  * linear basis, as Tests/test_laplacebem3d.c:100-160 sets it up but solved
  * with GMRES(kmax): f = L2 projection of u(x) = 1/|x - (1.2, 1.2, 1.2)|,

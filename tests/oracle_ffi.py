@@ -100,10 +100,12 @@ class Oracle:
         return x, y, w, base.value
 
     @classmethod
-    def h2_addeval(cls, interp, N, x, y, alpha=1.0, trans=False, S=None):
+    def h2_addeval(cls, interp, N, x, y, alpha=1.0, trans=False, S=None,
+                   symm=False):
         """y += alpha H x (H^* x) for the host arrays of an
         h2lib_amd.h2matrix.Interpolation and packed near-field blocks N
-        (in the order of interp.near)."""
+        (in the order of interp.near); symm: the self-adjoint product from
+        the lower block triangle (orc_h2_addevalsymm)."""
         lib = cls.lib()
         if not hasattr(lib, "_h2sig"):
             lib.orc_h2_addeval.restype = C.c_int
@@ -135,10 +137,24 @@ class Oracle:
                 _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld)
         cb = CB(len(I.ctree), _ptr(I.ctree), _ptr(I.crank), _ptr(I.cidx),
                 _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld)
-        rc = lib.orc_h2_addeval(int(cplx), int(trans), _ptr(a), C.byref(rb),
-                                C.byref(cb), len(I.far), _ptr(I.far),
-                                _ptr(S), I.eld, len(I.near), _ptr(I.near),
-                                _ptr(N), _ptr(xv), _ptr(y))
+        if symm:
+            if not hasattr(lib, "_h2ssig"):
+                lib.orc_h2_addevalsymm.restype = C.c_int
+                lib.orc_h2_addevalsymm.argtypes = [
+                    C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                    C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                lib._h2ssig = True
+            rc = lib.orc_h2_addevalsymm(int(cplx), _ptr(a), C.byref(rb),
+                                        C.byref(cb), len(I.far), _ptr(I.far),
+                                        _ptr(S), I.eld, len(I.near),
+                                        _ptr(I.near), _ptr(N), _ptr(xv),
+                                        _ptr(y))
+        else:
+            rc = lib.orc_h2_addeval(int(cplx), int(trans), _ptr(a),
+                                    C.byref(rb), C.byref(cb), len(I.far),
+                                    _ptr(I.far), _ptr(S), I.eld, len(I.near),
+                                    _ptr(I.near), _ptr(N), _ptr(xv), _ptr(y))
         if rc != 0:
             raise ValueError("orc_h2_addeval rejected its input")
         return y
@@ -306,6 +322,8 @@ class Reference:
             lib.ref_h2_near_entries.argtypes = [C.c_void_p, C.c_uint,
                                                 C.c_void_p]
             lib.ref_h2_sizes.argtypes = [C.c_void_p, C.c_void_p]
+            lib.ref_h2_addevalsymm.argtypes = [C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
             lib.ref_h2_addeval.argtypes = [C.c_void_p, C.c_int, C.c_void_p,
                                            C.c_void_p, C.c_void_p]
             lib.ref_h2_gmres.restype = C.c_uint
@@ -481,6 +499,13 @@ class ReferenceH2:
         it = self._lib.ref_h2_hip_gmres(self.h, int(bool(device)), _ptr(bb),
                                         _ptr(xx), eps, maxiter, kmax)
         return xx, it
+
+    def addevalsymm(self, x, y, alpha=1.0):
+        a = np.array([np.real(alpha), np.imag(alpha)])
+        xx = np.ascontiguousarray(x, dtype=self.bem.dtype)
+        yy = np.ascontiguousarray(y, dtype=self.bem.dtype).copy()
+        self._lib.ref_h2_addevalsymm(self.h, _ptr(a), _ptr(xx), _ptr(yy))
+        return yy
 
     def gmres(self, b, x0, eps, maxiter, kmax):
         bb = np.ascontiguousarray(b, dtype=self.bem.dtype)

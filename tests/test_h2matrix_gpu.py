@@ -205,3 +205,59 @@ def test_level_synchronous_sweep_equals_counters(kern, basis):
             a, b = res[1, trans, kind], res[0, trans, kind]
             assert np.abs(a - b).max() < 1e-13 * np.abs(b).max(), (trans,
                                                                     kind)
+
+
+@pytest.mark.skipif(not have_reference(False), reason="reference not built")
+@pytest.mark.parametrize("basis,clf,m", [(1, 16, 3), (0, 24, 3), (1, 32, 5)])
+def test_symmetric_product_matches_reference(basis, clf, m):
+    # addevalsymm from the blocks on and below the block diagonal only
+    # (half the storage) against the reference's
+    # addevalsymm_h2matrix_avector and the restatement, 1e-12; m = 5 gives
+    # ranks of 125 (two rows per lane in the coupling kernel)
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    from oracle_ffi import Oracle
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis)
+    rt = G.ClusterTree(mesh, basis, clf)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    Hs = H2Matrix(op, mesh, rt, rt, bt, m, symmetric=True)
+    Hf = H2Matrix(op, mesh, rt, rt, bt, m)
+    ref = ReferenceBem(0, 0, basis, basis, mesh.x, mesh.t, q=2, q2=4)
+    rh = ReferenceH2(ref, clf, 2.0, m, 0)
+    assert np.array_equal(rh.idx(), rt.idx)
+    rng = np.random.default_rng(3)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, n)
+    want = rh.addevalsymm(x, y0, alpha=-0.5)
+    dx = h.DeviceArray.from_host(x)
+    dy = h.DeviceArray.from_host(y0)
+    Hs.addevalsymm(-0.5, dx, dy)
+    got = dy.download()
+    print("symm basis", basis, "clf", clf, "m", m,
+          "err %.2e" % (np.abs(got - want).max() / np.abs(want).max()),
+          flush=True)
+    assert np.abs(got - want).max() < TOL * np.abs(want).max()
+    # storage: about half of the full matrix's near field and couplings
+    bs, bf = Hs.bytes(), Hf.bytes()
+    assert bs["coupling"] < 0.55 * bf["coupling"]
+    assert bs["near"] < 0.6 * bf["near"]
+    # the full product of the same (nearly symmetric) matrix is close
+    full = product(Hf, x, alpha=-0.5, y0=y0)
+    assert np.abs(full - want).max() < 1e-2 * np.abs(want).max()
+
+
+def test_symmetric_storage_rejects_unsymmetric_operators():
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(4)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = h.new_dlp_laplace_bem3d(gr, 2, 4, 1, 1, 0.5)
+    rt = G.ClusterTree(mesh, 1, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    with pytest.raises(ValueError):
+        H2Matrix(op, mesh, rt, rt, bt, 3, symmetric=True)

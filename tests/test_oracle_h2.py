@@ -136,3 +136,28 @@ def test_helmholtz_couplings_match_live_reference(layer, basis, k):
         got = Oracle.h2_addeval(interp, N, x, y0.copy(), alpha=0.5 - 0.25j,
                                 trans=trans, S=S)
         assert np.abs(got - want).max() < TOL * np.abs(want).max(), trans
+
+
+@pytest.mark.skipif(not have_reference(False), reason="reference not built")
+@pytest.mark.parametrize("basis", [0, 1])
+def test_symmetric_product_matches_reference(basis):
+    # addevalsymm_h2matrix_avector on the reference's single layer matrix
+    # (self-adjoint) against the restatement from the lower block triangle;
+    # mathematics: for a symmetric matrix it equals the full product up to
+    # the approximation's asymmetry (the interpolation H2-matrix of the
+    # symmetric kernel is symmetric when row and column bases coincide,
+    # the quadrature only nearly so)
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(8)
+    rt, bt, interp, N, _ = host_h2(mesh, 0, basis, 16, 2.0, 3)
+    ref = ReferenceBem(0, 0, basis, basis, mesh.x, mesh.t, q=2, q2=4)
+    rh = ReferenceH2(ref, 16, 2.0, 3, 0)
+    rng = np.random.default_rng(11)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, n)
+    want = rh.addevalsymm(x, y0, alpha=0.75)
+    got = Oracle.h2_addeval(interp, N, x, y0.copy(), alpha=0.75, symm=True)
+    assert np.abs(got - want).max() < TOL * np.abs(want).max()
+    full = Oracle.h2_addeval(interp, N, x, y0.copy(), alpha=0.75)
+    assert np.abs(got - full).max() < 1e-2 * np.abs(full).max()
