@@ -473,6 +473,25 @@ def bench_dirichlet(h, mesh, args, stream):
     for H in (prob.V, prob.KM):
         H.set_schedule(best)
     mv_ms = mv_by[best]
+    # the single layer matrix in symmetric storage (blocks on and below the
+    # block diagonal) and its self-adjoint product
+    from h2lib_amd.h2matrix import H2Matrix
+    Vs = H2Matrix(prob.slp, mesh, prob.tree, prob.tree, prob.blocks, 4,
+                  symmetric=True, stream=stream)
+    Vs.set_schedule(best)
+    stream.sync()
+    for _ in range(3):
+        Vs.addevalsymm(1.0, d_x, d_y, stream)
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(reps):
+        Vs.addevalsymm(1.0, d_x, d_y, stream)
+    e1.record(stream)
+    stream.sync()
+    mvs_ms = e0.elapsed_ms(e1) / reps
+    nbs = Vs.bytes()
+    mvs_bytes = sum(nbs.values()) + 3 * 8 * n
     nbytes = prob.V.bytes()
     mv_bytes = sum(nbytes.values()) + 3 * 8 * n
     mv_gbs = mv_bytes / (mv_ms * 1e-3) / 1e9
@@ -511,7 +530,11 @@ def bench_dirichlet(h, mesh, args, stream):
             "hbm_frac": mv_gbs / 8000.0,
             "ms_by_upward_schedule": {"counters": mv_by[0],
                                       "level_sync": mv_by[1]},
-            "schedule": "level_sync" if best else "counters"},
+            "schedule": "level_sync" if best else "counters",
+            "symmetric": {"ms": mvs_ms, "bytes": int(mvs_bytes),
+                          "GB/s": mvs_bytes / (mvs_ms * 1e-3) / 1e9,
+                          "storage_bytes": int(sum(nbs.values())),
+                          "storage_bytes_full": int(sum(nbytes.values()))}},
     }
     if ref is not None:
         xr = ref["x"]

@@ -113,9 +113,12 @@ def helmholtz_point_errors(mesh, x, kind, k, x0, q=2):
 
 def solve_gmres(H, d_b, d_x, eps, maxiter, kmax, stream=None):
     """GMRES(kmax) for H x = b on device arrays; x holds the start vector.
-    Returns (steps, residual estimate)."""
+    Returns (steps, residual estimate).  A matrix in symmetric storage is
+    applied with the self-adjoint product."""
     _sigs()
-    fn = C.cast(hip().h2hip_addeval_h2matrix, C.c_void_p)
+    fn = C.cast(hip().h2hip_addevalsymm_h2matrix
+                if getattr(H, "symmetric", False)
+                else hip().h2hip_addeval_h2matrix, C.c_void_p)
     it = C.c_uint32()
     res = C.c_double()
     check(hip().h2hip_solve_gmres(fn, H._h, int(H.dtype == np.complex128),
