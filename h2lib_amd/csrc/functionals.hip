@@ -226,3 +226,183 @@ extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
   H2HIP_CHECK(hipGetLastError());
   return H2HIP_OK;
 }
+
+// ---- exterior potential of the Brakhage-Werner density ---------------------
+//
+// u(x) = kernel_const sum_t g_t w_t sum_q ww_q e^{ik r}/r^3 ((1 - ik r)
+// <x - y_q, n_t> - i eta r^2), r = |x - y_q|, y_q the points of the
+// single-triangle rule of order q on triangle t, as the reference's
+// Helmholtz test evaluates its solution (eval_brakhage_werner_c,
+// Tests/test_helmholtzbem3d.c:67-132).  One workgroup per evaluation
+// point, the triangles split over its threads, one reduction at the end.
+
+namespace {
+
+constexpr int BW_NT = 256;
+constexpr int BW_MAXQ = 16;
+
+struct BWArgs {
+  const double *x, *n, *g;
+  const uint32_t *t;
+  uint32_t nt;
+  const double *w;  // complex density, one per triangle
+  const double *pts;
+  uint32_t nq;
+  double rx0[BW_MAXQ], rx1[BW_MAXQ], rw[BW_MAXQ];
+  double kr, ki, er, ei;  // k and eta
+  double *out;            // complex, one per point
+};
+
+__global__ __launch_bounds__(BW_NT) void bw_potential(BWArgs a) {
+  const uint32_t p = blockIdx.x;
+  const double px = a.pts[3 * p], py = a.pts[3 * p + 1], pz = a.pts[3 * p + 2];
+  double sre = 0.0, sim = 0.0;
+  for (uint32_t t = threadIdx.x; t < a.nt; t += BW_NT) {
+    const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
+                 *C = a.x + 3 * a.t[3 * t + 2];
+    const double nx = a.n[3 * t], ny = a.n[3 * t + 1], nz = a.n[3 * t + 2];
+    double qre = 0.0, qim = 0.0;
+    for (uint32_t q = 0; q < a.nq; q++) {
+      const double tx = a.rx0[q], sx = a.rx1[q];
+      const double ca = 1.0 - tx, cb = tx - sx, cc = sx;
+      const double dx = px - (A[0] * ca + B[0] * cb + C[0] * cc);
+      const double dy = py - (A[1] * ca + B[1] * cb + C[1] * cc);
+      const double dz = pz - (A[2] * ca + B[2] * cb + C[2] * cc);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double rs = 1.0 / sqrt(r2);
+      const double r = r2 * rs;
+      const double rs3 = rs * rs * rs;
+      // e^{ikr} = e^{-ki r} (cos kr r + i sin kr r)
+      double sn, cs;
+      sincos(a.kr * r, &sn, &cs);
+      const double damp = a.ki != 0.0 ? exp(-a.ki * r) : 1.0;
+      const double ere = damp * cs, eim = damp * sn;
+      const double dn = dx * nx + dy * ny + dz * nz;
+      // (1 - ik r) dn - i eta r^2
+      const double fre = dn * (1.0 + a.ki * r) + a.ei * r2;
+      const double fim = -dn * a.kr * r - a.er * r2;
+      const double vre = (ere * fre - eim * fim) * rs3;
+      const double vim = (ere * fim + eim * fre) * rs3;
+      qre += a.rw[q] * vre;
+      qim += a.rw[q] * vim;
+    }
+    const double gw = a.g[t];
+    const double wr = a.w[2 * t], wi = a.w[2 * t + 1];
+    sre += gw * (qre * wr - qim * wi);
+    sim += gw * (qre * wi + qim * wr);
+  }
+  __shared__ double red[2][BW_NT / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sre += __shfl_xor(sre, off, 64);
+    sim += __shfl_xor(sim, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = sre;
+    red[1][threadIdx.x >> 6] = sim;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double re = 0.0, im = 0.0;
+    for (int v = 0; v < BW_NT / 64; v++) {
+      re += red[0][v];
+      im += red[1][v];
+    }
+    constexpr double KCONST = 0.0795774715459476679;
+    a.out[2 * p] = KCONST * re;
+    a.out[2 * p + 1] = KCONST * im;
+  }
+}
+
+}  // namespace
+
+extern "C" int h2hip_eval_brakhage_werner_c(const h2hip_surface *s,
+                                            uint32_t q, const double *k,
+                                            const double *eta,
+                                            const void *d_w, uint32_t npts,
+                                            const double *pts, void *out,
+                                            void *stream) {
+  if (!s || !k || !eta || !d_w || (npts && (!pts || !out)) || q < 1 ||
+      q > 4) {
+    h2hip::set_error("h2hip_eval_brakhage_werner_c: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  if (npts == 0) return H2HIP_OK;
+  const h2hip::TriRule tr = h2hip::make_triangle_rule(q);
+  BWArgs a;
+  a.x = s->d_x.p;
+  a.n = s->d_n.p;
+  a.g = s->d_g.p;
+  a.t = s->d_t.p;
+  a.nt = s->nt;
+  a.w = (const double *)d_w;
+  a.nq = (uint32_t)tr.w.size();
+  for (uint32_t i = 0; i < a.nq; i++) {
+    a.rx0[i] = tr.x0[i];
+    a.rx1[i] = tr.x1[i];
+    a.rw[i] = tr.w[i];
+  }
+  a.kr = k[0];
+  a.ki = k[1];
+  a.er = eta[0];
+  a.ei = eta[1];
+  h2hip::DevBuf<double> d_pts, d_out;
+  H2HIP_CHECK(d_pts.upload(pts, 3 * (size_t)npts));
+  H2HIP_CHECK(d_out.alloc(2 * (size_t)npts));
+  a.pts = d_pts.p;
+  a.out = d_out.p;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(bw_potential, dim3(npts), dim3(BW_NT), 0, st, a);
+  H2HIP_CHECK(hipGetLastError());
+  H2HIP_CHECK(hipMemcpyAsync(out, d_out.p, sizeof(double) * 2 * npts,
+                             hipMemcpyDeviceToHost, st));
+  H2HIP_CHECK(hipStreamSynchronize(st));
+  return H2HIP_OK;
+}
+
+// ---- linear combinations of operators --------------------------------------
+
+struct h2hip_combined {
+  h2hip_addeval_t f1 = nullptr, f2 = nullptr;
+  void *A1 = nullptr, *A2 = nullptr;
+  double c1[2] = {1.0, 0.0}, c2[2] = {0.0, 0.0};
+};
+
+extern "C" int h2hip_combined_create(h2hip_combined **out, h2hip_addeval_t f1,
+                                     void *A1, const double *c1,
+                                     h2hip_addeval_t f2, void *A2,
+                                     const double *c2) {
+  if (!out || !f1 || !c1 || (f2 && !c2)) {
+    h2hip::set_error("h2hip_combined_create: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  h2hip_combined *C = new h2hip_combined;
+  C->f1 = f1;
+  C->A1 = A1;
+  C->c1[0] = c1[0];
+  C->c1[1] = c1[1];
+  C->f2 = f2;
+  C->A2 = A2;
+  if (f2) {
+    C->c2[0] = c2[0];
+    C->c2[1] = c2[1];
+  }
+  *out = C;
+  return H2HIP_OK;
+}
+
+extern "C" void h2hip_combined_destroy(h2hip_combined *C) { delete C; }
+
+extern "C" int h2hip_addeval_combined(const double *alpha, void *A,
+                                      const void *x, void *y, void *stream) {
+  const h2hip_combined *C = (const h2hip_combined *)A;
+  if (!alpha || !C) return H2HIP_EINVAL;
+  // alpha c (complex products)
+  const double a1[2] = {alpha[0] * C->c1[0] - alpha[1] * C->c1[1],
+                        alpha[0] * C->c1[1] + alpha[1] * C->c1[0]};
+  int rc = C->f1(a1, C->A1, x, y, stream);
+  if (rc || !C->f2) return rc;
+  const double a2[2] = {alpha[0] * C->c2[0] - alpha[1] * C->c2[1],
+                        alpha[0] * C->c2[1] + alpha[1] * C->c2[0]};
+  return C->f2(a2, C->A2, x, y, stream);
+}

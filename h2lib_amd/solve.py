@@ -31,6 +31,9 @@ def _sigs():
                                                C.c_int, C.c_void_p,
                                                C.c_void_p, C.c_void_p]
     h.h2hip_solve_gmres.restype = C.c_int
+    h.h2hip_addeval_combined.restype = C.c_int
+    h.h2hip_addeval_combined.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
     h.h2hip_solve_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_uint64, C.c_void_p, C.c_void_p,
                                     C.c_double, C.c_uint32, C.c_uint32,
@@ -79,6 +82,15 @@ def _sigs():
     b.h2b_normL2.argtypes = [dp, C.c_uint, C.c_int, dp]
     b.h2b_projectL2_c_helmholtz.argtypes = [dp, C.c_uint, C.c_int,
                                             C.c_double, dp, dp]
+    b.h2b_integrate_c_helmholtz.argtypes = [dp, C.c_uint, C.c_int,
+                                            C.c_double, dp, dp]
+    h.h2hip_combined_create.restype = C.c_int
+    h.h2hip_combined_create.argtypes = [C.POINTER(C.c_void_p), dp, dp, dp,
+                                        dp, dp, dp]
+    h.h2hip_combined_destroy.argtypes = [dp]
+    h.h2hip_eval_brakhage_werner_c.restype = C.c_int
+    h.h2hip_eval_brakhage_werner_c.argtypes = [dp, C.c_uint32, dp, dp, dp,
+                                               C.c_uint32, dp, dp, dp]
     b.h2b_normL2_c_helmholtz.argtypes = [dp, C.c_uint, dp, C.c_int,
                                          C.c_double, dp, dp]
     _done = True
@@ -111,14 +123,61 @@ def helmholtz_point_errors(mesh, x, kind, k, x0, q=2):
     return out
 
 
+def _addeval_fn(H):
+    """The device addeval of an operator object: an H2Matrix (self-adjoint
+    product for symmetric storage) or a Combined."""
+    if isinstance(H, Combined):
+        return C.cast(hip().h2hip_addeval_combined, C.c_void_p)
+    return C.cast(hip().h2hip_addevalsymm_h2matrix
+                  if getattr(H, "symmetric", False)
+                  else hip().h2hip_addeval_h2matrix, C.c_void_p)
+
+
+class Combined:
+    """c1 A1 + c2 A2 of two device H2-matrices as one operator (the
+    Brakhage-Werner operator KM - i eta V of the reference's Helmholtz
+    test, Tests/test_helmholtzbem3d.c:24-65)."""
+
+    def __init__(self, A1, c1, A2=None, c2=0.0):
+        _sigs()
+        self.A1, self.A2 = A1, A2
+        self.dtype = np.complex128 if (np.iscomplexobj(c1) or
+                                       np.iscomplexobj(c2) or
+                                       A1.dtype == np.complex128) \
+            else np.float64
+        self.n = A1.n
+        a = np.array([np.real(c1), np.imag(c1)], dtype=np.float64)
+        b = np.array([np.real(c2), np.imag(c2)], dtype=np.float64)
+        h = C.c_void_p()
+        check(hip().h2hip_combined_create(
+            C.byref(h), _addeval_fn(A1), A1._h,
+            a.ctypes.data_as(C.c_void_p),
+            _addeval_fn(A2) if A2 is not None else None,
+            A2._h if A2 is not None else None,
+            b.ctypes.data_as(C.c_void_p)), "h2hip_combined_create")
+        self._h = h
+
+    def addeval(self, alpha, d_x, d_y, stream=None):
+        a = np.array([np.real(alpha), np.imag(alpha)], dtype=np.float64)
+        check(hip().h2hip_addeval_combined(
+            a.ctypes.data_as(C.c_void_p), self._h, C.c_void_p(d_x.ptr),
+            C.c_void_p(d_y.ptr), C.c_void_p(stream.handle if stream else 0)),
+            "h2hip_addeval_combined")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            hip().h2hip_combined_destroy(h)
+            self._h = None
+
+
 def solve_gmres(H, d_b, d_x, eps, maxiter, kmax, stream=None):
     """GMRES(kmax) for H x = b on device arrays; x holds the start vector.
-    Returns (steps, residual estimate).  A matrix in symmetric storage is
-    applied with the self-adjoint product."""
+    Returns (steps, residual estimate).  H: an H2Matrix (a matrix in
+    symmetric storage is applied with the self-adjoint product) or a
+    Combined operator."""
     _sigs()
-    fn = C.cast(hip().h2hip_addevalsymm_h2matrix
-                if getattr(H, "symmetric", False)
-                else hip().h2hip_addeval_h2matrix, C.c_void_p)
+    fn = _addeval_fn(H)
     it = C.c_uint32()
     res = C.c_double()
     check(hip().h2hip_solve_gmres(fn, H._h, int(H.dtype == np.complex128),
@@ -396,3 +455,92 @@ class HelmholtzExterior:
         """|x - Neumann data|_L2 / |Neumann data|_L2."""
         d, nrm = helmholtz_point_errors(self.mesh, x, 1, self.k, x0, self.q)
         return d / nrm
+
+
+def helmholtz_point_integrals(mesh, kind, k, x0, q=2):
+    """Integrals of the point-source data against the constant basis
+    (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629)."""
+    _sigs()
+    out = np.zeros(mesh.triangles, dtype=np.complex128)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    bem().h2b_integrate_c_helmholtz(mesh._sp, q, kind, float(k),
+                                    x0.ctypes.data_as(C.c_void_p),
+                                    out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+class HelmholtzBrakhageWerner:
+    """The reference's Helmholtz test problem (Tests/test_helmholtzbem3d.c:
+    353-450, exterior, constant basis): b = integrals of the Dirichlet data
+    of a point source inside, (KM - i eta V) x = b with KM the double layer
+    operator with alpha = 1/2, eta = k; the solution's exterior potential
+    (eval_brakhage_werner_c) is compared with the point-source field.
+    precision 1 assembles both near fields in mixed precision."""
+
+    def __init__(self, mesh, k, clf=32, eta_adm=1.0, m=5, q=2, q2=4,
+                 precision=0, stream=None):
+        from .bem3d import new_dlp_helmholtz_bem3d, new_slp_helmholtz_bem3d
+        _sigs()
+        self.mesh, self.k, self.q = mesh, k, q
+        self.eta = k
+        self.gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+        self.slp = new_slp_helmholtz_bem3d(k, self.gr, q, q2, 0, 0)
+        self.dlp = new_dlp_helmholtz_bem3d(k, self.gr, q, q2, 0, 0, 0.5)
+        if precision:
+            self.slp.set_precision(precision)
+            self.dlp.set_precision(precision)
+        self.rt = ClusterTree(mesh, 0, clf)
+        self.bt = BlockTree(self.rt, self.rt, eta_adm, 0)
+        self.V = H2Matrix(self.slp, mesh, self.rt, self.rt, self.bt, m,
+                          stream=stream)
+        self.KM = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
+                           stream=stream)
+        self.A = Combined(self.KM, 1.0, self.V, -1j * self.eta)
+        self.n = mesh.triangles
+
+    def rhs(self, x0):
+        return DeviceArray.from_host(
+            helmholtz_point_integrals(self.mesh, 0, self.k, x0, self.q))
+
+    def solve(self, d_b, eps=1e-12, maxiter=1000, kmax=50, stream=None):
+        d_x = DeviceArray(self.n, np.complex128)
+        d_x.zero(stream)
+        it, res = solve_gmres(self.A, d_b, d_x, eps, maxiter, kmax, stream)
+        return d_x, it, res
+
+    def potential(self, d_w, pts, stream=None):
+        """The Brakhage-Werner potential of the density at the points."""
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(pts), dtype=np.complex128)
+        k = np.array([np.real(self.k), np.imag(self.k)], dtype=np.float64)
+        e = np.array([np.real(self.eta), np.imag(self.eta)], dtype=np.float64)
+        check(hip().h2hip_eval_brakhage_werner_c(
+            self.gr.handle, self.q, k.ctypes.data_as(C.c_void_p),
+            e.ctypes.data_as(C.c_void_p), C.c_void_p(d_w.ptr), len(pts),
+            pts.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+            C.c_void_p(stream.handle if stream else 0)),
+            "h2hip_eval_brakhage_werner_c")
+        return out
+
+    @staticmethod
+    def outer_grid():
+        """The test's evaluation points: a 20 x 20 grid of [-10, 10]^2 in
+        the xz-plane outside the unit disc (max_rel_outer_error,
+        Tests/test_helmholtzbem3d.c:224-287)."""
+        pts = []
+        for j in range(20):
+            for i in range(20):
+                x = -10.0 + (20.0 / 19) * i
+                z = -10.0 + (20.0 / 19) * j
+                if x * x + z * z > 1:
+                    pts.append((x, 0.0, z))
+        return np.array(pts)
+
+    def outer_error(self, d_w, x0):
+        """The test's error: max relative error of the potential against
+        the point-source field e^{ik|x - x0|}/|x - x0| on the outer grid."""
+        pts = self.outer_grid()
+        u = self.potential(d_w, pts)
+        r = np.linalg.norm(pts - np.asarray(x0)[None, :], axis=1)
+        g = np.exp(1j * self.k * r) / r
+        return float(np.max(np.abs(u - g) / np.abs(g)))

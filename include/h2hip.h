@@ -253,6 +253,32 @@ int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A, int is_complex,
                       uint32_t maxiter, uint32_t kmax, void *stream,
                       uint32_t *iterations, double *residual);
 
+/* ---- operator combinations and the exterior potential ---------------------
+ * A = c1 A1 + c2 A2 as one operator for the Krylov solvers: the
+ * Brakhage-Werner operator KM - i eta V of the reference's Helmholtz test
+ * (addeval_A, Tests/test_helmholtzbem3d.c:24-65) is
+ * h2hip_combined_create(&C, h2hip_addeval_h2matrix, KM, {1, 0},
+ * h2hip_addeval_h2matrix, V, {0, -eta}).  f2 may be NULL (A = c1 A1).
+ * c1, c2 complex as (re, im). */
+typedef struct h2hip_combined h2hip_combined;
+int h2hip_combined_create(h2hip_combined **out, h2hip_addeval_t f1,
+                          void *A1, const double *c1, h2hip_addeval_t f2,
+                          void *A2, const double *c2);
+void h2hip_combined_destroy(h2hip_combined *C);
+/* y += alpha (c1 A1 + c2 A2) x; signature of h2hip_addeval_t. */
+int h2hip_addeval_combined(const double *alpha, void *A, const void *x,
+                           void *y, void *stream);
+/* The Brakhage-Werner potential of the constant-basis density d_w (complex,
+ * device) at npts points (host array, 3 doubles each) into out (host,
+ * complex): kernel_const sum_t g_t w_t sum_q w_q e^{ikr}/r^3 ((1 - ikr)
+ * <x - y_q, n_t> - i eta r^2) with the single-triangle rule of order q
+ * (eval_brakhage_werner_c, Tests/test_helmholtzbem3d.c:67-132).  k and eta
+ * complex as (re, im).  Returns when out is written. */
+int h2hip_eval_brakhage_werner_c(const h2hip_surface *s, uint32_t q,
+                                 const double *k, const double *eta,
+                                 const void *d_w, uint32_t npts,
+                                 const double *pts, void *out, void *stream);
+
 /* ---- boundary data ----------------------------------------------------------
  * L2 projection of the Dirichlet (kind 0: 1/|x - x0|) or Neumann (kind 1:
  * -<x - x0, n>/|x - x0|^3) data of a point source at x0 onto the piecewise

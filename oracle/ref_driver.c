@@ -623,6 +623,115 @@ ref_helmholtz_point_project(void *h, int kind, double k, const double *x0,
   del_avector(f);
 }
 
+/* The reference's Helmholtz test (Tests/test_helmholtzbem3d.c, compiled
+ * into the complex library with its main renamed). */
+field eval_brakhage_werner_c(pcbem3d bem, pcavector w, field eta, real * x);
+real max_rel_outer_error(pcbem3d bem, helmholtz_data * hdata, pcavector x,
+			 boundary_func3d rhs, basisfunctionbem3d basis);
+
+/* b = integral of the point-source Dirichlet data against the constant
+ * basis (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629), as the
+ * Helmholtz test forms its right-hand side. */
+void
+ref_helmholtz_point_integrate(void *h, double k, const double *x0, void *out)
+{
+  refbem   *rb = (refbem *) h;
+  real      kvec[3] = { k, 0.0, 0.0 }, src[3] = { x0[0], x0[1], x0[2] };
+  helmholtz_data hd;
+  pavector  f;
+
+  hd.kvec = kvec;
+  hd.source = src;
+  hd.eta = 0.0;
+  f = new_avector(rb->gr->triangles);
+  integrate_bem3d_c_avector(rb->bem, rhs_dirichlet_point_helmholtzbem3d, f,
+			    (void *) &hd);
+  memcpy(out, f->v, sizeof(field) * rb->gr->triangles);
+  del_avector(f);
+}
+
+typedef struct {
+  ph2matrix V, KM;
+  field     eta;
+} bwop;
+
+/* y += alpha (KM - i eta V) x, the test's addeval_A
+ * (Tests/test_helmholtzbem3d.c:24-65) for two H2-matrices. */
+static void
+bw_addeval(field alpha, void *matrix, pcavector x, pavector y)
+{
+  bwop     *op = (bwop *) matrix;
+
+  addeval_h2matrix_avector(alpha, op->KM, x, y);
+  addeval_h2matrix_avector(-I * alpha * op->eta, op->V, x, y);
+}
+
+/* Brakhage-Werner solve of the test's exterior problem: (KM - i eta V) x =
+ * b by GMRES(kmax) (Tests/test_helmholtzbem3d.c:433); x starts at zero.
+ * Returns the steps. */
+unsigned
+ref_helmholtz_bw_solve(void *slp, void *dlp, double eta, const void *b,
+		       void *x, double eps, unsigned maxiter, unsigned kmax)
+{
+  refh2    *V = (refh2 *) slp, *KM = (refh2 *) dlp;
+  uint      n = V->root->size;
+  avector   bt, xt;
+  pavector  bv, xv;
+  bwop      op;
+
+  op.V = V->h2;
+  op.KM = KM->h2;
+  op.eta = eta;
+  bv = init_pointer_avector(&bt, (field *) b, n);
+  xv = init_pointer_avector(&xt, (field *) x, n);
+  clear_avector(xv);
+  return solve_gmres_avector(&op, bw_addeval, bv, xv, eps, maxiter, kmax);
+}
+
+/* The Brakhage-Werner potential of the density w at npts points
+ * (eval_brakhage_werner_c, Tests/test_helmholtzbem3d.c:67-132). */
+void
+ref_eval_bw_c(void *h, const void *w, double eta, const double *pts,
+	      unsigned npts, void *out)
+{
+  refbem   *rb = (refbem *) h;
+  avector   wt;
+  pavector  wv;
+  field    *o = (field *) out;
+  real      x[3];
+  unsigned  j;
+
+  wv = init_pointer_avector(&wt, (field *) w, rb->gr->triangles);
+  for (j = 0; j < npts; j++) {
+    x[0] = pts[3 * j];
+    x[1] = pts[3 * j + 1];
+    x[2] = pts[3 * j + 2];
+    o[j] = eval_brakhage_werner_c(rb->bem, wv, eta, x);
+  }
+}
+
+/* The test's error measure (max_rel_outer_error,
+ * Tests/test_helmholtzbem3d.c:224-287): the largest relative error of the
+ * potential of x against the point-source field on its grid outside the
+ * unit disc of the xz-plane. */
+double
+ref_helmholtz_outer_error(void *h, const void *x, double k, const double *x0)
+{
+  refbem   *rb = (refbem *) h;
+  real      kvec[3] = { k, 0.0, 0.0 }, src[3] = { x0[0], x0[1], x0[2] };
+  helmholtz_data hd;
+  avector   xt;
+  pavector  xv;
+
+  hd.kvec = kvec;
+  hd.source = src;
+  hd.eta = 0.0;
+  xv = init_pointer_avector(&xt, (field *) x, rb->gr->triangles);
+  return max_rel_outer_error(rb->bem, &hd, xv,
+			     rhs_dirichlet_point_helmholtzbem3d,
+			     BASIS_CONSTANT_BEM3D);
+}
+
 /* out[0] = |x - g|_L2 (normL2diff_c_bem3d, Library/bem3d.c:8776-8825) and
  * out[1] = |g|_L2 (normL2_bem3d, Library/bem3d.c:8704-8751) for the point
  * source data g of kind 0 (Dirichlet) or 1 (Neumann). */

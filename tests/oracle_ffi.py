@@ -299,6 +299,18 @@ class Reference:
                                         C.c_void_p, C.c_void_p]
             lib.ref_bem_del.argtypes = [C.c_void_p]
             if complex_:
+                lib.ref_helmholtz_point_integrate.argtypes = [
+                    C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+                lib.ref_helmholtz_bw_solve.restype = C.c_uint
+                lib.ref_helmholtz_bw_solve.argtypes = [
+                    C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                    C.c_void_p, C.c_double, C.c_uint, C.c_uint]
+                lib.ref_eval_bw_c.argtypes = [C.c_void_p, C.c_void_p,
+                                              C.c_double, C.c_void_p,
+                                              C.c_uint, C.c_void_p]
+                lib.ref_helmholtz_outer_error.restype = C.c_double
+                lib.ref_helmholtz_outer_error.argtypes = [
+                    C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
                 lib.ref_helmholtz_point_project.argtypes = [
                     C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
                 lib.ref_helmholtz_point_errors.argtypes = [
@@ -399,6 +411,29 @@ class ReferenceBem(_BemBase):
         self._lib.ref_helmholtz_point_project(self.h, kind, k, _ptr(x0),
                                               _ptr(out))
         return out
+
+    def helmholtz_point_integrate(self, k, x0):
+        out = np.zeros(len(self.t), dtype=np.complex128)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self._lib.ref_helmholtz_point_integrate(self.h, k, _ptr(x0),
+                                                _ptr(out))
+        return out
+
+    def eval_bw(self, w, eta, pts):
+        """eval_brakhage_werner_c of the reference's Helmholtz test."""
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        ww = np.ascontiguousarray(w, dtype=np.complex128)
+        out = np.zeros(len(pts), dtype=np.complex128)
+        self._lib.ref_eval_bw_c(self.h, _ptr(ww), eta, _ptr(pts), len(pts),
+                                _ptr(out))
+        return out
+
+    def outer_error(self, x, k, x0):
+        """max_rel_outer_error of the reference's Helmholtz test."""
+        xx = np.ascontiguousarray(x, dtype=np.complex128)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        return self._lib.ref_helmholtz_outer_error(self.h, _ptr(xx), k,
+                                                   _ptr(x0))
 
     def helmholtz_point_errors(self, kind, k, x0, x):
         """(|x - data|_L2, |data|_L2) for the point-source data."""
@@ -506,6 +541,15 @@ class ReferenceH2:
         yy = np.ascontiguousarray(y, dtype=self.bem.dtype).copy()
         self._lib.ref_h2_addevalsymm(self.h, _ptr(a), _ptr(xx), _ptr(yy))
         return yy
+
+    def bw_solve(self, dlp, eta, b, eps, maxiter, kmax):
+        """(KM - i eta V) x = b with this (single layer) matrix and the
+        double layer H2-matrix dlp, the reference's GMRES."""
+        bb = np.ascontiguousarray(b, dtype=np.complex128)
+        xx = np.zeros_like(bb)
+        it = self._lib.ref_helmholtz_bw_solve(self.h, dlp.h, eta, _ptr(bb),
+                                              _ptr(xx), eps, maxiter, kmax)
+        return xx, it
 
     def gmres(self, b, x0, eps, maxiter, kmax):
         bb = np.ascontiguousarray(b, dtype=self.bem.dtype)

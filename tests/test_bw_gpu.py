@@ -1,0 +1,123 @@
+"""Brakhage-Werner formulation of the exterior Helmholtz problem on the GPU
+(needs a GPU): the reference's Helmholtz test (Tests/test_helmholtzbem3d.c:
+353-450, 695-725) with the piecewise constant basis, unit sphere with 512
+triangles, k = 2, leaves of at most 32 triangles, max-cluster admissibility
+with eta = 1, interpolation order 5, point source (0, 0, 0.2) inside:
+b = integrals of the Dirichlet data, (KM - i k V) x = b with KM the double
+layer operator with alpha = 1/2, GMRES(50) to 1e-12, then the potential
+eval_brakhage_werner_c of x on a 20 x 20 grid of the xz-plane outside the
+unit disc against the point-source field.
+
+* the potential kernel against the reference's eval_brakhage_werner_c on a
+  random density to 1e-12 (relative, max-norm);
+* the combined operator's product against the reference's two products;
+* the solve: the outer error lies in the reference test's accepted band
+  [1e-3, 2e-3] and within 1e-6 of the reference library's own outer error
+  on its own solution (the reference's complex Givens steps,
+  Library/krylov.c:551-610, are not unitary, so step counts differ; both
+  solutions are solved to a residual far below the discretisation error);
+* mixed precision near fields keep the outer error within 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle_ffi import ReferenceBem, ReferenceH2, have_reference
+
+pytestmark = pytest.mark.gpu
+
+X0 = np.array([0.0, 0.0, 0.2])
+K = 2.0
+
+
+def _problem(precision=0):
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import HelmholtzBrakhageWerner
+    mesh = G.sphere(8)
+    return mesh, HelmholtzBrakhageWerner(mesh, K, clf=32, eta_adm=1.0, m=5,
+                                         precision=precision)
+
+
+@pytest.mark.skipif(not have_reference(True), reason="reference not built")
+def test_potential_matches_reference():
+    from h2lib_amd.device import DeviceArray
+    mesh, prob = _problem()
+    rng = np.random.default_rng(7)
+    w = rng.standard_normal(mesh.triangles) + 1j * rng.standard_normal(
+        mesh.triangles)
+    pts = prob.outer_grid()
+    assert len(pts) == 396
+    got = prob.potential(DeviceArray.from_host(w), pts)
+    ref = ReferenceBem(1, 0, 0, 0, mesh.x, mesh.t, q=2, q2=4, k=K)
+    want = ref.eval_bw(w, K, pts)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("potential error %.2e" % err, flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.skipif(not have_reference(True), reason="reference not built")
+def test_combined_product_matches_reference():
+    from h2lib_amd.device import DeviceArray
+    mesh, prob = _problem()
+    rslp = ReferenceBem(1, 0, 0, 0, mesh.x, mesh.t, q=2, q2=4, k=K)
+    rdlp = ReferenceBem(1, 1, 0, 0, mesh.x, mesh.t, q=2, q2=4, k=K,
+                        alpha=0.5)
+    V = ReferenceH2(rslp, 32, 1.0, 5, 0)
+    KM = ReferenceH2(rdlp, 32, 1.0, 5, 0)
+    rng = np.random.default_rng(3)
+    n = mesh.triangles
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    alpha = 0.7 - 0.4j
+    want = KM.addeval(x, y0.copy(), alpha=alpha)
+    want = V.addeval(x, want, alpha=-1j * K * alpha)
+    d_x = DeviceArray.from_host(x)
+    d_y = DeviceArray.from_host(y0)
+    prob.A.addeval(alpha, d_x, d_y)
+    got = d_y.download()
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("combined product error %.2e" % err, flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.skipif(not have_reference(True), reason="reference not built")
+def test_bw_solve_outer_error_matches_reference():
+    mesh, prob = _problem()
+    d_b = prob.rhs(X0)
+    d_x, it, res = prob.solve(d_b, eps=1e-12, maxiter=1000, kmax=50)
+    x = d_x.download()
+    err = prob.outer_error(d_x, X0)
+    rslp = ReferenceBem(1, 0, 0, 0, mesh.x, mesh.t, q=2, q2=4, k=K)
+    rdlp = ReferenceBem(1, 1, 0, 0, mesh.x, mesh.t, q=2, q2=4, k=K,
+                        alpha=0.5)
+    V = ReferenceH2(rslp, 32, 1.0, 5, 0)
+    KM = ReferenceH2(rdlp, 32, 1.0, 5, 0)
+    b_ref = rdlp.helmholtz_point_integrate(K, X0)
+    x_ref, it_ref = V.bw_solve(KM, K, b_ref, 1e-12, 1000, 50)
+    err_ref = rslp.outer_error(x_ref, K, X0)
+    # the reference's error measure on this solution
+    err_cross = rslp.outer_error(x, K, X0)
+    b = d_b.download()
+    r = KM.addeval(x, b.copy(), alpha=-1.0)
+    r = V.addeval(x, r, alpha=1j * K)
+    rel_res = np.linalg.norm(r) / np.linalg.norm(b)
+    print("bw steps", it, it_ref, "true residual %.2e outer error %.6e "
+          "(reference measure %.6e, reference solution %.6e)"
+          % (rel_res, err, err_cross, err_ref), flush=True)
+    assert np.abs(b - b_ref).max() < 1e-13 * np.abs(b_ref).max()
+    assert rel_res < 1e-11
+    assert 1e-3 <= err <= 2e-3
+    assert abs(err - err_cross) < 1e-12
+    assert abs(err - err_ref) < 1e-6
+
+
+def test_bw_mixed_precision_outer_error():
+    errs = []
+    for prec in (0, 1):
+        _, prob = _problem(prec)
+        d_x, it, _ = prob.solve(prob.rhs(X0), eps=1e-12, maxiter=1000,
+                                kmax=50)
+        errs.append(prob.outer_error(d_x, X0))
+        print("precision", prec, "steps", it, "outer error %.6e" % errs[-1],
+              flush=True)
+    assert 1e-3 <= errs[1] <= 2e-3
+    assert abs(errs[1] - errs[0]) < 1e-6
