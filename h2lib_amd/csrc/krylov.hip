@@ -522,6 +522,387 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   return H2HIP_OK;
 }
 
+// ---- conjugate gradients -----------------------------------------------------
+//
+// Semantics of solve_cg_avector (Library/krylovsolvers.c:18-52) with
+// init_cg / step_cg (Library/krylov.c:162-196): r = b - A x, p = r; per
+// step a = A p, gamma = <p, a>, lambda = <p, r> / gamma, x += lambda p,
+// r -= lambda a, mu = <a, r> / gamma, p = r - mu p; the iteration stops when
+// |r| <= eps |b| or after maxiter - 1 steps.  <u, v> = sum conj(u_i) v_i.
+//
+// The stopping test runs on the device: the kernel that forms |r| also
+// counts the step and sets a done flag when the reference's loop would end;
+// the kernels of later steps see the flag and return at once, so x and r
+// stay those of the stopping step (only the product a = A p, which nothing
+// reads any more, still runs).  Every step is the same launch sequence with
+// the same arguments, so the host enqueues CHUNK steps at a time, directly
+// or as one captured graph, and reads the flag once per chunk while the
+// next chunk runs.  The reductions are deterministic: fixed partial sums
+// per block, summed in block order by the last block to finish.
+
+constexpr int CG_NB = 512;  // blocks of the CG vector kernels
+
+// Sums nv <= 8 doubles per block (part[b * 8 + v]) over the grid in the
+// last block to finish and leaves the totals in tot[0 .. nv) (LDS) after a
+// barrier; true in that block only.  Same fence protocol as last_block_sum.
+__device__ bool cg_last_block(const double *part, int nv, uint32_t *cnt,
+                              double *tot) {
+  __shared__ bool last;
+  __shared__ double red[NT];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+    if (last) __threadfence();
+  }
+  __syncthreads();
+  if (!last) return false;
+  const int groups = NT / 8, t = threadIdx.x, v = t & 7;
+  double s = 0.0;
+  if (v < nv)
+    for (uint32_t b = t >> 3; b < gridDim.x; b += groups)
+      s += part[(uint64_t)b * 8 + v];
+  red[t] = s;
+  __syncthreads();
+  if (t < nv) {
+    double r = 0.0;
+    for (int g = 0; g < groups; g++) r += red[g * 8 + t];
+    tot[t] = r;
+  }
+  if (t == 0) *cnt = 0;
+  __syncthreads();
+  return true;
+}
+
+// block sums of nv <= 8 per-thread values into part[blockIdx.x * 8 + v]
+__device__ void cg_block_sums(const double *val, int nv, double *part) {
+  __shared__ double red[NT / 64][8];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int v = 0; v < nv; v++) {
+    double s = val[v];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) red[wv][v] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double s = 0.0;
+    for (int w = 0; w < NT / 64; w++) s += red[w][threadIdx.x];
+    part[(uint64_t)blockIdx.x * 8 + threadIdx.x] = s;
+  }
+}
+
+// device state of one CG solve
+struct CgDev {
+  double *s;       // [0,1] gamma, [2,3] <p, r>, [4,5] <a, r>, [6] |r|^2,
+                   // [8] |r|, [9] eps |b|, [10] |b|
+  uint32_t *flag;  // [0] done, [1] steps, [2..3] block counters
+  uint32_t maxiter;
+};
+
+__device__ __forceinline__ void cdiv(double ar, double ai, double br,
+                                     double bi, double &cr, double &ci) {
+  const double d = br * br + bi * bi;
+  cr = (ar * br + ai * bi) / d;
+  ci = (ai * br - ar * bi) / d;
+}
+
+// r holds b - A x: p = r, a = 0, |b|, |r|, and the test before the first
+// step
+template <int C>
+__global__ __launch_bounds__(NT) void cg_init(const double *__restrict__ b,
+                                              const double *__restrict__ r,
+                                              double *__restrict__ p,
+                                              double *__restrict__ a,
+                                              uint64_t n, double *part,
+                                              CgDev d, double eps) {
+  double acc[2] = {0.0, 0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * NT) {
+    if constexpr (C) {
+      const double rr = r[2 * i], ri = r[2 * i + 1];
+      const double br = b[2 * i], bi = b[2 * i + 1];
+      p[2 * i] = rr;
+      p[2 * i + 1] = ri;
+      a[2 * i] = 0.0;
+      a[2 * i + 1] = 0.0;
+      acc[0] = fma(rr, rr, fma(ri, ri, acc[0]));
+      acc[1] = fma(br, br, fma(bi, bi, acc[1]));
+    } else {
+      const double rv = r[i], bv = b[i];
+      p[i] = rv;
+      a[i] = 0.0;
+      acc[0] = fma(rv, rv, acc[0]);
+      acc[1] = fma(bv, bv, acc[1]);
+    }
+  }
+  cg_block_sums(acc, 2, part);
+  __shared__ double tot[8];
+  if (!cg_last_block(part, 2, d.flag + 2, tot)) return;
+  if (threadIdx.x == 0) {
+    const double err = sqrt(tot[0]), nb = sqrt(tot[1]);
+    d.s[8] = err;
+    d.s[9] = eps * nb;
+    d.s[10] = nb;
+    d.flag[1] = 0;
+    d.flag[0] = (err > eps * nb && 1u != d.maxiter) ? 0u : 1u;
+  }
+}
+
+// gamma = <p, a>, <p, r>
+template <int C>
+__global__ __launch_bounds__(NT) void cg_dots(const double *__restrict__ p,
+                                              const double *__restrict__ a,
+                                              const double *__restrict__ r,
+                                              uint64_t n, double *part,
+                                              CgDev d) {
+  if (d.flag[0]) return;  // uniform over the grid
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * NT) {
+    if constexpr (C) {
+      const double pr = p[2 * i], pi = p[2 * i + 1];
+      const double ar = a[2 * i], ai = a[2 * i + 1];
+      const double rr = r[2 * i], ri = r[2 * i + 1];
+      acc[0] = fma(pr, ar, fma(pi, ai, acc[0]));
+      acc[1] = fma(pr, ai, fma(-pi, ar, acc[1]));
+      acc[2] = fma(pr, rr, fma(pi, ri, acc[2]));
+      acc[3] = fma(pr, ri, fma(-pi, rr, acc[3]));
+    } else {
+      const double pv = p[i];
+      acc[0] = fma(pv, a[i], acc[0]);
+      acc[2] = fma(pv, r[i], acc[2]);
+    }
+  }
+  cg_block_sums(acc, 4, part);
+  __shared__ double tot[8];
+  if (!cg_last_block(part, 4, d.flag + 2, tot)) return;
+  if (threadIdx.x < 4) d.s[threadIdx.x] = tot[threadIdx.x];
+}
+
+// lambda = <p, r> / gamma, x += lambda p, r -= lambda a, then <a, r>,
+// |r|^2, mu, the step count and the stopping test
+template <int C>
+__global__ __launch_bounds__(NT) void cg_update(
+    double *__restrict__ x, const double *__restrict__ p,
+    double *__restrict__ r, const double *__restrict__ a, uint64_t n,
+    double *part, CgDev d) {
+  if (d.flag[0]) return;
+  double lr, li = 0.0;
+  if constexpr (C)
+    cdiv(d.s[2], d.s[3], d.s[0], d.s[1], lr, li);
+  else
+    lr = d.s[2] / d.s[0];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * NT) {
+    if constexpr (C) {
+      const double pr = p[2 * i], pi = p[2 * i + 1];
+      const double ar = a[2 * i], ai = a[2 * i + 1];
+      x[2 * i] += lr * pr - li * pi;
+      x[2 * i + 1] += lr * pi + li * pr;
+      const double rr = r[2 * i] - (lr * ar - li * ai);
+      const double ri = r[2 * i + 1] - (lr * ai + li * ar);
+      r[2 * i] = rr;
+      r[2 * i + 1] = ri;
+      acc[0] = fma(ar, rr, fma(ai, ri, acc[0]));
+      acc[1] = fma(ar, ri, fma(-ai, rr, acc[1]));
+      acc[2] = fma(rr, rr, fma(ri, ri, acc[2]));
+    } else {
+      const double av = a[i];
+      x[i] = fma(lr, p[i], x[i]);
+      const double rv = fma(-lr, av, r[i]);
+      r[i] = rv;
+      acc[0] = fma(av, rv, acc[0]);
+      acc[2] = fma(rv, rv, acc[2]);
+    }
+  }
+  cg_block_sums(acc, 3, part);
+  __shared__ double tot[8];
+  if (!cg_last_block(part, 3, d.flag + 3, tot)) return;
+  if (threadIdx.x == 0) {
+    d.s[4] = tot[0];
+    d.s[5] = tot[1];
+    d.s[6] = tot[2];
+    const double err = sqrt(tot[2]);
+    d.s[8] = err;
+    const uint32_t it = d.flag[1] + 1;
+    d.flag[1] = it;
+    if (!(err > d.s[9] && it + 1 != d.maxiter)) d.flag[0] = 1;
+  }
+}
+
+// p = r - mu p with mu = <a, r> / gamma; a = 0 for the next product
+template <int C>
+__global__ __launch_bounds__(NT) void cg_direction(
+    double *__restrict__ p, const double *__restrict__ r,
+    double *__restrict__ a, uint64_t n, CgDev d) {
+  if (d.flag[0]) return;
+  double mr, mi = 0.0;
+  if constexpr (C)
+    cdiv(d.s[4], d.s[5], d.s[0], d.s[1], mr, mi);
+  else
+    mr = d.s[4] / d.s[0];
+  for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * NT) {
+    if constexpr (C) {
+      const double pr = p[2 * i], pi = p[2 * i + 1];
+      p[2 * i] = r[2 * i] - (mr * pr - mi * pi);
+      p[2 * i + 1] = r[2 * i + 1] - (mr * pi + mi * pr);
+      a[2 * i] = 0.0;
+      a[2 * i + 1] = 0.0;
+    } else {
+      p[i] = fma(-mr, p[i], r[i]);
+      a[i] = 0.0;
+    }
+  }
+}
+
+template <int C>
+int cg(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
+       double *x, double eps, uint32_t maxiter, int use_graph,
+       hipStream_t caller, uint32_t *iters, double *resid) {
+  const uint64_t n2 = (C ? 2 : 1) * n;
+  const dim3 gv((uint32_t)std::min<uint64_t>(CG_NB, (n + NT - 1) / NT));
+  const dim3 gc((uint32_t)((n2 + NT - 1) / NT));
+  h2hip::DevBuf<double> vec, part, sbuf;
+  h2hip::DevBuf<uint32_t> fbuf;
+  if (vec.alloc(3 * n2) != hipSuccess ||
+      part.alloc((uint64_t)CG_NB * 8) != hipSuccess ||
+      sbuf.alloc(16) != hipSuccess || fbuf.alloc(4) != hipSuccess) {
+    h2hip::set_error("h2hip_solve_cg: device allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  double *r = vec.p, *p = vec.p + n2, *a = vec.p + 2 * n2;
+  const CgDev d{sbuf.p, fbuf.p, maxiter};
+  // host copies of (done, steps) and |r| after a chunk; two chunks are in
+  // flight, each with its own copy and event
+  struct Look {
+    uint32_t flag[2];
+    double err;
+  };
+  struct Res {
+    Look *look = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // 2 looks, 1 join
+    hipStream_t own = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~Res() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (own) {
+        (void)hipStreamSynchronize(own);
+        (void)hipStreamDestroy(own);
+      }
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+      if (look) (void)hipHostFree(look);
+    }
+  } R;
+  if (hipHostMalloc((void **)&R.look, 2 * sizeof(Look),
+                    hipHostMallocDefault) != hipSuccess) {
+    h2hip::set_error("h2hip_solve_cg: pinned allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  for (auto &e : R.ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      h2hip::set_error("h2hip_solve_cg: event creation failed");
+      return H2HIP_EDEVICE;
+    }
+  // the null stream cannot be captured: a graph runs on a stream of its
+  // own, ordered after the caller's work
+  hipStream_t st = caller;
+  if (use_graph && caller == nullptr) {
+    if (hipStreamCreateWithFlags(&R.own, hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipEventRecord(R.ev[2], caller) != hipSuccess ||
+        hipStreamWaitEvent(R.own, R.ev[2], 0) != hipSuccess) {
+      h2hip::set_error("h2hip_solve_cg: stream setup failed");
+      return H2HIP_EDEVICE;
+    }
+    st = R.own;
+  }
+  if (hipMemsetAsync(fbuf.p, 0, sizeof(uint32_t) * 4, st) != hipSuccess) {
+    h2hip::set_error("h2hip_solve_cg: device error");
+    return H2HIP_EDEVICE;
+  }
+  const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
+  // r = b - A x, p = r, a = 0, |b|, |r|
+  hipLaunchKernelGGL(copy_kernel, gc, dim3(NT), 0, st, b, r, n2);
+  int rc = addeval(mone, A, x, r, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(cg_init<C>, gv, dim3(NT), 0, st, b, (const double *)r,
+                     p, a, n, part.p, d, eps);
+  auto step = [&](hipStream_t s) -> int {
+    int e = addeval(one, A, p, a, s);
+    if (e) return e;
+    hipLaunchKernelGGL(cg_dots<C>, gv, dim3(NT), 0, s, (const double *)p,
+                       (const double *)a, (const double *)r, n, part.p, d);
+    hipLaunchKernelGGL(cg_update<C>, gv, dim3(NT), 0, s, x,
+                       (const double *)p, r, (const double *)a, n, part.p,
+                       d);
+    hipLaunchKernelGGL(cg_direction<C>, gv, dim3(NT), 0, s, p,
+                       (const double *)r, a, n, d);
+    return H2HIP_OK;
+  };
+  if (use_graph) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) !=
+        hipSuccess) {
+      h2hip::set_error("h2hip_solve_cg: capture failed to start");
+      return H2HIP_EDEVICE;
+    }
+    for (int j = 0; j < CHUNK && !rc; j++) rc = step(st);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (!rc && e == hipSuccess &&
+        hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0) !=
+            hipSuccess)
+      R.exec = nullptr;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    if (!R.exec) {
+      h2hip::set_error("h2hip_solve_cg: capture of the CG steps failed");
+      return H2HIP_EDEVICE;
+    }
+  }
+  auto chunk = [&](int slot) -> int {
+    if (R.exec) {
+      if (hipGraphLaunch(R.exec, st) != hipSuccess) return H2HIP_EDEVICE;
+    } else {
+      for (int j = 0; j < CHUNK; j++) {
+        int e = step(st);
+        if (e) return e;
+      }
+    }
+    if (hipMemcpyAsync(R.look[slot].flag, fbuf.p, 2 * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&R.look[slot].err, sbuf.p + 8, sizeof(double),
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(R.ev[slot], st) != hipSuccess)
+      return H2HIP_EDEVICE;
+    return H2HIP_OK;
+  };
+  // while the host waits for the look after one chunk, the next chunk is
+  // already queued behind it
+  int slot = 0;
+  rc = chunk(0);
+  while (!rc) {
+    rc = chunk(slot ^ 1);
+    if (rc) break;
+    if (hipEventSynchronize(R.ev[slot]) != hipSuccess) {
+      rc = H2HIP_EDEVICE;
+      break;
+    }
+    if (R.look[slot].flag[0]) break;
+    slot ^= 1;
+  }
+  if (!rc && (hipStreamSynchronize(st) != hipSuccess ||
+              hipGetLastError() != hipSuccess))
+    rc = H2HIP_EDEVICE;
+  if (rc == H2HIP_EDEVICE) h2hip::set_error("h2hip_solve_cg: device error");
+  if (rc) return rc;
+  if (iters) *iters = R.look[slot].flag[1];
+  if (resid) *resid = R.look[slot].err;
+  return H2HIP_OK;
+}
+
 // ---- batched GMRES: several independent solves sharing every product --------
 //
 // Up to MAXSLOT solves with the same H2-matrix advance in lock step: one
@@ -1353,4 +1734,22 @@ extern "C" int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A,
              : gmres<0>(addeval, A, n, (const double *)d_b, (double *)d_x, eps,
                         maxiter, kmax, (hipStream_t)stream, iterations,
                         residual);
+}
+
+extern "C" int h2hip_solve_cg(h2hip_addeval_t addeval, void *A, int is_complex,
+                              uint64_t n, const void *d_b, void *d_x,
+                              double eps, uint32_t maxiter, int use_graph,
+                              void *stream, uint32_t *iterations,
+                              double *residual) {
+  if (!addeval || !d_b || !d_x || n == 0) {
+    h2hip::set_error("h2hip_solve_cg: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  return is_complex
+             ? cg<1>(addeval, A, n, (const double *)d_b, (double *)d_x, eps,
+                     maxiter, use_graph, (hipStream_t)stream, iterations,
+                     residual)
+             : cg<0>(addeval, A, n, (const double *)d_b, (double *)d_x, eps,
+                     maxiter, use_graph, (hipStream_t)stream, iterations,
+                     residual);
 }

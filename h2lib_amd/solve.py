@@ -1,7 +1,8 @@
 """Krylov solves with device H2-matrices, and the interior Dirichlet problem
 of the Laplace equation with the piecewise linear basis.
 
-solve_gmres mirrors solve_gmres_avector (Library/krylovsolvers.c:183-227).
+solve_gmres mirrors solve_gmres_avector (Library/krylovsolvers.c:183-227),
+solve_cg solve_cg_avector (Library/krylovsolvers.c:18-52).
 LaplaceDirichlet follows the reference's Dirichlet test
 (Tests/test_laplacebem3d.c:100-160): f = L2 projection of the Dirichlet data
 u(x) = 1/|x - x0|, b = (K + M/2) f with the double layer H2-matrix, V x = b
@@ -34,6 +35,12 @@ def _sigs():
     h.h2hip_addeval_combined.restype = C.c_int
     h.h2hip_addeval_combined.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
+    h.h2hip_solve_cg.restype = C.c_int
+    h.h2hip_solve_cg.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
+                                 C.c_uint64, C.c_void_p, C.c_void_p,
+                                 C.c_double, C.c_uint32, C.c_int,
+                                 C.c_void_p, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_double)]
     h.h2hip_solve_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_uint64, C.c_void_p, C.c_void_p,
                                     C.c_double, C.c_uint32, C.c_uint32,
@@ -187,6 +194,25 @@ def solve_gmres(H, d_b, d_x, eps, maxiter, kmax, stream=None):
                                   C.c_void_p(stream.handle if stream else 0),
                                   C.byref(it), C.byref(res)),
           "h2hip_solve_gmres")
+    return it.value, res.value
+
+
+def solve_cg(H, d_b, d_x, eps, maxiter, use_graph=False, stream=None):
+    """Conjugate gradients for H x = b (H self-adjoint positive definite)
+    on device arrays; x holds the start vector.  Mirrors solve_cg_avector
+    (Library/krylovsolvers.c:18-52).  use_graph replays 8 steps at a time as
+    one captured graph.  Returns (steps, final |r|)."""
+    _sigs()
+    fn = _addeval_fn(H)
+    it = C.c_uint32()
+    res = C.c_double()
+    check(hip().h2hip_solve_cg(fn, H._h, int(H.dtype == np.complex128),
+                               d_b.n, C.c_void_p(d_b.ptr),
+                               C.c_void_p(d_x.ptr), eps, maxiter,
+                               int(bool(use_graph)),
+                               C.c_void_p(stream.handle if stream else 0),
+                               C.byref(it), C.byref(res)),
+          "h2hip_solve_cg")
     return it.value, res.value
 
 
@@ -380,11 +406,22 @@ class LaplaceDirichlet:
         return d_f, d_b
 
     def solve(self, d_b, eps=1e-8, maxiter=1000, kmax=30, stream=None,
-              d_x=None):
+              d_x=None, solver="gmres", use_graph=False):
+        """V x = b from x = d_x (zero when None) by GMRES(kmax) or, with
+        solver="cg", by conjugate gradients (V is symmetric positive
+        definite; the reference's Laplace test solves with CG,
+        Tests/test_laplacebem3d.c:16-46)."""
         if d_x is None:
             d_x = DeviceArray(self.n)
             d_x.zero(stream)
-        it, res = solve_gmres(self.V, d_b, d_x, eps, maxiter, kmax, stream)
+        if solver == "cg":
+            it, res = solve_cg(self.V, d_b, d_x, eps, maxiter, use_graph,
+                               stream)
+        elif solver == "gmres":
+            it, res = solve_gmres(self.V, d_b, d_x, eps, maxiter, kmax,
+                                  stream)
+        else:
+            raise ValueError("solver: 'gmres' or 'cg'")
         return d_x, it, res
 
     def neumann_error(self, x, x0):

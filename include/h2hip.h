@@ -252,6 +252,19 @@ int h2hip_solve_gmres(h2hip_addeval_t addeval, void *A, int is_complex,
                       uint64_t n, const void *d_b, void *d_x, double eps,
                       uint32_t maxiter, uint32_t kmax, void *stream,
                       uint32_t *iterations, double *residual);
+/* Conjugate gradients for A x = b, A self-adjoint positive definite, on
+ * device vectors (x holds the start vector): stops when |r| <= eps |b| or
+ * after maxiter - 1 steps (maxiter 0: no limit).  solve_cg_avector
+ * (Library/krylovsolvers.c:18-52) with init_cg / step_cg
+ * (Library/krylov.c:162-196), <u, v> = sum conj(u_i) v_i.  The stopping
+ * test runs on the device, the host looks at it every 8 steps;
+ * use_graph != 0 replays the 8 steps as one captured graph (addeval must
+ * then be capturable, as h2hip_addeval_h2matrix is).  Returns the steps
+ * and the final |r|; the solve is complete on return. */
+int h2hip_solve_cg(h2hip_addeval_t addeval, void *A, int is_complex,
+                   uint64_t n, const void *d_b, void *d_x, double eps,
+                   uint32_t maxiter, int use_graph, void *stream,
+                   uint32_t *iterations, double *residual);
 
 /* ---- operator combinations and the exterior potential ---------------------
  * A = c1 A1 + c2 A2 as one operator for the Krylov solvers: the

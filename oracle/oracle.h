@@ -22,6 +22,12 @@
  *                        system (steps and solution), the residual bound
  *                        |b - A x| <= eps |b| and a direct solve on random
  *                        dense systems
+ *   orc_solve_cg         the reference's solve_cg_avector on the same H2
+ *                        systems (real: steps and solution; complex: the
+ *                        iterates after a fixed number of steps), the
+ *                        residual bound and a direct solve on random SPD
+ *                        and Hermitian positive definite systems, and
+ *                        termination in at most n steps on tiny ones
  */
 #ifndef ORACLE_H
 #define ORACLE_H
@@ -163,6 +169,21 @@ typedef void (*orc_addeval_t)(double alpha, void *A, const double *x,
 unsigned orc_solve_gmres(orc_addeval_t addeval, void *A, size_t n,
                          const double *b, double *x, double eps,
                          unsigned maxiter, unsigned kmax, double *residual);
+
+/* ---- conjugate gradients -------------------------------------------------- */
+
+/* y += alpha A x with alpha = (re, im); x and y interleaved complex when the
+ * field is complex. */
+typedef void (*orc_addeval_c_t)(const double *alpha, void *A, const double *x,
+                                double *y);
+
+/* Conjugate gradients for A x = b (cplx: complex vectors), x holding the
+ * start vector; returns the steps, *residual the final |r|.  Restates
+ * solve_cg_avector (Library/krylovsolvers.c:18-52) with init_cg / step_cg
+ * (Library/krylov.c:162-196). */
+unsigned orc_solve_cg(orc_addeval_c_t addeval, void *A, int cplx, size_t n,
+                      const double *b, double *x, double eps,
+                      unsigned maxiter, double *residual);
 
 #ifdef __cplusplus
 }

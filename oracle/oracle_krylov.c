@@ -1,5 +1,5 @@
-/* CPU restatement of the reference's restarted GMRES (TEST USE ONLY; see
- * oracle.h), real field.
+/* CPU restatement of the reference's restarted GMRES (real field) and
+ * conjugate gradients (real or complex field), TEST USE ONLY (see oracle.h).
  *
  * Follows, step by step:
  *   solve_gmres_avector             Library/krylovsolvers.c:183-227
@@ -219,5 +219,91 @@ unsigned orc_solve_gmres(orc_addeval_t addeval, void *A, size_t n,
   free(G.tau);
   free(G.rhat);
   free(G.q);
+  return iter;
+}
+
+/* ---- conjugate gradients ------------------------------------------------
+ * solve_cg_avector (Library/krylovsolvers.c:18-52) with init_cg and step_cg
+ * (Library/krylov.c:162-196), real or complex field (dotprod_avector
+ * conjugates its first argument, Library/avector.c:366-379). */
+
+static double complex cg_dot(int cplx, const double *u, const double *v,
+                             size_t n) {
+  double complex s = 0.0;
+  if (cplx) {
+    const double complex *a = (const double complex *)u;
+    const double complex *b = (const double complex *)v;
+    for (size_t i = 0; i < n; i++) s += conj(a[i]) * b[i];
+  } else {
+    for (size_t i = 0; i < n; i++) s += u[i] * v[i];
+  }
+  return s;
+}
+
+/* y += alpha x */
+static void cg_axpy(int cplx, double complex alpha, const double *x,
+                    double *y, size_t n) {
+  if (cplx) {
+    const double complex *a = (const double complex *)x;
+    double complex *b = (double complex *)y;
+    for (size_t i = 0; i < n; i++) b[i] += alpha * a[i];
+  } else {
+    for (size_t i = 0; i < n; i++) y[i] += creal(alpha) * x[i];
+  }
+}
+
+static void cg_scale(int cplx, double complex alpha, double *x, size_t n) {
+  if (cplx) {
+    double complex *a = (double complex *)x;
+    for (size_t i = 0; i < n; i++) a[i] *= alpha;
+  } else {
+    for (size_t i = 0; i < n; i++) x[i] *= creal(alpha);
+  }
+}
+
+static void cg_apply(orc_addeval_c_t addeval, void *A, double complex alpha,
+                     const double *x, double *y) {
+  const double a[2] = {creal(alpha), cimag(alpha)};
+  addeval(a, A, x, y);
+}
+
+unsigned orc_solve_cg(orc_addeval_c_t addeval, void *A, int cplx, size_t n,
+                      const double *b, double *x, double eps,
+                      unsigned maxiter, double *residual) {
+  const size_t len = (cplx ? 2 : 1) * n;
+  double *r = calloc(len, sizeof(double));
+  double *p = calloc(len, sizeof(double));
+  double *a = calloc(len, sizeof(double));
+  if (!r || !p || !a) {
+    free(r);
+    free(p);
+    free(a);
+    return 0;
+  }
+  const double norm = sqrt(creal(cg_dot(cplx, b, b, n)));
+  /* init_cg: r = b - A x, p = r */
+  memcpy(r, b, len * sizeof(double));
+  cg_apply(addeval, A, -1.0, x, r);
+  memcpy(p, r, len * sizeof(double));
+  double error = sqrt(creal(cg_dot(cplx, r, r, n)));
+  unsigned iter = 0;
+  while (error > eps * norm && iter + 1 != maxiter) {
+    /* step_cg */
+    memset(a, 0, len * sizeof(double));
+    cg_apply(addeval, A, 1.0, p, a);
+    const double complex gamma = cg_dot(cplx, p, a, n);
+    const double complex lambda = cg_dot(cplx, p, r, n) / gamma;
+    cg_axpy(cplx, lambda, p, x, n);
+    cg_axpy(cplx, -lambda, a, r, n);
+    const double complex mu = cg_dot(cplx, a, r, n) / gamma;
+    cg_scale(cplx, -mu, p, n);
+    cg_axpy(cplx, 1.0, r, p, n);
+    error = sqrt(creal(cg_dot(cplx, r, r, n)));
+    iter++;
+  }
+  if (residual) *residual = error;
+  free(r);
+  free(p);
+  free(a);
   return iter;
 }

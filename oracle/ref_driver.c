@@ -774,6 +774,24 @@ ref_h2_gmres(void *h, const void *b, void *x, double eps, unsigned maxiter,
 			     eps, maxiter, kmax);
 }
 
+/* Conjugate gradients on the H2-matrix (solve_cg_avector,
+ * Library/krylovsolvers.c:18-52): x holds the start vector.  Returns the
+ * iteration count. */
+unsigned
+ref_h2_cg(void *h, const void *b, void *x, double eps, unsigned maxiter)
+{
+  refh2    *r = (refh2 *) h;
+  avector   bt, xt;
+  pavector  bv, xv;
+  uint      n = r->root->size;
+
+  bv = init_pointer_avector(&bt, (field *) b, n);
+  xv = init_pointer_avector(&xt, (field *) x, n);
+  return solve_cg_avector((void *) r->h2,
+			  (addeval_t) addeval_h2matrix_avector, bv, xv, eps,
+			  maxiter);
+}
+
 #ifdef REF_WITH_HIP
 /* ---------------------------------------------------------------
  * The reference driving the GPU through the H2Lib-side binding

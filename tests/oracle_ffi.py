@@ -190,6 +190,40 @@ def oracle_gmres(matvec, n, b, x0, eps, maxiter, kmax):
     return xx, it, res.value
 
 
+ADDEVAL_C_T = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_void_p,
+                          C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def oracle_cg(matvec, n, b, x0, eps, maxiter, cplx=False):
+    """The restated CG with y += alpha A x computed by the Python callable
+    matvec(x) -> A x (float64 or complex128 arrays); returns x, steps,
+    final |r|."""
+    lib = Oracle.lib()
+    if not hasattr(lib, "_cgsig"):
+        lib.orc_solve_cg.restype = C.c_uint
+        lib.orc_solve_cg.argtypes = [ADDEVAL_C_T, C.c_void_p, C.c_int,
+                                     C.c_size_t, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_uint,
+                                     C.POINTER(C.c_double)]
+        lib._cgsig = True
+    dt = np.complex128 if cplx else np.float64
+    m = (2 if cplx else 1) * n
+
+    def cb(alpha, _A, xp, yp):
+        xv = np.ctypeslib.as_array(xp, shape=(m,)).view(dt)
+        yv = np.ctypeslib.as_array(yp, shape=(m,)).view(dt)
+        a = complex(alpha[0], alpha[1]) if cplx else alpha[0]
+        yv += a * matvec(xv.copy())
+
+    f = ADDEVAL_C_T(cb)
+    bb = np.ascontiguousarray(b, dtype=dt)
+    xx = np.ascontiguousarray(x0, dtype=dt).copy()
+    res = C.c_double()
+    it = lib.orc_solve_cg(f, None, int(cplx), n, _ptr(bb), _ptr(xx), eps,
+                          maxiter, C.byref(res))
+    return xx, it, res.value
+
+
 def oracle_nearfield_packed(ob, blocks, ridx_all, cidx_all):
     """Near-field blocks (roff, rows, coff, cols) of an OracleBem computed
     one by one and packed column-major one after the other."""
@@ -341,6 +375,9 @@ class Reference:
             lib.ref_h2_gmres.restype = C.c_uint
             lib.ref_h2_gmres.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_double, C.c_uint, C.c_uint]
+            lib.ref_h2_cg.restype = C.c_uint
+            lib.ref_h2_cg.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_double, C.c_uint]
             if hip:
                 lib.ref_bem_use_hip.argtypes = [C.c_void_p]
                 lib.ref_bem_hip_precision.argtypes = [C.c_void_p, C.c_int]
@@ -549,6 +586,13 @@ class ReferenceH2:
         xx = np.zeros_like(bb)
         it = self._lib.ref_helmholtz_bw_solve(self.h, dlp.h, eta, _ptr(bb),
                                               _ptr(xx), eps, maxiter, kmax)
+        return xx, it
+
+    def cg(self, b, x0, eps, maxiter):
+        """The reference's solve_cg_avector on this H2-matrix."""
+        bb = np.ascontiguousarray(b, dtype=self.bem.dtype)
+        xx = np.ascontiguousarray(x0, dtype=self.bem.dtype).copy()
+        it = self._lib.ref_h2_cg(self.h, _ptr(bb), _ptr(xx), eps, maxiter)
         return xx, it
 
     def gmres(self, b, x0, eps, maxiter, kmax):
