@@ -515,6 +515,38 @@ def bench_dirichlet(h, mesh, args, stream):
             lat.append(dt)
             steps.append(it)
     lat = np.array(lat)
+
+    # solve only (b resident): GMRES(30) against the reference test's own
+    # solver, CG, on the full matrix and on the symmetric storage (exactly
+    # symmetric, which CG assumes), CG steps replayed as captured graphs
+    from h2lib_amd.solve import solve_cg, solve_gmres
+    d_f, d_b = prob.rhs(x0, stream)
+    solvers = {}
+    for name, H, use_cg in (("gmres30_full", prob.V, False),
+                            ("cg_full", prob.V, True),
+                            ("cg_symmetric", Vs, True)):
+        ts, its = [], []
+        for r in range(1 + max(1, args.solves // 4)):
+            d_c = h.DeviceArray(n)
+            d_c.zero(stream)
+            t = time.perf_counter()
+            if use_cg:
+                it, _ = solve_cg(H, d_b, d_c, args.eps, 1000, True, stream)
+            else:
+                it, _ = solve_gmres(H, d_b, d_c, args.eps, 1000, 30, stream)
+            dt = time.perf_counter() - t
+            if r:
+                ts.append(dt)
+                its.append(it)
+        xc = d_c.download()
+        e = {"solve_ms": float(np.median(ts) * 1e3),
+             "steps": int(np.median(its)),
+             "neumann_rel_err": prob.neumann_error(xc, x0)}
+        if ref is not None:
+            e["rel_err"] = float(np.linalg.norm(xc - ref["x"]) /
+                                 np.linalg.norm(ref["x"]))
+        solvers[name] = e
+
     out = {
         "solves_per_s": len(lat) / lat.sum(),
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
@@ -535,6 +567,7 @@ def bench_dirichlet(h, mesh, args, stream):
                           "GB/s": mvs_bytes / (mvs_ms * 1e-3) / 1e9,
                           "storage_bytes": int(sum(nbs.values())),
                           "storage_bytes_full": int(sum(nbytes.values()))}},
+        "solvers": solvers,
     }
     if ref is not None:
         xr = ref["x"]

@@ -233,8 +233,11 @@ extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
 // <x - y_q, n_t> - i eta r^2), r = |x - y_q|, y_q the points of the
 // single-triangle rule of order q on triangle t, as the reference's
 // Helmholtz test evaluates its solution (eval_brakhage_werner_c,
-// Tests/test_helmholtzbem3d.c:67-132).  One workgroup per evaluation
-// point, the triangles split over its threads, one reduction at the end.
+// Tests/test_helmholtzbem3d.c:67-132; for a piecewise linear density w
+// enters through the corner values weighted by the basis functions,
+// eval_brakhage_werner_l, Tests/test_helmholtzbem3d.c:133-221).  One
+// workgroup per evaluation point, the triangles split over its threads,
+// one reduction at the end.
 
 namespace {
 
@@ -253,6 +256,9 @@ struct BWArgs {
   double *out;            // complex, one per point
 };
 
+// LIN: piecewise linear density (one value per vertex, w_i weighted by
+// the basis function of corner i at each point) instead of constant
+template <int LIN>
 __global__ __launch_bounds__(BW_NT) void bw_potential(BWArgs a) {
   const uint32_t p = blockIdx.x;
   const double px = a.pts[3 * p], py = a.pts[3 * p + 1], pz = a.pts[3 * p + 2];
@@ -262,6 +268,7 @@ __global__ __launch_bounds__(BW_NT) void bw_potential(BWArgs a) {
                  *C = a.x + 3 * a.t[3 * t + 2];
     const double nx = a.n[3 * t], ny = a.n[3 * t + 1], nz = a.n[3 * t + 2];
     double qre = 0.0, qim = 0.0;
+    double lre[3] = {0.0, 0.0, 0.0}, lim[3] = {0.0, 0.0, 0.0};
     for (uint32_t q = 0; q < a.nq; q++) {
       const double tx = a.rx0[q], sx = a.rx1[q];
       const double ca = 1.0 - tx, cb = tx - sx, cc = sx;
@@ -283,13 +290,32 @@ __global__ __launch_bounds__(BW_NT) void bw_potential(BWArgs a) {
       const double fim = -dn * a.kr * r - a.er * r2;
       const double vre = (ere * fre - eim * fim) * rs3;
       const double vim = (ere * fim + eim * fre) * rs3;
-      qre += a.rw[q] * vre;
-      qim += a.rw[q] * vim;
+      if constexpr (LIN) {
+        const double phi[3] = {ca, cb, cc};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          lre[i] += a.rw[q] * phi[i] * vre;
+          lim[i] += a.rw[q] * phi[i] * vim;
+        }
+      } else {
+        qre += a.rw[q] * vre;
+        qim += a.rw[q] * vim;
+      }
     }
     const double gw = a.g[t];
-    const double wr = a.w[2 * t], wi = a.w[2 * t + 1];
-    sre += gw * (qre * wr - qim * wi);
-    sim += gw * (qre * wi + qim * wr);
+    if constexpr (LIN) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const uint32_t v = a.t[3 * t + i];
+        const double wr = a.w[2 * v], wi = a.w[2 * v + 1];
+        sre += gw * (lre[i] * wr - lim[i] * wi);
+        sim += gw * (lre[i] * wi + lim[i] * wr);
+      }
+    } else {
+      const double wr = a.w[2 * t], wi = a.w[2 * t + 1];
+      sre += gw * (qre * wr - qim * wi);
+      sim += gw * (qre * wi + qim * wr);
+    }
   }
   __shared__ double red[2][BW_NT / 64];
 #pragma unroll
@@ -316,15 +342,13 @@ __global__ __launch_bounds__(BW_NT) void bw_potential(BWArgs a) {
 
 }  // namespace
 
-extern "C" int h2hip_eval_brakhage_werner_c(const h2hip_surface *s,
-                                            uint32_t q, const double *k,
-                                            const double *eta,
-                                            const void *d_w, uint32_t npts,
-                                            const double *pts, void *out,
-                                            void *stream) {
-  if (!s || !k || !eta || !d_w || (npts && (!pts || !out)) || q < 1 ||
-      q > 4) {
-    h2hip::set_error("h2hip_eval_brakhage_werner_c: invalid arguments");
+namespace {
+
+int eval_bw(const char *name, int lin, const h2hip_surface *s, uint32_t q,
+            double kr, double ki, const double *eta, const void *d_w,
+            uint32_t npts, const double *pts, void *out, void *stream) {
+  if (!s || !eta || !d_w || (npts && (!pts || !out)) || q < 1 || q > 4) {
+    h2hip::set_error(std::string(name) + ": invalid arguments");
     return H2HIP_EINVAL;
   }
   if (npts == 0) return H2HIP_OK;
@@ -342,8 +366,8 @@ extern "C" int h2hip_eval_brakhage_werner_c(const h2hip_surface *s,
     a.rx1[i] = tr.x1[i];
     a.rw[i] = tr.w[i];
   }
-  a.kr = k[0];
-  a.ki = k[1];
+  a.kr = kr;
+  a.ki = ki;
   a.er = eta[0];
   a.ei = eta[1];
   h2hip::DevBuf<double> d_pts, d_out;
@@ -352,12 +376,47 @@ extern "C" int h2hip_eval_brakhage_werner_c(const h2hip_surface *s,
   a.pts = d_pts.p;
   a.out = d_out.p;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(bw_potential, dim3(npts), dim3(BW_NT), 0, st, a);
+  if (lin)
+    hipLaunchKernelGGL(bw_potential<1>, dim3(npts), dim3(BW_NT), 0, st, a);
+  else
+    hipLaunchKernelGGL(bw_potential<0>, dim3(npts), dim3(BW_NT), 0, st, a);
   H2HIP_CHECK(hipGetLastError());
   H2HIP_CHECK(hipMemcpyAsync(out, d_out.p, sizeof(double) * 2 * npts,
                              hipMemcpyDeviceToHost, st));
   H2HIP_CHECK(hipStreamSynchronize(st));
   return H2HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int h2hip_eval_brakhage_werner_c(const h2hip_surface *s,
+                                            uint32_t q, const double *k,
+                                            const double *eta,
+                                            const void *d_w, uint32_t npts,
+                                            const double *pts, void *out,
+                                            void *stream) {
+  if (!k) {
+    h2hip::set_error("h2hip_eval_brakhage_werner_c: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  return eval_bw("h2hip_eval_brakhage_werner_c", 0, s, q, k[0], k[1], eta,
+                 d_w, npts, pts, out, stream);
+}
+
+// The reference's linear variant declares its wave number real (real k =
+// bem->k, Tests/test_helmholtzbem3d.c:144), so only Re k enters.
+extern "C" int h2hip_eval_brakhage_werner_l(const h2hip_surface *s,
+                                            uint32_t q, const double *k,
+                                            const double *eta,
+                                            const void *d_w, uint32_t npts,
+                                            const double *pts, void *out,
+                                            void *stream) {
+  if (!k) {
+    h2hip::set_error("h2hip_eval_brakhage_werner_l: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  return eval_bw("h2hip_eval_brakhage_werner_l", 1, s, q, k[0], 0.0, eta,
+                 d_w, npts, pts, out, stream);
 }
 
 // ---- linear combinations of operators --------------------------------------

@@ -308,3 +308,34 @@ h2b_integrate_c_helmholtz(const h2b_surface *s, unsigned q, int kind,
     out[2 * t + 1] = sim * s->g[t];
   }
 }
+
+void
+h2b_integrate_l_helmholtz(const h2b_surface *s, unsigned q, int kind,
+                          double k, const double *x0, double *out)
+{
+  double rx0[256], rx1[256], rw[256];
+  const unsigned nq = tri_rule(q, rx0, rx1, rw);
+  memset(out, 0, sizeof(double) * 2 * s->vertices);
+  for (uint32_t t = 0; t < s->triangles; t++) {
+    const uint32_t *tv = s->t + 3 * t;
+    const double *A = s->x + 3 * tv[0], *B = s->x + 3 * tv[1],
+      *C = s->x + 3 * tv[2];
+    double sre[3] = {0.0, 0.0, 0.0}, sim[3] = {0.0, 0.0, 0.0};
+    for (unsigned j = 0; j < nq; j++) {
+      const double u = rx0[j], v = rx1[j];
+      const double phi[3] = {1.0 - u, u - v, v};
+      double x[3], re, im;
+      for (int d = 0; d < 3; d++)
+        x[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+      helm_value(kind, k, x0, x, s->n + 3 * t, &re, &im);
+      for (int l = 0; l < 3; l++) {
+        sre[l] += rw[j] * phi[l] * re;
+        sim[l] += rw[j] * phi[l] * im;
+      }
+    }
+    for (int l = 0; l < 3; l++) {
+      out[2 * tv[l]] += sre[l] * s->g[t];
+      out[2 * tv[l] + 1] += sim[l] * s->g[t];
+    }
+  }
+}

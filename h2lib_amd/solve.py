@@ -91,6 +91,11 @@ def _sigs():
                                             C.c_double, dp, dp]
     b.h2b_integrate_c_helmholtz.argtypes = [dp, C.c_uint, C.c_int,
                                             C.c_double, dp, dp]
+    b.h2b_integrate_l_helmholtz.argtypes = [dp, C.c_uint, C.c_int,
+                                            C.c_double, dp, dp]
+    h.h2hip_eval_brakhage_werner_l.restype = C.c_int
+    h.h2hip_eval_brakhage_werner_l.argtypes = [dp, C.c_uint32, dp, dp, dp,
+                                               C.c_uint32, dp, dp, dp]
     h.h2hip_combined_create.restype = C.c_int
     h.h2hip_combined_create.argtypes = [C.POINTER(C.c_void_p), dp, dp, dp,
                                         dp, dp, dp]
@@ -494,50 +499,56 @@ class HelmholtzExterior:
         return d / nrm
 
 
-def helmholtz_point_integrals(mesh, kind, k, x0, q=2):
-    """Integrals of the point-source data against the constant basis
-    (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629)."""
+def helmholtz_point_integrals(mesh, kind, k, x0, q=2, basis=0):
+    """Integrals of the point-source data against the constant (basis 0,
+    integrate_bem3d_c_avector, Library/bem3d.c:8579-8629) or the linear
+    basis (basis 1, integrate_bem3d_l_avector, Library/bem3d.c:8631-8702)."""
     _sigs()
-    out = np.zeros(mesh.triangles, dtype=np.complex128)
+    out = np.zeros(mesh.vertices if basis else mesh.triangles,
+                   dtype=np.complex128)
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
-    bem().h2b_integrate_c_helmholtz(mesh._sp, q, kind, float(k),
-                                    x0.ctypes.data_as(C.c_void_p),
-                                    out.ctypes.data_as(C.c_void_p))
+    fn = bem().h2b_integrate_l_helmholtz if basis else \
+        bem().h2b_integrate_c_helmholtz
+    fn(mesh._sp, q, kind, float(k), x0.ctypes.data_as(C.c_void_p),
+       out.ctypes.data_as(C.c_void_p))
     return out
 
 
 class HelmholtzBrakhageWerner:
     """The reference's Helmholtz test problem (Tests/test_helmholtzbem3d.c:
-    353-450, exterior, constant basis): b = integrals of the Dirichlet data
-    of a point source inside, (KM - i eta V) x = b with KM the double layer
-    operator with alpha = 1/2, eta = k; the solution's exterior potential
-    (eval_brakhage_werner_c) is compared with the point-source field.
+    353-450, exterior): b = integrals of the Dirichlet data of a point
+    source inside, (KM - i eta V) x = b with KM the double layer operator
+    with alpha = 1/2, eta = k; the solution's exterior potential
+    (eval_brakhage_werner_c / _l) is compared with the point-source field.
+    basis 0 piecewise constant, 1 piecewise linear (both operators).
     precision 1 assembles both near fields in mixed precision."""
 
     def __init__(self, mesh, k, clf=32, eta_adm=1.0, m=5, q=2, q2=4,
-                 precision=0, stream=None):
+                 precision=0, stream=None, basis=0):
         from .bem3d import new_dlp_helmholtz_bem3d, new_slp_helmholtz_bem3d
         _sigs()
-        self.mesh, self.k, self.q = mesh, k, q
+        self.mesh, self.k, self.q, self.basis = mesh, k, q, basis
         self.eta = k
         self.gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
-        self.slp = new_slp_helmholtz_bem3d(k, self.gr, q, q2, 0, 0)
-        self.dlp = new_dlp_helmholtz_bem3d(k, self.gr, q, q2, 0, 0, 0.5)
+        self.slp = new_slp_helmholtz_bem3d(k, self.gr, q, q2, basis, basis)
+        self.dlp = new_dlp_helmholtz_bem3d(k, self.gr, q, q2, basis, basis,
+                                           0.5)
         if precision:
             self.slp.set_precision(precision)
             self.dlp.set_precision(precision)
-        self.rt = ClusterTree(mesh, 0, clf)
+        self.rt = ClusterTree(mesh, basis, clf)
         self.bt = BlockTree(self.rt, self.rt, eta_adm, 0)
         self.V = H2Matrix(self.slp, mesh, self.rt, self.rt, self.bt, m,
                           stream=stream)
         self.KM = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
                            stream=stream)
         self.A = Combined(self.KM, 1.0, self.V, -1j * self.eta)
-        self.n = mesh.triangles
+        self.n = mesh.vertices if basis else mesh.triangles
 
     def rhs(self, x0):
         return DeviceArray.from_host(
-            helmholtz_point_integrals(self.mesh, 0, self.k, x0, self.q))
+            helmholtz_point_integrals(self.mesh, 0, self.k, x0, self.q,
+                                      self.basis))
 
     def solve(self, d_b, eps=1e-12, maxiter=1000, kmax=50, stream=None):
         d_x = DeviceArray(self.n, np.complex128)
@@ -551,12 +562,14 @@ class HelmholtzBrakhageWerner:
         out = np.zeros(len(pts), dtype=np.complex128)
         k = np.array([np.real(self.k), np.imag(self.k)], dtype=np.float64)
         e = np.array([np.real(self.eta), np.imag(self.eta)], dtype=np.float64)
-        check(hip().h2hip_eval_brakhage_werner_c(
-            self.gr.handle, self.q, k.ctypes.data_as(C.c_void_p),
-            e.ctypes.data_as(C.c_void_p), C.c_void_p(d_w.ptr), len(pts),
-            pts.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
-            C.c_void_p(stream.handle if stream else 0)),
-            "h2hip_eval_brakhage_werner_c")
+        fn = hip().h2hip_eval_brakhage_werner_l if self.basis else \
+            hip().h2hip_eval_brakhage_werner_c
+        check(fn(self.gr.handle, self.q, k.ctypes.data_as(C.c_void_p),
+                 e.ctypes.data_as(C.c_void_p), C.c_void_p(d_w.ptr), len(pts),
+                 pts.ctypes.data_as(C.c_void_p),
+                 out.ctypes.data_as(C.c_void_p),
+                 C.c_void_p(stream.handle if stream else 0)),
+              "h2hip_eval_brakhage_werner")
         return out
 
     @staticmethod

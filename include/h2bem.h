@@ -166,6 +166,12 @@ void h2b_projectL2_c_helmholtz(const h2b_surface *s, unsigned q, int kind,
  * w_q data(x_q) (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629). */
 void h2b_integrate_c_helmholtz(const h2b_surface *s, unsigned q, int kind,
                                double k, const double *x0, double *out);
+/* The same data against the piecewise linear basis, out[i] = sum over the
+ * triangles t at vertex i of g_t sum_q w_q phi_i(x_q) data(x_q)
+ * (integrate_bem3d_l_avector, Library/bem3d.c:8631-8702); 2 vertices
+ * doubles. */
+void h2b_integrate_l_helmholtz(const h2b_surface *s, unsigned q, int kind,
+                               double k, const double *x0, double *out);
 /* out[0] = |x - data|_L2 (normL2diff_c_bem3d, Library/bem3d.c:8776-8825;
  * x may be NULL for 0), out[1] = |data|_L2 (normL2_bem3d,
  * Library/bem3d.c:8704-8751). */

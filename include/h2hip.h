@@ -291,6 +291,14 @@ int h2hip_eval_brakhage_werner_c(const h2hip_surface *s, uint32_t q,
                                  const double *k, const double *eta,
                                  const void *d_w, uint32_t npts,
                                  const double *pts, void *out, void *stream);
+/* The same for a piecewise linear density d_w (one complex value per
+ * vertex): kernel_const sum_t g_t sum_i w_{t_i} sum_q w_q phi_i(y_q) ...
+ * (eval_brakhage_werner_l, Tests/test_helmholtzbem3d.c:133-221, which
+ * takes k real: only k[0] is used). */
+int h2hip_eval_brakhage_werner_l(const h2hip_surface *s, uint32_t q,
+                                 const double *k, const double *eta,
+                                 const void *d_w, uint32_t npts,
+                                 const double *pts, void *out, void *stream);
 
 /* ---- boundary data ----------------------------------------------------------
  * L2 projection of the Dirichlet (kind 0: 1/|x - x0|) or Neumann (kind 1:

@@ -626,12 +626,14 @@ ref_helmholtz_point_project(void *h, int kind, double k, const double *x0,
 /* The reference's Helmholtz test (Tests/test_helmholtzbem3d.c, compiled
  * into the complex library with its main renamed). */
 field eval_brakhage_werner_c(pcbem3d bem, pcavector w, field eta, real * x);
+field eval_brakhage_werner_l(pcbem3d bem, pcavector w, field eta, real * x);
 real max_rel_outer_error(pcbem3d bem, helmholtz_data * hdata, pcavector x,
 			 boundary_func3d rhs, basisfunctionbem3d basis);
 
-/* b = integral of the point-source Dirichlet data against the constant
- * basis (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629), as the
- * Helmholtz test forms its right-hand side. */
+/* b = integral of the point-source Dirichlet data against the operator's
+ * column basis (integrate_bem3d_c_avector / integrate_bem3d_l_avector,
+ * Library/bem3d.c:8579-8702), as the Helmholtz test forms its right-hand
+ * side. */
 void
 ref_helmholtz_point_integrate(void *h, double k, const double *x0, void *out)
 {
@@ -643,10 +645,17 @@ ref_helmholtz_point_integrate(void *h, double k, const double *x0, void *out)
   hd.kvec = kvec;
   hd.source = src;
   hd.eta = 0.0;
-  f = new_avector(rb->gr->triangles);
-  integrate_bem3d_c_avector(rb->bem, rhs_dirichlet_point_helmholtzbem3d, f,
-			    (void *) &hd);
-  memcpy(out, f->v, sizeof(field) * rb->gr->triangles);
+  if (rb->basis_c) {
+    f = new_avector(rb->gr->vertices);
+    integrate_bem3d_l_avector(rb->bem, rhs_dirichlet_point_helmholtzbem3d, f,
+			      (void *) &hd);
+  }
+  else {
+    f = new_avector(rb->gr->triangles);
+    integrate_bem3d_c_avector(rb->bem, rhs_dirichlet_point_helmholtzbem3d,
+			      f, (void *) &hd);
+  }
+  memcpy(out, f->v, sizeof(field) * f->dim);
   del_avector(f);
 }
 
@@ -689,7 +698,8 @@ ref_helmholtz_bw_solve(void *slp, void *dlp, double eta, const void *b,
 }
 
 /* The Brakhage-Werner potential of the density w at npts points
- * (eval_brakhage_werner_c, Tests/test_helmholtzbem3d.c:67-132). */
+ * (eval_brakhage_werner_c / _l by the operator's row basis,
+ * Tests/test_helmholtzbem3d.c:67-221). */
 void
 ref_eval_bw_c(void *h, const void *w, double eta, const double *pts,
 	      unsigned npts, void *out)
@@ -701,12 +711,14 @@ ref_eval_bw_c(void *h, const void *w, double eta, const double *pts,
   real      x[3];
   unsigned  j;
 
-  wv = init_pointer_avector(&wt, (field *) w, rb->gr->triangles);
+  wv = init_pointer_avector(&wt, (field *) w,
+			    rb->basis_r ? rb->gr->vertices : rb->gr->triangles);
   for (j = 0; j < npts; j++) {
     x[0] = pts[3 * j];
     x[1] = pts[3 * j + 1];
     x[2] = pts[3 * j + 2];
-    o[j] = eval_brakhage_werner_c(rb->bem, wv, eta, x);
+    o[j] = rb->basis_r ? eval_brakhage_werner_l(rb->bem, wv, eta, x)
+      : eval_brakhage_werner_c(rb->bem, wv, eta, x);
   }
 }
 
@@ -726,9 +738,11 @@ ref_helmholtz_outer_error(void *h, const void *x, double k, const double *x0)
   hd.kvec = kvec;
   hd.source = src;
   hd.eta = 0.0;
-  xv = init_pointer_avector(&xt, (field *) x, rb->gr->triangles);
+  xv = init_pointer_avector(&xt, (field *) x,
+			    rb->basis_r ? rb->gr->vertices : rb->gr->triangles);
   return max_rel_outer_error(rb->bem, &hd, xv,
 			     rhs_dirichlet_point_helmholtzbem3d,
+			     rb->basis_r ? BASIS_LINEAR_BEM3D :
 			     BASIS_CONSTANT_BEM3D);
 }
 

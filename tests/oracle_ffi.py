@@ -450,7 +450,7 @@ class ReferenceBem(_BemBase):
         return out
 
     def helmholtz_point_integrate(self, k, x0):
-        out = np.zeros(len(self.t), dtype=np.complex128)
+        out = np.zeros(self.ndof(self.basis_c), dtype=np.complex128)
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         self._lib.ref_helmholtz_point_integrate(self.h, k, _ptr(x0),
                                                 _ptr(out))

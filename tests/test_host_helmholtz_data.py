@@ -56,3 +56,17 @@ def test_point_integrals_match_reference(k, x0):
     want = ref.helmholtz_point_integrate(k, x0)
     got = helmholtz_point_integrals(mesh, 0, k, x0)
     assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+
+
+@pytest.mark.skipif(not have_reference(True), reason="reference not built")
+@pytest.mark.parametrize("k,x0", [(2.0, (0.0, 0.0, 0.2)), (4.0, (0.3, 0.1, -0.2))])
+def test_point_integrals_linear_match_reference(k, x0):
+    # the linear-basis Brakhage-Werner test's right-hand side
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import helmholtz_point_integrals
+    mesh = G.sphere(8)
+    ref = ReferenceBem(1, 1, 1, 1, mesh.x, mesh.t, q=2, q2=4, k=k, alpha=0.5)
+    want = ref.helmholtz_point_integrate(k, x0)
+    got = helmholtz_point_integrals(mesh, 0, k, x0, basis=1)
+    assert got.shape == (mesh.vertices,)
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
