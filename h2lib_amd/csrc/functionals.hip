@@ -227,6 +227,264 @@ extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
   return H2HIP_OK;
 }
 
+// ---- Helmholtz point-source data and error norms ---------------------------
+//
+// Dirichlet data e^{ikr}/r (kind 0) or Neumann data e^{ikr}/r^3 (ikr - 1)
+// <n, x - x0> (kind 1) of a point source x0, r = |x - x0|, real k
+// (rhs_dirichlet_point_helmholtzbem3d / rhs_neumann_point_helmholtzbem3d,
+// Library/helmholtzbem3d.c:1521-1578), against the single-triangle rule of
+// order q:
+//   mode 0  projection onto the constants: out[t] = 2 sum_q w_q f(x_q)
+//           (projectL2_bem3d_c_avector, Library/bem3d.c:8827-8876; the
+//           rule's weights sum to 1/2)
+//   mode 1  integrals against the constants: out[t] = g_t sum_q w_q f
+//           (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629)
+//   mode 2  integrals against the linear basis: out[i] = sum over the
+//           triangles t at vertex i of g_t sum_q w_q phi_i(x_q) f(x_q)
+//           (integrate_bem3d_l_avector, Library/bem3d.c:8631-8702), one
+//           thread per vertex over its triangles (no atomics)
+// and the norms |x - f|_L2, |f|_L2 of a constant-basis x
+// (normL2diff_c_bem3d, Library/bem3d.c:8776-8825, normL2_bem3d,
+// Library/bem3d.c:8704-8751).
+
+namespace {
+
+constexpr int HD_NT = 256;
+constexpr int HD_MAXQ = 64;
+
+struct HelmArgs {
+  const double *x, *n, *g;
+  const uint32_t *t;
+  const uint32_t *vt_ptr, *vt;
+  uint32_t nt, nv;
+  uint32_t nq;
+  double rx0[HD_MAXQ], rx1[HD_MAXQ], rw[HD_MAXQ];
+  double src[3];
+  double k;
+  int kind, accumulate;
+  double *out;
+  const double *xv;  // error norms: the constant-basis vector
+  double *part;      // error norms: 2 per block
+};
+
+__device__ __forceinline__ void helm_point(const HelmArgs &a,
+                                           const double *nn, double px,
+                                           double py, double pz, double &re,
+                                           double &im) {
+  const double d0 = px - a.src[0], d1 = py - a.src[1], d2 = pz - a.src[2];
+  const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  const double rs = 1.0 / sqrt(r2), r = r2 * rs;
+  double sn, cs;
+  sincos(a.k * r, &sn, &cs);
+  re = cs * rs;
+  im = sn * rs;
+  if (a.kind == 1) {
+    const double f = (nn[0] * d0 + nn[1] * d1 + nn[2] * d2) * rs * rs;
+    const double kr = a.k * r;
+    const double nr = -re - im * kr, ni = re * kr - im;
+    re = nr * f;
+    im = ni * f;
+  }
+}
+
+// sum_q w_q phi(x_q) f(x_q) on triangle t; corner < 0: phi = 1
+__device__ void helm_tri(const HelmArgs &a, uint32_t t, int corner,
+                         double &sre, double &sim) {
+  const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
+               *C = a.x + 3 * a.t[3 * t + 2];
+  const double *nn = a.n + 3 * t;
+  sre = 0.0;
+  sim = 0.0;
+  for (uint32_t q = 0; q < a.nq; q++) {
+    const double u = a.rx0[q], v = a.rx1[q];
+    const double ca = 1.0 - u, cb = u - v, cc = v;
+    double re, im;
+    helm_point(a, nn, A[0] * ca + B[0] * cb + C[0] * cc,
+               A[1] * ca + B[1] * cb + C[1] * cc,
+               A[2] * ca + B[2] * cb + C[2] * cc, re, im);
+    const double w = a.rw[q] * (corner < 0 ? 1.0 : corner == 0 ? ca
+                                                 : corner == 1 ? cb : cc);
+    sre += w * re;
+    sim += w * im;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(HD_NT) void helm_data(HelmArgs a) {
+  const uint32_t i = blockIdx.x * HD_NT + threadIdx.x;
+  double re = 0.0, im = 0.0;
+  if constexpr (MODE == 2) {
+    if (i >= a.nv) return;
+    for (uint32_t e = a.vt_ptr[i]; e < a.vt_ptr[i + 1]; e++) {
+      const uint32_t code = a.vt[e], t = code >> 2;
+      double sr, si;
+      helm_tri(a, t, (int)(code & 3), sr, si);
+      re += sr * a.g[t];
+      im += si * a.g[t];
+    }
+  } else {
+    if (i >= a.nt) return;
+    helm_tri(a, i, -1, re, im);
+    const double f = MODE == 0 ? 2.0 : a.g[i];
+    re *= f;
+    im *= f;
+  }
+  if (a.accumulate) {
+    a.out[2 * i] += re;
+    a.out[2 * i + 1] += im;
+  } else {
+    a.out[2 * i] = re;
+    a.out[2 * i + 1] = im;
+  }
+}
+
+__global__ __launch_bounds__(HD_NT) void helm_errors(HelmArgs a) {
+  const uint32_t t = blockIdx.x * HD_NT + threadIdx.x;
+  double sd = 0.0, sn = 0.0;
+  if (t < a.nt) {
+    const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
+                 *C = a.x + 3 * a.t[3 * t + 2];
+    const double *nn = a.n + 3 * t;
+    const double xr = a.xv[2 * t], xi = a.xv[2 * t + 1];
+    for (uint32_t q = 0; q < a.nq; q++) {
+      const double u = a.rx0[q], v = a.rx1[q];
+      const double ca = 1.0 - u, cb = u - v, cc = v;
+      double re, im;
+      helm_point(a, nn, A[0] * ca + B[0] * cb + C[0] * cc,
+                 A[1] * ca + B[1] * cb + C[1] * cc,
+                 A[2] * ca + B[2] * cb + C[2] * cc, re, im);
+      const double er = re - xr, ei = im - xi;
+      sd += a.rw[q] * (er * er + ei * ei);
+      sn += a.rw[q] * (re * re + im * im);
+    }
+    sd *= a.g[t];
+    sn *= a.g[t];
+  }
+  __shared__ double red[2][HD_NT / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    sd += __shfl_xor(sd, off, 64);
+    sn += __shfl_xor(sn, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = sd;
+    red[1][threadIdx.x >> 6] = sn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d = 0.0, m = 0.0;
+    for (int w = 0; w < HD_NT / 64; w++) {
+      d += red[0][w];
+      m += red[1][w];
+    }
+    a.part[2 * blockIdx.x] = d;
+    a.part[2 * blockIdx.x + 1] = m;
+  }
+}
+
+int helm_setup(const h2hip_surface *s, uint32_t q, int kind, double k,
+               const double *x0, HelmArgs &a) {
+  const h2hip::TriRule tr = h2hip::make_triangle_rule(q);
+  a.x = s->d_x.p;
+  a.n = s->d_n.p;
+  a.g = s->d_g.p;
+  a.t = s->d_t.p;
+  a.vt_ptr = s->d_vt_ptr.p;
+  a.vt = s->d_vt.p;
+  a.nt = s->nt;
+  a.nv = s->nv;
+  a.nq = (uint32_t)tr.w.size();
+  if (a.nq > (uint32_t)HD_MAXQ) return H2HIP_EINVAL;
+  for (uint32_t i = 0; i < a.nq; i++) {
+    a.rx0[i] = tr.x0[i];
+    a.rx1[i] = tr.x1[i];
+    a.rw[i] = tr.w[i];
+  }
+  for (int c = 0; c < 3; c++) a.src[c] = x0[c];
+  a.k = k;
+  a.kind = kind;
+  a.accumulate = 0;
+  a.out = nullptr;
+  a.xv = nullptr;
+  a.part = nullptr;
+  return H2HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int h2hip_helmholtz_point_data(const h2hip_surface *s, uint32_t q,
+                                          int kind, int mode, double k,
+                                          const double *x0, int accumulate,
+                                          void *d_out, void *stream) {
+  if (!s || !x0 || !d_out || q < 1 || q > 8 || (kind != 0 && kind != 1) ||
+      mode < 0 || mode > 2) {
+    h2hip::set_error("h2hip_helmholtz_point_data: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  if (mode == 2) {
+    std::lock_guard<std::mutex> guard(s->fl_lock);
+    if (!s->fl_ready) {
+      const int rc = build_projection_data(s);
+      if (rc != H2HIP_OK) return rc;
+      s->fl_ready = true;
+    }
+  }
+  HelmArgs a;
+  if (helm_setup(s, q, kind, k, x0, a) != H2HIP_OK) {
+    h2hip::set_error("h2hip_helmholtz_point_data: rule too large");
+    return H2HIP_EINVAL;
+  }
+  a.accumulate = accumulate ? 1 : 0;
+  a.out = (double *)d_out;
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t n = mode == 2 ? s->nv : s->nt;
+  const dim3 grid((n + HD_NT - 1) / HD_NT);
+  if (mode == 0)
+    hipLaunchKernelGGL(helm_data<0>, grid, dim3(HD_NT), 0, st, a);
+  else if (mode == 1)
+    hipLaunchKernelGGL(helm_data<1>, grid, dim3(HD_NT), 0, st, a);
+  else
+    hipLaunchKernelGGL(helm_data<2>, grid, dim3(HD_NT), 0, st, a);
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
+}
+
+extern "C" int h2hip_helmholtz_point_errors_c(const h2hip_surface *s,
+                                              uint32_t q, int kind, double k,
+                                              const double *x0,
+                                              const void *d_x, double *out,
+                                              void *stream) {
+  if (!s || !x0 || !d_x || !out || q < 1 || q > 8 ||
+      (kind != 0 && kind != 1)) {
+    h2hip::set_error("h2hip_helmholtz_point_errors_c: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  HelmArgs a;
+  if (helm_setup(s, q, kind, k, x0, a) != H2HIP_OK) {
+    h2hip::set_error("h2hip_helmholtz_point_errors_c: rule too large");
+    return H2HIP_EINVAL;
+  }
+  const uint32_t nb = (s->nt + HD_NT - 1) / HD_NT;
+  h2hip::DevBuf<double> part;
+  H2HIP_CHECK(part.alloc(2 * (size_t)nb));
+  a.xv = (const double *)d_x;
+  a.part = part.p;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(helm_errors, dim3(nb), dim3(HD_NT), 0, st, a);
+  H2HIP_CHECK(hipGetLastError());
+  std::vector<double> h(2 * (size_t)nb);
+  H2HIP_CHECK(hipMemcpyAsync(h.data(), part.p, sizeof(double) * h.size(),
+                             hipMemcpyDeviceToHost, st));
+  H2HIP_CHECK(hipStreamSynchronize(st));
+  double d = 0.0, m = 0.0;
+  for (uint32_t b = 0; b < nb; b++) {
+    d += h[2 * b];
+    m += h[2 * b + 1];
+  }
+  out[0] = std::sqrt(d);
+  out[1] = std::sqrt(m);
+  return H2HIP_OK;
+}
+
 // ---- exterior potential of the Brakhage-Werner density ---------------------
 //
 // u(x) = kernel_const sum_t g_t w_t sum_q ww_q e^{ik r}/r^3 ((1 - ik r)

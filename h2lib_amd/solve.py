@@ -93,6 +93,12 @@ def _sigs():
                                             C.c_double, dp, dp]
     b.h2b_integrate_l_helmholtz.argtypes = [dp, C.c_uint, C.c_int,
                                             C.c_double, dp, dp]
+    h.h2hip_helmholtz_point_data.restype = C.c_int
+    h.h2hip_helmholtz_point_data.argtypes = [dp, C.c_uint32, C.c_int, C.c_int,
+                                             C.c_double, dp, C.c_int, dp, dp]
+    h.h2hip_helmholtz_point_errors_c.restype = C.c_int
+    h.h2hip_helmholtz_point_errors_c.argtypes = [dp, C.c_uint32, C.c_int,
+                                                 C.c_double, dp, dp, dp, dp]
     h.h2hip_eval_brakhage_werner_l.restype = C.c_int
     h.h2hip_eval_brakhage_werner_l.argtypes = [dp, C.c_uint32, dp, dp, dp,
                                                C.c_uint32, dp, dp, dp]
@@ -132,6 +138,39 @@ def helmholtz_point_errors(mesh, x, kind, k, x0, q=2):
         mesh._sp, q, None if xx is None else xx.ctypes.data_as(C.c_void_p),
         kind, float(k), x0.ctypes.data_as(C.c_void_p),
         out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+PROJECT_C, INTEGRATE_C, INTEGRATE_L = 0, 1, 2
+
+
+def helmholtz_point_data_device(gr, q, kind, mode, k, x0, d_out,
+                                accumulate=False, stream=None):
+    """Point-source data on the device (h2hip_helmholtz_point_data):
+    mode PROJECT_C projection onto the constants, INTEGRATE_C integrals
+    against the constants, INTEGRATE_L integrals against the linear basis,
+    into the device array d_out (added with accumulate), ordered on the
+    stream.  gr: the Surface3d of the operators."""
+    _sigs()
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    check(hip().h2hip_helmholtz_point_data(
+        gr.handle, q, kind, mode, float(k), x0.ctypes.data_as(C.c_void_p),
+        int(bool(accumulate)), C.c_void_p(d_out.ptr),
+        C.c_void_p(stream.handle if stream else 0)),
+        "h2hip_helmholtz_point_data")
+
+
+def helmholtz_point_errors_device(gr, q, kind, k, x0, d_x, stream=None):
+    """(|x - data|_L2, |data|_L2) for the constant-basis device vector d_x
+    (h2hip_helmholtz_point_errors_c)."""
+    _sigs()
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    out = np.zeros(2)
+    check(hip().h2hip_helmholtz_point_errors_c(
+        gr.handle, q, kind, float(k), x0.ctypes.data_as(C.c_void_p),
+        C.c_void_p(d_x.ptr), out.ctypes.data_as(C.c_void_p),
+        C.c_void_p(stream.handle if stream else 0)),
+        "h2hip_helmholtz_point_errors_c")
     return out
 
 
@@ -467,21 +506,20 @@ class HelmholtzExterior:
         self.n = mesh.triangles
 
     def rhs(self, x0, stream=None):
-        """(f, b) on the device for the point source at x0."""
-        f = helmholtz_point_data(self.mesh, 0, self.k, x0, self.q)
-        d_f = DeviceArray.from_host(f)
-        d_b = DeviceArray(self.n, np.complex128)
-        d_b.zero(stream)
-        self.K.addeval(1.0, d_f, d_b, stream)
-        return d_f, d_b
+        """(f, b) on the device for the point source at x0, formed on the
+        device and ordered on the stream."""
+        return self.rhs_sources([x0], stream)
 
     def rhs_sources(self, x0s, stream=None):
         """(f, b) with b = (K - M/2) f for several unit point sources (f =
-        sum of their projected Dirichlet data).  Keep f alive until the
-        stream has passed the product."""
-        f = sum(helmholtz_point_data(self.mesh, 0, self.k, x0, self.q)
-                for x0 in x0s)
-        d_f = DeviceArray.from_host(np.asarray(f, dtype=np.complex128))
+        sum of their projected Dirichlet data), formed on the device and
+        ordered on the stream.  Keep f alive until the stream has passed
+        the product."""
+        d_f = DeviceArray(self.n, np.complex128)
+        for j, x0 in enumerate(x0s):
+            helmholtz_point_data_device(self.gr, self.q, 0, PROJECT_C, self.k,
+                                        x0, d_f, accumulate=j > 0,
+                                        stream=stream)
         d_b = DeviceArray(self.n, np.complex128)
         d_b.zero(stream)
         self.K.addeval(1.0, d_f, d_b, stream)
@@ -494,8 +532,14 @@ class HelmholtzExterior:
         return d_x, it, res
 
     def neumann_error(self, x, x0):
-        """|x - Neumann data|_L2 / |Neumann data|_L2."""
-        d, nrm = helmholtz_point_errors(self.mesh, x, 1, self.k, x0, self.q)
+        """|x - Neumann data|_L2 / |Neumann data|_L2 for x on the host or
+        the device."""
+        if isinstance(x, DeviceArray):
+            d, nrm = helmholtz_point_errors_device(self.gr, self.q, 1, self.k,
+                                                   x0, x)
+        else:
+            d, nrm = helmholtz_point_errors(self.mesh, x, 1, self.k, x0,
+                                            self.q)
         return d / nrm
 
 
@@ -545,10 +589,15 @@ class HelmholtzBrakhageWerner:
         self.A = Combined(self.KM, 1.0, self.V, -1j * self.eta)
         self.n = mesh.vertices if basis else mesh.triangles
 
-    def rhs(self, x0):
-        return DeviceArray.from_host(
-            helmholtz_point_integrals(self.mesh, 0, self.k, x0, self.q,
-                                      self.basis))
+    def rhs(self, x0, stream=None):
+        """b = integrals of the Dirichlet data against the basis, formed on
+        the device."""
+        d_b = DeviceArray(self.n, np.complex128)
+        helmholtz_point_data_device(self.gr, self.q, 0,
+                                    INTEGRATE_L if self.basis else
+                                    INTEGRATE_C, self.k, x0, d_b,
+                                    stream=stream)
+        return d_b
 
     def solve(self, d_b, eps=1e-12, maxiter=1000, kmax=50, stream=None):
         d_x = DeviceArray(self.n, np.complex128)

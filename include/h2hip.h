@@ -266,6 +266,31 @@ int h2hip_solve_cg(h2hip_addeval_t addeval, void *A, int is_complex,
                    uint32_t maxiter, int use_graph, void *stream,
                    uint32_t *iterations, double *residual);
 
+/* Helmholtz point-source data on the device (real wave number k, source
+ * x0, Dirichlet data e^{ikr}/r for kind 0, Neumann data for kind 1,
+ * rhs_dirichlet_point_helmholtzbem3d / rhs_neumann_point_helmholtzbem3d,
+ * Library/helmholtzbem3d.c:1521-1578) with the single-triangle rule of
+ * order q (1..8), complex output d_out on the device, ordered on the
+ * stream without synchronising:
+ *   mode 0: L2 projection onto the constants, one value per triangle
+ *           (projectL2_bem3d_c_avector, Library/bem3d.c:8827-8876);
+ *   mode 1: integrals against the constants, one per triangle
+ *           (integrate_bem3d_c_avector, Library/bem3d.c:8579-8629);
+ *   mode 2: integrals against the linear basis, one per vertex
+ *           (integrate_bem3d_l_avector, Library/bem3d.c:8631-8702).
+ * accumulate != 0 adds to d_out (data of several sources). */
+int h2hip_helmholtz_point_data(const h2hip_surface *s, uint32_t q, int kind,
+                               int mode, double k, const double *x0,
+                               int accumulate, void *d_out, void *stream);
+/* out[0] = |x - data|_L2 and out[1] = |data|_L2 for the constant-basis
+ * device vector d_x (normL2diff_c_bem3d, Library/bem3d.c:8776-8825;
+ * normL2_bem3d, Library/bem3d.c:8704-8751).  Returns when out is
+ * written. */
+int h2hip_helmholtz_point_errors_c(const h2hip_surface *s, uint32_t q,
+                                   int kind, double k, const double *x0,
+                                   const void *d_x, double *out,
+                                   void *stream);
+
 /* ---- operator combinations and the exterior potential ---------------------
  * A = c1 A1 + c2 A2 as one operator for the Krylov solvers: the
  * Brakhage-Werner operator KM - i eta V of the reference's Helmholtz test
