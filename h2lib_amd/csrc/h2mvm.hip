@@ -393,6 +393,41 @@ __global__ __launch_bounds__(NT) void fwd_level(FwdArgs a,
   }
 }
 
+// One coupling entry of an interpolation H2-matrix from the coordinate
+// differences of the two tensor Chebyshev points: kernel_const / r
+// (Laplace) or kernel_const e^{i kr r} e^{kdamp r} / r (Helmholtz), as
+// assemble_bem3d_inter_uniform fills them (Library/bem3d.c:6079-6108).
+// Shared by the kernel that stores the couplings and the product that
+// recomputes them, so that both produce the same values.
+__device__ __forceinline__ void cpl_value(double d0, double d1, double d2,
+                                          int helmholtz, double kr,
+                                          double kdamp, double &sr,
+                                          double &si) {
+  constexpr double KCONST = 0.0795774715459476679;
+  const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  double rs = 1.0 / sqrt(r2);
+  if (!helmholtz) {
+    sr = KCONST * rs;
+    si = 0.0;
+  } else {
+    const double r = r2 * rs;
+    if (kdamp != 0.0) rs *= exp(kdamp * r);
+    double sn, cs;
+    sincos(kr * r, &sn, &cs);
+    sr = KCONST * (rs * cs);
+    si = KCONST * (sn * rs);
+  }
+}
+
+// Interpolation data for couplings computed in the product: Chebyshev
+// points (3 m per node, x then y then z) of both trees and the kernel.
+struct CplSrc {
+  const double *rpts, *cpts;
+  uint32_t m;
+  int helmholtz;
+  double kr, kdamp;
+};
+
 struct CNArgs {
   const FarD *far;
   uint32_t nfar;
@@ -405,14 +440,56 @@ struct CNArgs {
   double *yt;
   uint64_t cstride;
   VecSet v;
+  CplSrc cs;  // MF products: couplings computed here
 };
 
+// acc += sum_j M_j xs_j over n inputs for a column-major block read along
+// a row: M points at the row's first entry, entry j lies stride entries
+// further (complex entries as pairs); 16 (complex: 8) entries are loaded
+// before their products, so that each wave keeps enough loads in flight.
+template <int C>
+__device__ __forceinline__ void dot_row(const double *__restrict__ M,
+                                        uint64_t stride, uint32_t n,
+                                        const double *xs, double &ar,
+                                        double &ai) {
+  constexpr int B = C ? 8 : 16;
+  uint32_t j = 0;
+  for (; j + B <= n; j += B) {
+    double mr[B], mi[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) ld<C>(M, (uint64_t)(j + u) * stride, mr[u], mi[u]);
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const double xr = xs[2 * (j + u)], xi = xs[2 * (j + u) + 1];
+      ar = fma(mr[u], xr, ar);
+      if constexpr (C) {
+        ar = fma(-mi[u], xi, ar);
+        ai = fma(mr[u], xi, fma(mi[u], xr, ai));
+      }
+    }
+  }
+  for (; j < n; j++) {
+    double mr, mi;
+    ld<C>(M, (uint64_t)j * stride, mr, mi);
+    const double xr = xs[2 * j], xi = xs[2 * j + 1];
+    ar = fma(mr, xr, ar);
+    if constexpr (C) {
+      ar = fma(-mi, xi, ar);
+      ai = fma(mr, xi, fma(mi, xr, ai));
+    }
+  }
+}
+
 // yt_r += S_rc xt_c and y|_r += alpha N_rc x|_c; TRANS: yt_c += S^* xt_r and
-// y|_c += alpha N^* x|_r.  One wave per block; with several vectors the
-// wave first gathers its block's input entries into its own LDS slice.
-template <int C, bool TRANS, int NV>
+// y|_c += alpha N^* x|_r.  One wave per block; the wave first gathers its
+// block's input entries into its own LDS slice.
+// MF (one vector, not transposed): the couplings are computed from the
+// interpolation points instead of read (cpl_value), lane i keeping row
+// point i and stepping through the column points j = (ja m + jb) m + jc.
+template <int C, bool TRANS, int NV, bool MF = false>
 __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
-  constexpr int XS = NV == 1 ? 1 : NV * 2 * 64;  // LDS doubles per wave
+  constexpr int XR = NV == 1 ? MAXRANK : 64;  // inputs per LDS vector
+  constexpr int XS = NV * 2 * XR;             // LDS doubles per wave
   constexpr int UNR = NV == 1 ? 8 : 4;
   __shared__ double xs_all[WPB * XS];
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
@@ -422,43 +499,65 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
     const FarD b = a.far[w];
     const NodeD r = a.rn[b.r], c = a.cn[b.c];
     const NodeD out = TRANS ? c : r, in = TRANS ? r : c;
-    if constexpr (NV > 1) {
-      // ranks <= 64 in this mode: one pass
-      for (uint32_t e = lane; e < NV * in.rank; e += 64) {
-        const uint32_t v = e / in.rank, j = e - v * in.rank;
-        double xr = 0.0, xi = 0.0;
-        if (v < a.v.nv) ld<C>(a.xt + v * a.cstride, in.coef + j, xr, xi);
-        xs[v * 128 + 2 * j] = xr;
-        xs[v * 128 + 2 * j + 1] = xi;
-      }
-      // the wave reads what its own lanes wrote: LDS keeps the order of
-      // one wave's accesses, the compiler must not move them
-      __builtin_amdgcn_wave_barrier();
+    // ranks <= 128 (one vector) or <= 64 (several)
+    for (uint32_t e = lane; e < NV * in.rank; e += 64) {
+      const uint32_t v = e / in.rank, j = e - v * in.rank;
+      double xr = 0.0, xi = 0.0;
+      if (v < a.v.nv) ld<C>(a.xt + v * a.cstride, in.coef + j, xr, xi);
+      xs[v * 2 * XR + 2 * j] = xr;
+      xs[v * 2 * XR + 2 * j + 1] = xi;
     }
+    // the wave reads what its own lanes wrote: LDS keeps the order of
+    // one wave's accesses, the compiler must not move them
+    __builtin_amdgcn_wave_barrier();
     for (uint32_t i = lane; i < out.rank; i += 64) {
       double ar[NV], ai[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
-#pragma unroll UNR
-      for (uint32_t j = 0; j < in.rank; j++) {
-        const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
-                                 : (uint64_t)i + (uint64_t)j * r.rank;
-        double sr, si;
-        ld<C>(a.S, b.soff + e, sr, si);
-        if (TRANS) si = -si;
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          double xr, xi;
-          if constexpr (NV == 1) {
-            ld<C>(a.xt, in.coef + j, xr, xi);
-          } else {
-            xr = xs[v * 128 + 2 * j];
-            xi = xs[v * 128 + 2 * j + 1];
+      if constexpr (MF) {
+        const uint32_t m = a.cs.m;
+        const double *pr = a.cs.rpts + 3 * (uint64_t)m * b.r;
+        const double *pc = a.cs.cpts + 3 * (uint64_t)m * b.c;
+        const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
+        const double px = pr[ia], py = pr[m + ib], pz = pr[2 * m + ic];
+        uint32_t j = 0;
+        for (uint32_t ja = 0; ja < m; ja++) {
+          const double d0 = px - pc[ja];
+          for (uint32_t jb = 0; jb < m; jb++) {
+            const double d1 = py - pc[m + jb];
+            for (uint32_t jc = 0; jc < m; jc++, j++) {
+              double sr, si;
+              cpl_value(d0, d1, pz - pc[2 * m + jc], a.cs.helmholtz, a.cs.kr,
+                        a.cs.kdamp, sr, si);
+              const double xr = xs[2 * j], xi = xs[2 * j + 1];
+              ar[0] = fma(sr, xr, ar[0]);
+              if constexpr (C) {
+                ar[0] = fma(-si, xi, ar[0]);
+                ai[0] = fma(sr, xi, fma(si, xr, ai[0]));
+              }
+            }
           }
-          ar[v] = fma(sr, xr, ar[v]);
-          if constexpr (C) {
-            ar[v] = fma(-si, xi, ar[v]);
-            ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
+        }
+      } else if constexpr (NV == 1 && !TRANS) {
+        dot_row<C>(a.S + (C ? 2 : 1) * (b.soff + i), r.rank, in.rank, xs,
+                   ar[0], ai[0]);
+      } else {
+#pragma unroll UNR
+        for (uint32_t j = 0; j < in.rank; j++) {
+          const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
+                                   : (uint64_t)i + (uint64_t)j * r.rank;
+          double sr, si;
+          ld<C>(a.S, b.soff + e, sr, si);
+          if (TRANS) si = -si;
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            const double xr = xs[v * 2 * XR + 2 * j];
+            const double xi = xs[v * 2 * XR + 2 * j + 1];
+            ar[v] = fma(sr, xr, ar[v]);
+            if constexpr (C) {
+              ar[v] = fma(-si, xi, ar[v]);
+              ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
+            }
           }
         }
       }
@@ -475,41 +574,39 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
   const uint32_t nout = TRANS ? b.cols : b.rows, nin = TRANS ? b.rows : b.cols;
   const uint32_t *oidx = TRANS ? a.cidx + b.coff : a.ridx + b.roff;
   const uint32_t *iidx = TRANS ? a.ridx + b.roff : a.cidx + b.coff;
-  if constexpr (NV > 1) {
-    // leaf sizes <= 64 in this mode: one pass
-    for (uint32_t e = lane; e < NV * nin; e += 64) {
-      const uint32_t v = e / nin, j = e - v * nin;
-      double xr = 0.0, xi = 0.0;
-      if (v < a.v.nv) ld<C>(a.v.x[v], iidx[j], xr, xi);
-      xs[v * 128 + 2 * j] = xr;
-      xs[v * 128 + 2 * j + 1] = xi;
-    }
-    __builtin_amdgcn_wave_barrier();
+  // leaf sizes <= 128 (one vector) or <= 64 (several)
+  for (uint32_t e = lane; e < NV * nin; e += 64) {
+    const uint32_t v = e / nin, j = e - v * nin;
+    double xr = 0.0, xi = 0.0;
+    if (v < a.v.nv) ld<C>(a.v.x[v], iidx[j], xr, xi);
+    xs[v * 2 * XR + 2 * j] = xr;
+    xs[v * 2 * XR + 2 * j + 1] = xi;
   }
+  __builtin_amdgcn_wave_barrier();
   for (uint32_t i = lane; i < nout; i += 64) {
     double ar[NV], ai[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+    if constexpr (NV == 1 && !TRANS) {
+      dot_row<C>(a.N + (C ? 2 : 1) * (b.noff + i), b.rows, nin, xs, ar[0],
+                 ai[0]);
+    } else {
 #pragma unroll UNR
-    for (uint32_t j = 0; j < nin; j++) {
-      const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * b.rows
-                               : (uint64_t)i + (uint64_t)j * b.rows;
-      double nr, ni;
-      ld<C>(a.N, b.noff + e, nr, ni);
-      if (TRANS) ni = -ni;
+      for (uint32_t j = 0; j < nin; j++) {
+        const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * b.rows
+                                 : (uint64_t)i + (uint64_t)j * b.rows;
+        double nr, ni;
+        ld<C>(a.N, b.noff + e, nr, ni);
+        if (TRANS) ni = -ni;
 #pragma unroll
-      for (int v = 0; v < NV; v++) {
-        double xr, xi;
-        if constexpr (NV == 1) {
-          ld<C>(a.v.x[0], iidx[j], xr, xi);
-        } else {
-          xr = xs[v * 128 + 2 * j];
-          xi = xs[v * 128 + 2 * j + 1];
-        }
-        ar[v] = fma(nr, xr, ar[v]);
-        if constexpr (C) {
-          ar[v] = fma(-ni, xi, ar[v]);
-          ai[v] = fma(nr, xi, fma(ni, xr, ai[v]));
+        for (int v = 0; v < NV; v++) {
+          const double xr = xs[v * 2 * XR + 2 * j];
+          const double xi = xs[v * 2 * XR + 2 * j + 1];
+          ar[v] = fma(nr, xr, ar[v]);
+          if constexpr (C) {
+            ar[v] = fma(-ni, xi, ar[v]);
+            ai[v] = fma(nr, xi, fma(ni, xr, ai[v]));
+          }
         }
       }
     }
@@ -711,6 +808,190 @@ __global__ __launch_bounds__(NT) void down_back(DBArgs a) {
   }
 }
 
+// ---- flat transforms ----------------------------------------------------------
+//
+// With the explicit cluster bases restricted to the leaves, W_a|_l = V_l
+// E_l E_{father(l)} ... (the rows of leaf l of the basis of its ancestor a;
+// nested bases make them equal to the bases built directly), the two
+// transforms lose their dependence chains:
+//   xt_a = sum over the leaves l below a of W_a|_l^T x|_l
+//   y|_l += alpha sum over the ancestors a of l of W_a|_l yt_a
+// One wavefront per (leaf, ancestor) pair in both launches; the upward one
+// adds into the coefficients with FP64 atomics (cleared by a launch of its
+// own), the downward one into y.  The blocks W_a|_l (|l| x k_a) are built
+// once on the device from V and E, row-major for the upward sweep and
+// column-major for the downward one, so that both read 64 consecutive
+// doubles per lane step.
+
+struct FlatItem {
+  uint32_t leaf, node;  // node: the ancestor a (the leaf itself included)
+  uint64_t woff;        // offset of the block W_a|_l
+};
+
+// acc += sum_i M[i * ld] xs[i] over n inputs, 16 loads issued before their
+// products (the sweeps are bound by the loads in flight per wave)
+template <int C>
+__device__ __forceinline__ void dot_strided(const double *__restrict__ M,
+                                            uint64_t ld, uint32_t n,
+                                            const double *xs, double &ar,
+                                            double &ai) {
+  constexpr int B = 16;
+  uint32_t i = 0;
+  for (; i + B <= n; i += B) {
+    double m[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) m[u] = M[(uint64_t)(i + u) * ld];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      ar = fma(m[u], xs[2 * (i + u)], ar);
+      if constexpr (C) ai = fma(m[u], xs[2 * (i + u) + 1], ai);
+    }
+  }
+  for (; i < n; i++) {
+    const double m = M[(uint64_t)i * ld];
+    ar = fma(m, xs[2 * i], ar);
+    if constexpr (C) ai = fma(m, xs[2 * i + 1], ai);
+  }
+}
+
+struct FlatArgs {
+  const NodeD *nodes;
+  const FlatItem *items;
+  uint32_t nitems;
+  const uint32_t *idx;
+  const double *W;
+  double *xt;        // upward: coefficients (atomics)
+  const double *yt;  // downward: coefficients
+  double *zero;      // cleared by this launch (the buffer the next launch
+  uint64_t nzero;    // accumulates into / the next product starts from)
+  VecSet v;
+};
+
+// a slice of a.zero per workgroup
+__device__ __forceinline__ void flat_zero(const FlatArgs &a) {
+  const uint64_t chunk = (a.nzero + gridDim.x - 1) / gridDim.x;
+  const uint64_t e0 = (uint64_t)blockIdx.x * chunk,
+                 e1 = min(a.nzero, e0 + chunk);
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += NT) a.zero[e] = 0.0;
+}
+
+template <int C, int K>
+__global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
+  __shared__ double xs_all[WPB][2 * MAXRANK];
+  flat_zero(a);  // the coupling accumulators yt
+  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= a.nitems) return;  // uniform over the wave; no block barrier
+  double *xs = xs_all[threadIdx.x >> 6];
+  const FlatItem it = a.items[w];
+  const NodeD L = a.nodes[it.leaf];
+  const NodeD A = a.nodes[it.node];
+  const uint32_t sz = L.size, k = K ? K : A.rank;
+  for (uint32_t i = lane; i < sz; i += 64) {
+    double xr, xi;
+    ld<C>(a.v.x[0], a.idx[L.off + i], xr, xi);
+    xs[2 * i] = xr;
+    xs[2 * i + 1] = xi;
+  }
+  // the wave reads what its own lanes wrote: LDS keeps the order of one
+  // wave's accesses, the compiler must not move them
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t j = lane; j < k; j += 64) {
+    double ar = 0.0, ai = 0.0;
+    dot_strided<C>(a.W + it.woff + j, k, sz, xs, ar, ai);
+    atomic_add<C>(a.xt, A.coef + j, ar, ai);
+  }
+}
+
+template <int C, int K>
+__global__ __launch_bounds__(NT) void back_flat(FlatArgs a) {
+  __shared__ double ys_all[WPB][2 * MAXRANK];
+  flat_zero(a);  // the input coefficients xt, for the next product
+  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= a.nitems) return;
+  double *ys = ys_all[threadIdx.x >> 6];
+  const FlatItem it = a.items[w];
+  const NodeD L = a.nodes[it.leaf];
+  const NodeD A = a.nodes[it.node];
+  const uint32_t sz = L.size, k = K ? K : A.rank;
+  for (uint32_t j = lane; j < k; j += 64) {
+    double yr, yi;
+    ld<C>(a.yt, A.coef + j, yr, yi);
+    ys[2 * j] = yr;
+    ys[2 * j + 1] = yi;
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = lane; i < sz; i += 64) {
+    double ar = 0.0, ai = 0.0;
+    dot_strided<C>(a.W + it.woff + i, sz, k, ys, ar, ai);
+    atomic_add<C>(a.v.y[0], a.idx[L.off + i],
+                  a.v.are[0] * ar - a.v.aim[0] * ai,
+                  a.v.are[0] * ai + a.v.aim[0] * ar);
+  }
+}
+
+struct WBuildArgs {
+  const NodeD *nodes;
+  const FlatItem *items;
+  const uint32_t *ibeg;     // items of leaf l: ibeg[l] .. ibeg[l + 1]
+  const uint32_t *leaves;   // node numbers of the leaves
+  const uint32_t *rowleaf;  // index position -> leaf number
+  uint32_t ndof;
+  const double *VT;  // leaf: rank x size column-major
+  const double *ET;  // per son: rank_father x rank_son column-major
+  uint64_t eld;
+  double *WF, *WB;  // row-major / column-major blocks (either may be null)
+};
+
+// One wavefront per row i of a leaf l: the row of V_l, then of V_l E_l,
+// V_l E_l E_f, ... up to the root, written to every block of the leaf.
+__global__ __launch_bounds__(NT) void w_build(WBuildArgs a) {
+  __shared__ double row_all[WPB][MAXRANK];
+  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= a.ndof) return;
+  double *row = row_all[threadIdx.x >> 6];
+  const uint32_t l = a.rowleaf[w];
+  uint32_t cur = a.leaves[l];
+  const NodeD L = a.nodes[cur];
+  const uint32_t i = w - L.off, b = a.ibeg[l];
+  uint32_t k = L.rank;
+  for (uint32_t j = lane; j < k; j += 64)
+    row[j] = a.VT[L.voff + j + (uint64_t)i * k];
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t p = a.ibeg[l + 1] - b; p-- > 0;) {
+    const FlatItem it = a.items[b + p];
+    for (uint32_t j = lane; j < k; j += 64) {
+      if (a.WF) a.WF[it.woff + (uint64_t)i * k + j] = row[j];
+      if (a.WB) a.WB[it.woff + (uint64_t)j * L.size + i] = row[j];
+    }
+    if (p == 0) break;
+    const uint32_t f = a.nodes[cur].father;
+    const uint32_t kf = a.nodes[f].rank;
+    const double *ET = a.ET + (uint64_t)cur * a.eld;
+    double nx[MAXRANK / 64];
+#pragma unroll
+    for (int h = 0; h < MAXRANK / 64; h++) {
+      const uint32_t jf = lane + 64 * h;
+      double acc = 0.0;
+      if (jf < kf)
+        for (uint32_t t = 0; t < k; t++)
+          acc = fma(row[t], ET[jf + (uint64_t)t * kf], acc);
+      nx[h] = acc;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int h = 0; h < MAXRANK / 64; h++) {
+      const uint32_t jf = lane + 64 * h;
+      if (jf < kf) row[jf] = nx[h];
+    }
+    __builtin_amdgcn_wave_barrier();
+    k = kf;
+    cur = f;
+  }
+}
+
 // Couplings of interpolation H2-matrices on the device: S_b(i, j) =
 // kernel_const G(xi_i, eta_j) at the tensor Chebyshev points of the row
 // and column boxes of admissible block b (i = (a m + b) m + c over the
@@ -735,25 +1016,20 @@ __global__ __launch_bounds__(NT) void interp_couplings(CplArgs a) {
   const uint32_t m = a.m, k = m * m * m, kk = k * k;
   const double *pr = a.rpts + 3 * (uint64_t)m * f.r;
   const double *pc = a.cpts + 3 * (uint64_t)m * f.c;
-  constexpr double KCONST = 0.0795774715459476679;
   for (uint32_t e = threadIdx.x; e < kk; e += NT) {
     const uint32_t i = e % k, j = e / k;
     const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
     const uint32_t ja = j / (m * m), jb = (j / m) % m, jc = j % m;
-    const double d0 = pr[ia] - pc[ja], d1 = pr[m + ib] - pc[m + jb],
-                 d2 = pr[2 * m + ic] - pc[2 * m + jc];
-    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    double rs = 1.0 / sqrt(r2);
+    double sr, si;
+    cpl_value(pr[ia] - pc[ja], pr[m + ib] - pc[m + jb],
+              pr[2 * m + ic] - pc[2 * m + jc], a.helmholtz, a.kr, a.kdamp,
+              sr, si);
     const uint64_t o = f.soff + e;
     if (!a.helmholtz) {
-      a.S[o] = KCONST * rs;
+      a.S[o] = sr;
     } else {
-      const double r = r2 * rs;
-      if (a.kdamp != 0.0) rs *= exp(a.kdamp * r);
-      double sn, cs;
-      sincos(a.kr * r, &sn, &cs);
-      a.S[2 * o] = KCONST * (rs * cs);
-      a.S[2 * o + 1] = KCONST * (sn * rs);
+      a.S[2 * o] = sr;
+      a.S[2 * o + 1] = si;
     }
   }
 }
@@ -771,6 +1047,14 @@ struct h2hip_h2matrix {
     // height h at hnodes[hoff[h - 2] .. hoff[h - 1])
     std::vector<uint32_t> hnodes, hoff;
     uint64_t ncoef = 0, vlen = 0, eld = 0;
+    // flat transforms: one item per (leaf, ancestor), leaf by leaf
+    // (ancestors root first); W blocks of wlen doubles per orientation
+    std::vector<FlatItem> items;
+    std::vector<uint32_t> ibeg, rowleaf;
+    uint64_t wlen = 0;
+    bool flat_ok = true;  // the leaves partition the index positions
+    h2hip::DevBuf<FlatItem> d_items;
+    h2hip::DevBuf<double> d_WF, d_WB;
     h2hip::DevBuf<NodeD> d_nodes;
     h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sons, d_paths, d_plen, d_cnt,
         d_fwork, d_hnodes;
@@ -783,9 +1067,18 @@ struct h2hip_h2matrix {
   h2hip::DevBuf<FarD> d_far;
   h2hip::DevBuf<NearD> d_near;
   h2hip::DevBuf<double> d_S, d_N, d_xt, d_yt;
+  h2hip::DevBuf<double> d_xtf, d_ytf;  // coefficients of the flat products
+  // interpolation couplings (h2hip_h2matrix_interp_couplings): the points
+  // and the kernel, kept for products that compute the couplings
+  h2hip::DevBuf<double> d_rpts, d_cpts;
+  CplSrc cpl = {};
+  bool cpl_known = false;
+  int cpl_recompute = 0;
   uint32_t maxleaf = 0, maxrank = 0;  // over both sides
-  // upward sweep: 0 one launch with counters (the last son computes its
-  // father), 1 one launch per height above the leaves and twigs
+  // schedule of the transforms: 0 one upward launch with counters (the
+  // last son computes its father), 1 one upward launch per height above
+  // the leaves and twigs, 2 flat transforms (products y += alpha H x of one
+  // vector; the others use 1)
   int level_sync = 0;
   // checksums of the bases as set, per side (the self-adjoint product
   // requires equal row and column bases)
@@ -926,6 +1219,27 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       if (height[i] == h) S.hnodes.push_back(i);
     S.hoff.push_back((uint32_t)S.hnodes.size());
   }
+  // flat-transform items: the ancestors of every leaf, root first
+  S.items.clear();
+  S.ibeg.assign(1, 0);
+  S.rowleaf.assign(ndof, NONE);
+  uint64_t w = 0;
+  for (size_t l = 0; l < S.leaves.size(); l++) {
+    const NodeD &L = S.nodes[S.leaves[l]];
+    for (uint32_t k = 0; k < S.plen[l]; k++) {
+      const uint32_t a = S.paths[l * MAXPATH + k];
+      S.items.push_back(FlatItem{S.leaves[l], a, w});
+      w += (uint64_t)L.size * S.nodes[a].rank;
+    }
+    S.ibeg.push_back((uint32_t)S.items.size());
+    for (uint32_t i = 0; i < L.size; i++) {
+      if (S.rowleaf[L.off + i] != NONE) S.flat_ok = false;  // overlap
+      S.rowleaf[L.off + i] = (uint32_t)l;
+    }
+  }
+  for (uint32_t i = 0; i < ndof; i++)
+    if (S.rowleaf[i] == NONE) S.flat_ok = false;  // not covered
+  S.wlen = w;
   S.ncoef = coef;
   S.vlen = v;
   S.eld = maxr * maxr;
@@ -938,7 +1252,8 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       S.d_plen.upload(S.plen.data(), S.plen.size()) != hipSuccess ||
       S.d_cnt.upload(zero.data(), nnodes) != hipSuccess ||
       S.d_fwork.upload(S.fwork.data(), S.fwork.size()) != hipSuccess ||
-      S.d_hnodes.upload(S.hnodes.data(), S.hnodes.size()) != hipSuccess) {
+      S.d_hnodes.upload(S.hnodes.data(), S.hnodes.size()) != hipSuccess ||
+      S.d_items.upload(S.items.data(), S.items.size()) != hipSuccess) {
     set_error("h2matrix: device allocation failed");
     return H2HIP_ENOMEM;
   }
@@ -950,6 +1265,29 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
             hipStream_t st, bool trans, bool symm = false) {
   const auto &in = H->side[trans ? 0 : 1];
   const auto &out = H->side[trans ? 1 : 0];
+  const bool flat = NV == 1 && !trans && H->level_sync == 2 &&
+                    in.d_WF.p && out.d_WB.p && H->d_xtf.p;
+  FlatArgs fa;
+  if (flat) {
+    // coefficient buffers of the flat products only: xt is zero on entry
+    // (cleared by the previous flat product's downward launch, or at
+    // allocation); the upward launch clears yt before the couplings add
+    // into it
+    xt = H->d_xtf.p;
+    yt = H->d_ytf.p;
+    fa.nodes = in.d_nodes.p;
+    fa.items = in.d_items.p;
+    fa.nitems = (uint32_t)in.items.size();
+    fa.idx = in.d_idx.p;
+    fa.W = in.d_WF.p;
+    fa.xt = xt;
+    fa.yt = nullptr;
+    fa.zero = yt;
+    fa.nzero = (C ? 2 : 1) * out.ncoef;
+    fa.v = vs;
+    hipLaunchKernelGGL((fwd_flat<C, K>), dim3((fa.nitems + WPB - 1) / WPB),
+                       dim3(NT), 0, st, fa);
+  }
   FwdArgs f;
   f.nodes = in.d_nodes.p;
   f.work = in.d_fwork.p;
@@ -966,8 +1304,9 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   f.nyt = (C ? 2 : 1) * out.ncoef;
   f.v = vs;
   f.climb = H->level_sync ? 0 : 1;
-  hipLaunchKernelGGL((fwd_up<C, K, NV>), dim3(f.nwork), dim3(NT), 0, st, f);
-  if (H->level_sync) {
+  if (!flat)
+    hipLaunchKernelGGL((fwd_up<C, K, NV>), dim3(f.nwork), dim3(NT), 0, st, f);
+  if (H->level_sync && !flat) {
     for (size_t h = 0; h + 1 < in.hoff.size(); h++) {
       const uint32_t cnt = in.hoff[h + 1] - in.hoff[h];
       if (cnt)
@@ -997,14 +1336,32 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
       hipLaunchKernelGGL((couple_symm<C>), dim3((nw + WPB - 1) / WPB),
                          dim3(NT), 0, st, c);
   } else if (nw) {
+    c.cs = H->cpl;
     if (trans)
       hipLaunchKernelGGL((couple_near<C, true, NV>),
+                         dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
+    else if (NV == 1 && H->cpl_recompute)
+      hipLaunchKernelGGL((couple_near<C, false, NV, NV == 1>),
                          dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
     else
       hipLaunchKernelGGL((couple_near<C, false, NV>),
                          dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
   }
 
+  if (flat) {
+    fa.nodes = out.d_nodes.p;
+    fa.items = out.d_items.p;
+    fa.nitems = (uint32_t)out.items.size();
+    fa.idx = out.d_idx.p;
+    fa.W = out.d_WB.p;
+    fa.xt = nullptr;
+    fa.yt = yt;
+    fa.zero = xt;
+    fa.nzero = (C ? 2 : 1) * in.ncoef;
+    hipLaunchKernelGGL((back_flat<C, K>), dim3((fa.nitems + WPB - 1) / WPB),
+                       dim3(NT), 0, st, fa);
+    return;
+  }
   DBArgs d;
   d.nodes = out.d_nodes.p;
   d.leaves = out.d_leaves.p;
@@ -1110,6 +1467,70 @@ int apply_batch(uint32_t nvec, const double *alpha, const h2hip_h2matrix *H,
     launch_batch<C, 8>(vs, H, xt, yt, st, trans);
   H2HIP_CHECK(hipGetLastError());
   return H2HIP_OK;
+}
+
+// Builds the blocks W_a|_l of one side from its bases on the device:
+// row-major (upward sweep) and / or column-major (downward sweep).
+int build_flat(h2hip_h2matrix::Side &S, bool rowmajor, bool colmajor) {
+  if (!S.d_VT.p || !S.d_ET.p) {
+    set_error("h2matrix: flat transforms need the bases");
+    return H2HIP_EINVAL;
+  }
+  if (!S.flat_ok) {
+    set_error("h2matrix: flat transforms need leaves that partition the "
+              "index positions");
+    return H2HIP_EINVAL;
+  }
+  if (rowmajor && S.d_WF.alloc(S.wlen) != hipSuccess) return H2HIP_ENOMEM;
+  if (colmajor && S.d_WB.alloc(S.wlen) != hipSuccess) return H2HIP_ENOMEM;
+  h2hip::DevBuf<uint32_t> d_ibeg, d_rowleaf;
+  if (d_ibeg.upload(S.ibeg.data(), S.ibeg.size()) != hipSuccess ||
+      d_rowleaf.upload(S.rowleaf.data(), S.rowleaf.size()) != hipSuccess)
+    return H2HIP_ENOMEM;
+  WBuildArgs a;
+  a.nodes = S.d_nodes.p;
+  a.items = S.d_items.p;
+  a.ibeg = d_ibeg.p;
+  a.leaves = S.d_leaves.p;
+  a.rowleaf = d_rowleaf.p;
+  a.ndof = S.ndof;
+  a.VT = S.d_VT.p;
+  a.ET = S.d_ET.p;
+  a.eld = S.eld;
+  a.WF = rowmajor ? S.d_WF.p : nullptr;
+  a.WB = colmajor ? S.d_WB.p : nullptr;
+  hipLaunchKernelGGL(w_build, dim3((S.ndof + WPB - 1) / WPB), dim3(NT), 0,
+                     nullptr, a);
+  H2HIP_CHECK(hipGetLastError());
+  // the index buffers are freed on return
+  H2HIP_CHECK(hipStreamSynchronize(nullptr));
+  return H2HIP_OK;
+}
+
+// flat transforms of the products y += alpha H x: row-major blocks of the
+// column side, column-major blocks of the row side
+int build_flat_pair(h2hip_h2matrix *H) {
+  int rc = build_flat(H->side[1], true, false);
+  if (rc == H2HIP_OK) rc = build_flat(H->side[0], false, true);
+  if (rc == H2HIP_OK && !H->d_xtf.p) {
+    if (H->d_xtf.alloc(H->cstride) != hipSuccess ||
+        H->d_ytf.alloc(H->cstride) != hipSuccess ||
+        hipMemsetAsync(H->d_xtf.p, 0, sizeof(double) * H->cstride,
+                       nullptr) != hipSuccess ||
+        hipMemsetAsync(H->d_ytf.p, 0, sizeof(double) * H->cstride,
+                       nullptr) != hipSuccess ||
+        hipStreamSynchronize(nullptr) != hipSuccess)
+      rc = H2HIP_ENOMEM;
+  }
+  if (rc != H2HIP_OK) {
+    H->side[1].d_WF.release();
+    H->side[0].d_WB.release();
+    H->d_xtf.release();
+    H->d_ytf.release();
+    if (rc == H2HIP_ENOMEM) set_error("h2matrix: flat transforms: device "
+                                      "allocation failed");
+  }
+  return rc;
 }
 
 }  // namespace
@@ -1238,6 +1659,14 @@ int h2hip_h2matrix_set_basis(h2hip_h2matrix *H, int side, const double *V,
   H2HIP_CHECK(S.d_VT.upload(vt.data(), vt.size()));
   H2HIP_CHECK(S.d_E.upload(e.data(), e.size()));
   H2HIP_CHECK(S.d_ET.upload(et.data(), et.size()));
+  // new bases: rebuild the flat transforms' blocks
+  if (H->level_sync == 2 && H->side[0].d_VT.p && H->side[1].d_VT.p) {
+    const int rc = build_flat_pair(H);
+    if (rc != H2HIP_OK) {
+      H->level_sync = 1;
+      return rc;
+    }
+  }
   return H2HIP_OK;
 }
 
@@ -1255,6 +1684,23 @@ int h2hip_h2matrix_set_couplings(h2hip_h2matrix *H, const void *S,
   }
   H2HIP_CHECK(hipMemcpy(H->d_S.p, s.data(), sizeof(double) * s.size(),
                         hipMemcpyHostToDevice));
+  // couplings given as values: products read them
+  H->cpl_known = false;
+  H->cpl_recompute = 0;
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_set_coupling_mode(h2hip_h2matrix *H, int recompute) {
+  if (!H || recompute < 0 || recompute > 1) {
+    set_error("h2hip_h2matrix_set_coupling_mode: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  if (recompute && !H->cpl_known && H->nfar) {
+    set_error("h2hip_h2matrix_set_coupling_mode: the couplings were not "
+              "set by h2hip_h2matrix_interp_couplings");
+    return H2HIP_EINVAL;
+  }
+  H->cpl_recompute = recompute;
   return H2HIP_OK;
 }
 
@@ -1296,9 +1742,16 @@ int h2hip_addevalsymm_h2matrix(const double *alpha, const h2hip_h2matrix *H,
 }
 
 int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync) {
-  if (!H || level_sync < 0 || level_sync > 1) {
+  if (!H || level_sync < 0 || level_sync > 2) {
     set_error("h2hip_h2matrix_set_schedule: invalid arguments");
     return H2HIP_EINVAL;
+  }
+  if (level_sync == 2) {
+    const int rc = build_flat_pair(H);
+    if (rc != H2HIP_OK) return rc;
+  } else {
+    H->side[1].d_WF.release();
+    H->side[0].d_WB.release();
   }
   H->level_sync = level_sync;
   return H2HIP_OK;
@@ -1324,14 +1777,15 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
       }
   if (H->nfar == 0) return H2HIP_OK;
   const uint32_t rn = H->side[0].nnodes, cn = H->side[1].nnodes;
-  h2hip::DevBuf<double> d_rp, d_cp;
-  H2HIP_CHECK(d_rp.upload(rpts, 3 * (uint64_t)m * rn));
-  H2HIP_CHECK(d_cp.upload(cpts, 3 * (uint64_t)m * cn));
+  // the points stay on the device for products that compute the couplings
+  H->cpl_known = false;
+  H2HIP_CHECK(H->d_rpts.upload(rpts, 3 * (uint64_t)m * rn));
+  H2HIP_CHECK(H->d_cpts.upload(cpts, 3 * (uint64_t)m * cn));
   CplArgs a;
   a.far = H->d_far.p;
   a.nfar = H->nfar;
-  a.rpts = d_rp.p;
-  a.cpts = d_cp.p;
+  a.rpts = H->d_rpts.p;
+  a.cpts = H->d_cpts.p;
   a.m = m;
   a.helmholtz = kernel == H2HIP_HELMHOLTZ;
   a.kr = a.helmholtz ? k[0] : 0.0;
@@ -1340,8 +1794,9 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(interp_couplings, dim3(H->nfar), dim3(NT), 0, st, a);
   H2HIP_CHECK(hipGetLastError());
-  // the point buffers are freed on return: wait for the kernel
   H2HIP_CHECK(hipStreamSynchronize(st));
+  H->cpl = CplSrc{H->d_rpts.p, H->d_cpts.p, m, a.helmholtz, a.kr, a.kdamp};
+  H->cpl_known = true;
   return H2HIP_OK;
 }
 
@@ -1353,8 +1808,14 @@ int h2hip_h2matrix_bytes(const h2hip_h2matrix *H, uint64_t *out3) {
   if (!H || !out3) return H2HIP_EINVAL;
   const int cm = H->cplx ? 2 : 1;
   out3[0] = 8ull * cm * H->nlen;
-  out3[1] = 8ull * cm * H->slen;
+  // products that compute the couplings read none of them
+  out3[1] = H->cpl_recompute ? 0 : 8ull * cm * H->slen;
   uint64_t b = 0;
+  if (H->level_sync == 2 && H->side[1].d_WF.p && H->side[0].d_WB.p) {
+    // the flat transforms read the blocks W_a|_l of both sides
+    out3[2] = 8ull * (H->side[0].wlen + H->side[1].wlen);
+    return H2HIP_OK;
+  }
   for (int s = 0; s < 2; s++) {
     b += 8ull * H->side[s].vlen;
     for (uint32_t i = 0; i < H->side[s].nnodes; i++) {

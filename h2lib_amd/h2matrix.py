@@ -58,6 +58,8 @@ def _sigs():
     h.h2hip_addevalsymm_h2matrix.argtypes = [dp, vp, vp, vp, vp]
     h.h2hip_h2matrix_set_schedule.restype = C.c_int
     h.h2hip_h2matrix_set_schedule.argtypes = [vp, C.c_int]
+    h.h2hip_h2matrix_set_coupling_mode.restype = C.c_int
+    h.h2hip_h2matrix_set_coupling_mode.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_interp_couplings.restype = C.c_int
     h.h2hip_h2matrix_interp_couplings.argtypes = [vp, C.c_int, dp, u32, dp,
                                                   dp, vp]
@@ -268,11 +270,21 @@ class H2Matrix:
             check(hip().h2hip_stream_sync(None), "h2hip_stream_sync")
 
     def set_schedule(self, level_sync):
-        """Upward sweep in one launch with counters (0) or one launch per
-        tree height (1)."""
+        """Transforms: upward sweep in one launch with counters (0), one
+        launch per tree height (1), or flat transforms over the ancestors'
+        bases restricted to the leaves (2, single-vector products
+        y += alpha H x)."""
         check(hip().h2hip_h2matrix_set_schedule(self._h, int(level_sync)),
               "h2hip_h2matrix_set_schedule")
         self.level_sync = int(level_sync)
+
+    def set_coupling_mode(self, recompute):
+        """Products y += alpha H x read the stored couplings (0) or compute
+        them from the interpolation points (1; device couplings only)."""
+        check(hip().h2hip_h2matrix_set_coupling_mode(self._h,
+                                                     int(bool(recompute))),
+              "h2hip_h2matrix_set_coupling_mode")
+        self.recompute = int(bool(recompute))
 
     def bytes(self):
         out = np.zeros(3, dtype=np.uint64)

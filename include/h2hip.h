@@ -202,13 +202,33 @@ void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H);
  * interpolation bases of a single layer operator are). */
 int h2hip_addevalsymm_h2matrix(const double *alpha, const h2hip_h2matrix *H,
                                const void *x, void *y, void *stream);
-/* Schedule of the upward sweep of the products (results agree to the
- * rounding of the atomic sums): 0 one launch in which the last son to
- * finish computes its father (counters and fences), 1 one launch for the
- * leaves and the clusters whose sons are all leaves, then one launch per
- * height above them.  Not thread-safe against products in flight.  No
- * counterpart in the reference. */
+/* Schedule of the basis transforms of the products (results agree to the
+ * rounding of the atomic sums): 0 one upward launch in which the last son
+ * to finish computes its father (counters and fences), 1 one upward launch
+ * for the leaves and the clusters whose sons are all leaves, then one
+ * launch per height above them; the downward sweep walks each leaf's path
+ * from the root.  2 flat transforms for the products y += alpha H x of one
+ * vector (h2hip_addeval_h2matrix, h2hip_addevalsymm_h2matrix): the bases
+ * of all ancestors restricted to each leaf (W_a|_l = V_l E_l E_f ...) are
+ * built once on the device (row-major for the column side, column-major
+ * for the row side; h2hip_h2matrix_bytes reports them as the bases), and
+ * both sweeps become one launch without dependence chains; the other
+ * products use schedule 1.  Needs the bases set and leaves that partition
+ * the index positions; setting new bases rebuilds the blocks.  Not
+ * thread-safe against products in flight.  No counterpart in the
+ * reference. */
 int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync);
+/* Couplings of the products y += alpha H x (h2hip_addeval_h2matrix, one
+ * vector): recompute = 0 reads the stored couplings, 1 computes each entry
+ * in the product from the interpolation points kept by
+ * h2hip_h2matrix_interp_couplings (the same arithmetic, so the same
+ * values), which trades the product's largest read (k^2 doubles per
+ * admissible block) for k^2 kernel evaluations.  The other products keep
+ * reading the stored couplings.  Needs couplings set by
+ * h2hip_h2matrix_interp_couplings; h2hip_h2matrix_set_couplings switches
+ * back to 0.  h2hip_h2matrix_bytes then counts no coupling bytes.  No
+ * counterpart in the reference. */
+int h2hip_h2matrix_set_coupling_mode(h2hip_h2matrix *H, int recompute);
 int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
                                     const double *k, uint32_t m,
                                     const double *rpts, const double *cpts,

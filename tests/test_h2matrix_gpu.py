@@ -207,6 +207,113 @@ def test_level_synchronous_sweep_equals_counters(kern, basis):
                                                                     kind)
 
 
+@pytest.mark.parametrize("kern,layer,basis,m", [(0, 0, 1, 3), (0, 1, 1, 4),
+                                                (1, 0, 0, 4), (1, 1, 0, 3),
+                                                (0, 0, 0, 5)])
+def test_flat_transforms_equal_counters(kern, layer, basis, m):
+    # schedule 2 (explicit bases of the ancestors restricted to the leaves)
+    # computes the same products as the counter climb, to the rounding of
+    # the atomic sums; the products it does not cover (adjoint, batched)
+    # keep working through the level-synchronous sweep
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    if kern == 0:
+        op = (h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis) if layer == 0
+              else h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5))
+    else:
+        op = (h.new_slp_helmholtz_bem3d(2.0, gr, 2, 4, basis, basis)
+              if layer == 0 else
+              h.new_dlp_helmholtz_bem3d(2.0, gr, 2, 4, basis, basis, -0.5))
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, m)
+    rng = np.random.default_rng(11)
+    n = len(rt.idx)
+    xs = [rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if kern else 0)
+          for _ in range(2)]
+    res = {}
+    for sched in (0, 2):
+        H.set_schedule(sched)
+        for trans in (False, True):
+            res[sched, trans] = product(H, xs[0], trans=trans,
+                                        alpha=0.75 - (0.5j if kern else 0))
+        if m ** 3 <= 64:  # batched products: ranks of at most 64
+            d_x = [h.DeviceArray.from_host(x) for x in xs]
+            d_y = [h.DeviceArray.from_host(np.zeros_like(x)) for x in xs]
+            H.addeval_batch(np.ones(2), d_x, d_y)
+            res[sched, "batch"] = np.concatenate([d.download() for d in d_y])
+    keys = ((False,), (True,)) + ((("batch",),) if m ** 3 <= 64 else ())
+    for key in keys:
+        a, b = res[(2,) + key], res[(0,) + key]
+        assert np.abs(a - b).max() < 1e-13 * np.abs(b).max(), key
+
+
+@pytest.mark.parametrize("kern,layer,basis,m,k", [(0, 0, 1, 4, 0.0),
+                                                  (0, 1, 1, 3, 0.0),
+                                                  (1, 0, 0, 4, 2.0),
+                                                  (1, 1, 0, 3, 3.0 + 0.5j)])
+def test_computed_couplings_equal_stored(kern, layer, basis, m, k):
+    # products that compute the couplings from the interpolation points
+    # equal those that read them (same arithmetic for every entry), with
+    # both transform schedules; the adjoint keeps reading them
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    if kern == 0:
+        op = (h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis) if layer == 0
+              else h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5))
+    else:
+        op = (h.new_slp_helmholtz_bem3d(k, gr, 2, 4, basis, basis)
+              if layer == 0 else
+              h.new_dlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, 0.5))
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, m)
+    rng = np.random.default_rng(5)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if kern else 0)
+    alpha = 0.5 + (0.25j if kern else 0)
+    want = product(H, x, alpha=alpha)
+    want_t = product(H, x, trans=True)
+    nb = H.bytes()
+    H.set_coupling_mode(1)
+    assert H.bytes()["coupling"] == 0 and nb["coupling"] > 0
+    for sched in (0, 2):
+        H.set_schedule(sched)
+        got = product(H, x, alpha=alpha)
+        assert np.abs(got - want).max() < 1e-13 * np.abs(want).max(), sched
+        got_t = product(H, x, trans=True)
+        assert np.abs(got_t - want_t).max() < 1e-13 * np.abs(want_t).max()
+    H.set_coupling_mode(0)
+    assert H.bytes()["coupling"] == nb["coupling"]
+
+
+def test_flat_transforms_symmetric_storage():
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = h.new_slp_laplace_bem3d(gr, 2, 4, 1, 1)
+    rt = G.ClusterTree(mesh, 1, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, 4, symmetric=True)
+    x = np.random.default_rng(2).uniform(-1, 1, len(rt.idx))
+    out = {}
+    for sched in (0, 2):
+        H.set_schedule(sched)
+        d_x = h.DeviceArray.from_host(x)
+        d_y = h.DeviceArray.from_host(np.zeros_like(x))
+        H.addevalsymm(1.0, d_x, d_y)
+        out[sched] = d_y.download()
+    assert np.abs(out[2] - out[0]).max() < 1e-13 * np.abs(out[0]).max()
+
+
 @pytest.mark.skipif(not have_reference(False), reason="reference not built")
 @pytest.mark.parametrize("basis,clf,m", [(1, 16, 3), (0, 24, 3), (1, 32, 5)])
 def test_symmetric_product_matches_reference(basis, clf, m):
