@@ -59,8 +59,14 @@ def footprint(mesh, kind, clf, m, slots, kmax, eta=2.0):
     near = int((nb[:, 1] * nb[:, 3]).sum())
     k = m ** 3
     n = tree.n
+    leaf = tree.sons == 0
+    # the flat transforms' blocks: each leaf's rows of all its ancestors'
+    # bases, one orientation per side
+    flat = 2 * 8 * k * int((tree.size[leaf].astype(np.int64) *
+                            (tree.level[leaf].astype(np.int64) + 1)).sum())
     op = es * near + es * len(bt.far) * k * k + 8 * (4 * n * k +
                                                        4 * tree.nnodes * k * k)
+    op += flat
     gm = slots * es * (n * (kmax + 2))
     return 2 * op + gm
 

@@ -391,7 +391,10 @@ class GmresMulti:
 class LaplaceDirichlet:
     """Interior Dirichlet problem on a mesh (piecewise linear basis)."""
 
-    def __init__(self, mesh, clf=64, eta=2.0, m=4, q=2, q2=4):
+    def __init__(self, mesh, clf=64, eta=2.0, m=4, q=2, q2=4, schedule=2):
+        """schedule: transforms of the H2 products (H2Matrix.set_schedule;
+        2, the flat transforms, is the fastest and holds the ancestors'
+        bases restricted to the leaves on the device)."""
         _sigs()
         self.mesh = mesh
         self.q = q
@@ -405,6 +408,8 @@ class LaplaceDirichlet:
                           m)
         self.KM = H2Matrix(self.dlp, mesh, self.tree, self.tree, self.blocks,
                            m)
+        for H in (self.V, self.KM):
+            H.set_schedule(schedule)
         self.n = mesh.vertices
 
     def project(self, x0, kind=0):
@@ -487,7 +492,7 @@ class HelmholtzExterior:
     precision 1 assembles both near fields in mixed precision."""
 
     def __init__(self, mesh, k, clf=64, eta=2.0, m=4, q=2, q2=4,
-                 precision=0, stream=None):
+                 precision=0, stream=None, schedule=2):
         from .bem3d import new_dlp_helmholtz_bem3d, new_slp_helmholtz_bem3d
         _sigs()
         self.mesh, self.k, self.q = mesh, k, q
@@ -503,6 +508,10 @@ class HelmholtzExterior:
                           stream=stream)
         self.K = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
                           stream=stream)
+        if stream is not None:
+            stream.sync()
+        for H in (self.V, self.K):
+            H.set_schedule(schedule)
         self.n = mesh.triangles
 
     def rhs(self, x0, stream=None):
@@ -568,7 +577,7 @@ class HelmholtzBrakhageWerner:
     precision 1 assembles both near fields in mixed precision."""
 
     def __init__(self, mesh, k, clf=32, eta_adm=1.0, m=5, q=2, q2=4,
-                 precision=0, stream=None, basis=0):
+                 precision=0, stream=None, basis=0, schedule=2):
         from .bem3d import new_dlp_helmholtz_bem3d, new_slp_helmholtz_bem3d
         _sigs()
         self.mesh, self.k, self.q, self.basis = mesh, k, q, basis
@@ -586,6 +595,10 @@ class HelmholtzBrakhageWerner:
                           stream=stream)
         self.KM = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
                            stream=stream)
+        if stream is not None:
+            stream.sync()
+        for H in (self.V, self.KM):
+            H.set_schedule(schedule)
         self.A = Combined(self.KM, 1.0, self.V, -1j * self.eta)
         self.n = mesh.vertices if basis else mesh.triangles
 

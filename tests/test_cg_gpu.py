@@ -13,8 +13,9 @@ h2hip_solve_cg against the reference's solve_cg_avector
   single layer matrix: the iterates after 10 steps from a start vector
   against the reference's to 1e-9;
 * the stopping rule: maxiter - 1 steps, no step from a converged start;
-* graph replay against direct launches (same steps within 1, solutions to
-  1e-8), also with the self-adjoint product of a matrix kept in symmetric
+* graph replay against direct launches (same steps within max(2, 2%):
+  the products' atomic sums round differently from run to run; solutions
+  to 1e-8), also with the self-adjoint product of a matrix kept in symmetric
   storage (exactly symmetric, unlike the full matrix whose near field is
   symmetric only to quadrature accuracy).
 """
@@ -147,8 +148,10 @@ def test_symmetric_storage_and_graph():
             assert rel < 1.01e-10
             out[symm, g] = (it, d_x.download())
     for symm in (False, True):
+        # the products sum with atomics, so the rounding and with it the
+        # last steps before the stopping test vary between runs
         (it0, x0), (it1, x1) = out[symm, False], out[symm, True]
-        assert abs(it1 - it0) <= 1
+        assert abs(it1 - it0) <= max(2, it0 // 50)
         assert np.abs(x1 - x0).max() < 1e-8 * np.abs(x0).max()
     # the two storages hold different matrices: the full one's near field is
     # symmetric only to quadrature accuracy, the symmetric one exactly, and
