@@ -532,7 +532,8 @@ def bench_dirichlet(h, mesh, args, stream):
             d_c.zero(stream)
             t = time.perf_counter()
             if use_cg:
-                it, _ = solve_cg(H, d_b, d_c, args.eps, 1000, True, stream)
+                it, _ = solve_cg(H, d_b, d_c, args.eps, 1000,
+                                 not args.cg_direct, stream)
             else:
                 it, _ = solve_gmres(H, d_b, d_c, args.eps, 1000, 30, stream)
             dt = time.perf_counter() - t
@@ -558,7 +559,10 @@ def bench_dirichlet(h, mesh, args, stream):
         "neumann_rel_err": prob.neumann_error(x, x0),
         "operator_build_s": build_s,
         "matvec": {
-            "kernels": "fwd_up + couple_near + down_back (Laplace SLP)",
+            "kernels": ("fwd_flat + couple_near + back_flat (Laplace SLP)"
+                        if best == 2 else
+                        "fwd_up (+ fwd_level) + couple_near + down_back "
+                        "(Laplace SLP)"),
             "ms": mv_ms, "bytes": int(mv_bytes), "GB/s": mv_gbs,
             "hbm_frac": mv_gbs / 8000.0,
             "ms_by_schedule": {"counters": mv_by[0], "level_sync": mv_by[1],
@@ -692,6 +696,9 @@ def main():
     ap.add_argument("--multi-check", type=int, default=6,
                     help="requests whose solutions are checked against a "
                     "tight solve")
+    ap.add_argument("--cg-direct", action="store_true",
+                    help="CG timings with direct launches instead of "
+                         "graph-replayed chunks")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
