@@ -828,32 +828,6 @@ struct FlatItem {
   uint64_t woff;        // offset of the block W_a|_l
 };
 
-// acc += sum_i M[i * ld] xs[i] over n inputs, 16 loads issued before their
-// products (the sweeps are bound by the loads in flight per wave)
-template <int C>
-__device__ __forceinline__ void dot_strided(const double *__restrict__ M,
-                                            uint64_t ld, uint32_t n,
-                                            const double *xs, double &ar,
-                                            double &ai) {
-  constexpr int B = 16;
-  uint32_t i = 0;
-  for (; i + B <= n; i += B) {
-    double m[B];
-#pragma unroll
-    for (int u = 0; u < B; u++) m[u] = M[(uint64_t)(i + u) * ld];
-#pragma unroll
-    for (int u = 0; u < B; u++) {
-      ar = fma(m[u], xs[2 * (i + u)], ar);
-      if constexpr (C) ai = fma(m[u], xs[2 * (i + u) + 1], ai);
-    }
-  }
-  for (; i < n; i++) {
-    const double m = M[(uint64_t)i * ld];
-    ar = fma(m, xs[2 * i], ar);
-    if constexpr (C) ai = fma(m, xs[2 * i + 1], ai);
-  }
-}
-
 struct FlatArgs {
   const NodeD *nodes;
   const FlatItem *items;
@@ -862,22 +836,61 @@ struct FlatArgs {
   const double *W;
   double *xt;        // upward: coefficients (atomics)
   const double *yt;  // downward: coefficients
+  uint64_t cstride;  // doubles between the vectors' coefficients
   double *zero;      // cleared by this launch (the buffer the next launch
-  uint64_t nzero;    // accumulates into / the next product starts from)
+  uint64_t nzero;    // accumulates into / the next product starts from),
+                     // nzero doubles per vector in use, cstride apart
   VecSet v;
 };
 
-// a slice of a.zero per workgroup
+// a slice of a.zero per workgroup, for every vector in use
 __device__ __forceinline__ void flat_zero(const FlatArgs &a) {
   const uint64_t chunk = (a.nzero + gridDim.x - 1) / gridDim.x;
   const uint64_t e0 = (uint64_t)blockIdx.x * chunk,
                  e1 = min(a.nzero, e0 + chunk);
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += NT) a.zero[e] = 0.0;
+  for (uint32_t v = 0; v < a.v.nv; v++)
+    for (uint64_t e = e0 + threadIdx.x; e < e1; e += NT)
+      a.zero[v * a.cstride + e] = 0.0;
 }
 
-template <int C, int K>
+// acc_v += sum_i M[i * stride] xs_v[i] for NV LDS vectors (XR entries
+// apart, complex pairs), 16 matrix entries loaded before their products
+template <int C, int NV, int XR>
+__device__ __forceinline__ void dot_strided_nv(const double *__restrict__ M,
+                                               uint64_t stride, uint32_t n,
+                                               const double *xs, double *ar,
+                                               double *ai) {
+  constexpr int B = 16;
+  uint32_t i = 0;
+  for (; i + B <= n; i += B) {
+    double m[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) m[u] = M[(uint64_t)(i + u) * stride];
+#pragma unroll
+    for (int u = 0; u < B; u++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        ar[v] = fma(m[u], xs[v * 2 * XR + 2 * (i + u)], ar[v]);
+        if constexpr (C) ai[v] = fma(m[u], xs[v * 2 * XR + 2 * (i + u) + 1], ai[v]);
+      }
+  }
+  for (; i < n; i++) {
+    const double m = M[(uint64_t)i * stride];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      ar[v] = fma(m, xs[v * 2 * XR + 2 * i], ar[v]);
+      if constexpr (C) ai[v] = fma(m, xs[v * 2 * XR + 2 * i + 1], ai[v]);
+    }
+  }
+}
+
+// Upward sweep, one wave per (leaf, ancestor) block, NV vectors at once
+// (leaves and ranks <= 64 when NV > 1): x of the leaf gathered into LDS,
+// W read once for all vectors, atomics into the coefficients.
+template <int C, int K, int NV>
 __global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
-  __shared__ double xs_all[WPB][2 * MAXRANK];
+  constexpr int XR = NV == 1 ? MAXRANK : 64;
+  __shared__ double xs_all[WPB][NV * 2 * XR];
   flat_zero(a);  // the coupling accumulators yt
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -887,25 +900,34 @@ __global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
   const NodeD L = a.nodes[it.leaf];
   const NodeD A = a.nodes[it.node];
   const uint32_t sz = L.size, k = K ? K : A.rank;
-  for (uint32_t i = lane; i < sz; i += 64) {
-    double xr, xi;
-    ld<C>(a.v.x[0], a.idx[L.off + i], xr, xi);
-    xs[2 * i] = xr;
-    xs[2 * i + 1] = xi;
+  for (uint32_t e = lane; e < NV * sz; e += 64) {
+    const uint32_t v = e / sz, i = e - v * sz;
+    double xr = 0.0, xi = 0.0;
+    if (v < a.v.nv) ld<C>(a.v.x[v], a.idx[L.off + i], xr, xi);
+    xs[v * 2 * XR + 2 * i] = xr;
+    xs[v * 2 * XR + 2 * i + 1] = xi;
   }
   // the wave reads what its own lanes wrote: LDS keeps the order of one
   // wave's accesses, the compiler must not move them
   __builtin_amdgcn_wave_barrier();
   for (uint32_t j = lane; j < k; j += 64) {
-    double ar = 0.0, ai = 0.0;
-    dot_strided<C>(a.W + it.woff + j, k, sz, xs, ar, ai);
-    atomic_add<C>(a.xt, A.coef + j, ar, ai);
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+    dot_strided_nv<C, NV, XR>(a.W + it.woff + j, k, sz, xs, ar, ai);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      if (v < (int)a.v.nv)
+        atomic_add<C>(a.xt + v * a.cstride, A.coef + j, ar[v], ai[v]);
   }
 }
 
-template <int C, int K>
+// Downward sweep, one wave per (leaf, ancestor) block, NV vectors at once:
+// yt of the ancestor in LDS, one row per lane, atomics into y.
+template <int C, int K, int NV>
 __global__ __launch_bounds__(NT) void back_flat(FlatArgs a) {
-  __shared__ double ys_all[WPB][2 * MAXRANK];
+  constexpr int XR = NV == 1 ? MAXRANK : 64;
+  __shared__ double ys_all[WPB][NV * 2 * XR];
   flat_zero(a);  // the input coefficients xt, for the next product
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -915,19 +937,25 @@ __global__ __launch_bounds__(NT) void back_flat(FlatArgs a) {
   const NodeD L = a.nodes[it.leaf];
   const NodeD A = a.nodes[it.node];
   const uint32_t sz = L.size, k = K ? K : A.rank;
-  for (uint32_t j = lane; j < k; j += 64) {
-    double yr, yi;
-    ld<C>(a.yt, A.coef + j, yr, yi);
-    ys[2 * j] = yr;
-    ys[2 * j + 1] = yi;
+  for (uint32_t e = lane; e < NV * k; e += 64) {
+    const uint32_t v = e / k, j = e - v * k;
+    double yr = 0.0, yi = 0.0;
+    if (v < a.v.nv) ld<C>(a.yt + v * a.cstride, A.coef + j, yr, yi);
+    ys[v * 2 * XR + 2 * j] = yr;
+    ys[v * 2 * XR + 2 * j + 1] = yi;
   }
   __builtin_amdgcn_wave_barrier();
   for (uint32_t i = lane; i < sz; i += 64) {
-    double ar = 0.0, ai = 0.0;
-    dot_strided<C>(a.W + it.woff + i, sz, k, ys, ar, ai);
-    atomic_add<C>(a.v.y[0], a.idx[L.off + i],
-                  a.v.are[0] * ar - a.v.aim[0] * ai,
-                  a.v.are[0] * ai + a.v.aim[0] * ar);
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+    dot_strided_nv<C, NV, XR>(a.W + it.woff + i, sz, k, ys, ar, ai);
+    const uint32_t d = a.idx[L.off + i];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      if (v < (int)a.v.nv)
+        atomic_add<C>(a.v.y[v], d, a.v.are[v] * ar[v] - a.v.aim[v] * ai[v],
+                      a.v.are[v] * ai[v] + a.v.aim[v] * ar[v]);
   }
 }
 
@@ -1077,8 +1105,8 @@ struct h2hip_h2matrix {
   uint32_t maxleaf = 0, maxrank = 0;  // over both sides
   // schedule of the transforms: 0 one upward launch with counters (the
   // last son computes its father), 1 one upward launch per height above
-  // the leaves and twigs, 2 flat transforms (products y += alpha H x of one
-  // vector; the others use 1)
+  // the leaves and twigs, 2 flat transforms (products y += alpha H x of
+  // one to eight vectors; the adjoint uses 1)
   int level_sync = 0;
   // checksums of the bases as set, per side (the self-adjoint product
   // requires equal row and column bases)
@@ -1265,8 +1293,8 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
             hipStream_t st, bool trans, bool symm = false) {
   const auto &in = H->side[trans ? 0 : 1];
   const auto &out = H->side[trans ? 1 : 0];
-  const bool flat = NV == 1 && !trans && H->level_sync == 2 &&
-                    in.d_WF.p && out.d_WB.p && H->d_xtf.p;
+  const bool flat = !trans && H->level_sync == 2 && in.d_WF.p &&
+                    out.d_WB.p && H->d_xtf.p;
   FlatArgs fa;
   if (flat) {
     // coefficient buffers of the flat products only: xt is zero on entry
@@ -1275,6 +1303,7 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     // into it
     xt = H->d_xtf.p;
     yt = H->d_ytf.p;
+    fa.cstride = H->cstride;
     fa.nodes = in.d_nodes.p;
     fa.items = in.d_items.p;
     fa.nitems = (uint32_t)in.items.size();
@@ -1285,8 +1314,9 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     fa.zero = yt;
     fa.nzero = (C ? 2 : 1) * out.ncoef;
     fa.v = vs;
-    hipLaunchKernelGGL((fwd_flat<C, K>), dim3((fa.nitems + WPB - 1) / WPB),
-                       dim3(NT), 0, st, fa);
+    hipLaunchKernelGGL((fwd_flat<C, K, NV>),
+                       dim3((fa.nitems + WPB - 1) / WPB), dim3(NT), 0, st,
+                       fa);
   }
   FwdArgs f;
   f.nodes = in.d_nodes.p;
@@ -1358,8 +1388,9 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     fa.yt = yt;
     fa.zero = xt;
     fa.nzero = (C ? 2 : 1) * in.ncoef;
-    hipLaunchKernelGGL((back_flat<C, K>), dim3((fa.nitems + WPB - 1) / WPB),
-                       dim3(NT), 0, st, fa);
+    hipLaunchKernelGGL((back_flat<C, K, NV>),
+                       dim3((fa.nitems + WPB - 1) / WPB), dim3(NT), 0, st,
+                       fa);
     return;
   }
   DBArgs d;
@@ -1513,12 +1544,14 @@ int build_flat_pair(h2hip_h2matrix *H) {
   int rc = build_flat(H->side[1], true, false);
   if (rc == H2HIP_OK) rc = build_flat(H->side[0], false, true);
   if (rc == H2HIP_OK && !H->d_xtf.p) {
-    if (H->d_xtf.alloc(H->cstride) != hipSuccess ||
-        H->d_ytf.alloc(H->cstride) != hipSuccess ||
-        hipMemsetAsync(H->d_xtf.p, 0, sizeof(double) * H->cstride,
-                       nullptr) != hipSuccess ||
-        hipMemsetAsync(H->d_ytf.p, 0, sizeof(double) * H->cstride,
-                       nullptr) != hipSuccess ||
+    // coefficients of up to MAXNV vectors (batched products)
+    const uint64_t len = (uint64_t)MAXNV * H->cstride;
+    if (H->d_xtf.alloc(len) != hipSuccess ||
+        H->d_ytf.alloc(len) != hipSuccess ||
+        hipMemsetAsync(H->d_xtf.p, 0, sizeof(double) * len, nullptr) !=
+            hipSuccess ||
+        hipMemsetAsync(H->d_ytf.p, 0, sizeof(double) * len, nullptr) !=
+            hipSuccess ||
         hipStreamSynchronize(nullptr) != hipSuccess)
       rc = H2HIP_ENOMEM;
   }

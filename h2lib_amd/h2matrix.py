@@ -272,8 +272,8 @@ class H2Matrix:
     def set_schedule(self, level_sync):
         """Transforms: upward sweep in one launch with counters (0), one
         launch per tree height (1), or flat transforms over the ancestors'
-        bases restricted to the leaves (2, single-vector products
-        y += alpha H x)."""
+        bases restricted to the leaves (2, products y += alpha H x of one or
+        several vectors; the adjoint uses 1)."""
         check(hip().h2hip_h2matrix_set_schedule(self._h, int(level_sync)),
               "h2hip_h2matrix_set_schedule")
         self.level_sync = int(level_sync)

@@ -207,12 +207,13 @@ int h2hip_addevalsymm_h2matrix(const double *alpha, const h2hip_h2matrix *H,
  * to finish computes its father (counters and fences), 1 one upward launch
  * for the leaves and the clusters whose sons are all leaves, then one
  * launch per height above them; the downward sweep walks each leaf's path
- * from the root.  2 flat transforms for the products y += alpha H x of one
- * vector (h2hip_addeval_h2matrix, h2hip_addevalsymm_h2matrix): the bases
+ * from the root.  2 flat transforms for the products y += alpha H x
+ * (h2hip_addeval_h2matrix, h2hip_addevalsymm_h2matrix and the batched
+ * h2hip_addeval_h2matrix_batch without trans): the bases
  * of all ancestors restricted to each leaf (W_a|_l = V_l E_l E_f ...) are
  * built once on the device (row-major for the column side, column-major
  * for the row side; h2hip_h2matrix_bytes reports them as the bases), and
- * both sweeps become one launch without dependence chains; the other
+ * both sweeps become one launch without dependence chains; the adjoint
  * products use schedule 1.  Needs the bases set and leaves that partition
  * the index positions; setting new bases rebuilds the blocks.  Not
  * thread-safe against products in flight.  No counterpart in the
