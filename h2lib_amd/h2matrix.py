@@ -150,11 +150,14 @@ class H2Matrix:
     Library/h2matrix.h:48-71)."""
 
     def __init__(self, op, mesh, rt, ct, btree, m, build_nearfield=True,
-                 stream=None, device_couplings=True, symmetric=False):
+                 stream=None, device_couplings=True, symmetric=False,
+                 nearplan=None):
         """symmetric: keep only the blocks on and below the block diagonal
         (half the storage) for addevalsymm; the operator must be
         self-adjoint with equal row and column bases (a Laplace single
-        layer operator)."""
+        layer operator).  nearplan: the near-field plan of another
+        H2Matrix on the same surface, trees, blocks and bases (the plan
+        does not depend on the kernel), reused instead of planned again."""
         _sigs()
         if symmetric and (rt is not ct or op.row_basis != op.col_basis or
                           op.layer == DLP or op.kernel != LAPLACE):
@@ -209,8 +212,12 @@ class H2Matrix:
                   "h2hip_h2matrix_set_couplings")
         self.nearplan = None
         if build_nearfield:
-            self.nearplan = NearfieldPlan(op.gr, op.row_basis, op.col_basis,
-                                          rt.idx, ct.idx, nbl)
+            if nearplan is not None and not symmetric:
+                self.nearplan = nearplan
+            else:
+                self.nearplan = NearfieldPlan(op.gr, op.row_basis,
+                                              op.col_basis, rt.idx, ct.idx,
+                                              nbl)
             self.assemble_nearfield(stream)
 
     @classmethod

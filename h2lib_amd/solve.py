@@ -407,7 +407,7 @@ class LaplaceDirichlet:
         self.V = H2Matrix(self.slp, mesh, self.tree, self.tree, self.blocks,
                           m)
         self.KM = H2Matrix(self.dlp, mesh, self.tree, self.tree, self.blocks,
-                           m)
+                           m, nearplan=self.V.nearplan)
         for H in (self.V, self.KM):
             H.set_schedule(schedule)
         self.n = mesh.vertices
@@ -507,7 +507,7 @@ class HelmholtzExterior:
         self.V = H2Matrix(self.slp, mesh, self.rt, self.rt, self.bt, m,
                           stream=stream)
         self.K = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
-                          stream=stream)
+                          stream=stream, nearplan=self.V.nearplan)
         if stream is not None:
             stream.sync()
         for H in (self.V, self.K):
@@ -594,7 +594,7 @@ class HelmholtzBrakhageWerner:
         self.V = H2Matrix(self.slp, mesh, self.rt, self.rt, self.bt, m,
                           stream=stream)
         self.KM = H2Matrix(self.dlp, mesh, self.rt, self.rt, self.bt, m,
-                           stream=stream)
+                           stream=stream, nearplan=self.V.nearplan)
         if stream is not None:
             stream.sync()
         for H in (self.V, self.KM):
