@@ -177,6 +177,16 @@ int build_projection_data(const h2hip_surface *s) {
 
 }  // namespace
 
+int h2hip::surface_vertex_lists(const h2hip_surface *s) {
+  std::lock_guard<std::mutex> guard(s->fl_lock);
+  if (!s->fl_ready) {
+    const int rc = build_projection_data(s);
+    if (rc != H2HIP_OK) return rc;
+    s->fl_ready = true;
+  }
+  return H2HIP_OK;
+}
+
 extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
                                             uint32_t q, int kind,
                                             const double *x0, double *d_out,

@@ -170,6 +170,12 @@ struct h2hip_surface {
   mutable h2hip::DevBuf<double> d_mval, d_fwork;
 };
 
+namespace h2hip {
+// Builds (once) the surface's vertex -> (4 triangle + local vertex) lists
+// and the linear-basis mass matrix on the device (d_vt_ptr, d_vt, ...).
+int surface_vertex_lists(const h2hip_surface *s);
+}  // namespace h2hip
+
 struct h2hip_op {
   const h2hip_surface *s = nullptr;
   int kernel = 0, layer = 0, rb = 0, cb = 0;

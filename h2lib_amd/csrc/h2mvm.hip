@@ -1062,6 +1062,134 @@ __global__ __launch_bounds__(NT) void interp_couplings(CplArgs a) {
   }
 }
 
+// ---- cluster bases by interpolation on the device ---------------------------
+//
+// Leaf matrices V_t[i, j] = int phi_i L_j (or int phi_i dL_j/dn with the
+// triangle normal for the column basis of the double layer) with the
+// single-triangle Gauss rule of order q, and transfer matrices E_s[i, j] =
+// L^f_j(xi^s_i) for a son s of f, over the tensor Lagrange polynomials of
+// the clusters' Chebyshev points, j = (ja m + jb) m + jc
+// (Library/bem3d.c:5937-6005, 7555-7884, 8024-8336).  The arithmetic is
+// that of the host construction: products of one-dimensional Lagrange
+// polynomials in the same order, sums over the rule points (and over the
+// support triangles of a vertex in increasing triangle order).
+
+struct IBArgs {
+  const NodeD *nodes;
+  const uint32_t *idx, *leaves;
+  uint32_t nnodes;
+  const double *x, *n, *g;
+  const uint32_t *t;
+  const uint32_t *vt_ptr, *vt;  // vertex -> 4 triangle + local vertex
+  const double *pts;            // 3 m per node
+  uint32_t m;
+  int basis, dn;
+  uint32_t nq;
+  double rx0[64], rx1[64], rw[64];
+  double *V, *VT, *E, *ET;
+  uint64_t eld;
+};
+
+__device__ __forceinline__ double lagr1d(double x, const double *xi,
+                                         uint32_t mu, uint32_t m) {
+  double v = 1.0;
+  for (uint32_t l = 0; l < mu; l++) v *= (x - xi[l]) / (xi[mu] - xi[l]);
+  for (uint32_t l = mu + 1; l < m; l++) v *= (x - xi[l]) / (xi[mu] - xi[l]);
+  return v;
+}
+
+__device__ __forceinline__ double dlagr1d(double x, const double *xi,
+                                          uint32_t mu, uint32_t m) {
+  double sum = 0.0;
+  for (uint32_t l = 0; l < m; l++) {
+    if (l == mu) continue;
+    double v = 1.0;
+    for (uint32_t l2 = 0; l2 < m; l2++) {
+      if (l2 == mu || l2 == l) continue;
+      v *= (x - xi[l2]) / (xi[mu] - xi[l2]);
+    }
+    sum += v / (xi[mu] - xi[l]);
+  }
+  return sum;
+}
+
+// L_j (dn: <n, grad L_j>) at x for the points p of one cluster
+__device__ __forceinline__ double lagr3d_one(const double *x, const double *p,
+                                             uint32_t m, uint32_t ja,
+                                             uint32_t jb, uint32_t jc,
+                                             const double *nrm) {
+  const double lx = lagr1d(x[0], p, ja, m), ly = lagr1d(x[1], p + m, jb, m),
+               lz = lagr1d(x[2], p + 2 * m, jc, m);
+  if (!nrm) return lx * ly * lz;
+  const double dx = dlagr1d(x[0], p, ja, m), dy = dlagr1d(x[1], p + m, jb, m),
+               dz = dlagr1d(x[2], p + 2 * m, jc, m);
+  return nrm[0] * (dx * ly * lz) + nrm[1] * (lx * dy * lz) +
+         nrm[2] * (lx * ly * dz);
+}
+
+__global__ __launch_bounds__(NT) void interp_leaf(IBArgs a) {
+  const uint32_t node = a.leaves[blockIdx.x];
+  const NodeD L = a.nodes[node];
+  const uint32_t rows = L.size, k = L.rank, m = a.m;
+  const double *p = a.pts + 3 * (uint64_t)m * node;
+  for (uint32_t e = threadIdx.x; e < rows * k; e += NT) {
+    const uint32_t i = e % rows, j = e / rows;
+    const uint32_t ja = j / (m * m), jb = (j / m) % m, jc = j % m;
+    double acc = 0.0, xq[3];
+    if (!a.basis) {
+      const uint32_t tri = a.idx[L.off + i];
+      const double *A = a.x + 3 * a.t[3 * tri], *B = a.x + 3 * a.t[3 * tri + 1],
+                   *C = a.x + 3 * a.t[3 * tri + 2];
+      const double *nrm = a.dn ? a.n + 3 * tri : nullptr;
+      for (uint32_t q = 0; q < a.nq; q++) {
+        const double u = a.rx0[q], v = a.rx1[q];
+        for (int d = 0; d < 3; d++)
+          xq[d] = A[d] * (1.0 - u) + B[d] * (u - v) + C[d] * v;
+        acc += a.rw[q] * lagr3d_one(xq, p, m, ja, jb, jc, nrm);
+      }
+      acc *= a.g[tri];
+    } else {
+      const uint32_t vtx = a.idx[L.off + i];
+      for (uint32_t pp = a.vt_ptr[vtx]; pp < a.vt_ptr[vtx + 1]; pp++) {
+        const uint32_t tri = a.vt[pp] >> 2, l = a.vt[pp] & 3;
+        const double *A = a.x + 3 * a.t[3 * tri],
+                     *B = a.x + 3 * a.t[3 * tri + 1],
+                     *C = a.x + 3 * a.t[3 * tri + 2];
+        const double *nrm = a.dn ? a.n + 3 * tri : nullptr;
+        for (uint32_t q = 0; q < a.nq; q++) {
+          const double u = a.rx0[q], w = a.rx1[q];
+          for (int d = 0; d < 3; d++)
+            xq[d] = A[d] * (1.0 - u) + B[d] * (u - w) + C[d] * w;
+          const double phi = l == 0 ? 1.0 - u : (l == 1 ? u - w : w);
+          const double wq = a.rw[q] * phi * a.g[tri];
+          acc += wq * lagr3d_one(xq, p, m, ja, jb, jc, nrm);
+        }
+      }
+    }
+    a.V[L.voff + i + (uint64_t)j * rows] = acc;
+    a.VT[L.voff + j + (uint64_t)i * k] = acc;
+  }
+}
+
+__global__ __launch_bounds__(NT) void interp_transfer(IBArgs a) {
+  const uint32_t s = blockIdx.x;
+  const NodeD S = a.nodes[s];
+  if (S.father == NONE) return;
+  const uint32_t m = a.m, ks = S.rank, kf = a.nodes[S.father].rank;
+  const double *ps = a.pts + 3 * (uint64_t)m * s;
+  const double *pf = a.pts + 3 * (uint64_t)m * S.father;
+  for (uint32_t e = threadIdx.x; e < ks * kf; e += NT) {
+    const uint32_t i = e % ks, j = e / ks;
+    const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
+    const uint32_t ja = j / (m * m), jb = (j / m) % m, jc = j % m;
+    double v = lagr1d(ps[ia], pf, ja, m);
+    v *= lagr1d(ps[m + ib], pf + m, jb, m);
+    v *= lagr1d(ps[2 * m + ic], pf + 2 * m, jc, m);
+    a.E[(uint64_t)s * a.eld + i + (uint64_t)j * ks] = v;
+    a.ET[(uint64_t)s * a.eld + j + (uint64_t)i * kf] = v;
+  }
+}
+
 }  // namespace
 
 struct h2hip_h2matrix {
@@ -1720,6 +1848,97 @@ int h2hip_h2matrix_set_couplings(h2hip_h2matrix *H, const void *S,
   // couplings given as values: products read them
   H->cpl_known = false;
   H->cpl_recompute = 0;
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_interp_bases(h2hip_h2matrix *H, int side,
+                                const h2hip_surface *s, int basis, int dn,
+                                uint32_t m, uint32_t q, const double *pts,
+                                void *stream) {
+  if (!H || side < 0 || side > 1 || !s || !pts || m < 1 || m > 5 ||
+      (basis != 0 && basis != 1) || q < 1 || q > 8) {
+    set_error("h2hip_h2matrix_interp_bases: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  auto &S = H->side[side];
+  const uint32_t k = m * m * m;
+  for (const NodeD &d : S.nodes)
+    if (d.rank != k) {
+      set_error("h2hip_h2matrix_interp_bases: ranks differ from m^3");
+      return H2HIP_EINVAL;
+    }
+  if (S.ndof != (basis ? s->nv : s->nt)) {
+    set_error("h2hip_h2matrix_interp_bases: the tree's index range does not "
+              "match the surface's degrees of freedom");
+    return H2HIP_EINVAL;
+  }
+  if (basis) {
+    const int rc = h2hip::surface_vertex_lists(s);
+    if (rc != H2HIP_OK) return rc;
+  }
+  const h2hip::TriRule tr = h2hip::make_triangle_rule(q);
+  IBArgs a;
+  if (tr.w.size() > 64) return H2HIP_EINVAL;
+  a.nq = (uint32_t)tr.w.size();
+  for (uint32_t i = 0; i < a.nq; i++) {
+    a.rx0[i] = tr.x0[i];
+    a.rx1[i] = tr.x1[i];
+    a.rw[i] = tr.w[i];
+  }
+  h2hip::DevBuf<double> d_pts;
+  H2HIP_CHECK(d_pts.upload(pts, 3 * (uint64_t)m * S.nnodes));
+  if (S.d_V.n != S.vlen) H2HIP_CHECK(S.d_V.alloc(S.vlen));
+  if (S.d_VT.n != S.vlen) H2HIP_CHECK(S.d_VT.alloc(S.vlen));
+  const uint64_t elen = S.eld * S.nnodes;
+  if (S.d_E.n != elen) H2HIP_CHECK(S.d_E.alloc(elen));
+  if (S.d_ET.n != elen) H2HIP_CHECK(S.d_ET.alloc(elen));
+  hipStream_t st = (hipStream_t)stream;
+  H2HIP_CHECK(hipMemsetAsync(S.d_E.p, 0, sizeof(double) * elen, st));
+  H2HIP_CHECK(hipMemsetAsync(S.d_ET.p, 0, sizeof(double) * elen, st));
+  a.nodes = S.d_nodes.p;
+  a.idx = S.d_idx.p;
+  a.leaves = S.d_leaves.p;
+  a.nnodes = S.nnodes;
+  a.x = s->d_x.p;
+  a.n = s->d_n.p;
+  a.g = s->d_g.p;
+  a.t = s->d_t.p;
+  a.vt_ptr = basis ? s->d_vt_ptr.p : nullptr;
+  a.vt = basis ? s->d_vt.p : nullptr;
+  a.pts = d_pts.p;
+  a.m = m;
+  a.basis = basis;
+  a.dn = dn ? 1 : 0;
+  a.V = S.d_V.p;
+  a.VT = S.d_VT.p;
+  a.E = S.d_E.p;
+  a.ET = S.d_ET.p;
+  a.eld = S.eld;
+  if (!S.leaves.empty())
+    hipLaunchKernelGGL(interp_leaf, dim3((uint32_t)S.leaves.size()),
+                       dim3(NT), 0, st, a);
+  hipLaunchKernelGGL(interp_transfer, dim3(S.nnodes), dim3(NT), 0, st, a);
+  H2HIP_CHECK(hipGetLastError());
+  // the point buffer is freed on return
+  H2HIP_CHECK(hipStreamSynchronize(st));
+  // identity of the side's bases for the self-adjoint product: the same
+  // surface, tree, basis, derivative, order and points
+  {
+    double cs = (double)(uintptr_t)s * 1e-12 + basis * 3.0 + dn * 7.0 +
+                m * 11.0 + q * 13.0;
+    double w = 1.0;
+    for (uint64_t i = 0; i < 3 * (uint64_t)m * S.nnodes; i++, w += 1e-7)
+      cs += w * pts[i];
+    H->basis_sum[side] = cs;
+    H->basis_set[side] = 1;
+  }
+  if (H->level_sync == 2 && H->basis_set[0] && H->basis_set[1]) {
+    const int rc = build_flat_pair(H);
+    if (rc != H2HIP_OK) {
+      H->level_sync = 1;
+      return rc;
+    }
+  }
   return H2HIP_OK;
 }
 

@@ -60,6 +60,11 @@ def _sigs():
     h.h2hip_h2matrix_set_schedule.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_set_coupling_mode.restype = C.c_int
     h.h2hip_h2matrix_set_coupling_mode.argtypes = [vp, C.c_int]
+    h.h2hip_h2matrix_interp_bases.restype = C.c_int
+    h.h2hip_h2matrix_interp_bases.argtypes = [vp, C.c_int, vp, C.c_int,
+                                              C.c_int, C.c_uint32, C.c_uint32,
+                                              vp, vp]
+    b.h2b_interp_points.argtypes = [C.c_void_p, C.c_uint, C.c_void_p]
     h.h2hip_h2matrix_interp_couplings.restype = C.c_int
     h.h2hip_h2matrix_interp_couplings.argtypes = [vp, C.c_int, dp, u32, dp,
                                                   dp, vp]
@@ -97,8 +102,31 @@ class Interpolation:
     i: k x k column-major at i k^2; S of far block b at b k^2."""
 
     def __init__(self, mesh, rt, ct, btree, row_basis, col_basis, dlp, m, q,
-                 couplings=True):
+                 couplings=True, bases=True):
+        """bases=False: only the trees, blocks and interpolation points
+        (the bases are then built on the device, H2Matrix
+        device_bases)."""
         _sigs()
+        if not bases:
+            self._ip = None
+            self.m, self.k = m, m ** 3
+            self.eld = self.k * self.k
+            self.rtree, self.ctree = tree_arrays(rt), tree_arrays(ct)
+            self.rrank = np.full(rt.nnodes, self.k, dtype=np.uint32)
+            self.crank = np.full(ct.nnodes, self.k, dtype=np.uint32)
+            self.ridx, self.cidx = rt.idx, ct.idx
+            self.near = np.ascontiguousarray(btree.near, dtype=np.uint32)
+            self.far = np.ascontiguousarray(btree.far, dtype=np.uint32)
+            self.rpts = np.zeros(3 * m * rt.nnodes)
+            self.cpts = np.zeros(3 * m * ct.nnodes)
+            bem().h2b_interp_points(rt._tp, m, self.rpts.ctypes.data_as(
+                C.c_void_p))
+            bem().h2b_interp_points(ct._tp, m, self.cpts.ctypes.data_as(
+                C.c_void_p))
+            self.S = None
+            self._bp = btree._bp
+            self.nfar = len(self.far)
+            return
         ip = bem().h2b_build_h2interp(mesh._sp, rt._tp, ct._tp, btree._bp,
                                       row_basis, col_basis, int(dlp), m, q)
         if not ip:
@@ -151,7 +179,7 @@ class H2Matrix:
 
     def __init__(self, op, mesh, rt, ct, btree, m, build_nearfield=True,
                  stream=None, device_couplings=True, symmetric=False,
-                 nearplan=None):
+                 nearplan=None, device_bases=True):
         """symmetric: keep only the blocks on and below the block diagonal
         (half the storage) for addevalsymm; the operator must be
         self-adjoint with equal row and column bases (a Laplace single
@@ -167,10 +195,12 @@ class H2Matrix:
         self.op, self.rt, self.ct, self.btree = op, rt, ct, btree
         self.dtype = op.dtype
         self.m = m
+        device_bases = device_bases and device_couplings
         I = Interpolation(mesh, rt, ct, btree, op.row_basis, op.col_basis,
                           op.layer == DLP, m, op.q_regular,
                           couplings=not device_couplings
-                          and op.kernel == LAPLACE)
+                          and op.kernel == LAPLACE,
+                          bases=not device_bases)
         self.interp = I
         self.k = I.k
         self.n = rt.n
@@ -191,12 +221,21 @@ class H2Matrix:
             _ptr(I.crank), ct.n, _ptr(ct.idx), len(near), _ptr(near),
             len(far), _ptr(far)), "h2hip_h2matrix_create")
         self._h = h
-        check(hip().h2hip_h2matrix_set_basis(
-            h, 0, _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld),
-            "h2hip_h2matrix_set_basis")
-        check(hip().h2hip_h2matrix_set_basis(
-            h, 1, _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld),
-            "h2hip_h2matrix_set_basis")
+        if device_bases:
+            for sd, basis, dn, pts in ((0, op.row_basis, 0, I.rpts),
+                                       (1, op.col_basis, int(op.layer == DLP),
+                                        I.cpts)):
+                check(hip().h2hip_h2matrix_interp_bases(
+                    h, sd, op.gr.handle, basis, dn, m, op.q_regular,
+                    _ptr(pts), C.c_void_p(stream.handle if stream else 0)),
+                    "h2hip_h2matrix_interp_bases")
+        else:
+            check(hip().h2hip_h2matrix_set_basis(
+                h, 0, _ptr(I.rV), _ptr(I.rvoff), _ptr(I.rE), I.eld),
+                "h2hip_h2matrix_set_basis")
+            check(hip().h2hip_h2matrix_set_basis(
+                h, 1, _ptr(I.cV), _ptr(I.cvoff), _ptr(I.cE), I.eld),
+                "h2hip_h2matrix_set_basis")
         if device_couplings:
             kk = np.array([np.real(op.k), np.imag(op.k)], dtype=np.float64)
             check(hip().h2hip_h2matrix_interp_couplings(

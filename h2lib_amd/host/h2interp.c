@@ -293,6 +293,20 @@ side(const h2b_surface *s, const h2b_ctree *T, int basis, int dn,
   *E = EE;
 }
 
+void
+h2b_interp_points(const h2b_ctree *T, unsigned m, double *out)
+{
+  double cheb[16];
+  /* the same nodes as h2b_build_h2interp, bit for bit */
+  for (unsigned i = 0; i < m; i++) {
+    double e = 1.0 / (2.0 * m);
+    cheb[i] = cos(M_PI * (2.0 * i * e + e));
+  }
+  for (uint32_t i = 0; i < T->nnodes; i++)
+    interp_points(T->nodes[i].bmin, T->nodes[i].bmax, cheb, m,
+                  out + 3 * (size_t) m * i);
+}
+
 h2b_h2interp *
 h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
                    const h2b_ctree *ct, const h2b_blocks *bl, int row_basis,

@@ -120,6 +120,11 @@ h2b_h2interp *h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
                                  const h2b_ctree *ct, const h2b_blocks *bl,
                                  int row_basis, int col_basis, int col_dn,
                                  unsigned m, unsigned q);
+/* The tensor Chebyshev points of every cluster of T (3 m per node: x,
+ * then y, then z; the points h2b_build_h2interp uses, bit for bit), for
+ * h2hip_h2matrix_interp_bases and h2hip_h2matrix_interp_couplings
+ * (setup_interpolation_bem3d, Library/bem3d.c:4978-4996, 5179-5283). */
+void h2b_interp_points(const h2b_ctree *T, unsigned m, double *out);
 /* Laplace couplings S = g(xi_r, xi_c) / (4 pi) for all far blocks. */
 void h2b_h2interp_coupling_laplace(h2b_h2interp *H, const h2b_blocks *bl);
 /* Helmholtz couplings (complex, re / im interleaved, 2 k^2 doubles per

@@ -230,6 +230,21 @@ int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync);
  * back to 0.  h2hip_h2matrix_bytes then counts no coupling bytes.  No
  * counterpart in the reference. */
 int h2hip_h2matrix_set_coupling_mode(h2hip_h2matrix *H, int recompute);
+/* Cluster bases of one side (0 rows, 1 columns) built on the device by
+ * interpolation, in place of h2hip_h2matrix_set_basis: leaf matrices
+ * V_t[i, j] = int phi_i L_j (dn != 0: int phi_i dL_j/dn, the column basis of
+ * the double layer) with the single-triangle rule of order q, and transfer
+ * matrices E_s[i, j] = L^father_j(xi^s_i), over the tensor Chebyshev points
+ * pts (3 m per cluster: x, then y, then z; h2b_interp_points) of the
+ * side's tree on the surface s (basis 0: DOFs are triangles, 1: vertices).
+ * Every rank must be m^3 (m <= 5).  assemble_bem3d_h2matrix_row/
+ * col_clusterbasis with setup_h2matrix_aprx_inter_bem3d
+ * (Library/bem3d.c:9658-9686, 10377-10446, 5937-6005, 7555-8336).
+ * Complete on return. */
+int h2hip_h2matrix_interp_bases(h2hip_h2matrix *H, int side,
+                                const h2hip_surface *s, int basis, int dn,
+                                uint32_t m, uint32_t q, const double *pts,
+                                void *stream);
 int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
                                     const double *k, uint32_t m,
                                     const double *rpts, const double *cpts,

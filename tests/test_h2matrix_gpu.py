@@ -293,6 +293,39 @@ def test_computed_couplings_equal_stored(kern, layer, basis, m, k):
     assert H.bytes()["coupling"] == nb["coupling"]
 
 
+@pytest.mark.parametrize("kern,layer,basis,m", [(0, 0, 0, 3), (0, 1, 1, 4),
+                                                (0, 0, 1, 5), (1, 1, 0, 3),
+                                                (1, 0, 1, 2)])
+def test_device_bases_equal_host_bases(kern, layer, basis, m):
+    # cluster bases built on the device (leaf matrices with the single-
+    # triangle rule, normal derivatives for the double layer's column
+    # basis, transfer matrices) give the products of the host-built bases
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(12)
+    gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    if kern == 0:
+        op = (h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis) if layer == 0
+              else h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5))
+    else:
+        op = (h.new_slp_helmholtz_bem3d(2.0, gr, 2, 4, basis, basis)
+              if layer == 0 else
+              h.new_dlp_helmholtz_bem3d(2.0, gr, 2, 4, basis, basis, -0.5))
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    Hd = H2Matrix(op, mesh, rt, rt, bt, m)
+    Hh = H2Matrix(op, mesh, rt, rt, bt, m, device_bases=False)
+    rng = np.random.default_rng(13)
+    n = len(rt.idx)
+    x = rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if kern else 0)
+    for trans in (False, True):
+        a = product(Hd, x, trans=trans)
+        b = product(Hh, x, trans=trans)
+        assert np.abs(a - b).max() < 1e-13 * np.abs(b).max(), trans
+    assert Hd.bytes() == Hh.bytes()
+
+
 def test_flat_transforms_symmetric_storage():
     import h2lib_amd as h
     from h2lib_amd import geometry as G
