@@ -473,6 +473,12 @@ new_hip_h2matrix(pch2matrix h2)
   if (h2hip_h2matrix_set_nearfield(A->H, N) != H2HIP_OK)
     hip_fail("near field");
   free(N);
+  /* flat basis transforms where the tree allows them (leaves that
+   * partition the index positions, as H2Lib's cluster trees do), else the
+   * level-synchronous sweep */
+  if (h2hip_h2matrix_set_schedule(A->H, 2) != H2HIP_OK
+      && h2hip_h2matrix_set_schedule(A->H, 1) != H2HIP_OK)
+    hip_fail("schedule");
 
   if (h2hip_malloc(&A->d_x, sizeof(field) * (A->ncols > A->nrows ?
                                              A->ncols : A->nrows) + 16)
