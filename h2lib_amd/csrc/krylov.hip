@@ -80,7 +80,10 @@ __device__ void last_block_sum(const double *part, int m, uint32_t *cnt,
 }
 
 // out[0 .. 2m) = Q[:, 0..m)^* w (complex pairs; imaginary parts 0 when real)
-template <int C>
+// ATOM: every block adds its sums into out (zeroed beforehand) with FP64
+// atomics instead of the last-block reduction, whose agent-scope fences
+// write back and invalidate the L2 of every XCD
+template <int C, bool ATOM = false>
 __device__ __forceinline__ void multidot_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
@@ -133,25 +136,28 @@ __device__ __forceinline__ void multidot_body(
     if (threadIdx.x < 2 * mj) {
       double s = 0.0;
       for (int v = 0; v < NT / 64; v++) s += red[v][threadIdx.x];
-      part[((uint64_t)blockIdx.x * 64 + (j0 + threadIdx.x / 2)) * 2 +
-           (threadIdx.x & 1)] = s;
+      if constexpr (ATOM)
+        atomicAdd(&out[2 * j0 + threadIdx.x], s);
+      else
+        part[((uint64_t)blockIdx.x * 64 + (j0 + threadIdx.x / 2)) * 2 +
+             (threadIdx.x & 1)] = s;
     }
     __syncthreads();
   }
-  last_block_sum(part, m, cnt, out);
+  if constexpr (!ATOM) last_block_sum(part, m, cnt, out);
 }
 
-template <int C>
+template <int C, bool ATOM = false>
 __global__ __launch_bounds__(NT) void multidot(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
     uint32_t *cnt, double *out) {
-  multidot_body<C>(Q, ldq, m, w, n, part, cnt, out);
+  multidot_body<C, ATOM>(Q, ldq, m, w, n, part, cnt, out);
 }
 
 // w -= sign Q h (h: m complex pairs); with nrm != nullptr also
 // nrm[0] = |w|^2 of the result (last-block reduction)
-template <int C>
+template <int C, bool ATOM = false>
 __device__ __forceinline__ void multiaxpy_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ h, double sign, double *__restrict__ w,
@@ -191,18 +197,22 @@ __device__ __forceinline__ void multiaxpy_body(
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int v = 0; v < NT / 64; v++) s += red[v];
-    part[(uint64_t)blockIdx.x * 128] = s;
-    part[(uint64_t)blockIdx.x * 128 + 1] = 0.0;
+    if constexpr (ATOM) {
+      atomicAdd(nrm, s);
+    } else {
+      part[(uint64_t)blockIdx.x * 128] = s;
+      part[(uint64_t)blockIdx.x * 128 + 1] = 0.0;
+    }
   }
-  last_block_sum(part, 1, cnt, nrm);
+  if constexpr (!ATOM) last_block_sum(part, 1, cnt, nrm);
 }
 
-template <int C>
+template <int C, bool ATOM = false>
 __global__ __launch_bounds__(NT) void multiaxpy(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ h, double sign, double *__restrict__ w,
     uint64_t n, double *__restrict__ part, uint32_t *cnt, double *nrm) {
-  multiaxpy_body<C>(Q, ldq, m, h, sign, w, n, part, cnt, nrm);
+  multiaxpy_body<C, ATOM>(Q, ldq, m, h, sign, w, n, part, cnt, nrm);
 }
 
 __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
@@ -213,6 +223,22 @@ __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
 __global__ void zero_kernel(double *dst, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
   if (i < n) dst[i] = 0.0;
+}
+
+// dst[0 .. n) = 0 and acc[0 .. nacc) = 0 (the step's atomic accumulators)
+__global__ void zero2_kernel(double *dst, uint64_t n, double *acc,
+                             uint32_t nacc) {
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n) dst[i] = 0.0;
+  if (i < nacc) acc[i] = 0.0;
+}
+
+// dst = src and acc[0 .. nacc) = 0
+__global__ void copy2_kernel(const double *src, double *dst, uint64_t n,
+                             double *acc, uint32_t nacc) {
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+  if (i < nacc) acc[i] = 0.0;
 }
 
 // ---- small complex helpers for the single-thread parts ----------------------
@@ -417,9 +443,16 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // multi-dot grid: one row per thread up to NB blocks
   const dim3 gdot((uint32_t)std::min<uint64_t>(NB, (n + NT - 1) / NT));
 
+  // the step's atomic accumulators hA, hB (with y between them) and nrm
+  // lie at the start of the small workspace, before the residual
+  // estimates
+  const uint32_t nacc = (uint32_t)(W.nrm + 8 - W.small.p);
+  const dim3 gz((uint32_t)((std::max<uint64_t>(W.n2, nacc) + NT - 1) / NT));
   // |b|
-  hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st, b, (uint64_t)0,
-                     1, b, n, W.part.p, W.cnt.p, W.nrm);
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(NT), 0, st, W.nrm,
+                     (uint64_t)2);
+  hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st, b,
+                     (uint64_t)0, 1, b, n, W.part.p, W.cnt.p, W.nrm);
   double nb2[2];
   H2HIP_CHECK(hipMemcpyAsync(nb2, W.nrm, sizeof(nb2), hipMemcpyDeviceToHost,
                              st));
@@ -429,10 +462,11 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // r = b - A x in column 0, normalised; starts the least-squares problem
   auto init = [&]() -> int {
     double *q0 = W.col(0);
-    hipLaunchKernelGGL(copy_kernel, W.gv, dim3(NT), 0, st, b, q0, W.n2);
+    hipLaunchKernelGGL(copy2_kernel, W.gv, dim3(NT), 0, st, b, q0, W.n2,
+                       W.nrm, 2u);
     int r = addeval(mone, A, x, q0, st);
     if (r) return r;
-    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
                        (const double *)q0, (uint64_t)0, 1,
                        (const double *)q0, n, W.part.p, W.cnt.p, W.nrm);
     hipLaunchKernelGGL(start_cycle, W.gv, dim3(NT), 0, st, q0, W.n2,
@@ -452,19 +486,20 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // one Arnoldi step on column k; its residual estimate goes to err[slot]
   auto step = [&](int k, int slot) -> int {
     double *qk = W.col(k), *w = W.w.p;
-    hipLaunchKernelGGL(zero_kernel, W.gv, dim3(NT), 0, st, w, W.n2);
+    hipLaunchKernelGGL(zero2_kernel, gz, dim3(NT), 0, st, w, W.n2,
+                       W.small.p, nacc);
     int r = addeval(one, A, qk, w, st);
     if (r) return r;
-    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
                        (const double *)W.Q.p, n, k + 1, (const double *)w, n,
                        W.part.p, W.cnt.p + 2, W.hA);
-    hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
+    hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
                        (const double *)W.Q.p, n, k + 1, (const double *)W.hA,
                        1.0, w, n, W.part.p, W.cnt.p + 1, (double *)nullptr);
-    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
                        (const double *)W.Q.p, n, k + 1, (const double *)w, n,
                        W.part.p, W.cnt.p + 2, W.hB);
-    hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
+    hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
                        (const double *)W.Q.p, n, k + 1, (const double *)W.hB,
                        1.0, w, n, W.part.p, W.cnt.p + 1, W.nrm);
     hipLaunchKernelGGL(scale_givens, W.gv, dim3(NT), 0, st,
