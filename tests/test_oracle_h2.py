@@ -161,3 +161,27 @@ def test_symmetric_product_matches_reference(basis):
     assert np.abs(got - want).max() < TOL * np.abs(want).max()
     full = Oracle.h2_addeval(interp, N, x, y0.copy(), alpha=0.75)
     assert np.abs(got - full).max() < 1e-2 * np.abs(full).max()
+
+
+@pytest.mark.parametrize("basis,m", [(0, 3), (1, 4), (1, 5)])
+def test_interp_points_equal_host_build(basis, m):
+    # the points handed to the device basis construction are those of the
+    # host construction, bit for bit, and lie in the (widened) boxes
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import Interpolation
+    mesh = G.sphere(8)
+    rt = G.ClusterTree(mesh, basis, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    full = Interpolation(mesh, rt, rt, bt, basis, basis, False, m, 2,
+                         couplings=False)
+    pts = Interpolation(mesh, rt, rt, bt, basis, basis, False, m, 2,
+                        couplings=False, bases=False)
+    assert np.array_equal(full.rpts, pts.rpts)
+    assert np.array_equal(full.cpts, pts.cpts)
+    p = pts.rpts.reshape(rt.nnodes, 3, m)
+    lo = np.minimum(rt.bmin, rt.bmax - 5e-3)[:, :, None] - 5e-3
+    hi = np.maximum(rt.bmax, rt.bmin + 5e-3)[:, :, None] + 5e-3
+    assert np.all(p > lo) and np.all(p < hi)
+    # Chebyshev nodes of the first kind are symmetric about the centre
+    c = 0.5 * (p[:, :, 0] + p[:, :, -1])
+    assert np.allclose(c, 0.5 * (p[:, :, 1] + p[:, :, -2]), atol=1e-14)
