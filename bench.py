@@ -367,9 +367,9 @@ def bench_serving_multi(h, args):
     stream (args.multi_rate per second, seeded); each picks a mesh from a
     catalog of sphere / cube / cylinder meshes with 2048 to 131072
     triangles, Laplace (interior Dirichlet, linear basis, sources outside)
-    or Helmholtz (exterior Dirichlet, constant basis, sources inside, k in
-    {1, 1.5, 2}: below the first interior Dirichlet eigenvalue of all three
-    shapes, where the single layer operator is invertible), with one to four
+    or Helmholtz (exterior Dirichlet, constant basis, sources inside, k
+    uniform in [1, 6] in steps of 1/2, Brakhage-Werner form (KM - i k V) x = b,
+    free of the resonances of the single layer operator), with one to four
     unit point sources.  Operators (H2, interpolation order 4, clf 64) are
     cached on the device within a budget of args.multi_budget_gb GB and
     built on demand by a worker thread; GMRES(30) to 1e-10 in batches of up
@@ -421,7 +421,7 @@ def bench_serving_multi(h, args):
         f, b = e.prob.rhs_sources(r.sources, e.stream)
         x = h.DeviceArray(e.prob.n, b.dtype)
         x.zero()
-        solve_gmres(e.prob.V, b, x, 1e-13, 6000, 30, e.stream)
+        solve_gmres(e.op, b, x, 1e-13, 6000, 30, e.stream)
         xr = x.download()
         errs["%s/%s/k=%g" % r.key] = float(np.linalg.norm(r.x - xr) /
                                            np.linalg.norm(xr))

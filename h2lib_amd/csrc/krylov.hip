@@ -1006,6 +1006,9 @@ __global__ __launch_bounds__(NT) void tick_final(TickArgs a) {
 
 struct h2hip_gmres_batch {
   const h2hip_h2matrix *H = nullptr;
+  // optional combination c1 H + c2 H2 (complex coefficients)
+  const h2hip_h2matrix *H2 = nullptr;
+  double c1[2] = {1.0, 0.0}, c2[2] = {0.0, 0.0};
   int C = 0;
   uint64_t n = 0, n2 = 0;
   int kmax = 0, nslots = 0;
@@ -1088,8 +1091,26 @@ int batch_tick(h2hip_gmres_batch *G, const int *act, const int *ks, int na,
   const uint32_t gax = (uint32_t)((G->n + NT - 1) / NT);
   const uint32_t gdot = (uint32_t)std::min<uint64_t>(NB, gax);
   hipLaunchKernelGGL(tick_prep, dim3(gv, na), dim3(NT), 0, G->st, a);
-  int rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, alpha, xs, ys, 0,
+  int rc;
+  if (!G->H2) {
+    rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, alpha, xs, ys, 0,
+                                      G->st);
+  } else {
+    // y += alpha (c1 H + c2 H2) x: two batched products
+    double a1[2 * MAXSLOT], a2[2 * MAXSLOT];
+    for (int j = 0; j < na; j++) {
+      const double ar = alpha[2 * j], ai = alpha[2 * j + 1];
+      a1[2 * j] = ar * G->c1[0] - ai * G->c1[1];
+      a1[2 * j + 1] = ar * G->c1[1] + ai * G->c1[0];
+      a2[2 * j] = ar * G->c2[0] - ai * G->c2[1];
+      a2[2 * j + 1] = ar * G->c2[1] + ai * G->c2[0];
+    }
+    rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, a1, xs, ys, 0,
+                                      G->st);
+    if (!rc)
+      rc = h2hip_addeval_h2matrix_batch(G->H2, (uint32_t)na, a2, xs, ys, 0,
                                         G->st);
+  }
   if (rc) return rc;
   hipLaunchKernelGGL(tick_dot<C>, dim3(gdot, na), dim3(NT), 0, G->st, a, 0);
   hipLaunchKernelGGL(tick_axpy<C>, dim3(gax, na), dim3(NT), 0, G->st, a, 0);
@@ -1271,6 +1292,25 @@ int h2hip_gmres_batch_create(h2hip_gmres_batch **out, const h2hip_h2matrix *H,
 }
 
 void h2hip_gmres_batch_destroy(h2hip_gmres_batch *G) { delete G; }
+
+int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
+                              const h2hip_h2matrix *H2, const double *c2) {
+  if (!G || !c1 || !H2 || !c2) {
+    h2hip::set_error("h2hip_gmres_batch_combine: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  for (int s = 0; s < G->nslots; s++)
+    if (G->slot[s].active) {
+      h2hip::set_error("h2hip_gmres_batch_combine: solves in progress");
+      return H2HIP_EINVAL;
+    }
+  G->c1[0] = c1[0];
+  G->c1[1] = c1[1];
+  G->H2 = H2;
+  G->c2[0] = c2[0];
+  G->c2[1] = c2[1];
+  return H2HIP_OK;
+}
 
 int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G) {
   if (!G) return H2HIP_EINVAL;

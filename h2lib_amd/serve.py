@@ -17,7 +17,12 @@ Tests/test_helmholtzbem3d.c):
 * Laplace: interior Dirichlet problem, piecewise linear basis, data of unit
   point sources outside the surface, V x = (K + M/2) f;
 * Helmholtz: exterior Dirichlet problem, piecewise constant basis, data of
-  unit point sources inside the surface, V x = (K - M/2) f.
+  unit point sources inside the surface, in the Brakhage-Werner form of the
+  reference's Helmholtz test: (KM - i k V) x = b with KM the double layer
+  operator with alpha = 1/2 and b the integrals of the Dirichlet data
+  (Tests/test_helmholtzbem3d.c:24-65, 353-450).  Unlike V x = (K - M/2) f
+  it has no resonances (V is singular at the interior Dirichlet
+  eigenvalues, of which the unit ball has several below 6).
 """
 import threading
 import time
@@ -27,7 +32,7 @@ import numpy as np
 
 from . import geometry as G
 from .device import DeviceArray, Stream
-from .solve import GmresBatch, HelmholtzExterior, LaplaceDirichlet
+from .solve import GmresBatch, HelmholtzBrakhageWerner, LaplaceDirichlet
 
 LAPLACE, HELMHOLTZ = "laplace", "helmholtz"
 
@@ -72,11 +77,12 @@ def footprint(mesh, kind, clf, m, slots, kmax, eta=2.0):
 
 
 class Entry:
-    """One cached operator pair with its stream and batched GMRES."""
+    """One cached operator pair with its stream and batched GMRES; op is
+    the operator the requests solve with (V, or KM - i k V)."""
 
-    def __init__(self, key, prob, batch, stream, nbytes, build_s):
-        self.key, self.prob, self.batch, self.stream = key, prob, batch, \
-            stream
+    def __init__(self, key, prob, op, batch, stream, nbytes, build_s):
+        self.key, self.prob, self.op, self.batch, self.stream = \
+            key, prob, op, batch, stream
         self.bytes, self.build_s = nbytes, build_s
         self.running = {}  # rid -> (request, f, b, x)
 
@@ -113,15 +119,17 @@ class Server:
         mesh = self.meshes[mesh_id]
         if kind == LAPLACE:
             prob = LaplaceDirichlet(mesh, clf=self.clf, m=self.m[kind])
+            op = prob.V
         else:
-            prob = HelmholtzExterior(mesh, k, clf=self.clf, m=self.m[kind],
-                                     stream=stream)
+            prob = HelmholtzBrakhageWerner(mesh, k, clf=self.clf,
+                                           eta_adm=2.0, m=self.m[kind],
+                                           stream=stream)
             stream.sync()
-        H = prob.V
-        batch = GmresBatch(H, slots=self.slots, kmax=self.kmax,
+            op = prob.A
+        batch = GmresBatch(op, slots=self.slots, kmax=self.kmax,
                            stream=stream)
-        out["entry"] = Entry(key, prob, batch, stream, self._footprint(key),
-                             time.perf_counter() - t0)
+        out["entry"] = Entry(key, prob, op, batch, stream,
+                             self._footprint(key), time.perf_counter() - t0)
 
     def _evict_for(self, need):
         """Evicts idle entries (LRU first) until need more bytes fit; False
@@ -240,11 +248,15 @@ class Server:
         return requests
 
 
-def draw_requests(count, rate, mesh_ids, bounds, seed, ks=(1.0, 1.5, 2.0)):
+K_GRID = tuple(1.0 + 0.5 * i for i in range(11))  # 1, 1.5, ..., 6
+
+
+def draw_requests(count, rate, mesh_ids, bounds, seed, ks=K_GRID):
     """Poisson arrivals at `rate` per second; each request picks a mesh,
-    Laplace or Helmholtz (even odds), a wave number from ks, and one to
-    four unit point sources: outside the mesh's bounding sphere for
-    Laplace (interior problem), inside half its inscribed radius for
+    Laplace or Helmholtz (even odds), a wave number uniformly from ks (by
+    default [1, 6] in steps of 1/2, so that requests can share operators),
+    and one to four unit point sources: outside the mesh's bounding sphere
+    for Laplace (interior problem), inside half its inscribed radius for
     Helmholtz (exterior problem).  bounds: {mesh_id: (inner, outer)}."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))

@@ -53,6 +53,9 @@ def _sigs():
                                            C.c_int, C.c_uint64, C.c_uint32,
                                            C.c_uint32, C.c_void_p]
     h.h2hip_gmres_batch_destroy.argtypes = [C.c_void_p]
+    h.h2hip_gmres_batch_combine.restype = C.c_int
+    h.h2hip_gmres_batch_combine.argtypes = [C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
     h.h2hip_gmres_batch_free_slots.restype = C.c_int
     h.h2hip_gmres_batch_free_slots.argtypes = [C.c_void_p]
     h.h2hip_gmres_batch_submit.restype = C.c_int
@@ -192,6 +195,7 @@ class Combined:
     def __init__(self, A1, c1, A2=None, c2=0.0):
         _sigs()
         self.A1, self.A2 = A1, A2
+        self.c1, self.c2 = c1, c2
         self.dtype = np.complex128 if (np.iscomplexobj(c1) or
                                        np.iscomplexobj(c2) or
                                        A1.dtype == np.complex128) \
@@ -267,16 +271,29 @@ class GmresBatch:
     solve_gmres)."""
 
     def __init__(self, H, slots=8, kmax=30, stream=None):
+        """H: an H2Matrix, or a Combined of two H2-matrices (c1 A1 + c2 A2,
+        two batched products per step)."""
         _sigs()
         self.H = H
         self.stream = stream
+        A1 = H.A1 if isinstance(H, Combined) else H
         h = C.c_void_p()
         check(hip().h2hip_gmres_batch_create(
-            C.byref(h), H._h, int(H.dtype == np.complex128), len(H.interp.ridx),
-            slots, kmax, C.c_void_p(stream.handle if stream else 0)),
+            C.byref(h), A1._h, int(H.dtype == np.complex128),
+            len(A1.interp.ridx), slots, kmax,
+            C.c_void_p(stream.handle if stream else 0)),
             "h2hip_gmres_batch_create")
         self._h = h
         self._keep = {}
+        if isinstance(H, Combined):
+            if H.A2 is None:
+                raise ValueError("GmresBatch: a Combined needs two operators")
+            c1 = np.array([np.real(H.c1), np.imag(H.c1)])
+            c2 = np.array([np.real(H.c2), np.imag(H.c2)])
+            check(hip().h2hip_gmres_batch_combine(
+                h, c1.ctypes.data_as(C.c_void_p), H.A2._h,
+                c2.ctypes.data_as(C.c_void_p)), "h2hip_gmres_batch_combine")
+            self._coef = (c1, c2)
 
     def free_slots(self):
         return hip().h2hip_gmres_batch_free_slots(self._h)
@@ -611,6 +628,17 @@ class HelmholtzBrakhageWerner:
                                     INTEGRATE_C, self.k, x0, d_b,
                                     stream=stream)
         return d_b
+
+    def rhs_sources(self, x0s, stream=None):
+        """(b, b) for the sum of the data of several unit point sources
+        (the pair mirrors the other problems' (f, b))."""
+        d_b = DeviceArray(self.n, np.complex128)
+        for j, x0 in enumerate(x0s):
+            helmholtz_point_data_device(self.gr, self.q, 0,
+                                        INTEGRATE_L if self.basis else
+                                        INTEGRATE_C, self.k, x0, d_b,
+                                        accumulate=j > 0, stream=stream)
+        return d_b, d_b
 
     def solve(self, d_b, eps=1e-12, maxiter=1000, kmax=50, stream=None):
         d_x = DeviceArray(self.n, np.complex128)

@@ -391,6 +391,13 @@ int h2hip_gmres_batch_create(h2hip_gmres_batch **out, const h2hip_h2matrix *H,
                              int is_complex, uint64_t n, uint32_t slots,
                              uint32_t kmax, void *stream);
 void h2hip_gmres_batch_destroy(h2hip_gmres_batch *G);
+/* The batch's operator becomes c1 H + c2 H2 (complex coefficients as
+ * (re, im); H2 of the same size and batched-product limits), e.g. the
+ * Brakhage-Werner operator KM - i eta V of the reference's Helmholtz test
+ * (addeval_A, Tests/test_helmholtzbem3d.c:24-65): every step makes two
+ * batched products.  Only while no solve is in progress. */
+int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
+                              const h2hip_h2matrix *H2, const double *c2);
 /* Number of free slots. */
 int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G);
 /* Starts a solve of A x = b (device vectors; x holds the start vector and

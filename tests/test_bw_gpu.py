@@ -174,3 +174,49 @@ def test_bw_linear_solve_matches_reference():
     assert abs(err - err_cross) < 1e-12
     assert abs(err - err_ref) < 1e-7
     assert abs(it - it_ref) <= 2
+
+
+def test_bw_batched_gmres_equals_single():
+    """The batched GMRES on the combined operator KM - i k V (two batched
+    products per step, the server's Helmholtz path) gives each system the
+    solution of the one-system GMRES on the same operator, to 1e-8.
+    Interpolation order 4 (rank 64, the batched product's limit) and eta 2,
+    as the server builds it."""
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import (GmresBatch, HelmholtzBrakhageWerner,
+                                 solve_gmres)
+    prob = HelmholtzBrakhageWerner(G.sphere(8), K, clf=32, eta_adm=2.0, m=4)
+    srcs = [[X0], [np.array([0.1, -0.2, 0.0])],
+            [X0, np.array([-0.3, 0.1, 0.1])]]
+    bs = [prob.rhs_sources(s)[1] for s in srcs]
+    alone = []
+    for d_b in bs:
+        d_x = h.DeviceArray(prob.n, np.complex128)
+        d_x.zero()
+        it, _ = solve_gmres(prob.A, d_b, d_x, 1e-10, 2000, 30)
+        alone.append((d_x.download(), it))
+    G = GmresBatch(prob.A, slots=2, kmax=30)
+    xs = []
+    for _ in bs:
+        d_x = h.DeviceArray(prob.n, np.complex128)
+        d_x.zero()
+        xs.append(d_x)
+    pending = list(range(len(bs)))
+    done = {}
+    rounds = 0
+    while len(done) < len(bs):
+        while pending and G.free_slots() > 0:
+            t = pending.pop(0)
+            G.submit(bs[t], xs[t], 1e-10, 2000, t)
+        for tag, it, res in G.run():
+            done[tag] = it
+        rounds += 1
+        assert rounds < 10000
+    print("bw batch steps", [done[t] for t in range(len(bs))],
+          [a[1] for a in alone], flush=True)
+    for t in range(len(bs)):
+        x = xs[t].download()
+        assert np.linalg.norm(x - alone[t][0]) < \
+            1e-8 * np.linalg.norm(alone[t][0]), t
+        assert abs(done[t] - alone[t][1]) <= 1, t

@@ -3,8 +3,9 @@
 * Each request's solution is the same in the mix (admitted while solves on
   other meshes and problems run, batched with other requests of its
   operators) as solved alone with the one-system GMRES on the same
-  operators: to 1e-8 relative (both stop at residual 1e-10 |b|; the batched
-  product sums in another order).
+  operators (V for Laplace, the Brakhage-Werner operator KM - i k V for
+  Helmholtz): to 1e-8 relative (both stop at residual 1e-10 |b|; the
+  batched product sums in another order).
 * With a device-memory budget that holds about one operator pair, idle
   operators are evicted and rebuilt on demand, and every request still
   completes with its residual bound met.
@@ -35,7 +36,7 @@ def check_against_alone(srv, reqs, tol):
         f, b = e.prob.rhs_sources(r.sources, e.stream)
         x = h.DeviceArray(e.prob.n, b.dtype)
         x.zero()
-        it, _ = solve_gmres(e.prob.V, b, x, 1e-10, 3000, 30, e.stream)
+        it, _ = solve_gmres(e.op, b, x, 1e-10, 3000, 30, e.stream)
         xa = x.download()
         err = np.abs(r.x - xa).max() / np.abs(xa).max()
         print("request", r.rid, r.key, "steps", r.steps, it,
