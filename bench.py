@@ -15,8 +15,8 @@ HBM.  value = near-field entries per second over the timed steps.
 The second half of the metric is measured on the same mesh after the
 timed steps: the H2 matrices of both operators (interpolation order 4) are
 built once and kept on the device, then Dirichlet solves run one after the
-other (boundary data -> right-hand side -> GMRES(30) -> solution on the
-host); the line reports solves/s, p50/p99 latency and the relative error
+other (boundary data -> right-hand side -> CG, the reference Laplace
+test's solver -> solution on the host); the line reports solves/s, p50/p99 latency and the relative error
 of the solution against the reference library's tightly solved solution
 (tests/golden/dirichlet_sphere_s64.npz).
 
@@ -438,8 +438,10 @@ def bench_dirichlet(h, mesh, args, stream):
     """Dirichlet solves of the interior Laplace problem on the same mesh with
     the operators resident on the device: each solve projects the boundary
     data u = 1/|x - x0|, x0 = (1.2, 1.2, 1.2), forms b = (K + M/2) f and
-    runs GMRES(30) on V x = b; the latency of a solve runs from the boundary
-    data on the host to the solution on the host."""
+    solves V x = b by CG (the reference test's solver, its steps replayed as
+    captured graphs) or GMRES(30) (--dirichlet-solver); the latency of a
+    solve runs from the boundary data on the host to the solution on the
+    host."""
     from h2lib_amd.solve import LaplaceDirichlet
     x0 = np.array([1.2, 1.2, 1.2])
     t0 = time.perf_counter()
@@ -509,7 +511,9 @@ def bench_dirichlet(h, mesh, args, stream):
         t = time.perf_counter()
         d_f, d_b = prob.rhs(x0, stream)
         d_sol, it, res = prob.solve(d_b, eps=args.eps, maxiter=1000, kmax=30,
-                                    stream=stream)
+                                    stream=stream,
+                                    solver=args.dirichlet_solver,
+                                    use_graph=not args.cg_direct)
         x = d_sol.download()
         dt = time.perf_counter() - t
         if r >= args.solve_warmup:
@@ -554,8 +558,10 @@ def bench_dirichlet(h, mesh, args, stream):
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "solves": len(lat),
-        "gmres_eps": args.eps,
-        "gmres_steps": int(np.median(steps)),
+        "solver": ("CG, the reference test's solver" if
+                   args.dirichlet_solver == "cg" else "GMRES(30)"),
+        "eps": args.eps,
+        "steps": int(np.median(steps)),
         "neumann_rel_err": prob.neumann_error(x, x0),
         "operator_build_s": build_s,
         "matvec": {
@@ -696,6 +702,10 @@ def main():
     ap.add_argument("--multi-check", type=int, default=6,
                     help="requests whose solutions are checked against a "
                     "tight solve")
+    ap.add_argument("--dirichlet-solver", choices=("cg", "gmres"),
+                    default="cg",
+                    help="solver of the Dirichlet solves/s line (CG is the "
+                    "reference Laplace test's)")
     ap.add_argument("--cg-direct", action="store_true",
                     help="CG timings with direct launches instead of "
                          "graph-replayed chunks")

@@ -644,12 +644,10 @@ __device__ __forceinline__ void cdiv(double ar, double ai, double br,
 // r holds b - A x: p = r, a = 0, |b|, |r|, and the test before the first
 // step
 template <int C>
-__global__ __launch_bounds__(NT) void cg_init(const double *__restrict__ b,
-                                              const double *__restrict__ r,
-                                              double *__restrict__ p,
-                                              double *__restrict__ a,
-                                              uint64_t n, double *part,
-                                              CgDev d, double eps) {
+__device__ void cg_init_body(const double *__restrict__ b,
+                             const double *__restrict__ r,
+                             double *__restrict__ p, double *__restrict__ a,
+                             uint64_t n, double *part, CgDev d, double eps) {
   double acc[2] = {0.0, 0.0};
   for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * NT) {
@@ -685,11 +683,10 @@ __global__ __launch_bounds__(NT) void cg_init(const double *__restrict__ b,
 
 // gamma = <p, a>, <p, r>
 template <int C>
-__global__ __launch_bounds__(NT) void cg_dots(const double *__restrict__ p,
-                                              const double *__restrict__ a,
-                                              const double *__restrict__ r,
-                                              uint64_t n, double *part,
-                                              CgDev d) {
+__device__ void cg_dots_body(const double *__restrict__ p,
+                             const double *__restrict__ a,
+                             const double *__restrict__ r, uint64_t n,
+                             double *part, CgDev d) {
   if (d.flag[0]) return;  // uniform over the grid
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
@@ -717,10 +714,11 @@ __global__ __launch_bounds__(NT) void cg_dots(const double *__restrict__ p,
 // lambda = <p, r> / gamma, x += lambda p, r -= lambda a, then <a, r>,
 // |r|^2, mu, the step count and the stopping test
 template <int C>
-__global__ __launch_bounds__(NT) void cg_update(
-    double *__restrict__ x, const double *__restrict__ p,
-    double *__restrict__ r, const double *__restrict__ a, uint64_t n,
-    double *part, CgDev d) {
+__device__ void cg_update_body(double *__restrict__ x,
+                               const double *__restrict__ p,
+                               double *__restrict__ r,
+                               const double *__restrict__ a, uint64_t n,
+                               double *part, CgDev d) {
   if (d.flag[0]) return;
   double lr, li = 0.0;
   if constexpr (C)
@@ -768,9 +766,10 @@ __global__ __launch_bounds__(NT) void cg_update(
 
 // p = r - mu p with mu = <a, r> / gamma; a = 0 for the next product
 template <int C>
-__global__ __launch_bounds__(NT) void cg_direction(
-    double *__restrict__ p, const double *__restrict__ r,
-    double *__restrict__ a, uint64_t n, CgDev d) {
+__device__ void cg_direction_body(double *__restrict__ p,
+                                  const double *__restrict__ r,
+                                  double *__restrict__ a, uint64_t n,
+                                  CgDev d) {
   if (d.flag[0]) return;
   double mr, mi = 0.0;
   if constexpr (C)
@@ -790,6 +789,39 @@ __global__ __launch_bounds__(NT) void cg_direction(
       a[i] = 0.0;
     }
   }
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cg_init(const double *b,
+                                              const double *r, double *p,
+                                              double *a, uint64_t n,
+                                              double *part, CgDev d,
+                                              double eps) {
+  cg_init_body<C>(b, r, p, a, n, part, d, eps);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cg_dots(const double *p,
+                                              const double *a,
+                                              const double *r, uint64_t n,
+                                              double *part, CgDev d) {
+  cg_dots_body<C>(p, a, r, n, part, d);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cg_update(double *x, const double *p,
+                                                double *r, const double *a,
+                                                uint64_t n, double *part,
+                                                CgDev d) {
+  cg_update_body<C>(x, p, r, a, n, part, d);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cg_direction(double *p,
+                                                   const double *r,
+                                                   double *a, uint64_t n,
+                                                   CgDev d) {
+  cg_direction_body<C>(p, r, a, n, d);
 }
 
 template <int C>
@@ -1002,6 +1034,63 @@ __global__ __launch_bounds__(NT) void tick_final(TickArgs a) {
                       a.hA[s], a.hB[s], a.lsq[s], a.k[s], a.err[s]);
 }
 
+// Batched CG: the same slots, one CG step of every slot per tick (the
+// kernels of the single solver above, grid dimension y = slot).  A slot's
+// first tick starts its solve (r = b - A x, then the initialisation); the
+// device's done flag stops its updates, so the host only looks at the flags
+// after each chunk.  Vectors r, p, a live in the first three Krylov columns
+// of the slot's workspace, the CG scalars in hA, the flags in cnt.
+struct CgTickArgs {
+  const double *b[MAXSLOT];
+  double *x[MAXSLOT], *r[MAXSLOT], *p[MAXSLOT], *a[MAXSLOT], *part[MAXSLOT];
+  CgDev d[MAXSLOT];
+  double eps[MAXSLOT];
+  int start[MAXSLOT];  // 1: this tick starts the solve
+  int slot[MAXSLOT];   // the slot's index (where its look goes)
+  uint64_t n, n2;
+};
+
+__global__ __launch_bounds__(NT) void cgt_prep(CgTickArgs t) {
+  const int s = blockIdx.y;
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (!t.start[s] || i >= t.n2) return;
+  t.r[s][i] = t.b[s][i];
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cgt_dots(CgTickArgs t) {
+  const int s = blockIdx.y;
+  if (t.start[s])
+    cg_init_body<C>(t.b[s], t.r[s], t.p[s], t.a[s], t.n, t.part[s], t.d[s],
+                    t.eps[s]);
+  else
+    cg_dots_body<C>(t.p[s], t.a[s], t.r[s], t.n, t.part[s], t.d[s]);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cgt_update(CgTickArgs t) {
+  const int s = blockIdx.y;
+  if (t.start[s]) return;  // uniform per block
+  cg_update_body<C>(t.x[s], t.p[s], t.r[s], t.a[s], t.n, t.part[s], t.d[s]);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void cgt_direction(CgTickArgs t) {
+  const int s = blockIdx.y;
+  if (t.start[s]) return;
+  cg_direction_body<C>(t.p[s], t.r[s], t.a[s], t.n, t.d[s]);
+}
+
+// (done, steps, |r|) of the na slots into out[slot * CHUNK + 0..2]
+__global__ void cgt_look(CgTickArgs t, int na, double *out) {
+  const int j = threadIdx.x;
+  if (j >= na) return;
+  double *o = out + (uint64_t)t.slot[j] * CHUNK;
+  o[0] = (double)t.d[j].flag[0];
+  o[1] = (double)t.d[j].flag[1];
+  o[2] = t.d[j].s[8];
+}
+
 }  // namespace
 
 struct h2hip_gmres_batch {
@@ -1010,6 +1099,7 @@ struct h2hip_gmres_batch {
   const h2hip_h2matrix *H2 = nullptr;
   double c1[2] = {1.0, 0.0}, c2[2] = {0.0, 0.0};
   int C = 0;
+  int method = 0;  // 0: GMRES(kmax), 1: CG
   uint64_t n = 0, n2 = 0;
   int kmax = 0, nslots = 0;
   hipStream_t st = nullptr;
@@ -1059,6 +1149,29 @@ int batch_finish(h2hip_gmres_batch *G, h2hip_gmres_batch::Slot &S) {
   return H2HIP_OK;
 }
 
+// ys[j] += alpha[j] A xs[j] for na vectors, A = H or c1 H + c2 H2
+int batch_product(h2hip_gmres_batch *G, int na, const double *alpha,
+                  const void *const *xs, void *const *ys) {
+  if (!G->H2)
+    return h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, alpha, xs, ys, 0,
+                                        G->st);
+  // y += alpha (c1 H + c2 H2) x: two batched products
+  double a1[2 * MAXSLOT], a2[2 * MAXSLOT];
+  for (int j = 0; j < na; j++) {
+    const double ar = alpha[2 * j], ai = alpha[2 * j + 1];
+    a1[2 * j] = ar * G->c1[0] - ai * G->c1[1];
+    a1[2 * j + 1] = ar * G->c1[1] + ai * G->c1[0];
+    a2[2 * j] = ar * G->c2[0] - ai * G->c2[1];
+    a2[2 * j + 1] = ar * G->c2[1] + ai * G->c2[0];
+  }
+  int rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, a1, xs, ys, 0,
+                                        G->st);
+  if (!rc)
+    rc = h2hip_addeval_h2matrix_batch(G->H2, (uint32_t)na, a2, xs, ys, 0,
+                                      G->st);
+  return rc;
+}
+
 // one tick for the slots in `act` (k[s] = -1: start of a cycle)
 template <int C>
 int batch_tick(h2hip_gmres_batch *G, const int *act, const int *ks, int na,
@@ -1091,26 +1204,7 @@ int batch_tick(h2hip_gmres_batch *G, const int *act, const int *ks, int na,
   const uint32_t gax = (uint32_t)((G->n + NT - 1) / NT);
   const uint32_t gdot = (uint32_t)std::min<uint64_t>(NB, gax);
   hipLaunchKernelGGL(tick_prep, dim3(gv, na), dim3(NT), 0, G->st, a);
-  int rc;
-  if (!G->H2) {
-    rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, alpha, xs, ys, 0,
-                                      G->st);
-  } else {
-    // y += alpha (c1 H + c2 H2) x: two batched products
-    double a1[2 * MAXSLOT], a2[2 * MAXSLOT];
-    for (int j = 0; j < na; j++) {
-      const double ar = alpha[2 * j], ai = alpha[2 * j + 1];
-      a1[2 * j] = ar * G->c1[0] - ai * G->c1[1];
-      a1[2 * j + 1] = ar * G->c1[1] + ai * G->c1[0];
-      a2[2 * j] = ar * G->c2[0] - ai * G->c2[1];
-      a2[2 * j + 1] = ar * G->c2[1] + ai * G->c2[0];
-    }
-    rc = h2hip_addeval_h2matrix_batch(G->H, (uint32_t)na, a1, xs, ys, 0,
-                                      G->st);
-    if (!rc)
-      rc = h2hip_addeval_h2matrix_batch(G->H2, (uint32_t)na, a2, xs, ys, 0,
-                                        G->st);
-  }
+  int rc = batch_product(G, na, alpha, xs, ys);
   if (rc) return rc;
   hipLaunchKernelGGL(tick_dot<C>, dim3(gdot, na), dim3(NT), 0, G->st, a, 0);
   hipLaunchKernelGGL(tick_axpy<C>, dim3(gax, na), dim3(NT), 0, G->st, a, 0);
@@ -1121,11 +1215,77 @@ int batch_tick(h2hip_gmres_batch *G, const int *act, const int *ks, int na,
   return H2HIP_OK;
 }
 
+// CG: CHUNK ticks of every active slot, then the look at the flags
+template <int C>
+int cg_batch_enqueue(h2hip_gmres_batch *G) {
+  CgTickArgs t = {};
+  t.n = G->n;
+  t.n2 = G->n2;
+  int act[MAXSLOT], na = 0;
+  for (int s = 0; s < G->nslots; s++) {
+    G->part[s] = G->slot[s].active ? 1 : 0;
+    if (G->slot[s].active) act[na++] = s;
+  }
+  if (na == 0) return H2HIP_OK;
+  bool start0[MAXSLOT];
+  for (int j = 0; j < na; j++) {
+    auto &S = G->slot[act[j]];
+    t.b[j] = S.b;
+    t.x[j] = S.x;
+    t.r[j] = S.W.col(0);
+    t.p[j] = S.W.col(1);
+    t.a[j] = S.W.col(2);
+    t.part[j] = S.W.part.p;
+    t.d[j] = CgDev{S.W.hA, S.W.cnt.p, S.maxiter};
+    t.eps[j] = S.eps;
+    t.slot[j] = act[j];
+    start0[j] = S.need_start;
+    S.need_start = false;
+  }
+  const uint32_t gv = (uint32_t)std::min<uint64_t>(CG_NB, (G->n + NT - 1) /
+                                                              NT);
+  const uint32_t gc = (uint32_t)((G->n2 + NT - 1) / NT);
+  for (int tick = 0; tick < CHUNK; tick++) {
+    const void *xs[MAXSLOT];
+    void *ys[MAXSLOT];
+    double alpha[2 * MAXSLOT];
+    bool any_start = false;
+    for (int j = 0; j < na; j++) {
+      const bool st = tick == 0 && start0[j];
+      t.start[j] = st ? 1 : 0;
+      any_start |= st;
+      xs[j] = st ? (const double *)t.x[j] : (const double *)t.p[j];
+      ys[j] = st ? t.r[j] : t.a[j];
+      alpha[2 * j] = st ? -1.0 : 1.0;
+      alpha[2 * j + 1] = 0.0;
+    }
+    if (any_start)
+      hipLaunchKernelGGL(cgt_prep, dim3(gc, na), dim3(NT), 0, G->st, t);
+    int rc = batch_product(G, na, alpha, xs, ys);
+    if (rc) return rc;
+    hipLaunchKernelGGL(cgt_dots<C>, dim3(gv, na), dim3(NT), 0, G->st, t);
+    hipLaunchKernelGGL(cgt_update<C>, dim3(gv, na), dim3(NT), 0, G->st, t);
+    hipLaunchKernelGGL(cgt_direction<C>, dim3(gv, na), dim3(NT), 0, G->st,
+                       t);
+  }
+  hipLaunchKernelGGL(cgt_look, dim3(1), dim3(64), 0, G->st, t, na,
+                     G->d_errs.p);
+  H2HIP_CHECK(hipGetLastError());
+  H2HIP_CHECK(hipMemcpyAsync(G->h_errs, G->d_errs.p,
+                             sizeof(double) * MAXSLOT * CHUNK,
+                             hipMemcpyDeviceToHost, G->st));
+  H2HIP_CHECK(hipEventRecord(G->done, G->st));
+  G->len = CHUNK;
+  G->inflight = true;
+  return H2HIP_OK;
+}
+
 // Enqueues one chunk: restarts due now, up to CHUNK ticks (none past a
 // restart or maxiter of any slot), the copy of the residual estimates to
 // pinned host memory and an event behind it.  Nothing waits.
 template <int C>
 int batch_enqueue(h2hip_gmres_batch *G) {
+  if (G->method == 1) return cg_batch_enqueue<C>(G);
   // restarts due at this point
   for (int s = 0; s < G->nslots; s++) {
     auto &S = G->slot[s];
@@ -1197,6 +1357,27 @@ int batch_complete(h2hip_gmres_batch *G, int *pending, uint32_t *ndone,
   const double *errs = G->h_errs;
   const int len = G->len;
   uint32_t nd = 0;
+  if (G->method == 1) {
+    // CG: x is final once the device has set the slot's done flag (the
+    // chunk's last kernels ran before the event)
+    for (int s = 0; s < G->nslots; s++) {
+      auto &S = G->slot[s];
+      if (!G->part[s] || !S.active) continue;
+      const double *o = errs + (uint64_t)s * CHUNK;
+      if (o[0] == 0.0) continue;
+      S.active = false;
+      S.iter = (uint32_t)o[1];
+      S.error = o[2];
+      if (nd < cap) {
+        if (tags) tags[nd] = S.tag;
+        if (iters) iters[nd] = S.iter;
+        if (resid) resid[nd] = S.error;
+      }
+      nd++;
+    }
+    *ndone = nd;
+    return H2HIP_OK;
+  }
   for (int s = 0; s < G->nslots; s++) {
     auto &S = G->slot[s];
     if (!G->part[s] || !S.active) continue;
@@ -1312,6 +1493,27 @@ int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
   return H2HIP_OK;
 }
 
+int h2hip_gmres_batch_set_method(h2hip_gmres_batch *G, int method) {
+  if (!G || (method != 0 && method != 1) || (method == 1 && G->kmax < 3)) {
+    h2hip::set_error("h2hip_gmres_batch_set_method: invalid arguments "
+                     "(0: GMRES, 1: CG, which needs kmax >= 3)");
+    return H2HIP_EINVAL;
+  }
+  for (int s = 0; s < G->nslots; s++)
+    if (G->slot[s].active || G->inflight) {
+      h2hip::set_error("h2hip_gmres_batch_set_method: solves in progress");
+      return H2HIP_EINVAL;
+    }
+  if (method == G->method) return H2HIP_OK;
+  // the flags and block counters of both methods share cnt: start at zero
+  for (int s = 0; s < G->nslots; s++)
+    H2HIP_CHECK(hipMemsetAsync(G->slot[s].W.cnt.p, 0, sizeof(uint32_t) * 4,
+                               G->st));
+  H2HIP_CHECK(hipStreamSynchronize(G->st));
+  G->method = method;
+  return H2HIP_OK;
+}
+
 int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G) {
   if (!G) return H2HIP_EINVAL;
   int f = 0;
@@ -1344,6 +1546,10 @@ int h2hip_gmres_batch_submit(h2hip_gmres_batch *G, const void *d_b,
   S.iter = 0;
   S.started = false;
   S.need_start = true;
+  if (G->method == 1) {  // CG forms |b| on the device
+    S.active = true;
+    return s;
+  }
   // |b| (one small reduction and a wait, once per request)
   const uint32_t gdot = (uint32_t)std::min<uint64_t>(NB, (G->n + NT - 1) / NT);
   if (G->C)

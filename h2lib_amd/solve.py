@@ -53,6 +53,8 @@ def _sigs():
                                            C.c_int, C.c_uint64, C.c_uint32,
                                            C.c_uint32, C.c_void_p]
     h.h2hip_gmres_batch_destroy.argtypes = [C.c_void_p]
+    h.h2hip_gmres_batch_set_method.restype = C.c_int
+    h.h2hip_gmres_batch_set_method.argtypes = [C.c_void_p, C.c_int]
     h.h2hip_gmres_batch_combine.restype = C.c_int
     h.h2hip_gmres_batch_combine.argtypes = [C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p]
@@ -268,9 +270,10 @@ class GmresBatch:
     """Up to 8 independent GMRES(kmax) solves with one H2-matrix, advanced
     together so that every step reads the matrix once (continuous batching:
     solves enter free slots and leave when done, each with the semantics of
-    solve_gmres)."""
+    solve_gmres); with method="cg" conjugate gradients instead, each solve
+    with the semantics of solve_cg (for Hermitian positive definite H)."""
 
-    def __init__(self, H, slots=8, kmax=30, stream=None):
+    def __init__(self, H, slots=8, kmax=30, stream=None, method="gmres"):
         """H: an H2Matrix, or a Combined of two H2-matrices (c1 A1 + c2 A2,
         two batched products per step)."""
         _sigs()
@@ -294,6 +297,17 @@ class GmresBatch:
                 h, c1.ctypes.data_as(C.c_void_p), H.A2._h,
                 c2.ctypes.data_as(C.c_void_p)), "h2hip_gmres_batch_combine")
             self._coef = (c1, c2)
+        self.method = "gmres"
+        self.set_method(method)
+
+    def set_method(self, method):
+        """Switches between "gmres" and "cg" while no solve is active."""
+        if method not in ("gmres", "cg"):
+            raise ValueError("GmresBatch: method is 'gmres' or 'cg'")
+        check(hip().h2hip_gmres_batch_set_method(self._h,
+                                                  int(method == "cg")),
+              "h2hip_gmres_batch_set_method")
+        self.method = method
 
     def free_slots(self):
         return hip().h2hip_gmres_batch_free_slots(self._h)

@@ -398,6 +398,14 @@ void h2hip_gmres_batch_destroy(h2hip_gmres_batch *G);
  * batched products.  Only while no solve is in progress. */
 int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
                               const h2hip_h2matrix *H2, const double *c2);
+/* The batch's solves use method 0, GMRES(kmax) (the default), or 1,
+ * conjugate gradients with the semantics of h2hip_solve_cg
+ * (solve_cg_avector, Library/krylovsolvers.c:18-52, the reference Laplace
+ * test's solver; for Hermitian positive definite operators; kmax >= 3, as
+ * CG keeps its vectors in the first three Krylov columns).  With CG a chunk
+ * is always 8 steps and the device's stopping flag freezes each solve at
+ * its stopping step.  Only while no solve is in progress. */
+int h2hip_gmres_batch_set_method(h2hip_gmres_batch *G, int method);
 /* Number of free slots. */
 int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G);
 /* Starts a solve of A x = b (device vectors; x holds the start vector and

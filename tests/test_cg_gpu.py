@@ -17,7 +17,9 @@ h2hip_solve_cg against the reference's solve_cg_avector
   the products' atomic sums round differently from run to run; solutions
   to 1e-8), also with the self-adjoint product of a matrix kept in symmetric
   storage (exactly symmetric, unlike the full matrix whose near field is
-  symmetric only to quadrature accuracy).
+  symmetric only to quadrature accuracy);
+* the batched solver's CG mode against the one-system CG (and the same
+  batch switched back to GMRES against the one-system GMRES).
 """
 import numpy as np
 import pytest
@@ -161,3 +163,79 @@ def test_symmetric_storage_and_graph():
     print("symmetric vs full storage: steps %d / %d, solutions differ by "
           "%.2e" % (out[True, False][0], out[False, False][0], d), flush=True)
     assert d < 1e-3
+
+
+def _run_batch(G, bs, xs, eps, maxiters):
+    pending = list(range(len(bs)))
+    done = {}
+    rounds = 0
+    while len(done) < len(bs):
+        while pending and G.free_slots() > 0:
+            t = pending.pop(0)
+            G.submit(bs[t], xs[t], eps, maxiters[t], t)
+        for tag, it, res in G.run():
+            done[tag] = (it, res)
+        rounds += 1
+        assert rounds < 10000
+    return done
+
+
+def test_batched_cg_equals_single():
+    """CG in the batched solver (products of up to four systems at once,
+    systems entering as slots free up) against the one-system CG on the same
+    operator: solutions to 1e-8 relative (both stop at |r| <= 1e-10 |b|;
+    the batched product sums in another order), steps within max(2, 2%);
+    the stopping rule per slot (maxiter - 1 steps, no step from a converged
+    start); then the same batch switched back to GMRES against the
+    one-system GMRES."""
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import (GmresBatch, LaplaceDirichlet, solve_cg,
+                                 solve_gmres)
+    mesh = G.sphere(8)
+    prob = LaplaceDirichlet(mesh, clf=16, eta=2.0, m=3)
+    n = prob.n
+    rng = np.random.default_rng(12)
+    rhs = [rng.standard_normal(n) for _ in range(6)]
+    d_bs = [h.DeviceArray.from_host(b) for b in rhs]
+    alone = []
+    for d_b in d_bs:
+        d_x = h.DeviceArray(n)
+        d_x.zero()
+        it, _ = solve_cg(prob.V, d_b, d_x, 1e-10, 3000)
+        alone.append((d_x.download(), it))
+    B = GmresBatch(prob.V, slots=4, kmax=30, method="cg")
+    xs = [h.DeviceArray(n) for _ in rhs]
+    for x in xs:
+        x.zero()
+    done = _run_batch(B, d_bs, xs, 1e-10, [3000] * len(rhs))
+    print("batch cg steps", [done[t][0] for t in range(len(rhs))],
+          [a[1] for a in alone], flush=True)
+    for t in range(len(rhs)):
+        x0, it0 = alone[t]
+        assert np.abs(xs[t].download() - x0).max() < \
+            1e-8 * np.abs(x0).max(), t
+        assert abs(done[t][0] - it0) <= max(2, it0 // 50), t
+    # stopping rule: maxiter 1, 2, 10, 17 -> 0, 1, 9, 16 steps
+    xs2 = [h.DeviceArray(n) for _ in range(4)]
+    for x in xs2:
+        x.zero()
+    done = _run_batch(B, d_bs[:4], xs2, 1e-30, [1, 2, 10, 17])
+    assert [done[t][0] for t in range(4)] == [0, 1, 9, 16]
+    assert not xs2[0].download().any()
+    # a converged start takes no step and is left as it is
+    before = xs[1].download()
+    done = _run_batch(B, [d_bs[1]], [xs[1]], 1e-6, [3000])
+    assert done[0][0] == 0
+    assert np.array_equal(xs[1].download(), before)
+    # back to GMRES on the same slots
+    B.set_method("gmres")
+    d_x = h.DeviceArray(n)
+    d_x.zero()
+    it_g, _ = solve_gmres(prob.V, d_bs[2], d_x, 1e-10, 3000, 30)
+    xg = h.DeviceArray(n)
+    xg.zero()
+    done = _run_batch(B, [d_bs[2]], [xg], 1e-10, [3000])
+    assert abs(done[0][0] - it_g) <= 1
+    x0 = d_x.download()
+    assert np.abs(xg.download() - x0).max() < 1e-8 * np.abs(x0).max()
