@@ -593,17 +593,21 @@ def bench_dirichlet(h, mesh, args, stream):
     return out
 
 
-def bench_serving(h, prob, ref, args):
+def bench_serving(h, prob, ref, args, rate=None):
     """Continuous batching of Dirichlet solves with the operators resident
     on the device: requests arrive as a Poisson stream (seeded), each with
     the data of a point source x0 outside the unit sphere (the first one at
     x0 = (1.2, 1.2, 1.2), whose solution is checked against the reference);
     its right-hand side is formed on a second stream as soon as it arrives
-    and it enters the batched GMRES(30) as soon as one of the slots is free.
-    Latency: arrival to solution complete on the device."""
+    and it enters the batched solver (CG, the reference test's solver, or
+    GMRES(30): --serve-method) as soon as one of the slots is free.
+    Latency: arrival to solution complete on the device.  rate overrides
+    the arrival rate (a rate far above the capacity measures the capacity:
+    solves/s of a saturated server)."""
     from h2lib_amd.solve import GmresBatch
     rng = np.random.default_rng(2024)
-    R, rate = args.serve_requests, args.serve_rate
+    R = args.serve_requests
+    rate = args.serve_rate if rate is None else rate
     arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
     arrivals -= arrivals[0]
     x0s = [np.array([1.2, 1.2, 1.2])]
@@ -611,7 +615,8 @@ def bench_serving(h, prob, ref, args):
         d = rng.standard_normal(3)
         x0s.append(d / np.linalg.norm(d) * rng.uniform(1.5, 3.0))
     run_st, rhs_st = h.Stream(), h.Stream()
-    G = GmresBatch(prob.V, slots=args.serve_slots, kmax=30, stream=run_st)
+    G = GmresBatch(prob.V, slots=args.serve_slots, kmax=30, stream=run_st,
+                   method=args.serve_method)
     n = prob.n
     d_x = [h.DeviceArray(n) for _ in range(R)]
     for d in d_x:
@@ -650,8 +655,9 @@ def bench_serving(h, prob, ref, args):
         "solves_per_s": R / makespan,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "gmres_steps_median": int(np.median(steps)),
-        "gmres_eps": args.eps,
+        "method": args.serve_method,
+        "steps_median": int(np.median(steps)),
+        "eps": args.eps,
     }
     if ref is not None:
         x = d_x[0].download()
@@ -673,12 +679,16 @@ def main():
                     help="timed Dirichlet solves (0: skip them)")
     ap.add_argument("--solve-warmup", type=int, default=2)
     ap.add_argument("--eps", type=float, default=1e-10,
-                    help="GMRES stopping tolerance relative to |b|")
+                    help="stopping tolerance of the Dirichlet solves "
+                    "relative to |b|")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the serving run (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=60.0,
                     help="Poisson arrival rate of the serving run (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-method", choices=("cg", "gmres"), default="cg",
+                    help="batched solver of the serving runs' Laplace "
+                    "requests")
     ap.add_argument("--helm-split", type=int, default=128,
                     help="sphere split of the Helmholtz FP64 / mixed "
                     "near-field comparison (0: skip it)")
@@ -829,6 +839,8 @@ def main():
         if args.serve_requests > 0:
             log("bench: serving")
             dirichlet["serving"] = bench_serving(h, prob, ref, args)
+            dirichlet["serving_saturated"] = bench_serving(h, prob, ref,
+                                                           args, rate=1e5)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -10,9 +10,16 @@
 // eval_neumann_fundamental_laplacebem3d (Library/laplacebem3d.c:1390-1430)
 // for the source point x0.
 //
-// The whole projection is one workgroup of 1024 threads: the vectors are
-// short (one entry per vertex) and the conjugate-gradient loop runs on the
-// device without any host round trip.
+// The projection runs over the whole device: one launch for the triangle
+// integrals, one for the load vector, then three launches per CG step
+// (product with the mass matrix and <p, Mp>; the updates of x and r and
+// |r|^2; the new direction), each reduction summed deterministically by the
+// last block to finish (fixed partial sums per block, summed in block
+// order).  The stopping test runs on the device: the step that meets it
+// sets a done flag and the launches after it return at once.  The host
+// enqueues CHUNK steps at a time and reads the flag once per chunk while
+// the next chunk is already queued.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -21,8 +28,10 @@
 
 namespace {
 
-constexpr int PT = 1024;
+constexpr int PT = 256;      // threads per block
 constexpr int PW = PT / 64;
+constexpr int PNB = 512;     // blocks at most (grid-stride loops beyond)
+constexpr int PCHUNK = 8;    // CG steps between two looks at the flag
 constexpr int MAXQ2 = 64;  // points of the triangle rule (q <= 8)
 
 struct ProjArgs {
@@ -39,25 +48,80 @@ struct ProjArgs {
   double *tri;                      // 3 nt
   double *b, *r, *p, *ap;           // nv each
   double *out;
+  double *scal;   // [0] |b|^2, [1] |r|^2, [2] alpha, [3] beta
+  double *part;   // 2 partial sums per block
+  uint32_t *flag; // [0] done, [1] steps, [2] block counter
 };
 
-__device__ double block_sum(double v, double *red) {
+// Sums two per-thread values over the grid: every block stores its sums
+// in part, the last block to finish adds them in block order and leaves
+// the totals in tot[0..1] (LDS); true in that block only.  The barrier
+// completes the block's stores; the agent-scope fence publishes them before
+// the counter increment, and in the last block makes the other blocks'
+// sums visible.
+__device__ bool grid_sum2(double v0, double v1, double *part, uint32_t *cnt,
+                          double *tot) {
+  __shared__ double red[2][PW];
+  __shared__ bool last;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+  }
+  if (lane == 0) {
+    red[0][wv] = v0;
+    red[1][wv] = v1;
+  }
   __syncthreads();
-  double s = 0.0;
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < PW; w++) {
+      s0 += red[0][w];
+      s1 += red[1][w];
+    }
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
+    __threadfence();
+    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+    if (last) __threadfence();
+  }
+  __syncthreads();
+  if (!last) return false;
+  double s0 = 0.0, s1 = 0.0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += PT) {
+    s0 += part[2 * b];
+    s1 += part[2 * b + 1];
+  }
 #pragma unroll
-  for (int w = 0; w < PW; w++) s += red[w];
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+  }
+  __syncthreads();  // red is reused
+  if (lane == 0) {
+    red[0][wv] = s0;
+    red[1][wv] = s1;
+  }
   __syncthreads();
-  return s;
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int w = 0; w < PW; w++) {
+      t0 += red[0][w];
+      t1 += red[1][w];
+    }
+    tot[0] = t0;
+    tot[1] = t1;
+    *cnt = 0;
+  }
+  __syncthreads();
+  return true;
 }
 
-__global__ __launch_bounds__(PT) void project_kernel(ProjArgs a) {
-  __shared__ double red[PW];
-  const uint32_t tid = threadIdx.x;
-  // per triangle: g_t sum_k w_k f(x_k) phi_l(x_k), l = 0, 1, 2
-  for (uint32_t t = tid; t < a.nt; t += PT) {
+// per triangle: g_t sum_k w_k f(x_k) phi_l(x_k), l = 0, 1, 2
+__global__ __launch_bounds__(PT) void proj_tri(ProjArgs a) {
+  for (uint32_t t = blockIdx.x * PT + threadIdx.x; t < a.nt;
+       t += gridDim.x * PT) {
     const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
                  *C = a.x + 3 * a.t[3 * t + 2];
     const double *nn = a.n + 3 * t;
@@ -82,10 +146,14 @@ __global__ __launch_bounds__(PT) void project_kernel(ProjArgs a) {
     a.tri[3 * t + 1] = s1 * a.g[t];
     a.tri[3 * t + 2] = s2 * a.g[t];
   }
-  __syncthreads();
-  // b_i = sum over the triangles of vertex i; CG start x = 0, r = p = b
+}
+
+// b_i = sum over the triangles of vertex i; CG start x = 0, r = p = b,
+// and the test before the first step
+__global__ __launch_bounds__(PT) void proj_init(ProjArgs a) {
   double part = 0.0;
-  for (uint32_t i = tid; i < a.nv; i += PT) {
+  for (uint32_t i = blockIdx.x * PT + threadIdx.x; i < a.nv;
+       i += gridDim.x * PT) {
     double s = 0.0;
     for (uint32_t e = a.vt_ptr[i]; e < a.vt_ptr[i + 1]; e++) {
       const uint32_t code = a.vt[e];
@@ -97,35 +165,65 @@ __global__ __launch_bounds__(PT) void project_kernel(ProjArgs a) {
     a.out[i] = 0.0;
     part += s * s;
   }
-  double rr = block_sum(part, red);
-  const double bb = rr;
-  for (int it = 0; it < 500 && rr > 1e-30 * bb; it++) {
-    // ap = M p, pap = <p, ap>
-    part = 0.0;
-    for (uint32_t i = tid; i < a.nv; i += PT) {
-      double s = 0.0;
-      for (uint32_t e = a.mrow[i]; e < a.mrow[i + 1]; e++)
-        s += a.mval[e] * a.p[a.mcol[e]];
-      a.ap[i] = s;
-      part += a.p[i] * s;
-    }
-    const double pap = block_sum(part, red);
-    const double alpha = rr / pap;
-    part = 0.0;
-    for (uint32_t i = tid; i < a.nv; i += PT) {
-      a.out[i] += alpha * a.p[i];
-      const double ri = a.r[i] - alpha * a.ap[i];
-      a.r[i] = ri;
-      part += ri * ri;
-    }
-    const double rr2 = block_sum(part, red);
-    const double beta = rr2 / rr;
-    rr = rr2;
-    // p is read by other threads in the next product: update after the
-    // barrier of block_sum, and barrier again before the product
-    for (uint32_t i = tid; i < a.nv; i += PT) a.p[i] = a.r[i] + beta * a.p[i];
-    __syncthreads();
+  __shared__ double tot[2];
+  if (!grid_sum2(part, 0.0, a.part, a.flag + 2, tot)) return;
+  if (threadIdx.x == 0) {
+    a.scal[0] = tot[0];
+    a.scal[1] = tot[0];
+    a.flag[1] = 0;
+    a.flag[0] = tot[0] > 1e-30 * tot[0] ? 0u : 1u;
   }
+}
+
+// ap = M p, alpha = |r|^2 / <p, ap>
+__global__ __launch_bounds__(PT) void proj_spmv(ProjArgs a) {
+  if (a.flag[0]) return;  // uniform over the grid
+  double part = 0.0;
+  for (uint32_t i = blockIdx.x * PT + threadIdx.x; i < a.nv;
+       i += gridDim.x * PT) {
+    double s = 0.0;
+    for (uint32_t e = a.mrow[i]; e < a.mrow[i + 1]; e++)
+      s += a.mval[e] * a.p[a.mcol[e]];
+    a.ap[i] = s;
+    part += a.p[i] * s;
+  }
+  __shared__ double tot[2];
+  if (!grid_sum2(part, 0.0, a.part, a.flag + 2, tot)) return;
+  if (threadIdx.x == 0) a.scal[2] = a.scal[1] / tot[0];
+}
+
+// x += alpha p, r -= alpha ap, then |r|^2, beta, the step count and the
+// stopping test (at most 500 steps, |r|^2 <= 1e-30 |b|^2)
+__global__ __launch_bounds__(PT) void proj_update(ProjArgs a) {
+  if (a.flag[0]) return;
+  const double alpha = a.scal[2];
+  double part = 0.0;
+  for (uint32_t i = blockIdx.x * PT + threadIdx.x; i < a.nv;
+       i += gridDim.x * PT) {
+    a.out[i] += alpha * a.p[i];
+    const double ri = a.r[i] - alpha * a.ap[i];
+    a.r[i] = ri;
+    part += ri * ri;
+  }
+  __shared__ double tot[2];
+  if (!grid_sum2(part, 0.0, a.part, a.flag + 2, tot)) return;
+  if (threadIdx.x == 0) {
+    const double rr = tot[0];
+    a.scal[3] = rr / a.scal[1];
+    a.scal[1] = rr;
+    const uint32_t it = a.flag[1] + 1;
+    a.flag[1] = it;
+    if (!(it < 500 && rr > 1e-30 * a.scal[0])) a.flag[0] = 1;
+  }
+}
+
+// p = r + beta p
+__global__ __launch_bounds__(PT) void proj_dir(ProjArgs a) {
+  if (a.flag[0]) return;
+  const double beta = a.scal[3];
+  for (uint32_t i = blockIdx.x * PT + threadIdx.x; i < a.nv;
+       i += gridDim.x * PT)
+    a.p[i] = a.r[i] + beta * a.p[i];
 }
 
 // vertex -> (triangle, local vertex) lists and the CSR mass matrix
@@ -168,9 +266,23 @@ int build_projection_data(const h2hip_surface *s) {
       s->d_mrow.upload(mrow.data(), mrow.size()) != hipSuccess ||
       s->d_mcol.upload(mcol.data(), mcol.size()) != hipSuccess ||
       s->d_mval.upload(mval.data(), mval.size()) != hipSuccess ||
-      s->d_fwork.alloc(3 * (size_t)nt + 4 * (size_t)nv) != hipSuccess) {
+      s->d_fwork.alloc(3 * (size_t)nt + 4 * (size_t)nv + 16 + 2 * PNB) !=
+          hipSuccess ||
+      s->d_pflag.alloc(4) != hipSuccess ||
+      hipHostMalloc((void **)&s->h_plook, 4 * sizeof(uint32_t),
+                    hipHostMallocDefault) != hipSuccess) {
     h2hip::set_error("h2hip_projectL2: device allocation failed");
     return H2HIP_ENOMEM;
+  }
+  for (auto &e : s->pev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      h2hip::set_error("h2hip_projectL2: event creation failed");
+      return H2HIP_EDEVICE;
+    }
+  // the block counter starts at zero; every reduction resets it
+  if (hipMemset(s->d_pflag.p, 0, 4 * sizeof(uint32_t)) != hipSuccess) {
+    h2hip::set_error("h2hip_projectL2: device error");
+    return H2HIP_EDEVICE;
   }
   return H2HIP_OK;
 }
@@ -231,8 +343,41 @@ extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
   a.p = a.r + s->nv;
   a.ap = a.p + s->nv;
   a.out = d_out;
-  hipLaunchKernelGGL(project_kernel, dim3(1), dim3(PT), 0,
-                     (hipStream_t)stream, a);
+  a.scal = a.ap + s->nv;
+  a.part = a.scal + 16;
+  a.flag = s->d_pflag.p;
+  const hipStream_t st = (hipStream_t)stream;
+  const dim3 gt((uint32_t)std::min<uint64_t>(PNB, (s->nt + PT - 1) / PT));
+  const dim3 gv((uint32_t)std::min<uint64_t>(PNB, (s->nv + PT - 1) / PT));
+  // one projection per surface at a time (shared workspace and looks)
+  std::lock_guard<std::mutex> guard(s->proj_lock);
+  hipLaunchKernelGGL(proj_tri, gt, dim3(PT), 0, st, a);
+  hipLaunchKernelGGL(proj_init, gv, dim3(PT), 0, st, a);
+  auto chunk = [&](int slot) -> hipError_t {
+    for (int j = 0; j < PCHUNK; j++) {
+      hipLaunchKernelGGL(proj_spmv, gv, dim3(PT), 0, st, a);
+      hipLaunchKernelGGL(proj_update, gv, dim3(PT), 0, st, a);
+      hipLaunchKernelGGL(proj_dir, gv, dim3(PT), 0, st, a);
+    }
+    hipError_t e = hipMemcpyAsync(s->h_plook + 2 * slot, s->d_pflag.p,
+                                  2 * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(s->pev[slot], st);
+    return e;
+  };
+  // while the host waits for the look after one chunk, the next chunk is
+  // already queued; at most 500 steps, so the loop ends
+  int slot = 0;
+  H2HIP_CHECK(chunk(0));
+  for (;;) {
+    H2HIP_CHECK(chunk(slot ^ 1));
+    H2HIP_CHECK(hipEventSynchronize(s->pev[slot]));
+    if (s->h_plook[2 * slot]) break;
+    slot ^= 1;
+  }
+  // the chunk queued behind the last look returns at once; it finishes
+  // before the workspace can be used again
+  H2HIP_CHECK(hipEventSynchronize(s->pev[slot ^ 1]));
   H2HIP_CHECK(hipGetLastError());
   return H2HIP_OK;
 }

@@ -166,8 +166,17 @@ struct h2hip_surface {
   // and the workspace of the projection kernel
   mutable std::mutex fl_lock;
   mutable bool fl_ready = false;
-  mutable h2hip::DevBuf<uint32_t> d_vt_ptr, d_vt, d_mrow, d_mcol;
+  mutable h2hip::DevBuf<uint32_t> d_vt_ptr, d_vt, d_mrow, d_mcol, d_pflag;
   mutable h2hip::DevBuf<double> d_mval, d_fwork;
+  // the projection's looks at its stopping flag: two pinned slots, events
+  mutable std::mutex proj_lock;
+  mutable uint32_t *h_plook = nullptr;
+  mutable hipEvent_t pev[2] = {nullptr, nullptr};
+  ~h2hip_surface() {
+    if (h_plook) (void)hipHostFree(h_plook);
+    for (auto e : pev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 namespace h2hip {

@@ -486,15 +486,14 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
 // MF (one vector, not transposed): the couplings are computed from the
 // interpolation points instead of read (cpl_value), lane i keeping row
 // point i and stepping through the column points j = (ja m + jb) m + jc.
+// Item w of the wave (w < nfar: far block w, else near block w - nfar);
+// xs: the wave's LDS slice of NV * 2 * XR doubles.
 template <int C, bool TRANS, int NV, bool MF = false>
-__global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
+__device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
+                                                double *xs) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;  // inputs per LDS vector
-  constexpr int XS = NV * 2 * XR;             // LDS doubles per wave
   constexpr int UNR = NV == 1 ? 8 : 4;
-  __shared__ double xs_all[WPB * XS];
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  double *xs = xs_all + (threadIdx.x >> 6) * XS;
   if (w < a.nfar) {
     const FarD b = a.far[w];
     const NodeD r = a.rn[b.r], c = a.cn[b.c];
@@ -617,6 +616,14 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
                       a.v.are[v] * ar[v] - a.v.aim[v] * ai[v],
                       a.v.are[v] * ai[v] + a.v.aim[v] * ar[v]);
   }
+}
+
+template <int C, bool TRANS, int NV, bool MF = false>
+__global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
+  constexpr int XS = NV * 2 * (NV == 1 ? MAXRANK : 64);  // per wave
+  __shared__ double xs_all[WPB * XS];
+  couple_near_dev<C, TRANS, NV, MF>(a, blockIdx.x * WPB + (threadIdx.x >> 6),
+                                    xs_all + (threadIdx.x >> 6) * XS);
 }
 
 // Self-adjoint product from the lower block triangle (row and column
@@ -844,9 +851,10 @@ struct FlatArgs {
 };
 
 // a slice of a.zero per workgroup, for every vector in use
-__device__ __forceinline__ void flat_zero(const FlatArgs &a) {
-  const uint64_t chunk = (a.nzero + gridDim.x - 1) / gridDim.x;
-  const uint64_t e0 = (uint64_t)blockIdx.x * chunk,
+__device__ __forceinline__ void flat_zero(const FlatArgs &a, uint32_t blk,
+                                          uint32_t nblk) {
+  const uint64_t chunk = (a.nzero + nblk - 1) / nblk;
+  const uint64_t e0 = (uint64_t)blk * chunk,
                  e1 = min(a.nzero, e0 + chunk);
   for (uint32_t v = 0; v < a.v.nv; v++)
     for (uint64_t e = e0 + threadIdx.x; e < e1; e += NT)
@@ -887,15 +895,15 @@ __device__ __forceinline__ void dot_strided_nv(const double *__restrict__ M,
 // Upward sweep, one wave per (leaf, ancestor) block, NV vectors at once
 // (leaves and ranks <= 64 when NV > 1): x of the leaf gathered into LDS,
 // W read once for all vectors, atomics into the coefficients.
+// block blk of nblk; xs: the wave's LDS slice of NV * 2 * XR doubles
 template <int C, int K, int NV>
-__global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
+__device__ __forceinline__ void fwd_flat_dev(const FlatArgs &a, uint32_t blk,
+                                             uint32_t nblk, double *xs) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;
-  __shared__ double xs_all[WPB][NV * 2 * XR];
-  flat_zero(a);  // the coupling accumulators yt
-  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  flat_zero(a, blk, nblk);  // the coupling accumulators yt
+  const uint32_t w = blk * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= a.nitems) return;  // uniform over the wave; no block barrier
-  double *xs = xs_all[threadIdx.x >> 6];
   const FlatItem it = a.items[w];
   const NodeD L = a.nodes[it.leaf];
   const NodeD A = a.nodes[it.node];
@@ -922,13 +930,38 @@ __global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
   }
 }
 
+template <int C, int K, int NV>
+__global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
+  constexpr int XS = NV * 2 * (NV == 1 ? MAXRANK : 64);
+  __shared__ double xs_all[WPB * XS];
+  fwd_flat_dev<C, K, NV>(a, blockIdx.x, gridDim.x,
+                         xs_all + (threadIdx.x >> 6) * XS);
+}
+
+// The upward sweep in blocks [0, fb) and the near-field blocks (c.nfar = 0)
+// after them, in one launch: the near blocks do not wait for the
+// coefficients, so they fill the device while the short upward items run.
+template <int C, int K, int NV>
+__global__ __launch_bounds__(NT) void fwd_flat_near(FlatArgs a, CNArgs c,
+                                                    uint32_t fb) {
+  constexpr int XS = NV * 2 * (NV == 1 ? MAXRANK : 64);
+  __shared__ double xs_all[WPB * XS];
+  double *xs = xs_all + (threadIdx.x >> 6) * XS;
+  if (blockIdx.x < fb)  // uniform over the block
+    fwd_flat_dev<C, K, NV>(a, blockIdx.x, fb, xs);
+  else
+    couple_near_dev<C, false, NV>(
+        c, (blockIdx.x - fb) * WPB + (threadIdx.x >> 6), xs);
+}
+
 // Downward sweep, one wave per (leaf, ancestor) block, NV vectors at once:
 // yt of the ancestor in LDS, one row per lane, atomics into y.
 template <int C, int K, int NV>
 __global__ __launch_bounds__(NT) void back_flat(FlatArgs a) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;
   __shared__ double ys_all[WPB][NV * 2 * XR];
-  flat_zero(a);  // the input coefficients xt, for the next product
+  flat_zero(a, blockIdx.x, gridDim.x);  // the input coefficients xt, for
+                                        // the next product
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= a.nitems) return;
@@ -1423,6 +1456,27 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   const auto &out = H->side[trans ? 1 : 0];
   const bool flat = !trans && H->level_sync == 2 && in.d_WF.p &&
                     out.d_WB.p && H->d_xtf.p;
+  // flat, not symmetric: the near blocks run in the upward launch
+  const bool fuse = flat && !symm && H->nnear > 0 && !in.items.empty();
+  auto coupling_args = [&]() {
+    CNArgs c;
+    c.far = H->d_far.p;
+    c.nfar = H->nfar;
+    c.near = H->d_near.p;
+    c.nnear = H->nnear;
+    c.rn = H->side[0].d_nodes.p;
+    c.cn = H->side[1].d_nodes.p;
+    c.ridx = H->side[0].d_idx.p;
+    c.cidx = H->side[1].d_idx.p;
+    c.S = H->d_S.p;
+    c.N = H->d_N.p;
+    c.xt = xt;
+    c.yt = yt;
+    c.cstride = H->cstride;
+    c.v = vs;
+    c.cs = H->cpl;
+    return c;
+  };
   FlatArgs fa;
   if (flat) {
     // coefficient buffers of the flat products only: xt is zero on entry
@@ -1442,9 +1496,17 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     fa.zero = yt;
     fa.nzero = (C ? 2 : 1) * out.ncoef;
     fa.v = vs;
-    hipLaunchKernelGGL((fwd_flat<C, K, NV>),
-                       dim3((fa.nitems + WPB - 1) / WPB), dim3(NT), 0, st,
-                       fa);
+    const uint32_t fb = (fa.nitems + WPB - 1) / WPB;
+    if (fuse) {
+      CNArgs cn = coupling_args();
+      cn.nfar = 0;
+      hipLaunchKernelGGL((fwd_flat_near<C, K, NV>),
+                         dim3(fb + (H->nnear + WPB - 1) / WPB), dim3(NT), 0,
+                         st, fa, cn, fb);
+    } else {
+      hipLaunchKernelGGL((fwd_flat<C, K, NV>), dim3(fb), dim3(NT), 0, st,
+                         fa);
+    }
   }
   FwdArgs f;
   f.nodes = in.d_nodes.p;
@@ -1473,28 +1535,14 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     }
   }
 
-  CNArgs c;
-  c.far = H->d_far.p;
-  c.nfar = H->nfar;
-  c.near = H->d_near.p;
-  c.nnear = H->nnear;
-  c.rn = H->side[0].d_nodes.p;
-  c.cn = H->side[1].d_nodes.p;
-  c.ridx = H->side[0].d_idx.p;
-  c.cidx = H->side[1].d_idx.p;
-  c.S = H->d_S.p;
-  c.N = H->d_N.p;
-  c.xt = xt;
-  c.yt = yt;
-  c.cstride = H->cstride;
-  c.v = vs;
-  const uint32_t nw = H->nfar + H->nnear;
+  CNArgs c = coupling_args();
+  if (fuse) c.nnear = 0;  // done in the upward launch
+  const uint32_t nw = c.nfar + c.nnear;
   if (nw && symm) {
     if constexpr (NV == 1)
       hipLaunchKernelGGL((couple_symm<C>), dim3((nw + WPB - 1) / WPB),
                          dim3(NT), 0, st, c);
   } else if (nw) {
-    c.cs = H->cpl;
     if (trans)
       hipLaunchKernelGGL((couple_near<C, true, NV>),
                          dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);

@@ -4,9 +4,10 @@ Requests (a mesh, Laplace or Helmholtz, the wave number, one to four unit
 point sources) arrive over time.  Assembled operators stay on the device in
 an operator cache; a request is admitted when its operators are cached, or
 when they fit into the device-memory budget after evicting idle ones (least
-recently used first), and when a slot of its operators' batched GMRES is
-free.  Each cached operator pair owns a stream and a batched GMRES
-(h2hip_gmres_batch, up to 8 solves sharing every product); chunks of
+recently used first), and when a slot of its operators' batched solver is
+free.  Each cached operator pair owns a stream and a batched solver
+(h2hip_gmres_batch, up to 8 solves sharing every product: CG for Laplace,
+GMRES for Helmholtz); chunks of
 different operators are launched without waiting, so solves on different
 meshes overlap on the device, and a request enters between two chunks of
 the solves already running.  New operators are built by a worker thread
@@ -91,8 +92,13 @@ class Server:
     """The request loop; meshes: {mesh_id: Mesh}."""
 
     def __init__(self, meshes, budget_bytes, slots=8, kmax=30, eps=1e-8,
-                 maxiter=1000, clf=64, m_laplace=4, m_helmholtz=4):
+                 maxiter=1000, clf=64, m_laplace=4, m_helmholtz=4,
+                 laplace_method="cg"):
+        """laplace_method: batched solver of the Laplace requests, "cg"
+        (V is symmetric positive definite; the reference Laplace test's
+        solver) or "gmres"; Helmholtz requests use GMRES(kmax)."""
         self.meshes = meshes
+        self.laplace_method = laplace_method
         self.budget = int(budget_bytes)
         self.slots, self.kmax, self.eps, self.maxiter = slots, kmax, eps, \
             maxiter
@@ -119,15 +125,15 @@ class Server:
         mesh = self.meshes[mesh_id]
         if kind == LAPLACE:
             prob = LaplaceDirichlet(mesh, clf=self.clf, m=self.m[kind])
-            op = prob.V
+            op, method = prob.V, self.laplace_method
         else:
             prob = HelmholtzBrakhageWerner(mesh, k, clf=self.clf,
                                            eta_adm=2.0, m=self.m[kind],
                                            stream=stream)
             stream.sync()
-            op = prob.A
+            op, method = prob.A, "gmres"
         batch = GmresBatch(op, slots=self.slots, kmax=self.kmax,
-                           stream=stream)
+                           stream=stream, method=method)
         out["entry"] = Entry(key, prob, op, batch, stream,
                              self._footprint(key), time.perf_counter() - t0)
 

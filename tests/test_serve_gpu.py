@@ -2,9 +2,9 @@
 
 * Each request's solution is the same in the mix (admitted while solves on
   other meshes and problems run, batched with other requests of its
-  operators) as solved alone with the one-system GMRES on the same
-  operators (V for Laplace, the Brakhage-Werner operator KM - i k V for
-  Helmholtz): to 1e-8 relative (both stop at residual 1e-10 |b|; the
+  operators) as solved alone with the one-system solver of its method on
+  the same operators (CG with V for Laplace, GMRES with the Brakhage-Werner
+  operator KM - i k V for Helmholtz): to 1e-8 relative (both stop at residual 1e-10 |b|; the
   batched product sums in another order).
 * With a device-memory budget that holds about one operator pair, idle
   operators are evicted and rebuilt on demand, and every request still
@@ -26,7 +26,7 @@ def catalog():
 
 def check_against_alone(srv, reqs, tol):
     import h2lib_amd as h
-    from h2lib_amd.solve import solve_gmres
+    from h2lib_amd.solve import solve_cg, solve_gmres
     for r in reqs:
         e = srv.cache.get(r.key)
         if e is None:  # evicted: rebuild through the server's builder
@@ -36,7 +36,10 @@ def check_against_alone(srv, reqs, tol):
         f, b = e.prob.rhs_sources(r.sources, e.stream)
         x = h.DeviceArray(e.prob.n, b.dtype)
         x.zero()
-        it, _ = solve_gmres(e.op, b, x, 1e-10, 3000, 30, e.stream)
+        if e.batch.method == "cg":
+            it, _ = solve_cg(e.op, b, x, 1e-10, 3000, stream=e.stream)
+        else:
+            it, _ = solve_gmres(e.op, b, x, 1e-10, 3000, 30, e.stream)
         xa = x.download()
         err = np.abs(r.x - xa).max() / np.abs(xa).max()
         print("request", r.rid, r.key, "steps", r.steps, it,
