@@ -476,6 +476,35 @@ def bench_dirichlet(h, mesh, args, stream):
     for H in (prob.V, prob.KM):
         H.set_schedule(best)
     mv_ms = mv_by[best]
+    # the same product with the couplings computed from the interpolation
+    # points instead of read (no coupling storage)
+    mv_mf = None
+    try:
+        prob.V.set_coupling_mode(1)
+    except h.H2HipError as exc:  # host-built couplings: no points kept
+        log("bench: computed couplings unavailable: %s" % exc)
+    else:
+        for _ in range(3):
+            prob.V.addeval(1.0, d_x, d_y, stream)
+        e0, e1 = h.Event(), h.Event()
+        stream.sync()
+        e0.record(stream)
+        for _ in range(reps):
+            prob.V.addeval(1.0, d_x, d_y, stream)
+        e1.record(stream)
+        stream.sync()
+        mv_mf = {"ms": e0.elapsed_ms(e1) / reps,
+                 "bytes": int(sum(prob.V.bytes().values()) + 3 * 8 * n)}
+        prob.V.set_coupling_mode(0)
+    # the faster of the two serves the solves (both operators)
+    computed = mv_mf is not None and mv_mf["ms"] < mv_ms
+    if computed:
+        prob.V.set_coupling_mode(1)
+        try:
+            prob.KM.set_coupling_mode(1)
+        except h.H2HipError as exc:
+            log("bench: double layer operator keeps stored couplings: %s"
+                % exc)
     # the single layer matrix in symmetric storage (blocks on and below the
     # block diagonal) and its self-adjoint product
     from h2lib_amd.h2matrix import H2Matrix
@@ -574,6 +603,8 @@ def bench_dirichlet(h, mesh, args, stream):
             "ms_by_schedule": {"counters": mv_by[0], "level_sync": mv_by[1],
                                "flat": mv_by[2]},
             "schedule": ("counters", "level_sync", "flat")[best],
+            "computed_couplings": mv_mf,
+            "couplings_in_solves": "computed" if computed else "stored",
             "symmetric": {"ms": mvs_ms, "bytes": int(mvs_bytes),
                           "GB/s": mvs_bytes / (mvs_ms * 1e-3) / 1e9,
                           "storage_bytes": int(sum(nbs.values())),

@@ -399,13 +399,22 @@ __global__ __launch_bounds__(NT) void fwd_level(FwdArgs a,
 // assemble_bem3d_inter_uniform fills them (Library/bem3d.c:6079-6108).
 // Shared by the kernel that stores the couplings and the product that
 // recomputes them, so that both produce the same values.
+// 1/sqrt(x) for positive finite x: hardware estimate and one third-order
+// correction y (1 + e/2 + 3 e^2/8), e = 1 - x y^2, as in the near-field
+// kernels (a fraction of the instructions of a square root and a divide)
+__device__ __forceinline__ double cpl_rsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 __device__ __forceinline__ void cpl_value(double d0, double d1, double d2,
                                           int helmholtz, double kr,
                                           double kdamp, double &sr,
                                           double &si) {
   constexpr double KCONST = 0.0795774715459476679;
   const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-  double rs = 1.0 / sqrt(r2);
+  double rs = cpl_rsqrt(r2);
   if (!helmholtz) {
     sr = KCONST * rs;
     si = 0.0;
