@@ -492,9 +492,10 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
 // yt_r += S_rc xt_c and y|_r += alpha N_rc x|_c; TRANS: yt_c += S^* xt_r and
 // y|_c += alpha N^* x|_r.  One wave per block; the wave first gathers its
 // block's input entries into its own LDS slice.
-// MF (one vector, not transposed): the couplings are computed from the
-// interpolation points instead of read (cpl_value), lane i keeping row
-// point i and stepping through the column points j = (ja m + jb) m + jc.
+// MF (not transposed): the couplings are computed from the interpolation
+// points instead of read (cpl_value), lane i keeping row point i and
+// stepping through the column points j = (ja m + jb) m + jc; each value
+// serves all NV vectors.
 // Item w of the wave (w < nfar: far block w, else near block w - nfar);
 // xs: the wave's LDS slice of NV * 2 * XR doubles.
 template <int C, bool TRANS, int NV, bool MF = false>
@@ -537,11 +538,15 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
               double sr, si;
               cpl_value(d0, d1, pz - pc[2 * m + jc], a.cs.helmholtz, a.cs.kr,
                         a.cs.kdamp, sr, si);
-              const double xr = xs[2 * j], xi = xs[2 * j + 1];
-              ar[0] = fma(sr, xr, ar[0]);
-              if constexpr (C) {
-                ar[0] = fma(-si, xi, ar[0]);
-                ai[0] = fma(sr, xi, fma(si, xr, ai[0]));
+#pragma unroll
+              for (int v = 0; v < NV; v++) {
+                const double xr = xs[v * 2 * XR + 2 * j];
+                const double xi = xs[v * 2 * XR + 2 * j + 1];
+                ar[v] = fma(sr, xr, ar[v]);
+                if constexpr (C) {
+                  ar[v] = fma(-si, xi, ar[v]);
+                  ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
+                }
               }
             }
           }
@@ -1555,8 +1560,8 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     if (trans)
       hipLaunchKernelGGL((couple_near<C, true, NV>),
                          dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
-    else if (NV == 1 && H->cpl_recompute)
-      hipLaunchKernelGGL((couple_near<C, false, NV, NV == 1>),
+    else if (H->cpl_recompute)
+      hipLaunchKernelGGL((couple_near<C, false, NV, true>),
                          dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
     else
       hipLaunchKernelGGL((couple_near<C, false, NV>),

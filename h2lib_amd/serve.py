@@ -93,12 +93,17 @@ class Server:
 
     def __init__(self, meshes, budget_bytes, slots=8, kmax=30, eps=1e-8,
                  maxiter=1000, clf=64, m_laplace=4, m_helmholtz=4,
-                 laplace_method="cg"):
+                 laplace_method="cg", computed_couplings=True):
         """laplace_method: batched solver of the Laplace requests, "cg"
         (V is symmetric positive definite; the reference Laplace test's
-        solver) or "gmres"; Helmholtz requests use GMRES(kmax)."""
+        solver) or "gmres"; Helmholtz requests use GMRES(kmax).
+        computed_couplings: the Laplace products compute their couplings
+        from the interpolation points instead of reading them (measured
+        faster for the Laplace kernel; Helmholtz operators read them: their
+        kernel adds a sine and a cosine per entry)."""
         self.meshes = meshes
         self.laplace_method = laplace_method
+        self.computed_couplings = computed_couplings
         self.budget = int(budget_bytes)
         self.slots, self.kmax, self.eps, self.maxiter = slots, kmax, eps, \
             maxiter
@@ -126,6 +131,8 @@ class Server:
         if kind == LAPLACE:
             prob = LaplaceDirichlet(mesh, clf=self.clf, m=self.m[kind])
             op, method = prob.V, self.laplace_method
+            if self.computed_couplings:
+                prob.V.set_coupling_mode(1)
         else:
             prob = HelmholtzBrakhageWerner(mesh, k, clf=self.clf,
                                            eta_adm=2.0, m=self.m[kind],

@@ -219,13 +219,14 @@ int h2hip_addevalsymm_h2matrix(const double *alpha, const h2hip_h2matrix *H,
  * thread-safe against products in flight.  No counterpart in the
  * reference. */
 int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync);
-/* Couplings of the products y += alpha H x (h2hip_addeval_h2matrix, one
- * vector): recompute = 0 reads the stored couplings, 1 computes each entry
- * in the product from the interpolation points kept by
- * h2hip_h2matrix_interp_couplings (the same arithmetic, so the same
- * values), which trades the product's largest read (k^2 doubles per
- * admissible block) for k^2 kernel evaluations.  The other products keep
- * reading the stored couplings.  Needs couplings set by
+/* Couplings of the products y += alpha H x (h2hip_addeval_h2matrix and
+ * h2hip_addeval_h2matrix_batch, not transposed): recompute = 0 reads the
+ * stored couplings, 1 computes each entry in the product from the
+ * interpolation points kept by h2hip_h2matrix_interp_couplings (the same
+ * arithmetic, so the same values), which trades the product's largest read
+ * (k^2 doubles per admissible block) for k^2 kernel evaluations, shared by
+ * all vectors of a batched product.  Transposed and self-adjoint products
+ * keep reading the stored couplings.  Needs couplings set by
  * h2hip_h2matrix_interp_couplings; h2hip_h2matrix_set_couplings switches
  * back to 0.  h2hip_h2matrix_bytes then counts no coupling bytes.  No
  * counterpart in the reference. */

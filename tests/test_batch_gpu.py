@@ -120,3 +120,41 @@ def test_gmres_batch_same_alone_and_in_mix():
         x_alone, it_alone = alone[t]
         assert rel(d_xs[t].download(), x_alone) < 1e-8, t
         assert abs(done[t] - it_alone) <= 1, t
+
+
+@pytest.mark.parametrize("kern", [0, 1])
+def test_batch_product_computed_couplings(kern):
+    """Batched products with the couplings computed from the interpolation
+    points (each value shared by all vectors) against the batched products
+    reading the stored couplings: 1e-13, real (Laplace) and complex
+    (Helmholtz, k = 1.5) kernels, both transform schedules."""
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.bem3d import Surface3d
+    from h2lib_amd.h2matrix import H2Matrix
+    mesh = G.sphere(8)
+    gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+    op = (h.new_slp_laplace_bem3d(gr, 2, 4, 1, 1) if kern == 0 else
+          h.new_slp_helmholtz_bem3d(1.5, gr, 2, 4, 0, 0))
+    rt = G.ClusterTree(mesh, 1 if kern == 0 else 0, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    H = H2Matrix(op, mesh, rt, rt, bt, 3)
+    n = rt.n
+    rng = np.random.default_rng(21 + kern)
+    for nvec in (2, 5, 8):
+        xs = [rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n)
+                                        if kern else 0) for _ in range(nvec)]
+        alphas = rng.uniform(-2, 2, nvec) + (0.5j if kern else 0)
+        d_x = [h.DeviceArray.from_host(x) for x in xs]
+        for sched in (0, 2):
+            H.set_schedule(sched)
+            got = []
+            for mode in (0, 1):
+                H.set_coupling_mode(mode)
+                d_y = [h.DeviceArray.from_host(np.zeros_like(x)) for x in xs]
+                H.addeval_batch(alphas, d_x, d_y)
+                got.append([d.download() for d in d_y])
+            H.set_coupling_mode(0)
+            print("computed couplings batch", kern, nvec, sched, flush=True)
+            for v in range(nvec):
+                assert rel(got[1][v], got[0][v]) < 1e-13, (nvec, sched, v)
