@@ -458,7 +458,7 @@ def bench_dirichlet(h, mesh, args, stream):
     # the three schedules of the basis transforms; the fastest serves the
     # solves
     mv_by = {}
-    for sched in (0, 1, 2):
+    for sched in args.schedules:
         for H in (prob.V, prob.KM):
             H.set_schedule(sched)
         for _ in range(3):
@@ -600,8 +600,9 @@ def bench_dirichlet(h, mesh, args, stream):
                         "(Laplace SLP)"),
             "ms": mv_ms, "bytes": int(mv_bytes), "GB/s": mv_gbs,
             "hbm_frac": mv_gbs / 8000.0,
-            "ms_by_schedule": {"counters": mv_by[0], "level_sync": mv_by[1],
-                               "flat": mv_by[2]},
+            "ms_by_schedule": {name: mv_by[i] for i, name in
+                               enumerate(("counters", "level_sync", "flat"))
+                               if i in mv_by},
             "schedule": ("counters", "level_sync", "flat")[best],
             "computed_couplings": mv_mf,
             "couplings_in_solves": "computed" if computed else "stored",
@@ -743,6 +744,11 @@ def main():
     ap.add_argument("--multi-check", type=int, default=6,
                     help="requests whose solutions are checked against a "
                     "tight solve")
+    ap.add_argument("--schedules", type=int, nargs="+", default=[0, 1, 2],
+                    choices=(0, 1, 2),
+                    help="transform schedules of the H2 product timed "
+                    "(0 counter climb, 1 level-synchronous, 2 flat); the "
+                    "fastest serves the solves")
     ap.add_argument("--dirichlet-solver", choices=("cg", "gmres"),
                     default="cg",
                     help="solver of the Dirichlet solves/s line (CG is the "
