@@ -246,6 +246,14 @@ def bench_batched(h, args, stream):
 
     t0 = time.perf_counter()
     mats = mm.h2matrices(out, 3)
+    # transforms of every mesh's products (flat where the tree allows it)
+    sched = []
+    for H in mats:
+        try:
+            H.set_schedule(args.batch_schedule)
+        except h.H2HipError:
+            H.set_schedule(1)
+        sched.append(H.level_sync)
     t_h2 = time.perf_counter() - t0
     G = GmresMulti(mats, kmax=30, stream=stream)
     # b_i = integral of 1 over triangle i = g_i / 2 (original DOF order)
@@ -288,6 +296,7 @@ def bench_batched(h, args, stream):
                           "quadrature), so restarted GMRES stagnates; the "
                           "timing per step is unaffected")
     gm["graph_captures"] = G.graph_builds
+    gm["transform_schedules"] = {str(k): sched.count(k) for k in set(sched)}
     gm["speedup_graph_vs_direct"] = gm["direct"]["s"] / gm["graph"]["s"]
     tri = [sp.triangles for sp in specs]
     return {
@@ -731,6 +740,9 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=64,
                     help="meshes of the batched run (0: skip it)")
     ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-schedule", type=int, default=2, choices=(0, 1, 2),
+                    help="transform schedule of the batched meshes' H2 "
+                    "products (2 flat, 1 level-synchronous, 0 counters)")
     ap.add_argument("--batch-maxiter", type=int, default=300,
                     help="GMRES step limit of the batched solves (the "
                     "cylinder meshes give indefinite matrices at q=2 q2=4 "
