@@ -50,11 +50,14 @@ def test_joined_plan_matches_per_mesh_plans():
         assert np.array_equal(flat[a:b], o1.download()), i
 
 
-def test_graph_gmres_matches_direct_and_single():
+@pytest.mark.parametrize("schedule", [0, 2])
+def test_graph_gmres_matches_direct_and_single(schedule):
     # The H2 product adds coupling and near-field contributions with FP64
     # atomics, so two runs agree to rounding only and a step count may move
     # by one at the threshold; every solution meets the stopping rule on
-    # its true residual.
+    # its true residual.  schedule 2: the flat transforms, whose coefficient
+    # buffers are cleared by the products themselves, also across graph
+    # replays.
     import h2lib_amd as h
     from h2lib_amd.solve import GmresMulti, solve_gmres
     specs, mm = small_batch(4, seed=11)
@@ -64,6 +67,8 @@ def test_graph_gmres_matches_direct_and_single():
     mm.execute(op, out.ptr, stream.handle)
     stream.sync()
     mats = mm.h2matrices(out, 3)
+    for H in mats:
+        H.set_schedule(schedule)
     rng = np.random.default_rng(5)
     bs = [rng.standard_normal(H.n) for H in mats]
     eps = 1e-9
