@@ -118,10 +118,13 @@ class MultiMesh:
         buffer at d_out (op must live on self.gr)."""
         self.plan.execute(op, d_out, stream)
 
-    def h2matrices(self, d_out, m, dlp=False):
+    def h2matrices(self, d_out, m, dlp=False, schedule=2):
         """Device H2-matrices of the meshes by interpolation of order m
         (couplings of the Laplace kernel), their near fields copied from the
-        packed output d_out of execute (device to device)."""
+        packed output d_out of execute (device to device); products with
+        the transform schedule `schedule` (2 flat, measured fastest; 1
+        where a tree does not allow it)."""
+        from ._lib import H2HipError
         es = 8
         mats = []
         for i, mesh in enumerate(self.meshes):
@@ -131,5 +134,9 @@ class MultiMesh:
             H = H2Matrix.from_arrays(I, None, rt=tree)
             a, b = self.entry_range[i]
             H.copy_nearfield_from_device(d_out.ptr + es * a, b - a)
+            try:
+                H.set_schedule(schedule)
+            except H2HipError:
+                H.set_schedule(1)
             mats.append(H)
         return mats

@@ -120,10 +120,10 @@ def test_gmres_batch_same_alone_and_in_mix():
         x_alone, it_alone = alone[t]
         assert rel(d_xs[t].download(), x_alone) < 1e-8, t
         # the products sum with FP64 atomics, so rounding differs from run
-        # to run, and a solve that ends near a restart can end a few steps
-        # apart (the one-system solver gave 146 and 151 steps for the same
-        # system in two runs)
-        assert abs(done[t] - it_alone) <= max(2, it_alone // 20), t
+        # to run, and a solve whose threshold falls at a restart can end up
+        # to one restart cycle (kmax = 30 steps) apart (the one-system
+        # solver gave 146 and 151 steps for the same system in two runs)
+        assert abs(done[t] - it_alone) <= 30, t
 
 
 @pytest.mark.parametrize("kern", [0, 1])

@@ -95,15 +95,16 @@ def test_graph_gmres_matches_direct_and_single(schedule):
         xg, xd = sols[True][0][s], sols[False][0][s]
         assert np.abs(xg - xd).max() <= 1e-6 * np.abs(xd).max()
     # near the threshold the rounding of the atomic sums can move the
-    # stopping step of a slowly converging system by a few steps
-    allowed = np.maximum(2, np.ceil(0.03 * sols[False][1])).astype(int)
+    # stopping step of a slowly converging system, by up to one restart
+    # cycle when the threshold falls at a restart (seen: 97 and 101 steps)
+    allowed = np.maximum(10, np.ceil(0.03 * sols[False][1])).astype(int)
     assert np.all(np.abs(sols[True][1].astype(int) -
                          sols[False][1].astype(int)) <= allowed)
     for s, H in enumerate(mats):
         d_b = h.DeviceArray.from_host(bs[s])
         d_x = h.DeviceArray.from_host(np.zeros(H.n))
         it1, _ = solve_gmres(H, d_b, d_x, eps, 500, 10, stream)
-        assert abs(int(it1) - int(sols[True][1][s])) <= max(2, 0.03 * it1)
+        assert abs(int(it1) - int(sols[True][1][s])) <= max(10, 0.03 * it1)
         x1 = d_x.download()
         assert np.abs(x1 - sols[True][0][s]).max() <= 1e-6 * np.abs(x1).max()
     # a second solve with the same graphs and workspaces

@@ -45,7 +45,11 @@ def check_against_alone(srv, reqs, tol):
         print("request", r.rid, r.key, "steps", r.steps, it,
               "err %.2e" % err, flush=True)
         assert err < tol, r.rid
-        assert abs(int(r.steps) - int(it)) <= max(2, 0.03 * it)
+        # FP64 atomics in the products: CG steps agree to a few; a GMRES
+        # solve whose threshold falls at a restart can end up to one
+        # restart cycle (kmax = 30) apart
+        allowed = 30 if e.batch.method == "gmres" else max(2, 0.03 * it)
+        assert abs(int(r.steps) - int(it)) <= allowed
 
 
 def test_each_result_same_alone_and_in_mix():
