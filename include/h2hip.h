@@ -369,10 +369,12 @@ int h2hip_eval_brakhage_werner_l(const h2hip_surface *s, uint32_t q,
  * entry per vertex): the load vector with the single-triangle Gauss rule of
  * order q (1..8) and conjugate gradients on the mass matrix to
  * |r| <= 1e-15 |b| (at most 500 steps), all on the device, ordered on the
- * stream, no host synchronisation.  projectL2_bem3d_l_avector
- * (Library/bem3d.c:9009-9088) with eval_dirichlet_fundamental_laplacebem3d /
+ * stream; the stopping test runs on the device and the host reads its flag
+ * every 8 steps, so the call returns when d_out is complete.
+ * projectL2_bem3d_l_avector (Library/bem3d.c:9009-9088) with
+ * eval_dirichlet_fundamental_laplacebem3d /
  * eval_neumann_fundamental_laplacebem3d (Library/laplacebem3d.c:1390-1430).
- * Calls on one surface share a workspace and must not overlap. */
+ * Calls on one surface share a workspace and are serialised by a lock. */
 int h2hip_projectL2_point_source(const h2hip_surface *s, uint32_t q, int kind,
                                  const double *x0, double *d_out,
                                  void *stream);
