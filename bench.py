@@ -28,7 +28,11 @@ import os
 import sys
 import time
 
-import numpy as np
+# threads of the CPU baseline stay on neighbouring cores (read by the
+# OpenMP runtime when it starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -44,6 +48,83 @@ FLOPS_PER_EVAL = {0: 46.0, 1: 53.0}
 FLOPS_LL_PER_EVAL = 18.0
 
 
+SECTIONS = ("mixed", "helmholtz", "batched", "helmholtz_solve",
+            "serving_multi", "dense", "dirichlet")
+# the summary line stays well inside the tail of stdout that is kept
+LINE_MAX = 1900
+
+
+def r4(v):
+    """v to four significant digits (None stays None)."""
+    return None if v is None else float("%.4g" % v)
+
+
+def compact(d):
+    """Section summaries for the last stdout line (the detail file keeps
+    everything)."""
+    out = {}
+    dr = d.get("dirichlet")
+    if dr:
+        g, c = dr["gmres30"], dr["cg"]
+        out.update({
+            "dirichlet_solves_per_s": r4(g["solves_per_s"]),
+            "p50_ms": r4(g["p50_ms"]), "p99_ms": r4(g["p99_ms"]),
+            "rel_err": r4(g["rel_err"]),
+            "dirichlet": "GMRES(30) to 1e-10|b|, %d steps, data->solution "
+                         "on the host" % g["steps"],
+            "cg": {"solves_per_s": r4(c["solves_per_s"]),
+                   "p99_ms": r4(c["p99_ms"]), "rel_err": r4(c["rel_err"]),
+                   "steps": c["steps"]}})
+        for key, name in (("serving", "served"),
+                          ("serving_saturated", "served_capacity")):
+            sv = dr.get(key)
+            if sv:
+                out[name] = {"solves_per_s": r4(sv["solves_per_s"]),
+                             "p50_ms": r4(sv["p50_ms"]),
+                             "p99_ms": r4(sv["p99_ms"]),
+                             "rel_err": r4(sv.get("rel_err"))}
+        mv = dr["matvec"]
+        out["matvec"] = {"ms": r4(mv["stored"]["ms"]),
+                         "GBps": r4(mv["stored"]["GB/s"]),
+                         "frac": r4(mv["stored"]["frac"]),
+                         "traffic": mv.get("traffic"),
+                         "adjoint_ms": r4(mv["adjoint_ms"]),
+                         "computed_cpl_ms": r4(
+                             (mv["computed_couplings"] or {}).get("ms"))}
+    mp = d.get("mixed_precision")
+    if mp:
+        out["mixed"] = {"entries_per_s": r4(mp["entries_per_s"]),
+                        "err": r4(max(e["max_rel_to_max"] for e in
+                                      mp["entry_err_vs_fp64"].values()))}
+    hm = d.get("helmholtz_mixed")
+    if hm:
+        out["helmholtz"] = {k: [r4(v["fp64_entries_per_s"]),
+                                r4(v["mixed_entries_per_s"]),
+                                r4(v["entry_err_vs_fp64"]["max_rel_to_max"])]
+                            for k, v in hm["k"].items()}
+    hs = d.get("helmholtz_solve")
+    if hs:
+        out["helmholtz_solve"] = {
+            k: [v["fp64"]["steps"], r4(v["fp64"]["solve_s"]),
+                r4(v["mixed_solution_err_vs_fp64"])]
+            for k, v in hs["k"].items()}
+    bt = d.get("batched")
+    if bt:
+        out["batched"] = {"entries_per_s": r4(bt["entries_per_s"]),
+                          "graph_ms_per_step": r4(
+                              bt["gmres"]["graph"]["ms_per_step_all_meshes"])}
+    sm = d.get("serving_multi")
+    if sm:
+        out["serving_multi"] = {
+            name: [r4(sm[name]["solves_per_s"]), r4(sm[name]["p50_ms"]),
+                   r4(sm[name]["p99_ms"])] for name in ("cold", "warm")}
+    dn = d.get("dense")
+    if dn:
+        out["dense"] = {"entries_per_s": r4(dn["entries_per_s"]),
+                        "matvec_ms": r4(dn.get("matvec_ms"))}
+    return out
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -56,24 +137,57 @@ def build_workload(split, clf, eta):
     return mesh, tree, blocks, blocks.near_block_list()
 
 
-def cpu_baseline(mesh, tree, nb, entries):
-    """The reference library (built from its sources) on the host cores:
-    the SLP near field of all blocks, OpenMP over blocks."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh, tree, nb, entries, args):
+    """The reference library, built from its sources for speed (gcc -O3
+    -march=x86-64-v3, OpenMP), on the host cores: the single and double
+    layer near fields of all blocks (the work of one GPU step), OpenMP over
+    blocks with OMP_PROC_BIND=close, and a one-thread figure on every 16th
+    block."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ffi import ReferenceBem, have_reference
-    if not have_reference(False):
+    fast = have_reference(fast=True)
+    if not fast and not have_reference(False):
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
-    ref = ReferenceBem(0, 0, 1, 1, mesh.x, mesh.t, q=2, q2=4)
+    # the box's CPU share for one GPU is 16 cores, whatever the host shows
+    avail = len(os.sched_getaffinity(0))
+    threads = max(1, min(args.cpu_threads, avail))
+    ops = [ReferenceBem(0, 0, 1, 1, mesh.x, mesh.t, q=2, q2=4, fast=fast),
+           ReferenceBem(0, 1, 1, 1, mesh.x, mesh.t, q=2, q2=4, alpha=0.5,
+                        fast=fast)]
     t0 = time.perf_counter()
-    ref.nearfield_blocks(tree.idx, tree.idx, nb, threads)
+    for op in ops:
+        op.nearfield_blocks(tree.idx, tree.idx, nb, threads)
     dt = time.perf_counter() - t0
-    return {"value": entries / dt, "unit": "entries/s", "cores": threads,
-            "kind": "reference",
-            "sample": ("Laplace SLP near field of all %d near-field blocks "
-                       "(%d entries) of the same workload; reference library "
-                       "compiled from its sources with gcc -O2, OpenMP over "
-                       "blocks; %.2f s" % (len(nb), entries, dt))}
+    nb = np.asarray(nb)
+    sample = nb[::16]
+    ents1 = int(sum(int(b[1]) * int(b[3]) for b in sample)) * len(ops)
+    t0 = time.perf_counter()
+    for op in ops:
+        op.nearfield_blocks(tree.idx, tree.idx, sample, 1)
+    dt1 = time.perf_counter() - t0
+    return {"value": 2 * entries / dt, "unit": "entries/s", "cores": threads,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(), "affinity_cpus": avail,
+            "value_1thread": ents1 / dt1,
+            "build": ("gcc -O3 -march=x86-64-v3 -fopenmp" if fast else
+                      "gcc -O2 -ffp-contract=off -fopenmp"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "sample": ("Laplace SLP + DLP near field (linear basis) of all "
+                       "%d near-field blocks (%d entries) of the same "
+                       "workload, %.2f s on %d threads; one thread: every "
+                       "16th block (%d entries), %.2f s" %
+                       (len(nb), 2 * entries, dt, threads, ents1, dt1))}
 
 
 def bench_dense(h, stream, reps=20):
@@ -100,6 +214,21 @@ def bench_dense(h, stream, reps=20):
     e1.record(stream)
     stream.sync()
     ms = e0.elapsed_ms(e1) / reps
+    # the product of the assembled matrix with a seeded uniform[-1, 1]
+    # vector (addeval_amatrix_avector)
+    from h2lib_amd.amatrix import addeval_amatrix
+    d_x = h.DeviceArray.from_host(
+        np.random.default_rng(16).uniform(-1.0, 1.0, T))
+    d_y = h.DeviceArray(T)
+    d_y.zero()
+    for _ in range(3):
+        addeval_amatrix(1.0, out, T, T, T, d_x, d_y, stream=stream)
+    e0.record(stream)
+    for _ in range(reps):
+        addeval_amatrix(1.0, out, T, T, T, d_x, d_y, stream=stream)
+    e1.record(stream)
+    stream.sync()
+    mv_ms = e0.elapsed_ms(e1) / reps
     N = np.zeros((T, T), order="F")
     op.nearfield(None, None, N=N)
     t0 = time.perf_counter()
@@ -109,6 +238,8 @@ def bench_dense(h, stream, reps=20):
     return {"workload": "Laplace SLP cc, sphere 2048 triangles, dense "
                         "2048 x 2048, q=2 q2=4, FP64",
             "entries_per_s": T * T / (ms * 1e-3), "ms": ms,
+            "matvec_ms": mv_ms,
+            "matvec_GBps": 8.0 * (T * T + 3 * T) / (mv_ms * 1e-3) / 1e9,
             "entries_per_s_host_matrix": T * T / host_s,
             "host_matrix_ms": host_s * 1e3}
 
@@ -337,8 +468,7 @@ def bench_helmholtz_solve(h, args, stream):
     x0 = np.array([0.3, 0.2, 0.1])
     out = {"workload": ("exterior Helmholtz, unit sphere %d triangles, cc, "
                         "point source (0.3, 0.2, 0.1), GMRES(30) to 1e-8, "
-                        "interpolation order 4, clf 64, couplings computed "
-                        "in the products" % mesh.triangles),
+                        "interpolation order 4, clf 64" % mesh.triangles),
            "k": {}}
     for k in args.helm_k:
         res = {}
@@ -353,8 +483,10 @@ def bench_helmholtz_solve(h, args, stream):
             # 1.47 against 1.52 ms reading them, equal to rounding)
             try:
                 prob.V.set_coupling_mode(1)
+                mode = "computed"
             except h.H2HipError as exc:
                 log("bench: stored Helmholtz couplings: %s" % exc)
+                mode = "stored"
             t0 = time.perf_counter()
             _, d_b = prob.rhs(x0, stream)
             d_x, it, r = prob.solve(d_b, eps=1e-8, maxiter=args.helm_maxiter,
@@ -363,6 +495,7 @@ def bench_helmholtz_solve(h, args, stream):
             dt = time.perf_counter() - t0
             sol[name] = x
             res[name] = {"solve_s": dt, "steps": int(it),
+                         "single_layer_couplings": mode,
                          "operator_build_s": build,
                          "neumann_rel_err": prob.neumann_error(x, x0)}
             del prob
@@ -450,14 +583,40 @@ def bench_serving_multi(h, args):
     return out
 
 
+def time_product(h, H, d_x, d_y, stream, reps=50, symm=False):
+    """Mean milliseconds of one product y += H x (events on its stream)."""
+    f = H.addevalsymm if symm else H.addeval
+    for _ in range(3):
+        f(1.0, d_x, d_y, stream)
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(reps):
+        f(1.0, d_x, d_y, stream)
+    e1.record(stream)
+    stream.sync()
+    return e0.elapsed_ms(e1) / reps
+
+
+def product_line(H, ms, n):
+    """GB/s and roofline fraction of one product: algorithmic bytes (near
+    field, couplings, leaf and transfer matrices of both sides, vector
+    traffic) and the bytes the kernels read in the mode timed."""
+    alg = H.algorithmic_bytes()
+    rd = sum(H.bytes().values()) + 3 * 8 * n
+    return {"ms": ms, "algorithmic_bytes": int(alg), "read_bytes": int(rd),
+            "GB/s": alg / (ms * 1e-3) / 1e9,
+            "frac": alg / (ms * 1e-3) / 8e12}
+
+
 def bench_dirichlet(h, mesh, args, stream):
     """Dirichlet solves of the interior Laplace problem on the same mesh with
     the operators resident on the device: each solve projects the boundary
     data u = 1/|x - x0|, x0 = (1.2, 1.2, 1.2), forms b = (K + M/2) f and
-    solves V x = b by CG (the reference test's solver, its steps replayed as
-    captured graphs) or GMRES(30) (--dirichlet-solver); the latency of a
-    solve runs from the boundary data on the host to the solution on the
-    host."""
+    solves V x = b by GMRES(30) (the configuration's solver) and by CG (the
+    reference Laplace test's solver, its steps replayed as captured graphs);
+    the latency of a solve runs from the boundary data on the host to the
+    solution on the host."""
     from h2lib_amd.solve import LaplaceDirichlet
     x0 = np.array([1.2, 1.2, 1.2])
     t0 = time.perf_counter()
@@ -465,62 +624,56 @@ def bench_dirichlet(h, mesh, args, stream):
     h.hip().h2hip_stream_sync(None)
     build_s = time.perf_counter() - t0
 
-    # matvec of the single layer operator, events on its stream
+    # products of the single layer operator, events on its stream
     n = prob.n
     d_x = h.DeviceArray.from_host(
         np.random.default_rng(1).uniform(-1.0, 1.0, n))
     d_y = h.DeviceArray(n)
     d_y.zero()
-    # the three schedules of the basis transforms; the fastest serves the
-    # solves
+    # the three schedules of the basis transforms, stored couplings; the
+    # fastest serves the solves
     mv_by = {}
     for sched in args.schedules:
         for H in (prob.V, prob.KM):
             H.set_schedule(sched)
-        for _ in range(3):
-            prob.V.addeval(1.0, d_x, d_y, stream)
-        reps = 50
-        e0, e1 = h.Event(), h.Event()
-        stream.sync()
-        e0.record(stream)
-        for _ in range(reps):
-            prob.V.addeval(1.0, d_x, d_y, stream)
-        e1.record(stream)
-        stream.sync()
-        mv_by[sched] = e0.elapsed_ms(e1) / reps
+        mv_by[sched] = time_product(h, prob.V, d_x, d_y, stream)
     best = min(mv_by, key=mv_by.get)
     for H in (prob.V, prob.KM):
         H.set_schedule(best)
-    mv_ms = mv_by[best]
+    stored = product_line(prob.V, mv_by[best], n)
+    # the adjoint product y += V^* x on the same schedule
+    for _ in range(3):
+        prob.V.addeval(1.0, d_x, d_y, stream, trans=True)
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(50):
+        prob.V.addeval(1.0, d_x, d_y, stream, trans=True)
+    e1.record(stream)
+    stream.sync()
+    trans_ms = e0.elapsed_ms(e1) / 50
     # the same product with the couplings computed from the interpolation
-    # points instead of read (no coupling storage)
-    mv_mf = None
+    # points instead of read
+    computed = None
     try:
         prob.V.set_coupling_mode(1)
     except h.H2HipError as exc:  # host-built couplings: no points kept
         log("bench: computed couplings unavailable: %s" % exc)
     else:
-        for _ in range(3):
-            prob.V.addeval(1.0, d_x, d_y, stream)
-        e0, e1 = h.Event(), h.Event()
-        stream.sync()
-        e0.record(stream)
-        for _ in range(reps):
-            prob.V.addeval(1.0, d_x, d_y, stream)
-        e1.record(stream)
-        stream.sync()
-        mv_mf = {"ms": e0.elapsed_ms(e1) / reps,
-                 "bytes": int(sum(prob.V.bytes().values()) + 3 * 8 * n)}
+        computed = product_line(prob.V,
+                                time_product(h, prob.V, d_x, d_y, stream),
+                                n)
         prob.V.set_coupling_mode(0)
     # the faster of the two serves the solves (both operators)
-    computed = mv_mf is not None and mv_mf["ms"] < mv_ms
-    if computed:
-        prob.V.set_coupling_mode(1)
-        try:
-            prob.KM.set_coupling_mode(1)
-        except h.H2HipError as exc:
-            log("bench: double layer operator keeps stored couplings: %s"
-                % exc)
+    use_computed = computed is not None and computed["ms"] < stored["ms"]
+    modes = {"V": "stored", "KM": "stored"}
+    if use_computed:
+        for name, H in (("V", prob.V), ("KM", prob.KM)):
+            try:
+                H.set_coupling_mode(1)
+                modes[name] = "computed"
+            except h.H2HipError as exc:
+                log("bench: %s keeps stored couplings: %s" % (name, exc))
     # the single layer matrix in symmetric storage (blocks on and below the
     # block diagonal) and its self-adjoint product
     from h2lib_amd.h2matrix import H2Matrix
@@ -528,21 +681,10 @@ def bench_dirichlet(h, mesh, args, stream):
                   symmetric=True, stream=stream)
     Vs.set_schedule(best)
     stream.sync()
-    for _ in range(3):
-        Vs.addevalsymm(1.0, d_x, d_y, stream)
-    e0, e1 = h.Event(), h.Event()
-    stream.sync()
-    e0.record(stream)
-    for _ in range(reps):
-        Vs.addevalsymm(1.0, d_x, d_y, stream)
-    e1.record(stream)
-    stream.sync()
-    mvs_ms = e0.elapsed_ms(e1) / reps
-    nbs = Vs.bytes()
-    mvs_bytes = sum(nbs.values()) + 3 * 8 * n
-    nbytes = prob.V.bytes()
-    mv_bytes = sum(nbytes.values()) + 3 * 8 * n
-    mv_gbs = mv_bytes / (mv_ms * 1e-3) / 1e9
+    symm = product_line(Vs, time_product(h, Vs, d_x, d_y, stream,
+                                         symm=True), n)
+    symm["device_bytes"] = Vs.storage()["device"]
+    symm["device_bytes_full"] = prob.V.storage()["device"]
 
     ref = None
     gpath = os.path.join(ROOT, "tests", "golden", "dirichlet_sphere_s64.npz")
@@ -550,89 +692,80 @@ def bench_dirichlet(h, mesh, args, stream):
             os.path.exists(gpath):
         ref = np.load(gpath)
 
-    lat, steps = [], []
-    x = None
-    for r in range(args.solve_warmup + args.solves):
-        t = time.perf_counter()
-        d_f, d_b = prob.rhs(x0, stream)
-        d_sol, it, res = prob.solve(d_b, eps=args.eps, maxiter=1000, kmax=30,
-                                    stream=stream,
-                                    solver=args.dirichlet_solver,
-                                    use_graph=not args.cg_direct)
-        x = d_sol.download()
-        dt = time.perf_counter() - t
-        if r >= args.solve_warmup:
-            lat.append(dt)
-            steps.append(it)
-    lat = np.array(lat)
+    def rel(x):
+        if ref is None:
+            return None
+        return float(np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"]))
 
-    # solve only (b resident): GMRES(30) against the reference test's own
-    # solver, CG, on the full matrix and on the symmetric storage (exactly
-    # symmetric, which CG assumes), CG steps replayed as captured graphs
-    from h2lib_amd.solve import solve_cg, solve_gmres
-    d_f, d_b = prob.rhs(x0, stream)
-    solvers = {}
-    for name, H, use_cg in (("gmres30_full", prob.V, False),
-                            ("cg_full", prob.V, True),
-                            ("cg_symmetric", Vs, True)):
-        ts, its = [], []
-        for r in range(1 + max(1, args.solves // 4)):
-            d_c = h.DeviceArray(n)
-            d_c.zero(stream)
+    # solves: boundary data on the host -> solution on the host
+    lines = {}
+    x = None
+    for solver in ("gmres", "cg"):
+        lat, steps = [], []
+        for r in range(args.solve_warmup + args.solves):
             t = time.perf_counter()
-            if use_cg:
-                it, _ = solve_cg(H, d_b, d_c, args.eps, 1000,
-                                 not args.cg_direct, stream)
-            else:
-                it, _ = solve_gmres(H, d_b, d_c, args.eps, 1000, 30, stream)
+            d_f, d_b = prob.rhs(x0, stream)
+            d_sol, it, res = prob.solve(d_b, eps=args.eps, maxiter=1000,
+                                        kmax=30, stream=stream,
+                                        solver=solver,
+                                        use_graph=not args.cg_direct)
+            x = d_sol.download()
             dt = time.perf_counter() - t
-            if r:
-                ts.append(dt)
-                its.append(it)
-        xc = d_c.download()
-        e = {"solve_ms": float(np.median(ts) * 1e3),
-             "steps": int(np.median(its)),
-             "neumann_rel_err": prob.neumann_error(xc, x0)}
-        if ref is not None:
-            e["rel_err"] = float(np.linalg.norm(xc - ref["x"]) /
-                                 np.linalg.norm(ref["x"]))
-        solvers[name] = e
+            if r >= args.solve_warmup:
+                lat.append(dt)
+                steps.append(it)
+        lat = np.array(lat)
+        lines[solver] = {
+            "solves_per_s": len(lat) / lat.sum(),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "solves": len(lat), "steps": int(np.median(steps)),
+            "rel_err": rel(x),
+            "neumann_rel_err": prob.neumann_error(x, x0)}
+
+    # solve only (b resident), the symmetric storage with CG as well
+    from h2lib_amd.solve import solve_cg
+    d_f, d_b = prob.rhs(x0, stream)
+    ts, its = [], []
+    for r in range(1 + max(1, args.solves // 4)):
+        d_c = h.DeviceArray(n)
+        d_c.zero(stream)
+        t = time.perf_counter()
+        it, _ = solve_cg(Vs, d_b, d_c, args.eps, 1000, not args.cg_direct,
+                         stream)
+        dt = time.perf_counter() - t
+        if r:
+            ts.append(dt)
+            its.append(it)
+    cg_symm = {"solve_ms": float(np.median(ts) * 1e3),
+               "steps": int(np.median(its)), "rel_err": rel(d_c.download())}
 
     out = {
-        "solves_per_s": len(lat) / lat.sum(),
-        "p50_ms": float(np.percentile(lat, 50) * 1e3),
-        "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "solves": len(lat),
-        "solver": ("CG, the reference test's solver" if
-                   args.dirichlet_solver == "cg" else "GMRES(30)"),
-        "eps": args.eps,
-        "steps": int(np.median(steps)),
-        "neumann_rel_err": prob.neumann_error(x, x0),
+        "workload": ("interior Laplace Dirichlet problem, unit sphere %d "
+                     "triangles, linear basis, H2 by interpolation m=4 "
+                     "clf=%d eta=%g, u = 1/|x - (1.2,1.2,1.2)|, solves to "
+                     "%g |b|" % (mesh.triangles, args.clf, args.eta,
+                                 args.eps)),
+        "gmres30": lines["gmres"], "cg": lines["cg"],
+        "cg_symmetric_b_resident": cg_symm,
         "operator_build_s": build_s,
+        "couplings_in_solves": modes,
         "matvec": {
-            "kernels": ("fwd_flat + couple_near + back_flat (Laplace SLP)"
-                        if best == 2 else
+            "kernels": ("fwd_flat_near + couple_near + back_flat (Laplace "
+                        "SLP)" if best == 2 else
                         "fwd_up (+ fwd_level) + couple_near + down_back "
                         "(Laplace SLP)"),
-            "ms": mv_ms, "bytes": int(mv_bytes), "GB/s": mv_gbs,
-            "hbm_frac": mv_gbs / 8000.0,
+            "schedule": ("counters", "level_sync", "flat")[best],
             "ms_by_schedule": {name: mv_by[i] for i, name in
                                enumerate(("counters", "level_sync", "flat"))
                                if i in mv_by},
-            "schedule": ("counters", "level_sync", "flat")[best],
-            "computed_couplings": mv_mf,
-            "couplings_in_solves": "computed" if computed else "stored",
-            "symmetric": {"ms": mvs_ms, "bytes": int(mvs_bytes),
-                          "GB/s": mvs_bytes / (mvs_ms * 1e-3) / 1e9,
-                          "storage_bytes": int(sum(nbs.values())),
-                          "storage_bytes_full": int(sum(nbytes.values()))}},
-        "solvers": solvers,
+            "stored": stored, "computed_couplings": computed,
+            "adjoint_ms": trans_ms, "symmetric": symm},
     }
     if ref is not None:
-        xr = ref["x"]
-        out["rel_err"] = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
         out["rel_err_reference_eps8"] = float(
-            np.linalg.norm(ref["x_eps8"] - xr) / np.linalg.norm(xr))
+            np.linalg.norm(ref["x_eps8"] - ref["x"]) /
+            np.linalg.norm(ref["x"]))
         out["rel_err_against"] = ("the reference library's solution of the "
                                   "same discrete problem at residual "
                                   "1e-12 |b|")
@@ -768,13 +901,21 @@ def main():
                     help="transform schedules of the H2 product timed "
                     "(0 counter climb, 1 level-synchronous, 2 flat); the "
                     "fastest serves the solves")
-    ap.add_argument("--dirichlet-solver", choices=("cg", "gmres"),
-                    default="cg",
-                    help="solver of the Dirichlet solves/s line (CG is the "
-                    "reference Laplace test's)")
     ap.add_argument("--cg-direct", action="store_true",
                     help="CG timings with direct launches instead of "
                          "graph-replayed chunks")
+    ap.add_argument("--sections", nargs="*", default=list(SECTIONS),
+                    choices=SECTIONS,
+                    help="sections to run besides the headline near field "
+                    "(one section per profiler run keeps kernel averages "
+                    "apart)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the CPU baseline (at most the CPUs "
+                    "this process may run on)")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out",
+                                                     "bench_detail.json"),
+                    help="file for the full results of every section (the "
+                    "last stdout line is the compact summary)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this size "
                     "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -783,6 +924,7 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    sections = set(args.sections)
 
     import h2lib_amd as h
 
@@ -847,6 +989,8 @@ def main():
     value = entries_per_step / (ms_per_step * 1e-3)
     k_avg = kms.mean(axis=0)
     achieved = sum(flops) / (k_avg.sum() * 1e-3) / 1e12
+    log("bench: near field %.3e entries/s, %.3f ms per step, %.1f TFLOP/s" %
+        (value, ms_per_step, achieved))
 
     # HBM bytes per launch pair from the committed counter summary
     # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a plain copy); valid for
@@ -860,33 +1004,28 @@ def main():
             counters = json.load(f)
         traffic = counters.get("bytes_per_launch")
 
-    log("bench: mixed precision, same blocks")
-    mixed = bench_mixed(h, ops, plan, outs, stream, args.steps)
-    helm = None
-    if args.helm_split > 0:
+    detail = {}
+    if "mixed" in sections:
+        log("bench: mixed precision, same blocks")
+        detail["mixed_precision"] = bench_mixed(h, ops, plan, outs, stream,
+                                                args.steps)
+    if "helmholtz" in sections and args.helm_split > 0:
         log("bench: Helmholtz near field, FP64 and mixed")
-        helm = bench_helmholtz_mixed(h, args, stream)
-
-    batched = None
-    if args.batch_meshes > 0:
+        detail["helmholtz_mixed"] = bench_helmholtz_mixed(h, args, stream)
+    if "batched" in sections and args.batch_meshes > 0:
         log("bench: %d meshes in one launch set" % args.batch_meshes)
-        batched = bench_batched(h, args, stream)
-
-    helm_solve = None
-    if args.helm_split > 0 and args.helm_k:
+        detail["batched"] = bench_batched(h, args, stream)
+    if "helmholtz_solve" in sections and args.helm_split > 0 and \
+            args.helm_k:
         log("bench: exterior Helmholtz solves, FP64 and mixed")
-        helm_solve = bench_helmholtz_solve(h, args, stream)
-
-    serving_multi = None
-    if args.multi_requests > 0:
+        detail["helmholtz_solve"] = bench_helmholtz_solve(h, args, stream)
+    if "serving_multi" in sections and args.multi_requests > 0:
         log("bench: serving across meshes")
-        serving_multi = bench_serving_multi(h, args)
-
-    log("bench: dense 2048 x 2048 matrix")
-    dense = bench_dense(h, stream)
-
-    dirichlet = None
-    if args.solves > 0:
+        detail["serving_multi"] = bench_serving_multi(h, args)
+    if "dense" in sections:
+        log("bench: dense 2048 x 2048 matrix")
+        detail["dense"] = bench_dense(h, stream)
+    if "dirichlet" in sections and args.solves > 0:
         log("bench: Dirichlet solves")
         dirichlet = bench_dirichlet(h, mesh, args, stream)
         if counters.get("h2_product_bytes"):
@@ -897,11 +1036,13 @@ def main():
             dirichlet["serving"] = bench_serving(h, prob, ref, args)
             dirichlet["serving_saturated"] = bench_serving(h, prob, ref,
                                                            args, rate=1e5)
+        detail["dirichlet"] = dirichlet
+        del prob
 
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the reference on the host cores")
-        cpu = cpu_baseline(mesh, tree, nb, plan.entries)
+        cpu = cpu_baseline(mesh, tree, nb, plan.entries, args)
 
     line = {
         "metric": METRIC,
@@ -922,17 +1063,12 @@ def main():
                          "strict block tree clf=%d eta=%g (max-norm "
                          "admissibility), q=2 q2=4, FP64" %
                          (mesh.triangles, mesh.vertices, args.clf, args.eta)),
-            "near_blocks": int(len(nb)),
             "entries_per_step": int(entries_per_step),
-            "kernel_evaluations_per_launch": float(work[0].sum()),
-            "pairs_per_launch_by_class": [int(v) for v in work[1]],
-            "kernel_ms": {"slp": float(k_avg[0]), "dlp": float(k_avg[1])},
-            "wall_s_timed": wall,
+            "kernel_ms": [round(float(v), 4) for v in k_avg],
         },
         "roofline": {
             "bound": "valu",
-            "kernel": ("nf_regular_kernel + nf_singular_kernel (ll, Laplace SLP "
-                       "and DLP), one launch pair per operator"),
+            "kernel": "nf_regular_kernel + nf_singular_kernel",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -940,15 +1076,37 @@ def main():
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
-        "mixed_precision": mixed,
-        "helmholtz_mixed": helm,
-        "helmholtz_solve": helm_solve,
-        "batched": batched,
-        "serving_multi": serving_multi,
-        "dirichlet": dirichlet,
-        "dense": dense,
     }
-    print(json.dumps(line), flush=True)
+    detail["headline"] = {
+        "near_blocks": int(len(nb)),
+        "kernel_evaluations_per_launch": float(work[0].sum()),
+        "pairs_per_launch_by_class": [int(v) for v in work[1]],
+        "flops_per_launch": [float(f) for f in flops],
+        "wall_s_timed": wall}
+    line.update(compact(detail))
+    full = dict(line)
+    full["detail"] = detail
+    if args.detail:
+        try:
+            d = os.path.dirname(args.detail)
+            if d:
+                os.makedirs(d, exist_ok=True)
+            with open(args.detail, "w") as f:
+                json.dump(full, f, indent=1, default=float)
+            log("bench: full results in %s" % args.detail)
+        except OSError as exc:
+            log("bench: could not write %s: %s" % (args.detail, exc))
+    out = json.dumps(line, separators=(",", ":"), default=float)
+    if len(out) > LINE_MAX:
+        log("bench: summary line %d characters, dropping the section "
+            "summaries" % len(out))
+        for key in ("serving_multi", "helmholtz", "batched", "dense",
+                    "mixed", "matvec"):
+            line.pop(key, None)
+            out = json.dumps(line, separators=(",", ":"), default=float)
+            if len(out) <= LINE_MAX:
+                break
+    print(out, flush=True)
 
 
 if __name__ == "__main__":

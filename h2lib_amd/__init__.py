@@ -4,7 +4,8 @@ products, behind the reference (H2Lib) operator interface.
 Compute runs in libh2hip.so (hand-written HIP for gfx950).  This package is
 the host-side mirror of the reference's interface for that path.
 """
-from ._lib import H2HipError, hip  # noqa: F401
+from ._lib import (H2HipError, get_reproducible, hip,  # noqa: F401
+                   set_reproducible)
 from .device import DeviceArray, Event, Stream  # noqa: F401
 from .bem3d import (BASIS_CONSTANT_BEM3D, BASIS_LINEAR_BEM3D,  # noqa: F401
                     Bem3d, NearfieldPlan, Surface3d, new_dlp_helmholtz_bem3d,

@@ -76,3 +76,19 @@ def check(rc, what):
         msg = hip().h2hip_last_error()
         raise H2HipError("%s failed (%d): %s" % (
             what, rc, msg.decode() if msg else ""))
+
+
+def set_reproducible(on=True):
+    """Reproducible mode for the process: H2 products and Krylov inner
+    products add in a fixed order (the same bits on every run, and each
+    vector of a batched product gets the bits of the single product)
+    instead of with floating-point atomics (default, faster)."""
+    f = hip().h2hip_set_reproducible
+    f.restype, f.argtypes = C.c_int, [C.c_int]
+    check(f(int(bool(on))), "h2hip_set_reproducible")
+
+
+def get_reproducible():
+    f = hip().h2hip_get_reproducible
+    f.restype, f.argtypes = C.c_int, []
+    return bool(f())

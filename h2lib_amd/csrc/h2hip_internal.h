@@ -16,6 +16,11 @@ namespace h2hip {
 
 void set_error(const std::string &msg);
 
+// Reproducible mode (h2hip_set_reproducible): sums in a fixed order instead
+// of atomics in the H2 products and the Krylov reductions.
+bool reproducible();
+void set_reproducible(int on);
+
 #define H2HIP_CHECK(call)                                                    \
   do {                                                                       \
     hipError_t err__ = (call);                                               \

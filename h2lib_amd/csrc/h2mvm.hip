@@ -450,6 +450,13 @@ struct CNArgs {
   uint64_t cstride;
   VecSet v;
   CplSrc cs;  // MF products: couplings computed here
+  // reproducible products: block b's results go to slot pslot[b] of the
+  // partial-sum buffer P (far blocks first, then near blocks from pnear0;
+  // pstride doubles per vector) instead of being added with atomics
+  double *P;
+  const uint64_t *pslot;
+  uint64_t pstride;
+  uint32_t pnear0;
 };
 
 // acc += sum_j M_j xs_j over n inputs for a column-major block read along
@@ -498,7 +505,7 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
 // serves all NV vectors.
 // Item w of the wave (w < nfar: far block w, else near block w - nfar);
 // xs: the wave's LDS slice of NV * 2 * XR doubles.
-template <int C, bool TRANS, int NV, bool MF = false>
+template <int C, bool TRANS, int NV, bool MF = false, bool ORD = false>
 __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
                                                 double *xs) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;  // inputs per LDS vector
@@ -523,7 +530,39 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
       double ar[NV], ai[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
-      if constexpr (MF) {
+      if constexpr (MF && TRANS) {
+        // out = column point i, inputs = row points j; the entry is the
+        // coupling S(j, i) (the same differences row - column as the stored
+        // one), conjugated
+        const uint32_t m = a.cs.m;
+        const double *pr = a.cs.rpts + 3 * (uint64_t)m * b.r;
+        const double *pc = a.cs.cpts + 3 * (uint64_t)m * b.c;
+        const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
+        const double qx = pc[ia], qy = pc[m + ib], qz = pc[2 * m + ic];
+        uint32_t j = 0;
+        for (uint32_t ja = 0; ja < m; ja++) {
+          const double d0 = pr[ja] - qx;
+          for (uint32_t jb = 0; jb < m; jb++) {
+            const double d1 = pr[m + jb] - qy;
+            for (uint32_t jc = 0; jc < m; jc++, j++) {
+              double sr, si;
+              cpl_value(d0, d1, pr[2 * m + jc] - qz, a.cs.helmholtz,
+                        a.cs.kr, a.cs.kdamp, sr, si);
+              si = -si;
+#pragma unroll
+              for (int v = 0; v < NV; v++) {
+                const double xr = xs[v * 2 * XR + 2 * j];
+                const double xi = xs[v * 2 * XR + 2 * j + 1];
+                ar[v] = fma(sr, xr, ar[v]);
+                if constexpr (C) {
+                  ar[v] = fma(-si, xi, ar[v]);
+                  ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
+                }
+              }
+            }
+          }
+        }
+      } else if constexpr (MF) {
         const uint32_t m = a.cs.m;
         const double *pr = a.cs.rpts + 3 * (uint64_t)m * b.r;
         const double *pc = a.cs.cpts + 3 * (uint64_t)m * b.c;
@@ -576,8 +615,12 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
       }
 #pragma unroll
       for (int v = 0; v < NV; v++)
-        if (v < (int)a.v.nv)
-          atomic_add<C>(a.yt + v * a.cstride, out.coef + i, ar[v], ai[v]);
+        if (v < (int)a.v.nv) {
+          if constexpr (ORD)
+            st<C>(a.P + v * a.pstride, a.pslot[w] + i, ar[v], ai[v]);
+          else
+            atomic_add<C>(a.yt + v * a.cstride, out.coef + i, ar[v], ai[v]);
+        }
     }
     return;
   }
@@ -625,19 +668,25 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
     }
 #pragma unroll
     for (int v = 0; v < NV; v++)
-      if (v < (int)a.v.nv)
-        atomic_add<C>(a.v.y[v], oidx[i],
-                      a.v.are[v] * ar[v] - a.v.aim[v] * ai[v],
-                      a.v.are[v] * ai[v] + a.v.aim[v] * ar[v]);
+      if (v < (int)a.v.nv) {
+        if constexpr (ORD)
+          st<C>(a.P + v * a.pstride, a.pslot[a.pnear0 + nb] + i, ar[v],
+                ai[v]);
+        else
+          atomic_add<C>(a.v.y[v], oidx[i],
+                        a.v.are[v] * ar[v] - a.v.aim[v] * ai[v],
+                        a.v.are[v] * ai[v] + a.v.aim[v] * ar[v]);
+      }
   }
 }
 
-template <int C, bool TRANS, int NV, bool MF = false>
+template <int C, bool TRANS, int NV, bool MF = false, bool ORD = false>
 __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
   constexpr int XS = NV * 2 * (NV == 1 ? MAXRANK : 64);  // per wave
   __shared__ double xs_all[WPB * XS];
-  couple_near_dev<C, TRANS, NV, MF>(a, blockIdx.x * WPB + (threadIdx.x >> 6),
-                                    xs_all + (threadIdx.x >> 6) * XS);
+  couple_near_dev<C, TRANS, NV, MF, ORD>(
+      a, blockIdx.x * WPB + (threadIdx.x >> 6),
+      xs_all + (threadIdx.x >> 6) * XS);
 }
 
 // Self-adjoint product from the lower block triangle (row and column
@@ -862,6 +911,11 @@ struct FlatArgs {
   uint64_t nzero;    // accumulates into / the next product starts from),
                      // nzero doubles per vector in use, cstride apart
   VecSet v;
+  // reproducible products: upward item w writes its k coefficients to
+  // P[v pstride + (w pk + j)] (pk: the largest rank) instead of adding
+  double *P;
+  uint64_t pstride;
+  uint32_t pk;
 };
 
 // a slice of a.zero per workgroup, for every vector in use
@@ -910,11 +964,11 @@ __device__ __forceinline__ void dot_strided_nv(const double *__restrict__ M,
 // (leaves and ranks <= 64 when NV > 1): x of the leaf gathered into LDS,
 // W read once for all vectors, atomics into the coefficients.
 // block blk of nblk; xs: the wave's LDS slice of NV * 2 * XR doubles
-template <int C, int K, int NV>
+template <int C, int K, int NV, bool ORD = false>
 __device__ __forceinline__ void fwd_flat_dev(const FlatArgs &a, uint32_t blk,
                                              uint32_t nblk, double *xs) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;
-  flat_zero(a, blk, nblk);  // the coupling accumulators yt
+  if constexpr (!ORD) flat_zero(a, blk, nblk);  // the coupling accumulators
   const uint32_t w = blk * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= a.nitems) return;  // uniform over the wave; no block barrier
@@ -939,33 +993,184 @@ __device__ __forceinline__ void fwd_flat_dev(const FlatArgs &a, uint32_t blk,
     dot_strided_nv<C, NV, XR>(a.W + it.woff + j, k, sz, xs, ar, ai);
 #pragma unroll
     for (int v = 0; v < NV; v++)
-      if (v < (int)a.v.nv)
-        atomic_add<C>(a.xt + v * a.cstride, A.coef + j, ar[v], ai[v]);
+      if (v < (int)a.v.nv) {
+        if constexpr (ORD)
+          st<C>(a.P + v * a.pstride, (uint64_t)w * a.pk + j, ar[v], ai[v]);
+        else
+          atomic_add<C>(a.xt + v * a.cstride, A.coef + j, ar[v], ai[v]);
+      }
   }
 }
 
-template <int C, int K, int NV>
+template <int C, int K, int NV, bool ORD = false>
 __global__ __launch_bounds__(NT) void fwd_flat(FlatArgs a) {
   constexpr int XS = NV * 2 * (NV == 1 ? MAXRANK : 64);
   __shared__ double xs_all[WPB * XS];
-  fwd_flat_dev<C, K, NV>(a, blockIdx.x, gridDim.x,
-                         xs_all + (threadIdx.x >> 6) * XS);
+  fwd_flat_dev<C, K, NV, ORD>(a, blockIdx.x, gridDim.x,
+                              xs_all + (threadIdx.x >> 6) * XS);
 }
 
 // The upward sweep in blocks [0, fb) and the near-field blocks (c.nfar = 0)
 // after them, in one launch: the near blocks do not wait for the
 // coefficients, so they fill the device while the short upward items run.
-template <int C, int K, int NV>
+// TRANS: the adjoint product (near blocks N^*).
+template <int C, int K, int NV, bool ORD = false, bool TRANS = false>
 __global__ __launch_bounds__(NT) void fwd_flat_near(FlatArgs a, CNArgs c,
                                                     uint32_t fb) {
   constexpr int XS = NV * 2 * (NV == 1 ? MAXRANK : 64);
   __shared__ double xs_all[WPB * XS];
   double *xs = xs_all + (threadIdx.x >> 6) * XS;
   if (blockIdx.x < fb)  // uniform over the block
-    fwd_flat_dev<C, K, NV>(a, blockIdx.x, fb, xs);
+    fwd_flat_dev<C, K, NV, ORD>(a, blockIdx.x, fb, xs);
   else
-    couple_near_dev<C, false, NV>(
+    couple_near_dev<C, TRANS, NV, false, ORD>(
         c, (blockIdx.x - fb) * WPB + (threadIdx.x >> 6), xs);
+}
+
+// ---- reproducible products ----------------------------------------------------
+//
+// With h2hip_set_reproducible(1) no product adds with atomics: every upward
+// item and every coupling or near-field block writes its result to a slot of
+// a partial-sum buffer, and the launches below add the slots of each output
+// in a fixed order (items of an ancestor in leaf order, blocks of an output
+// cluster in block order, ancestors of a leaf root first).  Every vector's
+// arithmetic is the same whatever the number of vectors of the product.
+
+// xt_a = sum of the upward items of cluster a, one workgroup per cluster of
+// the input side
+template <int C, int NV>
+__global__ __launch_bounds__(NT) void ord_fwd_reduce(FlatArgs a,
+                                                     const uint32_t *ptr,
+                                                     const uint32_t *ids,
+                                                     uint32_t nnodes) {
+  const uint32_t node = blockIdx.x;
+  if (node >= nnodes) return;
+  const NodeD A = a.nodes[node];
+  const uint32_t b0 = ptr[node], b1 = ptr[node + 1];
+  for (uint32_t e = threadIdx.x; e < a.v.nv * A.rank; e += NT) {
+    const uint32_t v = e / A.rank, j = e - v * A.rank;
+    double sr = 0.0, si = 0.0;
+    for (uint32_t t = b0; t < b1; t++) {
+      double pr, pi;
+      ld<C>(a.P + v * a.pstride, (uint64_t)ids[t] * a.pk + j, pr, pi);
+      sr += pr;
+      si += pi;
+    }
+    st<C>(a.xt + v * a.cstride, A.coef + j, sr, si);
+  }
+}
+
+struct OrdReduce {
+  const NodeD *onodes;  // nodes of the output side
+  const uint32_t *oidx;
+  uint32_t nnodes;
+  const uint32_t *fptr, *fids;  // far blocks per output cluster
+  const uint32_t *nptr, *nids;  // near blocks per output cluster
+  double *yt;
+};
+
+// One workgroup per cluster of the output side: yt of the cluster = sum of
+// its far blocks' slots (zero without far blocks); for a leaf with near
+// blocks, y|_leaf += alpha sum of their slots.
+template <int C, int NV>
+__global__ __launch_bounds__(NT) void ord_reduce(CNArgs a, OrdReduce r) {
+  const uint32_t node = blockIdx.x;
+  if (node >= r.nnodes) return;
+  const NodeD O = r.onodes[node];
+  const uint32_t f0 = r.fptr[node], f1 = r.fptr[node + 1];
+  for (uint32_t e = threadIdx.x; e < a.v.nv * O.rank; e += NT) {
+    const uint32_t v = e / O.rank, i = e - v * O.rank;
+    double sr = 0.0, si = 0.0;
+    for (uint32_t t = f0; t < f1; t++) {
+      double pr, pi;
+      ld<C>(a.P + v * a.pstride, a.pslot[r.fids[t]] + i, pr, pi);
+      sr += pr;
+      si += pi;
+    }
+    st<C>(r.yt + v * a.cstride, O.coef + i, sr, si);
+  }
+  const uint32_t n0 = r.nptr[node], n1 = r.nptr[node + 1];
+  if (n0 == n1) return;
+  for (uint32_t e = threadIdx.x; e < a.v.nv * O.size; e += NT) {
+    const uint32_t v = e / O.size, i = e - v * O.size;
+    double sr = 0.0, si = 0.0;
+    for (uint32_t t = n0; t < n1; t++) {
+      double pr, pi;
+      ld<C>(a.P + v * a.pstride, a.pslot[a.pnear0 + r.nids[t]] + i, pr, pi);
+      sr += pr;
+      si += pi;
+    }
+    const uint32_t d = r.oidx[O.off + i];
+    double *y = a.v.y[v];
+    if constexpr (C) {
+      y[2 * d] += a.v.are[v] * sr - a.v.aim[v] * si;
+      y[2 * d + 1] += a.v.are[v] * si + a.v.aim[v] * sr;
+    } else {
+      y[d] += a.v.are[v] * sr;
+    }
+  }
+}
+
+// Downward sweep, one wave per leaf: y|_l += alpha sum over the ancestors a
+// of l (root first) of W_a|_l yt_a, one row per lane, no atomics.
+template <int C, int K, int NV>
+__global__ __launch_bounds__(NT) void back_flat_ord(FlatArgs a,
+                                                    const uint32_t *ibeg,
+                                                    uint32_t nleaves) {
+  constexpr int XR = NV == 1 ? MAXRANK : 64;
+  __shared__ double ys_all[WPB][NV * 2 * XR];
+  // the input coefficients xt, zero for the next product whatever its mode
+  flat_zero(a, blockIdx.x, gridDim.x);
+  const uint32_t l = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (l >= nleaves) return;
+  double *ys = ys_all[threadIdx.x >> 6];
+  const uint32_t t0 = ibeg[l], t1 = ibeg[l + 1];
+  const NodeD L = a.nodes[a.items[t0].leaf];
+  const uint32_t sz = L.size;
+  double ar[2][NV], ai[2][NV];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int v = 0; v < NV; v++) ar[h][v] = ai[h][v] = 0.0;
+  for (uint32_t t = t0; t < t1; t++) {
+    const FlatItem it = a.items[t];
+    const NodeD A = a.nodes[it.node];
+    const uint32_t k = K ? K : A.rank;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e = lane; e < NV * k; e += 64) {
+      const uint32_t v = e / k, j = e - v * k;
+      double yr = 0.0, yi = 0.0;
+      if (v < a.v.nv) ld<C>(a.yt + v * a.cstride, A.coef + j, yr, yi);
+      ys[v * 2 * XR + 2 * j] = yr;
+      ys[v * 2 * XR + 2 * j + 1] = yi;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const uint32_t i = lane + 64 * h;
+      if (i < sz)
+        dot_strided_nv<C, NV, XR>(a.W + it.woff + i, sz, k, ys, ar[h],
+                                  ai[h]);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t i = lane + 64 * h;
+    if (i >= sz) continue;
+    const uint32_t d = a.idx[L.off + i];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      if (v >= (int)a.v.nv) continue;
+      double *y = a.v.y[v];
+      if constexpr (C) {
+        y[2 * d] += a.v.are[v] * ar[h][v] - a.v.aim[v] * ai[h][v];
+        y[2 * d + 1] += a.v.are[v] * ai[h][v] + a.v.aim[v] * ar[h][v];
+      } else {
+        y[d] += a.v.are[v] * ar[h][v];
+      }
+    }
+  }
 }
 
 // Downward sweep, one wave per (leaf, ancestor) block, NV vectors at once:
@@ -1245,7 +1450,7 @@ struct h2hip_h2matrix {
   struct Side {
     uint32_t nnodes = 0, ndof = 0, depth = 0;
     std::vector<NodeD> nodes;
-    std::vector<uint32_t> leaves, sons, paths, plen, fwork;
+    std::vector<uint32_t> leaves, sons, paths, plen, fwork, idx;
     // clusters above the fwd_up work items by height (2, 3, ...): nodes of
     // height h at hnodes[hoff[h - 2] .. hoff[h - 1])
     std::vector<uint32_t> hnodes, hoff;
@@ -1256,8 +1461,23 @@ struct h2hip_h2matrix {
     std::vector<uint32_t> ibeg, rowleaf;
     uint64_t wlen = 0;
     bool flat_ok = true;  // the leaves partition the index positions
+    bool near_leaf = true;  // near blocks have leaves of this side as
+                            // output clusters
     h2hip::DevBuf<FlatItem> d_items;
     h2hip::DevBuf<double> d_WF, d_WB;
+    // reproducible products: items of each cluster (in leaf order), far and
+    // near blocks of each cluster as output of the product (block order;
+    // side 0 for y += H x, side 1 for the adjoint), the item ranges of the
+    // leaves
+    std::vector<uint32_t> it_ptr, it_ids, far_ptr, far_ids, near_ptr,
+        near_ids;
+    h2hip::DevBuf<uint32_t> d_it_ptr, d_it_ids, d_far_ptr, d_far_ids,
+        d_near_ptr, d_near_ids, d_ibeg;
+    // the flat blocks of the other orientation, for the adjoint product
+    // (built on its first use; shared with the other side when both sides
+    // hold the same bases)
+    h2hip::DevBuf<double> d_WFt, d_WBt;
+    const double *WF_adj = nullptr, *WB_adj = nullptr;
     h2hip::DevBuf<NodeD> d_nodes;
     h2hip::DevBuf<uint32_t> d_idx, d_leaves, d_sons, d_paths, d_plen, d_cnt,
         d_fwork, d_hnodes;
@@ -1292,6 +1512,16 @@ struct h2hip_h2matrix {
   // on first use
   mutable std::mutex batch_lock;
   mutable h2hip::DevBuf<double> d_xtb, d_ytb;
+  // reproducible products: slot of every far block, then every near block
+  // (the larger of its two orientations' output sizes), partial sums of the
+  // blocks and of the flat upward items for up to ord_nv vectors
+  std::vector<uint64_t> pslot;
+  uint64_t pslot_len = 0;
+  h2hip::DevBuf<uint64_t> d_pslot;
+  mutable h2hip::DevBuf<double> d_P, d_Pf;
+  mutable uint32_t ord_nv = 0;
+  // the adjoint product's flat blocks, built on first use
+  mutable std::mutex adj_lock;
 };
 
 using h2hip::set_error;
@@ -1302,6 +1532,7 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
                const uint32_t *rank, uint32_t ndof, const uint32_t *idx) {
   S.nnodes = nnodes;
   S.ndof = ndof;
+  S.idx.assign(idx, idx + ndof);
   S.nodes.resize(nnodes);
   std::vector<uint32_t> nsons(nnodes, 0), level(nnodes, 0);
   uint32_t root = NONE;
@@ -1446,6 +1677,16 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
   S.ncoef = coef;
   S.vlen = v;
   S.eld = maxr * maxr;
+  // reproducible products: the items of every cluster, in item order
+  S.it_ptr.assign(nnodes + 1, 0);
+  for (const FlatItem &it : S.items) S.it_ptr[it.node + 1]++;
+  for (uint32_t i = 0; i < nnodes; i++) S.it_ptr[i + 1] += S.it_ptr[i];
+  S.it_ids.assign(S.items.size(), 0);
+  {
+    std::vector<uint32_t> pos(S.it_ptr.begin(), S.it_ptr.end() - 1);
+    for (uint32_t t = 0; t < (uint32_t)S.items.size(); t++)
+      S.it_ids[pos[S.items[t].node]++] = t;
+  }
   std::vector<uint32_t> zero(nnodes, 0);
   if (S.d_nodes.upload(S.nodes.data(), nnodes) != hipSuccess ||
       S.d_idx.upload(idx, ndof) != hipSuccess ||
@@ -1456,7 +1697,10 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       S.d_cnt.upload(zero.data(), nnodes) != hipSuccess ||
       S.d_fwork.upload(S.fwork.data(), S.fwork.size()) != hipSuccess ||
       S.d_hnodes.upload(S.hnodes.data(), S.hnodes.size()) != hipSuccess ||
-      S.d_items.upload(S.items.data(), S.items.size()) != hipSuccess) {
+      S.d_items.upload(S.items.data(), S.items.size()) != hipSuccess ||
+      S.d_it_ptr.upload(S.it_ptr.data(), S.it_ptr.size()) != hipSuccess ||
+      S.d_it_ids.upload(S.it_ids.data(), S.it_ids.size()) != hipSuccess ||
+      S.d_ibeg.upload(S.ibeg.data(), S.ibeg.size()) != hipSuccess) {
     set_error("h2matrix: device allocation failed");
     return H2HIP_ENOMEM;
   }
@@ -1468,10 +1712,18 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
             hipStream_t st, bool trans, bool symm = false) {
   const auto &in = H->side[trans ? 0 : 1];
   const auto &out = H->side[trans ? 1 : 0];
-  const bool flat = !trans && H->level_sync == 2 && in.d_WF.p &&
-                    out.d_WB.p && H->d_xtf.p;
+  // flat blocks: row-major of the input side, column-major of the output
+  // side (the adjoint's are built on its first use)
+  const double *WFin = trans ? in.WF_adj : in.d_WF.p;
+  const double *WBout = trans ? out.WB_adj : out.d_WB.p;
+  const bool flat = H->level_sync == 2 && WFin && WBout && H->d_xtf.p;
+  // fixed-order sums (not for the self-adjoint product)
+  const bool ord = h2hip::reproducible() && !symm && H->d_P.p &&
+                   H->ord_nv >= vs.nv;
   // flat, not symmetric: the near blocks run in the upward launch
   const bool fuse = flat && !symm && H->nnear > 0 && !in.items.empty();
+  const int cm = C ? 2 : 1;
+  const bool mf = H->cpl_recompute && !symm;
   auto coupling_args = [&]() {
     CNArgs c;
     c.far = H->d_far.p;
@@ -1489,6 +1741,10 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     c.cstride = H->cstride;
     c.v = vs;
     c.cs = H->cpl;
+    c.P = H->d_P.p;
+    c.pslot = H->d_pslot.p;
+    c.pstride = (uint64_t)cm * H->pslot_len;
+    c.pnear0 = H->nfar;
     return c;
   };
   FlatArgs fa;
@@ -1504,22 +1760,43 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     fa.items = in.d_items.p;
     fa.nitems = (uint32_t)in.items.size();
     fa.idx = in.d_idx.p;
-    fa.W = in.d_WF.p;
+    fa.W = WFin;
     fa.xt = xt;
     fa.yt = nullptr;
     fa.zero = yt;
-    fa.nzero = (C ? 2 : 1) * out.ncoef;
+    fa.nzero = (uint64_t)cm * out.ncoef;
     fa.v = vs;
+    fa.P = H->d_Pf.p;
+    fa.pk = H->maxrank;
+    fa.pstride = (uint64_t)cm * H->maxrank *
+                 std::max(H->side[0].items.size(), H->side[1].items.size());
     const uint32_t fb = (fa.nitems + WPB - 1) / WPB;
-    if (fuse) {
-      CNArgs cn = coupling_args();
-      cn.nfar = 0;
-      hipLaunchKernelGGL((fwd_flat_near<C, K, NV>),
-                         dim3(fb + (H->nnear + WPB - 1) / WPB), dim3(NT), 0,
-                         st, fa, cn, fb);
+    const dim3 gf(fb + (fuse ? (H->nnear + WPB - 1) / WPB : 0));
+    CNArgs cn = coupling_args();
+    cn.xt = xt;
+    cn.yt = yt;
+    cn.nfar = 0;
+    if (ord) {
+      if (fuse && trans)
+        hipLaunchKernelGGL((fwd_flat_near<C, K, NV, true, true>), gf,
+                           dim3(NT), 0, st, fa, cn, fb);
+      else if (fuse)
+        hipLaunchKernelGGL((fwd_flat_near<C, K, NV, true, false>), gf,
+                           dim3(NT), 0, st, fa, cn, fb);
+      else
+        hipLaunchKernelGGL((fwd_flat<C, K, NV, true>), gf, dim3(NT), 0, st,
+                           fa);
+      hipLaunchKernelGGL((ord_fwd_reduce<C, NV>), dim3(in.nnodes), dim3(NT),
+                         0, st, fa, (const uint32_t *)in.d_it_ptr.p,
+                         (const uint32_t *)in.d_it_ids.p, in.nnodes);
+    } else if (fuse && trans) {
+      hipLaunchKernelGGL((fwd_flat_near<C, K, NV, false, true>), gf,
+                         dim3(NT), 0, st, fa, cn, fb);
+    } else if (fuse) {
+      hipLaunchKernelGGL((fwd_flat_near<C, K, NV, false, false>), gf,
+                         dim3(NT), 0, st, fa, cn, fb);
     } else {
-      hipLaunchKernelGGL((fwd_flat<C, K, NV>), dim3(fb), dim3(NT), 0, st,
-                         fa);
+      hipLaunchKernelGGL((fwd_flat<C, K, NV>), gf, dim3(NT), 0, st, fa);
     }
   }
   FwdArgs f;
@@ -1535,7 +1812,7 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   f.cstride = H->cstride;
   f.cnt = in.d_cnt.p;
   f.yt = yt;
-  f.nyt = (C ? 2 : 1) * out.ncoef;
+  f.nyt = (uint64_t)cm * out.ncoef;
   f.v = vs;
   f.climb = H->level_sync ? 0 : 1;
   if (!flat)
@@ -1552,20 +1829,54 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   CNArgs c = coupling_args();
   if (fuse) c.nnear = 0;  // done in the upward launch
   const uint32_t nw = c.nfar + c.nnear;
+  const dim3 gc((nw + WPB - 1) / WPB);
   if (nw && symm) {
     if constexpr (NV == 1)
-      hipLaunchKernelGGL((couple_symm<C>), dim3((nw + WPB - 1) / WPB),
-                         dim3(NT), 0, st, c);
+      hipLaunchKernelGGL((couple_symm<C>), gc, dim3(NT), 0, st, c);
   } else if (nw) {
-    if (trans)
-      hipLaunchKernelGGL((couple_near<C, true, NV>),
-                         dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
-    else if (H->cpl_recompute)
-      hipLaunchKernelGGL((couple_near<C, false, NV, true>),
-                         dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
-    else
-      hipLaunchKernelGGL((couple_near<C, false, NV>),
-                         dim3((nw + WPB - 1) / WPB), dim3(NT), 0, st, c);
+    if (trans && mf) {
+      if (ord)
+        hipLaunchKernelGGL((couple_near<C, true, NV, true, true>), gc,
+                           dim3(NT), 0, st, c);
+      else
+        hipLaunchKernelGGL((couple_near<C, true, NV, true>), gc, dim3(NT), 0,
+                           st, c);
+    } else if (trans) {
+      if (ord)
+        hipLaunchKernelGGL((couple_near<C, true, NV, false, true>), gc,
+                           dim3(NT), 0, st, c);
+      else
+        hipLaunchKernelGGL((couple_near<C, true, NV>), gc, dim3(NT), 0, st,
+                           c);
+    } else if (mf) {
+      if (ord)
+        hipLaunchKernelGGL((couple_near<C, false, NV, true, true>), gc,
+                           dim3(NT), 0, st, c);
+      else
+        hipLaunchKernelGGL((couple_near<C, false, NV, true>), gc, dim3(NT),
+                           0, st, c);
+    } else {
+      if (ord)
+        hipLaunchKernelGGL((couple_near<C, false, NV, false, true>), gc,
+                           dim3(NT), 0, st, c);
+      else
+        hipLaunchKernelGGL((couple_near<C, false, NV>), gc, dim3(NT), 0, st,
+                           c);
+    }
+  }
+  if (ord) {
+    // coupling slots into yt, near-field slots into y, in block order
+    OrdReduce r;
+    r.onodes = out.d_nodes.p;
+    r.oidx = out.d_idx.p;
+    r.nnodes = out.nnodes;
+    r.fptr = out.d_far_ptr.p;
+    r.fids = out.d_far_ids.p;
+    r.nptr = out.d_near_ptr.p;
+    r.nids = out.d_near_ids.p;
+    r.yt = yt;
+    hipLaunchKernelGGL((ord_reduce<C, NV>), dim3(out.nnodes), dim3(NT), 0,
+                       st, coupling_args(), r);
   }
 
   if (flat) {
@@ -1573,14 +1884,21 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     fa.items = out.d_items.p;
     fa.nitems = (uint32_t)out.items.size();
     fa.idx = out.d_idx.p;
-    fa.W = out.d_WB.p;
+    fa.W = WBout;
     fa.xt = nullptr;
     fa.yt = yt;
     fa.zero = xt;
-    fa.nzero = (C ? 2 : 1) * in.ncoef;
-    hipLaunchKernelGGL((back_flat<C, K, NV>),
-                       dim3((fa.nitems + WPB - 1) / WPB), dim3(NT), 0, st,
-                       fa);
+    fa.nzero = (uint64_t)cm * in.ncoef;
+    if (ord) {
+      const uint32_t nl = (uint32_t)out.leaves.size();
+      hipLaunchKernelGGL((back_flat_ord<C, K, NV>),
+                         dim3((nl + WPB - 1) / WPB), dim3(NT), 0, st, fa,
+                         (const uint32_t *)out.d_ibeg.p, nl);
+    } else {
+      hipLaunchKernelGGL((back_flat<C, K, NV>),
+                         dim3((fa.nitems + WPB - 1) / WPB), dim3(NT), 0, st,
+                         fa);
+    }
     return;
   }
   DBArgs d;
@@ -1621,9 +1939,17 @@ int apply_symm(const double *alpha, const h2hip_h2matrix *H, const double *x,
   return H2HIP_OK;
 }
 
+// Buffers a product needs beyond the matrix: the adjoint's flat blocks and
+// the partial sums of reproducible products (allocated on first use).
+int ensure_product(const h2hip_h2matrix *H, uint32_t nv, bool trans);
+
 template <int C>
 int apply(const double *alpha, const h2hip_h2matrix *H, const double *x,
           double *y, hipStream_t st, bool trans) {
+  {
+    const int rc = ensure_product(H, 1, trans);
+    if (rc != H2HIP_OK) return rc;
+  }
   VecSet vs = {};
   vs.x[0] = x;
   vs.y[0] = y;
@@ -1671,6 +1997,10 @@ int apply_batch(uint32_t nvec, const double *alpha, const h2hip_h2matrix *H,
       }
     }
   }
+  {
+    const int rc = ensure_product(H, nvec, trans);
+    if (rc != H2HIP_OK) return rc;
+  }
   VecSet vs = {};
   for (uint32_t v = 0; v < nvec; v++) {
     vs.x[v] = (const double *)x[v];
@@ -1690,9 +2020,10 @@ int apply_batch(uint32_t nvec, const double *alpha, const h2hip_h2matrix *H,
   return H2HIP_OK;
 }
 
-// Builds the blocks W_a|_l of one side from its bases on the device:
-// row-major (upward sweep) and / or column-major (downward sweep).
-int build_flat(h2hip_h2matrix::Side &S, bool rowmajor, bool colmajor) {
+// Builds the blocks W_a|_l of one side from its bases on the device into WF
+// (row-major, upward sweep) and / or WB (column-major, downward sweep);
+// either may be null.  Complete on return.
+int build_flat_into(const h2hip_h2matrix::Side &S, double *WF, double *WB) {
   if (!S.d_VT.p || !S.d_ET.p) {
     set_error("h2matrix: flat transforms need the bases");
     return H2HIP_EINVAL;
@@ -1702,30 +2033,44 @@ int build_flat(h2hip_h2matrix::Side &S, bool rowmajor, bool colmajor) {
               "index positions");
     return H2HIP_EINVAL;
   }
-  if (rowmajor && S.d_WF.alloc(S.wlen) != hipSuccess) return H2HIP_ENOMEM;
-  if (colmajor && S.d_WB.alloc(S.wlen) != hipSuccess) return H2HIP_ENOMEM;
-  h2hip::DevBuf<uint32_t> d_ibeg, d_rowleaf;
-  if (d_ibeg.upload(S.ibeg.data(), S.ibeg.size()) != hipSuccess ||
-      d_rowleaf.upload(S.rowleaf.data(), S.rowleaf.size()) != hipSuccess)
+  h2hip::DevBuf<uint32_t> d_rowleaf;
+  if (d_rowleaf.upload(S.rowleaf.data(), S.rowleaf.size()) != hipSuccess)
     return H2HIP_ENOMEM;
   WBuildArgs a;
   a.nodes = S.d_nodes.p;
   a.items = S.d_items.p;
-  a.ibeg = d_ibeg.p;
+  a.ibeg = S.d_ibeg.p;
   a.leaves = S.d_leaves.p;
   a.rowleaf = d_rowleaf.p;
   a.ndof = S.ndof;
   a.VT = S.d_VT.p;
   a.ET = S.d_ET.p;
   a.eld = S.eld;
-  a.WF = rowmajor ? S.d_WF.p : nullptr;
-  a.WB = colmajor ? S.d_WB.p : nullptr;
+  a.WF = WF;
+  a.WB = WB;
   hipLaunchKernelGGL(w_build, dim3((S.ndof + WPB - 1) / WPB), dim3(NT), 0,
                      nullptr, a);
   H2HIP_CHECK(hipGetLastError());
-  // the index buffers are freed on return
+  // the index buffer is freed on return
   H2HIP_CHECK(hipStreamSynchronize(nullptr));
   return H2HIP_OK;
+}
+
+int build_flat(h2hip_h2matrix::Side &S, bool rowmajor, bool colmajor) {
+  if (rowmajor && S.d_WF.alloc(S.wlen) != hipSuccess) return H2HIP_ENOMEM;
+  if (colmajor && S.d_WB.alloc(S.wlen) != hipSuccess) return H2HIP_ENOMEM;
+  return build_flat_into(S, rowmajor ? S.d_WF.p : nullptr,
+                         colmajor ? S.d_WB.p : nullptr);
+}
+
+// the adjoint's blocks are rebuilt on their next use
+void drop_adjoint_flat(h2hip_h2matrix *H) {
+  std::lock_guard<std::mutex> guard(H->adj_lock);
+  for (auto &S : H->side) {
+    S.d_WFt.release();
+    S.d_WBt.release();
+    S.WF_adj = S.WB_adj = nullptr;
+  }
 }
 
 // flat transforms of the products y += alpha H x: row-major blocks of the
@@ -1753,7 +2098,87 @@ int build_flat_pair(h2hip_h2matrix *H) {
     if (rc == H2HIP_ENOMEM) set_error("h2matrix: flat transforms: device "
                                       "allocation failed");
   }
+  drop_adjoint_flat(H);
   return rc;
+}
+
+// Do both sides hold the same bases (same tree layout, same interpolation
+// data)?  Then the blocks of one side serve the other.
+bool same_bases(const h2hip_h2matrix *H) {
+  const auto &r = H->side[0], &c = H->side[1];
+  bool same = r.nnodes == c.nnodes && r.ndof == c.ndof &&
+              H->basis_set[0] && H->basis_set[1] &&
+              H->basis_sum[0] == H->basis_sum[1] && r.idx == c.idx;
+  for (uint32_t i = 0; same && i < r.nnodes; i++)
+    same = r.nodes[i].off == c.nodes[i].off &&
+           r.nodes[i].size == c.nodes[i].size &&
+           r.nodes[i].rank == c.nodes[i].rank &&
+           r.nodes[i].father == c.nodes[i].father;
+  return same;
+}
+
+int ensure_product(const h2hip_h2matrix *Hc, uint32_t nv, bool trans) {
+  h2hip_h2matrix *H = const_cast<h2hip_h2matrix *>(Hc);
+  if (trans && H->level_sync == 2 && H->d_xtf.p) {
+    // the adjoint's flat blocks: row-major of the row side, column-major
+    // of the column side
+    std::lock_guard<std::mutex> guard(H->adj_lock);
+    auto &R = H->side[0], &Cs = H->side[1];
+    if (!R.WF_adj || !Cs.WB_adj) {
+      if (same_bases(H)) {
+        R.WF_adj = Cs.d_WF.p;
+        Cs.WB_adj = R.d_WB.p;
+      } else {
+        if (R.d_WFt.alloc(R.wlen) != hipSuccess ||
+            Cs.d_WBt.alloc(Cs.wlen) != hipSuccess) {
+          R.d_WFt.release();
+          Cs.d_WBt.release();
+          set_error("h2matrix: adjoint flat blocks: device allocation "
+                    "failed");
+          return H2HIP_ENOMEM;
+        }
+        int rc = build_flat_into(R, R.d_WFt.p, nullptr);
+        if (rc == H2HIP_OK) rc = build_flat_into(Cs, nullptr, Cs.d_WBt.p);
+        if (rc != H2HIP_OK) {
+          R.d_WFt.release();
+          Cs.d_WBt.release();
+          return rc;
+        }
+        R.WF_adj = R.d_WFt.p;
+        Cs.WB_adj = Cs.d_WBt.p;
+      }
+    }
+  }
+  if (h2hip::reproducible()) {
+    // every output row is written by one workgroup: the leaves partition
+    // the index positions and the near blocks have leaves as output
+    // clusters (as in strict block trees)
+    const auto &O = H->side[trans ? 1 : 0];
+    if (!O.flat_ok || !O.near_leaf) {
+      set_error("h2matrix: reproducible products need leaves that partition "
+                "the index positions and near blocks between leaves");
+      return H2HIP_EINVAL;
+    }
+    // partial sums for the vector count class of the launch (1, 2, 4, 8)
+    const uint32_t cap = nv <= 1 ? 1 : nv <= 2 ? 2 : nv <= 4 ? 4 : 8;
+    std::lock_guard<std::mutex> guard(H->batch_lock);
+    if (H->ord_nv < cap) {
+      const uint64_t cm = H->cplx ? 2 : 1;
+      const uint64_t items = std::max(H->side[0].items.size(),
+                                      H->side[1].items.size());
+      if (H->d_P.alloc(cm * H->pslot_len * cap) != hipSuccess ||
+          H->d_Pf.alloc(cm * items * H->maxrank * cap) != hipSuccess) {
+        H->d_P.release();
+        H->d_Pf.release();
+        H->ord_nv = 0;
+        set_error("h2matrix: reproducible products: device allocation "
+                  "failed");
+        return H2HIP_ENOMEM;
+      }
+      H->ord_nv = cap;
+    }
+  }
+  return H2HIP_OK;
 }
 
 }  // namespace
@@ -1814,6 +2239,54 @@ int h2hip_h2matrix_create(h2hip_h2matrix **out, int is_complex,
   }
   H->nlen = no;
   H->slen = so;
+  // reproducible products: a slot per block, sized for either orientation,
+  // and the blocks of every cluster as output (side 0: y += H x, side 1:
+  // the adjoint), in block order
+  H->pslot.resize((size_t)nfar + nnear);
+  uint64_t ps = 0;
+  for (uint32_t b = 0; b < nfar; b++) {
+    H->pslot[b] = ps;
+    ps += std::max(H->side[0].nodes[far[2 * b]].rank,
+                   H->side[1].nodes[far[2 * b + 1]].rank);
+  }
+  for (uint32_t b = 0; b < nnear; b++) {
+    H->pslot[nfar + b] = ps;
+    ps += std::max(H->near[b].rows, H->near[b].cols);
+  }
+  H->pslot_len = ps;
+  for (int sd = 0; sd < 2; sd++) {
+    auto &S = H->side[sd];
+    for (uint32_t b = 0; b < nnear && S.near_leaf; b++)
+      S.near_leaf = S.nodes[near[2 * b + sd]].nsons == 0;
+    auto csr = [&](const uint32_t *pairs, uint32_t cnt,
+                   std::vector<uint32_t> &ptr, std::vector<uint32_t> &ids) {
+      ptr.assign(S.nnodes + 1, 0);
+      for (uint32_t b = 0; b < cnt; b++) ptr[pairs[2 * b + sd] + 1]++;
+      for (uint32_t i = 0; i < S.nnodes; i++) ptr[i + 1] += ptr[i];
+      ids.assign(cnt, 0);
+      std::vector<uint32_t> pos(ptr.begin(), ptr.end() - 1);
+      for (uint32_t b = 0; b < cnt; b++) ids[pos[pairs[2 * b + sd]]++] = b;
+    };
+    csr(far, nfar, S.far_ptr, S.far_ids);
+    csr(near, nnear, S.near_ptr, S.near_ids);
+    if (S.d_far_ptr.upload(S.far_ptr.data(), S.far_ptr.size()) !=
+            hipSuccess ||
+        S.d_far_ids.upload(S.far_ids.data(), S.far_ids.size()) !=
+            hipSuccess ||
+        S.d_near_ptr.upload(S.near_ptr.data(), S.near_ptr.size()) !=
+            hipSuccess ||
+        S.d_near_ids.upload(S.near_ids.data(), S.near_ids.size()) !=
+            hipSuccess) {
+      set_error("h2hip_h2matrix_create: device allocation failed");
+      delete H;
+      return H2HIP_ENOMEM;
+    }
+  }
+  if (H->d_pslot.upload(H->pslot.data(), H->pslot.size()) != hipSuccess) {
+    set_error("h2hip_h2matrix_create: device allocation failed");
+    delete H;
+    return H2HIP_ENOMEM;
+  }
   const int cm = H->cplx ? 2 : 1;
   const uint64_t ncoef =
       std::max(H->side[0].ncoef, H->side[1].ncoef) * (uint64_t)cm;
@@ -2064,6 +2537,7 @@ int h2hip_h2matrix_set_schedule(h2hip_h2matrix *H, int level_sync) {
     const int rc = build_flat_pair(H);
     if (rc != H2HIP_OK) return rc;
   } else {
+    drop_adjoint_flat(H);
     H->side[1].d_WF.release();
     H->side[0].d_WB.release();
   }
@@ -2139,6 +2613,59 @@ int h2hip_h2matrix_bytes(const h2hip_h2matrix *H, uint64_t *out3) {
     }
   }
   out3[2] = b;
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_storage(const h2hip_h2matrix *H, uint64_t *out4) {
+  if (!H || !out4) return H2HIP_EINVAL;
+  const uint64_t cm = H->cplx ? 2 : 1;
+  out4[0] = 8ull * cm * H->nlen;
+  // every admissible block's k_r x k_c coupling, stored or computed
+  out4[1] = 8ull * cm * H->slen;
+  // leaf and transfer matrices of both sides, one copy each (real)
+  uint64_t b = 0;
+  for (int s = 0; s < 2; s++) {
+    b += 8ull * H->side[s].vlen;
+    for (const NodeD &d : H->side[s].nodes)
+      if (d.father != NONE)
+        b += 8ull * d.rank * H->side[s].nodes[d.father].rank;
+  }
+  out4[2] = b;
+  // every device buffer the matrix holds
+  uint64_t r = 0;
+  auto add = [&r](const auto &buf) { r += sizeof(*buf.p) * buf.n; };
+  for (int s = 0; s < 2; s++) {
+    const auto &S = H->side[s];
+    add(S.d_items);
+    add(S.d_WF);
+    add(S.d_WB);
+    add(S.d_nodes);
+    add(S.d_idx);
+    add(S.d_leaves);
+    add(S.d_sons);
+    add(S.d_paths);
+    add(S.d_plen);
+    add(S.d_cnt);
+    add(S.d_fwork);
+    add(S.d_hnodes);
+    add(S.d_V);
+    add(S.d_VT);
+    add(S.d_E);
+    add(S.d_ET);
+  }
+  add(H->d_far);
+  add(H->d_near);
+  add(H->d_S);
+  add(H->d_N);
+  add(H->d_xt);
+  add(H->d_yt);
+  add(H->d_xtf);
+  add(H->d_ytf);
+  add(H->d_rpts);
+  add(H->d_cpts);
+  add(H->d_xtb);
+  add(H->d_ytb);
+  out4[3] = r;
   return H2HIP_OK;
 }
 

@@ -448,11 +448,19 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // estimates
   const uint32_t nacc = (uint32_t)(W.nrm + 8 - W.small.p);
   const dim3 gz((uint32_t)((std::max<uint64_t>(W.n2, nacc) + NT - 1) / NT));
+  // reproducible mode: fixed-order reductions (the last block of a grid
+  // adds the blocks' sums in block order), the arithmetic of the batched
+  // solver's slots, instead of FP64 atomics
+  const bool rep = h2hip::reproducible();
   // |b|
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(NT), 0, st, W.nrm,
                      (uint64_t)2);
-  hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st, b,
-                     (uint64_t)0, 1, b, n, W.part.p, W.cnt.p, W.nrm);
+  if (rep)
+    hipLaunchKernelGGL((multidot<C>), gdot, dim3(NT), 0, st, b, (uint64_t)0,
+                       1, b, n, W.part.p, W.cnt.p, W.nrm);
+  else
+    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st, b,
+                       (uint64_t)0, 1, b, n, W.part.p, W.cnt.p, W.nrm);
   double nb2[2];
   H2HIP_CHECK(hipMemcpyAsync(nb2, W.nrm, sizeof(nb2), hipMemcpyDeviceToHost,
                              st));
@@ -466,9 +474,14 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
                        W.nrm, 2u);
     int r = addeval(mone, A, x, q0, st);
     if (r) return r;
-    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
-                       (const double *)q0, (uint64_t)0, 1,
-                       (const double *)q0, n, W.part.p, W.cnt.p, W.nrm);
+    if (rep)  // |r|^2 as the batched solver's cycle start forms it
+      hipLaunchKernelGGL((multiaxpy<C>), gax, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, 0, (const double *)W.hB,
+                         1.0, q0, n, W.part.p, W.cnt.p + 1, W.nrm);
+    else
+      hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
+                         (const double *)q0, (uint64_t)0, 1,
+                         (const double *)q0, n, W.part.p, W.cnt.p, W.nrm);
     hipLaunchKernelGGL(start_cycle, W.gv, dim3(NT), 0, st, q0, W.n2,
                        (const double *)W.nrm, W.lsq(), W.err);
     return H2HIP_OK;
@@ -490,18 +503,37 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
                        W.small.p, nacc);
     int r = addeval(one, A, qk, w, st);
     if (r) return r;
-    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
-                       (const double *)W.Q.p, n, k + 1, (const double *)w, n,
-                       W.part.p, W.cnt.p + 2, W.hA);
-    hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
-                       (const double *)W.Q.p, n, k + 1, (const double *)W.hA,
-                       1.0, w, n, W.part.p, W.cnt.p + 1, (double *)nullptr);
-    hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
-                       (const double *)W.Q.p, n, k + 1, (const double *)w, n,
-                       W.part.p, W.cnt.p + 2, W.hB);
-    hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
-                       (const double *)W.Q.p, n, k + 1, (const double *)W.hB,
-                       1.0, w, n, W.part.p, W.cnt.p + 1, W.nrm);
+    if (rep) {
+      hipLaunchKernelGGL((multidot<C>), gdot, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1, (const double *)w,
+                         n, W.part.p, W.cnt.p + 2, W.hA);
+      hipLaunchKernelGGL((multiaxpy<C>), gax, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1,
+                         (const double *)W.hA, 1.0, w, n, W.part.p,
+                         W.cnt.p + 1, (double *)nullptr);
+      hipLaunchKernelGGL((multidot<C>), gdot, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1, (const double *)w,
+                         n, W.part.p, W.cnt.p + 2, W.hB);
+      hipLaunchKernelGGL((multiaxpy<C>), gax, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1,
+                         (const double *)W.hB, 1.0, w, n, W.part.p,
+                         W.cnt.p + 1, W.nrm);
+    } else {
+      hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1, (const double *)w,
+                         n, W.part.p, W.cnt.p + 2, W.hA);
+      hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1,
+                         (const double *)W.hA, 1.0, w, n, W.part.p,
+                         W.cnt.p + 1, (double *)nullptr);
+      hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1, (const double *)w,
+                         n, W.part.p, W.cnt.p + 2, W.hB);
+      hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1,
+                         (const double *)W.hB, 1.0, w, n, W.part.p,
+                         W.cnt.p + 1, W.nrm);
+    }
     hipLaunchKernelGGL(scale_givens, W.gv, dim3(NT), 0, st,
                        (const double *)w, W.col(k + 1), W.n2,
                        (const double *)W.nrm, (const double *)W.hA,
@@ -1485,6 +1517,14 @@ int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
       h2hip::set_error("h2hip_gmres_batch_combine: solves in progress");
       return H2HIP_EINVAL;
     }
+  if (G->method == 1) {
+    // a combination such as KM - i k V is not Hermitian: CG would not
+    // converge
+    h2hip::set_error("h2hip_gmres_batch_combine: the solver is CG, which "
+                     "needs a Hermitian positive definite operator; switch "
+                     "to GMRES first");
+    return H2HIP_EINVAL;
+  }
   G->c1[0] = c1[0];
   G->c1[1] = c1[1];
   G->H2 = H2;
@@ -1504,6 +1544,12 @@ int h2hip_gmres_batch_set_method(h2hip_gmres_batch *G, int method) {
       h2hip::set_error("h2hip_gmres_batch_set_method: solves in progress");
       return H2HIP_EINVAL;
     }
+  if (method == 1 && G->H2) {
+    h2hip::set_error("h2hip_gmres_batch_set_method: CG needs a Hermitian "
+                     "positive definite operator, not the combination set "
+                     "by h2hip_gmres_batch_combine");
+    return H2HIP_EINVAL;
+  }
   if (method == G->method) return H2HIP_OK;
   // the flags and block counters of both methods share cnt: start at zero
   for (int s = 0; s < G->nslots; s++)
@@ -1706,9 +1752,16 @@ int multi_init(Multi *G, Multi::Sys &S, hipStream_t st) {
   hipLaunchKernelGGL(copy_kernel, S.W.gv, dim3(NT), 0, st, S.b, q0, S.n2);
   int r = h2hip_addeval_h2matrix(mone, S.H, S.x, q0, st);
   if (r) return r;
-  hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st, (const double *)q0,
-                     (uint64_t)0, 1, (const double *)q0, S.n, S.W.part.p,
-                     S.W.cnt.p, S.W.nrm);
+  if (h2hip::reproducible())  // |r|^2 as the other solvers form it
+    hipLaunchKernelGGL(multiaxpy<C>, dim3((uint32_t)((S.n + NT - 1) / NT)),
+                       dim3(NT), 0, st, (const double *)S.W.Q.p, S.n, 0,
+                       (const double *)S.W.hB, 1.0, q0, S.n, S.W.part.p,
+                       S.W.cnt.p + 1, S.W.nrm);
+  else
+    hipLaunchKernelGGL(multidot<C>, gdot, dim3(NT), 0, st,
+                       (const double *)q0, (uint64_t)0, 1,
+                       (const double *)q0, S.n, S.W.part.p, S.W.cnt.p,
+                       S.W.nrm);
   hipLaunchKernelGGL(start_cycle, S.W.gv, dim3(NT), 0, st, q0, S.n2,
                      (const double *)S.W.nrm, S.W.lsq(),
                      S.derr + G->kmax);

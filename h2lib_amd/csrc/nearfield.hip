@@ -13,6 +13,7 @@
 // found from the vertex-to-triangle map (no all-pairs scan) and collected
 // into the singular list with their rule class and vertex permutation.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -24,6 +25,14 @@ namespace h2hip {
 
 thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+static std::atomic<int> g_reproducible{0};
+bool reproducible() {
+  return g_reproducible.load(std::memory_order_relaxed) != 0;
+}
+void set_reproducible(int on) {
+  g_reproducible.store(on ? 1 : 0, std::memory_order_relaxed);
+}
 
 namespace {
 
@@ -377,6 +386,13 @@ int h2hip_device_count(void) {
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
 }
+
+int h2hip_set_reproducible(int on) {
+  set_reproducible(on);
+  return H2HIP_OK;
+}
+
+int h2hip_get_reproducible(void) { return reproducible() ? 1 : 0; }
 
 int h2hip_surface_create(h2hip_surface **out, uint32_t vertices,
                          const double *x, uint32_t triangles,

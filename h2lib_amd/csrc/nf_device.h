@@ -496,16 +496,10 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   // Moments M[a][b] = sum w v m_a(x) m_b(y) with m = (1, tx, sx) resp.
   // (1, ty, sy); the linear basis functions are phi = (1 - tx, tx - sx, sx)
   // = A m, so the element matrix A M A^T is formed once after the sum.
-  auto point = [&](double wq, double tx, double sx, double ty, double sy) {
-    double d[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-      d[k] = fma(-F2[k], sy,
-                 fma(-F1[k], ty, fma(E2[k], sx, fma(E1[k], tx, D0[k]))));
-    double vre, vim;
+  // kernel value at the difference d (FP64; mixed precision: d in FP64,
+  // the value in FP32)
+  auto value = [&](const double *d, double &vre, double &vim) {
     if constexpr (MP) {
-      // mixed precision: the Sauter-Schwab points and d in FP64, the kernel
-      // value in FP32, the moment sums in FP64
       float fre, fim;
       kval32<KER>((float)d[0], (float)d[1], (float)d[2], (float)nx,
                   (float)ny, (float)nz, a.krf, a.krev, a.kil2, fre, fim);
@@ -514,6 +508,15 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
     } else {
       kval<KER>(d[0], d[1], d[2], nx, ny, nz, a.kr, a.ki, vre, vim);
     }
+  };
+  auto point = [&](double wq, double tx, double sx, double ty, double sy) {
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      d[k] = fma(-F2[k], sy,
+                 fma(-F1[k], ty, fma(E2[k], sx, fma(E1[k], tx, D0[k]))));
+    double vre, vim;
+    value(d, vre, vim);
     double ure[NR], uim[NR];
     ure[0] = wq * vre;
     uim[0] = wq * vim;
@@ -553,12 +556,86 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
       const double gk = a.gauss[k], gl = a.gauss[l];
       const double wk = a.gauss[Q2 + k], wl = a.gauss[Q2 + l];
       if constexpr (CLS == CLS_VERTEX) {
+        // The two points of a tuple (SSTuple<CLS_VERTEX>) with e1 = g_i,
+        // e2 = g_j, e3 = g_k, xi = g_l:
+        //   P1 (tx, sx, ty, sy) = (xi, e2 xi, e1 xi, e1 e3 xi)
+        //   P2 (tx, sx, ty, sy) = (e1 xi, e1 e3 xi, xi, e2 xi)
+        // so d = D0 + xi (E1 + E2 e2 - (F1 + F2 e3) e1) for P1 and
+        // d = D0 + xi ((E1 + E2 e3) e1 - F1 - F2 e2) for P2: affine in e2
+        // for fixed i.  In P1 the column moments (1, ty, sy) do not depend
+        // on j and the row moments (1, tx, sx) are affine in e2 (P2 the
+        // other way round), so the sums over j need only sum u and sum u e2
+        // per point, and the moment matrix is updated once per i.
+        const double e3 = gk, xi = gl;
+        const double wkl = wk * wl * xi * xi * xi;
+        double a1[3], b1[3], c1[3], a2[3], b2[3], c2[3];
 #pragma unroll
-        for (int i = 0; i < Q2; i++)
+        for (int d = 0; d < 3; d++) {
+          a1[d] = fma(xi, E1[d], D0[d]);
+          b1[d] = xi * E2[d];
+          c1[d] = xi * fma(F2[d], e3, F1[d]);
+          a2[d] = fma(-xi, F1[d], D0[d]);
+          b2[d] = xi * fma(E2[d], e3, E1[d]);
+          c2[d] = -xi * F2[d];
+        }
 #pragma unroll
-          for (int j = 0; j < Q2; j++)
-            SSTuple<CLS>::run(gn[i], gn[j], gk, gl, gw[i], gw[j], wk, wl,
-                              point);
+        for (int i = 0; i < Q2; i++) {
+          const double e1 = gn[i];
+          const double wi1 = wkl * gw[i] * e1;
+          double t1[3], t2[3];
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            t1[d] = fma(-c1[d], e1, a1[d]);
+            t2[d] = fma(b2[d], e1, a2[d]);
+          }
+          double U0r = 0.0, U0i = 0.0, U2r = 0.0, U2i = 0.0;
+          double V0r = 0.0, V0i = 0.0, V2r = 0.0, V2i = 0.0;
+#pragma unroll
+          for (int j = 0; j < Q2; j++) {
+            const double e2 = gn[j], w = wi1 * gw[j];
+            double d1[3], d2[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              d1[d] = fma(b1[d], e2, t1[d]);
+              d2[d] = fma(c2[d], e2, t2[d]);
+            }
+            double v1r, v1i, v2r, v2i;
+            value(d1, v1r, v1i);
+            value(d2, v2r, v2i);
+            const double u1r = w * v1r, u2r = w * v2r;
+            U0r += u1r;
+            U2r = fma(u1r, e2, U2r);
+            V0r += u2r;
+            V2r = fma(u2r, e2, V2r);
+            if constexpr (CPLX) {
+              const double u1i = w * v1i, u2i = w * v2i;
+              U0i += u1i;
+              U2i = fma(u1i, e2, U2i);
+              V0i += u2i;
+              V2i = fma(u2i, e2, V2i);
+            }
+          }
+          // P1: row moments (U0, xi U0, xi U2), column moments
+          // (1, e1 xi, e1 e3 xi); P2: row moments (1, e1 xi, e1 e3 xi),
+          // column moments (V0, xi V0, xi V2)
+          const double f1 = e1 * xi, f2 = e1 * e3 * xi;
+          const double ms[3] = {1.0, f1, f2};
+          const double p1r[3] = {U0r, xi * U0r, xi * U2r};
+          const double p1i[3] = {U0i, xi * U0i, xi * U2i};
+          const double p2r[3] = {V0r, xi * V0r, xi * V2r};
+          const double p2i[3] = {V0i, xi * V0i, xi * V2i};
+#pragma unroll
+          for (int l = 0; l < NR; l++)
+#pragma unroll
+            for (int kk = 0; kk < NC; kk++) {
+              Sre[l][kk] = fma(p1r[l], ms[kk], Sre[l][kk]);
+              Sre[l][kk] = fma(ms[l], p2r[kk], Sre[l][kk]);
+              if constexpr (CPLX) {
+                Sim[l][kk] = fma(p1i[l], ms[kk], Sim[l][kk]);
+                Sim[l][kk] = fma(ms[l], p2i[kk], Sim[l][kk]);
+              }
+            }
+        }
       } else if constexpr (CLS == CLS_EDGE) {
         // six points per tuple: unrolling i as well costs too many registers
 #pragma unroll 1

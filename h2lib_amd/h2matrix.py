@@ -71,6 +71,7 @@ def _sigs():
     h.h2hip_h2matrix_nearfield_device.restype = C.c_void_p
     h.h2hip_h2matrix_nearfield_device.argtypes = [vp]
     h.h2hip_h2matrix_bytes.argtypes = [vp, dp]
+    h.h2hip_h2matrix_storage.argtypes = [vp, dp]
     h.h2hip_addeval_h2matrix.argtypes = [dp, vp, vp, vp, vp]
     h.h2hip_addevaltrans_h2matrix.argtypes = [dp, vp, vp, vp, vp]
     h.h2hip_addeval_h2matrix_batch.restype = C.c_int
@@ -333,11 +334,35 @@ class H2Matrix:
         self.recompute = int(bool(recompute))
 
     def bytes(self):
+        """Bytes one product y += alpha H x reads in the current schedule
+        and coupling mode (couplings 0 when computed in the product)."""
         out = np.zeros(3, dtype=np.uint64)
         check(hip().h2hip_h2matrix_bytes(self._h, _ptr(out)),
               "h2hip_h2matrix_bytes")
         return {"near": int(out[0]), "coupling": int(out[1]),
                 "bases": int(out[2])}
+
+    def storage(self):
+        """Storage in the reference's terms (near field, couplings of all
+        admissible blocks, leaf and transfer matrices of both sides, one
+        copy each) and the device bytes the matrix holds."""
+        out = np.zeros(4, dtype=np.uint64)
+        check(hip().h2hip_h2matrix_storage(self._h, _ptr(out)),
+              "h2hip_h2matrix_storage")
+        return {"near": int(out[0]), "coupling": int(out[1]),
+                "bases": int(out[2]), "device": int(out[3])}
+
+    def algorithmic_bytes(self):
+        """Bytes of one product by the reference's data: near field,
+        couplings, leaf and transfer matrices of both sides, and the vector
+        traffic (x read, y read and written, coefficient vectors written
+        and read once)."""
+        st = self.storage()
+        es = 16 if self.dtype == np.complex128 else 8
+        kt = sum(int(x) for x in self.interp.rrank) + \
+            sum(int(x) for x in self.interp.crank)
+        return (st["near"] + st["coupling"] + st["bases"] +
+                es * (3 * self.n + 2 * kt))
 
     def addeval(self, alpha, d_x, d_y, stream=None, trans=False):
         """y += alpha H x (trans: y += alpha H^* x) on device arrays."""

@@ -73,6 +73,14 @@ def footprint(mesh, kind, clf, m, slots, kmax, eta=2.0):
     op = es * near + es * len(bt.far) * k * k + 8 * (4 * n * k +
                                                        4 * tree.nnodes * k * k)
     op += flat
+    from ._lib import get_reproducible
+    if get_reproducible():
+        # partial sums of the fixed-order products for a full batch: a slot
+        # per block (its larger side) and per flat upward item
+        slots_len = len(bt.far) * k + int(np.maximum(nb[:, 1],
+                                                     nb[:, 3]).sum())
+        items = int((tree.level[leaf].astype(np.int64) + 1).sum())
+        op += es * 8 * (slots_len + items * k)
     gm = slots * es * (n * (kmax + 2))
     return 2 * op + gm
 

@@ -53,6 +53,17 @@ const char *h2hip_last_error(void);
 /* Device number used by the library (hipGetDevice of the caller). */
 int h2hip_device_count(void);
 
+/* Reproducible mode for the whole process (default off).  On: the H2
+ * products and the Krylov solvers' inner products add their contributions
+ * in a fixed order instead of with floating-point atomics, so a product or
+ * a solve gives the same bits on every run, and a vector of a batched
+ * product (h2hip_addeval_h2matrix_batch, the batched solvers) gets the bits
+ * of the single product; the products cost about one more reduction pass.
+ * Read when work is enqueued.  No counterpart in the reference (its
+ * products are sequential). */
+int h2hip_set_reproducible(int on);
+int h2hip_get_reproducible(void);
+
 /* ---- surfaces ------------------------------------------------------------
  * Device copy of a triangulated surface: vertices x[3*vertices], triangles
  * t[3*triangles] (counter-clockwise), unit normals n[3*triangles] and Gram
@@ -250,8 +261,34 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
                                     const double *k, uint32_t m,
                                     const double *rpts, const double *cpts,
                                     void *stream);
-/* Storage of the matrix in bytes: near field, couplings, bases. */
+/* ---- dense matrices ----------------------------------------------------------
+ * y += alpha A x (trans = 0) or y += alpha A^* x (trans != 0, conjugate
+ * transpose) for a column-major device matrix A of rows x cols with leading
+ * dimension ld >= rows, real or complex (is_complex: interleaved pairs),
+ * device vectors x and y, alpha = (re, im); y is not cleared.  Ordered on
+ * the stream; results do not depend on the launch schedule (partial sums
+ * over column chunks are added in chunk order).  The partial sums of A x
+ * live in stream-ordered device memory for the duration of the call.
+ * Replaces addeval_amatrix_avector / addevaltrans_amatrix_avector
+ * (Library/amatrix.c:862-925). */
+int h2hip_addeval_amatrix(const double *alpha, int is_complex, int trans,
+                          uint32_t rows, uint32_t cols, const void *A,
+                          uint64_t ld, const void *x, void *y, void *stream);
+
+/* Bytes one product y += alpha H x reads in the current schedule and
+ * coupling mode: near field, couplings (0 when computed), bases (the
+ * blocks W_a|_l of both sides with the flat transforms, else leaf and
+ * transfer matrices). */
 int h2hip_h2matrix_bytes(const h2hip_h2matrix *H, uint64_t *out3);
+/* Storage of the matrix in the reference's terms and on the device:
+ * out4[0] near-field entries, out4[1] couplings of all admissible blocks
+ * (k_r k_c entries each, whether kept or computed in the products),
+ * out4[2] leaf and transfer matrices of both sides, one copy each (the
+ * storage of the reference's clusterbasis, Library/clusterbasis.h:40-71),
+ * all in bytes; out4[3] bytes of every device buffer the matrix holds
+ * (both orientations of the bases, flat blocks, coefficient vectors,
+ * trees).  No counterpart in the reference. */
+int h2hip_h2matrix_storage(const h2hip_h2matrix *H, uint64_t *out4);
 /* y += alpha H x (addeval_h2matrix_avector, Library/h2matrix.c:1268-1286)
  * and y += alpha H^* x (addevaltrans_h2matrix_avector, Library/h2matrix.c:
  * 1345-1364) on device vectors in the original DOF order; alpha = (re, im).
@@ -398,7 +435,8 @@ void h2hip_gmres_batch_destroy(h2hip_gmres_batch *G);
  * (re, im); H2 of the same size and batched-product limits), e.g. the
  * Brakhage-Werner operator KM - i eta V of the reference's Helmholtz test
  * (addeval_A, Tests/test_helmholtzbem3d.c:24-65): every step makes two
- * batched products.  Only while no solve is in progress. */
+ * batched products.  Only while no solve is in progress and with GMRES as
+ * the method (the combination is not Hermitian; H2HIP_EINVAL otherwise). */
 int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
                               const h2hip_h2matrix *H2, const double *c2);
 /* The batch's solves use method 0, GMRES(kmax) (the default), or 1,
@@ -407,7 +445,8 @@ int h2hip_gmres_batch_combine(h2hip_gmres_batch *G, const double *c1,
  * test's solver; for Hermitian positive definite operators; kmax >= 3, as
  * CG keeps its vectors in the first three Krylov columns).  With CG a chunk
  * is always 8 steps and the device's stopping flag freezes each solve at
- * its stopping step.  Only while no solve is in progress. */
+ * its stopping step.  Only while no solve is in progress; CG is refused
+ * once h2hip_gmres_batch_combine has set a combination. */
 int h2hip_gmres_batch_set_method(h2hip_gmres_batch *G, int method);
 /* Number of free slots. */
 int h2hip_gmres_batch_free_slots(const h2hip_gmres_batch *G);

@@ -17,6 +17,20 @@ def pytest_configure(config):
                             "gpu: needs an MI355X (runs the HIP library)")
 
 
+@pytest.fixture(autouse=True, scope="session")
+def reproducible_products():
+    """GPU tests run the products and Krylov reductions in reproducible mode
+    (fixed-order sums), so that results compare bit for bit between runs and
+    between batched and single solves; tests of the atomic schedules switch
+    it off themselves."""
+    try:
+        import h2lib_amd
+        h2lib_amd.set_reproducible(True)
+    except Exception:  # library not built: CPU tests need no GPU state
+        pass
+    yield
+
+
 def golden(name):
     """Load a committed fixture (.npz, no pickles)."""
     with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as f:

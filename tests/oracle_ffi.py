@@ -19,6 +19,8 @@ REF_SO = {False: os.path.join(ROOT, "oracle", "_ref", "libh2ref_real.so"),
 REF_HIP_SO = {
     False: os.path.join(ROOT, "oracle", "_ref", "libh2ref_hip_real.so"),
     True: os.path.join(ROOT, "oracle", "_ref", "libh2ref_hip_complex.so")}
+# the reference built for speed (-O3, AVX2/FMA), timed as the CPU baseline
+REF_FAST_SO = os.path.join(ROOT, "oracle", "_ref", "libh2ref_fast_real.so")
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -34,7 +36,9 @@ def _idx(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
-def have_reference(complex_=False, hip=False):
+def have_reference(complex_=False, hip=False, fast=False):
+    if fast:
+        return not complex_ and not hip and os.path.exists(REF_FAST_SO)
     return os.path.exists((REF_HIP_SO if hip else REF_SO)[complex_])
 
 
@@ -312,11 +316,14 @@ class Reference:
     _libs = {}
 
     @classmethod
-    def lib(cls, complex_=False, hip=False):
-        key = (bool(complex_), bool(hip))
+    def lib(cls, complex_=False, hip=False, fast=False):
+        key = (bool(complex_), bool(hip), bool(fast))
         if key not in cls._libs:
-            lib = C.CDLL((REF_HIP_SO if hip else REF_SO)[complex_],
-                         mode=C.RTLD_LOCAL)
+            if fast and (complex_ or hip):
+                raise ValueError("the fast reference build is real only")
+            path = REF_FAST_SO if fast else \
+                (REF_HIP_SO if hip else REF_SO)[complex_]
+            lib = C.CDLL(path, mode=C.RTLD_LOCAL)
             lib.ref_mesh_size.argtypes = [C.c_int, C.c_uint,
                                           C.POINTER(C.c_uint),
                                           C.POINTER(C.c_uint)]
@@ -427,8 +434,9 @@ class ReferenceBem(_BemBase):
     holds the H2Lib-side binding of the GPU library; use_hip() then installs
     the GPU callbacks into the reference's own bem3d object."""
 
-    def __init__(self, kernel, op, basis_r, basis_c, x, t, hip=False, **kw):
-        lib = Reference.lib(kernel == 1, hip)
+    def __init__(self, kernel, op, basis_r, basis_c, x, t, hip=False,
+                 fast=False, **kw):
+        lib = Reference.lib(kernel == 1, hip, fast)
         self._lib = lib
         self.hip = hip
         self._new = lib.ref_bem_new

@@ -236,8 +236,6 @@ def test_batched_cg_equals_single():
     xg = h.DeviceArray(n)
     xg.zero()
     done = _run_batch(B, [d_bs[2]], [xg], 1e-10, [3000])
-    # FP64 atomics in the products: when the threshold falls at a restart
-    # the step count can differ by up to one restart cycle (kmax = 30)
-    assert abs(done[0][0] - it_g) <= 30
-    x0 = d_x.download()
-    assert np.abs(xg.download() - x0).max() < 1e-8 * np.abs(x0).max()
+    # reproducible mode: the slot repeats the one-system solver bit for bit
+    assert done[0][0] == it_g
+    assert np.array_equal(xg.download(), d_x.download())

@@ -145,3 +145,18 @@ def test_linear_basis_single_layer_of_unit_density(sphere8):
     for c in range(3):
         np.add.at(supp, t[:, c], area / 3.0)
     assert np.abs(V.sum(axis=1) / supp - 1.0).max() < 3e-2
+
+
+@pytest.mark.skipif(not have_reference(fast=True),
+                    reason="fast reference build absent")
+@pytest.mark.parametrize("name", ["lap_slp_ll", "lap_dlp_ll", "lap_slp_cc"])
+def test_fast_reference_build_matches(name, sphere8):
+    """The reference built for speed (the benchmark's CPU baseline, -O3 with
+    contracted multiply-adds) gives the entries of the -O2 build to 1e-13
+    of each block's largest entry, so its timing is of the same work."""
+    kern, op, br, bc, alpha, k = CASES[name]
+    a = ReferenceBem(kern, op, br, bc, sphere8["x"], sphere8["t"], q=2, q2=4,
+                     alpha=alpha, k=k)
+    b = ReferenceBem(kern, op, br, bc, sphere8["x"], sphere8["t"], q=2, q2=4,
+                     alpha=alpha, k=k, fast=True)
+    assert rel_block_err(b.nearfield(), a.nearfield()) < 1e-13

@@ -4,8 +4,8 @@
   other meshes and problems run, batched with other requests of its
   operators) as solved alone with the one-system solver of its method on
   the same operators (CG with V for Laplace, GMRES with the Brakhage-Werner
-  operator KM - i k V for Helmholtz): to 1e-8 relative (both stop at residual 1e-10 |b|; the
-  batched product sums in another order).
+  operator KM - i k V for Helmholtz): in reproducible mode bit for bit, with
+  the same number of steps.
 * With a device-memory budget that holds about one operator pair, idle
   operators are evicted and rebuilt on demand, and every request still
   completes with its residual bound met.
@@ -24,7 +24,13 @@ def catalog():
     return meshes, {k: mesh_bounds(m) for k, m in meshes.items()}
 
 
-def check_against_alone(srv, reqs, tol):
+def check_against_alone(srv, reqs, exact):
+    """exact: the request was served by the operator it is checked with
+    (same bits, same steps); otherwise the operator may have been evicted
+    and rebuilt, and a rebuilt linear-basis near field (element
+    contributions summed with atomics) matches the served one to rounding
+    only: 1e-8, CG steps within max(2, 3 %), GMRES within one restart cycle
+    (kmax = 30) when the threshold falls at a restart."""
     import h2lib_amd as h
     from h2lib_amd.solve import solve_cg, solve_gmres
     for r in reqs:
@@ -44,12 +50,15 @@ def check_against_alone(srv, reqs, tol):
         err = np.abs(r.x - xa).max() / np.abs(xa).max()
         print("request", r.rid, r.key, "steps", r.steps, it,
               "err %.2e" % err, flush=True)
-        assert err < tol, r.rid
-        # FP64 atomics in the products: CG steps agree to a few; a GMRES
-        # solve whose threshold falls at a restart can end up to one
-        # restart cycle (kmax = 30) apart
-        allowed = 30 if e.batch.method == "gmres" else max(2, 0.03 * it)
-        assert abs(int(r.steps) - int(it)) <= allowed
+        if exact:
+            # reproducible mode: a batched slot runs the one-system solver's
+            # arithmetic, so the same steps and the same bits
+            assert int(r.steps) == int(it), r.rid
+            assert err == 0.0, r.rid
+        else:
+            assert err < 1e-8, r.rid
+            allowed = 30 if e.batch.method == "gmres" else max(2, 0.03 * it)
+            assert abs(int(r.steps) - int(it)) <= allowed
 
 
 def test_each_result_same_alone_and_in_mix():
@@ -65,7 +74,8 @@ def test_each_result_same_alone_and_in_mix():
     # requests really overlapped: some were admitted before others finished
     assert any(a.admitted < b.done and a.rid != b.rid and a.key != b.key
                for a in reqs for b in reqs)
-    check_against_alone(srv, reqs, 1e-8)
+    assert srv.evictions == 0
+    check_against_alone(srv, reqs, True)
 
 
 def test_admission_by_footprint_evicts_idle_operators():
@@ -80,7 +90,7 @@ def test_admission_by_footprint_evicts_idle_operators():
     print("builds", srv.builds, "evictions", srv.evictions, flush=True)
     assert len({r.key for r in reqs}) > 1 and srv.evictions > 0
     assert srv.used <= srv.budget
-    check_against_alone(srv, reqs, 1e-8)
+    check_against_alone(srv, reqs, False)
 
 
 def test_fill_completes_before_work_on_other_streams():
