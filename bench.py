@@ -28,6 +28,10 @@ import os
 import sys
 import time
 
+# CPUs this process may run on, read before the OpenMP runtime starts (with
+# OMP_PROC_BIND the runtime binds the calling thread to one place, and the
+# affinity query would report that one CPU)
+AVAIL_CPUS = len(os.sched_getaffinity(0))
 # threads of the CPU baseline stay on neighbouring cores (read by the
 # OpenMP runtime when it starts)
 os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -70,8 +74,7 @@ def compact(d):
             "dirichlet_solves_per_s": r4(g["solves_per_s"]),
             "p50_ms": r4(g["p50_ms"]), "p99_ms": r4(g["p99_ms"]),
             "rel_err": r4(g["rel_err"]),
-            "dirichlet": "GMRES(30) to 1e-10|b|, %d steps, data->solution "
-                         "on the host" % g["steps"],
+            "dirichlet": "GMRES(30) to 1e-10|b|, %d steps" % g["steps"],
             "cg": {"solves_per_s": r4(c["solves_per_s"]),
                    "p99_ms": r4(c["p99_ms"]), "rel_err": r4(c["rel_err"]),
                    "steps": c["steps"]}})
@@ -93,9 +96,11 @@ def compact(d):
                              (mv["computed_couplings"] or {}).get("ms"))}
     mp = d.get("mixed_precision")
     if mp:
-        out["mixed"] = {"entries_per_s": r4(mp["entries_per_s"]),
-                        "err": r4(max(e["max_rel_to_max"] for e in
-                                      mp["entry_err_vs_fp64"].values()))}
+        out["mixed"] = {
+            m: [r4(mp[m]["entries_per_s"]),
+                r4(max(e["max_rel_to_max"] for e in
+                       mp[m]["entry_err_vs_fp64"].values()))]
+            for m in ("mode1", "mode2")}
     hm = d.get("helmholtz_mixed")
     if hm:
         out["helmholtz"] = {k: [r4(v["fp64_entries_per_s"]),
@@ -160,7 +165,7 @@ def cpu_baseline(mesh, tree, nb, entries, args):
     if not fast and not have_reference(False):
         return None
     # the box's CPU share for one GPU is 16 cores, whatever the host shows
-    avail = len(os.sched_getaffinity(0))
+    avail = AVAIL_CPUS
     threads = max(1, min(args.cpu_threads, avail))
     ops = [ReferenceBem(0, 0, 1, 1, mesh.x, mesh.t, q=2, q2=4, fast=fast),
            ReferenceBem(0, 1, 1, 1, mesh.x, mesh.t, q=2, q2=4, alpha=0.5,
@@ -178,16 +183,14 @@ def cpu_baseline(mesh, tree, nb, entries, args):
     dt1 = time.perf_counter() - t0
     return {"value": 2 * entries / dt, "unit": "entries/s", "cores": threads,
             "kind": "reference", "cpu_model": cpu_model(),
-            "host_cpus": os.cpu_count(), "affinity_cpus": avail,
             "value_1thread": ents1 / dt1,
             "build": ("gcc -O3 -march=x86-64-v3 -fopenmp" if fast else
-                      "gcc -O2 -ffp-contract=off -fopenmp"),
-            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "sample": ("Laplace SLP + DLP near field (linear basis) of all "
-                       "%d near-field blocks (%d entries) of the same "
-                       "workload, %.2f s on %d threads; one thread: every "
-                       "16th block (%d entries), %.2f s" %
-                       (len(nb), 2 * entries, dt, threads, ents1, dt1))}
+                      "gcc -O2 -fopenmp"),
+            "sample": ("SLP+DLP near field, all %d blocks, %.1f s; 1 thread: "
+                       "every 16th block, %.1f s" % (len(nb), dt, dt1)),
+            "_detail": {"host_cpus": os.cpu_count(), "affinity_cpus": avail,
+                        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                        "entries": 2 * entries, "entries_1thread": ents1}}
 
 
 def bench_dense(h, stream, reps=20):
@@ -270,22 +273,28 @@ def entry_errors(mp, f64):
 
 
 def bench_mixed(h, ops, plan, outs, stream, steps):
-    """The same near-field blocks with mixed-precision kernel evaluations
-    (FP32 kernel values, FP64 geometry differences and outer sums), timed
-    like the FP64 steps, with the entry error against the FP64 result."""
-    mp_outs = [h.DeviceArray(plan.entries, o.dtype) for o in outs]
-    for op in ops:
-        op.set_precision(1)
-    try:
-        ms = time_plan(h, ops, plan, mp_outs, stream, steps)
-        errs = [entry_errors(m.download(), o.download())
-                for m, o in zip(mp_outs, outs)]
-    finally:
+    """The same near-field blocks with mixed-precision kernel evaluations,
+    timed like the FP64 steps, with the entry error against the FP64
+    result: mode 1 (FP32 kernel values and inner sums, FP64 geometry
+    differences and outer sums) and mode 2 (FP32 accumulation throughout
+    for the regular pairs, outer sums compensated)."""
+    res = {}
+    for mode in (1, 2):
+        mp_outs = [h.DeviceArray(plan.entries, o.dtype) for o in outs]
         for op in ops:
-            op.set_precision(0)
-    return {"entries_per_s": plan.entries * len(ops) / (ms * 1e-3),
+            op.set_precision(mode)
+        try:
+            ms = time_plan(h, ops, plan, mp_outs, stream, steps)
+            errs = [entry_errors(m.download(), o.download())
+                    for m, o in zip(mp_outs, outs)]
+        finally:
+            for op in ops:
+                op.set_precision(0)
+        res["mode%d" % mode] = {
+            "entries_per_s": plan.entries * len(ops) / (ms * 1e-3),
             "ms_per_step": ms,
             "entry_err_vs_fp64": {"slp": errs[0], "dlp": errs[1]}}
+    return res
 
 
 def bench_helmholtz_mixed(h, args, stream):
@@ -311,14 +320,19 @@ def bench_helmholtz_mixed(h, args, stream):
         op32.set_precision(1)
         ms64 = time_plan(h, [op64], plan, [out64], stream, args.helm_steps)
         ms32 = time_plan(h, [op32], plan, [out32], stream, args.helm_steps)
+        e32 = entry_errors(out32.download(), out64.download())
+        op32.set_precision(2)
+        msc = time_plan(h, [op32], plan, [out32], stream, args.helm_steps)
         res["k"]["%g" % k] = {
             "fp64_entries_per_s": plan.entries / (ms64 * 1e-3),
             "mixed_entries_per_s": plan.entries / (ms32 * 1e-3),
-            "fp64_ms": ms64, "mixed_ms": ms32,
-            "entry_err_vs_fp64": entry_errors(out32.download(),
-                                              out64.download())}
-        log("bench: helmholtz k=%g fp64 %.2f ms mixed %.2f ms" %
-            (k, ms64, ms32))
+            "compensated_entries_per_s": plan.entries / (msc * 1e-3),
+            "fp64_ms": ms64, "mixed_ms": ms32, "compensated_ms": msc,
+            "entry_err_vs_fp64": e32,
+            "compensated_entry_err_vs_fp64": entry_errors(
+                out32.download(), out64.download())}
+        log("bench: helmholtz k=%g fp64 %.2f ms mixed %.2f ms compensated "
+            "%.2f ms" % (k, ms64, ms32, msc))
     return res
 
 
@@ -456,27 +470,41 @@ def bench_batched(h, args, stream):
 def bench_helmholtz_solve(h, args, stream):
     """Exterior Helmholtz problem on the unit sphere split args.helm_split
     (131072 triangles by default), piecewise constant basis, point source
-    (0.3, 0.2, 0.1) inside: f = projected Dirichlet data, b = (K - M/2) f,
-    V x = b by GMRES(30) to 1e-8, with both near fields in FP64 and in
-    mixed precision, H2-matrices by interpolation of order 4 (leaf size
-    64, eta 2) with complex FP64 couplings.  Per k: solve time, steps,
+    (0.3, 0.2, 0.1) inside, near fields in FP64 and in mixed precision,
+    H2-matrices by interpolation (leaf size 64), complex FP64 couplings.
+    k below 6: f = projected Dirichlet data, b = (K - M/2) f, V x = b by
+    GMRES(30) to 1e-8 (x: the Neumann data), order 4, eta 2; per k the
     Neumann error of each solution and the mixed solution's error against
-    the FP64 one."""
+    the FP64 one.  k = 8 lies 0.18 below an interior Dirichlet eigenvalue of
+    the ball, where V is singular, and order 4 does not resolve e^{ikr}
+    across the largest admissible blocks (17.8 % row-sum error on the
+    8192-triangle sphere, the same for the reference's H2-matrix): the
+    Brakhage-Werner form (KM - i k V) x = b of the reference's Helmholtz
+    test with order 5 and eta 1, and the error of its exterior potential
+    on the test's grid."""
     from h2lib_amd import geometry as G
-    from h2lib_amd.solve import HelmholtzExterior
+    from h2lib_amd.solve import (HelmholtzBrakhageWerner, HelmholtzExterior,
+                                 solve_gmres)
     mesh = G.sphere(args.helm_split)
     x0 = np.array([0.3, 0.2, 0.1])
     out = {"workload": ("exterior Helmholtz, unit sphere %d triangles, cc, "
                         "point source (0.3, 0.2, 0.1), GMRES(30) to 1e-8, "
-                        "interpolation order 4, clf 64" % mesh.triangles),
-           "k": {}}
+                        "clf 64; k < 6: V x = (K - M/2) f, order 4, eta 2; "
+                        "k >= 6: Brakhage-Werner, order 5, eta 1" %
+                        mesh.triangles), "k": {}}
     for k in args.helm_k:
-        res = {}
+        bw = k >= 6.0
+        res = {"form": "brakhage-werner" if bw else "single layer"}
         sol = {}
         for prec, name in ((0, "fp64"), (1, "mixed")):
             t0 = time.perf_counter()
-            prob = HelmholtzExterior(mesh, k, clf=64, m=4, precision=prec,
-                                     stream=stream)
+            if bw:
+                prob = HelmholtzBrakhageWerner(mesh, k, clf=64, eta_adm=1.0,
+                                               m=5, precision=prec,
+                                               stream=stream)
+            else:
+                prob = HelmholtzExterior(mesh, k, clf=64, m=4,
+                                         precision=prec, stream=stream)
             stream.sync()
             build = time.perf_counter() - t0
             # the single layer products compute their couplings (measured
@@ -488,24 +516,35 @@ def bench_helmholtz_solve(h, args, stream):
                 log("bench: stored Helmholtz couplings: %s" % exc)
                 mode = "stored"
             t0 = time.perf_counter()
-            _, d_b = prob.rhs(x0, stream)
-            d_x, it, r = prob.solve(d_b, eps=1e-8, maxiter=args.helm_maxiter,
-                                    kmax=30, stream=stream)
+            if bw:
+                d_b = prob.rhs(x0, stream)
+                d_x = h.DeviceArray(prob.n, np.complex128)
+                d_x.zero(stream)
+                it, r = solve_gmres(prob.A, d_b, d_x, 1e-8,
+                                    args.helm_maxiter, 30, stream)
+            else:
+                _, d_b = prob.rhs(x0, stream)
+                d_x, it, r = prob.solve(d_b, eps=1e-8,
+                                        maxiter=args.helm_maxiter, kmax=30,
+                                        stream=stream)
             x = d_x.download()
             dt = time.perf_counter() - t0
             sol[name] = x
             res[name] = {"solve_s": dt, "steps": int(it),
                          "single_layer_couplings": mode,
-                         "operator_build_s": build,
-                         "neumann_rel_err": prob.neumann_error(x, x0)}
+                         "operator_build_s": build}
+            if bw:
+                res[name]["outer_potential_err"] = prob.outer_error(d_x, x0)
+            else:
+                res[name]["neumann_rel_err"] = prob.neumann_error(x, x0)
             del prob
         res["mixed_solution_err_vs_fp64"] = float(
             np.linalg.norm(sol["mixed"] - sol["fp64"]) /
             np.linalg.norm(sol["fp64"]))
         out["k"]["%g" % k] = res
-        log("bench: helmholtz solve k=%g fp64 %d steps %.2f s, mixed %d "
+        log("bench: helmholtz solve k=%g (%s) fp64 %d steps %.2f s, mixed %d "
             "steps %.2f s, mixed vs fp64 %.2e" %
-            (k, res["fp64"]["steps"], res["fp64"]["solve_s"],
+            (k, res["form"], res["fp64"]["steps"], res["fp64"]["solve_s"],
              res["mixed"]["steps"], res["mixed"]["solve_s"],
              res["mixed_solution_err_vs_fp64"]))
     return out
@@ -1043,6 +1082,8 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the reference on the host cores")
         cpu = cpu_baseline(mesh, tree, nb, plan.entries, args)
+        if cpu:
+            detail["cpu_baseline"] = cpu.pop("_detail")
 
     line = {
         "metric": METRIC,
@@ -1056,19 +1097,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (octahedral sphere mesh generated in-process)",
+        "data": "synthetic sphere mesh",
         "config": {
-            "workload": ("Laplace SLP + DLP near field, piecewise linear "
-                         "basis, unit sphere %d triangles / %d vertices, "
-                         "strict block tree clf=%d eta=%g (max-norm "
-                         "admissibility), q=2 q2=4, FP64" %
-                         (mesh.triangles, mesh.vertices, args.clf, args.eta)),
+            "workload": ("Laplace SLP+DLP ll near field, sphere %d "
+                         "triangles, clf=%d eta=%g q=2 q2=4" %
+                         (mesh.triangles, args.clf, args.eta)),
             "entries_per_step": int(entries_per_step),
             "kernel_ms": [round(float(v), 4) for v in k_avg],
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "nf_regular_kernel + nf_singular_kernel",
+            "kernel": "nf_regular+nf_singular",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",

@@ -107,7 +107,9 @@ class Bem3d:
 
     def set_precision(self, mode):
         """0: FP64 kernel evaluations (default); 1: mixed precision (FP32
-        kernel values, FP64 geometry differences and outer sums)."""
+        kernel values, FP64 geometry differences and outer sums); 2: as 1
+        with FP32 accumulation of the regular pairs throughout, the outer
+        sums compensated (Kahan)."""
         check(hip().h2hip_op_set_precision(self._h, int(mode)),
               "h2hip_op_set_precision")
         self.precision = int(mode)

@@ -20,8 +20,8 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr uint32_t TARGET_BLOCKS = 2048;  // 8 workgroups per CU
-constexpr uint32_t MIN_CHUNK = 16;        // columns per chunk at least
+constexpr uint32_t TARGET_BLOCKS = 1024;  // 4 workgroups per CU
+constexpr uint32_t MIN_CHUNK = 32;        // columns per chunk at least
 
 struct GemvArgs {
   const double *A;
@@ -66,13 +66,14 @@ __global__ __launch_bounds__(NT) void gemv_part(GemvArgs a) {
   if (i >= a.rows) return;
   const uint32_t j0 = blockIdx.y * a.cchunk;
   const uint32_t j1 = min(a.cols, j0 + a.cchunk);
-  // four independent sums keep several loads in flight per lane
+  // four independent sums, eight loads in flight per lane
   double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t j = j0;
-  for (; j + 4 <= j1; j += 4) {
+  for (; j + 8 <= j1; j += 8) {
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-      madd<C>(a.A, i + (uint64_t)(j + u) * a.ld, a.x, j + u, sr[u], si[u]);
+    for (int u = 0; u < 8; u++)
+      madd<C>(a.A, i + (uint64_t)(j + u) * a.ld, a.x, j + u, sr[u & 3],
+              si[u & 3]);
   }
   for (; j < j1; j++) madd<C>(a.A, i + (uint64_t)j * a.ld, a.x, j, sr[0], si[0]);
   const double tr = (sr[0] + sr[1]) + (sr[2] + sr[3]);

@@ -331,7 +331,7 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.regoff = op->d_regoff.p;
   a.regwf = op->d_regwf.p;
   a.q2 = op->q2;
-  a.mp = op->precision == 1 ? 1 : 0;
+  a.mp = op->precision;
   a.kc = KERNEL_CONST;
   a.kr = op->k[0];
   a.ki = -op->k[1];
@@ -556,12 +556,12 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
 void h2hip_op_destroy(h2hip_op *op) { delete op; }
 
 int h2hip_op_set_precision(h2hip_op *op, int mode) {
-  if (!op || mode < 0 || mode > 1) {
+  if (!op || mode < 0 || mode > 2) {
     set_error("h2hip_op_set_precision: invalid arguments");
     return H2HIP_EINVAL;
   }
   std::lock_guard<std::mutex> guard(op->lock);
-  if (mode == 1 && !op->d_regoff.p) {
+  if (mode >= 1 && !op->d_regoff.p) {
     const uint32_t qq = op->q * op->q;
     std::vector<float> wf(op->regw.begin(), op->regw.end());
     if (op->d_regwf.upload(wf.data(), wf.size()) != hipSuccess ||

@@ -136,9 +136,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Kahan's compensated sum s += x in FP32 (c carries the rounding error of
+// the previous additions; the compiler must not reassociate, and does not
+// without fast-math)
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+  const float y = x - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
 // One tile of the regular kernel, run by all NTHREADS threads of a
-// workgroup.
-template <int KER, int BR, int BC, int Q, bool MP>
+// workgroup.  MP: 0 FP64; 1 mixed (FP32 kernel values and inner sums, FP64
+// outer sums); 2 mixed with FP32 accumulation throughout, the outer sums
+// compensated (Kahan).
+template <int KER, int BR, int BC, int Q, int MP>
 __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
   constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
@@ -219,6 +231,13 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
         nz = (float)a.n[3 * sc.tri + 2];
       }
       f2 = a.kc * a.g[sr.tri] * a.g[sc.tri];
+      // MP 2: the outer sums in FP32 with their compensations
+      float Fre[NR][NC], Fim[NR][NC], Cre[NR][NC], Cim[NR][NC];
+#pragma unroll
+      for (int l = 0; l < NR; l++)
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+          Fre[l][k] = Fim[l][k] = Cre[l][k] = Cim[l][k] = 0.0f;
 #pragma unroll
       for (int b = 0; b < QQ; b++) {
         const float qx = Os[3 * b], qy = Os[3 * b + 1], qz = Os[3 * b + 2];
@@ -239,13 +258,32 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
         }
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-          const double wc = BC ? W[QQ * (1 + k) + b] : W[b];
+          if constexpr (MP == 2) {
+            const float wc = BC ? WF[QQ * (1 + k) + b] : WF[b];
 #pragma unroll
-          for (int l = 0; l < NR; l++) {
-            Sre[l][k] = fma(wc, (double)Ure[l], Sre[l][k]);
-            if constexpr (CPLX) Sim[l][k] = fma(wc, (double)Uim[l], Sim[l][k]);
+            for (int l = 0; l < NR; l++) {
+              kahan_add(Fre[l][k], Cre[l][k], wc * Ure[l]);
+              if constexpr (CPLX) kahan_add(Fim[l][k], Cim[l][k], wc * Uim[l]);
+            }
+          } else {
+            const double wc = BC ? W[QQ * (1 + k) + b] : W[b];
+#pragma unroll
+            for (int l = 0; l < NR; l++) {
+              Sre[l][k] = fma(wc, (double)Ure[l], Sre[l][k]);
+              if constexpr (CPLX)
+                Sim[l][k] = fma(wc, (double)Uim[l], Sim[l][k]);
+            }
           }
         }
+      }
+      if constexpr (MP == 2) {
+#pragma unroll
+        for (int l = 0; l < NR; l++)
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            Sre[l][k] = (double)Fre[l][k] - (double)Cre[l][k];
+            Sim[l][k] = (double)Fim[l][k] - (double)Cim[l][k];
+          }
       }
     } else if (!MP && !shared) {
       // The reference's lc far integrator maps the first rule coordinates
@@ -345,7 +383,7 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
   }
 }
 
-template <int KER, int BR, int BC, int Q, bool MP>
+template <int KER, int BR, int BC, int Q, int MP>
 __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
   regular_tile<KER, BR, BC, Q, MP>(a, blockIdx.x);
 }
@@ -353,7 +391,7 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_kernel(KArgs a) {
 // Persistent form: a grid sized to the resident workgroups takes tiles from
 // a counter (zeroed before the launch) until all ntiles are taken; every
 // workgroup leaves the loop once the counter has passed ntiles.
-template <int KER, int BR, int BC, int Q, bool MP>
+template <int KER, int BR, int BC, int Q, int MP>
 __global__ __launch_bounds__(NTHREADS) void nf_regular_persistent(
     KArgs a, uint32_t ntiles, uint32_t *counter) {
   __shared__ uint32_t cur;
@@ -637,22 +675,197 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
             }
         }
       } else if constexpr (CLS == CLS_EDGE) {
-        // six points per tuple: unrolling i as well costs too many registers
+        // The six points of SSTuple<CLS_EDGE> with e1 = g_i, e2 = g_j,
+        // x1 = g_k, x2 = g_l (1 - x1), s = x1 + x2, p = x1 e1, pq = p e2:
+        //   (tx, sx, ty, sy) = (s, pq, s - p, x1 - p), (s, p, s - pq, x1 - pq),
+        //   (s, x1, p + x2, pq), (s - p + pq, pq, s, x1), (pq + x2, pq, s, p),
+        //   (p + x2, p, s, pq).
+        // The pair's first two (permuted) vertices are the shared ones, so
+        // D0 = 0 and E1 = F1: d = E1 (tx - ty) + E2 sx - F2 sy is affine in
+        // pq for fixed i.  In every point one side's moments are fixed for
+        // fixed i and the other side's affine in pq, so the sums over j need
+        // sum u and sum u pq per point and the moment matrix is updated once
+        // per i.
+        const double x1 = gk, x2 = gl * (1.0 - x1), s_ = x1 + x2;
+        const double wkl = wk * (wl * (1.0 - x1)) * x1 * x1;
+        // d = base + cp p + cq pq per point m
+        double cA[3], cB[3], cC[3], Gv[3], Hv[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          cA[d] = -F2[d] * x1;
+          cB[d] = (E1[d] + E2[d]) * x1;
+          cC[d] = -E1[d] * x1;
+          Gv[d] = E1[d] + F2[d];
+          Hv[d] = E1[d] + E2[d];
+        }
 #pragma unroll 1
         for (int i = 0; i < Q2; i++) {
-          const double gi = a.gauss[i], wi = a.gauss[Q2 + i];
+          const double e1 = gn[i];
+          const double wi1 = wkl * gw[i] * e1;
+          const double p = x1 * e1;
+          double bs[6][3];
 #pragma unroll
-          for (int j = 0; j < Q2; j++)
-            SSTuple<CLS>::run(gi, gn[j], gk, gl, wi, gw[j], wk, wl, point);
+          for (int d = 0; d < 3; d++) {
+            bs[0][d] = fma(Gv[d], p, cA[d]);
+            bs[1][d] = fma(E2[d], p, cA[d]);
+            bs[2][d] = fma(-E1[d], p, cB[d]);
+            bs[3][d] = fma(-E1[d], p, cA[d]);
+            bs[4][d] = fma(-F2[d], p, cC[d]);
+            bs[5][d] = fma(Hv[d], p, cC[d]);
+          }
+          double W0r[6], W1r[6], W0i[6], W1i[6];
+#pragma unroll
+          for (int m = 0; m < 6; m++) W0r[m] = W1r[m] = W0i[m] = W1i[m] = 0.0;
+#pragma unroll
+          for (int j = 0; j < Q2; j++) {
+            const double pq = p * gn[j], w = wi1 * gw[j];
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+              double dd[3];
+#pragma unroll
+              for (int d = 0; d < 3; d++) {
+                const double cq = m == 0 ? E2[d]
+                                : m == 1 ? Gv[d]
+                                : m == 2 ? -F2[d]
+                                : m == 5 ? -F2[d]
+                                         : Hv[d];
+                dd[d] = fma(cq, pq, bs[m][d]);
+              }
+              double vr, vi;
+              value(dd, vr, vi);
+              const double ur = w * vr;
+              W0r[m] += ur;
+              W1r[m] = fma(ur, pq, W1r[m]);
+              if constexpr (CPLX) {
+                const double ui = w * vi;
+                W0i[m] += ui;
+                W1i[m] = fma(ui, pq, W1i[m]);
+              }
+            }
+          }
+          // row-affine points (1, 4, 5): M += T (x) ms; column-affine
+          // points (2, 3, 6): M += mt (x) S
+          const double ms1[3] = {1.0, s_ - p, x1 - p};
+          const double ms4[3] = {1.0, s_, x1};
+          const double ms5[3] = {1.0, s_, p};
+          const double mt2[3] = {1.0, s_, p};
+          const double mt3[3] = {1.0, s_, x1};
+          const double mt6[3] = {1.0, p + x2, p};
+          auto accum = [&](const double *W0, const double *W1, double (&Sx)[NR][NC]) {
+            const double T1[3] = {W0[0], s_ * W0[0], W1[0]};
+            const double S2[3] = {W0[1], s_ * W0[1] - W1[1],
+                                  x1 * W0[1] - W1[1]};
+            const double S3[3] = {W0[2], (p + x2) * W0[2], W1[2]};
+            const double T4[3] = {W0[3], (s_ - p) * W0[3] + W1[3], W1[3]};
+            const double T5[3] = {W0[4], x2 * W0[4] + W1[4], W1[4]};
+            const double S6[3] = {W0[5], s_ * W0[5], W1[5]};
+#pragma unroll
+            for (int l = 0; l < NR; l++)
+#pragma unroll
+              for (int kk = 0; kk < NC; kk++) {
+                double t = Sx[l][kk];
+                t = fma(T1[l], ms1[kk], t);
+                t = fma(mt2[l], S2[kk], t);
+                t = fma(mt3[l], S3[kk], t);
+                t = fma(T4[l], ms4[kk], t);
+                t = fma(T5[l], ms5[kk], t);
+                t = fma(mt6[l], S6[kk], t);
+                Sx[l][kk] = t;
+              }
+          };
+          accum(W0r, W1r, Sre);
+          if constexpr (CPLX) accum(W0i, W1i, Sim);
         }
       } else {
+        // The six points of SSTuple<CLS_IDENTICAL> with eta = g_i,
+        // x1 = g_j, x2 = g_k (1 - x1), x3 = g_l (1 - x1 - x2),
+        // a = x1 + x2 + x3, b = x1 (1 - eta):
+        //   (tx, sx, ty, sy) = (a, x1 + x2, b + x2 + x3, x2),
+        //   (b + x2 + x3, b + x2, a, x2), (a, x1 eta + x2, x2 + x3, x2),
+        //   (b + x2 + x3, x2, a, x1 + x2), (a, x2, b + x2 + x3, b + x2),
+        //   (x2 + x3, x2, a, x1 eta + x2).
+        // Both triangles are the same (D0 = 0, E1 = F1, E2 = F2), so
+        // d = E1 (tx - ty) + E2 (sx - sy) = x1 u with
+        // u = eta E1 + E2, -eta E1 + (1 - eta) E2, E1 + eta E2 and their
+        // negatives: the difference is formed without cancellation.  For
+        // fixed j one side's moments are fixed and the other's affine in b,
+        // so the sums over i need sum u and sum u b per point.
 #pragma unroll 1
-        for (int i = 0; i < Q2; i++) {
-          const double gi = a.gauss[i], wi = a.gauss[Q2 + i];
-#pragma unroll 1
-          for (int j = 0; j < Q2; j++)
-            SSTuple<CLS>::run(gi, a.gauss[j], gk, gl, wi, a.gauss[Q2 + j], wk,
-                              wl, point);
+        for (int j = 0; j < Q2; j++) {
+          const double x1 = gn[j];
+          const double x2 = gk * (1.0 - x1);
+          const double x3 = gl * (1.0 - x1 - x2);
+          const double aa = x1 + x2 + x3;
+          const double wj1 = gw[j] * (wk * (1.0 - x1)) *
+                             (wl * (1.0 - x1 - x2)) * x1;
+          double W0r[6], W1r[6], W0i[6], W1i[6];
+#pragma unroll
+          for (int m = 0; m < 6; m++) W0r[m] = W1r[m] = W0i[m] = W1i[m] = 0.0;
+#pragma unroll
+          for (int i = 0; i < Q2; i++) {
+            const double eta = gn[i], w = gw[i] * wj1;
+            const double bb = x1 * (1.0 - eta);
+            double dv[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              dv[0][d] = x1 * fma(eta, E1[d], E2[d]);
+              dv[1][d] = x1 * fma(1.0 - eta, E2[d], -eta * E1[d]);
+              dv[2][d] = x1 * fma(eta, E2[d], E1[d]);
+            }
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+              double dd[3];
+#pragma unroll
+              for (int d = 0; d < 3; d++)
+                dd[d] = m < 3 ? dv[m][d] : -dv[m - 3][d];
+              double vr, vi;
+              value(dd, vr, vi);
+              // point order: P1 = +u1, P2 = +u2, P3 = +u3, P4 = -u1,
+              // P5 = -u2, P6 = -u3
+              const double ur = w * vr;
+              W0r[m] += ur;
+              W1r[m] = fma(ur, bb, W1r[m]);
+              if constexpr (CPLX) {
+                const double ui = w * vi;
+                W0i[m] += ui;
+                W1i[m] = fma(ui, bb, W1i[m]);
+              }
+            }
+          }
+          // m = 0: P1, 1: P2, 2: P3, 3: P4, 4: P5, 5: P6
+          const double x23 = x2 + x3, x12 = x1 + x2;
+          const double mt1[3] = {1.0, aa, x12};
+          const double ms2[3] = {1.0, aa, x2};
+          const double ms3[3] = {1.0, x23, x2};
+          const double ms4[3] = {1.0, aa, x12};
+          const double mt5[3] = {1.0, aa, x2};
+          const double mt6[3] = {1.0, x23, x2};
+          auto accum = [&](const double *W0, const double *W1,
+                           double (&Sx)[NR][NC]) {
+            const double S1[3] = {W0[0], fma(x23, W0[0], W1[0]), x2 * W0[0]};
+            const double T2[3] = {W0[1], fma(x23, W0[1], W1[1]),
+                                  fma(x2, W0[1], W1[1])};
+            const double T3[3] = {W0[2], aa * W0[2], fma(x12, W0[2], -W1[2])};
+            const double T4[3] = {W0[3], fma(x23, W0[3], W1[3]), x2 * W0[3]};
+            const double S5[3] = {W0[4], fma(x23, W0[4], W1[4]),
+                                  fma(x2, W0[4], W1[4])};
+            const double S6[3] = {W0[5], aa * W0[5], fma(x12, W0[5], -W1[5])};
+#pragma unroll
+            for (int l = 0; l < NR; l++)
+#pragma unroll
+              for (int kk = 0; kk < NC; kk++) {
+                double t = Sx[l][kk];
+                t = fma(mt1[l], S1[kk], t);
+                t = fma(T2[l], ms2[kk], t);
+                t = fma(T3[l], ms3[kk], t);
+                t = fma(T4[l], ms4[kk], t);
+                t = fma(mt5[l], S5[kk], t);
+                t = fma(mt6[l], S6[kk], t);
+                Sx[l][kk] = t;
+              }
+          };
+          accum(W0r, W1r, Sre);
+          if constexpr (CPLX) accum(W0i, W1i, Sim);
         }
       }
     }
@@ -753,8 +966,10 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   }
 }
 
-// The launches of one precision (MP: mixed).
-template <int KER, int BR, int BC, bool MP>
+// The launches of one precision (MP: 0 FP64, 1 mixed, 2 mixed with
+// compensated FP32 outer sums; the singular kernels are the same for 1 and
+// 2).
+template <int KER, int BR, int BC, int MP>
 int launch_bases_p(uint32_t q, const KArgs &a, uint32_t ntiles,
                    const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
   if (ntiles && a.counter) {
@@ -795,12 +1010,13 @@ int launch_bases_p(uint32_t q, const KArgs &a, uint32_t ntiles,
                  n1 = ncls[CLS_VERTEX];
   using SK = void (*)(KArgs, const SPair *, uint32_t);
   const bool q4 = a.q2 == 4;
-  const SK k3 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 4, MP>
-                   : nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 0, MP>;
-  const SK k2 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_EDGE, 4, MP>
-                   : nf_singular_kernel<KER, BR, BC, CLS_EDGE, 0, MP>;
-  const SK k1 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 4, MP>
-                   : nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 0, MP>;
+  constexpr bool SMP = MP != 0;
+  const SK k3 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 4, SMP>
+                   : nf_singular_kernel<KER, BR, BC, CLS_IDENTICAL, 0, SMP>;
+  const SK k2 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_EDGE, 4, SMP>
+                   : nf_singular_kernel<KER, BR, BC, CLS_EDGE, 0, SMP>;
+  const SK k1 = q4 ? nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 4, SMP>
+                   : nf_singular_kernel<KER, BR, BC, CLS_VERTEX, 0, SMP>;
   if (n3)
     hipLaunchKernelGGL(k3, dim3((n3 + PPB - 1) / PPB), dim3(NTHREADS), 0,
                        stream, a, sp, n3);
@@ -820,10 +1036,11 @@ int launch_bases_p(uint32_t q, const KArgs &a, uint32_t ntiles,
 template <int KER, int BR, int BC>
 int launch_bases(uint32_t q, const KArgs &a, uint32_t ntiles,
                  const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
-  return a.mp ? launch_bases_p<KER, BR, BC, true>(q, a, ntiles, sp, ncls,
-                                                  stream)
-              : launch_bases_p<KER, BR, BC, false>(q, a, ntiles, sp, ncls,
-                                                   stream);
+  if (a.mp == 2)
+    return launch_bases_p<KER, BR, BC, 2>(q, a, ntiles, sp, ncls, stream);
+  if (a.mp == 1)
+    return launch_bases_p<KER, BR, BC, 1>(q, a, ntiles, sp, ncls, stream);
+  return launch_bases_p<KER, BR, BC, 0>(q, a, ntiles, sp, ncls, stream);
 }
 
 template <int KER>

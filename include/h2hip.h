@@ -97,13 +97,17 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
                     uint32_t q_regular, uint32_t q_singular,
                     const double *alpha, const double *k);
 void h2hip_op_destroy(h2hip_op *op);
-/* Precision of the kernel evaluations: 0 FP64 (default), 1 mixed.  Mixed:
+/* Precision of the kernel evaluations: 0 FP64 (default), 1 or 2 mixed.
+ * Mixed:
  * kernel values (1/r, the phase, e^{ikr}) in FP32 for every pair; regular
  * pairs form x - y from an FP64 difference of anchor points plus FP32
  * offsets and sum the inner rule dimension in FP32, the outer in FP64;
  * singular pairs keep the Sauter-Schwab points, x - y and the sums in FP64.
- * Entries agree with FP64 to about 1e-6 relative.  No counterpart in the
- * reference (it computes in its build's field type only). */
+ * Entries agree with FP64 to about 1e-6 relative.  2: as 1, but the
+ * regular pairs accumulate in FP32 throughout, the sums over the outer
+ * rule dimension compensated (Kahan); singular pairs as in 1.  No
+ * counterpart in the reference (it computes in its build's field type
+ * only). */
 int h2hip_op_set_precision(h2hip_op *op, int mode);
 
 /* ---- the near-field callback ---------------------------------------------

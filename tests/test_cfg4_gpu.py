@@ -11,15 +11,17 @@ Parity and accuracy, each against a checker independent of the path:
   wave number;
 * the row sums of the single layer operator (near field plus the H2
   far field) against the closed form |T_i| e^{ik} sin(k)/k of the unit
-  sphere (a single layer of unit density), within 2e-3 relative: the
-  polyhedral surface's error, measured at 2.4e-2 / 6.0e-3 (k = 2) on the
-  2048- / 8192-triangle spheres, falls as h^2 (about 1e-4 here);
+  sphere (a single layer of unit density): within 2e-3 relative at k = 2
+  (the polyhedral surface's error, 2.4e-2 / 6.0e-3 on the 512- / 2048-
+  triangle spheres, falls as h^2), within 3e-2 at k = 8 with interpolation
+  of order 5 and eta = 1, and mixed against FP64 within 1e-6;
 * the exterior Dirichlet problem with a point source at (0.3, 0.2, 0.1):
   FP64 and mixed solutions meet their true residual bound, agree within
   1e-5, and the Neumann error lies in the band of the discretisation
   (k = 2); at k = 8, close to an interior Dirichlet eigenvalue of the ball
   where V is singular, the Brakhage-Werner form KM - i k V is solved
-  instead, FP64 against mixed within 1e-5.
+  instead (interpolation of order 5, eta = 1), FP64 against mixed within
+  1e-5.
 """
 import numpy as np
 import pytest
@@ -73,25 +75,43 @@ def test_mixed_entries_against_fp64(cfg4, k):
             assert rel_block_err(plan.block(a64, b), want) < 1e-12, b
 
 
-@pytest.mark.parametrize("k", [2.0, 8.0])
-@pytest.mark.parametrize("precision", [0, 1])
-def test_row_sums_closed_form(cfg4, k, precision):
+@pytest.mark.parametrize("k,m,eta,tol", [(2.0, 4, 2.0, 2e-3),
+                                          (8.0, 5, 1.0, 3e-2)])
+def test_row_sums_closed_form(cfg4, k, m, eta, tol):
+    """At k = 8 interpolation of order 4 with eta = 2 cannot resolve
+    e^{ikr} across the largest admissible blocks: the reference's own
+    H2-matrix of that configuration misses the closed form by 17.8 % on the
+    8192-triangle sphere (order 5, eta = 1: 1.7 %; the surface's own error
+    there is 1.5e-3), so the check at k = 8 uses order 5 and eta = 1.  The
+    mixed-precision product must give the FP64 one's row sums to 1e-6."""
     import h2lib_amd as h
+    from h2lib_amd import geometry as G
     from h2lib_amd.h2matrix import H2Matrix
-    mesh, tree, bt, nb, gr, plan = cfg4
-    op = h.new_slp_helmholtz_bem3d(k, gr, 2, 4, 0, 0)
-    op.set_precision(precision)
-    H = H2Matrix(op, mesh, tree, tree, bt, 4, nearplan=plan)
-    H.set_schedule(2)
-    ones = h.DeviceArray.from_host(np.ones(mesh.triangles, np.complex128))
-    y = h.DeviceArray.from_host(np.zeros(mesh.triangles, np.complex128))
-    H.addeval(1.0, ones, y)
-    rs = y.download() / (0.5 * mesh.g)
+    mesh, tree, bt0, nb0, gr, plan0 = cfg4
+    bt = bt0 if eta == 2.0 else G.BlockTree(tree, tree, eta, 0)
+    plan = plan0 if eta == 2.0 else h.NearfieldPlan(
+        gr, 0, 0, tree.idx, tree.idx, bt.near_block_list())
     exact = np.exp(1j * k) * np.sin(k) / k
-    err = np.abs(rs - exact).max() / abs(exact)
-    print("row sums k", k, "precision", precision, "rel err %.2e" % err,
-          flush=True)
-    assert err < 2e-3
+    rows = {}
+    for precision in (0, 1):
+        op = h.new_slp_helmholtz_bem3d(k, gr, 2, 4, 0, 0)
+        op.set_precision(precision)
+        H = H2Matrix(op, mesh, tree, tree, bt, m, nearplan=plan)
+        H.set_schedule(2)
+        ones = h.DeviceArray.from_host(np.ones(mesh.triangles,
+                                               np.complex128))
+        y = h.DeviceArray.from_host(np.zeros(mesh.triangles, np.complex128))
+        H.addeval(1.0, ones, y)
+        rs = y.download() / (0.5 * mesh.g)
+        err = np.abs(rs - exact).max() / abs(exact)
+        print("row sums k", k, "m", m, "eta", eta, "precision", precision,
+              "rel err %.2e" % err, flush=True)
+        assert err < tol
+        rows[precision] = rs
+        del H
+    d = np.abs(rows[1] - rows[0]).max() / np.abs(rows[0]).max()
+    print("mixed vs fp64 row sums %.2e" % d, flush=True)
+    assert d < 1e-6
 
 
 def test_exterior_problem_k2_fp64_and_mixed(cfg4):
@@ -133,7 +153,7 @@ def test_brakhage_werner_k8_fp64_and_mixed(cfg4):
     st = h.Stream()
     sols = {}
     for prec in (0, 1):
-        prob = HelmholtzBrakhageWerner(mesh, 8.0, clf=64, eta_adm=2.0, m=4,
+        prob = HelmholtzBrakhageWerner(mesh, 8.0, clf=64, eta_adm=1.0, m=5,
                                        precision=prec, stream=st)
         d_b = prob.rhs(x0, st)
         d_x = h.DeviceArray(prob.n, np.complex128)
@@ -145,7 +165,9 @@ def test_brakhage_werner_k8_fp64_and_mixed(cfg4):
         st.sync()
         true = np.linalg.norm(r.download()) / np.linalg.norm(d_b.download())
         print("BW k=8 precision", prec, "steps", it,
-              "true residual %.2e" % true, flush=True)
+              "true residual %.2e" % true,
+              "outer potential error %.2e" % prob.outer_error(d_x, x0),
+              flush=True)
         assert it < 999 and true <= 1.5e-8
         sols[prec] = d_x.download()
         del prob

@@ -34,27 +34,42 @@ CASES = {
 }
 
 
-def pair(mesh, name, q=2, q2=4):
+def pair(mesh, name, q=2, q2=4, mode=1):
     import h2lib_amd as h
     kern, op, br, bc, alpha, k = CASES[name]
     gr = h.Surface3d(mesh["x"], mesh["t"])
     a = h.Bem3d(gr, kern, op, q, q2, br, bc, alpha=alpha, k=k)
     b = h.Bem3d(gr, kern, op, q, q2, br, bc, alpha=alpha, k=k)
-    b.set_precision(1)
+    b.set_precision(mode)
     return a, b
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_mixed_dense_close_to_fp64(name, sphere8):
-    a, b = pair(sphere8, name)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mixed_dense_close_to_fp64(name, mode, sphere8):
+    """mode 1: FP64 outer sums; mode 2: FP32 accumulation with compensated
+    outer sums (the same bound; the kernel values' FP32 rounding
+    dominates)."""
+    a, b = pair(sphere8, name, mode=mode)
     for far in (False, True):
         f64 = a.nearfield_far(None, None) if far else a.nearfield(None, None)
         mp = b.nearfield_far(None, None) if far else b.nearfield(None, None)
         err = rel_block_err(mp, f64)
-        print("mixed", name, "far" if far else "near", "err %.3e" % err,
-              flush=True)
+        print("mixed", mode, name, "far" if far else "near",
+              "err %.3e" % err, flush=True)
         assert err < TOL, far
         assert err > 0.0  # the mixed kernels did run
+
+
+def test_compensated_sums_differ_from_mode1(sphere8):
+    """Mode 2 runs its own kernels: its entries differ from mode 1's in the
+    last bits of the FP32 sums, and both stay within the bound."""
+    a, b1 = pair(sphere8, "lap_slp_ll", mode=1)
+    _, b2 = pair(sphere8, "lap_slp_ll", mode=2)
+    f64 = a.nearfield(None, None)
+    m1, m2 = b1.nearfield(None, None), b2.nearfield(None, None)
+    assert not np.array_equal(m1, m2)
+    assert rel_block_err(m2, f64) < TOL and rel_block_err(m1, f64) < TOL
 
 
 @pytest.mark.parametrize("name", ["lap_slp_ll", "helm_dlp_cc", "lap_dlp_lc",
@@ -86,4 +101,4 @@ def test_precision_switch_back_to_fp64(sphere8):
 def test_invalid_precision_mode_is_rejected(sphere8):
     a, _ = pair(sphere8, "lap_slp_cc")
     with pytest.raises(RuntimeError):
-        a.set_precision(2)
+        a.set_precision(3)
