@@ -174,6 +174,7 @@ def cpu_baseline(mesh, tree, nb, entries, args):
     for op in ops:
         op.nearfield_blocks(tree.idx, tree.idx, nb, threads)
     dt = time.perf_counter() - t0
+    dropin = dropin_assembly(mesh, args)
     nb = np.asarray(nb)
     sample = nb[::16]
     ents1 = int(sum(int(b[1]) * int(b[3]) for b in sample)) * len(ops)
@@ -188,9 +189,31 @@ def cpu_baseline(mesh, tree, nb, entries, args):
                       "gcc -O2 -fopenmp"),
             "sample": ("SLP+DLP near field, all %d blocks, %.1f s; 1 thread: "
                        "every 16th block, %.1f s" % (len(nb), dt, dt1)),
+            "dropin_assembly_s": dropin,
             "_detail": {"host_cpus": os.cpu_count(), "affinity_cpus": avail,
                         "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
                         "entries": 2 * entries, "entries_1thread": ents1}}
+
+
+def dropin_assembly(mesh, args):
+    """The reference's own assemble_bem3d_h2matrix (single layer, linear
+    basis, interpolation order 4, clf / eta of the workload) with its CPU
+    callbacks, with the GPU callbacks installed by setup_hip_bem3d, and with
+    the deferred GPU callbacks and one flush: wall seconds of each (the
+    reference built with the binding, -O2, its OpenMP tasks on the host
+    cores; the couplings are filled on the host in all three)."""
+    from oracle_ffi import ReferenceBem, ReferenceH2, have_reference
+    if not have_reference(False, True):
+        return None
+    out = {}
+    for mode in ("cpu", "gpu", "gpu_deferred"):
+        ref = ReferenceBem(0, 0, 1, 1, mesh.x, mesh.t, hip=True, q=2, q2=4)
+        if mode != "cpu":
+            ref.use_hip(deferred=mode == "gpu_deferred")
+        H = ReferenceH2(ref, args.clf, args.eta, 4, 0)
+        out[mode] = round(H.assemble_s, 4)
+        del H, ref
+    return out
 
 
 def bench_dense(h, stream, reps=20):

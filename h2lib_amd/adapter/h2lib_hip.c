@@ -106,8 +106,62 @@ hip_nearfield_far(const uint * ridx, const uint * cidx, pcbem3d bem,
   hip_fill(ridx, cidx, bem, ntrans, N, 1);
 }
 
+/* Deferred callbacks: the block is recorded and N written by the flush, as
+ * the reference's OpenCL callbacks enqueue their blocks
+ * (Library/laplaceoclbem3d.c:248-279). */
+static void
+hip_defer(const uint * ridx, const uint * cidx, pcbem3d bem, bool ntrans,
+          pamatrix N, int far)
+{
+  uint      rows = ntrans ? N->cols : N->rows;
+  uint      cols = ntrans ? N->rows : N->cols;
+
+  if (h2hip_nearfield_defer(lookup_op(bem), far, ridx, rows, cidx, cols,
+                            ntrans ? 1 : 0, N->a, N->ld) != H2HIP_OK)
+    hip_fail("nearfield (deferred)");
+}
+
+static void
+hip_nearfield_deferred(const uint * ridx, const uint * cidx, pcbem3d bem,
+                       bool ntrans, pamatrix N)
+{
+  hip_defer(ridx, cidx, bem, ntrans, N, 0);
+}
+
+static void
+hip_nearfield_far_deferred(const uint * ridx, const uint * cidx,
+                           pcbem3d bem, bool ntrans, pamatrix N)
+{
+  hip_defer(ridx, cidx, bem, ntrans, N, 1);
+}
+
+static void setup_hip(pbem3d bem, int helmholtz, int dlp);
+
 void
 setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp)
+{
+  setup_hip(bem, helmholtz, dlp);
+  bem->nearfield = hip_nearfield;
+  bem->nearfield_far = hip_nearfield_far;
+}
+
+void
+setup_hip_deferred_bem3d(pbem3d bem, int helmholtz, int dlp)
+{
+  setup_hip(bem, helmholtz, dlp);
+  bem->nearfield = hip_nearfield_deferred;
+  bem->nearfield_far = hip_nearfield_far_deferred;
+}
+
+void
+flush_hip_bem3d(pcbem3d bem)
+{
+  if (h2hip_nearfield_flush(lookup_op(bem)) != H2HIP_OK)
+    hip_fail("nearfield flush");
+}
+
+static void
+setup_hip(pbem3d bem, int helmholtz, int dlp)
 {
   pcsurface3d gr = bem->gr;
   hipbem   *e;
@@ -136,8 +190,6 @@ setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp)
   e->next = registry;
   registry = e;
   pthread_mutex_unlock(&registry_lock);
-  bem->nearfield = hip_nearfield;
-  bem->nearfield_far = hip_nearfield_far;
 }
 
 void

@@ -33,6 +33,16 @@
  * and the quadrature orders are taken from bem; alpha and k likewise. */
 void setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp);
 
+/* The same with deferred callbacks: bem->nearfield and bem->nearfield_far
+ * only record their blocks (N is not written yet), and flush_hip_bem3d
+ * computes all recorded blocks in one plan and writes them.  Callers flush
+ * after the assembly and before they read the near-field blocks, e.g.
+ *   assemble_bem3d_h2matrix(bem, h2); flush_hip_bem3d(bem);
+ * the deferred completion of the reference's OpenCL callbacks
+ * (Library/laplaceoclbem3d.c:248-279, Library/opencl.h:359-367). */
+void setup_hip_deferred_bem3d(pbem3d bem, int helmholtz, int dlp);
+void flush_hip_bem3d(pcbem3d bem);
+
 /* Releases the device data of bem (call before del_bem3d). */
 void release_hip_bem3d(pcbem3d bem);
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -54,6 +55,17 @@ template <class T> struct DevBuf {
     hipError_t e = alloc(count);
     if (e != hipSuccess || count == 0) return e;
     return hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice);
+  }
+  // as upload, but keeps an allocation that is large enough (n is then the
+  // capacity), ordered on a stream
+  hipError_t upload_grow(const T *src, size_t count, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    if (!p || n < count) {
+      hipError_t e = alloc(count);
+      if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice,
+                          st);
   }
 };
 
@@ -190,6 +202,11 @@ namespace h2hip {
 int surface_vertex_lists(const h2hip_surface *s);
 }  // namespace h2hip
 
+namespace h2hip {
+struct CbCtx;
+struct Deferred;
+}  // namespace h2hip
+
 struct h2hip_op {
   const h2hip_surface *s = nullptr;
   int kernel = 0, layer = 0, rb = 0, cb = 0;
@@ -209,8 +226,18 @@ struct h2hip_op {
   // premultiplied by the weights
   h2hip::DevBuf<double> d_regw;
   std::vector<double> regw;
-  mutable std::mutex lock;  // serialises the blocking nearfield callback
-  mutable h2hip::DevBuf<unsigned char> scratch;
+  mutable std::mutex lock;  // guards set_precision against the callbacks
+  // contexts of the blocking callback h2hip_nearfield (created on first
+  // use): calls from several host threads each take a free one and run
+  // concurrently on its stream
+  mutable std::mutex cb_lock;
+  mutable std::condition_variable cb_cv;
+  mutable std::vector<h2hip::CbCtx *> cb_ctx;
+  // blocks recorded by h2hip_nearfield_defer (0 near, 1 far rule) until
+  // h2hip_nearfield_flush
+  mutable std::mutex defer_lock;
+  mutable h2hip::Deferred *defer = nullptr;
+  ~h2hip_op();
 };
 
 struct h2hip_nfplan {

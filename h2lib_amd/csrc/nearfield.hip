@@ -14,6 +14,7 @@
 // into the singular list with their rule class and vertex permutation.
 #include <algorithm>
 #include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -48,7 +49,7 @@ __global__ void nf_fixup_kernel(const Fixup *fix, uint32_t nfix,
   const double v = 0.5 * g[f.tri];
   if (cplx) {
     atomicAdd(&out[2 * f.pos], v * are);
-    atomicAdd(&out[2 * f.pos + 1], v * (ntrans ? -aim : aim));
+    atomicAdd(&out[2 * f.pos + 1], v * ((ntrans | f.pad) ? -aim : aim));
   } else {
     atomicAdd(&out[f.pos], v * are);
   }
@@ -161,10 +162,21 @@ int classify(const uint32_t *tv, const uint32_t *sv, uint16_t &perm) {
   return p;
 }
 
+// ntr: per-block ntrans flags (nullptr: ntrans for every block); st: the
+// uploads keep large enough device buffers of P and are ordered on st
+// (otherwise new buffers, complete on return).
 int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
                const uint32_t *ridx_all, uint32_t nridx,
                const uint32_t *cidx_all, uint32_t ncidx, uint32_t nblocks,
-               const uint32_t *blocks, int ntrans) {
+               const uint32_t *blocks, int ntrans,
+               const uint8_t *ntr = nullptr, bool grow = false,
+               hipStream_t st = nullptr) {
+  P->off.clear();
+  P->tiles.clear();
+  P->supp.clear();
+  P->fix.clear();
+  P->spairs.clear();
+  for (auto &c : P->pairs) c = 0;
   P->s = s;
   P->rb = rb;
   P->cb = cb;
@@ -194,7 +206,8 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
         return H2HIP_EINVAL;
       }
     const uint64_t boff = total;
-    const uint32_t ld = ntrans ? cols : rows;
+    const int bnt = ntr ? (ntr[b] ? 1 : 0) : ntrans;
+    const uint32_t ld = bnt ? cols : rows;
     P->off[b] = boff;
     total += (uint64_t)rows * cols;
     for (uint32_t r0 = 0; r0 < rows; r0 += TILE) {
@@ -213,8 +226,8 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
           return H2HIP_EINVAL;
         }
         Tile T;
-        T.out_off = boff + (ntrans ? (uint64_t)c0 + (uint64_t)r0 * ld
-                                   : (uint64_t)r0 + (uint64_t)c0 * ld);
+        T.out_off = boff + (bnt ? (uint64_t)c0 + (uint64_t)r0 * ld
+                                : (uint64_t)r0 + (uint64_t)c0 * ld);
         T.ld = ld;
         T.rsup = rs0;
         T.csup = cs0;
@@ -224,7 +237,7 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
         T.col0 = c0;
         T.nr = (uint8_t)nr;
         T.nc = (uint8_t)nc;
-        T.ntrans = (uint8_t)ntrans;
+        T.ntrans = (uint8_t)bnt;
         // The reference's lc identity term is skipped for blocks with at
         // most three columns (Library/bem3d.c:2533 tests a loop variable
         // that equals 3 there).
@@ -248,7 +261,7 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
           Fixup f;
           f.pos = boff + (uint64_t)i + (uint64_t)i * ld;
           f.tri = ri[i];
-          f.pad = 0;
+          f.pad = (uint32_t)bnt;  // the block's ntrans
           P->fix.push_back(f);
         }
       }
@@ -303,6 +316,14 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
     P->pairs[c] = by_cls[c].size();
     P->pairs[0] -= by_cls[c].size();
     P->spairs.insert(P->spairs.end(), by_cls[c].begin(), by_cls[c].end());
+  }
+  if (grow) {
+    H2HIP_CHECK(P->d_tiles.upload_grow(P->tiles.data(), P->tiles.size(), st));
+    H2HIP_CHECK(P->d_supp.upload_grow(P->supp.data(), P->supp.size(), st));
+    H2HIP_CHECK(P->d_fix.upload_grow(P->fix.data(), P->fix.size(), st));
+    H2HIP_CHECK(
+        P->d_spairs.upload_grow(P->spairs.data(), P->spairs.size(), st));
+    return H2HIP_OK;
   }
   H2HIP_CHECK(P->d_tiles.upload(P->tiles.data(), P->tiles.size()));
   H2HIP_CHECK(P->d_supp.upload(P->supp.data(), P->supp.size()));
@@ -360,7 +381,7 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   H2HIP_CHECK(hipGetLastError());
   if (!far && !P->fix.empty() && (a.are != 0.0 || a.aim != 0.0)) {
     const uint32_t nfix = (uint32_t)P->fix.size();
-    const int ntr = P->tiles[0].ntrans;
+    const int ntr = 0;  // each fix carries its block's ntrans
     hipLaunchKernelGGL(nf_fixup_kernel, dim3((nfix + 255) / 256), dim3(256),
                        0, stream, P->d_fix.p, nfix, s->d_g.p, a.are, a.aim,
                        op->kernel == H2HIP_HELMHOLTZ ? 1 : 0, ntr,
@@ -371,7 +392,163 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
 }
 
 }  // namespace
+
+// One context of the blocking callback: its own stream, plan buffers,
+// device output and pinned staging memory, all kept between calls.
+struct CbCtx {
+  hipStream_t st = nullptr;
+  h2hip_nfplan plan;
+  DevBuf<unsigned char> out;
+  void *stage = nullptr;
+  size_t stage_cap = 0;
+  bool busy = false;
+  ~CbCtx() {
+    if (stage) (void)hipHostFree(stage);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  int stage_size(size_t bytes) {
+    if (stage_cap >= bytes) return H2HIP_OK;
+    if (stage) (void)hipHostFree(stage);
+    stage = nullptr;
+    stage_cap = 0;
+    if (hipHostMalloc(&stage, bytes, hipHostMallocDefault) != hipSuccess) {
+      stage = nullptr;
+      set_error("h2hip_nearfield: pinned host allocation failed");
+      return H2HIP_ENOMEM;
+    }
+    stage_cap = bytes;
+    return H2HIP_OK;
+  }
+};
+
+// Blocks recorded by h2hip_nearfield_defer: their index sets (appended to
+// ridx / cidx), (roff, rows, coff, cols), ntrans and the host windows.
+struct Deferred {
+  std::vector<uint32_t> ridx, cidx, blocks;
+  std::vector<uint8_t> ntr;
+  std::vector<void *> dst;
+  std::vector<size_t> ld;
+  void clear() {
+    ridx.clear();
+    cidx.clear();
+    blocks.clear();
+    ntr.clear();
+    dst.clear();
+    ld.clear();
+  }
+};
+
+namespace {
+
+constexpr int CB_CONTEXTS = 4;
+
+// Copies the packed column-major blocks of `stage` into their host
+// windows (block b: ld[b], written in the block's storage orientation),
+// split over a few threads for large copies.
+void scatter_blocks(const unsigned char *stage, const uint64_t *off,
+                    const uint32_t *blocks, const uint8_t *ntr,
+                    void *const *dst, const size_t *ld, size_t nblocks,
+                    size_t es) {
+  auto one = [&](size_t b) {
+    const uint32_t rows = blocks[4 * b + 1], cols = blocks[4 * b + 3];
+    const size_t wr = ntr[b] ? cols : rows, wc = ntr[b] ? rows : cols;
+    const unsigned char *src = stage + es * off[b];
+    unsigned char *d = (unsigned char *)dst[b];
+    if (ld[b] == wr) {
+      std::memcpy(d, src, es * wr * wc);
+    } else {
+      for (size_t j = 0; j < wc; j++)
+        std::memcpy(d + es * j * ld[b], src + es * j * wr, es * wr);
+    }
+  };
+  uint64_t bytes = 0;
+  for (size_t b = 0; b < nblocks; b++)
+    bytes += (uint64_t)blocks[4 * b + 1] * blocks[4 * b + 3];
+  bytes *= es;
+  const unsigned hw = std::max(1u, std::min(16u,
+                                            std::thread::hardware_concurrency()));
+  const unsigned nt = bytes < (16u << 20) ? 1u : hw;
+  if (nt <= 1 || nblocks < 2) {
+    for (size_t b = 0; b < nblocks; b++) one(b);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; t++)
+    pool.emplace_back([&]() {
+      for (size_t b; (b = next.fetch_add(1)) < nblocks;) one(b);
+    });
+  for (auto &th : pool) th.join();
+}
+
+// Plans, computes and writes one group of blocks (one rule) through a
+// callback context.
+int run_blocks(const h2hip_op *op, CbCtx &C, int far,
+               const uint32_t *ridx_all, uint32_t nridx,
+               const uint32_t *cidx_all, uint32_t ncidx, uint32_t nblocks,
+               const uint32_t *blocks, const uint8_t *ntr, void *const *dst,
+               const size_t *ld) {
+  if (!C.st && hipStreamCreateWithFlags(&C.st, hipStreamNonBlocking) !=
+                   hipSuccess) {
+    C.st = nullptr;
+    set_error("h2hip_nearfield: stream creation failed");
+    return H2HIP_EDEVICE;
+  }
+  int rc = build_plan(&C.plan, op->s, op->rb, op->cb, ridx_all, nridx,
+                      cidx_all, ncidx, nblocks, blocks, 0, ntr, true, C.st);
+  if (rc != H2HIP_OK) return rc;
+  const size_t es = op->kernel == H2HIP_HELMHOLTZ ? 16 : 8;
+  const size_t bytes = es * C.plan.entries;
+  if (bytes == 0) return H2HIP_OK;
+  if (C.out.n < bytes && C.out.alloc(bytes) != hipSuccess) {
+    set_error("h2hip_nearfield: device allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  rc = C.stage_size(bytes);
+  if (rc != H2HIP_OK) return rc;
+  {
+    std::lock_guard<std::mutex> guard(op->lock);  // precision stays fixed
+    rc = execute(op, &C.plan, far, C.out.p, C.st);
+  }
+  if (rc != H2HIP_OK) return rc;
+  H2HIP_CHECK(hipMemcpyAsync(C.stage, C.out.p, bytes, hipMemcpyDeviceToHost,
+                             C.st));
+  H2HIP_CHECK(hipStreamSynchronize(C.st));
+  scatter_blocks((const unsigned char *)C.stage, C.plan.off.data(), blocks,
+                 ntr, dst, ld, nblocks, es);
+  return H2HIP_OK;
+}
+
+// A free callback context of op (waits while all are busy).
+CbCtx *take_context(const h2hip_op *op) {
+  std::unique_lock<std::mutex> lk(op->cb_lock);
+  if (op->cb_ctx.empty())
+    for (int i = 0; i < CB_CONTEXTS; i++) op->cb_ctx.push_back(new CbCtx);
+  for (;;) {
+    for (CbCtx *c : op->cb_ctx)
+      if (!c->busy) {
+        c->busy = true;
+        return c;
+      }
+    op->cb_cv.wait(lk);
+  }
+}
+
+void give_context(const h2hip_op *op, CbCtx *c) {
+  {
+    std::lock_guard<std::mutex> lk(op->cb_lock);
+    c->busy = false;
+  }
+  op->cb_cv.notify_one();
+}
+
+}  // namespace
 }  // namespace h2hip
+
+h2hip_op::~h2hip_op() {
+  for (h2hip::CbCtx *c : cb_ctx) delete c;
+  delete[] defer;
+}
 
 using namespace h2hip;
 
@@ -593,7 +770,7 @@ int h2hip_nearfield(const h2hip_op *op, int far, const uint32_t *ridx,
     return H2HIP_EINVAL;
   }
   if (rows == 0 || cols == 0) return H2HIP_OK;
-  const size_t wr = ntrans ? cols : rows, wc = ntrans ? rows : cols;
+  const size_t wr = ntrans ? cols : rows;
   if (ld < wr) {
     set_error("h2hip_nearfield: leading dimension too small");
     return H2HIP_EINVAL;
@@ -609,24 +786,86 @@ int h2hip_nearfield(const h2hip_op *op, int far, const uint32_t *ridx,
     for (uint32_t j = 0; j < cols; j++) ci[j] = j;
     cidx = ci.data();
   }
-  h2hip_nfplan P;
   const uint32_t blk[4] = {0, rows, 0, cols};
-  int rc = build_plan(&P, op->s, op->rb, op->cb, ridx, rows, cidx, cols, 1,
-                      blk, ntrans);
-  if (rc != H2HIP_OK) return rc;
-  const size_t es = op->kernel == H2HIP_HELMHOLTZ ? 16 : 8;
-  const size_t bytes = es * wr * wc;
-  std::lock_guard<std::mutex> guard(op->lock);
-  if (op->scratch.n < bytes) {
-    if (op->scratch.alloc(bytes) != hipSuccess) {
-      set_error("h2hip_nearfield: device allocation failed");
-      return H2HIP_ENOMEM;
-    }
+  const uint8_t ntr = ntrans ? 1 : 0;
+  void *dst = N;
+  CbCtx *C = take_context(op);
+  const int rc = run_blocks(op, *C, far ? 1 : 0, ridx, rows, cidx, cols, 1,
+                            blk, &ntr, &dst, &ld);
+  give_context(op, C);
+  return rc;
+}
+
+int h2hip_nearfield_defer(const h2hip_op *op, int far, const uint32_t *ridx,
+                          uint32_t rows, const uint32_t *cidx, uint32_t cols,
+                          int ntrans, void *N, size_t ld) {
+  if (!op || (!N && rows && cols)) {
+    set_error("h2hip_nearfield_defer: invalid arguments");
+    return H2HIP_EINVAL;
   }
-  rc = execute(op, &P, far, op->scratch.p, nullptr);
-  if (rc != H2HIP_OK) return rc;
-  H2HIP_CHECK(hipMemcpy2D(N, ld * es, op->scratch.p, wr * es, wr * es, wc,
-                          hipMemcpyDeviceToHost));
+  if (rows == 0 || cols == 0) return H2HIP_OK;
+  const size_t wr = ntrans ? cols : rows;
+  if (ld < wr) {
+    set_error("h2hip_nearfield_defer: leading dimension too small");
+    return H2HIP_EINVAL;
+  }
+  const uint32_t nr = op->rb ? op->s->nv : op->s->nt;
+  const uint32_t nc = op->cb ? op->s->nv : op->s->nt;
+  for (uint32_t i = 0; ridx && i < rows; i++)
+    if (ridx[i] >= nr) {
+      set_error("h2hip_nearfield_defer: row index out of range");
+      return H2HIP_EINVAL;
+    }
+  for (uint32_t j = 0; cidx && j < cols; j++)
+    if (cidx[j] >= nc) {
+      set_error("h2hip_nearfield_defer: column index out of range");
+      return H2HIP_EINVAL;
+    }
+  std::lock_guard<std::mutex> guard(op->defer_lock);
+  if (!op->defer) op->defer = new Deferred[2];
+  Deferred &D = op->defer[far ? 1 : 0];
+  if (D.ridx.size() + rows > 0xffffffffull ||
+      D.cidx.size() + cols > 0xffffffffull) {
+    set_error("h2hip_nearfield_defer: too many deferred indices; flush "
+              "first");
+    return H2HIP_EINVAL;
+  }
+  D.blocks.push_back((uint32_t)D.ridx.size());
+  D.blocks.push_back(rows);
+  D.blocks.push_back((uint32_t)D.cidx.size());
+  D.blocks.push_back(cols);
+  for (uint32_t i = 0; i < rows; i++) D.ridx.push_back(ridx ? ridx[i] : i);
+  for (uint32_t j = 0; j < cols; j++) D.cidx.push_back(cidx ? cidx[j] : j);
+  D.ntr.push_back(ntrans ? 1 : 0);
+  D.dst.push_back(N);
+  D.ld.push_back(ld);
+  return H2HIP_OK;
+}
+
+int h2hip_nearfield_flush(const h2hip_op *op) {
+  if (!op) {
+    set_error("h2hip_nearfield_flush: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  Deferred D[2];
+  {
+    std::lock_guard<std::mutex> guard(op->defer_lock);
+    if (!op->defer) return H2HIP_OK;
+    std::swap(D[0], op->defer[0]);
+    std::swap(D[1], op->defer[1]);
+  }
+  for (int far = 0; far < 2; far++) {
+    const size_t nb = D[far].ntr.size();
+    if (nb == 0) continue;
+    CbCtx *C = take_context(op);
+    const int rc = run_blocks(
+        op, *C, far, D[far].ridx.data(), (uint32_t)D[far].ridx.size(),
+        D[far].cidx.data(), (uint32_t)D[far].cidx.size(), (uint32_t)nb,
+        D[far].blocks.data(), D[far].ntr.data(), D[far].dst.data(),
+        D[far].ld.data());
+    give_context(op, C);
+    if (rc != H2HIP_OK) return rc;
+  }
   return H2HIP_OK;
 }
 

@@ -116,13 +116,31 @@ int h2hip_op_set_precision(h2hip_op *op, int mode);
  * the column DOFs cidx (NULL: 0, 1, 2, ...).  With ntrans != 0 the
  * transposed cols x rows window is written (conjugated for the cc and ll
  * basis pairs, as in the reference).  far != 0 uses the regular rule for all
- * pairs (nearfield_far).  Blocks until N is written.
+ * pairs (nearfield_far).  Blocks until N is written.  Thread-safe: calls
+ * from several host threads run concurrently (up to four per operator, each
+ * on its own stream, with device and pinned host buffers kept between
+ * calls).
  * Replaces the callbacks bem->nearfield and bem->nearfield_far
  * (Library/bem3d.h:380-381, 405-406), i.e. assemble_{cc,cl,lc,ll}_near_bem3d
  * and assemble_{cc,cl,lc,ll}_far_bem3d (Library/bem3d.c:1150-3436). */
 int h2hip_nearfield(const h2hip_op *op, int far, const uint32_t *ridx,
                     uint32_t rows, const uint32_t *cidx, uint32_t cols,
                     int ntrans, void *N, size_t ld);
+/* The same callback, deferred: records the block (the index arrays are
+ * copied; N must stay valid and untouched until the flush) and returns at
+ * once without writing N.  Thread-safe; calls from several threads append
+ * to one queue per operator.  The queued blocks are computed and written by
+ * h2hip_nearfield_flush.  Replaces the enqueueing callbacks of the
+ * reference's OpenCL path (Library/laplaceoclbem3d.c:248-279). */
+int h2hip_nearfield_defer(const h2hip_op *op, int far, const uint32_t *ridx,
+                          uint32_t rows, const uint32_t *cidx, uint32_t cols,
+                          int ntrans, void *N, size_t ld);
+/* Computes every block deferred on op since the last flush (one plan and
+ * one launch set per rule) and writes each into its host window; returns
+ * when all are written.  Blocks deferred while the flush runs wait for the
+ * next one.  Replaces the wait for the OpenCL scheduler's queues
+ * (Library/opencl.h:359-367). */
+int h2hip_nearfield_flush(const h2hip_op *op);
 
 /* ---- batched, device-resident near field -----------------------------------
  * A plan covers many blocks of one index set pair at once: block b is the

@@ -22,6 +22,7 @@
  *   solve_gmres_avector                       Library/krylovsolvers.c:183
  */
 #include <stdlib.h>
+#include <omp.h>
 #include <string.h>
 
 #include "basic.h"
@@ -320,6 +321,10 @@ ref_nearfield(void *h, int far, const unsigned *ridx, unsigned rows,
     rb->bem->nearfield_far(ridx, cidx, rb->bem, ntrans ? true : false, A);
   else
     rb->bem->nearfield(ridx, cidx, rb->bem, ntrans ? true : false, A);
+#ifdef REF_WITH_HIP
+  if (rb->hip == 2)
+    flush_hip_bem3d(rb->bem);
+#endif
 }
 
 /* Near-field blocks (roff, rows, coff, cols) over the index arrays ridx /
@@ -349,6 +354,10 @@ ref_nearfield_blocks(void *h, const unsigned *ridx, const unsigned *cidx,
     rb->bem->nearfield(ridx + blocks[4 * b], cidx + blocks[4 * b + 2],
 		       rb->bem, false, A);
   }
+#ifdef REF_WITH_HIP
+  if (rb->hip == 2)
+    flush_hip_bem3d(rb->bem);
+#endif
 }
 
 /* ---------------------------------------------------------------
@@ -364,6 +373,7 @@ typedef struct {
   ph2matrix h2;
   real      eta;
   void     *hip;                /* phiph2matrix of the adapter build */
+  double    t_assemble;         /* seconds of assemble_bem3d_h2matrix */
 } refh2;
 
 /* admis: 0 admissible_max_cluster, 1 admissible_2_cluster. */
@@ -391,8 +401,23 @@ ref_h2_new(void *h, unsigned clf, double eta, unsigned m, int admis)
   assemble_bem3d_h2matrix_row_clusterbasis(rb->bem, r->rbasis);
   assemble_bem3d_h2matrix_col_clusterbasis(rb->bem, r->cbasis);
   r->h2 = build_from_block_h2matrix(r->broot, r->rbasis, r->cbasis);
+  r->t_assemble = omp_get_wtime();
   assemble_bem3d_h2matrix(rb->bem, r->h2);
+#ifdef REF_WITH_HIP
+  if (rb->hip == 2)
+    flush_hip_bem3d(rb->bem);
+#endif
+  r->t_assemble = omp_get_wtime() - r->t_assemble;
   return r;
+}
+
+/* Wall seconds of assemble_bem3d_h2matrix (near-field blocks through the
+ * callbacks and the couplings; with deferred callbacks including the
+ * flush) in ref_h2_new. */
+double
+ref_h2_assemble_seconds(void *h)
+{
+  return ((refh2 *) h)->t_assemble;
 }
 
 void
@@ -822,6 +847,17 @@ ref_bem_use_hip(void *h)
 
   setup_hip_bem3d(rb->bem, rb->kernel, rb->op);
   rb->hip = 1;
+}
+
+/* The deferred GPU callbacks (setup_hip_deferred_bem3d): ref_nearfield,
+ * ref_nearfield_blocks and ref_h2_new flush after their callers' loops. */
+void
+ref_bem_use_hip_deferred(void *h)
+{
+  refbem   *rb = (refbem *) h;
+
+  setup_hip_deferred_bem3d(rb->bem, rb->kernel, rb->op);
+  rb->hip = 2;
 }
 
 /* Precision of the GPU near field installed by ref_bem_use_hip. */

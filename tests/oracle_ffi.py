@@ -363,6 +363,8 @@ class Reference:
             lib.ref_nearfield_blocks.argtypes = [
                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p,
                 C.c_void_p, C.c_void_p, C.c_int]
+            lib.ref_h2_assemble_seconds.restype = C.c_double
+            lib.ref_h2_assemble_seconds.argtypes = [C.c_void_p]
             lib.ref_h2_new.restype = C.c_void_p
             lib.ref_h2_new.argtypes = [C.c_void_p, C.c_uint, C.c_double,
                                        C.c_uint, C.c_int]
@@ -387,6 +389,7 @@ class Reference:
                                       C.c_double, C.c_uint]
             if hip:
                 lib.ref_bem_use_hip.argtypes = [C.c_void_p]
+                lib.ref_bem_use_hip_deferred.argtypes = [C.c_void_p]
                 lib.ref_bem_hip_precision.argtypes = [C.c_void_p, C.c_int]
                 lib.ref_h2_hip_pack.argtypes = [C.c_void_p]
                 lib.ref_h2_hip_addeval.argtypes = [C.c_void_p, C.c_int,
@@ -489,12 +492,16 @@ class ReferenceBem(_BemBase):
                                              _ptr(xx), _ptr(out))
         return out
 
-    def use_hip(self):
+    def use_hip(self, deferred=False):
         """bem->nearfield / nearfield_far := the GPU callbacks
-        (setup_hip_bem3d)."""
+        (setup_hip_bem3d; deferred: setup_hip_deferred_bem3d, the driver
+        flushes after its loops)."""
         if not self.hip:
             raise RuntimeError("load the reference with hip=True")
-        self._lib.ref_bem_use_hip(self.h)
+        if deferred:
+            self._lib.ref_bem_use_hip_deferred(self.h)
+        else:
+            self._lib.ref_bem_use_hip(self.h)
 
     def hip_precision(self, mode):
         """Precision of the installed GPU callbacks
@@ -524,6 +531,8 @@ class ReferenceH2:
         self.bem = bem
         self._lib = bem._lib
         self.h = self._lib.ref_h2_new(bem.h, clf, eta, m, admis)
+        # wall seconds of the reference's assemble_bem3d_h2matrix
+        self.assemble_s = self._lib.ref_h2_assemble_seconds(self.h)
         self.n = self._lib.ref_h2_idx(self.h, None)
 
     def __del__(self):

@@ -142,3 +142,60 @@ def test_mixed_precision_through_binding(case):
     assert 0.0 < err < 2e-6
     hip.hip_precision(0)
     assert rel_block_err(hip.nearfield(), want) < TOL
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[3], CASES[5], CASES[7]])
+def test_deferred_callbacks_and_flush(case):
+    """setup_hip_deferred_bem3d: the reference's callbacks only record the
+    blocks, flush_hip_bem3d writes them (the driver flushes after its
+    loops): dense fills, transposed subsets and the far rule give the
+    immediate callbacks' entries."""
+    kern, layer, br, bc, alpha, k = case
+    g = golden("mesh_sphere_s4.npz")
+    x, t = g["x"], g["t"]
+    plain = ReferenceBem(kern, layer, br, bc, x, t, q=2, q2=4, alpha=alpha,
+                         k=k)
+    hip = ReferenceBem(kern, layer, br, bc, x, t, hip=True, q=2, q2=4,
+                       alpha=alpha, k=k)
+    hip.use_hip(deferred=True)
+    print("deferred binding case", case, flush=True)
+    for far in (False, True):
+        err = rel_block_err(hip.nearfield(far=far), plain.nearfield(far=far))
+        assert err < TOL, far
+    rng = np.random.default_rng(3)
+    nr, nc = plain.ndof(br), plain.ndof(bc)
+    ridx, cidx = subsets(rng, nr, nc, nr // 2, nc // 3, br == bc)
+    for ntrans in (False, True):
+        want = plain.nearfield(ridx, cidx, ntrans=ntrans)
+        got = hip.nearfield(ridx, cidx, ntrans=ntrans)
+        assert rel_block_err(got, want) < TOL, ntrans
+
+
+@pytest.mark.parametrize("kern,layer,basis,k", [(0, 1, 1, 0.0),
+                                                 (1, 0, 0, 2.0)])
+def test_deferred_reference_h2_assembly(kern, layer, basis, k):
+    """The reference's own assemble_bem3d_h2matrix with deferred callbacks
+    (its OpenMP tasks record their blocks concurrently, one flush computes
+    them) gives the near-field blocks and products of the CPU assembly."""
+    from h2lib_amd import geometry as G
+    mesh = G.sphere(12)
+    alpha = 0.5 if layer else 0.0
+    plain = ReferenceBem(kern, layer, basis, basis, mesh.x, mesh.t, q=2,
+                         q2=4, alpha=alpha, k=k)
+    hip = ReferenceBem(kern, layer, basis, basis, mesh.x, mesh.t, hip=True,
+                       q=2, q2=4, alpha=alpha, k=k)
+    hip.use_hip(deferred=True)
+    H0 = ReferenceH2(plain, 16, 2.0, 3, 0)
+    H1 = ReferenceH2(hip, 16, 2.0, 3, 0)
+    nb = H0.near_blocks()
+    print("deferred h2 assembly", kern, layer, basis, len(nb), "blocks",
+          flush=True)
+    for b in range(len(nb)):
+        r, c = int(nb[b, 1]), int(nb[b, 3])
+        assert rel(H1.near_entries(b, r, c), H0.near_entries(b, r, c)) < TOL
+    rng = np.random.default_rng(4)
+    xv = rng.uniform(-1, 1, H0.n).astype(plain.dtype)
+    y0 = np.zeros(H0.n, dtype=plain.dtype)
+    for trans in (False, True):
+        want = H0.addeval(xv, y0, trans=trans)
+        assert rel(H1.addeval(xv, y0, trans=trans), want) < TOL
