@@ -689,6 +689,108 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
       xs_all + (threadIdx.x >> 6) * XS);
 }
 
+// Adjoint blocks (yt_c += S^* xt_r, y|_c += alpha N^* x|_r) with one
+// workgroup per block for blocks of at most 64 x 64: the block is staged in
+// LDS with contiguous reads (a padded leading dimension keeps the column
+// reads of the lanes free of bank conflicts), each wave sums a quarter of
+// the inputs for every output (lane = output), and the four partial sums
+// are added in wave order.  Far blocks w < nfar, then near blocks.  The
+// per-block wave path above reads the adjoint's entries with a stride of a
+// column per lane.
+template <int C, int NV, bool ORD>
+__global__ __launch_bounds__(NT) void couple_trans_wg(CNArgs a) {
+  constexpr int LDM = 65;
+  __shared__ double Mr[64 * LDM];
+  __shared__ double Mi[C ? 64 * LDM : 1];
+  __shared__ double xs[NV][2][64];
+  __shared__ double part[WPB][NV][2][64];
+  const uint32_t w = blockIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const bool isfar = w < a.nfar;
+  uint32_t nin, nout;
+  const double *M;
+  FarD fb = {};
+  NearD nb = {};
+  if (isfar) {
+    fb = a.far[w];
+    nin = a.rn[fb.r].rank;
+    nout = a.cn[fb.c].rank;
+    M = a.S + (C ? 2 : 1) * fb.soff;
+  } else {
+    nb = a.near[w - a.nfar];
+    nin = nb.rows;
+    nout = nb.cols;
+    M = a.N + (C ? 2 : 1) * nb.noff;
+  }
+  // the block, column-major with leading dimension nin, into LDS
+  for (uint32_t e = tid; e < nin * nout; e += NT) {
+    const uint32_t i = e % nin, j = e / nin;
+    double mr, mi;
+    ld<C>(M, e, mr, mi);
+    Mr[i + j * LDM] = mr;
+    if constexpr (C) Mi[i + j * LDM] = mi;
+  }
+  for (uint32_t e = tid; e < NV * nin; e += NT) {
+    const uint32_t v = e / nin, i = e - v * nin;
+    double xr = 0.0, xi = 0.0;
+    if (v < a.v.nv) {
+      if (isfar)
+        ld<C>(a.xt + v * a.cstride, a.rn[fb.r].coef + i, xr, xi);
+      else
+        ld<C>(a.v.x[v], a.ridx[nb.roff + i], xr, xi);
+    }
+    xs[v][0][i] = xr;
+    xs[v][1][i] = xi;
+  }
+  __syncthreads();
+  const uint32_t q = (nin + WPB - 1) / WPB;
+  const uint32_t i0 = wv * q, i1 = min(nin, i0 + q);
+  if ((uint32_t)lane < nout) {
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+    for (uint32_t i = i0; i < i1; i++) {
+      const double mr = Mr[i + lane * LDM];
+      const double mi = C ? -Mi[i + lane * LDM] : 0.0;  // conjugate
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const double xr = xs[v][0][i], xi = xs[v][1][i];
+        ar[v] = fma(mr, xr, ar[v]);
+        if constexpr (C) {
+          ar[v] = fma(-mi, xi, ar[v]);
+          ai[v] = fma(mr, xi, fma(mi, xr, ai[v]));
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      part[wv][v][0][lane] = ar[v];
+      part[wv][v][1][lane] = ai[v];
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < NV * nout; e += NT) {
+    const uint32_t v = e / nout, j = e - v * nout;
+    if (v >= a.v.nv) continue;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int k = 0; k < WPB; k++) {
+      sr += part[k][v][0][j];
+      si += part[k][v][1][j];
+    }
+    if constexpr (ORD) {
+      st<C>(a.P + v * a.pstride,
+            a.pslot[isfar ? w : a.pnear0 + (w - a.nfar)] + j, sr, si);
+    } else if (isfar) {
+      atomic_add<C>(a.yt + v * a.cstride, a.cn[fb.c].coef + j, sr, si);
+    } else {
+      atomic_add<C>(a.v.y[v], a.cidx[nb.coff + j],
+                    a.v.are[v] * sr - a.v.aim[v] * si,
+                    a.v.are[v] * si + a.v.aim[v] * sr);
+    }
+  }
+}
+
 // Self-adjoint product from the lower block triangle (row and column
 // bases equal): one wave per block.  A far block (r, c) with r > c adds
 // S xt_c to yt_r and S^* xt_r to yt_c, reading S once: lane i keeps row
@@ -1126,6 +1228,7 @@ __global__ __launch_bounds__(NT) void back_flat_ord(FlatArgs a,
   if (l >= nleaves) return;
   double *ys = ys_all[threadIdx.x >> 6];
   const uint32_t t0 = ibeg[l], t1 = ibeg[l + 1];
+  if (t0 == t1) return;  // no ancestor of the leaf in an admissible block
   const NodeD L = a.nodes[a.items[t0].leaf];
   const uint32_t sz = L.size;
   double ar[2][NV], ai[2][NV];
@@ -1240,14 +1343,19 @@ __global__ __launch_bounds__(NT) void w_build(WBuildArgs a) {
   for (uint32_t j = lane; j < k; j += 64)
     row[j] = a.VT[L.voff + j + (uint64_t)i * k];
   __builtin_amdgcn_wave_barrier();
-  for (uint32_t p = a.ibeg[l + 1] - b; p-- > 0;) {
-    const FlatItem it = a.items[b + p];
-    for (uint32_t j = lane; j < k; j += 64) {
-      if (a.WF) a.WF[it.woff + (uint64_t)i * k + j] = row[j];
-      if (a.WB) a.WB[it.woff + (uint64_t)j * L.size + i] = row[j];
+  // the leaf's items (root first) name only the ancestors that take part
+  // in the products: walk every ancestor, write the blocks of those
+  for (uint32_t p = a.ibeg[l + 1] - b; p > 0;) {
+    const FlatItem it = a.items[b + p - 1];
+    if (it.node == cur) {
+      for (uint32_t j = lane; j < k; j += 64) {
+        if (a.WF) a.WF[it.woff + (uint64_t)i * k + j] = row[j];
+        if (a.WB) a.WB[it.woff + (uint64_t)j * L.size + i] = row[j];
+      }
+      if (--p == 0) break;
     }
-    if (p == 0) break;
     const uint32_t f = a.nodes[cur].father;
+    if (f == NONE) break;
     const uint32_t kf = a.nodes[f].rank;
     const double *ET = a.ET + (uint64_t)cur * a.eld;
     double nx[MAXRANK / 64];
@@ -1528,8 +1636,12 @@ using h2hip::set_error;
 
 namespace {
 
+// live: the clusters whose coefficients the products use (those in an
+// admissible block); the flat transforms keep items for them only (nullptr:
+// all clusters)
 int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
-               const uint32_t *rank, uint32_t ndof, const uint32_t *idx) {
+               const uint32_t *rank, uint32_t ndof, const uint32_t *idx,
+               const std::vector<char> *live = nullptr) {
   S.nnodes = nnodes;
   S.ndof = ndof;
   S.idx.assign(idx, idx + ndof);
@@ -1653,7 +1765,9 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
       if (height[i] == h) S.hnodes.push_back(i);
     S.hoff.push_back((uint32_t)S.hnodes.size());
   }
-  // flat-transform items: the ancestors of every leaf, root first
+  // flat-transform items: the ancestors of every leaf that take part in
+  // admissible blocks, root first (the coefficients of the other clusters
+  // are never read: their blocks W_a|_l are neither stored nor streamed)
   S.items.clear();
   S.ibeg.assign(1, 0);
   S.rowleaf.assign(ndof, NONE);
@@ -1662,6 +1776,7 @@ int build_side(h2hip_h2matrix::Side &S, uint32_t nnodes, const uint32_t *tree,
     const NodeD &L = S.nodes[S.leaves[l]];
     for (uint32_t k = 0; k < S.plen[l]; k++) {
       const uint32_t a = S.paths[l * MAXPATH + k];
+      if (live && !(*live)[a]) continue;
       S.items.push_back(FlatItem{S.leaves[l], a, w});
       w += (uint64_t)L.size * S.nodes[a].rank;
     }
@@ -1720,8 +1835,12 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   // fixed-order sums (not for the self-adjoint product)
   const bool ord = h2hip::reproducible() && !symm && H->d_P.p &&
                    H->ord_nv >= vs.nv;
-  // flat, not symmetric: the near blocks run in the upward launch
-  const bool fuse = flat && !symm && H->nnear > 0 && !in.items.empty();
+  // adjoint blocks of at most 64 x 64 go through LDS (couple_trans_wg)
+  const bool twg = trans && !symm && H->maxrank <= 64 && H->maxleaf <= 64;
+  // flat, not symmetric: the near blocks run in the upward launch (not for
+  // the workgroup adjoint blocks)
+  const bool fuse = flat && !symm && !twg && H->nnear > 0 &&
+                    !in.items.empty();
   const int cm = C ? 2 : 1;
   const bool mf = H->cpl_recompute && !symm;
   auto coupling_args = [&]() {
@@ -1776,6 +1895,8 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     cn.xt = xt;
     cn.yt = yt;
     cn.nfar = 0;
+    // no items: no cluster of the input side is in an admissible block,
+    // and no coefficient is needed
     if (ord) {
       if (fuse && trans)
         hipLaunchKernelGGL((fwd_flat_near<C, K, NV, true, true>), gf,
@@ -1783,7 +1904,7 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
       else if (fuse)
         hipLaunchKernelGGL((fwd_flat_near<C, K, NV, true, false>), gf,
                            dim3(NT), 0, st, fa, cn, fb);
-      else
+      else if (fb)
         hipLaunchKernelGGL((fwd_flat<C, K, NV, true>), gf, dim3(NT), 0, st,
                            fa);
       hipLaunchKernelGGL((ord_fwd_reduce<C, NV>), dim3(in.nnodes), dim3(NT),
@@ -1795,7 +1916,7 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     } else if (fuse) {
       hipLaunchKernelGGL((fwd_flat_near<C, K, NV, false, false>), gf,
                          dim3(NT), 0, st, fa, cn, fb);
-    } else {
+    } else if (fb) {
       hipLaunchKernelGGL((fwd_flat<C, K, NV>), gf, dim3(NT), 0, st, fa);
     }
   }
@@ -1828,6 +1949,23 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
 
   CNArgs c = coupling_args();
   if (fuse) c.nnear = 0;  // done in the upward launch
+  if (twg) {
+    // stored couplings and near blocks: one workgroup each; computed
+    // couplings: the wave path for the far blocks, workgroups for the near
+    CNArgs cw = c;
+    if (mf) cw.nfar = 0;
+    const uint32_t nwg = cw.nfar + cw.nnear;
+    if (nwg) {
+      if (ord)
+        hipLaunchKernelGGL((couple_trans_wg<C, NV, true>), dim3(nwg),
+                           dim3(NT), 0, st, cw);
+      else
+        hipLaunchKernelGGL((couple_trans_wg<C, NV, false>), dim3(nwg),
+                           dim3(NT), 0, st, cw);
+    }
+    c.nnear = 0;
+    if (!mf) c.nfar = 0;
+  }
   const uint32_t nw = c.nfar + c.nnear;
   const dim3 gc((nw + WPB - 1) / WPB);
   if (nw && symm) {
@@ -1894,7 +2032,7 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
       hipLaunchKernelGGL((back_flat_ord<C, K, NV>),
                          dim3((nl + WPB - 1) / WPB), dim3(NT), 0, st, fa,
                          (const uint32_t *)out.d_ibeg.p, nl);
-    } else {
+    } else if (fa.nitems) {
       hipLaunchKernelGGL((back_flat<C, K, NV>),
                          dim3((fa.nitems + WPB - 1) / WPB), dim3(NT), 0, st,
                          fa);
@@ -2200,9 +2338,22 @@ int h2hip_h2matrix_create(h2hip_h2matrix **out, int is_complex,
   }
   h2hip_h2matrix *H = new h2hip_h2matrix;
   H->cplx = is_complex ? 1 : 0;
-  int rc = build_side(H->side[0], rnodes, rtree, rrank, nrdof, ridx);
+  // clusters in admissible blocks, per side; with equal trees the union of
+  // both sides (the self-adjoint product reads both coefficients of a
+  // block from one side)
+  std::vector<char> rlive(rnodes, 0), clive(cnodes, 0);
+  for (uint32_t b = 0; b < nfar; b++) {
+    if (far[2 * b] < rnodes) rlive[far[2 * b]] = 1;
+    if (far[2 * b + 1] < cnodes) clive[far[2 * b + 1]] = 1;
+  }
+  if (rnodes == cnodes && nrdof == ncdof &&
+      std::equal(rtree, rtree + 3 * (size_t)rnodes, ctree) &&
+      std::equal(ridx, ridx + nrdof, cidx))
+    for (uint32_t i = 0; i < rnodes; i++)
+      rlive[i] = clive[i] = (char)(rlive[i] | clive[i]);
+  int rc = build_side(H->side[0], rnodes, rtree, rrank, nrdof, ridx, &rlive);
   if (rc == H2HIP_OK)
-    rc = build_side(H->side[1], cnodes, ctree, crank, ncdof, cidx);
+    rc = build_side(H->side[1], cnodes, ctree, crank, ncdof, cidx, &clive);
   if (rc != H2HIP_OK) {
     delete H;
     return rc;
