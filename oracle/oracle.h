@@ -87,11 +87,12 @@ int orc_sq_export(const orc_sq *sq, int cls, double *x, double *y, double *w,
 
 typedef struct orc_bem orc_bem;
 
-/* kernel: 0 Laplace, 1 Helmholtz.  op: 0 SLP, 1 DLP.  basis: 0 constant,
- * 1 linear.  alpha, k: complex numbers as (re, im).  Restates the
- * constructors of Library/laplacebem3d.c:843-1177 and
- * Library/helmholtzbem3d.c:1008-1345 (kernel constant, alpha, weighting,
- * mass matrices) and Library/surface3d.c:51-108 (normals and g). */
+/* kernel: 0 Laplace, 1 Helmholtz.  op: 0 SLP, 1 DLP, 2 ADLP (adjoint
+ * double layer).  basis: 0 constant, 1 linear.  alpha, k: complex numbers
+ * as (re, im).  Restates the constructors of Library/laplacebem3d.c:
+ * 843-1345 and Library/helmholtzbem3d.c:1008-1500 (kernel constant, alpha,
+ * weighting, mass matrices) and Library/surface3d.c:51-108 (normals and
+ * g). */
 orc_bem *orc_bem_new(int kernel, int op, int basis_r, int basis_c,
                      unsigned nv, const double *x, unsigned nt,
                      const unsigned *t, unsigned q, unsigned q2,

@@ -137,18 +137,27 @@ orc_bem_gram(const orc_bem *b)
 
 /* ---- kernel functions -------------------------------------------------- */
 
-/* Laplace SLP 1/|x-y|, DLP <x-y, n_y>/|x-y|^3 (Library/laplacebem3d.c:35-108)
- * and Helmholtz SLP e^{ik|x-y|}/|x-y|, DLP e^{ikr}(1-ikr)<x-y,n_y>/r^3
- * (Library/helmholtzbem3d.c:26-170).  The 1/(4 pi) is applied by the
- * integrators. */
+/* Laplace SLP 1/|x-y|, DLP <x-y, n_y>/|x-y|^3, ADLP <y-x, n_x>/|x-y|^3
+ * (Library/laplacebem3d.c:35-160) and Helmholtz SLP e^{ik|x-y|}/|x-y|, DLP
+ * e^{ikr}(1-ikr)<x-y,n_y>/r^3, ADLP e^{ikr}(1-ikr)<y-x,n_x>/r^3
+ * (Library/helmholtzbem3d.c:26-240).  nx, ny: the normals the integrators
+ * pass (those of the row and the column triangle).  The 1/(4 pi) is
+ * applied by the integrators. */
 static double complex
 kernel_eval(const orc_bem *b, const double *x, const double *y,
-            const double *ny)
+            const double *nx, const double *ny)
 {
   double d[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
   double norm2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   double rnorm = 1.0 / sqrt(norm2);
+  double nrm[3];
 
+  if (b->op == 2) {
+    /* the adjoint double layer: <y - x, n_x> = <x - y, -n_x> */
+    for (int j = 0; j < 3; j++)
+      nrm[j] = -nx[j];
+    ny = nrm;
+  }
   if (b->kernel == 0) {
     if (b->op == 0)
       return rnorm;
@@ -199,7 +208,7 @@ pair_values_swap(const orc_bem *b, int far, unsigned tt, unsigned ss,
     *Ct = b->x + 3 * xv[2];
   const double *As = b->x + 3 * yv[0], *Bs = b->x + 3 * yv[1],
     *Cs = b->x + 3 * yv[2];
-  const double *ny = b->n + 3 * ss;
+  const double *ny = b->n + 3 * ss, *nx = b->n + 3 * tt;
   unsigned n = r->n;
   for (unsigned q = 0; q < n; q++) {
     double tx = r->x[q], sx = r->x[q + n], ty = r->y[q], sy = r->y[q + n];
@@ -210,7 +219,7 @@ pair_values_swap(const orc_bem *b, int far, unsigned tt, unsigned ss,
       xx[j] = At[j] * Ax + Bt[j] * Bx + Ct[j] * Cx;
       yy[j] = As[j] * Ay + Bs[j] * By + Cs[j] * Cy;
     }
-    quad[q] = kernel_eval(b, xx, yy, ny);
+    quad[q] = kernel_eval(b, xx, yy, nx, ny);
   }
   return r;
 }

@@ -233,8 +233,9 @@ typedef struct {
   int       kernel, op, hip;
 } refbem;
 
-/* kernel: 0 Laplace, 1 Helmholtz.  op: 0 SLP, 1 DLP.  basis: 0 constant,
- * 1 linear.  alpha and k are complex numbers as (re, im) pairs. */
+/* kernel: 0 Laplace, 1 Helmholtz.  op: 0 SLP, 1 DLP, 2 ADLP.  basis: 0
+ * constant, 1 linear.  alpha and k are complex numbers as (re, im)
+ * pairs. */
 void     *
 ref_bem_new(int kernel, int op, int basis_r, int basis_c, unsigned nv,
 	    const double *x, unsigned nt, const unsigned *t, unsigned q,
@@ -261,16 +262,20 @@ ref_bem_new(int kernel, int op, int basis_r, int basis_c, unsigned nv,
   if (kernel == 0) {
     if (op == 0)
       rb->bem = new_slp_laplace_bem3d(rb->gr, q, q2, br, bc);
-    else
+    else if (op == 1)
       rb->bem = new_dlp_laplace_bem3d(rb->gr, q, q2, br, bc, a);
+    else
+      rb->bem = new_adlp_laplace_bem3d(rb->gr, q, q2, br, bc, a);
   }
   else {
 #ifdef USE_COMPLEX
     field     kk = k[0] + k[1] * I;
     if (op == 0)
       rb->bem = new_slp_helmholtz_bem3d(kk, rb->gr, q, q2, br, bc);
-    else
+    else if (op == 1)
       rb->bem = new_dlp_helmholtz_bem3d(kk, rb->gr, q, q2, br, bc, a);
+    else
+      rb->bem = new_adlp_helmholtz_bem3d(kk, rb->gr, q, q2, br, bc, a);
 #else
     (void) k;
     del_surface3d(rb->gr);

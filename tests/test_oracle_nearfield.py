@@ -35,6 +35,15 @@ CASES = {
     "helm_dlp_ll_calpha": (1, 1, 1, 1, 0.5 + 0.25j, 3.0),
     "helm_dlp_cl_calpha": (1, 1, 0, 1, -0.5 + 0.5j, 2.0),
     "helm_dlp_lc_calpha": (1, 1, 1, 0, 0.25j, 2.0),
+    # the adjoint double layer (layer 2); no dense fixture holds it, the
+    # live reference and the transpose identity below pin it
+    "lap_adlp_cc": (0, 2, 0, 0, 0.5, 0.0),
+    "lap_adlp_ll": (0, 2, 1, 1, -0.5, 0.0),
+    "lap_adlp_cl": (0, 2, 0, 1, 0.5, 0.0),
+    "lap_adlp_lc": (0, 2, 1, 0, 0.5, 0.0),
+    "helm_adlp_cc": (1, 2, 0, 0, 0.5, 2.0),
+    "helm_adlp_ll": (1, 2, 1, 1, -0.5, 2.0),
+    "helm_adlp_lc_calpha": (1, 2, 1, 0, 0.5 + 0.25j, 3.0),
 }
 
 
@@ -45,7 +54,8 @@ def make(cls, mesh, name):
 
 
 @pytest.mark.parametrize("name", [n for n in sorted(CASES)
-                                  if not n.endswith("_calpha")])
+                                  if not n.endswith("_calpha")
+                                  and "adlp" not in n])
 def test_dense_against_reference_fixture(name, sphere4):
     g = golden("dense_sphere_s4.npz")
     o = make(OracleBem, sphere4, name)
@@ -107,6 +117,34 @@ def test_single_layer_of_unit_density(sphere8):
     assert np.abs(rel).max() < 3e-2
     # a dropped or misplaced term would break the symmetry of V
     assert np.abs(V - V.T).max() < 1e-3 * np.abs(V).max()
+
+
+@pytest.mark.parametrize("kern,k", [(0, 0.0), (1, 2.0)])
+@pytest.mark.parametrize("br,bc", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_adjoint_double_layer_is_the_transposed_double_layer(kern, k, br, bc,
+                                                             sphere8):
+    # <y - x, n_x> / |x - y|^3 with x on the row triangle is the double
+    # layer kernel with the roles of the triangles swapped, so the Galerkin
+    # matrices satisfy K'_{ij} = K_{ji} (rows and columns of the bases
+    # swapped, same alpha: the mass matrices transpose too).  The regular
+    # Gauss rule is the same on both triangles, so pairs that share no
+    # vertex agree to rounding; the Sauter-Schwab rules are not symmetric
+    # in the two triangles, so touching pairs agree to quadrature accuracy.
+    # Taking n_y for n_x, or the wrong sign, breaks the first bound.
+    x, t = sphere8["x"], sphere8["t"]
+    A = OracleBem(kern, 2, br, bc, x, t, q=2, q2=4, alpha=0.5,
+                  k=k).nearfield()
+    K = OracleBem(kern, 1, bc, br, x, t, q=2, q2=4, alpha=0.5,
+                  k=k).nearfield()
+    D = np.abs(A - K.T)
+    scale = np.abs(A).max()
+    assert D.max() < 2e-5 * scale
+    if br == 0 and bc == 0:
+        tt = t.astype(np.int64)
+        touch = (tt[:, None, :, None] == tt[None, :, None, :]).any(axis=(2, 3))
+        assert D[~touch].max() < 1e-14 * scale
+        # the swap is not an identity: the two layers differ on this mesh
+        assert np.abs(A - K)[~touch].max() > 1e-6 * scale
 
 
 def test_double_layer_of_unit_density(sphere8):
