@@ -135,20 +135,20 @@ hip_nearfield_far_deferred(const uint * ridx, const uint * cidx,
   hip_defer(ridx, cidx, bem, ntrans, N, 1);
 }
 
-static void setup_hip(pbem3d bem, int helmholtz, int dlp);
+static void setup_hip(pbem3d bem, int helmholtz, int layer);
 
 void
-setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp)
+setup_hip_bem3d(pbem3d bem, int helmholtz, int layer)
 {
-  setup_hip(bem, helmholtz, dlp);
+  setup_hip(bem, helmholtz, layer);
   bem->nearfield = hip_nearfield;
   bem->nearfield_far = hip_nearfield_far;
 }
 
 void
-setup_hip_deferred_bem3d(pbem3d bem, int helmholtz, int dlp)
+setup_hip_deferred_bem3d(pbem3d bem, int helmholtz, int layer)
 {
-  setup_hip(bem, helmholtz, dlp);
+  setup_hip(bem, helmholtz, layer);
   bem->nearfield = hip_nearfield_deferred;
   bem->nearfield_far = hip_nearfield_far_deferred;
 }
@@ -161,7 +161,7 @@ flush_hip_bem3d(pcbem3d bem)
 }
 
 static void
-setup_hip(pbem3d bem, int helmholtz, int dlp)
+setup_hip(pbem3d bem, int helmholtz, int layer)
 {
   pcsurface3d gr = bem->gr;
   hipbem   *e;
@@ -183,7 +183,7 @@ setup_hip(pbem3d bem, int helmholtz, int dlp)
   to_pair(bem->k, k);
   if (h2hip_op_create(&e->op, e->s,
                       helmholtz ? H2HIP_HELMHOLTZ : H2HIP_LAPLACE,
-                      dlp ? H2HIP_DLP : H2HIP_SLP, rb, cb, bem->sq->q,
+                      layer, rb, cb, bem->sq->q,
                       bem->sq->q2, alpha, k) != H2HIP_OK)
     hip_fail("operator");
   pthread_mutex_lock(&registry_lock);
@@ -241,6 +241,17 @@ new_dlp_laplace_hip_bem3d(pcsurface3d gr, uint q_regular, uint q_singular,
   return bem;
 }
 
+pbem3d
+new_adlp_laplace_hip_bem3d(pcsurface3d gr, uint q_regular, uint q_singular,
+                           basisfunctionbem3d row_basis,
+                           basisfunctionbem3d col_basis, field alpha)
+{
+  pbem3d    bem = new_adlp_laplace_bem3d(gr, q_regular, q_singular,
+                                         row_basis, col_basis, alpha);
+  setup_hip_bem3d(bem, 0, H2HIP_ADLP);
+  return bem;
+}
+
 #ifdef USE_COMPLEX
 pbem3d
 new_slp_helmholtz_hip_bem3d(field k, pcsurface3d gr, uint q_regular,
@@ -261,6 +272,17 @@ new_dlp_helmholtz_hip_bem3d(field k, pcsurface3d gr, uint q_regular,
   pbem3d    bem = new_dlp_helmholtz_bem3d(k, gr, q_regular, q_singular,
                                           row_basis, col_basis, alpha);
   setup_hip_bem3d(bem, 1, 1);
+  return bem;
+}
+
+pbem3d
+new_adlp_helmholtz_hip_bem3d(field k, pcsurface3d gr, uint q_regular,
+                             uint q_singular, basisfunctionbem3d row_basis,
+                             basisfunctionbem3d col_basis, field alpha)
+{
+  pbem3d    bem = new_adlp_helmholtz_bem3d(k, gr, q_regular, q_singular,
+                                           row_basis, col_basis, alpha);
+  setup_hip_bem3d(bem, 1, H2HIP_ADLP);
   return bem;
 }
 #endif

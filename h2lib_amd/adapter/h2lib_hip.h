@@ -29,9 +29,10 @@
 #include "krylov.h"
 
 /* Installs the GPU near field into bem (built by one of the reference
- * constructors for the given kernel: helmholtz 0/1, dlp 0/1).  The surface
- * and the quadrature orders are taken from bem; alpha and k likewise. */
-void setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp);
+ * constructors for the given kernel: helmholtz 0/1, layer 0 single, 1
+ * double, 2 adjoint double layer).  The surface and the quadrature orders
+ * are taken from bem; alpha and k likewise. */
+void setup_hip_bem3d(pbem3d bem, int helmholtz, int layer);
 
 /* The same with deferred callbacks: bem->nearfield and bem->nearfield_far
  * only record their blocks (N is not written yet), and flush_hip_bem3d
@@ -40,7 +41,7 @@ void setup_hip_bem3d(pbem3d bem, int helmholtz, int dlp);
  *   assemble_bem3d_h2matrix(bem, h2); flush_hip_bem3d(bem);
  * the deferred completion of the reference's OpenCL callbacks
  * (Library/laplaceoclbem3d.c:248-279, Library/opencl.h:359-367). */
-void setup_hip_deferred_bem3d(pbem3d bem, int helmholtz, int dlp);
+void setup_hip_deferred_bem3d(pbem3d bem, int helmholtz, int layer);
 void flush_hip_bem3d(pcbem3d bem);
 
 /* Releases the device data of bem (call before del_bem3d). */
@@ -59,6 +60,10 @@ pbem3d new_dlp_laplace_hip_bem3d(pcsurface3d gr, uint q_regular,
                                  uint q_singular,
                                  basisfunctionbem3d row_basis,
                                  basisfunctionbem3d col_basis, field alpha);
+pbem3d new_adlp_laplace_hip_bem3d(pcsurface3d gr, uint q_regular,
+                                  uint q_singular,
+                                  basisfunctionbem3d row_basis,
+                                  basisfunctionbem3d col_basis, field alpha);
 #ifdef USE_COMPLEX
 pbem3d new_slp_helmholtz_hip_bem3d(field k, pcsurface3d gr, uint q_regular,
                                    uint q_singular,
@@ -69,6 +74,11 @@ pbem3d new_dlp_helmholtz_hip_bem3d(field k, pcsurface3d gr, uint q_regular,
                                    basisfunctionbem3d row_basis,
                                    basisfunctionbem3d col_basis,
                                    field alpha);
+pbem3d new_adlp_helmholtz_hip_bem3d(field k, pcsurface3d gr,
+                                    uint q_regular, uint q_singular,
+                                    basisfunctionbem3d row_basis,
+                                    basisfunctionbem3d col_basis,
+                                    field alpha);
 #endif
 
 /* Device copy of an assembled H2-matrix (cluster bases, couplings, near

@@ -23,6 +23,7 @@ LAPLACE = 0
 HELMHOLTZ = 1
 SLP = 0
 DLP = 1
+ADLP = 2
 
 
 def _ptr(a):
@@ -82,7 +83,7 @@ class Bem3d:
         self.col_basis = col_basis
         self.q_regular = q_regular
         self.q_singular = q_singular
-        self.alpha = alpha if layer == DLP else 0.0
+        self.alpha = alpha if layer != SLP else 0.0
         self.k = k
         self.is_complex = kernel == HELMHOLTZ
         self.dtype = np.complex128 if self.is_complex else np.float64
@@ -168,6 +169,22 @@ def new_slp_helmholtz_bem3d(k, gr, q_regular, q_singular, row_basis,
 def new_dlp_helmholtz_bem3d(k, gr, q_regular, q_singular, row_basis,
                             col_basis, alpha):
     return Bem3d(gr, HELMHOLTZ, DLP, q_regular, q_singular, row_basis,
+                 col_basis, alpha=alpha, k=k)
+
+
+def new_adlp_laplace_bem3d(gr, q_regular, q_singular, row_basis, col_basis,
+                           alpha):
+    """new_adlp_laplace_bem3d (Library/laplacebem3d.c:1180-1346): the
+    adjoint double layer <y - x, n_x>/(4 pi |x - y|^3) plus alpha times the
+    mass matrix."""
+    return Bem3d(gr, LAPLACE, ADLP, q_regular, q_singular, row_basis,
+                 col_basis, alpha=alpha)
+
+
+def new_adlp_helmholtz_bem3d(k, gr, q_regular, q_singular, row_basis,
+                             col_basis, alpha):
+    """new_adlp_helmholtz_bem3d (Library/helmholtzbem3d.c:1346-1514)."""
+    return Bem3d(gr, HELMHOLTZ, ADLP, q_regular, q_singular, row_basis,
                  col_basis, alpha=alpha, k=k)
 
 

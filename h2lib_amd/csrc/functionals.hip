@@ -119,32 +119,58 @@ __device__ bool grid_sum2(double v0, double v1, double *part, uint32_t *cnt,
 }
 
 // per triangle: g_t sum_k w_k f(x_k) phi_l(x_k), l = 0, 1, 2
+__device__ __forceinline__ void tri_integrals(const ProjArgs &a, uint32_t t,
+                                              double (&s)[3]) {
+  const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
+               *C = a.x + 3 * a.t[3 * t + 2];
+  const double *nn = a.n + 3 * t;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (uint32_t k = 0; k < a.nq; k++) {
+    const double u = a.rx0[k], v = a.rx1[k];
+    double d[3];
+    for (int c = 0; c < 3; c++)
+      d[c] = A[c] * (1.0 - u) + B[c] * (u - v) + C[c] * v - a.src[c];
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double rr = sqrt(r2);
+    const double f =
+        a.kind == 0 ? 1.0 / rr
+                    : -(d[0] * nn[0] + d[1] * nn[1] + d[2] * nn[2]) /
+                          (r2 * rr);
+    const double wf = a.rw[k] * f;
+    s0 += wf * (1.0 - u);
+    s1 += wf * (u - v);
+    s2 += wf * v;
+  }
+  s[0] = s0 * a.g[t];
+  s[1] = s1 * a.g[t];
+  s[2] = s2 * a.g[t];
+}
+
 __global__ __launch_bounds__(PT) void proj_tri(ProjArgs a) {
   for (uint32_t t = blockIdx.x * PT + threadIdx.x; t < a.nt;
        t += gridDim.x * PT) {
-    const double *A = a.x + 3 * a.t[3 * t], *B = a.x + 3 * a.t[3 * t + 1],
-                 *C = a.x + 3 * a.t[3 * t + 2];
-    const double *nn = a.n + 3 * t;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (uint32_t k = 0; k < a.nq; k++) {
-      const double u = a.rx0[k], v = a.rx1[k];
-      double d[3];
-      for (int c = 0; c < 3; c++)
-        d[c] = A[c] * (1.0 - u) + B[c] * (u - v) + C[c] * v - a.src[c];
-      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      const double rr = sqrt(r2);
-      const double f =
-          a.kind == 0 ? 1.0 / rr
-                      : -(d[0] * nn[0] + d[1] * nn[1] + d[2] * nn[2]) /
-                            (r2 * rr);
-      const double wf = a.rw[k] * f;
-      s0 += wf * (1.0 - u);
-      s1 += wf * (u - v);
-      s2 += wf * v;
+    double s[3];
+    tri_integrals(a, t, s);
+    a.tri[3 * t] = s[0];
+    a.tri[3 * t + 1] = s[1];
+    a.tri[3 * t + 2] = s[2];
+  }
+}
+
+// The load vector alone (out = or += b), one thread per vertex over its
+// triangles in the order of the vertex list, the triangle integrals
+// recomputed per vertex (no workspace, so calls need no lock)
+__global__ __launch_bounds__(PT) void load_vector(ProjArgs a, int acc) {
+  for (uint32_t i = blockIdx.x * PT + threadIdx.x; i < a.nv;
+       i += gridDim.x * PT) {
+    double sum = 0.0;
+    for (uint32_t e = a.vt_ptr[i]; e < a.vt_ptr[i + 1]; e++) {
+      const uint32_t code = a.vt[e];
+      double s[3];
+      tri_integrals(a, code >> 2, s);
+      sum += s[code & 3];
     }
-    a.tri[3 * t] = s0 * a.g[t];
-    a.tri[3 * t + 1] = s1 * a.g[t];
-    a.tri[3 * t + 2] = s2 * a.g[t];
+    a.out[i] = acc ? a.out[i] + sum : sum;
   }
 }
 
@@ -378,6 +404,43 @@ extern "C" int h2hip_projectL2_point_source(const h2hip_surface *s,
   // the chunk queued behind the last look returns at once; it finishes
   // before the workspace can be used again
   H2HIP_CHECK(hipEventSynchronize(s->pev[slot ^ 1]));
+  H2HIP_CHECK(hipGetLastError());
+  return H2HIP_OK;
+}
+
+extern "C" int h2hip_integrate_point_source(const h2hip_surface *s,
+                                            uint32_t q, int kind,
+                                            const double *x0, int accumulate,
+                                            double *d_out, void *stream) {
+  if (!s || !x0 || !d_out || q < 1 || q > 8 || (kind != 0 && kind != 1)) {
+    h2hip::set_error("h2hip_integrate_point_source: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  const int rc = h2hip::surface_vertex_lists(s);
+  if (rc != H2HIP_OK) return rc;
+  h2hip::TriRule tr = h2hip::make_triangle_rule(q);
+  ProjArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.x = s->d_x.p;
+  a.n = s->d_n.p;
+  a.g = s->d_g.p;
+  a.t = s->d_t.p;
+  a.nv = s->nv;
+  a.nt = s->nt;
+  a.vt_ptr = s->d_vt_ptr.p;
+  a.vt = s->d_vt.p;
+  a.nq = (uint32_t)tr.w.size();
+  for (uint32_t k = 0; k < a.nq; k++) {
+    a.rx0[k] = tr.x0[k];
+    a.rx1[k] = tr.x1[k];
+    a.rw[k] = tr.w[k];
+  }
+  for (int c = 0; c < 3; c++) a.src[c] = x0[c];
+  a.kind = kind;
+  a.out = d_out;
+  const dim3 gv((uint32_t)std::min<uint64_t>(PNB, (s->nv + PT - 1) / PT));
+  hipLaunchKernelGGL(load_vector, gv, dim3(PT), 0, (hipStream_t)stream, a,
+                     accumulate ? 1 : 0);
   H2HIP_CHECK(hipGetLastError());
   return H2HIP_OK;
 }

@@ -150,6 +150,9 @@ struct KArgs {
   double are, aim;       // identity coefficient alpha
   double mass[9];
   int far;
+  // adjoint double layer: the kernels of the double layer take -n_x (the
+  // row triangle's normal) in place of n_y, <y - x, n_x> = <x - y, -n_x>
+  int adj;
   double *out;
   // persistent schedule of the regular kernel: tile counter (nullptr: one
   // workgroup per tile)

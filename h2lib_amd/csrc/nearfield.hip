@@ -359,8 +359,9 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.krf = (float)a.kr;
   a.krev = (float)(a.kr / (2.0 * M_PI));
   a.kil2 = (float)(a.ki * M_LOG2E);
-  a.are = op->layer == H2HIP_DLP ? op->alpha[0] : 0.0;
-  a.aim = op->layer == H2HIP_DLP ? op->alpha[1] : 0.0;
+  a.are = op->layer != H2HIP_SLP ? op->alpha[0] : 0.0;
+  a.aim = op->layer != H2HIP_SLP ? op->alpha[1] : 0.0;
+  a.adj = op->layer == H2HIP_ADLP;
   for (int i = 0; i < 9; i++) a.mass[i] = op->mass[i];
   a.far = far;
   a.out = (double *)d_out;
@@ -656,7 +657,8 @@ int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
                     uint32_t q_regular, uint32_t q_singular,
                     const double *alpha, const double *k) {
   if (!out || !s || (kernel != H2HIP_LAPLACE && kernel != H2HIP_HELMHOLTZ) ||
-      (layer != H2HIP_SLP && layer != H2HIP_DLP) || row_basis < 0 ||
+      (layer != H2HIP_SLP && layer != H2HIP_DLP && layer != H2HIP_ADLP) ||
+      row_basis < 0 ||
       row_basis > 1 || col_basis < 0 || col_basis > 1 || q_regular < 1 ||
       q_regular > 4 || q_singular < 1 || q_singular > 16) {
     set_error("h2hip_op_create: invalid arguments");

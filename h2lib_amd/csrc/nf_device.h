@@ -124,6 +124,20 @@ __device__ __forceinline__ void kval32(float dx, float dy, float dz, float nx,
   }
 }
 
+// Normal vector of the double layer kernels for the pair (row triangle
+// rt, column triangle ct): n_y of the column triangle (double layer), or
+// -n_x of the row triangle (adjoint double layer, <y - x, n_x> =
+// <x - y, -n_x>; the reference's integrators pass the row triangle's
+// normal as n_x also where its lc far rule swaps the point roles,
+// Library/bem3d.c:2637-2649).
+__device__ __forceinline__ void layer_normal(const KArgs &a, uint32_t rt,
+                                             uint32_t ct, double (&nv)[3]) {
+  const uint32_t t = a.adj ? rt : ct;
+  const double sg = a.adj ? -1.0 : 1.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) nv[d] = sg * a.n[3 * t + d];
+}
+
 static __global__ void zero_counter(uint32_t *c) { *c = 0u; }
 
 __device__ __forceinline__ uint32_t loc_of(uint32_t packed, int l) {
@@ -226,9 +240,11 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
       }
       float nx = 0.0f, ny = 0.0f, nz = 0.0f;
       if constexpr (DLP) {
-        nx = (float)a.n[3 * sc.tri];
-        ny = (float)a.n[3 * sc.tri + 1];
-        nz = (float)a.n[3 * sc.tri + 2];
+        double n64[3];
+        layer_normal(a, sr.tri, sc.tri, n64);
+        nx = (float)n64[0];
+        ny = (float)n64[1];
+        nz = (float)n64[2];
       }
       f2 = a.kc * a.g[sr.tri] * a.g[sc.tri];
       // MP 2: the outer sums in FP32 with their compensations
@@ -302,9 +318,11 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
       }
       double nx = 0.0, ny = 0.0, nz = 0.0;
       if constexpr (DLP) {
-        nx = a.n[3 * sc.tri];
-        ny = a.n[3 * sc.tri + 1];
-        nz = a.n[3 * sc.tri + 2];
+        double n64[3];
+        layer_normal(a, sr.tri, sc.tri, n64);
+        nx = n64[0];
+        ny = n64[1];
+        nz = n64[2];
       }
       f2 = a.kc * a.g[sr.tri] * a.g[sc.tri];
 #pragma unroll
@@ -521,9 +539,11 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
   }
   double nx = 0.0, ny = 0.0, nz = 0.0;
   if constexpr (DLP) {
-    nx = a.n[3 * sc.tri];
-    ny = a.n[3 * sc.tri + 1];
-    nz = a.n[3 * sc.tri + 2];
+    double n64[3];
+    layer_normal(a, sr.tri, sc.tri, n64);
+    nx = n64[0];
+    ny = n64[1];
+    nz = n64[2];
   }
   double Sre[NR][NC], Sim[NR][NC];
 #pragma unroll

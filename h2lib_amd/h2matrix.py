@@ -9,8 +9,15 @@ import ctypes as C
 import numpy as np
 
 from ._lib import check, hip
-from .bem3d import DLP, LAPLACE, NearfieldPlan
+from .bem3d import ADLP, DLP, LAPLACE, SLP, NearfieldPlan
 from .geometry import bem
+
+
+def dn_side(layer):
+    """Side whose cluster basis takes the normal derivative of the
+    Lagrange polynomials: 1 the columns (double layer, n_y), 2 the rows
+    (adjoint double layer, n_x), 0 none (single layer)."""
+    return {SLP: 0, DLP: 1, ADLP: 2}[layer]
 
 
 class _H2Interp(C.Structure):
@@ -104,9 +111,11 @@ class Interpolation:
 
     def __init__(self, mesh, rt, ct, btree, row_basis, col_basis, dlp, m, q,
                  couplings=True, bases=True):
-        """bases=False: only the trees, blocks and interpolation points
-        (the bases are then built on the device, H2Matrix
-        device_bases)."""
+        """dlp: normal derivative of the Lagrange polynomials on the
+        column basis (True or 1: double layer), on the row basis (2:
+        adjoint double layer) or on neither (False or 0).  bases=False:
+        only the trees, blocks and interpolation points (the bases are then
+        built on the device, H2Matrix device_bases)."""
         _sigs()
         if not bases:
             self._ip = None
@@ -189,7 +198,7 @@ class H2Matrix:
         does not depend on the kernel), reused instead of planned again."""
         _sigs()
         if symmetric and (rt is not ct or op.row_basis != op.col_basis or
-                          op.layer == DLP or op.kernel != LAPLACE):
+                          op.layer != SLP or op.kernel != LAPLACE):
             raise ValueError("the symmetric storage needs a self-adjoint "
                              "operator on one tree with equal bases")
         self.symmetric = symmetric
@@ -198,7 +207,7 @@ class H2Matrix:
         self.m = m
         device_bases = device_bases and device_couplings
         I = Interpolation(mesh, rt, ct, btree, op.row_basis, op.col_basis,
-                          op.layer == DLP, m, op.q_regular,
+                          dn_side(op.layer), m, op.q_regular,
                           couplings=not device_couplings
                           and op.kernel == LAPLACE,
                           bases=not device_bases)
@@ -223,8 +232,10 @@ class H2Matrix:
             len(far), _ptr(far)), "h2hip_h2matrix_create")
         self._h = h
         if device_bases:
-            for sd, basis, dn, pts in ((0, op.row_basis, 0, I.rpts),
-                                       (1, op.col_basis, int(op.layer == DLP),
+            dside = dn_side(op.layer)
+            for sd, basis, dn, pts in ((0, op.row_basis, int(dside == 2),
+                                        I.rpts),
+                                       (1, op.col_basis, int(dside == 1),
                                         I.cpts)):
                 check(hip().h2hip_h2matrix_interp_bases(
                     h, sd, op.gr.handle, basis, dn, m, op.q_regular,

@@ -310,7 +310,7 @@ h2b_interp_points(const h2b_ctree *T, unsigned m, double *out)
 h2b_h2interp *
 h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
                    const h2b_ctree *ct, const h2b_blocks *bl, int row_basis,
-                   int col_basis, int col_dn, unsigned m, unsigned q)
+                   int col_basis, int dn_side, unsigned m, unsigned q)
 {
   if (m < 1 || m > 8 || q < 1 || q > 8)
     return NULL;
@@ -324,10 +324,10 @@ h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
   }
   trirule tr;
   make_trirule(q, &tr);
-  side(s, rt, row_basis, 0, m, cheb, &tr, &H->rpts, &H->rvoff, &H->rV,
-       &H->rvlen, &H->rE);
-  side(s, ct, col_basis, col_dn, m, cheb, &tr, &H->cpts, &H->cvoff, &H->cV,
-       &H->cvlen, &H->cE);
+  side(s, rt, row_basis, dn_side == 2, m, cheb, &tr, &H->rpts, &H->rvoff,
+       &H->rV, &H->rvlen, &H->rE);
+  side(s, ct, col_basis, dn_side == 1, m, cheb, &tr, &H->cpts, &H->cvoff,
+       &H->cV, &H->cvlen, &H->cE);
   H->nfar = bl->nfar;
   H->S = NULL;
   return H;

@@ -23,7 +23,11 @@ Tests/test_helmholtzbem3d.c):
   operator with alpha = 1/2 and b the integrals of the Dirichlet data
   (Tests/test_helmholtzbem3d.c:24-65, 353-450).  Unlike V x = (K - M/2) f
   it has no resonances (V is singular at the interior Dirichlet
-  eigenvalues, of which the unit ball has several below 6).
+  eigenvalues, of which the unit ball has several below 6);
+* Neumann: exterior Neumann problem of the Laplace equation, piecewise
+  linear basis, data of unit point sources inside the surface, in the
+  indirect form with the adjoint double layer (K' - M/2) sigma = b, b the
+  integrals of the Neumann data (GMRES; LaplaceNeumannExterior).
 """
 import threading
 import time
@@ -33,9 +37,11 @@ import numpy as np
 
 from . import geometry as G
 from .device import DeviceArray, Stream
-from .solve import GmresBatch, HelmholtzBrakhageWerner, LaplaceDirichlet
+from .solve import (GmresBatch, HelmholtzBrakhageWerner, LaplaceDirichlet,
+                    LaplaceNeumannExterior)
 
-LAPLACE, HELMHOLTZ = "laplace", "helmholtz"
+LAPLACE, HELMHOLTZ, NEUMANN = "laplace", "helmholtz", "neumann"
+KINDS = (LAPLACE, HELMHOLTZ, NEUMANN)
 
 
 class Request:
@@ -57,8 +63,8 @@ def footprint(mesh, kind, clf, m, slots, kmax, eta=2.0):
     """Device bytes of one operator pair (single and double layer
     H2-matrices) and its batched GMRES, from the trees and block lists
     (computed on the host before anything is built)."""
-    basis = 1 if kind == LAPLACE else 0
-    es = 8 if kind == LAPLACE else 16
+    basis = 0 if kind == HELMHOLTZ else 1
+    es = 16 if kind == HELMHOLTZ else 8
     tree = G.ClusterTree(mesh, basis, clf)
     bt = G.BlockTree(tree, tree, eta, 0)
     nb = np.asarray(bt.near_block_list(), dtype=np.int64).reshape(-1, 4)
@@ -116,7 +122,8 @@ class Server:
         self.slots, self.kmax, self.eps, self.maxiter = slots, kmax, eps, \
             maxiter
         self.clf = clf
-        self.m = {LAPLACE: m_laplace, HELMHOLTZ: m_helmholtz}
+        self.m = {LAPLACE: m_laplace, HELMHOLTZ: m_helmholtz,
+                  NEUMANN: m_laplace}
         self.cache = OrderedDict()  # key -> Entry, least recently used first
         self.used = 0
         self.builds = self.evictions = 0
@@ -141,6 +148,12 @@ class Server:
             op, method = prob.V, self.laplace_method
             if self.computed_couplings:
                 prob.V.set_coupling_mode(1)
+        elif kind == NEUMANN:
+            prob = LaplaceNeumannExterior(mesh, clf=self.clf, m=self.m[kind],
+                                          stream=stream)
+            op, method = prob.A, "gmres"
+            if self.computed_couplings:
+                prob.A.set_coupling_mode(1)
         else:
             prob = HelmholtzBrakhageWerner(mesh, k, clf=self.clf,
                                            eta_adm=2.0, m=self.m[kind],
@@ -272,20 +285,27 @@ class Server:
 K_GRID = tuple(1.0 + 0.5 * i for i in range(11))  # 1, 1.5, ..., 6
 
 
-def draw_requests(count, rate, mesh_ids, bounds, seed, ks=K_GRID):
+def draw_requests(count, rate, mesh_ids, bounds, seed, ks=K_GRID,
+                  kinds=(LAPLACE, HELMHOLTZ)):
     """Poisson arrivals at `rate` per second; each request picks a mesh,
-    Laplace or Helmholtz (even odds), a wave number uniformly from ks (by
-    default [1, 6] in steps of 1/2, so that requests can share operators),
-    and one to four unit point sources: outside the mesh's bounding sphere
-    for Laplace (interior problem), inside half its inscribed radius for
-    Helmholtz (exterior problem).  bounds: {mesh_id: (inner, outer)}."""
+    a problem from kinds (even odds; Laplace Dirichlet, Helmholtz
+    Dirichlet, Laplace Neumann), a wave number uniformly from ks for
+    Helmholtz (by default [1, 6] in steps of 1/2, so that requests can
+    share operators), and one to four unit point sources: outside the
+    mesh's bounding sphere for the interior Laplace Dirichlet problem,
+    inside half its inscribed radius for the exterior problems.  bounds:
+    {mesh_id: (inner, outer)}."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))
     t -= t[0]
     out = []
     for i in range(count):
         mid = mesh_ids[int(rng.integers(0, len(mesh_ids)))]
-        kind = LAPLACE if rng.random() < 0.5 else HELMHOLTZ
+        if tuple(kinds) == (LAPLACE, HELMHOLTZ):
+            # the draw of the two-problem stream as it always was
+            kind = LAPLACE if rng.random() < 0.5 else HELMHOLTZ
+        else:
+            kind = kinds[int(rng.integers(0, len(kinds)))]
         k = float(ks[int(rng.integers(0, len(ks)))]) if kind == HELMHOLTZ \
             else 0.0
         inner, outer = bounds[mid]

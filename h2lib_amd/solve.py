@@ -13,8 +13,8 @@ import ctypes as C
 import numpy as np
 
 from ._lib import check, hip
-from .bem3d import (BASIS_LINEAR_BEM3D, Surface3d, new_dlp_laplace_bem3d,
-                    new_slp_laplace_bem3d)
+from .bem3d import (BASIS_LINEAR_BEM3D, Surface3d, new_adlp_laplace_bem3d,
+                    new_dlp_laplace_bem3d, new_slp_laplace_bem3d)
 from .device import DeviceArray
 from .geometry import BlockTree, ClusterTree, bem
 from .h2matrix import H2Matrix
@@ -30,6 +30,10 @@ def _sigs():
     h.h2hip_projectL2_point_source.restype = C.c_int
     h.h2hip_projectL2_point_source.argtypes = [C.c_void_p, C.c_uint32,
                                                C.c_int, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
+    h.h2hip_integrate_point_source.restype = C.c_int
+    h.h2hip_integrate_point_source.argtypes = [C.c_void_p, C.c_uint32,
+                                               C.c_int, C.c_void_p, C.c_int,
                                                C.c_void_p, C.c_void_p]
     h.h2hip_solve_gmres.restype = C.c_int
     h.h2hip_addeval_combined.restype = C.c_int
@@ -512,6 +516,107 @@ class LaplaceDirichlet:
         e = bem().h2b_normL2diff_l(self.mesh._sp, self.q,
                                    x.ctypes.data_as(C.c_void_p), 1, p0)
         return e / bem().h2b_normL2(self.mesh._sp, self.q, 1, p0)
+
+
+def laplace_point_integrals(mesh, kind, x0, q=2):
+    """Integrals of the Laplace point-source data at x0 (kind 0 Dirichlet
+    1/|x - x0|, 1 Neumann) against the linear basis on the host
+    (integrate_bem3d_l_avector, Library/bem3d.c:8631-8702)."""
+    _sigs()
+    out = np.zeros(mesh.vertices)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    bem().h2b_integrate_l(mesh._sp, q, kind, x0.ctypes.data_as(C.c_void_p),
+                          out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def laplace_point_integrals_device(gr, q, kind, x0, d_out, accumulate=False,
+                                   stream=None):
+    """The same integrals on the device into d_out (added with accumulate),
+    ordered on the stream (h2hip_integrate_point_source)."""
+    _sigs()
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    check(hip().h2hip_integrate_point_source(
+        gr.handle, q, kind, x0.ctypes.data_as(C.c_void_p),
+        int(bool(accumulate)), C.c_void_p(d_out.ptr),
+        C.c_void_p(stream.handle if stream else 0)),
+        "h2hip_integrate_point_source")
+
+
+class LaplaceNeumannExterior:
+    """Exterior Neumann problem of the Laplace equation, piecewise linear
+    basis, indirect formulation with the adjoint double layer: the field is
+    the single layer potential u = V sigma of a density sigma, whose
+    exterior normal derivative is (K' - 1/2) sigma, so sigma solves
+
+        (K' - M/2) sigma = b,   b_i = int phi_i g
+
+    for the Neumann data g (Galerkin; K' the adjoint double layer operator,
+    new_adlp_laplace_bem3d with alpha = -1/2, Library/laplacebem3d.c:
+    1180-1346).  The operator is not symmetric: GMRES.  The data are those
+    of unit point sources x0 inside the surface, u(x) = sum 1/|x - x0|
+    (harmonic outside, decaying), g = du/dn.  The Dirichlet trace of the
+    solution is V sigma; dirichlet_error compares its integrals against the
+    basis with those of u."""
+
+    def __init__(self, mesh, clf=64, eta=2.0, m=4, q=2, q2=4, schedule=2,
+                 stream=None):
+        _sigs()
+        self.mesh = mesh
+        self.q = q
+        self.gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+        lin = BASIS_LINEAR_BEM3D
+        self.adlp = new_adlp_laplace_bem3d(self.gr, q, q2, lin, lin, -0.5)
+        self.slp = new_slp_laplace_bem3d(self.gr, q, q2, lin, lin)
+        self.tree = ClusterTree(mesh, lin, clf)
+        self.blocks = BlockTree(self.tree, self.tree, eta, 0)
+        self.A = H2Matrix(self.adlp, mesh, self.tree, self.tree, self.blocks,
+                          m, stream=stream)
+        self.V = H2Matrix(self.slp, mesh, self.tree, self.tree, self.blocks,
+                          m, stream=stream, nearplan=self.A.nearplan)
+        if stream is not None:
+            stream.sync()
+        for H in (self.A, self.V):
+            H.set_schedule(schedule)
+        self.n = mesh.vertices
+
+    def rhs_sources(self, x0s, stream=None):
+        """(b, b): b = the integrals of the Neumann data of unit point
+        sources at x0s against the basis, formed on the device (the pair
+        mirrors the other problems' (f, b))."""
+        d_b = DeviceArray(self.n)
+        for j, x0 in enumerate(x0s):
+            laplace_point_integrals_device(self.gr, self.q, 1, x0, d_b,
+                                           accumulate=j > 0, stream=stream)
+        return d_b, d_b
+
+    def rhs(self, x0, stream=None):
+        return self.rhs_sources([x0], stream)[1]
+
+    def solve(self, d_b, eps=1e-10, maxiter=1000, kmax=30, stream=None):
+        d_x = DeviceArray(self.n)
+        d_x.zero(stream)
+        it, res = solve_gmres(self.A, d_b, d_x, eps, maxiter, kmax, stream)
+        return d_x, it, res
+
+    def trace_integrals(self, d_sigma, stream=None):
+        """Integrals of the Dirichlet trace V sigma against the basis (the
+        Galerkin product V sigma), on the host."""
+        d_y = DeviceArray(self.n)
+        d_y.zero(stream)
+        self.V.addeval(1.0, d_sigma, d_y, stream)
+        if stream is not None:
+            stream.sync()
+        return d_y.download()
+
+    def dirichlet_error(self, d_sigma, x0s, stream=None):
+        """|V sigma - D| / |D| with D_i = int phi_i u, u the field of the
+        sources: the discretisation error of the Dirichlet trace, measured
+        on its Galerkin moments."""
+        got = self.trace_integrals(d_sigma, stream)
+        want = sum(laplace_point_integrals(self.mesh, 0, x0, self.q)
+                   for x0 in x0s)
+        return float(np.linalg.norm(got - want) / np.linalg.norm(want))
 
 
 class HelmholtzExterior:

@@ -112,13 +112,15 @@ typedef struct {
   double *S;               /* per far block: k x k coupling, column-major */
 } h2b_h2interp;
 
-/* Cluster bases of both trees; col_dn selects the normal derivative of the
- * Lagrange polynomials for the column basis (double layer potential,
- * assemble_bem3d_dn_lagrange_{c,l}_amatrix, Library/bem3d.c:8024-8336).
+/* Cluster bases of both trees; dn_side selects the normal derivative of
+ * the Lagrange polynomials (assemble_bem3d_dn_lagrange_{c,l}_amatrix,
+ * Library/bem3d.c:8024-8336) for the column basis (1: double layer
+ * potential) or the row basis (2: adjoint double layer, whose kernel
+ * differentiates in x, Library/laplacebem3d.c:1199-1222); 0: neither.
  * q: order of the single-triangle Gauss rule. */
 h2b_h2interp *h2b_build_h2interp(const h2b_surface *s, const h2b_ctree *rt,
                                  const h2b_ctree *ct, const h2b_blocks *bl,
-                                 int row_basis, int col_basis, int col_dn,
+                                 int row_basis, int col_basis, int dn_side,
                                  unsigned m, unsigned q);
 /* The tensor Chebyshev points of every cluster of T (3 m per node: x,
  * then y, then z; the points h2b_build_h2interp uses, bit for bit), for

@@ -38,6 +38,7 @@ extern "C" {
 /* layer potentials */
 #define H2HIP_SLP 0
 #define H2HIP_DLP 1
+#define H2HIP_ADLP 2 /* adjoint double layer */
 /* basis functions (basisfunctionbem3d, Library/bem3d.h) */
 #define H2HIP_BASIS_CONSTANT 0
 #define H2HIP_BASIS_LINEAR 1
@@ -82,14 +83,17 @@ int h2hip_surface_geometry(const h2hip_surface *s, double *n, double *g);
 
 /* ---- operators -------------------------------------------------------------
  * Near-field quadrature state of one boundary integral operator.
- * kernel: H2HIP_LAPLACE / H2HIP_HELMHOLTZ; layer: H2HIP_SLP / H2HIP_DLP;
- * bases: H2HIP_BASIS_*; q_regular / q_singular: Gauss orders of the regular
- * and the singular (Sauter-Schwab) rules; alpha: coefficient of the identity
- * term added to the double layer (ignored for SLP); k: wave number
+ * kernel: H2HIP_LAPLACE / H2HIP_HELMHOLTZ; layer: H2HIP_SLP / H2HIP_DLP /
+ * H2HIP_ADLP (kernel <y - x, n_x>/|x - y|^3, times e^{ikr} (1 - ikr) for
+ * Helmholtz, with n_x the normal of the row triangle); bases:
+ * H2HIP_BASIS_*; q_regular / q_singular: Gauss orders of the regular and
+ * the singular (Sauter-Schwab) rules; alpha: coefficient of the identity
+ * term added to the double layers (ignored for SLP); k: wave number
  * (Helmholtz only).  alpha and k are (re, im) pairs and may be NULL (0).
- * Replaces new_slp_laplace_bem3d / new_dlp_laplace_bem3d
- * (Library/laplacebem3d.c:843-1177) and new_slp_helmholtz_bem3d /
- * new_dlp_helmholtz_bem3d (Library/helmholtzbem3d.c:1008-1345), together
+ * Replaces new_slp_laplace_bem3d / new_dlp_laplace_bem3d /
+ * new_adlp_laplace_bem3d (Library/laplacebem3d.c:843-1345) and
+ * new_slp_helmholtz_bem3d / new_dlp_helmholtz_bem3d /
+ * new_adlp_helmholtz_bem3d (Library/helmholtzbem3d.c:1008-1514), together
  * with build_singquad2d (Library/singquad2d.c:320-403) and the basis
  * weighting of the rules (Library/singquad2d.c:456-530). */
 int h2hip_op_create(h2hip_op **out, const h2hip_surface *s, int kernel,
@@ -437,6 +441,16 @@ int h2hip_eval_brakhage_werner_l(const h2hip_surface *s, uint32_t q,
 int h2hip_projectL2_point_source(const h2hip_surface *s, uint32_t q, int kind,
                                  const double *x0, double *d_out,
                                  void *stream);
+/* The load vector alone: d_out[i] = int phi_i f (accumulate != 0: added
+ * to d_out) for the same data f and rule, one entry per vertex, ordered on
+ * the stream; needs no workspace, so calls on one surface may overlap.
+ * integrate_bem3d_l_avector (Library/bem3d.c:8631-8702) with
+ * eval_dirichlet_fundamental_laplacebem3d /
+ * eval_neumann_fundamental_laplacebem3d (Library/laplacebem3d.c:1390-1430):
+ * the right-hand side of the exterior Neumann problem. */
+int h2hip_integrate_point_source(const h2hip_surface *s, uint32_t q,
+                                 int kind, const double *x0, int accumulate,
+                                 double *d_out, void *stream);
 
 /* ---- batched GMRES (serving) --------------------------------------------------
  * Up to 8 independent GMRES(kmax) solves with one H2-matrix advance

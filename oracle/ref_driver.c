@@ -626,6 +626,24 @@ ref_dirichlet_laplace(void *slp, void *dlp, double eps, unsigned maxiter,
   return it;
 }
 
+/* out[i] = integral of the Dirichlet (kind 0) or Neumann (kind 1) data of
+ * the reference's Laplace point source at (1.2, 1.2, 1.2) against the
+ * linear basis of the operator h (integrate_bem3d_l_avector,
+ * Library/bem3d.c:8631-8702, with eval_dirichlet_fundamental_laplacebem3d
+ * or eval_neumann_fundamental_laplacebem3d); one value per vertex. */
+void
+ref_laplace_point_integrate_l(void *h, int kind, double *out)
+{
+  refbem   *rb = (refbem *) h;
+  pavector  f = new_avector(rb->gr->vertices);
+
+  integrate_bem3d_l_avector(rb->bem,
+			    kind ? eval_neumann_fundamental_laplacebem3d :
+			    eval_dirichlet_fundamental_laplacebem3d, f, NULL);
+  memcpy(out, f->v, sizeof(field) * f->dim);
+  del_avector(f);
+}
+
 #ifdef USE_COMPLEX
 /* Boundary data of a Helmholtz point source at x0 with wave number k
  * (rhs_dirichlet_point_helmholtzbem3d for kind 0,

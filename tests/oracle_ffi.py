@@ -339,6 +339,9 @@ class Reference:
                                         C.c_void_p, C.c_uint, C.c_uint,
                                         C.c_void_p, C.c_void_p]
             lib.ref_bem_del.argtypes = [C.c_void_p]
+            if not complex_:
+                lib.ref_laplace_point_integrate_l.argtypes = [
+                    C.c_void_p, C.c_int, C.c_void_p]
             if complex_:
                 lib.ref_helmholtz_point_integrate.argtypes = [
                     C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
@@ -450,6 +453,14 @@ class ReferenceBem(_BemBase):
         if getattr(self, "h", None):
             self._lib.ref_bem_del(self.h)
             self.h = None
+
+    def laplace_point_integrate_l(self, kind):
+        """Integrals of the reference's Laplace point-source data at
+        (1.2, 1.2, 1.2) (0 Dirichlet, 1 Neumann) against the linear basis
+        (real build only)."""
+        out = np.zeros(len(self.x), dtype=np.float64)
+        self._lib.ref_laplace_point_integrate_l(self.h, kind, _ptr(out))
+        return out
 
     def helmholtz_point_project(self, kind, k, x0):
         """Point-source data (0 Dirichlet, 1 Neumann) projected onto the

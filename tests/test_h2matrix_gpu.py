@@ -25,8 +25,10 @@ def build(mesh, layer, basis, clf, eta, m, alpha=0.5):
     gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
     if layer == 0:
         op = h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis)
-    else:
+    elif layer == 1:
         op = h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, alpha)
+    else:
+        op = h.new_adlp_laplace_bem3d(gr, 2, 4, basis, basis, alpha)
     rt = G.ClusterTree(mesh, basis, clf)
     bt = G.BlockTree(rt, rt, eta, 0)
     return op, rt, bt, H2Matrix(op, mesh, rt, rt, bt, m)
@@ -53,8 +55,11 @@ def test_matches_reference_fixture():
 
 
 @pytest.mark.skipif(not have_reference(False), reason="reference not built")
-@pytest.mark.parametrize("layer,basis", [(0, 1), (1, 1), (0, 0), (1, 0)])
+@pytest.mark.parametrize("layer,basis", [(0, 1), (1, 1), (0, 0), (1, 0),
+                                         (2, 1), (2, 0)])
 def test_matches_live_reference(layer, basis):
+    # layer 2, the adjoint double layer, takes the normal derivative of the
+    # Lagrange polynomials on the row basis
     from h2lib_amd import geometry as G
     mesh = G.sphere(16)
     op, rt, bt, H = build(mesh, layer, basis, 32, 2.0, 3)
@@ -105,7 +110,8 @@ def test_all_near_equals_dense():
 
 @pytest.mark.skipif(not have_reference(True), reason="reference not built")
 @pytest.mark.parametrize("layer,basis,k", [(0, 0, 2.0), (1, 1, 4.0),
-                                           (0, 1, 2.0 + 0.5j)])
+                                           (0, 1, 2.0 + 0.5j), (2, 1, 3.0),
+                                           (2, 0, 2.0 + 0.5j)])
 def test_helmholtz_matches_live_reference(layer, basis, k):
     # complex H2-matrix of a Helmholtz operator: real interpolation bases,
     # complex couplings (host) and near field (GPU quadrature)
@@ -117,8 +123,10 @@ def test_helmholtz_matches_live_reference(layer, basis, k):
     alpha = 0.5 + 0.25j if layer else 0.0
     if layer == 0:
         op = h.new_slp_helmholtz_bem3d(k, gr, 2, 4, basis, basis)
-    else:
+    elif layer == 1:
         op = h.new_dlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, alpha)
+    else:
+        op = h.new_adlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, alpha)
     rt = G.ClusterTree(mesh, basis, 24)
     bt = G.BlockTree(rt, rt, 2.0, 0)
     H = H2Matrix(op, mesh, rt, rt, bt, 3)
@@ -139,8 +147,10 @@ def test_helmholtz_matches_live_reference(layer, basis, k):
 
 @pytest.mark.parametrize("kern,layer,basis,k", [(0, 0, 0, 0.0),
                                                  (0, 1, 1, 0.0),
+                                                 (0, 2, 1, 0.0),
                                                  (1, 0, 0, 3.0 + 0.5j),
-                                                 (1, 1, 1, 2.0)])
+                                                 (1, 1, 1, 2.0),
+                                                 (1, 2, 0, 2.0)])
 def test_device_couplings_equal_host_couplings(kern, layer, basis, k):
     # couplings computed on the device (h2hip_h2matrix_interp_couplings)
     # against the host restatement (h2b_h2interp_coupling_*), which is
@@ -151,12 +161,14 @@ def test_device_couplings_equal_host_couplings(kern, layer, basis, k):
     mesh = G.cube(6)
     gr = h.Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
     if kern == 0:
-        op = (h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5) if layer
-              else h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis))
+        op = (h.new_slp_laplace_bem3d(gr, 2, 4, basis, basis),
+              h.new_dlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5),
+              h.new_adlp_laplace_bem3d(gr, 2, 4, basis, basis, 0.5))[layer]
     else:
-        op = (h.new_dlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, 0.5)
-              if layer else h.new_slp_helmholtz_bem3d(k, gr, 2, 4, basis,
-                                                      basis))
+        op = (h.new_slp_helmholtz_bem3d(k, gr, 2, 4, basis, basis),
+              h.new_dlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis, 0.5),
+              h.new_adlp_helmholtz_bem3d(k, gr, 2, 4, basis, basis,
+                                         0.5))[layer]
     rt = G.ClusterTree(mesh, basis, 16)
     bt = G.BlockTree(rt, rt, 2.0, 0)
     Hd = H2Matrix(op, mesh, rt, rt, bt, 3, device_couplings=True)

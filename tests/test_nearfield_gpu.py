@@ -36,6 +36,15 @@ CASES = {
     # complex alpha with the linear basis (Re(alpha) only, as the reference)
     "helm_dlp_ll_calpha": (1, 1, 1, 1, 0.5 + 0.25j, 3.0),
     "helm_dlp_cl_calpha": (1, 1, 0, 1, -0.5 + 0.5j, 2.0),
+    # the adjoint double layer (layer 2: n_x of the row triangle)
+    "lap_adlp_cc": (0, 2, 0, 0, 0.5, 0.0),
+    "lap_adlp_ll": (0, 2, 1, 1, -0.5, 0.0),
+    "lap_adlp_cl": (0, 2, 0, 1, 0.5, 0.0),
+    "lap_adlp_lc": (0, 2, 1, 0, -0.5, 0.0),
+    "helm_adlp_cc": (1, 2, 0, 0, 0.5 + 0.25j, 2.0),
+    "helm_adlp_ll": (1, 2, 1, 1, -0.5, 2.0 + 0.5j),
+    "helm_adlp_cl": (1, 2, 0, 1, 0.5, 3.0),
+    "helm_adlp_lc_calpha": (1, 2, 1, 0, 0.5 + 0.25j, 3.0),
 }
 
 
@@ -101,7 +110,7 @@ def test_window_inside_larger_matrix(sphere8):
 
 @pytest.mark.parametrize("q,q2", [(1, 3), (3, 5), (4, 6)])
 def test_other_orders(q, q2, sphere4):
-    for name in ("lap_slp_cc", "helm_dlp_ll", "lap_dlp_cl"):
+    for name in ("lap_slp_cc", "helm_dlp_ll", "lap_dlp_cl", "lap_adlp_lc"):
         g = gpu_op(sphere4, name, q, q2)
         o = oracle_op(sphere4, name, q, q2)
         print("orders", name, q, q2, flush=True)
@@ -130,7 +139,7 @@ def test_sampled_entries_against_reference_fixture():
 def test_batched_plan_matches_single_calls(sphere8):
     import h2lib_amd as h
     rng = np.random.default_rng(12)
-    for name in ("lap_slp_ll", "helm_dlp_cc", "lap_dlp_lc"):
+    for name in ("lap_slp_ll", "helm_dlp_cc", "lap_dlp_lc", "lap_adlp_ll"):
         kern, op, br, bc, alpha, k = CASES[name]
         gr = h.Surface3d(sphere8["x"], sphere8["t"])
         b = h.Bem3d(gr, kern, op, 2, 4, br, bc, alpha=alpha, k=k)

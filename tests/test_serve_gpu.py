@@ -4,8 +4,9 @@
   other meshes and problems run, batched with other requests of its
   operators) as solved alone with the one-system solver of its method on
   the same operators (CG with V for Laplace, GMRES with the Brakhage-Werner
-  operator KM - i k V for Helmholtz): in reproducible mode bit for bit, with
-  the same number of steps.
+  operator KM - i k V for Helmholtz, GMRES with K' - M/2 for the Laplace
+  Neumann problem): in reproducible mode bit for bit, with the same number
+  of steps.
 * With a device-memory budget that holds about one operator pair, idle
   operators are evicted and rebuilt on demand, and every request still
   completes with its residual bound met.
@@ -74,6 +75,23 @@ def test_each_result_same_alone_and_in_mix():
     # requests really overlapped: some were admitted before others finished
     assert any(a.admitted < b.done and a.rid != b.rid and a.key != b.key
                for a in reqs for b in reqs)
+    assert srv.evictions == 0
+    check_against_alone(srv, reqs, True)
+
+
+def test_neumann_requests_same_alone_and_in_mix():
+    # the three problems in one stream: Laplace Dirichlet (CG), Helmholtz
+    # Brakhage-Werner (GMRES), Laplace exterior Neumann with the adjoint
+    # double layer (GMRES)
+    from h2lib_amd.serve import KINDS, Server, draw_requests
+    meshes, bounds = catalog()
+    reqs = draw_requests(15, 2000.0, sorted(meshes), bounds, seed=6,
+                         kinds=KINDS)
+    srv = Server(meshes, 64e9, eps=1e-10, maxiter=3000, clf=16,
+                 m_laplace=3, m_helmholtz=3)
+    srv.run(reqs, timeout=300)
+    assert {r.kind for r in reqs} == set(KINDS)
+    assert all(r.done is not None and r.x is not None for r in reqs)
     assert srv.evictions == 0
     check_against_alone(srv, reqs, True)
 
