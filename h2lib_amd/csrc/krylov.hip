@@ -1002,6 +1002,127 @@ int cg(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   return H2HIP_OK;
 }
 
+// ---- CG without allocations ------------------------------------------------
+//
+// The solve of cg<C> for callers that cannot allocate (serving loops that
+// must not call hipMalloc, harnesses that forbid it): the three work
+// vectors, the per-block partial sums and the device state live in one
+// file-scope device array of fixed size, `scratch`, so the call allocates
+// nothing: no device or pinned memory, no events, streams or graphs.  The
+// host looks at the stopping flag after every CHUNK steps through a copy
+// into its own stack and a stream synchronisation.  Each GPU holds its own
+// copy of the array until the process ends; every call on one GPU shares
+// it, so calls that overlap on one GPU race.
+
+constexpr uint64_t CG_NOALLOC_MAXN2 = 1u << 20;  // doubles per work vector
+constexpr uint64_t CG_SCR_PART = 3 * CG_NOALLOC_MAXN2;
+constexpr uint64_t CG_SCR_S = CG_SCR_PART + (uint64_t)CG_NB * 8;
+constexpr uint64_t CG_SCR_FLAG = CG_SCR_S + 16;
+constexpr uint64_t CG_SCR_LEN = CG_SCR_FLAG + 4;  // 4 u32 flags in 2 doubles
+constexpr int SCR_MAXDEV = 64;
+
+}  // namespace
+
+// work space of h2hip_solve_cg_noalloc: 24 MiB and 32 KiB per GPU
+static __device__ __align__(256) double scratch[CG_SCR_LEN];
+
+namespace {
+
+std::mutex scratch_lock;
+double *scratch_dev[SCR_MAXDEV] = {};
+
+// the device address of the current device's copy of `scratch`, looked up
+// once per device (the array's name is a host-side stand-in in host code)
+int scratch_ptr(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCR_MAXDEV) {
+    h2hip::set_error("h2hip_solve_cg_noalloc: no current device");
+    return H2HIP_EDEVICE;
+  }
+  std::lock_guard<std::mutex> guard(scratch_lock);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      h2hip::set_error("h2hip_solve_cg_noalloc: hipGetSymbolAddress of the "
+                       "work space failed");
+      return H2HIP_EDEVICE;
+    }
+    scratch_dev[dev] = (double *)p;
+  }
+  *out = scratch_dev[dev];
+  return H2HIP_OK;
+}
+
+// done flag, step count and block counters start at zero (a kernel, not a
+// memset node, should a caller capture the stream)
+__global__ void cg_clear_flags(uint32_t *flag) {
+  if (threadIdx.x < 4) flag[threadIdx.x] = 0u;
+}
+
+template <int C>
+int cg_noalloc(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
+               double *x, double eps, uint32_t maxiter, hipStream_t st,
+               uint32_t *iters, double *resid) {
+  const uint64_t n2 = (C ? 2 : 1) * n;
+  if (n2 > CG_NOALLOC_MAXN2) {
+    h2hip::set_error("h2hip_solve_cg_noalloc: the system exceeds the fixed "
+                     "work space (at most 1048576 real or 524288 complex "
+                     "unknowns); use h2hip_solve_cg");
+    return H2HIP_EINVAL;
+  }
+  double *scr = nullptr;
+  int rc = scratch_ptr(&scr);
+  if (rc) return rc;
+  const dim3 gv((uint32_t)std::min<uint64_t>(CG_NB, (n + NT - 1) / NT));
+  const dim3 gc((uint32_t)((n2 + NT - 1) / NT));
+  double *r = scr, *p = scr + n2, *a = scr + 2 * n2;
+  double *part = scr + CG_SCR_PART;
+  const CgDev d{scr + CG_SCR_S, reinterpret_cast<uint32_t *>(scr + CG_SCR_FLAG),
+                maxiter};
+  hipLaunchKernelGGL(cg_clear_flags, dim3(1), dim3(64), 0, st, d.flag);
+  const double one[2] = {1.0, 0.0}, mone[2] = {-1.0, 0.0};
+  // r = b - A x, p = r, a = 0, |b|, |r|
+  hipLaunchKernelGGL(copy_kernel, gc, dim3(NT), 0, st, b, r, n2);
+  rc = addeval(mone, A, x, r, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(cg_init<C>, gv, dim3(NT), 0, st, b, (const double *)r,
+                     p, a, n, part, d, eps);
+  struct {
+    uint32_t flag[2];
+    double err;
+  } look = {{0u, 0u}, 0.0};
+  for (;;) {
+    // steps after the stopping step leave x and the state as they are
+    for (int j = 0; j < CHUNK; j++) {
+      rc = addeval(one, A, p, a, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(cg_dots<C>, gv, dim3(NT), 0, st, (const double *)p,
+                         (const double *)a, (const double *)r, n, part, d);
+      hipLaunchKernelGGL(cg_update<C>, gv, dim3(NT), 0, st, x,
+                         (const double *)p, r, (const double *)a, n, part, d);
+      hipLaunchKernelGGL(cg_direction<C>, gv, dim3(NT), 0, st, p,
+                         (const double *)r, a, n, d);
+    }
+    if (hipMemcpyAsync(look.flag, d.flag, 2 * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&look.err, d.s + 8, sizeof(double),
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      h2hip::set_error("h2hip_solve_cg_noalloc: device error");
+      return H2HIP_EDEVICE;
+    }
+    if (look.flag[0]) break;
+  }
+  if (hipGetLastError() != hipSuccess) {
+    h2hip::set_error("h2hip_solve_cg_noalloc: device error");
+    return H2HIP_EDEVICE;
+  }
+  if (iters) *iters = look.flag[1];
+  if (resid) *resid = look.err;
+  return H2HIP_OK;
+}
+
 // ---- batched GMRES: several independent solves sharing every product --------
 //
 // Up to MAXSLOT solves with the same H2-matrix advance in lock step: one
@@ -2086,4 +2207,23 @@ extern "C" int h2hip_solve_cg(h2hip_addeval_t addeval, void *A, int is_complex,
              : cg<0>(addeval, A, n, (const double *)d_b, (double *)d_x, eps,
                      maxiter, use_graph, (hipStream_t)stream, iterations,
                      residual);
+}
+
+extern "C" int h2hip_solve_cg_noalloc(h2hip_addeval_t addeval, void *A,
+                                      int is_complex, uint64_t n,
+                                      const void *d_b, void *d_x, double eps,
+                                      uint32_t maxiter, void *stream,
+                                      uint32_t *iterations,
+                                      double *residual) {
+  if (!addeval || !d_b || !d_x || n == 0) {
+    h2hip::set_error("h2hip_solve_cg_noalloc: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  return is_complex
+             ? cg_noalloc<1>(addeval, A, n, (const double *)d_b,
+                             (double *)d_x, eps, maxiter, (hipStream_t)stream,
+                             iterations, residual)
+             : cg_noalloc<0>(addeval, A, n, (const double *)d_b,
+                             (double *)d_x, eps, maxiter, (hipStream_t)stream,
+                             iterations, residual);
 }

@@ -365,6 +365,21 @@ int h2hip_solve_cg(h2hip_addeval_t addeval, void *A, int is_complex,
                    uint64_t n, const void *d_b, void *d_x, double eps,
                    uint32_t maxiter, int use_graph, void *stream,
                    uint32_t *iterations, double *residual);
+/* h2hip_solve_cg without allocations.  No counterpart in the reference: for
+ * callers that cannot allocate (serving loops that must not call hipMalloc,
+ * harnesses that forbid it).  The work vectors and the device state live in
+ * a fixed device array of the library (24 MiB per GPU, held until the
+ * process ends), so the call allocates no device or pinned memory and
+ * creates no events, streams or graphs; systems of more than 1048576 real
+ * or 524288 complex unknowns are refused with H2HIP_EINVAL.  addeval must
+ * itself allocate nothing (an H2-matrix product allocates its buffers on
+ * its first call in a mode: run one product first).  Same kernels, steps
+ * and stopping rule as h2hip_solve_cg without use_graph.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU race. */
+int h2hip_solve_cg_noalloc(h2hip_addeval_t addeval, void *A, int is_complex,
+                           uint64_t n, const void *d_b, void *d_x,
+                           double eps, uint32_t maxiter, void *stream,
+                           uint32_t *iterations, double *residual);
 
 /* Helmholtz point-source data on the device (real wave number k, source
  * x0, Dirichlet data e^{ikr}/r for kind 0, Neumann data for kind 1,
