@@ -52,6 +52,9 @@ FLOPS_PER_EVAL = {0: 46.0, 1: 53.0}
 FLOPS_LL_PER_EVAL = 18.0
 
 
+# near-field launch schedules (sorted tiles, persistent, fused)
+SCHEDULES = {"per_class": (0, 0, 0), "fused": (0, 0, 1),
+             "fused_persistent": (0, 1, 1)}
 SECTIONS = ("mixed", "helmholtz", "batched", "helmholtz_solve",
             "serving_multi", "dense", "dirichlet")
 # the summary line stays well inside the tail of stdout that is kept
@@ -476,7 +479,8 @@ def bench_batched(h, args, stream):
         "kinds": {k: sum(sp.kind == k for sp in specs)
                   for k in ("sphere", "cube", "cylinder")},
         "leaf_sizes": {str(c): clfs.count(c) for c in sorted(set(clfs))},
-        "schedule": {"sorted_tiles": int(best[0]), "persistent": int(best[1]),
+        "schedule": {"tile": int(best[0]), "sorted_tiles": int(best[1]),
+                     "persistent": int(best[2]), "fused": int(best[3]),
                      "ms": tuner.schedules["batched near field"]["ms"]},
         "entries": int(mm.entries),
         "kernel_evaluations": float(evals.sum()),
@@ -1022,6 +1026,29 @@ def main():
                      2.0 * plan.entries)
     work = plan.work(ops[0])
 
+    # launch schedule of the plan, chosen by a short measurement before the
+    # timed region: launches per rule class, or fused tiles (each tile's
+    # workgroup also evaluates its pairs with common vertices) with one
+    # workgroup per tile or a persistent grid
+    sched_ms = {}
+    for name, sched in SCHEDULES.items():
+        plan.set_schedule(*sched)
+        for op, out in zip(ops, outs):
+            plan.execute(op, out.ptr, stream.handle)
+        e0, e1 = h.Event(), h.Event()
+        stream.sync()
+        e0.record(stream)
+        for _ in range(3):
+            for op, out in zip(ops, outs):
+                plan.execute(op, out.ptr, stream.handle)
+        e1.record(stream)
+        stream.sync()
+        sched_ms[name] = e0.elapsed_ms(e1) / 3
+    schedule = min(sched_ms, key=sched_ms.get)
+    plan.set_schedule(*SCHEDULES[schedule])
+    log("bench: near-field schedules (ms per step) %s, using %s" %
+        (", ".join("%s %.3f" % kv for kv in sched_ms.items()), schedule))
+
     for _ in range(args.warmup):
         for op, out in zip(ops, outs):
             plan.execute(op, out.ptr, stream.handle)
@@ -1127,10 +1154,12 @@ def main():
                          (mesh.triangles, args.clf, args.eta)),
             "entries_per_step": int(entries_per_step),
             "kernel_ms": [round(float(v), 4) for v in k_avg],
+            "schedule": schedule,
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "nf_regular+nf_singular",
+            "kernel": ("nf_fused" if schedule.startswith("fused") else
+                       "nf_regular+nf_singular"),
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -1140,6 +1169,7 @@ def main():
         "cpu_baseline": cpu,
     }
     detail["headline"] = {
+        "schedule_ms": sched_ms,
         "near_blocks": int(len(nb)),
         "kernel_evaluations_per_launch": float(work[0].sum()),
         "pairs_per_launch_by_class": [int(v) for v in work[1]],

@@ -12,10 +12,12 @@ Two choices are made by measurement, never by a fixed rule:
   time of one H2 product; the candidate with the least
   host + near field + matvecs x product wins for the class;
 
-* the launch schedule of a near-field plan: tiles in plan order or by
-  decreasing work, one workgroup per tile or the persistent regular kernel
-  (the entries do not change beyond the order of the atomic sums of the
-  linear basis); the fastest of the four is kept.
+* the tile and quadrature schedule of a near-field plan: the tile edge
+  (64 or 32 DOFs), tiles in plan order or by decreasing work, the regular
+  pairs by one workgroup per tile or a persistent grid, and the singular
+  (Sauter-Schwab) quadrature either in launches of its own per rule class
+  or fused into the tiles' workgroups (the entries do not change beyond the
+  order of the sums); the fastest combination is kept.
 
 Results are kept per instance and can be saved to / loaded from JSON.
 """
@@ -95,20 +97,29 @@ class Autotuner:
                             "costs": costs}
         return self.leaf[c]["clf"]
 
+    SCHEDULES = ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1))
+    TILES = (64, 32)
+
     def tune_schedule(self, name, plan, op, d_out, reps=None):
-        """Times the four launch schedules of a plan, keeps the fastest on
-        the plan and returns it as (sorted, persistent)."""
+        """Times the tile edges and launch schedules of a plan, keeps the
+        fastest on the plan and returns it as (tile, sorted, persistent,
+        fused)."""
         reps = reps or self.reps
         times = {}
-        for sched in ((0, 0), (1, 0), (0, 1), (1, 1)):
-            plan.set_schedule(*sched)
-            times[sched] = _time_ms(self.stream, lambda: plan.execute(
-                op, d_out, self.stream.handle), reps)
+        for tile in self.TILES:
+            plan.set_tile(tile)
+            for sched in self.SCHEDULES:
+                plan.set_schedule(*sched)
+                times[(tile,) + sched] = _time_ms(
+                    self.stream, lambda: plan.execute(
+                        op, d_out, self.stream.handle), reps)
         best = min(times, key=times.get)
-        plan.set_schedule(*best)
+        plan.set_tile(best[0])
+        plan.set_schedule(*best[1:])
         self.schedules[name] = {
             "best": list(best),
-            "ms": {"%d%d" % k: v for k, v in times.items()}}
+            "fields": ["tile", "sorted", "persistent", "fused"],
+            "ms": {"%d-%d%d%d" % k: v for k, v in times.items()}}
         return best
 
     def to_json(self):

@@ -225,14 +225,27 @@ class NearfieldPlan:
                                       _ptr(out)), "h2hip_nfplan_work")
         return out[:4], out[4:]
 
-    def set_schedule(self, sorted_tiles, persistent):
+    def set_schedule(self, sorted_tiles, persistent, fused=False):
         """Launch schedule (entries are the same up to the order of atomic
         sums of the linear basis; constant basis bit for bit): tiles in
-        decreasing order of work, and/or the persistent regular kernel."""
+        decreasing order of work, and/or the persistent regular kernel;
+        fused: each tile's workgroup also evaluates the tile's pairs with
+        common vertices (one launch for all blocks; regular order 2,
+        singular order 4)."""
         check(hip().h2hip_nfplan_set_schedule(self._h, int(bool(sorted_tiles)),
                                               int(bool(persistent))),
               "h2hip_nfplan_set_schedule")
-        self.schedule = (bool(sorted_tiles), bool(persistent))
+        check(hip().h2hip_nfplan_set_fused(self._h, int(bool(fused))),
+              "h2hip_nfplan_set_fused")
+        self.schedule = (bool(sorted_tiles), bool(persistent), bool(fused))
+
+    def set_tile(self, tile):
+        """Tile edge (8..64 DOFs): the blocks are cut again into tiles of at
+        most tile x tile DOFs, the schedule kept; the packed output does not
+        change."""
+        check(hip().h2hip_nfplan_set_tile(self._h, int(tile)),
+              "h2hip_nfplan_set_tile")
+        self.tile = int(tile)
 
     def execute(self, op, d_out, stream=None, far=False):
         """Computes all blocks into the device buffer at address d_out."""

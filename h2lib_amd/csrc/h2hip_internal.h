@@ -118,6 +118,10 @@ struct Tile {
   uint8_t nr, nc;       // tile size
   uint8_t ntrans;
   uint8_t flags;        // bit 0: identity term allowed (lc quirk)
+  // the tile's pairs with common vertices in the list grouped by tile:
+  // from sbeg, ns[0] identical, then ns[1] edge, then ns[2] vertex pairs
+  uint32_t sbeg;
+  uint16_t ns[3];
 };
 
 struct Fixup {  // identity term of the cc double layer at (t, t), t != s
@@ -157,6 +161,10 @@ struct KArgs {
   // persistent schedule of the regular kernel: tile counter (nullptr: one
   // workgroup per tile)
   uint32_t *counter;
+  // fused tiles (regular order 2, singular order 4): the singular pairs
+  // grouped by tile, evaluated by the tile's workgroup (nullptr: separate
+  // launches per rule class over the class-sorted list)
+  const SPair *tlist;
 };
 
 // Launches the regular tile kernel over `ntiles` tiles and then (unless
@@ -253,6 +261,7 @@ struct h2hip_nfplan {
   std::vector<h2hip::Supp> supp;
   std::vector<h2hip::Fixup> fix;
   std::vector<h2hip::SPair> spairs;  // sorted by class, identical first
+  std::vector<h2hip::SPair> tpairs;  // grouped by tile (Tile::sbeg, ns)
   uint64_t pairs[4] = {0, 0, 0, 0};  // support pairs per class
   h2hip::DevBuf<h2hip::Tile> d_tiles;
   h2hip::DevBuf<h2hip::Supp> d_supp;
@@ -264,4 +273,12 @@ struct h2hip_nfplan {
   std::vector<h2hip::Tile> tiles0;    // plan order, kept once sorted
   std::vector<h2hip::SPair> spairs0;
   h2hip::DevBuf<uint32_t> d_counter;
+  // fused schedule: tiles by decreasing total work (regular and singular
+  // evaluations) and the singular pairs grouped by tile
+  int fused = 0;
+  // tile edge and the inputs of h2hip_nfplan_create (for a new tiling)
+  uint32_t tile = h2hip::TILE;
+  std::vector<uint32_t> in_ridx, in_cidx, in_blocks;
+  h2hip::DevBuf<h2hip::Tile> d_tiles_f;
+  h2hip::DevBuf<h2hip::SPair> d_tpairs;
 };

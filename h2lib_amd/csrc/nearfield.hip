@@ -162,9 +162,39 @@ int classify(const uint32_t *tv, const uint32_t *sv, uint16_t &perm) {
   return p;
 }
 
+// Runs f(t, lo, hi) on up to `threads` host threads over [0, n) cut into
+// contiguous ranges in order (thread t gets the t-th range).
+template <class F>
+void parallel_ranges(uint32_t n, unsigned threads, F &&f) {
+  threads = std::max(1u, std::min<unsigned>(threads, n ? n : 1));
+  if (threads == 1) {
+    f(0u, 0u, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < threads; t++) {
+    const uint32_t lo = (uint32_t)((uint64_t)n * t / threads);
+    const uint32_t hi = (uint32_t)((uint64_t)n * (t + 1) / threads);
+    pool.emplace_back([&f, t, lo, hi]() { f(t, lo, hi); });
+  }
+  for (auto &th : pool) th.join();
+}
+
+// host threads of the planner: up to 16, one for small plans
+unsigned plan_threads(uint64_t work) {
+  if (work < 4096) return 1;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return std::min(16u, hw);
+}
+
 // ntr: per-block ntrans flags (nullptr: ntrans for every block); st: the
 // uploads keep large enough device buffers of P and are ordered on st
 // (otherwise new buffers, complete on return).
+//
+// The blocks and then the tiles are split between host threads in
+// contiguous ranges, each with its own vertex / triangle scratch maps; the
+// pieces are joined in range order, so the plan is the one a single thread
+// builds.
 int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
                const uint32_t *ridx_all, uint32_t nridx,
                const uint32_t *cidx_all, uint32_t ncidx, uint32_t nblocks,
@@ -176,6 +206,11 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
   P->supp.clear();
   P->fix.clear();
   P->spairs.clear();
+  P->tpairs.clear();
+  // a fused schedule is set again for the new plan
+  P->fused = 0;
+  P->d_tiles_f.release();
+  P->d_tpairs.release();
   for (auto &c : P->pairs) c = 0;
   P->s = s;
   P->rb = rb;
@@ -184,9 +219,11 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
   P->off.resize(nblocks);
   const uint32_t nrdof = rb ? s->nv : s->nt;
   const uint32_t ncdof = cb ? s->nv : s->nt;
-  std::vector<int32_t> pos(s->nv, -1), tpos(s->nt, -1);
-  std::vector<Tile> tiles;
-  uint64_t total = 0;
+  // tile edge (at most TILE, the size of the kernels' LDS tile)
+  const uint32_t tile = std::min<uint32_t>(TILE, std::max<uint32_t>(8, P->tile));
+
+  // block offsets in the packed output, and the checks of the blocks
+  uint64_t total = 0, dofs = 0;
   for (uint32_t b = 0; b < nblocks; b++) {
     const uint32_t roff = blocks[4 * b], rows = blocks[4 * b + 1];
     const uint32_t coff = blocks[4 * b + 2], cols = blocks[4 * b + 3];
@@ -194,80 +231,119 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
       set_error("block exceeds the index arrays");
       return H2HIP_EINVAL;
     }
-    const uint32_t *ri = ridx_all + roff, *ci = cidx_all + coff;
-    for (uint32_t i = 0; i < rows; i++)
-      if (ri[i] >= nrdof) {
-        set_error("row index out of range");
-        return H2HIP_EINVAL;
-      }
-    for (uint32_t j = 0; j < cols; j++)
-      if (ci[j] >= ncdof) {
-        set_error("column index out of range");
-        return H2HIP_EINVAL;
-      }
-    const uint64_t boff = total;
-    const int bnt = ntr ? (ntr[b] ? 1 : 0) : ntrans;
-    const uint32_t ld = bnt ? cols : rows;
-    P->off[b] = boff;
+    P->off[b] = total;
     total += (uint64_t)rows * cols;
-    for (uint32_t r0 = 0; r0 < rows; r0 += TILE) {
-      const uint32_t nr = std::min<uint32_t>(TILE, rows - r0);
-      const uint32_t rs0 = (uint32_t)P->supp.size();
-      tile_side(s, rb, ri + r0, nr, pos, tpos, P->supp);
-      const uint32_t nrs = (uint32_t)P->supp.size() - rs0;
-      for (uint32_t c0 = 0; c0 < cols; c0 += TILE) {
-        const uint32_t nc = std::min<uint32_t>(TILE, cols - c0);
-        const uint32_t cs0 = (uint32_t)P->supp.size();
-        tile_side(s, cb, ci + c0, nc, pos, tpos, P->supp);
-        const uint32_t ncs = (uint32_t)P->supp.size() - cs0;
-        if (nrs > 0xffff || ncs > 0xffff ||
-            (uint64_t)nrs * ncs >= (1ull << 31)) {
-          set_error("tile support too large");
-          return H2HIP_EINVAL;
-        }
-        Tile T;
-        T.out_off = boff + (bnt ? (uint64_t)c0 + (uint64_t)r0 * ld
-                                : (uint64_t)r0 + (uint64_t)c0 * ld);
-        T.ld = ld;
-        T.rsup = rs0;
-        T.csup = cs0;
-        T.nrsup = (uint16_t)nrs;
-        T.ncsup = (uint16_t)ncs;
-        T.row0 = r0;
-        T.col0 = c0;
-        T.nr = (uint8_t)nr;
-        T.nc = (uint8_t)nc;
-        T.ntrans = (uint8_t)bnt;
-        // The reference's lc identity term is skipped for blocks with at
-        // most three columns (Library/bem3d.c:2533 tests a loop variable
-        // that equals 3 there).
-        T.flags = (cols > 3) ? 1 : 0;
-        tiles.push_back(T);
-      }
-    }
-    // cc identity terms the reference leaves at (t, t) with t != s
-    if (!rb && !cb) {
-      std::unordered_map<uint32_t, std::vector<uint32_t>> where;
-      for (uint32_t j = 0; j < cols; j++) where[ci[j]].push_back(j);
-      for (uint32_t i = 0; i < rows; i++) {
-        auto it = where.find(ri[i]);
-        if (it == where.end()) continue;
-        for (uint32_t j : it->second) {
-          // i == j: added with the entry itself.  The reference visits the
-          // columns in order and overwrites (t, t) when it computes column
-          // t, so a term added from column s < t does not survive; only
-          // s > t leaves it in place (and keeps (t, t) inside the window).
-          if (j <= i) continue;
-          Fixup f;
-          f.pos = boff + (uint64_t)i + (uint64_t)i * ld;
-          f.tri = ri[i];
-          f.pad = (uint32_t)bnt;  // the block's ntrans
-          P->fix.push_back(f);
-        }
-      }
-    }
+    dofs += (uint64_t)rows + cols;
   }
   P->entries = total;
+
+  // ---- tiles and support lists, per range of blocks ----
+  struct Part {
+    std::vector<Tile> tiles;
+    std::vector<Supp> supp;
+    std::vector<Fixup> fix;
+    std::string err;
+  };
+  const unsigned nthr = plan_threads(dofs);
+  std::vector<Part> parts(std::max(1u, std::min<unsigned>(nthr, nblocks)));
+  parallel_ranges(nblocks, (unsigned)parts.size(),
+                  [&](unsigned t, uint32_t lo, uint32_t hi) {
+    Part &W = parts[t];
+    std::vector<int32_t> pos(s->nv, -1), tpos(s->nt, -1);
+    for (uint32_t b = lo; b < hi; b++) {
+      const uint32_t roff = blocks[4 * b], rows = blocks[4 * b + 1];
+      const uint32_t coff = blocks[4 * b + 2], cols = blocks[4 * b + 3];
+      const uint32_t *ri = ridx_all + roff, *ci = cidx_all + coff;
+      for (uint32_t i = 0; i < rows; i++)
+        if (ri[i] >= nrdof) {
+          W.err = "row index out of range";
+          return;
+        }
+      for (uint32_t j = 0; j < cols; j++)
+        if (ci[j] >= ncdof) {
+          W.err = "column index out of range";
+          return;
+        }
+      const uint64_t boff = P->off[b];
+      const int bnt = ntr ? (ntr[b] ? 1 : 0) : ntrans;
+      const uint32_t ld = bnt ? cols : rows;
+      for (uint32_t r0 = 0; r0 < rows; r0 += tile) {
+        const uint32_t nr = std::min<uint32_t>(tile, rows - r0);
+        const uint32_t rs0 = (uint32_t)W.supp.size();
+        tile_side(s, rb, ri + r0, nr, pos, tpos, W.supp);
+        const uint32_t nrs = (uint32_t)W.supp.size() - rs0;
+        for (uint32_t c0 = 0; c0 < cols; c0 += tile) {
+          const uint32_t nc = std::min<uint32_t>(tile, cols - c0);
+          const uint32_t cs0 = (uint32_t)W.supp.size();
+          tile_side(s, cb, ci + c0, nc, pos, tpos, W.supp);
+          const uint32_t ncs = (uint32_t)W.supp.size() - cs0;
+          if (nrs > 0xffff || ncs > 0xffff ||
+              (uint64_t)nrs * ncs >= (1ull << 31)) {
+            W.err = "tile support too large";
+            return;
+          }
+          Tile T;
+          T.sbeg = 0;
+          T.ns[0] = T.ns[1] = T.ns[2] = 0;
+          T.out_off = boff + (bnt ? (uint64_t)c0 + (uint64_t)r0 * ld
+                                  : (uint64_t)r0 + (uint64_t)c0 * ld);
+          T.ld = ld;
+          T.rsup = rs0;  // relative to the part's support list
+          T.csup = cs0;
+          T.nrsup = (uint16_t)nrs;
+          T.ncsup = (uint16_t)ncs;
+          T.row0 = r0;
+          T.col0 = c0;
+          T.nr = (uint8_t)nr;
+          T.nc = (uint8_t)nc;
+          T.ntrans = (uint8_t)bnt;
+          // The reference's lc identity term is skipped for blocks with at
+          // most three columns (Library/bem3d.c:2533 tests a loop variable
+          // that equals 3 there).
+          T.flags = (cols > 3) ? 1 : 0;
+          W.tiles.push_back(T);
+        }
+      }
+      // cc identity terms the reference leaves at (t, t) with t != s
+      if (!rb && !cb) {
+        std::unordered_map<uint32_t, std::vector<uint32_t>> where;
+        for (uint32_t j = 0; j < cols; j++) where[ci[j]].push_back(j);
+        for (uint32_t i = 0; i < rows; i++) {
+          auto it = where.find(ri[i]);
+          if (it == where.end()) continue;
+          for (uint32_t j : it->second) {
+            // i == j: added with the entry itself.  The reference visits
+            // the columns in order and overwrites (t, t) when it computes
+            // column t, so a term added from column s < t does not
+            // survive; only s > t leaves it in place (and keeps (t, t)
+            // inside the window).
+            if (j <= i) continue;
+            Fixup f;
+            f.pos = boff + (uint64_t)i + (uint64_t)i * ld;
+            f.tri = ri[i];
+            f.pad = (uint32_t)bnt;  // the block's ntrans
+            W.fix.push_back(f);
+          }
+        }
+      }
+    }
+  });
+  std::vector<Tile> tiles;
+  for (auto &W : parts) {
+    if (!W.err.empty()) {
+      set_error(W.err);
+      return H2HIP_EINVAL;
+    }
+    const uint32_t base = (uint32_t)P->supp.size();
+    for (auto &T : W.tiles) {
+      T.rsup += base;
+      T.csup += base;
+    }
+    tiles.insert(tiles.end(), W.tiles.begin(), W.tiles.end());
+    P->supp.insert(P->supp.end(), W.supp.begin(), W.supp.end());
+    P->fix.insert(P->fix.end(), W.fix.begin(), W.fix.end());
+  }
+  parts.clear();
 
   // largest tiles first: the tail of the launch runs the small ones
   std::vector<uint32_t> order(tiles.size());
@@ -278,45 +354,93 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
   });
   P->tiles.resize(tiles.size());
   for (size_t i = 0; i < order.size(); i++) P->tiles[i] = tiles[order[i]];
+  tiles.clear();
 
-  // pairs with common vertices, found through the vertex-to-triangle map
-  std::vector<SPair> by_cls[4];
-  std::vector<int32_t> cpos(s->nt, -1);
-  for (uint32_t ti = 0; ti < (uint32_t)P->tiles.size(); ti++) {
-    const Tile &T = P->tiles[ti];
-    P->pairs[0] += (uint64_t)T.nrsup * T.ncsup;
-    for (uint32_t c = 0; c < T.ncsup; c++)
-      cpos[P->supp[T.csup + c].tri] = (int32_t)c;
-    for (uint32_t r = 0; r < T.nrsup; r++) {
-      const Supp &sr = P->supp[T.rsup + r];
-      uint32_t seen[64];
-      int nseen = 0;
-      for (int l = 0; l < 3; l++) {
-        const uint32_t v = sr.v[l];
-        for (uint32_t p = s->v2t_ptr[v]; p < s->v2t_ptr[v + 1]; p++) {
-          const uint32_t tri = s->v2t[p];
-          const int32_t c = cpos[tri];
-          if (c < 0) continue;
-          bool dup = false;
-          for (int k = 0; k < nseen; k++) dup |= (seen[k] == tri);
-          if (dup) continue;
-          if (nseen < 64) seen[nseen++] = tri;
-          SPair q;
-          q.tile = ti;
-          q.r = (uint16_t)r;
-          q.c = (uint16_t)c;
-          q.cls = (uint16_t)classify(sr.v, P->supp[T.csup + c].v, q.perm);
-          by_cls[q.cls].push_back(q);
+  // ---- pairs with common vertices, found through the vertex-to-triangle
+  // map, per range of tiles ----
+  struct SPart {
+    std::vector<SPair> by_cls[4];
+    std::vector<SPair> tp;  // grouped by tile
+    uint64_t pairs0 = 0;
+    std::string err;
+  };
+  const uint32_t ntiles = (uint32_t)P->tiles.size();
+  std::vector<SPart> sparts(
+      std::max(1u, std::min<unsigned>(plan_threads(P->supp.size()), ntiles)));
+  parallel_ranges(ntiles, (unsigned)sparts.size(),
+                  [&](unsigned t, uint32_t lo, uint32_t hi) {
+    SPart &W = sparts[t];
+    std::vector<int32_t> cpos(s->nt, -1);
+    for (uint32_t ti = lo; ti < hi; ti++) {
+      Tile &T = P->tiles[ti];
+      W.pairs0 += (uint64_t)T.nrsup * T.ncsup;
+      size_t b0[4];
+      for (int c = 1; c < 4; c++) b0[c] = W.by_cls[c].size();
+      for (uint32_t c = 0; c < T.ncsup; c++)
+        cpos[P->supp[T.csup + c].tri] = (int32_t)c;
+      for (uint32_t r = 0; r < T.nrsup; r++) {
+        const Supp &sr = P->supp[T.rsup + r];
+        uint32_t seen[64];
+        int nseen = 0;
+        for (int l = 0; l < 3; l++) {
+          const uint32_t v = sr.v[l];
+          for (uint32_t p = s->v2t_ptr[v]; p < s->v2t_ptr[v + 1]; p++) {
+            const uint32_t tri = s->v2t[p];
+            const int32_t c = cpos[tri];
+            if (c < 0) continue;
+            bool dup = false;
+            for (int k = 0; k < nseen; k++) dup |= (seen[k] == tri);
+            if (dup) continue;
+            if (nseen < 64) seen[nseen++] = tri;
+            SPair q;
+            q.tile = ti;
+            q.r = (uint16_t)r;
+            q.c = (uint16_t)c;
+            q.cls = (uint16_t)classify(sr.v, P->supp[T.csup + c].v, q.perm);
+            W.by_cls[q.cls].push_back(q);
+          }
         }
       }
+      for (uint32_t c = 0; c < T.ncsup; c++)
+        cpos[P->supp[T.csup + c].tri] = -1;
+      // the tile's pairs again, grouped by tile: identical, edge, vertex
+      T.sbeg = (uint32_t)W.tp.size();  // relative to the part's list
+      for (int c = 3; c >= 1; c--) {
+        const size_t cnt = W.by_cls[c].size() - b0[c];
+        if (cnt > 0xffff) {
+          W.err = "too many pairs with common vertices in one tile";
+          return;
+        }
+        T.ns[3 - c] = (uint16_t)cnt;
+        W.tp.insert(W.tp.end(), W.by_cls[c].begin() + b0[c],
+                    W.by_cls[c].end());
+      }
     }
-    for (uint32_t c = 0; c < T.ncsup; c++) cpos[P->supp[T.csup + c].tri] = -1;
+  });
+  for (auto &W : sparts) {
+    if (!W.err.empty()) {
+      set_error(W.err);
+      return H2HIP_EINVAL;
+    }
+  }
+  for (uint32_t part = 0, ti = 0; part < sparts.size(); part++) {
+    const uint32_t hi =
+        (uint32_t)((uint64_t)ntiles * (part + 1) / sparts.size());
+    const uint32_t base = (uint32_t)P->tpairs.size();
+    for (; ti < hi; ti++) P->tiles[ti].sbeg += base;
+    P->tpairs.insert(P->tpairs.end(), sparts[part].tp.begin(),
+                     sparts[part].tp.end());
+    P->pairs[0] += sparts[part].pairs0;
   }
   for (int c = 3; c >= 1; c--) {
-    P->pairs[c] = by_cls[c].size();
-    P->pairs[0] -= by_cls[c].size();
-    P->spairs.insert(P->spairs.end(), by_cls[c].begin(), by_cls[c].end());
+    for (auto &W : sparts) {
+      P->pairs[c] += W.by_cls[c].size();
+      P->spairs.insert(P->spairs.end(), W.by_cls[c].begin(),
+                       W.by_cls[c].end());
+    }
+    P->pairs[0] -= P->pairs[c];
   }
+  sparts.clear();
   if (grow) {
     H2HIP_CHECK(P->d_tiles.upload_grow(P->tiles.data(), P->tiles.size(), st));
     H2HIP_CHECK(P->d_supp.upload_grow(P->supp.data(), P->supp.size(), st));
@@ -366,6 +490,11 @@ int execute(const h2hip_op *op, const h2hip_nfplan *P, int far, void *d_out,
   a.far = far;
   a.out = (double *)d_out;
   a.counter = P->persistent ? P->d_counter.p : nullptr;
+  a.tlist = nullptr;
+  if (P->fused && !far && op->q == 2 && op->q2 == 4) {
+    a.tiles = P->d_tiles_f.p;
+    a.tlist = P->d_tpairs.p;
+  }
   const uint32_t ntiles = (uint32_t)P->tiles.size();
   uint32_t ncls[4];
   for (int c = 0; c < 4; c++) ncls[c] = (uint32_t)P->pairs[c];
@@ -887,8 +1016,43 @@ int h2hip_nfplan_create(h2hip_nfplan **out, const h2hip_surface *s,
     delete P;
     return rc;
   }
+  // the inputs, for a new tiling (h2hip_nfplan_set_tile)
+  P->in_ridx.assign(ridx_all, ridx_all + nridx);
+  P->in_cidx.assign(cidx_all, cidx_all + ncidx);
+  P->in_blocks.assign(blocks, blocks + 4 * (size_t)nblocks);
   *out = P;
   return H2HIP_OK;
+}
+
+int h2hip_nfplan_set_tile(h2hip_nfplan *p, uint32_t tile) {
+  if (!p || tile < 8 || tile > (uint32_t)TILE) {
+    set_error("h2hip_nfplan_set_tile: invalid arguments (tile 8..64)");
+    return H2HIP_EINVAL;
+  }
+  if (tile == p->tile) return H2HIP_OK;
+  const int sorted = p->sorted, persistent = p->persistent,
+            fused = p->fused;
+  const uint32_t old = p->tile;
+  p->tile = tile;
+  int rc = build_plan(p, p->s, p->rb, p->cb, p->in_ridx.data(),
+                      (uint32_t)p->in_ridx.size(), p->in_cidx.data(),
+                      (uint32_t)p->in_cidx.size(), p->nblocks,
+                      p->in_blocks.data(), 0);
+  if (rc != H2HIP_OK) {
+    // the previous tiling again (it was valid)
+    p->tile = old;
+    (void)build_plan(p, p->s, p->rb, p->cb, p->in_ridx.data(),
+                     (uint32_t)p->in_ridx.size(), p->in_cidx.data(),
+                     (uint32_t)p->in_cidx.size(), p->nblocks,
+                     p->in_blocks.data(), 0);
+  }
+  // the new tiles are in plan order: the schedule is applied again
+  p->sorted = 0;
+  p->tiles0.clear();
+  p->spairs0.clear();
+  int rc2 = h2hip_nfplan_set_schedule(p, sorted, persistent);
+  if (rc2 == H2HIP_OK) rc2 = h2hip_nfplan_set_fused(p, fused);
+  return rc != H2HIP_OK ? rc : rc2;
 }
 
 void h2hip_nfplan_destroy(h2hip_nfplan *p) { delete p; }
@@ -937,6 +1101,30 @@ int h2hip_nfplan_set_schedule(h2hip_nfplan *p, int sorted, int persistent) {
   H2HIP_CHECK(p->d_tiles.upload(p->tiles.data(), p->tiles.size()));
   H2HIP_CHECK(p->d_spairs.upload(p->spairs.data(), p->spairs.size()));
   p->sorted = sorted;
+  return H2HIP_OK;
+}
+
+int h2hip_nfplan_set_fused(h2hip_nfplan *p, int fused) {
+  if (!p || fused < 0 || fused > 1) {
+    set_error("h2hip_nfplan_set_fused: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  if (fused && !p->d_tiles_f.p && !p->tiles.empty()) {
+    // tiles by decreasing work at regular order 2 and singular order 4
+    // (16 evaluations per regular pair, 1536 per identical or edge pair,
+    // 512 per vertex pair)
+    std::vector<Tile> tf = p->tiles;
+    auto work = [](const Tile &T) {
+      return (uint64_t)T.nrsup * T.ncsup * 16 +
+             ((uint64_t)T.ns[0] + T.ns[1]) * 1536 + (uint64_t)T.ns[2] * 512;
+    };
+    std::stable_sort(tf.begin(), tf.end(), [&](const Tile &x, const Tile &y) {
+      return work(x) > work(y);
+    });
+    H2HIP_CHECK(p->d_tiles_f.upload(tf.data(), tf.size()));
+    H2HIP_CHECK(p->d_tpairs.upload(p->tpairs.data(), p->tpairs.size()));
+  }
+  p->fused = fused;
   return H2HIP_OK;
 }
 

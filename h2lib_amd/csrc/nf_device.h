@@ -164,8 +164,18 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
 // workgroup.  MP: 0 FP64; 1 mixed (FP32 kernel values and inner sums, FP64
 // outer sums); 2 mixed with FP32 accumulation throughout, the outer sums
 // compensated (Kahan).
-template <int KER, int BR, int BC, int Q, int MP>
-__device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
+template <int KER, int BR, int BC, int CLS, int Q2, bool MP>
+__device__ __forceinline__ void tile_singular(const KArgs &a, const Tile &T,
+                                              const SPair *list, uint32_t n,
+                                              double *acc_re, double *acc_im);
+
+// FUSED: the tile's workgroup also evaluates the tile's pairs with common
+// vertices (tlist + T.sbeg, T.ns[c] per class) before the tile is stored,
+// so they add into the LDS accumulator instead of the output (constant
+// basis: the regular pass leaves their entries to the singular pass).
+template <int KER, int BR, int BC, int Q, int MP, bool FUSED = false>
+__device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile,
+                                             const SPair *tlist = nullptr) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
   constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
   constexpr bool CC = (BR == 0) && (BC == 0);
@@ -356,7 +366,9 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
     }
     if constexpr (CC) {
       // entry (r, c) of the tile; pairs with common vertices are stored as
-      // zero and completed by the singular kernel
+      // zero and completed by the singular kernel (fused: stored by the
+      // singular pass below)
+      if (FUSED && shared) continue;
       const size_t pos = T.out_off + (T.ntrans ? (size_t)c + (size_t)r * T.ld
                                                : (size_t)r + (size_t)c * T.ld);
       if constexpr (CPLX) {
@@ -382,6 +394,21 @@ __device__ __forceinline__ void regular_tile(const KArgs &a, uint32_t tile) {
     }
   }
 
+  if constexpr (FUSED) {
+    if (!a.far) {
+      constexpr int Q2 = 4;
+      constexpr bool SMP = MP != 0;
+      const SPair *L = tlist + T.sbeg;
+      tile_singular<KER, BR, BC, CLS_IDENTICAL, Q2, SMP>(a, T, L, T.ns[0],
+                                                         acc_re, acc_im);
+      L += T.ns[0];
+      tile_singular<KER, BR, BC, CLS_EDGE, Q2, SMP>(a, T, L, T.ns[1], acc_re,
+                                                    acc_im);
+      L += T.ns[1];
+      tile_singular<KER, BR, BC, CLS_VERTEX, Q2, SMP>(a, T, L, T.ns[2],
+                                                      acc_re, acc_im);
+    }
+  }
   if constexpr (!CC) {
     __syncthreads();
     const int nr = T.nr, nc = T.nc;
@@ -422,6 +449,31 @@ __global__ __launch_bounds__(NTHREADS) void nf_regular_persistent(
     __syncthreads();
     if (t >= ntiles) break;
     regular_tile<KER, BR, BC, Q, MP>(a, t);
+  }
+}
+
+// Regular and singular pairs of each tile in one workgroup (singular rule
+// order 4): one workgroup per tile, or, persistent, a resident grid taking
+// tiles from a counter as nf_regular_persistent does.  tlist: the singular
+// pairs grouped by tile.
+template <int KER, int BR, int BC, int Q, int MP>
+__global__ __launch_bounds__(NTHREADS) void nf_fused_kernel(
+    KArgs a, const SPair *__restrict__ tlist) {
+  regular_tile<KER, BR, BC, Q, MP, true>(a, blockIdx.x, tlist);
+}
+
+template <int KER, int BR, int BC, int Q, int MP>
+__global__ __launch_bounds__(NTHREADS) void nf_fused_persistent(
+    KArgs a, const SPair *__restrict__ tlist, uint32_t ntiles,
+    uint32_t *counter) {
+  __shared__ uint32_t cur;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = cur;
+    __syncthreads();
+    if (t >= ntiles) break;
+    regular_tile<KER, BR, BC, Q, MP, true>(a, t, tlist);
   }
 }
 
@@ -497,26 +549,32 @@ __device__ __forceinline__ double row_sum(double v) {
   return v;
 }
 
-// One pair with common vertices per 16 lanes (16 pairs per workgroup), all
-// pairs of one rule class per launch.  Lane s of a row takes the tensor
-// index pairs (k, l) = (s / q2, s % q2), (s + 16) / q2, ... and loops over
-// (i, j); the quadrature points are formed on the fly from the Gauss nodes
-// (a.gauss = [g | w], q2 each), so the kernel reads no rule tables.
-template <int KER, int BR, int BC, int CLS, int Q2, bool MP>
-__global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
-    KArgs a, const SPair *__restrict__ list, uint32_t npairs) {
+// element i (0..2) of a three-vector by selects (a dynamic index would put
+// the vector in private memory)
+template <class V>
+__device__ __forceinline__ V sel3(const V (&v)[3], int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
+// One pair with common vertices per row of 16 lanes.  Lane s of the row
+// takes the tensor index pairs (k, l) = (s / q2, s % q2), (s + 16) / q2, ...
+// and loops over (i, j); the quadrature points are formed on the fly from
+// the Gauss nodes (a.gauss = [g | w], q2 each), so no rule tables are read.
+// The row's lanes sum their moments; lanes e < NR NC then hand the entry
+// (lr, lc) of the tile (tile-local row and column) to sink(lr, lc, re, im):
+// global atomics for the launches per rule class, the tile's LDS
+// accumulator (or a plain store, constant basis) when the tile's own
+// workgroup evaluates its singular pairs.  The value is not conjugated.
+template <int KER, int BR, int BC, int CLS, int Q2, bool MP, class Sink>
+__device__ __forceinline__ void singular_pair(const KArgs &a, const Tile &T,
+                                              const SPair &P, int sub,
+                                              Sink &&sink) {
   constexpr bool CPLX = KER >= K_HELM_SLP;
   constexpr bool DLP = (KER == K_LAP_DLP) || (KER == K_HELM_DLP);
   constexpr bool CC = (BR == 0) && (BC == 0);
   constexpr int NR = BR ? 3 : 1;
   constexpr int NC = BC ? 3 : 1;
 
-  const uint32_t w = blockIdx.x * (NTHREADS / 16) + (threadIdx.x >> 4);
-  const int sub = threadIdx.x & 15;
-  if (w >= npairs) return;  // uniform per row of 16 lanes
-
-  const SPair P = list[w];
-  const Tile T = a.tiles[P.tile];
   const Supp sr = a.supp[T.rsup + P.r], sc = a.supp[T.csup + P.c];
   int tp[3], sp[3];
 #pragma unroll
@@ -524,13 +582,19 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
     tp[i] = (P.perm >> (2 * i)) & 3;
     sp[i] = (P.perm >> (6 + 2 * i)) & 3;
   }
+  uint32_t tv[3], sv[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    tv[i] = sel3(sr.v, tp[i]);
+    sv[i] = sel3(sc.v, sp[i]);
+  }
   double D0[3], E1[3], E2[3], F1[3], F2[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const double At = a.x[3 * sr.v[tp[0]] + d], Bt = a.x[3 * sr.v[tp[1]] + d],
-                 Ct = a.x[3 * sr.v[tp[2]] + d];
-    const double As = a.x[3 * sc.v[sp[0]] + d], Bs = a.x[3 * sc.v[sp[1]] + d],
-                 Cs = a.x[3 * sc.v[sp[2]] + d];
+    const double At = a.x[3 * tv[0] + d], Bt = a.x[3 * tv[1] + d],
+                 Ct = a.x[3 * tv[2] + d];
+    const double As = a.x[3 * sv[0] + d], Bs = a.x[3 * sv[1] + d],
+                 Cs = a.x[3 * sv[2] + d];
     D0[d] = At - As;
     E1[d] = Bt - At;
     E2[d] = Ct - Bt;
@@ -934,8 +998,8 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
     }
   }
   if (sub >= NR * NC) return;
-  // lane e adds the entry of permuted local functions (l, k) = (e % NR,
-  // e / NR)
+  // lane e hands over the entry of permuted local functions (l, k) =
+  // (e % NR, e / NR)
   const int l = sub % NR, k = sub / NR;
   double vre = 0.0, vim = 0.0;
 #pragma unroll
@@ -946,8 +1010,8 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
         vre = Sre[ll][kk];
         vim = Sim[ll][kk];
       }
-  const uint32_t lr = loc_of(sr.loc, BR ? tp[l] : 0);
-  const uint32_t lc = loc_of(sc.loc, BC ? sp[k] : 0);
+  const uint32_t lr = loc_of(sr.loc, BR ? sel3(tp, l) : 0);
+  const uint32_t lc = loc_of(sc.loc, BC ? sel3(sp, k) : 0);
   if (lr == 0xffu || lc == 0xffu) return;
   const double gt = a.g[sr.tri];
   const double f2 = a.kc * gt * a.g[sc.tri];
@@ -974,15 +1038,71 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
       }
     }
   }
+  sink(lr, lc, vre, vim);
+}
+
+// The pairs with common vertices of one rule class, one pair per row of 16
+// lanes (16 pairs per workgroup), added into the output with FP64 atomics
+// (the regular kernel has stored the tiles before).
+template <int KER, int BR, int BC, int CLS, int Q2, bool MP>
+__global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
+    KArgs a, const SPair *__restrict__ list, uint32_t npairs) {
+  constexpr bool CPLX = KER >= K_HELM_SLP;
+  const uint32_t w = blockIdx.x * (NTHREADS / 16) + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & 15;
+  if (w >= npairs) return;  // uniform per row of 16 lanes
+  const SPair P = list[w];
+  const Tile T = a.tiles[P.tile];
   const bool conj = T.ntrans && (BR == BC);
-  const size_t pos =
-      T.out_off + (T.ntrans ? (size_t)lc + (size_t)lr * T.ld
-                            : (size_t)lr + (size_t)lc * T.ld);
-  if constexpr (CPLX) {
-    atomicAdd(&a.out[2 * pos], vre);
-    atomicAdd(&a.out[2 * pos + 1], conj ? -vim : vim);
-  } else {
-    atomicAdd(&a.out[pos], vre);
+  singular_pair<KER, BR, BC, CLS, Q2, MP>(
+      a, T, P, sub, [&](uint32_t lr, uint32_t lc, double vre, double vim) {
+        const size_t pos =
+            T.out_off + (T.ntrans ? (size_t)lc + (size_t)lr * T.ld
+                                  : (size_t)lr + (size_t)lc * T.ld);
+        if constexpr (CPLX) {
+          atomicAdd(&a.out[2 * pos], vre);
+          atomicAdd(&a.out[2 * pos + 1], conj ? -vim : vim);
+        } else {
+          atomicAdd(&a.out[pos], vre);
+        }
+      });
+}
+
+// The pairs with common vertices of one tile and one rule class, by the
+// tile's workgroup: 16 pairs per pass, the entries added into the tile's
+// LDS accumulator (constant basis: stored, each entry belongs to one pair).
+template <int KER, int BR, int BC, int CLS, int Q2, bool MP>
+__device__ __forceinline__ void tile_singular(const KArgs &a, const Tile &T,
+                                              const SPair *list, uint32_t n,
+                                              double *acc_re,
+                                              double *acc_im) {
+  constexpr bool CPLX = KER >= K_HELM_SLP;
+  constexpr bool CC = (BR == 0) && (BC == 0);
+  const uint32_t row = threadIdx.x >> 4;
+  const int sub = threadIdx.x & 15;
+  const bool conj = T.ntrans && (BR == BC);
+  for (uint32_t p0 = 0; p0 < n; p0 += NTHREADS / 16) {
+    const uint32_t w = p0 + row;
+    if (w >= n) continue;  // uniform per row of 16 lanes
+    singular_pair<KER, BR, BC, CLS, Q2, MP>(
+        a, T, list[w], sub,
+        [&](uint32_t lr, uint32_t lc, double vre, double vim) {
+          if constexpr (CC) {
+            const size_t pos =
+                T.out_off + (T.ntrans ? (size_t)lc + (size_t)lr * T.ld
+                                      : (size_t)lr + (size_t)lc * T.ld);
+            if constexpr (CPLX) {
+              a.out[2 * pos] = vre;
+              a.out[2 * pos + 1] = conj ? -vim : vim;
+            } else {
+              a.out[pos] = vre;
+            }
+          } else {
+            const uint32_t e = lr + TILE * lc;
+            atomicAdd(&acc_re[e], vre);
+            if constexpr (CPLX) atomicAdd(&acc_im[e], vim);
+          }
+        });
   }
 }
 
@@ -992,6 +1112,29 @@ __global__ __launch_bounds__(NTHREADS) void nf_singular_kernel(
 template <int KER, int BR, int BC, int MP>
 int launch_bases_p(uint32_t q, const KArgs &a, uint32_t ntiles,
                    const SPair *sp, const uint32_t *ncls, hipStream_t stream) {
+  if (a.tlist && !a.far && q == 2 && a.q2 == 4) {
+    // fused tiles: one launch for the regular and the singular pairs
+    if (!ntiles) return H2HIP_OK;
+    if (a.counter) {
+      auto fk = nf_fused_persistent<KER, BR, BC, 2, MP>;
+      int per_cu = 0, dev = 0, ncu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fk, NTHREADS,
+                                                       0) != hipSuccess ||
+          hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                dev) != hipSuccess)
+        return H2HIP_EDEVICE;
+      const uint32_t resident = (uint32_t)std::max(1, per_cu) * (uint32_t)ncu;
+      const uint32_t grid = std::min(ntiles, std::max(1u, resident));
+      hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, stream, a.counter);
+      hipLaunchKernelGGL(fk, dim3(grid), dim3(NTHREADS), 0, stream, a,
+                         a.tlist, ntiles, a.counter);
+    } else {
+      hipLaunchKernelGGL((nf_fused_kernel<KER, BR, BC, 2, MP>), dim3(ntiles),
+                         dim3(NTHREADS), 0, stream, a, a.tlist);
+    }
+    return H2HIP_OK;
+  }
   if (ntiles && a.counter) {
     void (*reg)(KArgs, uint32_t, uint32_t *) = nullptr;
     switch (q) {

@@ -173,6 +173,21 @@ int h2hip_nfplan_offsets(const h2hip_nfplan *p, uint64_t *off);
  * tiles from a device counter.  Not thread-safe against concurrent
  * executions of the same plan. */
 int h2hip_nfplan_set_schedule(h2hip_nfplan *p, int sorted, int persistent);
+/* Fused tiles (no counterpart in the reference): fused = 1 evaluates each
+ * tile's pairs with common vertices in the tile's own workgroup, added into
+ * its LDS accumulator before the tile is stored, so one launch (one
+ * workgroup per tile, or the persistent grid when persistent = 1) assembles
+ * every block, tiles taken by decreasing total work; no global atomics.
+ * Applies to regular order 2 with singular order 4 (other orders keep the
+ * launches per rule class).  Same entries up to the order of the sums. */
+int h2hip_nfplan_set_fused(h2hip_nfplan *p, int fused);
+/* Tile edge of the plan (8..64 DOFs, default 64; no counterpart in the
+ * reference): the blocks are cut again into tiles of at most tile x tile
+ * DOFs, the schedule kept.  Smaller tiles give more workgroups for small
+ * blocks; with the linear basis a triangle pair whose vertices fall into
+ * several tiles is evaluated once per tile.  The packed output and the
+ * entries (up to the order of the sums) do not change. */
+int h2hip_nfplan_set_tile(h2hip_nfplan *p, uint32_t tile);
 /* Kernel evaluations per rule class (regular, vertex, edge, identical) and
  * triangle pairs per class, as the reference's loops perform them
  * (counted on the host; out[0..3] evaluations, out[4..7] pairs). */

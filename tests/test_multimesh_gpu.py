@@ -120,9 +120,20 @@ def test_autotuner_picks_candidates_and_keeps_entries():
     mm.execute(op, ref.ptr, stream.handle)
     out = h.DeviceArray(mm.entries)
     best = tuner.tune_schedule("test", mm.plan, op, out.ptr)
-    assert tuple(best) in ((0, 0), (1, 0), (0, 1), (1, 1))
+    assert best[0] in tuner.TILES and tuple(best[1:]) in tuner.SCHEDULES
+    # constant basis: one store per entry, the same bits whatever the tile
+    # edge and the schedule (the tuned one, and every other one)
     out.fill_nan()
     mm.execute(op, out.ptr, stream.handle)
     stream.sync()
     assert np.array_equal(out.download(), ref.download())
     assert "test" in tuner.to_json()["schedules"]
+    for tile in tuner.TILES:
+        mm.plan.set_tile(tile)
+        for sched in tuner.SCHEDULES:
+            mm.plan.set_schedule(*sched)
+            out.fill_nan()
+            mm.execute(op, out.ptr, stream.handle)
+            stream.sync()
+            assert np.array_equal(out.download(), ref.download()), (tile,
+                                                                    sched)

@@ -168,14 +168,19 @@ def test_batched_plan_matches_single_calls(sphere8):
             assert rel_block_err(got, want) < 1e-14, (name, i)
 
 
-@pytest.mark.parametrize("sorted_tiles,persistent", [(1, 0), (0, 1), (1, 1)])
-def test_schedules_give_identical_entries(sorted_tiles, persistent, sphere8):
+@pytest.mark.parametrize("sorted_tiles,persistent,fused",
+                         [(1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1),
+                          (0, 1, 1)])
+def test_schedules_give_identical_entries(sorted_tiles, persistent, fused,
+                                          sphere8):
     # the launch schedule changes the order of the tiles only: the entries
     # are the same (bit for bit where no atomic sums are involved), in FP64
-    # and in mixed precision
+    # and in mixed precision; fused tiles evaluate their pairs with common
+    # vertices in their own workgroup (one launch, no global atomics)
     import h2lib_amd as h
     rng = np.random.default_rng(21)
-    for name in ("lap_slp_cc", "helm_dlp_ll"):
+    for name in ("lap_slp_cc", "helm_dlp_cc", "helm_dlp_ll", "lap_dlp_lc",
+                 "helm_slp_cl", "lap_adlp_ll"):
         kern, op, br, bc, alpha, k = CASES[name]
         gr = h.Surface3d(sphere8["x"], sphere8["t"])
         b = h.Bem3d(gr, kern, op, 2, 4, br, bc, alpha=alpha, k=k)
@@ -191,12 +196,12 @@ def test_schedules_give_identical_entries(sorted_tiles, persistent, sphere8):
                            int(rng.integers(0, nc - cols)), cols))
         plan = h.NearfieldPlan(gr, br, bc, ridx, cidx, blocks)
         stream = h.Stream()
-        for mode in (0, 1):
+        for mode in (0, 1, 2):
             b.set_precision(mode)
             plan.set_schedule(0, 0)
             ref = h.DeviceArray(plan.entries, b.dtype)
             plan.execute(b, ref.ptr, stream.handle)
-            plan.set_schedule(sorted_tiles, persistent)
+            plan.set_schedule(sorted_tiles, persistent, fused)
             out = h.DeviceArray(plan.entries, b.dtype)
             out.fill_nan()
             for _ in range(2):  # the tile counter is reset per launch
@@ -211,3 +216,46 @@ def test_schedules_give_identical_entries(sorted_tiles, persistent, sphere8):
                 # whose order varies from run to run
                 err = np.abs(got - want).max() / np.abs(want).max()
                 assert err < 1e-14, (name, mode, err)
+
+
+@pytest.mark.parametrize("tile", [32, 24, 8])
+def test_tile_edges_give_identical_entries(tile, sphere8):
+    # a plan cut into smaller tiles gives the entries of the 64-DOF tiles
+    # (constant basis bit for bit; the linear basis evaluates a triangle
+    # pair once per tile its vertices fall into and sums in another order),
+    # with the launches per rule class and with fused tiles
+    import h2lib_amd as h
+    rng = np.random.default_rng(5)
+    for name in ("lap_dlp_cc", "lap_slp_ll", "helm_dlp_lc"):
+        kern, op, br, bc, alpha, k = CASES[name]
+        gr = h.Surface3d(sphere8["x"], sphere8["t"])
+        b = h.Bem3d(gr, kern, op, 2, 4, br, bc, alpha=alpha, k=k)
+        nr = gr.vertices if br else gr.triangles
+        nc = gr.vertices if bc else gr.triangles
+        ridx = rng.permutation(nr).astype(np.uint32)
+        cidx = rng.permutation(nc).astype(np.uint32)
+        blocks = []
+        for _ in range(40):
+            rows = int(rng.integers(1, 150))
+            cols = int(rng.integers(1, 150))
+            blocks.append((int(rng.integers(0, nr - rows)), rows,
+                           int(rng.integers(0, nc - cols)), cols))
+        plan = h.NearfieldPlan(gr, br, bc, ridx, cidx, blocks)
+        stream = h.Stream()
+        ref = h.DeviceArray(plan.entries, b.dtype)
+        plan.execute(b, ref.ptr, stream.handle)
+        stream.sync()
+        want = ref.download()
+        plan.set_tile(tile)
+        for fused in (0, 1):
+            plan.set_schedule(0, 0, fused)
+            out = h.DeviceArray(plan.entries, b.dtype)
+            out.fill_nan()
+            plan.execute(b, out.ptr, stream.handle)
+            stream.sync()
+            got = out.download()
+            if br == 0 and bc == 0:
+                assert np.array_equal(got, want), (name, fused)
+            else:
+                err = np.abs(got - want).max() / np.abs(want).max()
+                assert err < 1e-14, (name, fused, err)
