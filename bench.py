@@ -96,7 +96,12 @@ def compact(d):
                          "traffic": mv.get("traffic"),
                          "adjoint_ms": r4(mv["adjoint_ms"]),
                          "computed_cpl_ms": r4(
-                             (mv["computed_couplings"] or {}).get("ms"))}
+                             (mv["computed_couplings"] or {}).get("ms")),
+                         "computed_cpl_frac": r4(
+                             (mv["computed_couplings"] or {}).get("frac")),
+                         "computed_cpl_adjoint_ms": r4(
+                             (mv["computed_couplings"] or {}).get(
+                                 "adjoint_ms_V"))}
     mp = d.get("mixed_precision")
     if mp:
         out["mixed"] = {
@@ -729,6 +734,26 @@ def bench_dirichlet(h, mesh, args, stream):
         computed = product_line(prob.V,
                                 time_product(h, prob.V, d_x, d_y, stream),
                                 n)
+        # the adjoint with computed couplings (the double layer operator,
+        # whose adjoint is not itself)
+        for H in (prob.V, prob.KM):
+            if H is prob.KM:
+                H.set_coupling_mode(1)
+            for _ in range(3):
+                H.addeval(1.0, d_x, d_y, stream, trans=True)
+            e0, e1 = h.Event(), h.Event()
+            stream.sync()
+            e0.record(stream)
+            for _ in range(50):
+                H.addeval(1.0, d_x, d_y, stream, trans=True)
+            e1.record(stream)
+            stream.sync()
+            computed["adjoint_ms_" + ("V" if H is prob.V else "KM")] = \
+                e0.elapsed_ms(e1) / 50
+            if H is prob.KM:
+                computed["ms_KM"] = time_product(h, prob.KM, d_x, d_y,
+                                                 stream)
+                H.set_coupling_mode(0)
         prob.V.set_coupling_mode(0)
     # the faster of the two serves the solves (both operators)
     use_computed = computed is not None and computed["ms"] < stored["ms"]
@@ -1030,7 +1055,12 @@ def main():
     # timed region: launches per rule class, or fused tiles (each tile's
     # workgroup also evaluates its pairs with common vertices) with one
     # workgroup per tile or a persistent grid
+    # (the device is brought to its working clocks first: the first
+    # launches of a run are slower and would penalise the first schedule)
     sched_ms = {}
+    for _ in range(max(5, args.warmup)):
+        for op, out in zip(ops, outs):
+            plan.execute(op, out.ptr, stream.handle)
     for name, sched in SCHEDULES.items():
         plan.set_schedule(*sched)
         for op, out in zip(ops, outs):
@@ -1038,12 +1068,12 @@ def main():
         e0, e1 = h.Event(), h.Event()
         stream.sync()
         e0.record(stream)
-        for _ in range(3):
+        for _ in range(5):
             for op, out in zip(ops, outs):
                 plan.execute(op, out.ptr, stream.handle)
         e1.record(stream)
         stream.sync()
-        sched_ms[name] = e0.elapsed_ms(e1) / 3
+        sched_ms[name] = e0.elapsed_ms(e1) / 5
     schedule = min(sched_ms, key=sched_ms.get)
     plan.set_schedule(*SCHEDULES[schedule])
     log("bench: near-field schedules (ms per step) %s, using %s" %

@@ -79,11 +79,42 @@ __device__ void last_block_sum(const double *part, int m, uint32_t *cnt,
   if (t == 0) *cnt = 0;
 }
 
+// Sums of 32 values over the 64 lanes of a wave by a butterfly that halves
+// the values a lane holds at each step (the lane keeps one half and adds
+// its partner's copy of it): lane l returns the sum of value l & 31 over
+// all lanes.  32 exchanges instead of 6 for each value.  v is clobbered.
+template <int S>
+__device__ __forceinline__ void bfly_step(double (&v)[32], int lane) {
+  const bool up = (lane & S) != 0;
+#pragma unroll
+  for (int t = 0; t < S; t++) {
+    const double send = up ? v[t] : v[t + S];
+    const double keep = up ? v[t + S] : v[t];
+    v[t] = keep + __shfl_xor(send, S, 64);
+  }
+}
+
+__device__ __forceinline__ double wave_sum32(double (&v)[32], int lane) {
+  bfly_step<16>(v, lane);
+  bfly_step<8>(v, lane);
+  bfly_step<4>(v, lane);
+  bfly_step<2>(v, lane);
+  bfly_step<1>(v, lane);
+  return v[0] + __shfl_xor(v[0], 32, 64);
+}
+
+// Ways of combining the blocks' sums of the vector kernels: the last block
+// of the grid adds them in block order (fixed order), every block adds
+// into the output with FP64 atomics, or every block leaves its sums in
+// `part` for the next kernel (sum_parts), with no atomics and no fences.
+enum { RED_LAST = 0, RED_ATOM = 1, RED_PART = 2 };
+
 // out[0 .. 2m) = Q[:, 0..m)^* w (complex pairs; imaginary parts 0 when real)
-// ATOM: every block adds its sums into out (zeroed beforehand) with FP64
-// atomics instead of the last-block reduction, whose agent-scope fences
-// write back and invalidate the L2 of every XCD
-template <int C, bool ATOM = false>
+// ATOM (RED_ATOM): every block adds its sums into out (zeroed beforehand)
+// with FP64 atomics instead of the last-block reduction, whose agent-scope
+// fences write back and invalidate the L2 of every XCD; RED_PART: the
+// block's sums stay in part[(b * 64 + j) * 2 + {0, 1}]
+template <int C, int ATOM = RED_LAST>
 __device__ __forceinline__ void multidot_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
@@ -118,25 +149,18 @@ __device__ __forceinline__ void multidot_body(
         }
       }
     }
-#pragma unroll
-    for (int j = 0; j < 32; j++) {
-      if (j < mj) {
-        double r = acc_re[j], im = acc_im[j];
-        for (int off = 32; off > 0; off >>= 1) {
-          r += __shfl_xor(r, off, 64);
-          if constexpr (C) im += __shfl_xor(im, off, 64);
-        }
-        if (lane == 0) {
-          red[wv][2 * j] = r;
-          red[wv][2 * j + 1] = im;
-        }
-      }
+    // the 32 sums over the wave's lanes: lane l ends with sum l & 31
+    const double r = wave_sum32(acc_re, lane);
+    const double im = C ? wave_sum32(acc_im, lane) : 0.0;
+    if (lane < mj) {
+      red[wv][2 * lane] = r;
+      red[wv][2 * lane + 1] = im;
     }
     __syncthreads();
     if (threadIdx.x < 2 * mj) {
       double s = 0.0;
       for (int v = 0; v < NT / 64; v++) s += red[v][threadIdx.x];
-      if constexpr (ATOM)
+      if constexpr (ATOM == RED_ATOM)
         atomicAdd(&out[2 * j0 + threadIdx.x], s);
       else
         part[((uint64_t)blockIdx.x * 64 + (j0 + threadIdx.x / 2)) * 2 +
@@ -144,10 +168,10 @@ __device__ __forceinline__ void multidot_body(
     }
     __syncthreads();
   }
-  if constexpr (!ATOM) last_block_sum(part, m, cnt, out);
+  if constexpr (ATOM == RED_LAST) last_block_sum(part, m, cnt, out);
 }
 
-template <int C, bool ATOM = false>
+template <int C, int ATOM = RED_LAST>
 __global__ __launch_bounds__(NT) void multidot(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
@@ -156,15 +180,16 @@ __global__ __launch_bounds__(NT) void multidot(
 }
 
 // w -= sign Q h (h: m complex pairs); with nrm != nullptr also
-// nrm[0] = |w|^2 of the result (last-block reduction)
-template <int C, bool ATOM = false>
+// nrm[0] = |w|^2 of the result (last-block reduction; RED_PART: the block's
+// sum in part[b * 128], with nrm only a flag)
+template <int C, int ATOM = RED_LAST>
 __device__ __forceinline__ void multiaxpy_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ h, double sign, double *__restrict__ w,
     uint64_t n, double *__restrict__ part, uint32_t *cnt, double *nrm) {
-  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
   double sq = 0.0;
-  if (i < n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * NT) {
     double ar = 0.0, ai = 0.0;
     for (int j = 0; j < m; j++) {
       const uint64_t q = (uint64_t)j * ldq + i;
@@ -182,11 +207,11 @@ __device__ __forceinline__ void multiaxpy_body(
       const double vi = fma(-sign, ai, w[2 * i + 1]);
       w[2 * i] = vr;
       w[2 * i + 1] = vi;
-      sq = fma(vr, vr, vi * vi);
+      sq = fma(vr, vr, fma(vi, vi, sq));
     } else {
       const double v = fma(-sign, ar, w[i]);
       w[i] = v;
-      sq = v * v;
+      sq = fma(v, v, sq);
     }
   }
   if (nrm == nullptr) return;  // uniform over the grid
@@ -197,22 +222,82 @@ __device__ __forceinline__ void multiaxpy_body(
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int v = 0; v < NT / 64; v++) s += red[v];
-    if constexpr (ATOM) {
+    if constexpr (ATOM == RED_ATOM) {
       atomicAdd(nrm, s);
     } else {
       part[(uint64_t)blockIdx.x * 128] = s;
       part[(uint64_t)blockIdx.x * 128 + 1] = 0.0;
     }
   }
-  if constexpr (!ATOM) last_block_sum(part, 1, cnt, nrm);
+  if constexpr (ATOM == RED_LAST) last_block_sum(part, 1, cnt, nrm);
 }
 
-template <int C, bool ATOM = false>
+template <int C, int ATOM = RED_LAST>
 __global__ __launch_bounds__(NT) void multiaxpy(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ h, double sign, double *__restrict__ w,
     uint64_t n, double *__restrict__ part, uint32_t *cnt, double *nrm) {
   multiaxpy_body<C, ATOM>(Q, ldq, m, h, sign, w, n, part, cnt, nrm);
+}
+
+// hs[0 .. 2m) = the sums over nblk blocks of the partial sums a previous
+// kernel left at part[(b * 64 + j) * 2 + {0, 1}] (RED_PART), formed by
+// every block of the next kernel: the threads split the blocks between
+// NT / (2 m) groups per value, the groups are combined in LDS.
+__device__ void sum_parts(const double *part, uint32_t nblk, int m,
+                          double *hs) {
+  __shared__ double red[NT];
+  const int nv = 2 * m, groups = NT / nv, t = threadIdx.x;
+  double s = 0.0;
+  if (t < groups * nv) {
+    const int v = t % nv;
+    for (uint32_t b = t / nv; b < nblk; b += groups)
+      s += part[((uint64_t)b * 64 + (v >> 1)) * 2 + (v & 1)];
+  }
+  red[t] = s;
+  __syncthreads();
+  if (t < nv) {
+    double r = 0.0;
+    for (int g = 0; g < groups; g++) r += red[g * nv + t];
+    hs[t] = r;
+  }
+  __syncthreads();
+}
+
+// Gram-Schmidt of one Arnoldi step without atomics (single solver):
+// gs_dot leaves the blocks' sums of Q^* w; gs_axpy_dot sums them (h, kept
+// in hout by block 0), forms w -= Q h on its rows and leaves the blocks'
+// sums of Q^* w again; gs_axpy_norm does the second w -= Q h and leaves
+// the blocks' sums of |w|^2.  The three kernels share the grid and the
+// rows of each thread.
+template <int C>
+__global__ __launch_bounds__(NT) void gs_dot(const double *__restrict__ Q,
+                                             uint64_t ldq, int m,
+                                             const double *__restrict__ w,
+                                             uint64_t n, double *pout) {
+  multidot_body<C, RED_PART>(Q, ldq, m, w, n, pout, nullptr, nullptr);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void gs_axpy_dot(
+    const double *__restrict__ Q, uint64_t ldq, int m, const double *pin,
+    uint32_t nblk, double *hout, double *w, uint64_t n, double *pout) {
+  __shared__ double hs[128];
+  sum_parts(pin, nblk, m, hs);
+  if (blockIdx.x == 0 && threadIdx.x < 2 * m) hout[threadIdx.x] = hs[threadIdx.x];
+  multiaxpy_body<C, RED_PART>(Q, ldq, m, hs, 1.0, w, n, nullptr, nullptr,
+                              nullptr);
+  multidot_body<C, RED_PART>(Q, ldq, m, w, n, pout, nullptr, nullptr);
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void gs_axpy_norm(
+    const double *__restrict__ Q, uint64_t ldq, int m, const double *pin,
+    uint32_t nblk, double *hout, double *w, uint64_t n, double *pout) {
+  __shared__ double hs[128];
+  sum_parts(pin, nblk, m, hs);
+  if (blockIdx.x == 0 && threadIdx.x < 2 * m) hout[threadIdx.x] = hs[threadIdx.x];
+  multiaxpy_body<C, RED_PART>(Q, ldq, m, hs, 1.0, w, n, pout, nullptr, pout);
 }
 
 __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
@@ -273,22 +358,41 @@ struct LSQ {
 // q_{k+1} = w / |w| by all threads; thread 0 of block 0 adds column k of the
 // Hessenberg matrix (h = hA + hB, subdiagonal |w|) to the QR factorisation
 // with Givens rotations and stores the new residual estimate |g[k+1]|.
+// With wz != nullptr, w is left zero for the next product, and nz doubles
+// at acc (the other set of accumulators, which the next step adds into) are
+// cleared.
 __device__ __forceinline__ void scale_givens_body(
-    const double *__restrict__ w, double *__restrict__ qn, uint64_t n2,
+    const double *w, double *__restrict__ qn, uint64_t n2,
     const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
-    double *err) {
+    double *err, double *wz = nullptr, double *acc = nullptr,
+    uint32_t nz = 0) {
   const double hn = sqrt(nrm2[0]);
   const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
-  if (i < n2) qn[i] = hn > 0.0 ? w[i] / hn : 0.0;
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (i < n2) {
+    const double wv = w[i];
+    qn[i] = hn > 0.0 ? wv / hn : 0.0;
+    if (wz) wz[i] = 0.0;  // the same element, after its read
+  }
+  if (i < nz) acc[i] = 0.0;
+  if (blockIdx.x != 0) return;  // uniform per block
+  // block 0 stages the Hessenberg column h = hA + hB and the previous
+  // rotations in LDS (loads in parallel), so that the sequential rotations
+  // of thread 0 wait for no memory access
+  __shared__ double sh[2 * (MAXK + 1)], scs[2 * MAXK], ssn[2 * MAXK];
+  for (int e = threadIdx.x; e < 2 * (k + 1); e += NT) sh[e] = hA[e] + hB[e];
+  for (int e = threadIdx.x; e < 2 * k; e += NT) {
+    scs[e] = s.cs[e];
+    ssn[e] = s.sn[e];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   // previous rotations, streamed: rotation j touches entries j and j + 1,
   // and entry j is final (column k of R) after it
-  cplx cur = {hA[0] + hB[0], hA[1] + hB[1]};
+  cplx cur = {sh[0], sh[1]};
   for (int j = 0; j < k; j++) {
-    const cplx c = {s.cs[2 * j], s.cs[2 * j + 1]};
-    const cplx sn = {s.sn[2 * j], s.sn[2 * j + 1]};
-    const cplx nx = {hA[2 * (j + 1)] + hB[2 * (j + 1)],
-                     hA[2 * (j + 1) + 1] + hB[2 * (j + 1) + 1]};
+    const cplx c = {scs[2 * j], scs[2 * j + 1]};
+    const cplx sn = {ssn[2 * j], ssn[2 * j + 1]};
+    const cplx nx = {sh[2 * (j + 1)], sh[2 * (j + 1) + 1]};
     const cplx t = cadd(cmul(c, cur), cmul(sn, nx));
     cur = csub(cmul(c, nx), cmul(cconj(sn), cur));
     s.R[2 * (j + (uint64_t)k * s.kmax)] = t.re;
@@ -330,6 +434,26 @@ __global__ __launch_bounds__(NT) void scale_givens(
     const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
     double *err) {
   scale_givens_body(w, qn, n2, nrm2, hA, hB, s, k, err);
+}
+
+// scale_givens of the single solver: also w = 0 and the next step's
+// accumulators cleared (w is not restrict here: read, then written, by the
+// same thread)
+__global__ __launch_bounds__(NT) void scale_givens_z(
+    double *w, double *__restrict__ qn, uint64_t n2, const double *nrm2,
+    const double *hA, const double *hB, LSQ s, int k, double *err,
+    double *acc, uint32_t nz) {
+  scale_givens_body(w, qn, n2, nrm2, hA, hB, s, k, err, w, acc, nz);
+}
+
+// the same with |w|^2 summed from the blocks' partial sums of gs_axpy_norm
+__global__ __launch_bounds__(NT) void scale_givens_p(
+    double *w, double *__restrict__ qn, uint64_t n2, const double *pn,
+    uint32_t nblk, const double *hA, const double *hB, LSQ s, int k,
+    double *err) {
+  __shared__ double hs[2];
+  sum_parts(pn, nblk, 1, hs);
+  scale_givens_body(w, qn, n2, hs, hA, hB, s, k, err, w, nullptr, 0);
 }
 
 // q_0 = r / |r| by all threads; thread 0 of block 0 starts the
@@ -379,12 +503,14 @@ struct Workspace {
   int kmax = 0;
   h2hip::DevBuf<double> Q, w, part, small;
   h2hip::DevBuf<uint32_t> cnt;
-  // views into `small`
+  // views into `small`; hA1, hB1, nrm1: the second set of accumulators of
+  // the single solver (steps alternate between the two sets)
   double *hA = nullptr, *hB = nullptr, *nrm = nullptr, *y = nullptr,
          *err = nullptr, *R = nullptr, *g = nullptr, *cs = nullptr,
-         *sn = nullptr;
+         *sn = nullptr, *hA1 = nullptr, *hB1 = nullptr, *nrm1 = nullptr;
   hipStream_t st = nullptr;
   dim3 gv;
+  uint64_t part2 = 0;  // offset of the second partial-sum area in `part`
 
   int alloc(int C, uint64_t n_, int kmax_, hipStream_t s) {
     n = n_;
@@ -392,14 +518,18 @@ struct Workspace {
     kmax = kmax_;
     st = s;
     gv = dim3((uint32_t)((n2 + NT - 1) / NT));
-    const uint64_t nsmall = 128 * 3 + 8 + CHUNK + 2 * (uint64_t)kmax * kmax +
+    const uint64_t nsmall = 128 * 3 + 8 + 128 * 2 + 8 + CHUNK +
+                            2 * (uint64_t)kmax * kmax +
                             2 * ((uint64_t)kmax + 1) + 4 * (uint64_t)kmax;
     // partial sums, 128 doubles per block: the multi-dot grid has at most
     // NB blocks, the multi-axpy grid (whose second pass also reduces |w|^2)
     // one block per NT entries
+    // (two such areas: the single solver's Gram-Schmidt kernels read one
+    // while writing the other)
     const uint64_t nblk = std::max<uint64_t>(NB, (n + NT - 1) / NT);
+    part2 = nblk * 64 * 2;
     if (Q.alloc(n2 * kmax) != hipSuccess || w.alloc(n2) != hipSuccess ||
-        part.alloc(nblk * 64 * 2) != hipSuccess ||
+        part.alloc(2 * part2) != hipSuccess ||
         small.alloc(nsmall) != hipSuccess || cnt.alloc(4) != hipSuccess)
       return H2HIP_ENOMEM;
     double *p = small.p;
@@ -410,6 +540,12 @@ struct Workspace {
     y = p;
     p += 128;
     nrm = p;
+    p += 8;
+    hA1 = p;
+    p += 128;
+    hB1 = p;
+    p += 128;
+    nrm1 = p;
     p += 8;
     err = p;
     p += CHUNK;
@@ -443,15 +579,18 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // multi-dot grid: one row per thread up to NB blocks
   const dim3 gdot((uint32_t)std::min<uint64_t>(NB, (n + NT - 1) / NT));
 
-  // the step's atomic accumulators hA, hB (with y between them) and nrm
-  // lie at the start of the small workspace, before the residual
-  // estimates
-  const uint32_t nacc = (uint32_t)(W.nrm + 8 - W.small.p);
-  const dim3 gz((uint32_t)((std::max<uint64_t>(W.n2, nacc) + NT - 1) / NT));
+  // The steps' atomic accumulators: set 0 (hA, hB, with y between them,
+  // and nrm) and set 1 (hA1, hB1, nrm1) lie one after the other at the
+  // start of the small workspace.  Step k adds into set k & 1, and its last
+  // kernel clears the other set and w for step k + 1.
+  const uint32_t nacc0 = (uint32_t)(W.nrm + 8 - W.small.p);
+  const uint32_t nacc1 = (uint32_t)(W.nrm1 + 8 - W.hA1);
+  const uint32_t nacc = nacc0 + nacc1;
   // reproducible mode: fixed-order reductions (the last block of a grid
   // adds the blocks' sums in block order), the arithmetic of the batched
   // solver's slots, instead of FP64 atomics
   const bool rep = h2hip::reproducible();
+  const dim3 gred = gdot;
   // |b|
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(NT), 0, st, W.nrm,
                      (uint64_t)2);
@@ -468,22 +607,26 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   const double normb = std::sqrt(nb2[0]);
 
   // r = b - A x in column 0, normalised; starts the least-squares problem
+  // (both sets of accumulators cleared, |r|^2 in set 1, w = 0 for the
+  // first step's product)
+  const dim3 gz((uint32_t)((std::max<uint64_t>(W.n2, nacc) + NT - 1) / NT));
   auto init = [&]() -> int {
     double *q0 = W.col(0);
-    hipLaunchKernelGGL(copy2_kernel, W.gv, dim3(NT), 0, st, b, q0, W.n2,
-                       W.nrm, 2u);
+    hipLaunchKernelGGL(copy2_kernel, gz, dim3(NT), 0, st, b, q0, W.n2,
+                       W.small.p, nacc);
+    hipLaunchKernelGGL(zero_kernel, W.gv, dim3(NT), 0, st, W.w.p, W.n2);
     int r = addeval(mone, A, x, q0, st);
     if (r) return r;
     if (rep)  // |r|^2 as the batched solver's cycle start forms it
       hipLaunchKernelGGL((multiaxpy<C>), gax, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, 0, (const double *)W.hB,
-                         1.0, q0, n, W.part.p, W.cnt.p + 1, W.nrm);
+                         1.0, q0, n, W.part.p, W.cnt.p + 1, W.nrm1);
     else
-      hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
+      hipLaunchKernelGGL((multidot<C, true>), gred, dim3(NT), 0, st,
                          (const double *)q0, (uint64_t)0, 1,
-                         (const double *)q0, n, W.part.p, W.cnt.p, W.nrm);
+                         (const double *)q0, n, W.part.p, W.cnt.p, W.nrm1);
     hipLaunchKernelGGL(start_cycle, W.gv, dim3(NT), 0, st, q0, W.n2,
-                       (const double *)W.nrm, W.lsq(), W.err);
+                       (const double *)W.nrm1, W.lsq(), W.err);
     return H2HIP_OK;
   };
   // x += Q_k y with R y = g
@@ -499,45 +642,53 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // one Arnoldi step on column k; its residual estimate goes to err[slot]
   auto step = [&](int k, int slot) -> int {
     double *qk = W.col(k), *w = W.w.p;
-    hipLaunchKernelGGL(zero2_kernel, gz, dim3(NT), 0, st, w, W.n2,
-                       W.small.p, nacc);
+    // this step's accumulators, and the other set, cleared for the next
+    double *hA = (k & 1) ? W.hA1 : W.hA, *hB = (k & 1) ? W.hB1 : W.hB;
+    double *nrm = (k & 1) ? W.nrm1 : W.nrm;
+    double *acc_next = (k & 1) ? W.small.p : W.hA1;
+    const uint32_t n_next = (k & 1) ? nacc0 : nacc1;
     int r = addeval(one, A, qk, w, st);
     if (r) return r;
     if (rep) {
       hipLaunchKernelGGL((multidot<C>), gdot, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1, (const double *)w,
-                         n, W.part.p, W.cnt.p + 2, W.hA);
+                         n, W.part.p, W.cnt.p + 2, hA);
       hipLaunchKernelGGL((multiaxpy<C>), gax, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1,
-                         (const double *)W.hA, 1.0, w, n, W.part.p,
+                         (const double *)hA, 1.0, w, n, W.part.p,
                          W.cnt.p + 1, (double *)nullptr);
       hipLaunchKernelGGL((multidot<C>), gdot, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1, (const double *)w,
-                         n, W.part.p, W.cnt.p + 2, W.hB);
+                         n, W.part.p, W.cnt.p + 2, hB);
       hipLaunchKernelGGL((multiaxpy<C>), gax, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1,
-                         (const double *)W.hB, 1.0, w, n, W.part.p,
-                         W.cnt.p + 1, W.nrm);
+                         (const double *)hB, 1.0, w, n, W.part.p,
+                         W.cnt.p + 1, nrm);
     } else {
-      hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
+      // no atomics: each kernel leaves its blocks' sums for the next one
+      double *pa = W.part.p, *pb = W.part.p + W.part2;
+      const uint32_t nb = gred.x;
+      hipLaunchKernelGGL((gs_dot<C>), gred, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1, (const double *)w,
-                         n, W.part.p, W.cnt.p + 2, W.hA);
-      hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
-                         (const double *)W.Q.p, n, k + 1,
-                         (const double *)W.hA, 1.0, w, n, W.part.p,
-                         W.cnt.p + 1, (double *)nullptr);
-      hipLaunchKernelGGL((multidot<C, true>), gdot, dim3(NT), 0, st,
-                         (const double *)W.Q.p, n, k + 1, (const double *)w,
-                         n, W.part.p, W.cnt.p + 2, W.hB);
-      hipLaunchKernelGGL((multiaxpy<C, true>), gax, dim3(NT), 0, st,
-                         (const double *)W.Q.p, n, k + 1,
-                         (const double *)W.hB, 1.0, w, n, W.part.p,
-                         W.cnt.p + 1, W.nrm);
+                         n, pa);
+      hipLaunchKernelGGL((gs_axpy_dot<C>), gred, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1, (const double *)pa,
+                         nb, hA, w, n, pb);
+      hipLaunchKernelGGL((gs_axpy_norm<C>), gred, dim3(NT), 0, st,
+                         (const double *)W.Q.p, n, k + 1, (const double *)pb,
+                         nb, hB, w, n, pa);
+      hipLaunchKernelGGL(scale_givens_p, W.gv, dim3(NT), 0, st, w,
+                         W.col(k + 1), W.n2, (const double *)pa, nb,
+                         (const double *)hA, (const double *)hB, W.lsq(), k,
+                         W.err + slot);
+      return H2HIP_OK;
     }
-    hipLaunchKernelGGL(scale_givens, W.gv, dim3(NT), 0, st,
-                       (const double *)w, W.col(k + 1), W.n2,
-                       (const double *)W.nrm, (const double *)W.hA,
-                       (const double *)W.hB, W.lsq(), k, W.err + slot);
+    const dim3 gs((uint32_t)((std::max<uint64_t>(W.n2, n_next) + NT - 1) /
+                             NT));
+    hipLaunchKernelGGL(scale_givens_z, gs, dim3(NT), 0, st, w, W.col(k + 1),
+                       W.n2, (const double *)nrm, (const double *)hA,
+                       (const double *)hB, W.lsq(), k, W.err + slot,
+                       acc_next, n_next);
     return H2HIP_OK;
   };
 

@@ -126,3 +126,52 @@ def test_gmres_with_more_rows_than_reduction_blocks():
     assert it > 0 and np.isfinite(true)
     # GMRES's least-squares residual is the true residual up to rounding
     assert abs(true - res) <= 1e-3 * res + 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kind", ["real", "complex"])
+def test_gmres_throughput_mode_matches_reproducible(kind):
+    # GMRES with the throughput reductions (products with atomics, the
+    # Gram-Schmidt sums of each step passed between its kernels without
+    # atomics) against the fixed-order solve of the same system: true
+    # residual at the requested level, steps within max(2, 3%), solutions
+    # within 1e-7 (both stop at |r| <= 1e-9 |b|)
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import LaplaceDirichlet, solve_gmres
+    mesh = G.sphere(10)
+    rng = np.random.default_rng(12)
+    if kind == "real":
+        prob = LaplaceDirichlet(mesh, clf=24, eta=2.0, m=3)
+        H = prob.V
+        b = rng.standard_normal(prob.n)
+    else:
+        from h2lib_amd.bem3d import Surface3d, new_slp_helmholtz_bem3d
+        from h2lib_amd.h2matrix import H2Matrix
+        gr = Surface3d(mesh.x, mesh.t, mesh.n, mesh.g)
+        op = new_slp_helmholtz_bem3d(2.0, gr, 2, 4, 0, 0)
+        rt = G.ClusterTree(mesh, 0, 24)
+        bt = G.BlockTree(rt, rt, 2.0, 0)
+        H = H2Matrix(op, mesh, rt, rt, bt, 3)
+        b = rng.standard_normal(H.n) + 1j * rng.standard_normal(H.n)
+    d_b = h.DeviceArray.from_host(b)
+    out = {}
+    for rep in (True, False):
+        h.set_reproducible(rep)
+        try:
+            for kmax in (10, 30):
+                d_x = h.DeviceArray(H.n, H.dtype)
+                d_x.zero()
+                it, res = solve_gmres(H, d_b, d_x, 1e-9, 4000, kmax)
+                out[(rep, kmax)] = (it, d_x.download())
+        finally:
+            h.set_reproducible(True)
+    for kmax in (10, 30):
+        it_r, x_r = out[(True, kmax)]
+        it_a, x_a = out[(False, kmax)]
+        print(kind, "kmax", kmax, "steps", it_r, it_a, flush=True)
+        assert it_r < 3999
+        assert abs(it_a - it_r) <= max(2, 0.03 * it_r)
+        assert np.abs(x_a - x_r).max() <= 1e-7 * np.abs(x_r).max()
+        r = h.DeviceArray.from_host(b)
+        H.addeval(-1.0, h.DeviceArray.from_host(x_a), r)
+        assert np.linalg.norm(r.download()) <= 2e-9 * np.linalg.norm(b)
