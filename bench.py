@@ -95,6 +95,8 @@ def compact(d):
                          "frac": r4(mv["stored"]["frac"]),
                          "traffic": mv.get("traffic"),
                          "adjoint_ms": r4(mv["adjoint_ms"]),
+                         "adjoint_copies_ms": r4(
+                             mv["adjoint_ms_transposed_copies"]),
                          "computed_cpl_ms": r4(
                              (mv["computed_couplings"] or {}).get("ms")),
                          "computed_cpl_frac": r4(
@@ -723,6 +725,20 @@ def bench_dirichlet(h, mesh, args, stream):
     e1.record(stream)
     stream.sync()
     trans_ms = e0.elapsed_ms(e1) / 50
+    # the same with the conjugate-transposed copies of the blocks kept (the
+    # adjoint then runs the forward kernels)
+    prob.V.set_adjoint_storage(True)
+    for _ in range(3):
+        prob.V.addeval(1.0, d_x, d_y, stream, trans=True)
+    e0, e1 = h.Event(), h.Event()
+    stream.sync()
+    e0.record(stream)
+    for _ in range(50):
+        prob.V.addeval(1.0, d_x, d_y, stream, trans=True)
+    e1.record(stream)
+    stream.sync()
+    trans_copies_ms = e0.elapsed_ms(e1) / 50
+    prob.V.set_adjoint_storage(False)
     # the same product with the couplings computed from the interpolation
     # points instead of read
     computed = None
@@ -776,6 +792,32 @@ def bench_dirichlet(h, mesh, args, stream):
                                          symm=True), n)
     symm["device_bytes"] = Vs.storage()["device"]
     symm["device_bytes_full"] = prob.V.storage()["device"]
+
+    # eight vectors per product (the serving path), stored couplings: the
+    # coupling blocks on the vector units and on FP64 MFMA
+    rng8 = np.random.default_rng(8)
+    xs8 = [h.DeviceArray.from_host(rng8.uniform(-1.0, 1.0, n))
+           for _ in range(8)]
+    ys8 = [h.DeviceArray(n) for _ in range(8)]
+    for y in ys8:
+        y.zero()
+    batch8 = {}
+    prob.V.set_coupling_mode(0)
+    for name, on in (("vector", False), ("mfma", True)):
+        prob.V.set_mfma(on)
+        for _ in range(3):
+            prob.V.addeval_batch([1.0] * 8, xs8, ys8, stream)
+        e0, e1 = h.Event(), h.Event()
+        stream.sync()
+        e0.record(stream)
+        for _ in range(30):
+            prob.V.addeval_batch([1.0] * 8, xs8, ys8, stream)
+        e1.record(stream)
+        stream.sync()
+        batch8[name + "_ms"] = e0.elapsed_ms(e1) / 30
+    prob.V.set_mfma(False)
+    prob.V.set_coupling_mode(1 if modes["V"] == "computed" else 0)
+    del xs8, ys8
 
     ref = None
     gpath = os.path.join(ROOT, "tests", "golden", "dirichlet_sphere_s64.npz")
@@ -851,7 +893,10 @@ def bench_dirichlet(h, mesh, args, stream):
                                enumerate(("counters", "level_sync", "flat"))
                                if i in mv_by},
             "stored": stored, "computed_couplings": computed,
-            "adjoint_ms": trans_ms, "symmetric": symm},
+            "adjoint_ms": trans_ms,
+            "adjoint_ms_transposed_copies": trans_copies_ms,
+            "eight_vectors_stored_couplings": batch8,
+            "symmetric": symm},
     }
     if ref is not None:
         out["rel_err_reference_eps8"] = float(
@@ -1111,17 +1156,30 @@ def main():
     log("bench: near field %.3e entries/s, %.3f ms per step, %.1f TFLOP/s" %
         (value, ms_per_step, achieved))
 
-    # HBM bytes per launch pair from the committed counter summary
-    # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a plain copy); valid for
+    # HBM bytes of one step (both operators' launches of the schedule in
+    # use) from the committed counter summary (FETCH_SIZE and WRITE_SIZE per
+    # dispatch, each scaled by what it reads for a plain copy); valid for
     # the default workload only
     traffic, counters = None, {}
     tpath = os.path.join(ROOT, "profiles",
-                         "traffic_nearfield_ll_sphere32768.json")
+                         "traffic_nearfield_h2_sphere32768.json")
     if os.path.exists(tpath) and (args.split, args.clf, args.eta) == \
             (64, 64, 2.0):
         with open(tpath) as f:
             counters = json.load(f)
-        traffic = counters.get("bytes_per_launch")
+        kb = counters.get("kernel_bytes", {})
+        names = []
+        for k in (0, 1):
+            if schedule == "per_class":
+                names.append("nf_regular_kernel<%d, 1, 1, 2, 0>" % k)
+                names += ["nf_singular_kernel<%d, 1, 1, %d, 4, false>" %
+                          (k, c) for c in (1, 2, 3)]
+            elif schedule == "fused":
+                names.append("nf_fused_kernel<%d, 1, 1, 2, 0>" % k)
+            else:
+                names.append("nf_fused_persistent<%d, 1, 1, 2, 0>" % k)
+        if all(n in kb for n in names):
+            traffic = sum(kb[n] for n in names)
 
     detail = {}
     if "mixed" in sections:
@@ -1188,6 +1246,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
+            "per": "step (SLP and DLP launches)",
             "kernel": ("nf_fused" if schedule.startswith("fused") else
                        "nf_regular+nf_singular"),
             "achieved": achieved,

@@ -689,6 +689,65 @@ __global__ __launch_bounds__(NT) void couple_near(CNArgs a) {
       xs_all + (threadIdx.x >> 6) * XS);
 }
 
+// Coupling blocks of a real matrix with all ranks 64 for NV (2..8) vectors
+// on FP64 matrix cores: yt_r += S xt_c as 64 v_mfma_f64_16x16x4_f64 per
+// block (four 16-row tiles of S times sixteen 4-row steps of the 64 x NV
+// inputs, the 16 columns of an MFMA padded beyond NV with zeros).  One wave
+// per far block; A operands straight from S (lane l: row l % 16, column
+// l / 16 of the 16 x 4 tile), B operands from the wave's LDS copy of the
+// inputs (lane l: input row l / 16, vector l % 16), results in lane l
+// (register r: row 4 r + l / 16 of the tile, vector l % 16) added to the
+// output coefficients.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int NV>
+__global__ __launch_bounds__(NT) void couple_mfma(CNArgs a) {
+  __shared__ double xs_all[WPB][64][16];
+  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= a.nfar) return;  // uniform over the wave
+  double(*xs)[16] = xs_all[threadIdx.x >> 6];
+  const FarD b = a.far[w];
+  const NodeD r = a.rn[b.r], c = a.cn[b.c];
+  // inputs: xs[k][v] = xt_v[c.coef + k], zero for v >= nv
+  for (uint32_t e = lane; e < 64 * 16; e += 64) {
+    const uint32_t k = e >> 4, v = e & 15;
+    xs[k][v] = (v < a.v.nv && v < (uint32_t)NV)
+                   ? a.xt[v * a.cstride + c.coef + k]
+                   : 0.0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const double *S = a.S + b.soff;
+  const int li = lane & 15, lk = lane >> 4;
+  dbl4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+  for (int k0 = 0; k0 < 64; k0 += 8) {
+    double av[2][4], bv[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int kk = k0 + 4 * h + lk;
+      bv[h] = xs[kk][li];
+#pragma unroll
+      for (int t = 0; t < 4; t++) av[h][t] = S[(16 * t + li) + 64 * kk];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h][t], bv[h],
+                                                       acc[t], 0, 0, 0);
+  }
+  if (li >= NV || li >= (int)a.v.nv) return;
+  double *yt = a.yt + li * a.cstride + r.coef;
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      atomicAdd(&yt[16 * t + 4 * q + lk], acc[t][q]);
+}
+
 // Adjoint blocks (yt_c += S^* xt_r, y|_c += alpha N^* x|_r) with one
 // workgroup per block for blocks of at most 64 x 64: the block is staged in
 // LDS with contiguous reads (a padded leading dimension keeps the column
@@ -1630,6 +1689,17 @@ struct h2hip_h2matrix {
   mutable uint32_t ord_nv = 0;
   // the adjoint product's flat blocks, built on first use
   mutable std::mutex adj_lock;
+  // transposed (conjugated) copies of the couplings and near blocks for the
+  // adjoint product (h2hip_h2matrix_set_adjoint_storage): made by the first
+  // adjoint product after the blocks were set, on its stream
+  int adj_keep = 0;
+  mutable bool adjT_ok = false;
+  // products of several vectors multiply the stored couplings on FP64
+  // matrix cores (real matrices of rank 64)
+  int cpl_mfma = 0;
+  mutable h2hip::DevBuf<FarD> d_farT;
+  mutable h2hip::DevBuf<NearD> d_nearT;
+  mutable h2hip::DevBuf<double> d_ST, d_NT;
 };
 
 using h2hip::set_error;
@@ -1835,26 +1905,34 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
   // fixed-order sums (not for the self-adjoint product)
   const bool ord = h2hip::reproducible() && !symm && H->d_P.p &&
                    H->ord_nv >= vs.nv;
-  // adjoint blocks of at most 64 x 64 go through LDS (couple_trans_wg)
-  const bool twg = trans && !symm && H->maxrank <= 64 && H->maxleaf <= 64;
+  // the adjoint from the transposed copies of the blocks when they are
+  // kept: the forward kernels on S^* and N^* (the couplings computed in the
+  // product still take the adjoint path, from the interpolation points)
+  const bool tv = trans && !symm && H->adjT_ok;
+  // other adjoint blocks of at most 64 x 64 go through LDS
+  // (couple_trans_wg)
+  const bool twg = trans && !symm && !tv && H->maxrank <= 64 &&
+                   H->maxleaf <= 64;
   // flat, not symmetric: the near blocks run in the upward launch (not for
   // the workgroup adjoint blocks)
   const bool fuse = flat && !symm && !twg && H->nnear > 0 &&
                     !in.items.empty();
   const int cm = C ? 2 : 1;
   const bool mf = H->cpl_recompute && !symm;
-  auto coupling_args = [&]() {
+  // view: the transposed blocks, whose rows are the column side's
+  auto coupling_args = [&](bool view = false) {
     CNArgs c;
-    c.far = H->d_far.p;
+    const int a = view ? 1 : 0;
+    c.far = view ? H->d_farT.p : H->d_far.p;
     c.nfar = H->nfar;
-    c.near = H->d_near.p;
+    c.near = view ? H->d_nearT.p : H->d_near.p;
     c.nnear = H->nnear;
-    c.rn = H->side[0].d_nodes.p;
-    c.cn = H->side[1].d_nodes.p;
-    c.ridx = H->side[0].d_idx.p;
-    c.cidx = H->side[1].d_idx.p;
-    c.S = H->d_S.p;
-    c.N = H->d_N.p;
+    c.rn = H->side[a].d_nodes.p;
+    c.cn = H->side[1 - a].d_nodes.p;
+    c.ridx = H->side[a].d_idx.p;
+    c.cidx = H->side[1 - a].d_idx.p;
+    c.S = view ? H->d_ST.p : H->d_S.p;
+    c.N = view ? H->d_NT.p : H->d_N.p;
     c.xt = xt;
     c.yt = yt;
     c.cstride = H->cstride;
@@ -1891,14 +1969,17 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
                  std::max(H->side[0].items.size(), H->side[1].items.size());
     const uint32_t fb = (fa.nitems + WPB - 1) / WPB;
     const dim3 gf(fb + (fuse ? (H->nnear + WPB - 1) / WPB : 0));
-    CNArgs cn = coupling_args();
+    CNArgs cn = coupling_args(tv);
     cn.xt = xt;
     cn.yt = yt;
     cn.nfar = 0;
+    // the near blocks of the adjoint: transposed copies (forward kernel) or
+    // the adjoint kernel
+    const bool ntrans = trans && !tv;
     // no items: no cluster of the input side is in an admissible block,
     // and no coefficient is needed
     if (ord) {
-      if (fuse && trans)
+      if (fuse && ntrans)
         hipLaunchKernelGGL((fwd_flat_near<C, K, NV, true, true>), gf,
                            dim3(NT), 0, st, fa, cn, fb);
       else if (fuse)
@@ -1910,7 +1991,7 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
       hipLaunchKernelGGL((ord_fwd_reduce<C, NV>), dim3(in.nnodes), dim3(NT),
                          0, st, fa, (const uint32_t *)in.d_it_ptr.p,
                          (const uint32_t *)in.d_it_ids.p, in.nnodes);
-    } else if (fuse && trans) {
+    } else if (fuse && ntrans) {
       hipLaunchKernelGGL((fwd_flat_near<C, K, NV, false, true>), gf,
                          dim3(NT), 0, st, fa, cn, fb);
     } else if (fuse) {
@@ -1947,8 +2028,25 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     }
   }
 
-  CNArgs c = coupling_args();
-  if (fuse) c.nnear = 0;  // done in the upward launch
+  // computed couplings keep the adjoint kernel for the far blocks; their
+  // near blocks then come from the transposed copies in a launch of their
+  // own (unless the upward launch did them)
+  CNArgs c = coupling_args(tv && !mf);
+  const bool ktrans = trans && !(tv && !mf);
+  if (fuse) {
+    c.nnear = 0;  // done in the upward launch
+  } else if (tv && mf && H->nnear) {
+    CNArgs cnr = coupling_args(true);
+    cnr.nfar = 0;
+    const dim3 gn((cnr.nnear + WPB - 1) / WPB);
+    if (ord)
+      hipLaunchKernelGGL((couple_near<C, false, NV, false, true>), gn,
+                         dim3(NT), 0, st, cnr);
+    else
+      hipLaunchKernelGGL((couple_near<C, false, NV>), gn, dim3(NT), 0, st,
+                         cnr);
+    c.nnear = 0;
+  }
   if (twg) {
     // stored couplings and near blocks: one workgroup each; computed
     // couplings: the wave path for the far blocks, workgroups for the near
@@ -1966,20 +2064,30 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
     c.nnear = 0;
     if (!mf) c.nfar = 0;
   }
+  // several vectors, stored real couplings of rank 64: the far blocks on
+  // matrix cores (h2hip_h2matrix_set_mfma)
+  if constexpr (!C && NV > 1) {
+    if (H->cpl_mfma && H->urank == 64 && !mf && !ord && !symm && !ktrans &&
+        c.nfar) {
+      hipLaunchKernelGGL((couple_mfma<NV>), dim3((c.nfar + WPB - 1) / WPB),
+                         dim3(NT), 0, st, c);
+      c.nfar = 0;  // the near blocks (if any are left) as below
+    }
+  }
   const uint32_t nw = c.nfar + c.nnear;
   const dim3 gc((nw + WPB - 1) / WPB);
   if (nw && symm) {
     if constexpr (NV == 1)
       hipLaunchKernelGGL((couple_symm<C>), gc, dim3(NT), 0, st, c);
   } else if (nw) {
-    if (trans && mf) {
+    if (ktrans && mf) {
       if (ord)
         hipLaunchKernelGGL((couple_near<C, true, NV, true, true>), gc,
                            dim3(NT), 0, st, c);
       else
         hipLaunchKernelGGL((couple_near<C, true, NV, true>), gc, dim3(NT), 0,
                            st, c);
-    } else if (trans) {
+    } else if (ktrans) {
       if (ord)
         hipLaunchKernelGGL((couple_near<C, true, NV, false, true>), gc,
                            dim3(NT), 0, st, c);
@@ -2077,15 +2185,61 @@ int apply_symm(const double *alpha, const h2hip_h2matrix *H, const double *x,
   return H2HIP_OK;
 }
 
+// One block of a transpose: rows x cols column-major at element offset off
+// of the source, cols x rows column-major at the same offset of the
+// destination.
+struct TJob {
+  uint64_t off;
+  uint32_t rows, cols;
+};
+
+// dst = src^* block by block, one workgroup per block, through 32 x 32 LDS
+// tiles (reads and writes along columns)
+template <int C>
+__global__ __launch_bounds__(NT) void block_transpose(const TJob *jobs,
+                                                      const double *src,
+                                                      double *dst) {
+  __shared__ double tr[32][33];
+  __shared__ double ti[C ? 32 : 1][33];
+  const TJob J = jobs[blockIdx.x];
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (uint32_t r0 = 0; r0 < J.rows; r0 += 32)
+    for (uint32_t c0 = 0; c0 < J.cols; c0 += 32) {
+      for (uint32_t yy = ty; yy < 32; yy += NT / 32) {
+        const uint32_t r = r0 + tx, c = c0 + yy;
+        if (r < J.rows && c < J.cols) {
+          double re, im;
+          ld<C>(src, J.off + r + (uint64_t)c * J.rows, re, im);
+          tr[yy][tx] = re;
+          if constexpr (C) ti[yy][tx] = im;
+        }
+      }
+      __syncthreads();
+      for (uint32_t yy = ty; yy < 32; yy += NT / 32) {
+        const uint32_t c = c0 + tx, r = r0 + yy;
+        if (r < J.rows && c < J.cols) {
+          if constexpr (C)
+            st<C>(dst, J.off + c + (uint64_t)r * J.cols, tr[tx][yy],
+                  -ti[tx][yy]);
+          else
+            st<C>(dst, J.off + c + (uint64_t)r * J.cols, tr[tx][yy], 0.0);
+        }
+      }
+      __syncthreads();
+    }
+}
+
 // Buffers a product needs beyond the matrix: the adjoint's flat blocks and
-// the partial sums of reproducible products (allocated on first use).
-int ensure_product(const h2hip_h2matrix *H, uint32_t nv, bool trans);
+// transposed blocks, the partial sums of reproducible products (allocated
+// on first use; the transposes ordered on st).
+int ensure_product(const h2hip_h2matrix *H, uint32_t nv, bool trans,
+                   hipStream_t st);
 
 template <int C>
 int apply(const double *alpha, const h2hip_h2matrix *H, const double *x,
           double *y, hipStream_t st, bool trans) {
   {
-    const int rc = ensure_product(H, 1, trans);
+    const int rc = ensure_product(H, 1, trans, st);
     if (rc != H2HIP_OK) return rc;
   }
   VecSet vs = {};
@@ -2136,7 +2290,7 @@ int apply_batch(uint32_t nvec, const double *alpha, const h2hip_h2matrix *H,
     }
   }
   {
-    const int rc = ensure_product(H, nvec, trans);
+    const int rc = ensure_product(H, nvec, trans, st);
     if (rc != H2HIP_OK) return rc;
   }
   VecSet vs = {};
@@ -2255,8 +2409,59 @@ bool same_bases(const h2hip_h2matrix *H) {
   return same;
 }
 
-int ensure_product(const h2hip_h2matrix *Hc, uint32_t nv, bool trans) {
+int ensure_product(const h2hip_h2matrix *Hc, uint32_t nv, bool trans,
+                   hipStream_t st) {
   h2hip_h2matrix *H = const_cast<h2hip_h2matrix *>(Hc);
+  if (trans && H->adj_keep) {
+    std::lock_guard<std::mutex> guard(H->adj_lock);
+    if (!H->adjT_ok) {
+      // far block (r, c) -> (c, r) and near block (roff, rows, coff, cols)
+      // -> (coff, cols, roff, rows), each at its own offset
+      std::vector<FarD> farT(H->far.size());
+      std::vector<TJob> js(H->far.size()), jn(H->near.size());
+      for (size_t w = 0; w < H->far.size(); w++) {
+        const FarD &b = H->far[w];
+        farT[w] = FarD{b.c, b.r, b.soff};
+        js[w] = TJob{b.soff, H->side[0].nodes[b.r].rank,
+                     H->side[1].nodes[b.c].rank};
+      }
+      std::vector<NearD> nearT(H->near.size());
+      for (size_t w = 0; w < H->near.size(); w++) {
+        const NearD &b = H->near[w];
+        nearT[w] = NearD{b.coff, b.cols, b.roff, b.rows, b.noff};
+        jn[w] = TJob{b.noff, b.rows, b.cols};
+      }
+      h2hip::DevBuf<TJob> d_js, d_jn;
+      const bool ok =
+          H->d_farT.upload(farT.data(), farT.size()) == hipSuccess &&
+          H->d_nearT.upload(nearT.data(), nearT.size()) == hipSuccess &&
+          d_js.upload(js.data(), js.size()) == hipSuccess &&
+          d_jn.upload(jn.data(), jn.size()) == hipSuccess &&
+          (H->d_ST.n == H->d_S.n || H->d_ST.alloc(H->d_S.n) == hipSuccess) &&
+          (H->d_NT.n == H->d_N.n || H->d_NT.alloc(H->d_N.n) == hipSuccess);
+      if (!ok) {
+        H->d_ST.release();
+        H->d_NT.release();
+        set_error("h2matrix: transposed blocks of the adjoint: device "
+                  "allocation failed");
+        return H2HIP_ENOMEM;
+      }
+      if (!js.empty())
+        hipLaunchKernelGGL(H->cplx ? block_transpose<1> : block_transpose<0>,
+                           dim3((uint32_t)js.size()), dim3(NT), 0, st,
+                           (const TJob *)d_js.p, (const double *)H->d_S.p,
+                           H->d_ST.p);
+      if (!jn.empty())
+        hipLaunchKernelGGL(H->cplx ? block_transpose<1> : block_transpose<0>,
+                           dim3((uint32_t)jn.size()), dim3(NT), 0, st,
+                           (const TJob *)d_jn.p, (const double *)H->d_N.p,
+                           H->d_NT.p);
+      // the job lists are freed on return: wait for the transposes
+      H2HIP_CHECK(hipStreamSynchronize(st));
+      H2HIP_CHECK(hipGetLastError());
+      H->adjT_ok = true;
+    }
+  }
   if (trans && H->level_sync == 2 && H->d_xtf.p) {
     // the adjoint's flat blocks: row-major of the row side, column-major
     // of the column side
@@ -2520,6 +2725,7 @@ int h2hip_h2matrix_set_basis(h2hip_h2matrix *H, int side, const double *V,
 int h2hip_h2matrix_set_couplings(h2hip_h2matrix *H, const void *S,
                                  uint64_t sld) {
   if (!H || (!S && H->nfar)) return H2HIP_EINVAL;
+  H->adjT_ok = false;
   const int cm = H->cplx ? 2 : 1;
   std::vector<double> s(cm * H->slen);
   const double *src = (const double *)S;
@@ -2644,6 +2850,7 @@ int h2hip_h2matrix_set_coupling_mode(h2hip_h2matrix *H, int recompute) {
 
 int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N) {
   if (!H || (!N && H->nlen)) return H2HIP_EINVAL;
+  H->adjT_ok = false;
   const int cm = H->cplx ? 2 : 1;
   H2HIP_CHECK(hipMemcpy(H->d_N.p, N, sizeof(double) * cm * H->nlen,
                         hipMemcpyHostToDevice));
@@ -2700,6 +2907,7 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
                                     const double *k, uint32_t m,
                                     const double *rpts, const double *cpts,
                                     void *stream) {
+  if (H) H->adjT_ok = false;
   if (!H || m < 1 || m > 8 || (!rpts && H->nfar) || (!cpts && H->nfar) ||
       (kernel != H2HIP_LAPLACE && kernel != H2HIP_HELMHOLTZ) ||
       (kernel == H2HIP_HELMHOLTZ && (!k || !H->cplx)) ||
@@ -2740,6 +2948,7 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
 }
 
 void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H) {
+  if (H) H->adjT_ok = false;  // the caller writes the near field
   return H ? (void *)H->d_N.p : nullptr;
 }
 
@@ -2816,7 +3025,41 @@ int h2hip_h2matrix_storage(const h2hip_h2matrix *H, uint64_t *out4) {
   add(H->d_cpts);
   add(H->d_xtb);
   add(H->d_ytb);
+  add(H->d_farT);
+  add(H->d_nearT);
+  add(H->d_ST);
+  add(H->d_NT);
+  for (int s = 0; s < 2; s++) {
+    add(H->side[s].d_WFt);
+    add(H->side[s].d_WBt);
+  }
   out4[3] = r;
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_set_mfma(h2hip_h2matrix *H, int on) {
+  if (!H || on < 0 || on > 1) {
+    set_error("h2hip_h2matrix_set_mfma: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  H->cpl_mfma = on;
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_set_adjoint_storage(h2hip_h2matrix *H, int keep) {
+  if (!H || keep < 0 || keep > 1) {
+    set_error("h2hip_h2matrix_set_adjoint_storage: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  std::lock_guard<std::mutex> guard(H->adj_lock);
+  H->adj_keep = keep;
+  H->adjT_ok = false;  // made again by the next adjoint product
+  if (!keep) {
+    H->d_farT.release();
+    H->d_nearT.release();
+    H->d_ST.release();
+    H->d_NT.release();
+  }
   return H2HIP_OK;
 }
 

@@ -11,6 +11,7 @@
 #include <cmath>
 
 #include "h2hip_internal.h"
+#include "../host/gauss01.h"
 
 namespace h2hip {
 
@@ -18,34 +19,7 @@ void gauss_legendre01(unsigned m, std::vector<double> &x,
                       std::vector<double> &w) {
   x.assign(m, 0.0);
   w.assign(m, 0.0);
-  for (unsigned i = 0; i < m; i++) {
-    // i-th node from the left on [-1, 1]
-    double z = -std::cos(M_PI * (4.0 * i + 3.0) / (4.0 * m + 2.0));
-    double dpm = 1.0;
-    for (int iter = 0; iter < 64; iter++) {
-      double pm1 = 0.0, pm = 1.0;  // P_{n-1}, P_n
-      for (unsigned n = 0; n < m; n++) {
-        double pn1 = ((2.0 * n + 1.0) * z * pm - n * pm1) / (n + 1.0);
-        pm1 = pm;
-        pm = pn1;
-      }
-      dpm = m * (pm1 - z * pm) / (1.0 - z * z);
-      double step = pm / dpm;
-      z -= step;
-      if (std::fabs(step) <= 1e-16 * std::fabs(z) + 1e-300) break;
-    }
-    double pm1 = 0.0, pm = 1.0;
-    for (unsigned n = 0; n < m; n++) {
-      double pn1 = ((2.0 * n + 1.0) * z * pm - n * pm1) / (n + 1.0);
-      pm1 = pm;
-      pm = pn1;
-    }
-    dpm = m * (pm1 - z * pm) / (1.0 - z * z);
-    double wz = 2.0 / ((1.0 - z * z) * dpm * dpm);
-    // map [-1, 1] -> [0, 1]
-    x[i] = 0.5 + 0.5 * z;
-    w[i] = 0.5 * wz;
-  }
+  h2_gauss01(m, x.data(), w.data());
 }
 
 TriRule make_triangle_rule(unsigned q) {

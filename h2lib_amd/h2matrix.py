@@ -67,6 +67,10 @@ def _sigs():
     h.h2hip_h2matrix_set_schedule.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_set_coupling_mode.restype = C.c_int
     h.h2hip_h2matrix_set_coupling_mode.argtypes = [vp, C.c_int]
+    h.h2hip_h2matrix_set_adjoint_storage.restype = C.c_int
+    h.h2hip_h2matrix_set_adjoint_storage.argtypes = [vp, C.c_int]
+    h.h2hip_h2matrix_set_mfma.restype = C.c_int
+    h.h2hip_h2matrix_set_mfma.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_interp_bases.restype = C.c_int
     h.h2hip_h2matrix_interp_bases.argtypes = [vp, C.c_int, vp, C.c_int,
                                               C.c_int, C.c_uint32, C.c_uint32,
@@ -343,6 +347,22 @@ class H2Matrix:
                                                      int(bool(recompute))),
               "h2hip_h2matrix_set_coupling_mode")
         self.recompute = int(bool(recompute))
+
+    def set_adjoint_storage(self, keep):
+        """Keep conjugate-transposed copies of the couplings and near blocks
+        for the adjoint product (made by its first call after the blocks
+        were set), which then runs the forward kernels."""
+        check(hip().h2hip_h2matrix_set_adjoint_storage(self._h,
+                                                       int(bool(keep))),
+              "h2hip_h2matrix_set_adjoint_storage")
+        self.adjoint_storage = int(bool(keep))
+
+    def set_mfma(self, on):
+        """Products of several vectors of a real matrix of rank 64 multiply
+        the stored couplings on FP64 matrix cores."""
+        check(hip().h2hip_h2matrix_set_mfma(self._h, int(bool(on))),
+              "h2hip_h2matrix_set_mfma")
+        self.mfma = int(bool(on))
 
     def bytes(self):
         """Bytes one product y += alpha H x reads in the current schedule

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include "h2bem.h"
+#include "gauss01.h"
 
 /* f(x, n): 0 Dirichlet data 1/|x - x0|, 1 Neumann data
  * -<x - x0, n>/|x - x0|^3 */
@@ -31,34 +32,10 @@ field_value(int kind, const double *x0, const double *x, const double *n)
 static unsigned
 tri_rule(unsigned q, double *x0, double *x1, double *w)
 {
-  /* Gauss-Legendre on [0, 1] (Newton on the Legendre recurrence), then the
-   * Duffy-collapsed tensor rule on the reference triangle */
+  /* Gauss-Legendre on [0, 1], then the Duffy-collapsed tensor rule on the
+   * reference triangle */
   double g[16], gw[16];
-  for (unsigned i = 0; i < q; i++) {
-    double z = -cos(M_PI * (4.0 * i + 3.0) / (4.0 * q + 2.0)), dp = 1.0;
-    for (int it = 0; it < 64; it++) {
-      double p0 = 0.0, p1 = 1.0;
-      for (unsigned n = 0; n < q; n++) {
-        double p2 = ((2.0 * n + 1.0) * z * p1 - n * p0) / (n + 1.0);
-        p0 = p1;
-        p1 = p2;
-      }
-      dp = q * (p0 - z * p1) / (1.0 - z * z);
-      double step = p1 / dp;
-      z -= step;
-      if (fabs(step) <= 1e-16 * fabs(z) + 1e-300)
-        break;
-    }
-    double p0 = 0.0, p1 = 1.0;
-    for (unsigned n = 0; n < q; n++) {
-      double p2 = ((2.0 * n + 1.0) * z * p1 - n * p0) / (n + 1.0);
-      p0 = p1;
-      p1 = p2;
-    }
-    dp = q * (p0 - z * p1) / (1.0 - z * z);
-    g[i] = 0.5 + 0.5 * z;
-    gw[i] = 0.5 * (2.0 / ((1.0 - z * z) * dp * dp));
-  }
+  h2_gauss01(q, g, gw);
   unsigned k = 0;
   for (unsigned i = 0; i < q; i++)
     for (unsigned j = 0; j < q; j++, k++) {

@@ -19,40 +19,10 @@
 #include <string.h>
 
 #include "h2bem.h"
+#include "gauss01.h"
 
 #define INTERP_EPS 5.0e-3
 #define KCONST 0.0795774715459476679
-
-/* Gauss-Legendre on [0, 1] by Newton's method on the Legendre recurrence. */
-static void
-gauss01(unsigned m, double *x, double *w)
-{
-  for (unsigned i = 0; i < m; i++) {
-    double z = -cos(M_PI * (4.0 * i + 3.0) / (4.0 * m + 2.0)), dp = 1.0;
-    for (int it = 0; it < 64; it++) {
-      double p0 = 0.0, p1 = 1.0;
-      for (unsigned n = 0; n < m; n++) {
-        double p2 = ((2.0 * n + 1.0) * z * p1 - n * p0) / (n + 1.0);
-        p0 = p1;
-        p1 = p2;
-      }
-      dp = m * (p0 - z * p1) / (1.0 - z * z);
-      double step = p1 / dp;
-      z -= step;
-      if (fabs(step) <= 1e-16 * fabs(z) + 1e-300)
-        break;
-    }
-    double p0 = 0.0, p1 = 1.0;
-    for (unsigned n = 0; n < m; n++) {
-      double p2 = ((2.0 * n + 1.0) * z * p1 - n * p0) / (n + 1.0);
-      p0 = p1;
-      p1 = p2;
-    }
-    dp = m * (p0 - z * p1) / (1.0 - z * z);
-    x[i] = 0.5 + 0.5 * z;
-    w[i] = 0.5 * (2.0 / ((1.0 - z * z) * dp * dp));
-  }
-}
 
 /* Interpolation points of a box: px[0..m), py, pz. */
 static void
@@ -111,7 +81,7 @@ static void
 make_trirule(unsigned q, trirule *r)
 {
   double g[8], gw[8];
-  gauss01(q, g, gw);
+  h2_gauss01(q, g, gw);
   r->nq = 0;
   for (unsigned i = 0; i < q; i++)
     for (unsigned j = 0; j < q; j++) {

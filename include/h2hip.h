@@ -237,6 +237,23 @@ int h2hip_h2matrix_set_couplings(h2hip_h2matrix *H, const void *S,
  * plan over the same blocks. */
 int h2hip_h2matrix_set_nearfield(h2hip_h2matrix *H, const void *N);
 void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H);
+/* Transposed copies for the adjoint product (no counterpart in the
+ * reference; keep = 1): the couplings and near blocks are also kept as
+ * their conjugate transposes (twice the memory of both), made by the first
+ * h2hip_addevaltrans_h2matrix after they were set, on its stream; the
+ * adjoint then runs the kernels of the forward product (couplings computed
+ * in the product still take the adjoint path).  Setting the couplings or
+ * the near field (h2hip_h2matrix_set_couplings, _interp_couplings,
+ * _set_nearfield, _nearfield_device) marks the copies stale; a caller that
+ * writes the near field again through an earlier pointer calls this
+ * function again before the next adjoint product.  keep = 0 frees them. */
+int h2hip_h2matrix_set_adjoint_storage(h2hip_h2matrix *H, int keep);
+/* Products of two to eight vectors (h2hip_addeval_h2matrix_batch) of a real
+ * matrix whose clusters all have rank 64 multiply the stored couplings on
+ * FP64 matrix cores (on = 1; no counterpart in the reference): 64
+ * v_mfma_f64_16x16x4_f64 per coupling block.  Other products, computed
+ * couplings and reproducible mode keep the vector kernels. */
+int h2hip_h2matrix_set_mfma(h2hip_h2matrix *H, int on);
 /* Couplings of an interpolation H2-matrix computed on the device (instead
  * of h2hip_h2matrix_set_couplings): kernel_const G(xi, eta) at the tensor
  * Chebyshev points of both boxes of every admissible block, G = 1/r

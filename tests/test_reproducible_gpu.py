@@ -141,3 +141,68 @@ def test_batched_cg_refuses_combined_operators():
         check(h.hip().h2hip_gmres_batch_combine(
             B._h, c.ctypes.data_as(C.c_void_p), Hs._h,
             c.ctypes.data_as(C.c_void_p)), "h2hip_gmres_batch_combine")
+
+
+@pytest.mark.parametrize("kind", ["lap_slp", "lap_dlp", "helm_slp"])
+@pytest.mark.parametrize("sched", [0, 2])
+def test_adjoint_from_transposed_copies(kind, sched):
+    # the adjoint product from the kept conjugate transposes (forward
+    # kernels) against the adjoint kernels: the same sums in another order,
+    # one and three vectors, stored and computed couplings; the copies follow
+    # a new near field
+    import h2lib_amd as h
+    H, gr, op = matrix(kind)
+    H.set_schedule(sched)
+    xs = vectors(H, 3, 11)
+    alphas = [1.0, -0.75, 0.5]
+    if H.dtype == np.complex128:
+        alphas = [a - 0.25j for a in alphas]
+    for mode in (0, 1):
+        H.set_coupling_mode(mode)
+        H.set_adjoint_storage(False)
+        want = products(H, xs, alphas, True, False)
+        H.set_adjoint_storage(True)
+        for batch in (False, True):
+            got = products(H, xs, alphas, True, batch)
+            for v in range(len(xs)):
+                assert rel(got[v], want[v]) < 1e-13, (mode, batch, v)
+    H.set_coupling_mode(0)
+    # a near field assembled again (mixed precision: entries change by
+    # about 1e-8 relative) reaches the adjoint's copies
+    before = products(H, xs[:1], alphas[:1], True, False)
+    op.set_precision(1)
+    H.assemble_nearfield()
+    got = products(H, xs[:1], alphas[:1], True, False)
+    H.set_adjoint_storage(False)
+    want = products(H, xs[:1], alphas[:1], True, False)
+    assert rel(got[0], want[0]) < 1e-13
+    assert rel(before[0], want[0]) > 1e-11
+    op.set_precision(0)
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_batched_couplings_on_matrix_cores(schedule):
+    # products of 2, 5 and 8 vectors with the couplings multiplied on FP64
+    # MFMA (rank 64) against the vector kernels: the same sums in another
+    # order
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.solve import LaplaceDirichlet
+    h.set_reproducible(False)
+    try:
+        prob = LaplaceDirichlet(G.sphere(16), clf=32, eta=2.0, m=4)
+        H = prob.KM
+        H.set_schedule(schedule)
+        assert len(H.interp.far) > 0
+        for nv in (2, 5, 8):
+            xs = vectors(H, nv, 30 + nv)
+            alphas = [1.0 - 0.25 * v for v in range(nv)]
+            H.set_mfma(False)
+            want = products(H, xs, alphas, False, True)
+            H.set_mfma(True)
+            got = products(H, xs, alphas, False, True)
+            for v in range(nv):
+                assert rel(got[v], want[v]) < 1e-13, (nv, v)
+        H.set_mfma(False)
+    finally:
+        h.set_reproducible(True)
