@@ -11,6 +11,9 @@ block its mesh alone would give (same tiles, same arithmetic), so the
 entries are those of per-mesh plans (bit for bit for the constant basis;
 the linear basis sums element matrices with atomic adds in varying order).
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import geometry as G
@@ -65,6 +68,16 @@ def random_meshes(count, seed, tmin=512, tmax=32768):
     return out
 
 
+def _host_threads():
+    """Host threads for per-mesh work: the CPUs this process may use, at
+    most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 class MultiMesh:
     """The near-field blocks of many meshes as one plan on one joined
     surface.  basis 0 (constant) or 1 (linear) on both sides; clfs: leaf
@@ -79,9 +92,17 @@ class MultiMesh:
         self.trees, self.btrees, self.dof_off, self.block_range = \
             [], [], [], []
         voff = doff = nblk = 0
-        for mesh, clf in zip(self.meshes, self.clfs):
-            tree = G.ClusterTree(mesh, basis, clf)
+
+        # cluster and block trees of the meshes on host threads (the host
+        # library's calls release the interpreter lock); joined in mesh order
+        def trees(i):
+            tree = G.ClusterTree(self.meshes[i], basis, self.clfs[i])
             bt = G.BlockTree(tree, tree, eta, 0)
+            return tree, bt
+
+        with ThreadPoolExecutor(_host_threads()) as ex:
+            built = list(ex.map(trees, range(len(self.meshes))))
+        for mesh, (tree, bt) in zip(self.meshes, built):
             nb = np.array(bt.near_block_list(), dtype=np.int64).reshape(-1, 4)
             xs.append(mesh.x)
             ts.append(mesh.t.astype(np.int64) + voff)
@@ -127,10 +148,19 @@ class MultiMesh:
         from ._lib import H2HipError
         es = 8
         mats = []
+
+        # the interpolation bases and couplings on a few host threads (each
+        # fill runs OpenMP loops of its own), the device matrices in mesh
+        # order
+        def interp(i):
+            return Interpolation(self.meshes[i], self.trees[i],
+                                 self.trees[i], self.btrees[i], self.basis,
+                                 self.basis, dlp, m, 2)
+
+        with ThreadPoolExecutor(max(1, _host_threads() // 4)) as ex:
+            interps = list(ex.map(interp, range(len(self.meshes))))
         for i, mesh in enumerate(self.meshes):
-            tree, bt = self.trees[i], self.btrees[i]
-            I = Interpolation(mesh, tree, tree, bt, self.basis, self.basis,
-                              dlp, m, 2)
+            tree, I = self.trees[i], interps[i]
             H = H2Matrix.from_arrays(I, None, rt=tree)
             a, b = self.entry_range[i]
             H.copy_nearfield_from_device(d_out.ptr + es * a, b - a)

@@ -413,7 +413,9 @@ __device__ __forceinline__ void cpl_value(double d0, double d1, double d2,
                                           double kdamp, double &sr,
                                           double &si) {
   constexpr double KCONST = 0.0795774715459476679;
-  const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  // (the order of the products of r2 is fixed: the unrolled product path
+  // forms the same sums with the first two hoisted)
+  const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
   double rs = cpl_rsqrt(r2);
   if (!helmholtz) {
     sr = KCONST * rs;
@@ -463,7 +465,22 @@ struct CNArgs {
 // a row: M points at the row's first entry, entry j lies stride entries
 // further (complex entries as pairs); 16 (complex: 8) entries are loaded
 // before their products, so that each wave keeps enough loads in flight.
+// NTL: the entries are loaded as streaming (non-temporal) data, which the
+// caches need not keep (the couplings: read once per product and more
+// bytes than the last-level cache holds)
 template <int C>
+__device__ __forceinline__ void ld_nt(const double *p, uint64_t i, double &re,
+                                      double &im) {
+  if constexpr (C) {
+    re = __builtin_nontemporal_load(p + 2 * i);
+    im = __builtin_nontemporal_load(p + 2 * i + 1);
+  } else {
+    re = __builtin_nontemporal_load(p + i);
+    im = 0.0;
+  }
+}
+
+template <int C, bool NTL = false>
 __device__ __forceinline__ void dot_row(const double *__restrict__ M,
                                         uint64_t stride, uint32_t n,
                                         const double *xs, double &ar,
@@ -473,7 +490,12 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
   for (; j + B <= n; j += B) {
     double mr[B], mi[B];
 #pragma unroll
-    for (int u = 0; u < B; u++) ld<C>(M, (uint64_t)(j + u) * stride, mr[u], mi[u]);
+    for (int u = 0; u < B; u++) {
+      if constexpr (NTL)
+        ld_nt<C>(M, (uint64_t)(j + u) * stride, mr[u], mi[u]);
+      else
+        ld<C>(M, (uint64_t)(j + u) * stride, mr[u], mi[u]);
+    }
 #pragma unroll
     for (int u = 0; u < B; u++) {
       const double xr = xs[2 * (j + u)], xi = xs[2 * (j + u) + 1];
@@ -492,6 +514,49 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
     if constexpr (C) {
       ar = fma(-mi, xi, ar);
       ai = fma(mr, xi, fma(mi, xr, ai));
+    }
+  }
+}
+
+// Laplace couplings computed in the product at interpolation order 4, one
+// output point per lane against the 64 input points, unrolled: the input
+// points in registers, the x and y differences and their squares formed
+// once per line of points, the same arithmetic as cpl_value (so the values
+// are those of the stored couplings) and the same order of the sums as the
+// general loop.  The output point is p (3 coordinates), the inputs q
+// (x, y, z lines of 4); sg = +1 forms p - q, -1 forms q - p.
+template <int C, int NV, int XR>
+__device__ __forceinline__ void mf_laplace4(const double *p3,
+                                            const double *q, double sg,
+                                            const double *xs, double (&ar)[NV],
+                                            double (&ai)[NV]) {
+  constexpr double KCONST = 0.0795774715459476679;
+  double qx[4], qy[4], qz[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    qx[t] = q[t];
+    qy[t] = q[4 + t];
+    qz[t] = q[8 + t];
+  }
+#pragma unroll
+  for (int ja = 0; ja < 4; ja++) {
+    const double d0 = sg * (p3[0] - qx[ja]);
+    const double s0 = d0 * d0;
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++) {
+      const double d1 = sg * (p3[1] - qy[jb]);
+      const double s1 = fma(d1, d1, s0);
+#pragma unroll
+      for (int jc = 0; jc < 4; jc++) {
+        const int j = 16 * ja + 4 * jb + jc;
+        const double d2 = sg * (p3[2] - qz[jc]);
+        const double sr = KCONST * cpl_rsqrt(fma(d2, d2, s1));
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          ar[v] = fma(sr, xs[v * 2 * XR + 2 * j], ar[v]);
+          if constexpr (C) ai[v] = fma(sr, xs[v * 2 * XR + 2 * j + 1], ai[v]);
+        }
+      }
     }
   }
 }
@@ -530,6 +595,8 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
       double ar[NV], ai[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
+      bool m4 = false;  // done by the unrolled order-4 Laplace path
+      (void)m4;
       if constexpr (MF && TRANS) {
         // out = column point i, inputs = row points j; the entry is the
         // coupling S(j, i) (the same differences row - column as the stored
@@ -540,7 +607,12 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
         const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
         const double qx = pc[ia], qy = pc[m + ib], qz = pc[2 * m + ic];
         uint32_t j = 0;
-        for (uint32_t ja = 0; ja < m; ja++) {
+        if (m == 4 && !a.cs.helmholtz) {  // uniform over the grid
+          const double p3[3] = {qx, qy, qz};
+          mf_laplace4<C, NV, XR>(p3, pr, -1.0, xs, ar, ai);
+          m4 = true;
+        }
+        for (uint32_t ja = 0; ja < (m4 ? 0u : m); ja++) {
           const double d0 = pr[ja] - qx;
           for (uint32_t jb = 0; jb < m; jb++) {
             const double d1 = pr[m + jb] - qy;
@@ -569,7 +641,12 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
         const uint32_t ia = i / (m * m), ib = (i / m) % m, ic = i % m;
         const double px = pr[ia], py = pr[m + ib], pz = pr[2 * m + ic];
         uint32_t j = 0;
-        for (uint32_t ja = 0; ja < m; ja++) {
+        if (m == 4 && !a.cs.helmholtz) {  // uniform over the grid
+          const double p3[3] = {px, py, pz};
+          mf_laplace4<C, NV, XR>(p3, pc, 1.0, xs, ar, ai);
+          m4 = true;
+        }
+        for (uint32_t ja = 0; ja < (m4 ? 0u : m); ja++) {
           const double d0 = px - pc[ja];
           for (uint32_t jb = 0; jb < m; jb++) {
             const double d1 = py - pc[m + jb];
@@ -591,15 +668,15 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
           }
         }
       } else if constexpr (NV == 1 && !TRANS) {
-        dot_row<C>(a.S + (C ? 2 : 1) * (b.soff + i), r.rank, in.rank, xs,
-                   ar[0], ai[0]);
+        dot_row<C, true>(a.S + (C ? 2 : 1) * (b.soff + i), r.rank, in.rank,
+                         xs, ar[0], ai[0]);
       } else {
 #pragma unroll UNR
         for (uint32_t j = 0; j < in.rank; j++) {
           const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
                                    : (uint64_t)i + (uint64_t)j * r.rank;
           double sr, si;
-          ld<C>(a.S, b.soff + e, sr, si);
+          ld_nt<C>(a.S, b.soff + e, sr, si);
           if (TRANS) si = -si;
 #pragma unroll
           for (int v = 0; v < NV; v++) {
@@ -730,7 +807,8 @@ __global__ __launch_bounds__(NT) void couple_mfma(CNArgs a) {
       const int kk = k0 + 4 * h + lk;
       bv[h] = xs[kk][li];
 #pragma unroll
-      for (int t = 0; t < 4; t++) av[h][t] = S[(16 * t + li) + 64 * kk];
+      for (int t = 0; t < 4; t++)
+        av[h][t] = __builtin_nontemporal_load(S + (16 * t + li) + 64 * kk);
     }
 #pragma unroll
     for (int h = 0; h < 2; h++)
@@ -785,7 +863,10 @@ __global__ __launch_bounds__(NT) void couple_trans_wg(CNArgs a) {
   for (uint32_t e = tid; e < nin * nout; e += NT) {
     const uint32_t i = e % nin, j = e / nin;
     double mr, mi;
-    ld<C>(M, e, mr, mi);
+    if (isfar)
+      ld_nt<C>(M, e, mr, mi);
+    else
+      ld<C>(M, e, mr, mi);
     Mr[i + j * LDM] = mr;
     if constexpr (C) Mi[i + j * LDM] = mi;
   }
@@ -887,7 +968,7 @@ __global__ __launch_bounds__(NT) void couple_symm(CNArgs a) {
         const uint32_t i = lane + 64 * h;
         if (i < r.rank) {
           double sr, si;
-          ld<C>(a.S, b.soff + i + (uint64_t)j * r.rank, sr, si);
+          ld_nt<C>(a.S, b.soff + i + (uint64_t)j * r.rank, sr, si);
           a1r[h] = fma(sr, xcr, a1r[h]);
           pr = fma(sr, xr[h][0], pr);
           if constexpr (C) {

@@ -10,7 +10,7 @@ free.  Each cached operator pair owns a stream and a batched solver
 GMRES for Helmholtz); chunks of
 different operators are launched without waiting, so solves on different
 meshes overlap on the device, and a request enters between two chunks of
-the solves already running.  New operators are built by a worker thread
+the solves already running.  New operators are built by worker threads
 while the loop keeps serving, smallest first.
 
 Problems (the reference's Dirichlet tests, Tests/test_laplacebem3d.c and
@@ -107,7 +107,8 @@ class Server:
 
     def __init__(self, meshes, budget_bytes, slots=8, kmax=30, eps=1e-8,
                  maxiter=1000, clf=64, m_laplace=4, m_helmholtz=4,
-                 laplace_method="cg", computed_couplings=True):
+                 laplace_method="cg", computed_couplings=True,
+                 max_builds=1):
         """laplace_method: batched solver of the Laplace requests, "cg"
         (V is symmetric positive definite; the reference Laplace test's
         solver) or "gmres"; Helmholtz requests use GMRES(kmax).
@@ -128,7 +129,12 @@ class Server:
         self.used = 0
         self.builds = self.evictions = 0
         self._fp = {}
-        self._build = None  # (key, thread, result dict)
+        # operator builds in flight: key -> (thread, result dict); up to
+        # max_builds at once (measured with two: the builds' near-field
+        # launches slow the running solves, cold-cache p50 96 -> 172 ms with
+        # the same p99, so one is the default)
+        self._builds = {}
+        self.max_builds = max_builds
 
     # ---- operators -------------------------------------------------------
     def _footprint(self, key):
@@ -192,22 +198,20 @@ class Server:
         out = {}
         th = threading.Thread(target=self._make, args=(key, out))
         th.start()
-        self._build = (key, th, out)
+        self._builds[key] = (th, out)
         return True
 
-    def _finish_build(self, wait=False):
-        if self._build is None:
-            return
-        key, th, out = self._build
-        if th.is_alive() and not wait:
-            return
-        th.join()
-        self._build = None
-        if "entry" not in out:
-            self.used -= self._footprint(key)
-            raise RuntimeError("operator build failed for %r" % (key,))
-        self.cache[key] = out["entry"]
-        self.builds += 1
+    def _finish_builds(self, wait=False):
+        for key, (th, out) in list(self._builds.items()):
+            if th.is_alive() and not wait:
+                continue
+            th.join()
+            del self._builds[key]
+            if "entry" not in out:
+                self.used -= self._footprint(key)
+                raise RuntimeError("operator build failed for %r" % (key,))
+            self.cache[key] = out["entry"]
+            self.builds += 1
 
     # ---- the loop ----------------------------------------------------------
     def run(self, requests, timeout=600.0):
@@ -224,7 +228,7 @@ class Server:
                 raise RuntimeError("serving did not finish in %g s" % timeout)
             while todo and todo[0].arrival <= now:
                 queue.append(todo.popleft())
-            self._finish_build()
+            self._finish_builds()
             progressed = False
             # completions
             for e in list(self.cache.values()):
@@ -243,11 +247,13 @@ class Server:
             # the next operator to build: the smallest one that a queued
             # request waits for (shortest job first keeps small meshes
             # from waiting behind large ones)
-            if self._build is None:
-                missing = {r.key for r in queue if r.key not in self.cache}
+            if len(self._builds) < self.max_builds:
+                missing = {r.key for r in queue if r.key not in self.cache
+                           and r.key not in self._builds}
                 for key in sorted(missing, key=self._footprint):
-                    if self._start_build(key):
+                    if len(self._builds) >= self.max_builds:
                         break
+                    self._start_build(key)
             # admissions, in arrival order
             for req in list(queue):
                 e = self.cache.get(req.key)
@@ -271,14 +277,14 @@ class Server:
                     progressed = True
             if not progressed:
                 busy = any(e.batch.inflight for e in self.cache.values())
-                if not busy and self._build is None and not queue and todo:
+                if not busy and not self._builds and not queue and todo:
                     time.sleep(max(0.0, todo[0].arrival -
                                    (time.perf_counter() - t0)))
-                elif not busy and self._build is not None:
-                    self._build[1].join(0.002)
+                elif not busy and self._builds:
+                    next(iter(self._builds.values()))[0].join(0.002)
                 else:
                     time.sleep(0.0002)
-        self._finish_build(wait=True)
+        self._finish_builds(wait=True)
         return requests
 
 

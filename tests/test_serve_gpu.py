@@ -38,7 +38,7 @@ def check_against_alone(srv, reqs, exact):
         e = srv.cache.get(r.key)
         if e is None:  # evicted: rebuild through the server's builder
             srv._start_build(r.key)
-            srv._finish_build(wait=True)
+            srv._finish_builds(wait=True)
             e = srv.cache[r.key]
         f, b = e.prob.rhs_sources(r.sources, e.stream)
         x = h.DeviceArray(e.prob.n, b.dtype)
