@@ -39,6 +39,30 @@ constexpr int NB = 240;    // blocks of the multi-dot kernel
 constexpr int MAXK = 64;   // kmax limit
 constexpr int CHUNK = 8;   // steps between two looks at the residual
 
+// Thread t's share of a sum over blocks of the partial sums stored at
+// part[(b * 64 + j) * 2 + {0, 1}]: value v = t % nv of the blocks
+// b = t / nv, t / nv + groups, ... below nblk, added in that order.  The
+// loads are issued eight at a time before their additions (one memory
+// round trip per eight blocks instead of one per block).
+__device__ __forceinline__ double strided_part_sum(const double *part,
+                                                   uint32_t nblk, int nv,
+                                                   int groups, int t) {
+  const int v = t % nv;
+  double s = 0.0;
+  for (uint32_t b0 = t / nv; b0 < nblk; b0 += 8 * groups) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t b = b0 + u * groups;
+      x[u] = b < nblk ? part[((uint64_t)b * 64 + (v >> 1)) * 2 + (v & 1)]
+                      : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += x[u];
+  }
+  return s;
+}
+
 // Sums, over the blocks of a grid, 2 m doubles (m <= 64) per block stored
 // at part[(b * 64 + j) * 2 + {0, 1}]; run by the last block of the grid to
 // finish (cnt counts the finished blocks and is reset to 0 here).  The
@@ -63,12 +87,7 @@ __device__ void last_block_sum(const double *part, int m, uint32_t *cnt,
   if (!last) return;
   const int nv = 2 * m, groups = NT / nv, t = threadIdx.x;
   double s = 0.0;
-  if (t < groups * nv) {
-    const int v = t % nv;
-#pragma unroll 4
-    for (uint32_t b = t / nv; b < gridDim.x; b += groups)
-      s += part[((uint64_t)b * 64 + (v >> 1)) * 2 + (v & 1)];
-  }
+  if (t < groups * nv) s = strided_part_sum(part, gridDim.x, nv, groups, t);
   red[t] = s;
   __syncthreads();
   if (t < nv) {
@@ -109,18 +128,48 @@ __device__ __forceinline__ double wave_sum32(double (&v)[32], int lane) {
 // `part` for the next kernel (sum_parts), with no atomics and no fences.
 enum { RED_LAST = 0, RED_ATOM = 1, RED_PART = 2 };
 
+// The block's sums of the threads' accumulators of columns j0 .. j0 + mj
+// (mj <= 32; imaginary parts when C): a butterfly over the wave's lanes,
+// then the waves' sums added in LDS; RED_ATOM adds them into out
+// (2 * j0 on), otherwise they go to part[(b * 64 + j) * 2 + {0, 1}].
+template <int C, int ATOM>
+__device__ __forceinline__ void block_sums32(double (&acc_re)[32],
+                                             double (&acc_im)[32], int j0,
+                                             int mj, double *part,
+                                             double *out) {
+  __shared__ double red[NT / 64][2 * 32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // lane l ends with the sum of accumulator l & 31
+  const double r = wave_sum32(acc_re, lane);
+  const double im = C ? wave_sum32(acc_im, lane) : 0.0;
+  if (lane < mj) {
+    red[wv][2 * lane] = r;
+    red[wv][2 * lane + 1] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * mj) {
+    double s = 0.0;
+    for (int v = 0; v < NT / 64; v++) s += red[v][threadIdx.x];
+    if constexpr (ATOM == RED_ATOM)
+      atomicAdd(&out[2 * j0 + threadIdx.x], s);
+    else
+      part[((uint64_t)blockIdx.x * 64 + (j0 + threadIdx.x / 2)) * 2 +
+           (threadIdx.x & 1)] = s;
+  }
+  __syncthreads();
+}
+
 // out[0 .. 2m) = Q[:, 0..m)^* w (complex pairs; imaginary parts 0 when real)
 // ATOM (RED_ATOM): every block adds its sums into out (zeroed beforehand)
 // with FP64 atomics instead of the last-block reduction, whose agent-scope
 // fences write back and invalidate the L2 of every XCD; RED_PART: the
-// block's sums stay in part[(b * 64 + j) * 2 + {0, 1}]
+// block's sums stay in part[(b * 64 + j) * 2 + {0, 1}].  Columns from mq on
+// are w itself (with mq = m - 1 the last sum is |w|^2).
 template <int C, int ATOM = RED_LAST>
 __device__ __forceinline__ void multidot_body(
     const double *__restrict__ Q, uint64_t ldq, int m,
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
-    uint32_t *cnt, double *out) {
-  __shared__ double red[NT / 64][2 * 32];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t *cnt, double *out, int mq = 0x7fffffff) {
   for (int j0 = 0; j0 < m; j0 += 32) {
     const int mj = min(32, m - j0);
     double acc_re[32], acc_im[32];
@@ -135,38 +184,47 @@ __device__ __forceinline__ void multidot_body(
       } else {
         wr = w[i];
       }
+      // the row's entries of LB columns are loaded before any of them is
+      // used, one memory round trip per LB columns instead of one per
+      // column (a load under a per-column branch is waited for at once);
+      // columns past the last repeat it, and their sums are never stored
+      constexpr int LB = C ? 16 : 32;
 #pragma unroll
-      for (int j = 0; j < 32; j++) {
-        if (j < mj) {
-          const uint64_t q = (uint64_t)(j0 + j) * ldq + i;
+      for (int h0 = 0; h0 < 32; h0 += LB) {
+        double qr[LB], qi[LB];
+#pragma unroll
+        for (int c = 0; c < LB; c += 8) {
+#pragma unroll
+          for (int t = 0; t < 8; t++) qr[c + t] = qi[c + t] = 0.0;
+          if (h0 + c < mj) {
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+              const int j = j0 + min(h0 + c + t, mj - 1);
+              const bool self = j >= mq;
+              const double *src = self ? w : Q;
+              const uint64_t q = self ? i : (uint64_t)j * ldq + i;
+              if constexpr (C) {
+                qr[c + t] = src[2 * q];
+                qi[c + t] = src[2 * q + 1];
+              } else {
+                qr[c + t] = src[q];
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < LB; t++) {
+          const int j = h0 + t;
           if constexpr (C) {
-            const double qr = Q[2 * q], qi = Q[2 * q + 1];
-            acc_re[j] = fma(qr, wr, fma(qi, wi, acc_re[j]));
-            acc_im[j] = fma(qr, wi, fma(-qi, wr, acc_im[j]));
+            acc_re[j] = fma(qr[t], wr, fma(qi[t], wi, acc_re[j]));
+            acc_im[j] = fma(qr[t], wi, fma(-qi[t], wr, acc_im[j]));
           } else {
-            acc_re[j] = fma(Q[q], wr, acc_re[j]);
+            acc_re[j] = fma(qr[t], wr, acc_re[j]);
           }
         }
       }
     }
-    // the 32 sums over the wave's lanes: lane l ends with sum l & 31
-    const double r = wave_sum32(acc_re, lane);
-    const double im = C ? wave_sum32(acc_im, lane) : 0.0;
-    if (lane < mj) {
-      red[wv][2 * lane] = r;
-      red[wv][2 * lane + 1] = im;
-    }
-    __syncthreads();
-    if (threadIdx.x < 2 * mj) {
-      double s = 0.0;
-      for (int v = 0; v < NT / 64; v++) s += red[v][threadIdx.x];
-      if constexpr (ATOM == RED_ATOM)
-        atomicAdd(&out[2 * j0 + threadIdx.x], s);
-      else
-        part[((uint64_t)blockIdx.x * 64 + (j0 + threadIdx.x / 2)) * 2 +
-             (threadIdx.x & 1)] = s;
-    }
-    __syncthreads();
+    block_sums32<C, ATOM>(acc_re, acc_im, j0, mj, part, out);
   }
   if constexpr (ATOM == RED_LAST) last_block_sum(part, m, cnt, out);
 }
@@ -177,6 +235,64 @@ __global__ __launch_bounds__(NT) void multidot(
     const double *__restrict__ w, uint64_t n, double *__restrict__ part,
     uint32_t *cnt, double *out) {
   multidot_body<C, ATOM>(Q, ldq, m, w, n, part, cnt, out);
+}
+
+// (ar, ai) = row i of Q h over m columns (h: complex pairs).  LB columns'
+// entries and coefficients are loaded before the products (one memory round
+// trip per LB columns); the products run in column order over the columns
+// that exist.
+template <int C>
+__device__ __forceinline__ void qh_row(const double *__restrict__ Q,
+                                       uint64_t ldq, int m, const double *h,
+                                       uint64_t i, double &ar, double &ai) {
+  ar = 0.0;
+  ai = 0.0;
+  constexpr int LB = C ? 8 : 16;
+  for (int j0 = 0; j0 < m; j0 += LB) {
+    const int mj = min(LB, m - j0);
+    double qr[LB], qi[LB], hr[LB], hi[LB];
+#pragma unroll
+    for (int t = 0; t < LB; t++) {
+      const int j = j0 + min(t, mj - 1);
+      const uint64_t q = (uint64_t)j * ldq + i;
+      if constexpr (C) {
+        qr[t] = Q[2 * q];
+        qi[t] = Q[2 * q + 1];
+        hi[t] = h[2 * j + 1];
+      } else {
+        qr[t] = Q[q];
+      }
+      hr[t] = h[2 * j];
+    }
+#pragma unroll
+    for (int t = 0; t < LB; t++) {
+      if (t < mj) {
+        if constexpr (C) {
+          ar = fma(qr[t], hr[t], fma(-qi[t], hi[t], ar));
+          ai = fma(qr[t], hi[t], fma(qi[t], hr[t], ai));
+        } else {
+          ar = fma(qr[t], hr[t], ar);
+        }
+      }
+    }
+  }
+}
+
+// Row i of the first m < 32 columns of Q (real case) into q; entries past
+// the last column repeat it.  The loads are issued together.
+__device__ __forceinline__ void load_row32(const double *__restrict__ Q,
+                                           uint64_t ldq, int m, uint64_t i,
+                                           double (&q)[32]) {
+#pragma unroll
+  for (int c = 0; c < 32; c += 8) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) q[c + t] = 0.0;
+    if (c < m) {
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+        q[c + t] = Q[(uint64_t)min(c + t, m - 1) * ldq + i];
+    }
+  }
 }
 
 // w -= sign Q h (h: m complex pairs); with nrm != nullptr also
@@ -190,18 +306,8 @@ __device__ __forceinline__ void multiaxpy_body(
   double sq = 0.0;
   for (uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * NT) {
-    double ar = 0.0, ai = 0.0;
-    for (int j = 0; j < m; j++) {
-      const uint64_t q = (uint64_t)j * ldq + i;
-      if constexpr (C) {
-        const double qr = Q[2 * q], qi = Q[2 * q + 1];
-        const double hr = h[2 * j], hi = h[2 * j + 1];
-        ar = fma(qr, hr, fma(-qi, hi, ar));
-        ai = fma(qr, hi, fma(qi, hr, ai));
-      } else {
-        ar = fma(Q[q], h[2 * j], ar);
-      }
-    }
+    double ar, ai;
+    qh_row<C>(Q, ldq, m, h, i, ar, ai);
     if constexpr (C) {
       const double vr = fma(-sign, ar, w[2 * i]);
       const double vi = fma(-sign, ai, w[2 * i + 1]);
@@ -249,11 +355,7 @@ __device__ void sum_parts(const double *part, uint32_t nblk, int m,
   __shared__ double red[NT];
   const int nv = 2 * m, groups = NT / nv, t = threadIdx.x;
   double s = 0.0;
-  if (t < groups * nv) {
-    const int v = t % nv;
-    for (uint32_t b = t / nv; b < nblk; b += groups)
-      s += part[((uint64_t)b * 64 + (v >> 1)) * 2 + (v & 1)];
-  }
+  if (t < groups * nv) s = strided_part_sum(part, nblk, nv, groups, t);
   red[t] = s;
   __syncthreads();
   if (t < nv) {
@@ -265,10 +367,10 @@ __device__ void sum_parts(const double *part, uint32_t nblk, int m,
 }
 
 // Gram-Schmidt of one Arnoldi step without atomics (single solver):
-// gs_dot leaves the blocks' sums of Q^* w; gs_axpy_dot sums them (h, kept
-// in hout by block 0), forms w -= Q h on its rows and leaves the blocks'
-// sums of Q^* w again; gs_axpy_norm does the second w -= Q h and leaves
-// the blocks' sums of |w|^2.  The three kernels share the grid and the
+// gs_dot leaves the blocks' sums of Q^* w; gs_axpy_dot sums them (h1, kept
+// in hout by block 0), forms w' = w - Q h1 on its rows and leaves the
+// blocks' sums of Q^* w' and of |w'|^2; gs_axpy_scale (below) does the
+// second pass and the normalisation.  The kernels share the grid and the
 // rows of each thread.
 template <int C>
 __global__ __launch_bounds__(NT) void gs_dot(const double *__restrict__ Q,
@@ -283,21 +385,50 @@ __global__ __launch_bounds__(NT) void gs_axpy_dot(
     const double *__restrict__ Q, uint64_t ldq, int m, const double *pin,
     uint32_t nblk, double *hout, double *w, uint64_t n, double *pout) {
   __shared__ double hs[128];
+  if constexpr (!C) {
+    if (m < 32) {  // uniform
+      // each entry of Q loaded once for both passes (the arithmetic of the
+      // two bodies below): w' = w - Q h, then the sums of Q^T w' and, as
+      // column m, |w'|^2.  The thread's first row is loaded before the
+      // blocks' sums are read, so that both round trips overlap.
+      const uint64_t i0 = (uint64_t)blockIdx.x * NT + threadIdx.x;
+      const uint64_t stride = (uint64_t)gridDim.x * NT;
+      double q[32], wv = 0.0;
+      if (i0 < n) {
+        load_row32(Q, ldq, m, i0, q);
+        wv = w[i0];
+      }
+      sum_parts(pin, nblk, m, hs);
+      if (blockIdx.x == 0 && threadIdx.x < 2 * m)
+        hout[threadIdx.x] = hs[threadIdx.x];
+      double acc[32], unused[32];
+#pragma unroll
+      for (int t = 0; t < 32; t++) acc[t] = unused[t] = 0.0;
+      for (uint64_t i = i0; i < n; i += stride) {
+        if (i != i0) {
+          load_row32(Q, ldq, m, i, q);
+          wv = w[i];
+        }
+        double ar = 0.0;
+#pragma unroll
+        for (int t = 0; t < 32; t++)
+          if (t < m) ar = fma(q[t], hs[2 * t], ar);
+        const double v = fma(-1.0, ar, wv);
+        w[i] = v;
+#pragma unroll
+        for (int t = 0; t < 32; t++)
+          acc[t] = fma(t == m ? v : q[t], v, acc[t]);
+      }
+      block_sums32<0, RED_PART>(acc, unused, 0, m + 1, pout, nullptr);
+      return;
+    }
+  }
   sum_parts(pin, nblk, m, hs);
   if (blockIdx.x == 0 && threadIdx.x < 2 * m) hout[threadIdx.x] = hs[threadIdx.x];
   multiaxpy_body<C, RED_PART>(Q, ldq, m, hs, 1.0, w, n, nullptr, nullptr,
                               nullptr);
-  multidot_body<C, RED_PART>(Q, ldq, m, w, n, pout, nullptr, nullptr);
-}
-
-template <int C>
-__global__ __launch_bounds__(NT) void gs_axpy_norm(
-    const double *__restrict__ Q, uint64_t ldq, int m, const double *pin,
-    uint32_t nblk, double *hout, double *w, uint64_t n, double *pout) {
-  __shared__ double hs[128];
-  sum_parts(pin, nblk, m, hs);
-  if (blockIdx.x == 0 && threadIdx.x < 2 * m) hout[threadIdx.x] = hs[threadIdx.x];
-  multiaxpy_body<C, RED_PART>(Q, ldq, m, hs, 1.0, w, n, pout, nullptr, pout);
+  // column m of the sums: w' itself, |w'|^2
+  multidot_body<C, RED_PART>(Q, ldq, m + 1, w, n, pout, nullptr, nullptr, m);
 }
 
 __global__ void copy_kernel(const double *src, double *dst, uint64_t n) {
@@ -355,29 +486,14 @@ struct LSQ {
   int kmax;
 };
 
-// q_{k+1} = w / |w| by all threads; thread 0 of block 0 adds column k of the
-// Hessenberg matrix (h = hA + hB, subdiagonal |w|) to the QR factorisation
-// with Givens rotations and stores the new residual estimate |g[k+1]|.
-// With wz != nullptr, w is left zero for the next product, and nz doubles
-// at acc (the other set of accumulators, which the next step adds into) are
-// cleared.
-__device__ __forceinline__ void scale_givens_body(
-    const double *w, double *__restrict__ qn, uint64_t n2,
-    const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
-    double *err, double *wz = nullptr, double *acc = nullptr,
-    uint32_t nz = 0) {
-  const double hn = sqrt(nrm2[0]);
-  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
-  if (i < n2) {
-    const double wv = w[i];
-    qn[i] = hn > 0.0 ? wv / hn : 0.0;
-    if (wz) wz[i] = 0.0;  // the same element, after its read
-  }
-  if (i < nz) acc[i] = 0.0;
-  if (blockIdx.x != 0) return;  // uniform per block
-  // block 0 stages the Hessenberg column h = hA + hB and the previous
-  // rotations in LDS (loads in parallel), so that the sequential rotations
-  // of thread 0 wait for no memory access
+// Column k of the Hessenberg matrix (h = hA + hB, subdiagonal hn) added to
+// the QR factorisation with Givens rotations by one workgroup (every thread
+// calls it): the column and the previous rotations are staged in LDS (loads
+// in parallel), so that the sequential rotations of thread 0 wait for no
+// memory access; thread 0 stores the new residual estimate |g[k+1]|.
+__device__ __forceinline__ void givens_column(const double *hA,
+                                              const double *hB, double hn,
+                                              LSQ s, int k, double *err) {
   __shared__ double sh[2 * (MAXK + 1)], scs[2 * MAXK], ssn[2 * MAXK];
   for (int e = threadIdx.x; e < 2 * (k + 1); e += NT) sh[e] = hA[e] + hB[e];
   for (int e = threadIdx.x; e < 2 * k; e += NT) {
@@ -429,6 +545,28 @@ __device__ __forceinline__ void scale_givens_body(
   err[0] = cabs_(g1);
 }
 
+// q_{k+1} = w / |w| by all threads; block 0 adds column k of the Hessenberg
+// matrix (h = hA + hB, subdiagonal |w|) to the QR factorisation.
+// With wz != nullptr, w is left zero for the next product, and nz doubles
+// at acc (the other set of accumulators, which the next step adds into) are
+// cleared.
+__device__ __forceinline__ void scale_givens_body(
+    const double *w, double *__restrict__ qn, uint64_t n2,
+    const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
+    double *err, double *wz = nullptr, double *acc = nullptr,
+    uint32_t nz = 0) {
+  const double hn = sqrt(nrm2[0]);
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n2) {
+    const double wv = w[i];
+    qn[i] = hn > 0.0 ? wv / hn : 0.0;
+    if (wz) wz[i] = 0.0;  // the same element, after its read
+  }
+  if (i < nz) acc[i] = 0.0;
+  if (blockIdx.x != 0) return;  // uniform per block
+  givens_column(hA, hB, hn, s, k, err);
+}
+
 __global__ __launch_bounds__(NT) void scale_givens(
     const double *__restrict__ w, double *__restrict__ qn, uint64_t n2,
     const double *nrm2, const double *hA, const double *hB, LSQ s, int k,
@@ -446,14 +584,67 @@ __global__ __launch_bounds__(NT) void scale_givens_z(
   scale_givens_body(w, qn, n2, nrm2, hA, hB, s, k, err, w, acc, nz);
 }
 
-// the same with |w|^2 summed from the blocks' partial sums of gs_axpy_norm
-__global__ __launch_bounds__(NT) void scale_givens_p(
-    double *w, double *__restrict__ qn, uint64_t n2, const double *pn,
-    uint32_t nblk, const double *hA, const double *hB, LSQ s, int k,
-    double *err) {
-  __shared__ double hs[2];
-  sum_parts(pn, nblk, 1, hs);
-  scale_givens_body(w, qn, n2, hs, hA, hB, s, k, err, w, nullptr, 0);
+// The second Gram-Schmidt pass fused with the normalisation: from the
+// blocks' sums that gs_axpy_dot left, h2 = Q^* w' and |w'|^2; then
+// w'' = w' - Q h2, whose norm follows from |w''|^2 = |w'|^2 - |h2|^2 (Q has
+// orthonormal columns) without a further reduction; q_{k+1} = w'' / |w''|,
+// and w = 0 for the next product.  Block 0 keeps h2 in hout and adds
+// column k (hA + h2, subdiagonal |w''|) to the QR factorisation.
+template <int C>
+__global__ __launch_bounds__(NT) void gs_axpy_scale(
+    const double *__restrict__ Q, uint64_t ldq, int m, const double *pin,
+    uint32_t nblk, const double *hA, double *hout, double *w,
+    double *__restrict__ qn, uint64_t n, LSQ s, int k, double *err) {
+  __shared__ double hs[128];
+  const uint64_t i0 = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * NT;
+  // real case, m < 32: the thread's first row of Q and w are loaded before
+  // the blocks' sums are read, so that both round trips overlap
+  const bool pre = !C && m < 32;
+  double q[32], wv = 0.0;
+  if (pre && i0 < n) {
+    load_row32(Q, ldq, m, i0, q);
+    wv = w[i0];
+  }
+  sum_parts(pin, nblk, m + 1, hs);
+  double h2 = 0.0;
+  for (int j = 0; j < m; j++)
+    h2 = fma(hs[2 * j], hs[2 * j], fma(hs[2 * j + 1], hs[2 * j + 1], h2));
+  const double hn = sqrt(fmax(hs[2 * m] - h2, 0.0));
+  for (uint64_t i = i0; i < n; i += stride) {
+    if constexpr (!C) {
+      if (pre) {  // the order of qh_row's products
+        if (i != i0) {
+          load_row32(Q, ldq, m, i, q);
+          wv = w[i];
+        }
+        double ar = 0.0;
+#pragma unroll
+        for (int t = 0; t < 32; t++)
+          if (t < m) ar = fma(q[t], hs[2 * t], ar);
+        const double v = wv - ar;
+        qn[i] = hn > 0.0 ? v / hn : 0.0;
+        w[i] = 0.0;
+        continue;
+      }
+    }
+    double ar, ai;
+    qh_row<C>(Q, ldq, m, hs, i, ar, ai);
+    if constexpr (C) {
+      const double vr = w[2 * i] - ar, vi = w[2 * i + 1] - ai;
+      qn[2 * i] = hn > 0.0 ? vr / hn : 0.0;
+      qn[2 * i + 1] = hn > 0.0 ? vi / hn : 0.0;
+      w[2 * i] = 0.0;
+      w[2 * i + 1] = 0.0;
+    } else {
+      const double v = w[i] - ar;
+      qn[i] = hn > 0.0 ? v / hn : 0.0;
+      w[i] = 0.0;
+    }
+  }
+  if (blockIdx.x != 0) return;  // uniform per block
+  if (threadIdx.x < 2 * m) hout[threadIdx.x] = hs[threadIdx.x];
+  givens_column(hA, hs, hn, s, k, err);
 }
 
 // q_0 = r / |r| by all threads; thread 0 of block 0 starts the
@@ -479,23 +670,60 @@ __global__ __launch_bounds__(NT) void start_cycle(double *q0, uint64_t n2,
   start_cycle_body(q0, n2, nrm2, s, err);
 }
 
-// y = -R^{-1} g for the first k columns (single thread), as coefficients
+// y = -R^{-1} g for the first k columns, as coefficients
 // for multiaxpy (which subtracts)
-__global__ void back_substitute(LSQ s, int k, double *y) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int i = k - 1; i >= 0; i--) {
-    cplx t = {s.g[2 * i], s.g[2 * i + 1]};
-    for (int j = i + 1; j < k; j++) {
-      const cplx r = {s.R[2 * (i + (uint64_t)j * s.kmax)],
-                      s.R[2 * (i + (uint64_t)j * s.kmax) + 1]};
-      t = cadd(t, cmul(r, {y[2 * j], y[2 * j + 1]}));  // y holds -y_j
+__global__ __launch_bounds__(NT) void back_substitute(LSQ s, int k,
+                                                     double *y) {
+  // R's upper triangle (column j holds rows 0..j from j (j + 1) / 2 on) and
+  // g are staged in LDS by all threads, eight loads in flight per thread;
+  // thread 0 then runs the substitution from LDS
+  __shared__ double sR[MAXK * (MAXK + 1)], sg[2 * MAXK], sy[2 * MAXK];
+  if (blockIdx.x != 0) return;  // uniform per block
+  const int ne = k * (k + 1) / 2;
+  for (int e0 = threadIdx.x; e0 < ne; e0 += 8 * NT) {
+    double vr[8], vi[8];
+    int at[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * NT;
+      at[u] = -1;
+      vr[u] = vi[u] = 0.0;
+      if (e < ne) {
+        int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while (j * (j + 1) / 2 > e) j--;  // guards the rounding of sqrt
+        while ((j + 1) * (j + 2) / 2 <= e) j++;
+        const int i = e - j * (j + 1) / 2;
+        const uint64_t q = 2 * (i + (uint64_t)j * s.kmax);
+        vr[u] = s.R[q];
+        vi[u] = s.R[q + 1];
+        at[u] = e;
+      }
     }
-    const cplx rii = {s.R[2 * (i + (uint64_t)i * s.kmax)],
-                      s.R[2 * (i + (uint64_t)i * s.kmax) + 1]};
-    const cplx yi = cdiv(t, rii);
-    y[2 * i] = -yi.re;
-    y[2 * i + 1] = -yi.im;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (at[u] >= 0) {
+        sR[2 * at[u]] = vr[u];
+        sR[2 * at[u] + 1] = vi[u];
+      }
   }
+  for (int e = threadIdx.x; e < 2 * k; e += NT) sg[e] = s.g[e];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = k - 1; i >= 0; i--) {
+      cplx t = {sg[2 * i], sg[2 * i + 1]};
+      for (int j = i + 1; j < k; j++) {
+        const int o = j * (j + 1) / 2 + i;
+        const cplx r = {sR[2 * o], sR[2 * o + 1]};
+        t = cadd(t, cmul(r, {sy[2 * j], sy[2 * j + 1]}));  // sy holds -y_j
+      }
+      const int o = i * (i + 1) / 2 + i;
+      const cplx yi = cdiv(t, {sR[2 * o], sR[2 * o + 1]});
+      sy[2 * i] = -yi.re;
+      sy[2 * i + 1] = -yi.im;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * k; e += NT) y[e] = sy[e];
 }
 
 struct Workspace {
@@ -632,7 +860,7 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   // x += Q_k y with R y = g
   auto finish = [&](int k) -> int {
     if (k == 0) return H2HIP_OK;
-    hipLaunchKernelGGL(back_substitute, dim3(1), dim3(64), 0, st, W.lsq(), k,
+    hipLaunchKernelGGL(back_substitute, dim3(1), dim3(NT), 0, st, W.lsq(), k,
                        W.y);
     hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
                        (const double *)W.Q.p, n, k, (const double *)W.y, 1.0,
@@ -674,13 +902,10 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
       hipLaunchKernelGGL((gs_axpy_dot<C>), gred, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1, (const double *)pa,
                          nb, hA, w, n, pb);
-      hipLaunchKernelGGL((gs_axpy_norm<C>), gred, dim3(NT), 0, st,
+      hipLaunchKernelGGL((gs_axpy_scale<C>), gred, dim3(NT), 0, st,
                          (const double *)W.Q.p, n, k + 1, (const double *)pb,
-                         nb, hB, w, n, pa);
-      hipLaunchKernelGGL(scale_givens_p, W.gv, dim3(NT), 0, st, w,
-                         W.col(k + 1), W.n2, (const double *)pa, nb,
-                         (const double *)hA, (const double *)hB, W.lsq(), k,
-                         W.err + slot);
+                         nb, (const double *)hA, hB, w, W.col(k + 1), n,
+                         W.lsq(), k, W.err + slot);
       return H2HIP_OK;
     }
     const dim3 gs((uint32_t)((std::max<uint64_t>(W.n2, n_next) + NT - 1) /
@@ -777,9 +1002,17 @@ __device__ bool cg_last_block(const double *part, int nv, uint32_t *cnt,
   if (!last) return false;
   const int groups = NT / 8, t = threadIdx.x, v = t & 7;
   double s = 0.0;
-  if (v < nv)
-    for (uint32_t b = t >> 3; b < gridDim.x; b += groups)
-      s += part[(uint64_t)b * 8 + v];
+  if (v < nv)  // in block order, eight loads in flight at a time
+    for (uint32_t b0 = t >> 3; b0 < gridDim.x; b0 += 8 * groups) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const uint32_t b = b0 + u * groups;
+        x[u] = b < gridDim.x ? part[(uint64_t)b * 8 + v] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += x[u];
+    }
   red[t] = s;
   __syncthreads();
   if (t < nv) {
@@ -1437,7 +1670,7 @@ namespace {
 int batch_finish(h2hip_gmres_batch *G, h2hip_gmres_batch::Slot &S) {
   if (S.k == 0) return H2HIP_OK;
   const dim3 gax((uint32_t)((G->n + NT - 1) / NT));
-  hipLaunchKernelGGL(back_substitute, dim3(1), dim3(64), 0, G->st,
+  hipLaunchKernelGGL(back_substitute, dim3(1), dim3(NT), 0, G->st,
                      S.W.lsq(), S.k, S.W.y);
   if (G->C)
     hipLaunchKernelGGL(multiaxpy<1>, gax, dim3(NT), 0, G->st,
@@ -2045,7 +2278,7 @@ template <int C>
 int multi_finish(Multi::Sys &S, int k, hipStream_t st) {
   if (k == 0) return H2HIP_OK;
   const dim3 gax((uint32_t)((S.n + NT - 1) / NT));
-  hipLaunchKernelGGL(back_substitute, dim3(1), dim3(64), 0, st, S.W.lsq(), k,
+  hipLaunchKernelGGL(back_substitute, dim3(1), dim3(NT), 0, st, S.W.lsq(), k,
                      S.W.y);
   hipLaunchKernelGGL(multiaxpy<C>, gax, dim3(NT), 0, st,
                      (const double *)S.W.Q.p, S.n, k, (const double *)S.W.y,
