@@ -484,9 +484,9 @@ template <int C, bool NTL = false>
 __device__ __forceinline__ void dot_row(const double *__restrict__ M,
                                         uint64_t stride, uint32_t n,
                                         const double *xs, double &ar,
-                                        double &ai) {
+                                        double &ai, uint32_t j0 = 0) {
   constexpr int B = C ? 8 : 16;
-  uint32_t j = 0;
+  uint32_t j = j0;
   for (; j + B <= n; j += B) {
     double mr[B], mi[B];
 #pragma unroll
@@ -516,6 +516,44 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
       ai = fma(mr, xi, fma(mi, xr, ai));
     }
   }
+}
+
+// The first columns of a row for dot_row_pf, loaded before the wave's input
+// gather so that the two round trips overlap (columns past n repeat the
+// last one and are not used)
+template <int C>
+__device__ __forceinline__ void row_prefetch(const double *__restrict__ M,
+                                             uint64_t stride, uint32_t n,
+                                             double (&mr)[C ? 8 : 16],
+                                             double (&mi)[C ? 8 : 16]) {
+  constexpr int B = C ? 8 : 16;
+#pragma unroll
+  for (int u = 0; u < B; u++)
+    ld<C>(M, (uint64_t)min((uint32_t)u, n - 1) * stride, mr[u], mi[u]);
+}
+
+// dot_row with columns 0 .. B - 1 from row_prefetch: the same products in
+// the same order
+template <int C>
+__device__ __forceinline__ void dot_row_pf(const double *__restrict__ M,
+                                           uint64_t stride, uint32_t n,
+                                           const double *xs,
+                                           const double (&mr)[C ? 8 : 16],
+                                           const double (&mi)[C ? 8 : 16],
+                                           double &ar, double &ai) {
+  constexpr int B = C ? 8 : 16;
+#pragma unroll
+  for (int u = 0; u < B; u++) {
+    if (u < (int)n) {
+      const double xr = xs[2 * u], xi = xs[2 * u + 1];
+      ar = fma(mr[u], xr, ar);
+      if constexpr (C) {
+        ar = fma(-mi[u], xi, ar);
+        ai = fma(mr[u], xi, fma(mi[u], xr, ai));
+      }
+    }
+  }
+  if (n > (uint32_t)B) dot_row<C>(M, stride, n, xs, ar, ai, B);
 }
 
 // Laplace couplings computed in the product at interpolation order 4, one
@@ -569,8 +607,11 @@ __device__ __forceinline__ void mf_laplace4(const double *p3,
 // stepping through the column points j = (ja m + jb) m + jc; each value
 // serves all NV vectors.
 // Item w of the wave (w < nfar: far block w, else near block w - nfar);
-// xs: the wave's LDS slice of NV * 2 * XR doubles.
-template <int C, bool TRANS, int NV, bool MF = false, bool ORD = false>
+// xs: the wave's LDS slice of NV * 2 * XR doubles.  PF: near blocks of one
+// vector load their first columns before the input gather (more registers;
+// used where the launch holds only near blocks).
+template <int C, bool TRANS, int NV, bool MF = false, bool ORD = false,
+          bool PF = false>
 __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
                                                 double *xs) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;  // inputs per LDS vector
@@ -707,6 +748,14 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
   const uint32_t nout = TRANS ? b.cols : b.rows, nin = TRANS ? b.rows : b.cols;
   const uint32_t *oidx = TRANS ? a.cidx + b.coff : a.ridx + b.roff;
   const uint32_t *iidx = TRANS ? a.ridx + b.roff : a.cidx + b.coff;
+  // one vector: the first columns of the lane's first row are loaded
+  // before the input gather
+  constexpr int PB = C ? 8 : 16;
+  double pr[PB], pim[PB];
+  if constexpr (PF && NV == 1 && !TRANS)
+    if (lane < nout && nin > 0)
+      row_prefetch<C>(a.N + (C ? 2 : 1) * (b.noff + lane), b.rows, nin, pr,
+                      pim);
   // leaf sizes <= 128 (one vector) or <= 64 (several)
   for (uint32_t e = lane; e < NV * nin; e += 64) {
     const uint32_t v = e / nin, j = e - v * nin;
@@ -721,8 +770,12 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
 #pragma unroll
     for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
     if constexpr (NV == 1 && !TRANS) {
-      dot_row<C>(a.N + (C ? 2 : 1) * (b.noff + i), b.rows, nin, xs, ar[0],
-                 ai[0]);
+      if (PF && i == (uint32_t)lane)
+        dot_row_pf<C>(a.N + (C ? 2 : 1) * (b.noff + i), b.rows, nin, xs, pr,
+                      pim, ar[0], ai[0]);
+      else
+        dot_row<C>(a.N + (C ? 2 : 1) * (b.noff + i), b.rows, nin, xs, ar[0],
+                   ai[0]);
     } else {
 #pragma unroll UNR
       for (uint32_t j = 0; j < nin; j++) {
@@ -1177,9 +1230,9 @@ template <int C, int NV, int XR>
 __device__ __forceinline__ void dot_strided_nv(const double *__restrict__ M,
                                                uint64_t stride, uint32_t n,
                                                const double *xs, double *ar,
-                                               double *ai) {
+                                               double *ai, uint32_t i0 = 0) {
   constexpr int B = 16;
-  uint32_t i = 0;
+  uint32_t i = i0;
   for (; i + B <= n; i += B) {
     double m[B];
 #pragma unroll
@@ -1202,6 +1255,24 @@ __device__ __forceinline__ void dot_strided_nv(const double *__restrict__ M,
   }
 }
 
+// dot_strided_nv for one vector with rows 0 .. 15 of M loaded beforehand
+// (m0; rows past n repeat the last one and are not used): the same products
+// in the same order
+template <int C, int XR>
+__device__ __forceinline__ void dot_strided_pf(const double *__restrict__ M,
+                                               uint64_t stride, uint32_t n,
+                                               const double *xs,
+                                               const double (&m0)[16],
+                                               double &ar, double &ai) {
+#pragma unroll
+  for (int u = 0; u < 16; u++)
+    if (u < (int)n) {
+      ar = fma(m0[u], xs[2 * u], ar);
+      if constexpr (C) ai = fma(m0[u], xs[2 * u + 1], ai);
+    }
+  if (n > 16u) dot_strided_nv<C, 1, XR>(M, stride, n, xs, &ar, &ai, 16);
+}
+
 // Upward sweep, one wave per (leaf, ancestor) block, NV vectors at once
 // (leaves and ranks <= 64 when NV > 1): x of the leaf gathered into LDS,
 // W read once for all vectors, atomics into the coefficients.
@@ -1218,6 +1289,15 @@ __device__ __forceinline__ void fwd_flat_dev(const FlatArgs &a, uint32_t blk,
   const NodeD L = a.nodes[it.leaf];
   const NodeD A = a.nodes[it.node];
   const uint32_t sz = L.size, k = K ? K : A.rank;
+  // one vector: the first rows of the lane's first column of W are loaded
+  // before the input gather, so that the two round trips overlap
+  double pw[16];
+  if constexpr (NV == 1)
+    if ((uint32_t)lane < k && sz > 0) {
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        pw[u] = a.W[it.woff + lane + (uint64_t)min((uint32_t)u, sz - 1) * k];
+    }
   for (uint32_t e = lane; e < NV * sz; e += 64) {
     const uint32_t v = e / sz, i = e - v * sz;
     double xr = 0.0, xi = 0.0;
@@ -1232,7 +1312,14 @@ __device__ __forceinline__ void fwd_flat_dev(const FlatArgs &a, uint32_t blk,
     double ar[NV], ai[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) ar[v] = ai[v] = 0.0;
-    dot_strided_nv<C, NV, XR>(a.W + it.woff + j, k, sz, xs, ar, ai);
+    if constexpr (NV == 1) {
+      if (j == (uint32_t)lane)
+        dot_strided_pf<C, XR>(a.W + it.woff + j, k, sz, xs, pw, ar[0], ai[0]);
+      else
+        dot_strided_nv<C, NV, XR>(a.W + it.woff + j, k, sz, xs, ar, ai);
+    } else {
+      dot_strided_nv<C, NV, XR>(a.W + it.woff + j, k, sz, xs, ar, ai);
+    }
 #pragma unroll
     for (int v = 0; v < NV; v++)
       if (v < (int)a.v.nv) {
@@ -1265,7 +1352,7 @@ __global__ __launch_bounds__(NT) void fwd_flat_near(FlatArgs a, CNArgs c,
   if (blockIdx.x < fb)  // uniform over the block
     fwd_flat_dev<C, K, NV, ORD>(a, blockIdx.x, fb, xs);
   else
-    couple_near_dev<C, TRANS, NV, false, ORD>(
+    couple_near_dev<C, TRANS, NV, false, ORD, true>(
         c, (blockIdx.x - fb) * WPB + (threadIdx.x >> 6), xs);
 }
 
