@@ -746,7 +746,7 @@ struct Workspace {
     kmax = kmax_;
     st = s;
     gv = dim3((uint32_t)((n2 + NT - 1) / NT));
-    const uint64_t nsmall = 128 * 3 + 8 + 128 * 2 + 8 + CHUNK +
+    const uint64_t nsmall = 128 * 3 + 8 + 128 * 2 + 8 + 2 * CHUNK +
                             2 * (uint64_t)kmax * kmax +
                             2 * ((uint64_t)kmax + 1) + 4 * (uint64_t)kmax;
     // partial sums, 128 doubles per block: the multi-dot grid has at most
@@ -775,8 +775,8 @@ struct Workspace {
     p += 128;
     nrm1 = p;
     p += 8;
-    err = p;
-    p += CHUNK;
+    err = p;  // two chunks' residual estimates
+    p += 2 * CHUNK;
     R = p;
     p += 2 * (uint64_t)kmax * kmax;
     g = p;
@@ -925,35 +925,80 @@ int gmres(h2hip_addeval_t addeval, void *A, uint64_t n, const double *b,
   H2HIP_CHECK(hipStreamSynchronize(st));
   H2HIP_CHECK(hipGetLastError());
 
-  int k = 0;
-  uint32_t iter = 0;
-  double errs[CHUNK];
-  while (error > eps * normb && iter + 1 != maxiter) {
-    if (k + 1 >= (int)kmax) {
+  // Chunks of up to CHUNK steps; while the host waits for the residual
+  // estimates of one chunk, the next chunk of the same restart cycle is
+  // already queued behind it, so the device does not wait for the host.
+  // Steps past the stopping point only touch later columns of Q, R and g,
+  // and are discarded; a restart waits for the decision.
+  struct Look {
+    double *host = nullptr;  // pinned, 2 * CHUNK estimates
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Look() {
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+      if (host) (void)hipHostFree(host);
+    }
+  } L;
+  if (hipHostMalloc((void **)&L.host, 2 * CHUNK * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess) {
+    h2hip::set_error("h2hip_solve_gmres: pinned allocation failed");
+    return H2HIP_ENOMEM;
+  }
+  for (auto &e : L.ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      h2hip::set_error("h2hip_solve_gmres: event creation failed");
+      return H2HIP_EDEVICE;
+    }
+  struct Pending {
+    uint32_t len;
+    int half;
+  } pend[2];
+  int npend = 0, next_half = 0;
+  int k = 0, kq = 0;           // steps accepted / queued in this cycle
+  uint32_t iter = 0, iq = 0;   // steps accepted / queued in all
+  bool stop = false;
+  while (!stop && error > eps * normb && iter + 1 != maxiter) {
+    if (k + 1 >= (int)kmax) {  // nothing is queued past a cycle's end
       rc = finish(k);
       if (rc) return rc;
       rc = init();
       if (rc) return rc;
-      k = 0;
+      k = kq = 0;
     }
-    // steps until the next restart, maxiter, or the chunk size
-    uint32_t len = (uint32_t)((int)kmax - 1 - k);
-    if (len > (uint32_t)CHUNK) len = CHUNK;
-    if (maxiter != 0 && len > maxiter - 1 - iter) len = maxiter - 1 - iter;
-    for (uint32_t j = 0; j < len; j++) {
-      rc = step(k + (int)j, (int)j);
-      if (rc) return rc;
+    // queue up to two chunks: until the next restart, maxiter, or CHUNK
+    while (npend < 2 && kq + 1 < (int)kmax &&
+           (maxiter == 0 || iq + 1 < maxiter)) {
+      uint32_t len = (uint32_t)((int)kmax - 1 - kq);
+      if (len > (uint32_t)CHUNK) len = CHUNK;
+      if (maxiter != 0 && len > maxiter - 1 - iq) len = maxiter - 1 - iq;
+      const int h = next_half;
+      for (uint32_t j = 0; j < len; j++) {
+        rc = step(kq + (int)j, h * CHUNK + (int)j);
+        if (rc) return rc;
+      }
+      H2HIP_CHECK(hipGetLastError());
+      H2HIP_CHECK(hipMemcpyAsync(L.host + h * CHUNK, W.err + h * CHUNK,
+                                 sizeof(double) * len, hipMemcpyDeviceToHost,
+                                 st));
+      H2HIP_CHECK(hipEventRecord(L.ev[h], st));
+      pend[npend++] = {len, h};
+      next_half ^= 1;
+      kq += (int)len;
+      iq += len;
     }
-    H2HIP_CHECK(hipGetLastError());
-    H2HIP_CHECK(hipMemcpyAsync(errs, W.err, sizeof(double) * len,
-                               hipMemcpyDeviceToHost, st));
-    H2HIP_CHECK(hipStreamSynchronize(st));
-    // replay the reference's loop over the steps just done
-    for (uint32_t j = 0; j < len; j++) {
+    // replay the reference's loop over the oldest chunk's steps
+    const Pending c = pend[0];
+    pend[0] = pend[1];
+    npend--;
+    H2HIP_CHECK(hipEventSynchronize(L.ev[c.half]));
+    for (uint32_t j = 0; j < c.len; j++) {
       k++;
       iter++;
-      error = errs[j];
-      if (!(error > eps * normb && iter + 1 != maxiter)) break;
+      error = L.host[c.half * CHUNK + j];
+      if (!(error > eps * normb && iter + 1 != maxiter)) {
+        stop = true;
+        break;
+      }
     }
   }
   rc = finish(k);
