@@ -1180,6 +1180,28 @@ def main():
                 names.append("nf_fused_persistent<%d, 1, 1, 2, 0>" % k)
         if all(n in kb for n in names):
             traffic = sum(kb[n] for n in names)
+    # FP64 flops the kernels execute per launch (counter-measured, committed
+    # beside the traffic): the executed rate over the same kernel time
+    executed = None
+    vpath = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                         "profiles", "valu_nearfield_sphere32768.json")
+    if os.path.exists(vpath) and (args.split, args.clf, args.eta) == \
+            (64, 64, 2.0):
+        with open(vpath) as f:
+            fl = json.load(f).get("flops_fp64_per_launch", {})
+        names = []
+        for k in (0, 1):
+            if schedule == "per_class":
+                names.append("nf_regular_kernel<%d, 1, 1, 2, 0>" % k)
+                names += ["nf_singular_kernel<%d, 1, 1, %d, 4, false>" %
+                          (k, c) for c in (1, 2, 3)]
+            elif schedule == "fused":
+                names.append("nf_fused_kernel<%d, 1, 1, 2, 0>" % k)
+            else:
+                names.append("nf_fused_persistent<%d, 1, 1, 2, 0>" % k)
+        if all(n in fl for n in names):
+            executed = (sum(fl[n] for n in names) /
+                        (k_avg.sum() * 1e-3) / 1e12)
 
     detail = {}
     if "mixed" in sections:
@@ -1254,6 +1276,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": traffic,
+            "executed": executed,
+            "executed_frac": (executed / PEAK_FP64_TFLOPS
+                              if executed is not None else None),
         },
         "cpu_baseline": cpu,
     }

@@ -31,6 +31,7 @@
 #include <mutex>
 
 #include "h2hip_internal.h"
+#include "wave_sum.h"
 
 namespace {
 
@@ -437,6 +438,15 @@ struct CplSrc {
   uint32_t m;
   int helmholtz;
   double kr, kdamp;
+};
+
+// A mirrored pair of far blocks (r, c) / (c, r), r > c, for
+// couple_mf_pairs: the nodes and the four coefficient offsets, so that a
+// wave needs one load before its inputs
+struct PairD {
+  uint32_t r, c;
+  uint64_t out_r, in_c;  // block (r, c): yt of row node r, xt of column node c
+  uint64_t out_c, in_r;  // mirror (c, r): yt of row node c, xt of column node r
 };
 
 struct CNArgs {
@@ -982,6 +992,65 @@ __global__ __launch_bounds__(NT) void couple_trans_wg(CNArgs a) {
                     a.v.are[v] * si + a.v.aim[v] * sr);
     }
   }
+}
+
+// Laplace couplings computed in the product at interpolation order 4, one
+// vector, for a symmetric block structure (row and column trees with the
+// same points, every admissible block (r, c) with its mirror (c, r), whose
+// coupling matrix is S_rc^T: the squared distances, and so the values, are
+// the same bit for bit).  One wave per pair r > c evaluates each value once
+// for both blocks: lane i (row point i of r) forms yt_r[i] += sum_j S_ij
+// xt_c[j] with the arithmetic and order of mf_laplace4 (the sums of the
+// block's own wave), and the products S_ij xt_r[i] of 16 columns at a time,
+// whose sums over the lanes, yt_c[j] += sum_i S_ij xt_r[i] for the mirror
+// block, a butterfly forms.  Half the evaluations of the two blocks' waves.
+__global__ __launch_bounds__(NT) void couple_mf_pairs(CNArgs a,
+                                                      const PairD *pairs,
+                                                      uint32_t npair) {
+  constexpr double KCONST = 0.0795774715459476679;
+  __shared__ double xs_all[WPB][64];
+  const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= npair) return;  // uniform over the wave; no block barrier
+  const PairD b = pairs[w];
+  const uint64_t out_r = b.out_r, in_c = b.in_c, out_c = b.out_c,
+                 in_r = b.in_r;
+  double *xs = xs_all[threadIdx.x >> 6];
+  xs[lane] = a.xt[in_c + lane];  // ranks are 64
+  const double xr = a.xt[in_r + lane];
+  __builtin_amdgcn_wave_barrier();
+  const double *pr = a.cs.rpts + 12 * (uint64_t)b.r;
+  const double *pc = a.cs.cpts + 12 * (uint64_t)b.c;
+  const double px = pr[lane >> 4], py = pr[4 + ((lane >> 2) & 3)],
+               pz = pr[8 + (lane & 3)];
+  double qy[4], qz[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    qy[t] = pc[4 + t];
+    qz[t] = pc[8 + t];
+  }
+  double acc = 0.0;
+#pragma unroll 1
+  for (int ja = 0; ja < 4; ja++) {  // columns 16 ja .. 16 ja + 15
+    const double d0 = px - pc[ja];
+    const double s0 = d0 * d0;
+    double pt[16];
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++) {
+      const double d1 = py - qy[jb];
+      const double s1 = fma(d1, d1, s0);
+#pragma unroll
+      for (int jc = 0; jc < 4; jc++) {
+        const double d2 = pz - qz[jc];
+        const double sr = KCONST * cpl_rsqrt(fma(d2, d2, s1));
+        acc = fma(sr, xs[16 * ja + 4 * jb + jc], acc);
+        pt[4 * jb + jc] = sr * xr;
+      }
+    }
+    const double cs = h2hip::wave_sum16(pt, lane);  // column 16 ja + (lane & 15)
+    if ((lane >> 4) == ja) atomicAdd(a.yt + out_c + lane, cs);
+  }
+  atomicAdd(a.yt + out_r + lane, acc);
 }
 
 // Self-adjoint product from the lower block triangle (row and column
@@ -1830,6 +1899,12 @@ struct h2hip_h2matrix {
   // and the kernel, kept for products that compute the couplings
   h2hip::DevBuf<double> d_rpts, d_cpts;
   CplSrc cpl = {};
+  // symmetric block structure (same points on both sides, every far block
+  // with its mirror): the far blocks r > c, whose waves compute each
+  // coupling value once for both blocks (couple_mf_pairs)
+  h2hip::DevBuf<PairD> d_pairs;
+  uint32_t npair = 0;
+  int pairs_on = 1;
   bool cpl_known = false;
   int cpl_recompute = 0;
   uint32_t maxleaf = 0, maxrank = 0;  // over both sides
@@ -2263,7 +2338,16 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
         hipLaunchKernelGGL((couple_near<C, true, NV>), gc, dim3(NT), 0, st,
                            c);
     } else if (mf) {
-      if (ord)
+      // symmetric structure, real order-4 couplings, one vector, far blocks
+      // only: each value once for a block and its mirror
+      bool paired = false;
+      if constexpr (!C && NV == 1)
+        paired = H->pairs_on && H->npair && c.nnear == 0 && !ord;
+      if (paired)
+        hipLaunchKernelGGL(couple_mf_pairs,
+                           dim3((H->npair + WPB - 1) / WPB), dim3(NT), 0, st,
+                           c, (const PairD *)H->d_pairs.p, H->npair);
+      else if (ord)
         hipLaunchKernelGGL((couple_near<C, false, NV, true, true>), gc,
                            dim3(NT), 0, st, c);
       else
@@ -3112,7 +3196,47 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
   H2HIP_CHECK(hipStreamSynchronize(st));
   H->cpl = CplSrc{H->d_rpts.p, H->d_cpts.p, m, a.helmholtz, a.kr, a.kdamp};
   H->cpl_known = true;
+  // the pairs of mirrored far blocks, when the structure is symmetric
+  H->npair = 0;
+  if (!a.helmholtz && m == 4 && rn == cn &&
+      std::memcmp(rpts, cpts, sizeof(double) * 3 * (uint64_t)m * rn) == 0) {
+    std::vector<std::pair<uint64_t, uint32_t>> key(H->nfar);
+    for (uint32_t f = 0; f < H->nfar; f++)
+      key[f] = {(uint64_t)H->far[f].r << 32 | H->far[f].c, f};
+    std::sort(key.begin(), key.end());
+    std::vector<PairD> pairs;
+    const std::vector<NodeD> &rnodes = H->side[0].nodes,
+                             &cnodes = H->side[1].nodes;
+    bool symmetric = true;
+    for (uint32_t f = 0; f < H->nfar && symmetric; f++) {
+      const FarD &b = H->far[f];
+      const uint64_t mirror = (uint64_t)b.c << 32 | b.r;
+      const auto it = std::lower_bound(
+          key.begin(), key.end(), std::make_pair(mirror, 0u));
+      symmetric = b.r != b.c && it != key.end() && it->first == mirror;
+      if (b.r > b.c)
+        pairs.push_back(PairD{b.r, b.c, rnodes[b.r].coef, cnodes[b.c].coef,
+                              rnodes[b.c].coef, cnodes[b.r].coef});
+    }
+    if (symmetric && !pairs.empty()) {
+      H2HIP_CHECK(H->d_pairs.upload(pairs.data(), pairs.size()));
+      H->npair = (uint32_t)pairs.size();
+    }
+  }
   return H2HIP_OK;
+}
+
+int h2hip_h2matrix_set_coupling_pairs(h2hip_h2matrix *H, int on) {
+  if (!H || on < 0 || on > 1) {
+    set_error("h2hip_h2matrix_set_coupling_pairs: invalid arguments");
+    return H2HIP_EINVAL;
+  }
+  H->pairs_on = on;
+  return H2HIP_OK;
+}
+
+int h2hip_h2matrix_coupling_pairs(const h2hip_h2matrix *H) {
+  return H ? (int)H->npair : -1;
 }
 
 void *h2hip_h2matrix_nearfield_device(h2hip_h2matrix *H) {

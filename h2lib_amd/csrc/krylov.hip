@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "h2hip_internal.h"
+#include "wave_sum.h"
 
 namespace {
 
@@ -98,29 +99,7 @@ __device__ void last_block_sum(const double *part, int m, uint32_t *cnt,
   if (t == 0) *cnt = 0;
 }
 
-// Sums of 32 values over the 64 lanes of a wave by a butterfly that halves
-// the values a lane holds at each step (the lane keeps one half and adds
-// its partner's copy of it): lane l returns the sum of value l & 31 over
-// all lanes.  32 exchanges instead of 6 for each value.  v is clobbered.
-template <int S>
-__device__ __forceinline__ void bfly_step(double (&v)[32], int lane) {
-  const bool up = (lane & S) != 0;
-#pragma unroll
-  for (int t = 0; t < S; t++) {
-    const double send = up ? v[t] : v[t + S];
-    const double keep = up ? v[t + S] : v[t];
-    v[t] = keep + __shfl_xor(send, S, 64);
-  }
-}
-
-__device__ __forceinline__ double wave_sum32(double (&v)[32], int lane) {
-  bfly_step<16>(v, lane);
-  bfly_step<8>(v, lane);
-  bfly_step<4>(v, lane);
-  bfly_step<2>(v, lane);
-  bfly_step<1>(v, lane);
-  return v[0] + __shfl_xor(v[0], 32, 64);
-}
+using h2hip::wave_sum32;
 
 // Ways of combining the blocks' sums of the vector kernels: the last block
 // of the grid adds them in block order (fixed order), every block adds

@@ -71,6 +71,10 @@ def _sigs():
     h.h2hip_h2matrix_set_adjoint_storage.argtypes = [vp, C.c_int]
     h.h2hip_h2matrix_set_mfma.restype = C.c_int
     h.h2hip_h2matrix_set_mfma.argtypes = [vp, C.c_int]
+    h.h2hip_h2matrix_set_coupling_pairs.restype = C.c_int
+    h.h2hip_h2matrix_set_coupling_pairs.argtypes = [vp, C.c_int]
+    h.h2hip_h2matrix_coupling_pairs.restype = C.c_int
+    h.h2hip_h2matrix_coupling_pairs.argtypes = [vp]
     h.h2hip_h2matrix_interp_bases.restype = C.c_int
     h.h2hip_h2matrix_interp_bases.argtypes = [vp, C.c_int, vp, C.c_int,
                                               C.c_int, C.c_uint32, C.c_uint32,
@@ -363,6 +367,17 @@ class H2Matrix:
         check(hip().h2hip_h2matrix_set_mfma(self._h, int(bool(on))),
               "h2hip_h2matrix_set_mfma")
         self.mfma = int(bool(on))
+
+    def set_coupling_pairs(self, on):
+        """Computed couplings of a symmetric real order-4 structure: each
+        value evaluated once for a block and its mirror (the default)."""
+        check(hip().h2hip_h2matrix_set_coupling_pairs(self._h, int(bool(on))),
+              "h2hip_h2matrix_set_coupling_pairs")
+
+    def coupling_pairs(self):
+        """Mirrored pairs of far blocks found when the couplings were
+        interpolated (0 when the structure is not symmetric)."""
+        return int(hip().h2hip_h2matrix_coupling_pairs(self._h))
 
     def bytes(self):
         """Bytes one product y += alpha H x reads in the current schedule
