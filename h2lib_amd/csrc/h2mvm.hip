@@ -516,14 +516,72 @@ __device__ __forceinline__ void dot_row(const double *__restrict__ M,
       }
     }
   }
-  for (; j < n; j++) {
-    double mr, mi;
-    ld<C>(M, (uint64_t)j * stride, mr, mi);
-    const double xr = xs[2 * j], xi = xs[2 * j + 1];
-    ar = fma(mr, xr, ar);
-    if constexpr (C) {
-      ar = fma(-mi, xi, ar);
-      ai = fma(mr, xi, fma(mi, xr, ai));
+  if (j < n) {
+    // the last n - j < B columns, loaded together (indices past n repeat
+    // the last column), then the products in column order
+    const uint32_t rem = n - j;
+    double mr[B], mi[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const uint64_t e = (uint64_t)(j + min((uint32_t)u, rem - 1)) * stride;
+      if constexpr (NTL)
+        ld_nt<C>(M, e, mr[u], mi[u]);
+      else
+        ld<C>(M, e, mr[u], mi[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      if ((uint32_t)u < rem) {
+        const double xr = xs[2 * (j + u)], xi = xs[2 * (j + u) + 1];
+        ar = fma(mr[u], xr, ar);
+        if constexpr (C) {
+          ar = fma(-mi[u], xi, ar);
+          ai = fma(mr[u], xi, fma(mi[u], xr, ai));
+        }
+      }
+    }
+  }
+}
+
+// ar/ai[v] += sum_j M[base + j sj] x_v[j] over n columns for NV vectors in
+// LDS (complex pairs when C; CONJ: the conjugate of M; NTL: non-temporal
+// loads): B columns' entries are loaded together, the last batch with its
+// indices past n repeating the last column, before their products, which
+// run in column order.
+template <int C, int NV, int XR, bool NTL, bool CONJ>
+__device__ __forceinline__ void dot_cols_nv(const double *__restrict__ M,
+                                            uint64_t base, uint64_t sj,
+                                            uint32_t n, const double *xs,
+                                            double (&ar)[NV],
+                                            double (&ai)[NV]) {
+  constexpr int B = C ? 4 : 8;
+  for (uint32_t j0 = 0; j0 < n; j0 += B) {
+    const uint32_t rem = min((uint32_t)B, n - j0);
+    double mr[B], mi[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const uint64_t e = base + (uint64_t)(j0 + min((uint32_t)u, rem - 1)) * sj;
+      if constexpr (NTL)
+        ld_nt<C>(M, e, mr[u], mi[u]);
+      else
+        ld<C>(M, e, mr[u], mi[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      if ((uint32_t)u < rem) {
+        const uint32_t j = j0 + u;
+        const double sr = mr[u], si = CONJ ? -mi[u] : mi[u];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const double xr = xs[v * 2 * XR + 2 * j];
+          const double xi = xs[v * 2 * XR + 2 * j + 1];
+          ar[v] = fma(sr, xr, ar[v]);
+          if constexpr (C) {
+            ar[v] = fma(-si, xi, ar[v]);
+            ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
+          }
+        }
+      }
     }
   }
 }
@@ -625,7 +683,6 @@ template <int C, bool TRANS, int NV, bool MF = false, bool ORD = false,
 __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
                                                 double *xs) {
   constexpr int XR = NV == 1 ? MAXRANK : 64;  // inputs per LDS vector
-  constexpr int UNR = NV == 1 ? 8 : 4;
   const int lane = threadIdx.x & 63;
   if (w < a.nfar) {
     const FarD b = a.far[w];
@@ -722,24 +779,9 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
         dot_row<C, true>(a.S + (C ? 2 : 1) * (b.soff + i), r.rank, in.rank,
                          xs, ar[0], ai[0]);
       } else {
-#pragma unroll UNR
-        for (uint32_t j = 0; j < in.rank; j++) {
-          const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * r.rank
-                                   : (uint64_t)i + (uint64_t)j * r.rank;
-          double sr, si;
-          ld_nt<C>(a.S, b.soff + e, sr, si);
-          if (TRANS) si = -si;
-#pragma unroll
-          for (int v = 0; v < NV; v++) {
-            const double xr = xs[v * 2 * XR + 2 * j];
-            const double xi = xs[v * 2 * XR + 2 * j + 1];
-            ar[v] = fma(sr, xr, ar[v]);
-            if constexpr (C) {
-              ar[v] = fma(-si, xi, ar[v]);
-              ai[v] = fma(sr, xi, fma(si, xr, ai[v]));
-            }
-          }
-        }
+        dot_cols_nv<C, NV, XR, true, TRANS>(
+            a.S, b.soff + (TRANS ? (uint64_t)i * r.rank : (uint64_t)i),
+            TRANS ? 1 : r.rank, in.rank, xs, ar, ai);
       }
 #pragma unroll
       for (int v = 0; v < NV; v++)
@@ -787,24 +829,9 @@ __device__ __forceinline__ void couple_near_dev(const CNArgs &a, uint32_t w,
         dot_row<C>(a.N + (C ? 2 : 1) * (b.noff + i), b.rows, nin, xs, ar[0],
                    ai[0]);
     } else {
-#pragma unroll UNR
-      for (uint32_t j = 0; j < nin; j++) {
-        const uint64_t e = TRANS ? (uint64_t)j + (uint64_t)i * b.rows
-                                 : (uint64_t)i + (uint64_t)j * b.rows;
-        double nr, ni;
-        ld<C>(a.N, b.noff + e, nr, ni);
-        if (TRANS) ni = -ni;
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          const double xr = xs[v * 2 * XR + 2 * j];
-          const double xi = xs[v * 2 * XR + 2 * j + 1];
-          ar[v] = fma(nr, xr, ar[v]);
-          if constexpr (C) {
-            ar[v] = fma(-ni, xi, ar[v]);
-            ai[v] = fma(nr, xi, fma(ni, xr, ai[v]));
-          }
-        }
-      }
+      dot_cols_nv<C, NV, XR, false, TRANS>(
+          a.N, b.noff + (TRANS ? (uint64_t)i * b.rows : (uint64_t)i),
+          TRANS ? 1 : b.rows, nin, xs, ar, ai);
     }
 #pragma unroll
     for (int v = 0; v < NV; v++)
@@ -1314,13 +1341,24 @@ __device__ __forceinline__ void dot_strided_nv(const double *__restrict__ M,
         if constexpr (C) ai[v] = fma(m[u], xs[v * 2 * XR + 2 * (i + u) + 1], ai[v]);
       }
   }
-  for (; i < n; i++) {
-    const double m = M[(uint64_t)i * stride];
+  if (i < n) {
+    // the last n - i < B rows, loaded together, then the products in row
+    // order
+    const uint32_t rem = n - i;
+    double m[B];
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      ar[v] = fma(m, xs[v * 2 * XR + 2 * i], ar[v]);
-      if constexpr (C) ai[v] = fma(m, xs[v * 2 * XR + 2 * i + 1], ai[v]);
-    }
+    for (int u = 0; u < B; u++)
+      m[u] = M[(uint64_t)(i + min((uint32_t)u, rem - 1)) * stride];
+#pragma unroll
+    for (int u = 0; u < B; u++)
+      if ((uint32_t)u < rem) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          ar[v] = fma(m[u], xs[v * 2 * XR + 2 * (i + u)], ar[v]);
+          if constexpr (C)
+            ai[v] = fma(m[u], xs[v * 2 * XR + 2 * (i + u) + 1], ai[v]);
+        }
+      }
   }
 }
 
