@@ -106,6 +106,48 @@ def test_device_gmres_matches_restatement(kmax):
     assert np.abs(d_x.download() - want).max() < 1e-8 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("kmax,maxiter", [(10, 1), (10, 2), (30, 9),
+                                          (30, 17), (10, 37), (30, 45)])
+def test_device_gmres_stops_at_maxiter_like_restatement(kmax, maxiter):
+    # a tolerance the solve does not reach: both stop after maxiter - 1
+    # steps (the reference's loop), also where the limit falls inside a
+    # chunk of steps queued ahead of the host's look at the estimates, or
+    # right after a restart; the device's residual estimate is the true
+    # residual of its iterate.  An iterate stopped early depends on the
+    # rounding along the Krylov path (Gram-Schmidt here, Householder in the
+    # reference), which this matrix amplifies (|x| is about 5e3 |b|;
+    # measured after 36 steps of GMRES(10): residuals 4.6e-7 apart,
+    # iterates 1.8e-8): both to 1e-5 of the restatement's
+    import h2lib_amd as h
+    from h2lib_amd import geometry as G
+    from h2lib_amd.h2matrix import H2Matrix, Interpolation
+    from h2lib_amd.solve import solve_gmres
+    from oracle_ffi import (Oracle, OracleBem, oracle_gmres,
+                            oracle_nearfield_packed)
+    mesh = G.sphere(8)
+    rt = G.ClusterTree(mesh, 1, 16)
+    bt = G.BlockTree(rt, rt, 2.0, 0)
+    I = Interpolation(mesh, rt, rt, bt, 1, 1, False, 3, 2)
+    ob = OracleBem(0, 0, 1, 1, mesh.x, mesh.t, q=2, q2=4)
+    N = oracle_nearfield_packed(ob, bt.near_block_list(), rt.idx, rt.idx)
+    H = H2Matrix.from_arrays(I, N, rt=rt)
+    n = rt.n
+    b = np.random.default_rng(7).standard_normal(n)
+    want, it_o, res_o = oracle_gmres(
+        lambda v: Oracle.h2_addeval(I, N, v, np.zeros(n)), n, b, np.zeros(n),
+        1e-15, maxiter, kmax)
+    d_b = h.DeviceArray.from_host(b)
+    d_x = h.DeviceArray.from_host(np.zeros(n))
+    it, res = solve_gmres(H, d_b, d_x, 1e-15, maxiter, kmax)
+    print("maxiter", maxiter, "kmax", kmax, "steps", it, it_o, flush=True)
+    assert it == it_o == maxiter - 1
+    got = d_x.download()
+    true = np.linalg.norm(b - Oracle.h2_addeval(I, N, got, np.zeros(n)))
+    assert abs(true - res) <= 1e-3 * res + 1e-10 * np.linalg.norm(b)
+    assert abs(res - res_o) <= 1e-5 * res_o
+    assert np.abs(got - want).max() <= 1e-5 * max(np.abs(want).max(), 1e-300)
+
+
 def test_gmres_with_more_rows_than_reduction_blocks():
     # 73728 complex unknowns: the update kernels run 288 workgroups of 256
     # rows, more than the 240 of the multi-dot grid, and every one of them
