@@ -409,14 +409,11 @@ __device__ __forceinline__ double cpl_rsqrt(double x) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
-__device__ __forceinline__ void cpl_value(double d0, double d1, double d2,
-                                          int helmholtz, double kr,
-                                          double kdamp, double &sr,
-                                          double &si) {
+// The coupling value from the squared distance r2 (cpl_value)
+__device__ __forceinline__ void cpl_from_r2(double r2, int helmholtz,
+                                            double kr, double kdamp,
+                                            double &sr, double &si) {
   constexpr double KCONST = 0.0795774715459476679;
-  // (the order of the products of r2 is fixed: the unrolled product path
-  // forms the same sums with the first two hoisted)
-  const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
   double rs = cpl_rsqrt(r2);
   if (!helmholtz) {
     sr = KCONST * rs;
@@ -429,6 +426,16 @@ __device__ __forceinline__ void cpl_value(double d0, double d1, double d2,
     sr = KCONST * (rs * cs);
     si = KCONST * (sn * rs);
   }
+}
+
+__device__ __forceinline__ void cpl_value(double d0, double d1, double d2,
+                                          int helmholtz, double kr,
+                                          double kdamp, double &sr,
+                                          double &si) {
+  // (the order of the products of r2 is fixed: the unrolled product paths
+  // form the same sums with the first two hoisted)
+  cpl_from_r2(fma(d2, d2, fma(d1, d1, d0 * d0)), helmholtz, kr, kdamp, sr,
+              si);
 }
 
 // Interpolation data for couplings computed in the product: Chebyshev
@@ -1021,63 +1028,91 @@ __global__ __launch_bounds__(NT) void couple_trans_wg(CNArgs a) {
   }
 }
 
-// Laplace couplings computed in the product at interpolation order 4, one
-// vector, for a symmetric block structure (row and column trees with the
-// same points, every admissible block (r, c) with its mirror (c, r), whose
-// coupling matrix is S_rc^T: the squared distances, and so the values, are
-// the same bit for bit).  One wave per pair r > c evaluates each value once
-// for both blocks: lane i (row point i of r) forms yt_r[i] += sum_j S_ij
-// xt_c[j] with the arithmetic and order of mf_laplace4 (the sums of the
-// block's own wave), and the products S_ij xt_r[i] of 16 columns at a time,
-// whose sums over the lanes, yt_c[j] += sum_i S_ij xt_r[i] for the mirror
-// block, a butterfly forms.  Half the evaluations of the two blocks' waves.
+// Couplings computed in the product at interpolation order 4, one vector,
+// for a symmetric block structure (row and column trees with the same
+// points, every admissible block (r, c) with its mirror (c, r), whose
+// coupling matrix is S_rc^T: the squared distances, and so the values of
+// 1/r or e^{ikr}/r, are the same bit for bit).  One wave per pair r > c
+// evaluates each value once for both blocks: lane i (row point i of r)
+// forms yt_r[i] += sum_j S_ij xt_c[j] with the arithmetic and order of the
+// per-block kernel (mf_laplace4, the general loop for C), and the products
+// S_ij xt_r[i] of a few columns at a time, whose sums over the lanes,
+// yt_c[j] += sum_i S_ij xt_r[i] for the mirror block, a butterfly forms.
+// Half the evaluations of the two blocks' waves.  C: Helmholtz (complex).
+template <int C>
 __global__ __launch_bounds__(NT) void couple_mf_pairs(CNArgs a,
                                                       const PairD *pairs,
                                                       uint32_t npair) {
   constexpr double KCONST = 0.0795774715459476679;
-  __shared__ double xs_all[WPB][64];
+  __shared__ double xs_all[WPB][2 * 64];
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= npair) return;  // uniform over the wave; no block barrier
   const PairD b = pairs[w];
-  const uint64_t out_r = b.out_r, in_c = b.in_c, out_c = b.out_c,
-                 in_r = b.in_r;
   double *xs = xs_all[threadIdx.x >> 6];
-  xs[lane] = a.xt[in_c + lane];  // ranks are 64
-  const double xr = a.xt[in_r + lane];
+  double xrr, xri = 0.0;  // xt_r[lane]
+  if constexpr (C) {
+    xs[2 * lane] = a.xt[2 * (b.in_c + lane)];  // ranks are 64
+    xs[2 * lane + 1] = a.xt[2 * (b.in_c + lane) + 1];
+    xrr = a.xt[2 * (b.in_r + lane)];
+    xri = a.xt[2 * (b.in_r + lane) + 1];
+  } else {
+    xs[lane] = a.xt[b.in_c + lane];
+    xrr = a.xt[b.in_r + lane];
+  }
   __builtin_amdgcn_wave_barrier();
   const double *pr = a.cs.rpts + 12 * (uint64_t)b.r;
   const double *pc = a.cs.cpts + 12 * (uint64_t)b.c;
   const double px = pr[lane >> 4], py = pr[4 + ((lane >> 2) & 3)],
                pz = pr[8 + (lane & 3)];
-  double qy[4], qz[4];
-#pragma unroll
-  for (int t = 0; t < 4; t++) {
-    qy[t] = pc[4 + t];
-    qz[t] = pc[8 + t];
-  }
-  double acc = 0.0;
+  double ar = 0.0, ai = 0.0;
+  // passes of W columns j = (16 ja + 4 jb + jc): the products of a pass
+  // stay in registers until their butterfly (4 for the complex values,
+  // whose sine and cosine take the registers)
+  constexpr int W = C ? 4 : 16;
 #pragma unroll 1
-  for (int ja = 0; ja < 4; ja++) {  // columns 16 ja .. 16 ja + 15
-    const double d0 = px - pc[ja];
-    const double s0 = d0 * d0;
-    double pt[16];
+  for (int p = 0; p < 64 / W; p++) {
+    double pt[W], pti[C ? W : 1];
 #pragma unroll
-    for (int jb = 0; jb < 4; jb++) {
-      const double d1 = py - qy[jb];
-      const double s1 = fma(d1, d1, s0);
-#pragma unroll
-      for (int jc = 0; jc < 4; jc++) {
-        const double d2 = pz - qz[jc];
-        const double sr = KCONST * cpl_rsqrt(fma(d2, d2, s1));
-        acc = fma(sr, xs[16 * ja + 4 * jb + jc], acc);
-        pt[4 * jb + jc] = sr * xr;
+    for (int u = 0; u < W; u++) {
+      const int j = W * p + u;
+      const double d0 = px - pc[j >> 4];
+      const double d1 = py - pc[4 + ((j >> 2) & 3)];
+      const double d2 = pz - pc[8 + (j & 3)];
+      const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+      if constexpr (C) {
+        double sr, si;
+        cpl_from_r2(r2, 1, a.cs.kr, a.cs.kdamp, sr, si);
+        const double xr = xs[2 * j], xi = xs[2 * j + 1];
+        ar = fma(sr, xr, ar);
+        ar = fma(-si, xi, ar);
+        ai = fma(sr, xi, fma(si, xr, ai));
+        pt[u] = fma(sr, xrr, -si * xri);
+        pti[u] = fma(sr, xri, si * xrr);
+      } else {
+        const double sr = KCONST * cpl_rsqrt(r2);
+        ar = fma(sr, xs[j], ar);
+        pt[u] = sr * xrr;
       }
     }
-    const double cs = h2hip::wave_sum16(pt, lane);  // column 16 ja + (lane & 15)
-    if ((lane >> 4) == ja) atomicAdd(a.yt + out_c + lane, cs);
+    // column W p + (lane & (W - 1))
+    const double cs = h2hip::wave_sum_n<W>(pt, lane);
+    if constexpr (C) {
+      const double csi = h2hip::wave_sum_n<W>(pti, lane);
+      if (lane / W == p) {
+        atomicAdd(a.yt + 2 * (b.out_c + lane), cs);
+        atomicAdd(a.yt + 2 * (b.out_c + lane) + 1, csi);
+      }
+    } else {
+      if (lane / W == p) atomicAdd(a.yt + b.out_c + lane, cs);
+    }
   }
-  atomicAdd(a.yt + out_r + lane, acc);
+  if constexpr (C) {
+    atomicAdd(a.yt + 2 * (b.out_r + lane), ar);
+    atomicAdd(a.yt + 2 * (b.out_r + lane) + 1, ai);
+  } else {
+    atomicAdd(a.yt + b.out_r + lane, ar);
+  }
 }
 
 // Self-adjoint product from the lower block triangle (row and column
@@ -2376,13 +2411,13 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
         hipLaunchKernelGGL((couple_near<C, true, NV>), gc, dim3(NT), 0, st,
                            c);
     } else if (mf) {
-      // symmetric structure, real order-4 couplings, one vector, far blocks
+      // symmetric structure, order-4 couplings, one vector, far blocks
       // only: each value once for a block and its mirror
       bool paired = false;
-      if constexpr (!C && NV == 1)
+      if constexpr (NV == 1)
         paired = H->pairs_on && H->npair && c.nnear == 0 && !ord;
       if (paired)
-        hipLaunchKernelGGL(couple_mf_pairs,
+        hipLaunchKernelGGL(couple_mf_pairs<C>,
                            dim3((H->npair + WPB - 1) / WPB), dim3(NT), 0, st,
                            c, (const PairD *)H->d_pairs.p, H->npair);
       else if (ord)
@@ -3236,7 +3271,7 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
   H->cpl_known = true;
   // the pairs of mirrored far blocks, when the structure is symmetric
   H->npair = 0;
-  if (!a.helmholtz && m == 4 && rn == cn &&
+  if (m == 4 && rn == cn &&
       std::memcmp(rpts, cpts, sizeof(double) * 3 * (uint64_t)m * rn) == 0) {
     std::vector<std::pair<uint64_t, uint32_t>> key(H->nfar);
     for (uint32_t f = 0; f < H->nfar; f++)

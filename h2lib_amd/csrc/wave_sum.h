@@ -32,15 +32,20 @@ __device__ __forceinline__ double wave_sum32(double (&v)[32], int lane) {
   return v[0] + __shfl_xor(v[0], 32, 64);
 }
 
-// The same for 16 values: lane l returns the sum of value l & 15.
-__device__ __forceinline__ double wave_sum16(double (&v)[16], int lane) {
-  bfly_step<8>(v, lane);
-  bfly_step<4>(v, lane);
-  bfly_step<2>(v, lane);
-  bfly_step<1>(v, lane);
+// The same for N = 1, 2, 4, .., 32 values: the butterfly over the values,
+// then the sums over the lanes that hold the same one; lane l returns the
+// sum of value l & (N - 1).
+template <int N>
+__device__ __forceinline__ double wave_sum_n(double (&v)[N], int lane) {
+  if constexpr (N >= 32) bfly_step<16>(v, lane);
+  if constexpr (N >= 16) bfly_step<8>(v, lane);
+  if constexpr (N >= 8) bfly_step<4>(v, lane);
+  if constexpr (N >= 4) bfly_step<2>(v, lane);
+  if constexpr (N >= 2) bfly_step<1>(v, lane);
   double s = v[0];
-  s += __shfl_xor(s, 16, 64);
-  return s + __shfl_xor(s, 32, 64);
+#pragma unroll
+  for (int off = N; off < 64; off <<= 1) s += __shfl_xor(s, off, 64);
+  return s;
 }
 
 }  // namespace h2hip

@@ -302,11 +302,10 @@ def test_computed_couplings_equal_stored(kern, layer, basis, m, k):
         got_t = product(H, x, trans=True)
         assert np.abs(got_t - want_t).max() < 1e-13 * np.abs(want_t).max()
     # one tree on both sides: the far blocks come in mirrored pairs, and the
-    # real order-4 couplings are evaluated once for both blocks of a pair
-    # (the flat schedule's forward product); the other products keep one
+    # order-4 couplings are evaluated once for both blocks of a pair (the
+    # flat schedule's forward product); the other products keep one
     # evaluation per block
-    assert H.coupling_pairs() == (H.interp.nfar // 2 if kern == 0 and m == 4
-                                  else 0)
+    assert H.coupling_pairs() == (H.interp.nfar // 2 if m == 4 else 0)
     H.set_schedule(2)
     H.set_coupling_pairs(False)
     off = product(H, x, alpha=alpha)
