@@ -541,14 +541,19 @@ def bench_helmholtz_solve(h, args, stream):
                                          precision=prec, stream=stream)
             stream.sync()
             build = time.perf_counter() - t0
-            # the single layer products compute their couplings (measured
-            # 1.47 against 1.52 ms reading them, equal to rounding)
-            try:
-                prob.V.set_coupling_mode(1)
-                mode = "computed"
-            except h.H2HipError as exc:
-                log("bench: stored Helmholtz couplings: %s" % exc)
-                mode = "stored"
+            # the products compute their couplings from the interpolation
+            # points, each value once for a block and its mirror (0.93 ms
+            # per single layer product at k = 2 against 1.30 ms once per
+            # block; the Brakhage-Werner operator's double layer too)
+            mode = {}
+            for nm, H in (("V", prob.V),) + ((("KM", prob.KM),) if bw
+                                              else ()):
+                try:
+                    H.set_coupling_mode(1)
+                    mode[nm] = "computed"
+                except h.H2HipError as exc:
+                    log("bench: stored Helmholtz couplings: %s" % exc)
+                    mode[nm] = "stored"
             t0 = time.perf_counter()
             if bw:
                 d_b = prob.rhs(x0, stream)
@@ -565,7 +570,7 @@ def bench_helmholtz_solve(h, args, stream):
             dt = time.perf_counter() - t0
             sol[name] = x
             res[name] = {"solve_s": dt, "steps": int(it),
-                         "single_layer_couplings": mode,
+                         "couplings": mode,
                          "operator_build_s": build}
             if bw:
                 res[name]["outer_potential_err"] = prob.outer_error(d_x, x0)

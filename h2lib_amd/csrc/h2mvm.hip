@@ -409,6 +409,54 @@ __device__ __forceinline__ double cpl_rsqrt(double x) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
+// sin and cos of x: Cody and Waite's reduction by pi/2 in three parts
+// (n = rint(2x / pi), r = x - n pi/2 with fused products) to |r| <= pi/4,
+// then the Taylor polynomials of sin r and cos r to degree 21 and 20 in
+// Horner form (coefficients (-1)^k / k!).  Absolute error below
+// 0.61 * 2^-52 for |x| < 2^45 (the same operations in C with fma, against
+// long-double sinl / cosl on 4e6 arguments, a quarter of them next to
+// multiples of pi/2); beyond, where k r of a coupling never comes, the
+// error grows with the rounding of 2x / pi; NaN and infinities give NaN.
+// No branches, so the evaluations of the coupling kernels stay branch-free
+// (the library's sincos carries a large-argument path).
+__device__ __forceinline__ void sincos_cw(double x, double &s, double &c) {
+  constexpr double TWO_OVER_PI = 0.6366197723675814;
+  constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17,
+                   P3 = -1.4973849048591698e-33;
+  const double n = rint(x * TWO_OVER_PI);
+  double r = fma(-n, P1, x);
+  r = fma(-n, P2, r);
+  r = fma(-n, P3, r);
+  const double r2 = r * r;
+  double ps = 1.0 / 51090942171709440000.0;  // 1 / 21!
+  ps = fma(ps, r2, -1.0 / 121645100408832000.0);
+  ps = fma(ps, r2, 1.0 / 355687428096000.0);
+  ps = fma(ps, r2, -1.0 / 1307674368000.0);
+  ps = fma(ps, r2, 1.0 / 6227020800.0);
+  ps = fma(ps, r2, -1.0 / 39916800.0);
+  ps = fma(ps, r2, 1.0 / 362880.0);
+  ps = fma(ps, r2, -1.0 / 5040.0);
+  ps = fma(ps, r2, 1.0 / 120.0);
+  ps = fma(ps, r2, -1.0 / 6.0);
+  const double sr = fma(r * r2, ps, r);
+  double pc = 1.0 / 2432902008176640000.0;  // 1 / 20!
+  pc = fma(pc, r2, -1.0 / 6402373705728000.0);
+  pc = fma(pc, r2, 1.0 / 20922789888000.0);
+  pc = fma(pc, r2, -1.0 / 87178291200.0);
+  pc = fma(pc, r2, 1.0 / 479001600.0);
+  pc = fma(pc, r2, -1.0 / 3628800.0);
+  pc = fma(pc, r2, 1.0 / 40320.0);
+  pc = fma(pc, r2, -1.0 / 720.0);
+  pc = fma(pc, r2, 1.0 / 24.0);
+  pc = fma(pc, r2, -0.5);
+  const double cr = fma(r2, pc, 1.0);
+  // quadrant n mod 4: (sin, cos) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr)
+  const int q = (int)(long long)n & 3;
+  const double s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+  s = (q & 2) ? -s0 : s0;
+  c = ((q + 1) & 2) ? -c0 : c0;
+}
+
 // The coupling value from the squared distance r2 (cpl_value)
 __device__ __forceinline__ void cpl_from_r2(double r2, int helmholtz,
                                             double kr, double kdamp,
@@ -422,7 +470,7 @@ __device__ __forceinline__ void cpl_from_r2(double r2, int helmholtz,
     const double r = r2 * rs;
     if (kdamp != 0.0) rs *= exp(kdamp * r);
     double sn, cs;
-    sincos(kr * r, &sn, &cs);
+    sincos_cw(kr * r, sn, cs);
     sr = KCONST * (rs * cs);
     si = KCONST * (sn * rs);
   }
@@ -1028,90 +1076,125 @@ __global__ __launch_bounds__(NT) void couple_trans_wg(CNArgs a) {
   }
 }
 
-// Couplings computed in the product at interpolation order 4, one vector,
-// for a symmetric block structure (row and column trees with the same
-// points, every admissible block (r, c) with its mirror (c, r), whose
+// Couplings computed in the product at interpolation order M = 4 or 5, one
+// vector, for a symmetric block structure (row and column trees with the
+// same points, every admissible block (r, c) with its mirror (c, r), whose
 // coupling matrix is S_rc^T: the squared distances, and so the values of
 // 1/r or e^{ikr}/r, are the same bit for bit).  One wave per pair r > c
-// evaluates each value once for both blocks: lane i (row point i of r)
-// forms yt_r[i] += sum_j S_ij xt_c[j] with the arithmetic and order of the
-// per-block kernel (mf_laplace4, the general loop for C), and the products
-// S_ij xt_r[i] of a few columns at a time, whose sums over the lanes,
-// yt_c[j] += sum_i S_ij xt_r[i] for the mirror block, a butterfly forms.
-// Half the evaluations of the two blocks' waves.  C: Helmholtz (complex).
-template <int C>
+// evaluates each value once for both blocks: lane l keeps the sums of row
+// points l and l + 64 (rank M^3 <= 128) of block (r, c), yt_r[i] += sum_j
+// S_ij xt_c[j], with the arithmetic and order of the per-block kernel
+// (mf_laplace4, the general loop otherwise), and the products S_ij xt_r[i]
+// of W columns at a time, whose sums over the rows, yt_c[j] += sum_i S_ij
+// xt_r[i] for the mirror block, a butterfly over the lanes forms.  Half the
+// evaluations of the two blocks' waves.  C: Helmholtz (complex).
+template <int C, int M>
 __global__ __launch_bounds__(NT) void couple_mf_pairs(CNArgs a,
                                                       const PairD *pairs,
                                                       uint32_t npair) {
   constexpr double KCONST = 0.0795774715459476679;
-  __shared__ double xs_all[WPB][2 * 64];
+  constexpr int R = M * M * M, RH = (R + 63) / 64;
+  // columns per butterfly: the products of a pass stay in registers until
+  // then (fewer for the complex values, whose sine and cosine take the
+  // registers)
+  constexpr int W = C ? (RH > 1 ? 2 : 4) : 16;
+  constexpr int NP = (R + W - 1) / W;
+  __shared__ double xs_all[WPB][2 * R];
   const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= npair) return;  // uniform over the wave; no block barrier
   const PairD b = pairs[w];
   double *xs = xs_all[threadIdx.x >> 6];
-  double xrr, xri = 0.0;  // xt_r[lane]
-  if constexpr (C) {
-    xs[2 * lane] = a.xt[2 * (b.in_c + lane)];  // ranks are 64
-    xs[2 * lane + 1] = a.xt[2 * (b.in_c + lane) + 1];
-    xrr = a.xt[2 * (b.in_r + lane)];
-    xri = a.xt[2 * (b.in_r + lane) + 1];
-  } else {
-    xs[lane] = a.xt[b.in_c + lane];
-    xrr = a.xt[b.in_r + lane];
+  for (int e = lane; e < R; e += 64) {
+    if constexpr (C) {
+      xs[2 * e] = a.xt[2 * (b.in_c + e)];
+      xs[2 * e + 1] = a.xt[2 * (b.in_c + e) + 1];
+    } else {
+      xs[e] = a.xt[b.in_c + e];
+    }
+  }
+  // the lane's row points and their inputs for the mirror block
+  double xrr[RH], xri[RH], px[RH], py[RH], pz[RH];
+  const double *pr = a.cs.rpts + 3 * M * (uint64_t)b.r;
+  const double *pc = a.cs.cpts + 3 * M * (uint64_t)b.c;
+#pragma unroll
+  for (int h = 0; h < RH; h++) {
+    const int i = lane + 64 * h;
+    const int ii = i < R ? i : R - 1;  // rows past the rank: never stored
+    xrr[h] = xri[h] = 0.0;
+    if (i < R) {
+      if constexpr (C) {
+        xrr[h] = a.xt[2 * (b.in_r + i)];
+        xri[h] = a.xt[2 * (b.in_r + i) + 1];
+      } else {
+        xrr[h] = a.xt[b.in_r + i];
+      }
+    }
+    px[h] = pr[ii / (M * M)];
+    py[h] = pr[M + (ii / M) % M];
+    pz[h] = pr[2 * M + ii % M];
   }
   __builtin_amdgcn_wave_barrier();
-  const double *pr = a.cs.rpts + 12 * (uint64_t)b.r;
-  const double *pc = a.cs.cpts + 12 * (uint64_t)b.c;
-  const double px = pr[lane >> 4], py = pr[4 + ((lane >> 2) & 3)],
-               pz = pr[8 + (lane & 3)];
-  double ar = 0.0, ai = 0.0;
-  // passes of W columns j = (16 ja + 4 jb + jc): the products of a pass
-  // stay in registers until their butterfly (4 for the complex values,
-  // whose sine and cosine take the registers)
-  constexpr int W = C ? 4 : 16;
+  double ar[RH], ai[RH];
+#pragma unroll
+  for (int h = 0; h < RH; h++) ar[h] = ai[h] = 0.0;
 #pragma unroll 1
-  for (int p = 0; p < 64 / W; p++) {
+  for (int p = 0; p < NP; p++) {
     double pt[W], pti[C ? W : 1];
 #pragma unroll
     for (int u = 0; u < W; u++) {
+      pt[u] = 0.0;
+      if constexpr (C) pti[u] = 0.0;
       const int j = W * p + u;
-      const double d0 = px - pc[j >> 4];
-      const double d1 = py - pc[4 + ((j >> 2) & 3)];
-      const double d2 = pz - pc[8 + (j & 3)];
-      const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
-      if constexpr (C) {
-        double sr, si;
-        cpl_from_r2(r2, 1, a.cs.kr, a.cs.kdamp, sr, si);
-        const double xr = xs[2 * j], xi = xs[2 * j + 1];
-        ar = fma(sr, xr, ar);
-        ar = fma(-si, xi, ar);
-        ai = fma(sr, xi, fma(si, xr, ai));
-        pt[u] = fma(sr, xrr, -si * xri);
-        pti[u] = fma(sr, xri, si * xrr);
-      } else {
-        const double sr = KCONST * cpl_rsqrt(r2);
-        ar = fma(sr, xs[j], ar);
-        pt[u] = sr * xrr;
+      if (j < R) {  // uniform
+        const double qx = pc[j / (M * M)], qy = pc[M + (j / M) % M],
+                     qz = pc[2 * M + j % M];
+#pragma unroll
+        for (int h = 0; h < RH; h++) {
+          const double d0 = px[h] - qx, d1 = py[h] - qy, d2 = pz[h] - qz;
+          const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+          if constexpr (C) {
+            double sr, si;
+            cpl_from_r2(r2, 1, a.cs.kr, a.cs.kdamp, sr, si);
+            const double xr = xs[2 * j], xi = xs[2 * j + 1];
+            ar[h] = fma(sr, xr, ar[h]);
+            ar[h] = fma(-si, xi, ar[h]);
+            ai[h] = fma(sr, xi, fma(si, xr, ai[h]));
+            pt[u] += fma(sr, xrr[h], -si * xri[h]);
+            pti[u] += fma(sr, xri[h], si * xrr[h]);
+          } else {
+            const double sr = KCONST * cpl_rsqrt(r2);
+            ar[h] = fma(sr, xs[j], ar[h]);
+            pt[u] = fma(sr, xrr[h], pt[u]);
+          }
+        }
       }
     }
-    // column W p + (lane & (W - 1))
+    // column W p + (lane & (W - 1)), written by the W lanes of that index
     const double cs = h2hip::wave_sum_n<W>(pt, lane);
+    const int col = W * p + (lane & (W - 1));
+    const bool writer = lane / W == p % (64 / W) && col < R;
     if constexpr (C) {
       const double csi = h2hip::wave_sum_n<W>(pti, lane);
-      if (lane / W == p) {
-        atomicAdd(a.yt + 2 * (b.out_c + lane), cs);
-        atomicAdd(a.yt + 2 * (b.out_c + lane) + 1, csi);
+      if (writer) {
+        atomicAdd(a.yt + 2 * (b.out_c + col), cs);
+        atomicAdd(a.yt + 2 * (b.out_c + col) + 1, csi);
       }
     } else {
-      if (lane / W == p) atomicAdd(a.yt + b.out_c + lane, cs);
+      if (writer) atomicAdd(a.yt + b.out_c + col, cs);
     }
   }
-  if constexpr (C) {
-    atomicAdd(a.yt + 2 * (b.out_r + lane), ar);
-    atomicAdd(a.yt + 2 * (b.out_r + lane) + 1, ai);
-  } else {
-    atomicAdd(a.yt + b.out_r + lane, ar);
+#pragma unroll
+  for (int h = 0; h < RH; h++) {
+    const int i = lane + 64 * h;
+    if (i < R) {
+      if constexpr (C) {
+        atomicAdd(a.yt + 2 * (b.out_r + i), ar[h]);
+        atomicAdd(a.yt + 2 * (b.out_r + i) + 1, ai[h]);
+      } else {
+        atomicAdd(a.yt + b.out_r + i, ar[h]);
+      }
+    }
   }
 }
 
@@ -2411,15 +2494,18 @@ void launch(const VecSet &vs, const h2hip_h2matrix *H, double *xt, double *yt,
         hipLaunchKernelGGL((couple_near<C, true, NV>), gc, dim3(NT), 0, st,
                            c);
     } else if (mf) {
-      // symmetric structure, order-4 couplings, one vector, far blocks
-      // only: each value once for a block and its mirror
+      // symmetric structure, order-4 or -5 couplings, one vector, far
+      // blocks only: each value once for a block and its mirror
       bool paired = false;
       if constexpr (NV == 1)
         paired = H->pairs_on && H->npair && c.nnear == 0 && !ord;
-      if (paired)
-        hipLaunchKernelGGL(couple_mf_pairs<C>,
-                           dim3((H->npair + WPB - 1) / WPB), dim3(NT), 0, st,
-                           c, (const PairD *)H->d_pairs.p, H->npair);
+      const dim3 gp((H->npair + WPB - 1) / WPB);
+      if (paired && H->cpl.m == 4)
+        hipLaunchKernelGGL((couple_mf_pairs<C, 4>), gp, dim3(NT), 0, st, c,
+                           (const PairD *)H->d_pairs.p, H->npair);
+      else if (paired)
+        hipLaunchKernelGGL((couple_mf_pairs<C, 5>), gp, dim3(NT), 0, st, c,
+                           (const PairD *)H->d_pairs.p, H->npair);
       else if (ord)
         hipLaunchKernelGGL((couple_near<C, false, NV, true, true>), gc,
                            dim3(NT), 0, st, c);
@@ -3271,7 +3357,7 @@ int h2hip_h2matrix_interp_couplings(h2hip_h2matrix *H, int kernel,
   H->cpl_known = true;
   // the pairs of mirrored far blocks, when the structure is symmetric
   H->npair = 0;
-  if (m == 4 && rn == cn &&
+  if ((m == 4 || m == 5) && rn == cn &&
       std::memcmp(rpts, cpts, sizeof(double) * 3 * (uint64_t)m * rn) == 0) {
     std::vector<std::pair<uint64_t, uint32_t>> key(H->nfar);
     for (uint32_t f = 0; f < H->nfar; f++)

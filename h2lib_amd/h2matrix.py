@@ -369,8 +369,8 @@ class H2Matrix:
         self.mfma = int(bool(on))
 
     def set_coupling_pairs(self, on):
-        """Computed couplings of a symmetric order-4 structure (Laplace or
-        Helmholtz): each value evaluated once for a block and its mirror
+        """Computed couplings of a symmetric order-4 or -5 structure (Laplace
+        or Helmholtz): each value evaluated once for a block and its mirror
         (the default)."""
         check(hip().h2hip_h2matrix_set_coupling_pairs(self._h, int(bool(on))),
               "h2hip_h2matrix_set_coupling_pairs")

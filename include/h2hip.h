@@ -254,7 +254,7 @@ int h2hip_h2matrix_set_adjoint_storage(h2hip_h2matrix *H, int keep);
  * v_mfma_f64_16x16x4_f64 per coupling block.  Other products, computed
  * couplings and reproducible mode keep the vector kernels. */
 int h2hip_h2matrix_set_mfma(h2hip_h2matrix *H, int on);
-/* Couplings computed in the product (coupling mode 1) of an order-4
+/* Couplings computed in the product (coupling mode 1) of an order-4 or -5
  * interpolation matrix (Laplace or Helmholtz) whose row and column boxes
  * have the same points and whose admissible blocks come in mirrored pairs
  * (r, c) / (c, r), as for the operators of one mesh: the one-vector
@@ -264,7 +264,7 @@ int h2hip_h2matrix_set_mfma(h2hip_h2matrix *H, int on);
  * block. */
 int h2hip_h2matrix_set_coupling_pairs(h2hip_h2matrix *H, int on);
 /* The number of mirrored pairs found by h2hip_h2matrix_interp_couplings
- * (0: the structure is not symmetric, or not order 4; -1: H null). */
+ * (0: the structure is not symmetric, or not order 4 or 5; -1: H null). */
 int h2hip_h2matrix_coupling_pairs(const h2hip_h2matrix *H);
 /* Couplings of an interpolation H2-matrix computed on the device (instead
  * of h2hip_h2matrix_set_couplings): kernel_const G(xi, eta) at the tensor

@@ -266,7 +266,9 @@ def test_flat_transforms_equal_counters(kern, layer, basis, m):
 @pytest.mark.parametrize("kern,layer,basis,m,k", [(0, 0, 1, 4, 0.0),
                                                   (0, 1, 1, 3, 0.0),
                                                   (1, 0, 0, 4, 2.0),
-                                                  (1, 1, 0, 3, 3.0 + 0.5j)])
+                                                  (1, 1, 0, 3, 3.0 + 0.5j),
+                                                  (0, 1, 1, 5, 0.0),
+                                                  (1, 1, 0, 5, 3.0 + 0.5j)])
 def test_computed_couplings_equal_stored(kern, layer, basis, m, k):
     # products that compute the couplings from the interpolation points
     # equal those that read them (same arithmetic for every entry), with
@@ -302,10 +304,10 @@ def test_computed_couplings_equal_stored(kern, layer, basis, m, k):
         got_t = product(H, x, trans=True)
         assert np.abs(got_t - want_t).max() < 1e-13 * np.abs(want_t).max()
     # one tree on both sides: the far blocks come in mirrored pairs, and the
-    # order-4 couplings are evaluated once for both blocks of a pair (the
-    # flat schedule's forward product); the other products keep one
+    # order-4 and -5 couplings are evaluated once for both blocks of a pair
+    # (the flat schedule's forward product); the other products keep one
     # evaluation per block
-    assert H.coupling_pairs() == (H.interp.nfar // 2 if m == 4 else 0)
+    assert H.coupling_pairs() == (H.interp.nfar // 2 if m in (4, 5) else 0)
     H.set_schedule(2)
     H.set_coupling_pairs(False)
     off = product(H, x, alpha=alpha)
