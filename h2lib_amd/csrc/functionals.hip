@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "dmath.h"
 #include "h2hip_internal.h"
 
 namespace {
@@ -493,7 +494,7 @@ __device__ __forceinline__ void helm_point(const HelmArgs &a,
   const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
   const double rs = 1.0 / sqrt(r2), r = r2 * rs;
   double sn, cs;
-  sincos(a.k * r, &sn, &cs);
+  h2hip::sincos_cw(a.k * r, sn, cs);
   re = cs * rs;
   im = sn * rs;
   if (a.kind == 1) {
@@ -757,7 +758,7 @@ __global__ __launch_bounds__(BW_NT) void bw_potential(BWArgs a) {
       const double rs3 = rs * rs * rs;
       // e^{ikr} = e^{-ki r} (cos kr r + i sin kr r)
       double sn, cs;
-      sincos(a.kr * r, &sn, &cs);
+      h2hip::sincos_cw(a.kr * r, sn, cs);
       const double damp = a.ki != 0.0 ? exp(-a.ki * r) : 1.0;
       const double ere = damp * cs, eim = damp * sn;
       const double dn = dx * nx + dy * ny + dz * nz;

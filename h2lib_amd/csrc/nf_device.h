@@ -24,6 +24,7 @@
 
 #include <algorithm>
 
+#include "dmath.h"
 #include "h2hip_internal.h"
 
 namespace h2hip {
@@ -66,7 +67,7 @@ __device__ __forceinline__ void kval(double dx, double dy, double dz,
     double amp = rs;
     if (ki != 0.0) amp *= exp(ki * r);
     double s, c;
-    sincos(kr * r, &s, &c);
+    h2hip::sincos_cw(kr * r, s, c);
     re = amp * c;
     im = amp * s;
   } else {
@@ -75,7 +76,7 @@ __device__ __forceinline__ void kval(double dx, double dy, double dz,
     if (ki != 0.0) amp *= exp(ki * r);
     const double kr_r = kr * r;
     double s, c;
-    sincos(kr_r, &s, &c);
+    h2hip::sincos_cw(kr_r, s, c);
     re = fma(kr_r, s, c) * amp;
     im = fma(-c, kr_r, s) * amp;
   }
