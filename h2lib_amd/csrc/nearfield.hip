@@ -250,6 +250,26 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
                   [&](unsigned t, uint32_t lo, uint32_t hi) {
     Part &W = parts[t];
     std::vector<int32_t> pos(s->nv, -1), tpos(s->nt, -1);
+    // A tile side's support list depends only on its index range, and a
+    // cluster takes part in many near blocks: each range's list is built
+    // once per thread and shared by the tiles that use it (key: offset in
+    // the index array and count; rows and columns apart).
+    using Memo = std::unordered_map<uint64_t, std::pair<uint32_t, uint32_t>>;
+    Memo rmemo, cmemo;
+    auto side = [&](Memo &memo, int basis, const uint32_t *all, uint32_t off,
+                    uint32_t cnt, uint32_t &n) {
+      const uint64_t key = (uint64_t)off << 32 | cnt;
+      auto it = memo.find(key);
+      if (it != memo.end()) {
+        n = it->second.second;
+        return it->second.first;
+      }
+      const uint32_t s0 = (uint32_t)W.supp.size();
+      tile_side(s, basis, all + off, cnt, pos, tpos, W.supp);
+      n = (uint32_t)W.supp.size() - s0;
+      memo.emplace(key, std::make_pair(s0, n));
+      return s0;
+    };
     for (uint32_t b = lo; b < hi; b++) {
       const uint32_t roff = blocks[4 * b], rows = blocks[4 * b + 1];
       const uint32_t coff = blocks[4 * b + 2], cols = blocks[4 * b + 3];
@@ -269,14 +289,12 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
       const uint32_t ld = bnt ? cols : rows;
       for (uint32_t r0 = 0; r0 < rows; r0 += tile) {
         const uint32_t nr = std::min<uint32_t>(tile, rows - r0);
-        const uint32_t rs0 = (uint32_t)W.supp.size();
-        tile_side(s, rb, ri + r0, nr, pos, tpos, W.supp);
-        const uint32_t nrs = (uint32_t)W.supp.size() - rs0;
+        uint32_t nrs;
+        const uint32_t rs0 = side(rmemo, rb, ridx_all, roff + r0, nr, nrs);
         for (uint32_t c0 = 0; c0 < cols; c0 += tile) {
           const uint32_t nc = std::min<uint32_t>(tile, cols - c0);
-          const uint32_t cs0 = (uint32_t)W.supp.size();
-          tile_side(s, cb, ci + c0, nc, pos, tpos, W.supp);
-          const uint32_t ncs = (uint32_t)W.supp.size() - cs0;
+          uint32_t ncs;
+          const uint32_t cs0 = side(cmemo, cb, cidx_all, coff + c0, nc, ncs);
           if (nrs > 0xffff || ncs > 0xffff ||
               (uint64_t)nrs * ncs >= (1ull << 31)) {
             W.err = "tile support too large";
@@ -306,23 +324,43 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
       }
       // cc identity terms the reference leaves at (t, t) with t != s
       if (!rb && !cb) {
-        std::unordered_map<uint32_t, std::vector<uint32_t>> where;
-        for (uint32_t j = 0; j < cols; j++) where[ci[j]].push_back(j);
-        for (uint32_t i = 0; i < rows; i++) {
-          auto it = where.find(ri[i]);
-          if (it == where.end()) continue;
-          for (uint32_t j : it->second) {
-            // i == j: added with the entry itself.  The reference visits
-            // the columns in order and overwrites (t, t) when it computes
-            // column t, so a term added from column s < t does not
-            // survive; only s > t leaves it in place (and keeps (t, t)
-            // inside the window).
-            if (j <= i) continue;
-            Fixup f;
-            f.pos = boff + (uint64_t)i + (uint64_t)i * ld;
-            f.tri = ri[i];
-            f.pad = (uint32_t)bnt;  // the block's ntrans
-            W.fix.push_back(f);
+        // i == j: added with the entry itself.  The reference visits the
+        // columns in order and overwrites (t, t) when it computes column
+        // t, so a term added from column s < t does not survive; only
+        // s > t leaves it in place (and keeps (t, t) inside the window).
+        auto fixup = [&](uint32_t i, uint32_t j) {
+          if (j <= i) return;
+          Fixup f;
+          f.pos = boff + (uint64_t)i + (uint64_t)i * ld;
+          f.tri = ri[i];
+          f.pad = (uint32_t)bnt;  // the block's ntrans
+          W.fix.push_back(f);
+        };
+        // the column of each triangle through the thread's triangle map;
+        // a block that repeats a column index takes a map of lists
+        bool unique = true;
+        uint32_t set = 0;
+        for (; set < cols; set++) {
+          if (tpos[ci[set]] >= 0) {
+            unique = false;
+            break;
+          }
+          tpos[ci[set]] = (int32_t)set;
+        }
+        if (unique) {
+          for (uint32_t i = 0; i < rows; i++) {
+            const int32_t j = tpos[ri[i]];
+            if (j >= 0) fixup(i, (uint32_t)j);
+          }
+        }
+        for (uint32_t j = 0; j < set; j++) tpos[ci[j]] = -1;
+        if (!unique) {
+          std::unordered_map<uint32_t, std::vector<uint32_t>> where;
+          for (uint32_t j = 0; j < cols; j++) where[ci[j]].push_back(j);
+          for (uint32_t i = 0; i < rows; i++) {
+            auto it = where.find(ri[i]);
+            if (it == where.end()) continue;
+            for (uint32_t j : it->second) fixup(i, j);
           }
         }
       }
@@ -423,23 +461,45 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
       return H2HIP_EINVAL;
     }
   }
-  for (uint32_t part = 0, ti = 0; part < sparts.size(); part++) {
-    const uint32_t hi =
-        (uint32_t)((uint64_t)ntiles * (part + 1) / sparts.size());
-    const uint32_t base = (uint32_t)P->tpairs.size();
-    for (; ti < hi; ti++) P->tiles[ti].sbeg += base;
-    P->tpairs.insert(P->tpairs.end(), sparts[part].tp.begin(),
-                     sparts[part].tp.end());
+  // where each part's pairs go: the tile-grouped list in part order; the
+  // class lists (identical, edge, vertex) one after the other, each in part
+  // order; then the parts copy their pieces at the same time
+  const unsigned np = (unsigned)sparts.size();
+  std::vector<size_t> tdst(np), cdst(4 * (size_t)np);
+  size_t ntp = 0, nsp = 0;
+  for (unsigned part = 0; part < np; part++) {
+    tdst[part] = ntp;
+    ntp += sparts[part].tp.size();
     P->pairs[0] += sparts[part].pairs0;
   }
   for (int c = 3; c >= 1; c--) {
-    for (auto &W : sparts) {
-      P->pairs[c] += W.by_cls[c].size();
-      P->spairs.insert(P->spairs.end(), W.by_cls[c].begin(),
-                       W.by_cls[c].end());
+    for (unsigned part = 0; part < np; part++) {
+      cdst[4 * part + c] = nsp;
+      nsp += sparts[part].by_cls[c].size();
+      P->pairs[c] += sparts[part].by_cls[c].size();
     }
     P->pairs[0] -= P->pairs[c];
   }
+  if (ntp > 0xffffffffull) {
+    set_error("too many pairs with common vertices");
+    return H2HIP_EINVAL;
+  }
+  P->tpairs.resize(ntp);
+  P->spairs.resize(nsp);
+  parallel_ranges(np, np, [&](unsigned, uint32_t lo, uint32_t hi) {
+    for (uint32_t part = lo; part < hi; part++) {
+      SPart &W = sparts[part];
+      const uint32_t t0 = (uint32_t)((uint64_t)ntiles * part / np);
+      const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (part + 1) / np);
+      for (uint32_t ti = t0; ti < t1; ti++)
+        P->tiles[ti].sbeg += (uint32_t)tdst[part];
+      std::copy(W.tp.begin(), W.tp.end(), P->tpairs.begin() + tdst[part]);
+      for (int c = 1; c < 4; c++)
+        std::copy(W.by_cls[c].begin(), W.by_cls[c].end(),
+                  P->spairs.begin() + cdst[4 * part + c]);
+      W = SPart();
+    }
+  });
   sparts.clear();
   if (grow) {
     H2HIP_CHECK(P->d_tiles.upload_grow(P->tiles.data(), P->tiles.size(), st));
