@@ -11,6 +11,7 @@ and the caller-owned column-major matrix ``N`` whose window is written.
 All work is done by libh2hip.so on the GPU.
 """
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -63,7 +64,9 @@ class Surface3d:
     def handle(self):
         return self._h
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_surface_destroy(h)
@@ -100,7 +103,9 @@ class Bem3d:
     def handle(self):
         return self._h
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_op_destroy(h)
@@ -212,7 +217,9 @@ class NearfieldPlan:
         check(hip().h2hip_nfplan_offsets(h, _ptr(self.offsets)),
               "h2hip_nfplan_offsets")
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_nfplan_destroy(h)

@@ -1,5 +1,6 @@
 """Device arrays, streams and events through libh2hip.so's C ABI."""
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -60,7 +61,9 @@ class DeviceArray:
             hip().h2hip_free(self._p)
             self._p = C.c_void_p()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         self.free()
 
 
@@ -77,7 +80,9 @@ class Stream:
     def sync(self):
         check(hip().h2hip_stream_sync(self._s), "h2hip_stream_sync")
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         if getattr(self, "_s", None) and self._s.value:
             hip().h2hip_stream_destroy(self._s)
             self._s = C.c_void_p()
@@ -99,7 +104,9 @@ class Event:
               "h2hip_event_elapsed")
         return ms.value
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         if getattr(self, "_e", None) and self._e.value:
             hip().h2hip_event_destroy(self._e)
             self._e = C.c_void_p()

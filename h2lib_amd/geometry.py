@@ -7,6 +7,7 @@ build_from_macrosurface3d_surface3d (Library/macrosurface3d.c:81, 390,
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -111,7 +112,9 @@ class Mesh:
     def triangles(self):
         return len(self.t)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         sp = getattr(self, "_sp", None)
         if sp:
             bem().h2b_surface_free(sp)
@@ -175,7 +178,9 @@ class ClusterTree:
     def nnodes(self):
         return len(self.off)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         tp = getattr(self, "_tp", None)
         if tp:
             bem().h2b_ctree_free(tp)
@@ -204,7 +209,9 @@ class BlockTree:
         return np.stack([self.rt.off[r], self.rt.size[r], self.ct.off[c],
                          self.ct.size[c]], axis=1).astype(np.uint32)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         bp = getattr(self, "_bp", None)
         if bp:
             bem().h2b_blocks_free(bp)

@@ -5,6 +5,7 @@ near field from the HIP quadrature, and the products addeval / addevaltrans
 (Library/h2matrix.c:1268-1286, 1345-1364) in libh2hip.so.
 """
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -184,7 +185,9 @@ class Interpolation:
             out.ctypes.data_as(C.c_void_p))
         return out.view(np.complex128)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         ip = getattr(self, "_ip", None)
         if ip:
             bem().h2b_h2interp_free(ip)
@@ -444,7 +447,9 @@ class H2Matrix:
             C.c_void_p(stream.handle if stream else 0)),
             "h2hip_addeval_h2matrix_batch")
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_h2matrix_destroy(h)

@@ -9,6 +9,7 @@ u(x) = 1/|x - x0|, b = (K + M/2) f with the double layer H2-matrix, V x = b
 with the single layer H2-matrix, error against the Neumann data of u.
 """
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -225,7 +226,9 @@ class Combined:
             C.c_void_p(d_y.ptr), C.c_void_p(stream.handle if stream else 0)),
             "h2hip_addeval_combined")
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_combined_destroy(h)
@@ -368,7 +371,9 @@ class GmresBatch:
     def inflight(self):
         return bool(hip().h2hip_gmres_batch_busy(self._h) & 2)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_gmres_batch_destroy(h)
@@ -416,7 +421,9 @@ class GmresMulti:
     def graph_builds(self):
         return int(hip().h2hip_gmres_multi_graph_builds(self._h))
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
+            return  # interpreter exit: the module globals are gone
         h = getattr(self, "_h", None)
         if h:
             hip().h2hip_gmres_multi_destroy(h)
