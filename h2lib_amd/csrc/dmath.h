@@ -7,9 +7,10 @@ namespace h2hip {
 
 // sin and cos of x: Cody and Waite's reduction by pi/2 in three parts
 // (n = rint(2x / pi), r = x - n pi/2 with fused products) to |r| <= pi/4,
-// then the Taylor polynomials of sin r and cos r to degree 21 and 20 in
-// Horner form (coefficients (-1)^k / k!).  Absolute error below
-// 0.61 * 2^-52 for |x| < 2^45 (the same operations in C with fma, against
+// then the Taylor polynomials of sin r and cos r to degree 17 and 16 in
+// Horner form (coefficients (-1)^k / k!; the first terms left out are
+// 8.4e-20 and 2.0e-18 at most, at |r| = pi/4).  Absolute error below
+// 0.59 * 2^-52 for |x| < 2^45 (the same operations in C with fma, against
 // long-double sinl / cosl on 4e6 arguments, a quarter of them next to
 // multiples of pi/2); beyond, where k r of a coupling never comes, the
 // error grows with the rounding of 2x / pi; NaN and infinities give NaN.
@@ -24,9 +25,7 @@ __device__ __forceinline__ void sincos_cw(double x, double &s, double &c) {
   r = fma(-n, P2, r);
   r = fma(-n, P3, r);
   const double r2 = r * r;
-  double ps = 1.0 / 51090942171709440000.0;  // 1 / 21!
-  ps = fma(ps, r2, -1.0 / 121645100408832000.0);
-  ps = fma(ps, r2, 1.0 / 355687428096000.0);
+  double ps = 1.0 / 355687428096000.0;  // 1 / 17!
   ps = fma(ps, r2, -1.0 / 1307674368000.0);
   ps = fma(ps, r2, 1.0 / 6227020800.0);
   ps = fma(ps, r2, -1.0 / 39916800.0);
@@ -35,9 +34,7 @@ __device__ __forceinline__ void sincos_cw(double x, double &s, double &c) {
   ps = fma(ps, r2, 1.0 / 120.0);
   ps = fma(ps, r2, -1.0 / 6.0);
   const double sr = fma(r * r2, ps, r);
-  double pc = 1.0 / 2432902008176640000.0;  // 1 / 20!
-  pc = fma(pc, r2, -1.0 / 6402373705728000.0);
-  pc = fma(pc, r2, 1.0 / 20922789888000.0);
+  double pc = 1.0 / 20922789888000.0;  // 1 / 16!
   pc = fma(pc, r2, -1.0 / 87178291200.0);
   pc = fma(pc, r2, 1.0 / 479001600.0);
   pc = fma(pc, r2, -1.0 / 3628800.0);
