@@ -408,14 +408,21 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
   parallel_ranges(ntiles, (unsigned)sparts.size(),
                   [&](unsigned t, uint32_t lo, uint32_t hi) {
     SPart &W = sparts[t];
-    std::vector<int32_t> cpos(s->nt, -1);
+    std::vector<int32_t> cpos(s->nt, -1), cnext;
     for (uint32_t ti = lo; ti < hi; ti++) {
       Tile &T = P->tiles[ti];
       W.pairs0 += (uint64_t)T.nrsup * T.ncsup;
       size_t b0[4];
       for (int c = 1; c < 4; c++) b0[c] = W.by_cls[c].size();
-      for (uint32_t c = 0; c < T.ncsup; c++)
-        cpos[P->supp[T.csup + c].tri] = (int32_t)c;
+      // the first column position of each triangle; the further positions
+      // of a triangle that the column indices repeat (constant basis)
+      // chained behind it in increasing order
+      cnext.assign(T.ncsup, -1);
+      for (uint32_t c = T.ncsup; c-- > 0;) {
+        const uint32_t tri = P->supp[T.csup + c].tri;
+        cnext[c] = cpos[tri];
+        cpos[tri] = (int32_t)c;
+      }
       for (uint32_t r = 0; r < T.nrsup; r++) {
         const Supp &sr = P->supp[T.rsup + r];
         uint32_t seen[64];
@@ -424,18 +431,20 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
           const uint32_t v = sr.v[l];
           for (uint32_t p = s->v2t_ptr[v]; p < s->v2t_ptr[v + 1]; p++) {
             const uint32_t tri = s->v2t[p];
-            const int32_t c = cpos[tri];
-            if (c < 0) continue;
+            if (cpos[tri] < 0) continue;
             bool dup = false;
             for (int k = 0; k < nseen; k++) dup |= (seen[k] == tri);
             if (dup) continue;
             if (nseen < 64) seen[nseen++] = tri;
-            SPair q;
-            q.tile = ti;
-            q.r = (uint16_t)r;
-            q.c = (uint16_t)c;
-            q.cls = (uint16_t)classify(sr.v, P->supp[T.csup + c].v, q.perm);
-            W.by_cls[q.cls].push_back(q);
+            for (int32_t c = cpos[tri]; c >= 0; c = cnext[c]) {
+              SPair q;
+              q.tile = ti;
+              q.r = (uint16_t)r;
+              q.c = (uint16_t)c;
+              q.cls =
+                  (uint16_t)classify(sr.v, P->supp[T.csup + c].v, q.perm);
+              W.by_cls[q.cls].push_back(q);
+            }
           }
         }
       }

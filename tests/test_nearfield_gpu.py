@@ -92,6 +92,39 @@ def test_subsets_match_oracle(name, ntrans, sphere8):
             assert rel_block_err(a, b) < TOL, (rows, cols, far)
 
 
+@pytest.mark.parametrize("name", ["lap_dlp_cc", "helm_dlp_cc",
+                                  "lap_adlp_cc"])
+@pytest.mark.parametrize("ntrans", [0, 1])
+def test_repeated_indices_match_oracle(name, ntrans, sphere8):
+    # Index arrays that repeat DOFs: the reference computes such columns
+    # again and adds the constant-basis identity term at (t, t) for every
+    # column s with ridx[t] == cidx[s], where a later column t overwrites
+    # the terms of the columns before it (Library/bem3d.c:1150-1247).
+    g = gpu_op(sphere8, name)
+    o = oracle_op(sphere8, name)
+    rng = np.random.default_rng(13)
+    nt = len(sphere8["t"])
+    base = rng.permutation(nt)[:20]
+    cidx = np.concatenate([base, base[[3, 0, 7, 3, 11]]]).astype(np.uint32)
+    # rows from the other DOFs: a shared DOF at row position >= cols would
+    # put the reference's identity term outside the window
+    taken = set(int(v) for v in base)
+    ridx = np.array([v for v in rng.permutation(nt) if int(v) not in taken][:30])
+    # shared DOFs at row positions below cols, before and after the
+    # columns that hold them, some of them repeated among the rows too
+    for pos, j in [(0, 0), (2, 3), (5, 21), (9, 7), (14, 3), (20, 11),
+                   (24, 0)]:
+        ridx[pos] = cidx[j]
+    ridx = ridx.astype(np.uint32)
+    cols = len(cidx)
+    assert cols == 25 and len(ridx) == 30
+    for far in (False, True):
+        f = g.nearfield_far if far else g.nearfield
+        a = f(ridx, cidx, ntrans=ntrans)
+        b = o.nearfield(ridx, cidx, far=far, ntrans=ntrans)
+        assert rel_block_err(a, b) < TOL, far
+
+
 def test_window_inside_larger_matrix(sphere8):
     # the callee writes only its window of a caller-owned matrix with ld
     g = gpu_op(sphere8, "lap_dlp_ll")
