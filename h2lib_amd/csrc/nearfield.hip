@@ -132,6 +132,66 @@ void tile_side(const h2hip_surface *s, int basis, const uint32_t *idx,
   for (uint32_t i = 0; i < cnt; i++) pos[idx[i]] = -1;
 }
 
+// Linear-basis tile side of a block whose indices repeat a vertex.  The
+// reference records for each support triangle the first three block
+// positions whose vertex it touches (Library/bem3d.c:3071-3116), so a
+// position past the third gets nothing from that triangle.  For the tile's
+// positions t0 .. t0 + cnt - 1 of the block's index array blk: each
+// triangle once with the first allowed position of each of its vertices,
+// then once more for every further allowed position of a repeated vertex,
+// with only that slot set.  Uses and restores tpos[] (-1) and used[] (0).
+void tile_side_repeated(const h2hip_surface *s, const uint32_t *blk,
+                        uint32_t t0, uint32_t cnt, std::vector<int32_t> &tpos,
+                        std::vector<int32_t> &used, std::vector<Supp> &out) {
+  std::vector<uint32_t> tris;
+  for (uint32_t i = t0; i < t0 + cnt; i++)
+    for (uint32_t p = s->v2t_ptr[blk[i]]; p < s->v2t_ptr[blk[i] + 1]; p++) {
+      const uint32_t tri = s->v2t[p];
+      if (tpos[tri] >= 0) continue;
+      tpos[tri] = (int32_t)tris.size();
+      tris.push_back(tri);
+    }
+  // the block positions before the tile that these triangles record
+  for (uint32_t i = 0; i < t0; i++)
+    for (uint32_t p = s->v2t_ptr[blk[i]]; p < s->v2t_ptr[blk[i] + 1]; p++)
+      if (tpos[s->v2t[p]] >= 0) used[s->v2t[p]]++;
+  std::vector<uint32_t> slots(tris.size(), 0xffffffu);
+  std::vector<std::pair<uint32_t, uint32_t>> extra;  // (triangle, slots)
+  for (uint32_t i = t0; i < t0 + cnt; i++) {
+    const uint32_t v = blk[i];
+    for (uint32_t p = s->v2t_ptr[v]; p < s->v2t_ptr[v + 1]; p++) {
+      const uint32_t tri = s->v2t[p];
+      if (used[tri] >= 3) continue;
+      used[tri]++;
+      int l = 0;
+      while (s->t[3 * tri + l] != v) l++;
+      const uint32_t sh = 8 * l, loc = i - t0;
+      uint32_t &m = slots[tpos[tri]];
+      if (((m >> sh) & 0xffu) == 0xffu)
+        m = (m & ~(0xffu << sh)) | (loc << sh);
+      else
+        extra.emplace_back(tri, (0xffffffu & ~(0xffu << sh)) | (loc << sh));
+    }
+  }
+  auto push = [&](uint32_t tri, uint32_t loc) {
+    Supp e;
+    e.tri = tri;
+    e.loc = loc;
+    e.v[0] = s->t[3 * tri];
+    e.v[1] = s->t[3 * tri + 1];
+    e.v[2] = s->t[3 * tri + 2];
+    e.pad = 0;
+    out.push_back(e);
+  };
+  for (size_t k = 0; k < tris.size(); k++)
+    if (slots[k] != 0xffffffu) push(tris[k], slots[k]);
+  for (auto &e : extra) push(e.first, e.second);
+  for (uint32_t tri : tris) {
+    tpos[tri] = -1;
+    used[tri] = 0;
+  }
+}
+
 // Shared vertices of two triangles and the reference's vertex permutation:
 // shared vertices first in the order of the first triangle, then the
 // remaining ones in their original order (Library/singquad2d.c:549-633).
@@ -256,6 +316,7 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
     // the index array and count; rows and columns apart).
     using Memo = std::unordered_map<uint64_t, std::pair<uint32_t, uint32_t>>;
     Memo rmemo, cmemo;
+    std::vector<int32_t> used;  // per triangle, for sides with repeats
     auto side = [&](Memo &memo, int basis, const uint32_t *all, uint32_t off,
                     uint32_t cnt, uint32_t &n) {
       const uint64_t key = (uint64_t)off << 32 | cnt;
@@ -287,14 +348,36 @@ int build_plan(h2hip_nfplan *P, const h2hip_surface *s, int rb, int cb,
       const uint64_t boff = P->off[b];
       const int bnt = ntr ? (ntr[b] ? 1 : 0) : ntrans;
       const uint32_t ld = bnt ? cols : rows;
+      // a linear-basis side that repeats a vertex follows the reference's
+      // rule of three positions per triangle, tile by tile
+      auto repeats = [&](const uint32_t *ix, uint32_t n) {
+        uint32_t k = 0;
+        for (; k < n && pos[ix[k]] < 0; k++) pos[ix[k]] = 0;
+        for (uint32_t j = 0; j < k; j++) pos[ix[j]] = -1;
+        return k < n;
+      };
+      const bool rrep = rb && repeats(ri, rows);
+      const bool crep = cb && repeats(ci, cols);
+      if ((rrep || crep) && used.empty()) used.assign(s->nt, 0);
+      auto side_rep = [&](const uint32_t *blk, uint32_t t0, uint32_t cnt,
+                          uint32_t &n) {
+        const uint32_t s0 = (uint32_t)W.supp.size();
+        tile_side_repeated(s, blk, t0, cnt, tpos, used, W.supp);
+        n = (uint32_t)W.supp.size() - s0;
+        return s0;
+      };
       for (uint32_t r0 = 0; r0 < rows; r0 += tile) {
         const uint32_t nr = std::min<uint32_t>(tile, rows - r0);
         uint32_t nrs;
-        const uint32_t rs0 = side(rmemo, rb, ridx_all, roff + r0, nr, nrs);
+        const uint32_t rs0 =
+            rrep ? side_rep(ri, r0, nr, nrs)
+                 : side(rmemo, rb, ridx_all, roff + r0, nr, nrs);
         for (uint32_t c0 = 0; c0 < cols; c0 += tile) {
           const uint32_t nc = std::min<uint32_t>(tile, cols - c0);
           uint32_t ncs;
-          const uint32_t cs0 = side(cmemo, cb, cidx_all, coff + c0, nc, ncs);
+          const uint32_t cs0 =
+              crep ? side_rep(ci, c0, nc, ncs)
+                   : side(cmemo, cb, cidx_all, coff + c0, nc, ncs);
           if (nrs > 0xffff || ncs > 0xffff ||
               (uint64_t)nrs * ncs >= (1ull << 31)) {
             W.err = "tile support too large";

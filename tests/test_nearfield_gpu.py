@@ -92,32 +92,43 @@ def test_subsets_match_oracle(name, ntrans, sphere8):
             assert rel_block_err(a, b) < TOL, (rows, cols, far)
 
 
-@pytest.mark.parametrize("name", ["lap_dlp_cc", "helm_dlp_cc",
-                                  "lap_adlp_cc"])
+@pytest.mark.parametrize("name", ["lap_dlp_cc", "helm_dlp_cc", "lap_adlp_cc",
+                                  "lap_slp_ll", "lap_dlp_ll", "helm_slp_ll",
+                                  "lap_dlp_cl", "lap_dlp_lc"])
 @pytest.mark.parametrize("ntrans", [0, 1])
 def test_repeated_indices_match_oracle(name, ntrans, sphere8):
-    # Index arrays that repeat DOFs: the reference computes such columns
-    # again and adds the constant-basis identity term at (t, t) for every
+    # Index arrays that repeat DOFs.  Constant basis: the reference computes
+    # such columns again and adds the identity term at (t, t) for every
     # column s with ridx[t] == cidx[s], where a later column t overwrites
     # the terms of the columns before it (Library/bem3d.c:1150-1247).
+    # Linear basis: a support triangle records the first three block
+    # positions whose vertex it touches (Library/bem3d.c:3071-3116); a
+    # vertex here appears three times, across the 64-row tile edge.
     g = gpu_op(sphere8, name)
     o = oracle_op(sphere8, name)
+    kern, op, br, bc, alpha, k = CASES[name]
     rng = np.random.default_rng(13)
-    nt = len(sphere8["t"])
-    base = rng.permutation(nt)[:20]
+    nr = len(sphere8["x"]) if br else len(sphere8["t"])
+    nc = len(sphere8["x"]) if bc else len(sphere8["t"])
+    base = rng.permutation(nc)[:20]
     cidx = np.concatenate([base, base[[3, 0, 7, 3, 11]]]).astype(np.uint32)
-    # rows from the other DOFs: a shared DOF at row position >= cols would
-    # put the reference's identity term outside the window
-    taken = set(int(v) for v in base)
-    ridx = np.array([v for v in rng.permutation(nt) if int(v) not in taken][:30])
-    # shared DOFs at row positions below cols, before and after the
-    # columns that hold them, some of them repeated among the rows too
-    for pos, j in [(0, 0), (2, 3), (5, 21), (9, 7), (14, 3), (20, 11),
-                   (24, 0)]:
-        ridx[pos] = cidx[j]
+    if not br and not bc:
+        # rows from the other DOFs: a shared DOF at row position >= cols
+        # would put the reference's identity term outside the window
+        taken = set(int(v) for v in base)
+        ridx = np.array([v for v in rng.permutation(nr)
+                         if int(v) not in taken][:30])
+        # shared DOFs at row positions below cols, before and after the
+        # columns that hold them, some of them repeated among the rows too
+        for pos, j in [(0, 0), (2, 3), (5, 21), (9, 7), (14, 3), (20, 11),
+                       (24, 0)]:
+            ridx[pos] = cidx[j]
+    else:
+        ridx = rng.permutation(nr)[:70]
+        ridx[[66, 67]] = ridx[10]
+        ridx[5] = ridx[2]
+        ridx[40] = ridx[41]
     ridx = ridx.astype(np.uint32)
-    cols = len(cidx)
-    assert cols == 25 and len(ridx) == 30
     for far in (False, True):
         f = g.nearfield_far if far else g.nearfield
         a = f(ridx, cidx, ntrans=ntrans)
