@@ -120,10 +120,13 @@ int h2hip_op_set_precision(h2hip_op *op, int mode);
  * the column DOFs cidx (NULL: 0, 1, 2, ...).  With ntrans != 0 the
  * transposed cols x rows window is written (conjugated for the cc and ll
  * basis pairs, as in the reference).  far != 0 uses the regular rule for all
- * pairs (nearfield_far).  Blocks until N is written.  Thread-safe: calls
- * from several host threads run concurrently (up to four per operator, each
- * on its own stream, with device and pinned host buffers kept between
- * calls).
+ * pairs (nearfield_far).  Index arrays that repeat DOFs give the
+ * reference's entries (constant basis: identity terms in its overwrite
+ * order; linear basis: a support triangle counts only the first three
+ * block positions that touch it).  Blocks until N is written.
+ * Thread-safe: calls from several host threads run concurrently (up to
+ * four per operator, each on its own stream, with device and pinned host
+ * buffers kept between calls).
  * Replaces the callbacks bem->nearfield and bem->nearfield_far
  * (Library/bem3d.h:380-381, 405-406), i.e. assemble_{cc,cl,lc,ll}_near_bem3d
  * and assemble_{cc,cl,lc,ll}_far_bem3d (Library/bem3d.c:1150-3436). */
